@@ -1,0 +1,137 @@
+"""Loader and call helpers for libcsg_hip.so, the gfx950 C-ABI library.
+
+Every device computation of this package goes through this library.  There
+is no CPU or torch fallback: if the library or a HIP device is missing the
+call raises.  The library is built in-tree by ``make -C csrc`` (see
+``__graft_entry__.build``) and lives next to this file.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+from typing import Any
+
+import torch
+
+_LIB_NAME = "libcsg_hip.so"
+_HERE = Path(__file__).resolve().parent
+_lib: ctypes.CDLL | None = None
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+_sz = ctypes.c_size_t
+_dp = ctypes.POINTER(ctypes.c_double)
+
+# argument signatures of the exported entry points (include/csg.h)
+_SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
+    "csg_last_error": (ctypes.c_char_p, []),
+    "csg_version": (ctypes.c_char_p, []),
+    "csg_device_count": (_i, [ctypes.POINTER(_i)]),
+    "csg_sample_field": (_i, [_vp, _i, _i, _vp, _vp, _sz, _i, _vp, _vp]),
+    "csg_composition": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),
+    "csg_composition_jacobian": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp]),
+    "csg_jacobian_of_displacement": (_i, [_vp, _d, _i, _i, _i, _vp, _vp]),
+    "csg_smalldef_jac": (_i, [_vp, _d, _i, _i, _i, _vp, _vp, _vp]),
+    "csg_jac_div_smalldef": (_i, [_vp, _vp, _d, _i, _i, _i, _vp, _vp]),
+    "csg_expdef_workspace_bytes": (_sz, [_i, _i, _i]),
+    "csg_expdef": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _sz, _vp]),
+    "csg_objfun_workspace_bytes": (_sz, [_i, _i]),
+    "csg_initialise_objfun": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _sz,
+                                   _vp]),
+    "csg_squaring_workspace_bytes": (_sz, [_i, _i, _i]),
+    "csg_squaring_update": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp,
+                                 _vp, _sz, _vp]),
+    "csg_regularization_operator": (_i, [_vp, _dp, _i, _i, _i, _i, _vp, _vp]),
+    "csg_apply_membrane_system": (_i, [_vp, _vp, _d, _d, _d, _i, _i, _i, _vp, _vp]),
+    "csg_relax_membrane": (_i, [_vp, _vp, _vp, _d, _d, _d, _i, _i, _i, _i, _vp]),
+    "csg_resize_field": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp]),
+    "csg_fmg2_workspace_bytes": (_sz, [_i, _i, _i, _i]),
+    "csg_fmg2_membrane": (_i, [_vp, _vp, _dp, _i, _i, _vp, _i, _i, _i, _i, _i, _vp, _vp, _sz,
+                               _vp]),
+    "csg_dartel_step_workspace_bytes": (_sz, [_i, _i, _i, _i]),
+    "csg_dartel_step": (_i, [_vp, _vp, _vp, _vp, _i, _dp, _d, _i, _i, _i, _i, _i, _i, _i, _i,
+                             _vp, _vp, _vp, _sz, _vp]),
+    "csg_solve_workspace_bytes": (_sz, [_i, _i, _i, _i]),
+    "csg_solve_dartel_maps": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _d, _i, _i, _i, _i, _i, _i,
+                                   _vp, _vp, _vp, _vp, _sz, _vp]),
+}
+
+
+class HipLibraryError(RuntimeError):
+    """The HIP library is missing or a call into it failed at run time."""
+
+
+def library_path() -> Path:
+    override = os.environ.get("CSG_HIP_LIBRARY")
+    return Path(override) if override else _HERE / _LIB_NAME
+
+
+def lib() -> ctypes.CDLL:
+    """Load libcsg_hip.so once; raise if it is not built."""
+
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not path.is_file():
+        raise HipLibraryError(
+            f"{_LIB_NAME} is not built at {path}; run `make -C cat-surface-gpu_amd/csrc` "
+            "(or __graft_entry__.build()).  There is no CPU fallback."
+        )
+    handle = ctypes.CDLL(str(path))
+    for name, (restype, argtypes) in _SIGNATURES.items():
+        fn = getattr(handle, name, None)
+        if fn is None:
+            continue
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _lib = handle
+    return handle
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def check(code: int, what: str) -> None:
+    """Map a C-ABI status code to the reference's exception types."""
+
+    if code == 0:
+        return
+    message = lib().csg_last_error().decode("utf-8", "replace")
+    text = f"{what}: {message}"
+    if code == -1:
+        raise ValueError(text)
+    if code == -3:
+        raise NotImplementedError(text)
+    raise HipLibraryError(text)
+
+
+def dtype_code(dtype: torch.dtype) -> int:
+    if dtype == torch.float32:
+        return 0
+    if dtype == torch.float64:
+        return 1
+    raise ValueError(f"only float32 and float64 are supported, got {dtype}")
+
+
+def stream_handle(device: torch.device) -> int:
+    return int(torch.cuda.current_stream(device).cuda_stream)
+
+
+def ptr(tensor: torch.Tensor | None) -> int | None:
+    if tensor is None:
+        return None
+    return int(tensor.data_ptr())
+
+
+def workspace(nbytes: int, device: torch.device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+
+
+def doubles(values) -> Any:
+    arr = (ctypes.c_double * len(values))(*[float(v) for v in values])
+    return arr

@@ -1,0 +1,198 @@
+"""DARTEL solve, warp application and the solve cache, on MI355X.
+
+Drop-in for the reference's src/cat_surface_gpu/surface_warp.py.  The whole
+multi-step solve (``solve_dartel_maps``) is one call into libcsg_hip.so, which
+launches every kernel of every Gauss-Newton step on the current stream; with
+``cuda_graph=True`` that stream of launches is captured once per shape into a
+hipGraph and replayed (the reference's graph cache, surface_warp.py:27-236).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+import time
+from typing import Sequence
+
+import torch
+
+from . import _lib
+from .dartel_grid import GridSpec, _check_kernel, resolve_device
+
+
+@dataclass(frozen=True)
+class DartelSolveResult:
+    """Store a solved flow, warped feature maps, and DARTEL diagnostics."""
+
+    flow: torch.Tensor
+    source_maps: torch.Tensor
+    target_maps: torch.Tensor
+    metrics: torch.Tensor
+
+
+def default_dartel_parameters(loop: int = 6, *, mu: float = 0.125, lambda_: float = 0.0,
+                              lmreg: float = 1e-3, muchange: int = 4, murate: float = 1.25):
+    """Upstream (1, 1, mu, lambda, lambda/2) per loop (surface_warp.py:76-107)."""
+
+    if loop <= 0:
+        raise ValueError(f"loop must be positive, got {loop}")
+    if muchange <= 0 or murate <= 0:
+        raise ValueError("muchange and murate must be positive")
+    values = []
+    current_mu = float(mu)
+    current_lambda = float(lambda_)
+    for index in range(loop):
+        values.append((1.0, 1.0, current_mu, current_lambda, current_lambda / 2.0))
+        if (index + 1) % muchange == 0:
+            current_mu /= murate
+        current_lambda /= 5.0
+    return tuple(values)
+
+
+@dataclass
+class _SolveGraph:
+    graph: torch.cuda.CUDAGraph
+    source_maps: torch.Tensor
+    target_maps: torch.Tensor
+    flow: torch.Tensor
+    metrics: torch.Tensor
+    workspace: torch.Tensor
+    capture_seconds: float
+
+
+_DARTEL_CUDA_GRAPH_CACHE: dict[tuple, _SolveGraph] = {}
+
+
+def _solve_key(source: torch.Tensor, target: torch.Tensor, **kw) -> tuple:
+    return (source.device.type, source.device.index, source.dtype, tuple(source.shape),
+            tuple(target.shape)) + tuple(sorted(kw.items()))
+
+
+def _launch_solve(source: torch.Tensor, target: torch.Tensor, *, steps_per_job: int, loop: int,
+                  lmreg: float, cycles: int, nit: int, its: int, code: int,
+                  assume_initial_nonzero: bool, flow: torch.Tensor, metrics: torch.Tensor | None,
+                  workspace: torch.Tensor, velocity: torch.Tensor | None = None) -> None:
+    batch, ny, nx = (int(d) for d in source.shape)
+    dcode = _lib.dtype_code(source.dtype)
+    rc = _lib.lib().csg_solve_dartel_maps(
+        _lib.ptr(source), _lib.ptr(target), nx, ny, batch, int(steps_per_job), int(loop),
+        float(lmreg), int(cycles), int(nit), int(its), int(code),
+        int(bool(assume_initial_nonzero)), dcode, _lib.ptr(flow), _lib.ptr(velocity),
+        _lib.ptr(metrics), _lib.ptr(workspace), workspace.numel(),
+        _lib.stream_handle(source.device))
+    _lib.check(rc, "csg_solve_dartel_maps")
+
+
+def solve_workspace(nx: int, ny: int, batch: int, dtype: torch.dtype,
+                    device: torch.device) -> torch.Tensor:
+    nbytes = _lib.lib().csg_solve_workspace_bytes(nx, ny, batch, _lib.dtype_code(dtype))
+    return _lib.workspace(nbytes, device)
+
+
+def _capture(source, target, *, loop, lmreg, cycles, nit, its, code, device) -> _SolveGraph:
+    static_source = source.detach().clone().contiguous()
+    static_target = target.detach().clone().contiguous()
+    steps, ny, nx = (int(d) for d in source.shape)
+    flow = torch.empty((2, ny, nx), dtype=source.dtype, device=device)
+    metrics = torch.empty((steps, loop * its, 3), dtype=torch.float64, device=device)
+    ws = solve_workspace(nx, ny, steps, source.dtype, device)
+    graph = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize(device)
+    started = time.perf_counter()
+    current = torch.cuda.current_stream(device)
+    capture_stream = torch.cuda.Stream(device=device)
+    capture_stream.wait_stream(current)
+    with torch.cuda.stream(capture_stream):
+        with torch.cuda.graph(graph, stream=capture_stream):
+            _launch_solve(static_source, static_target, steps_per_job=steps, loop=loop,
+                          lmreg=lmreg, cycles=cycles, nit=nit, its=its, code=code,
+                          assume_initial_nonzero=True, flow=flow, metrics=metrics,
+                          workspace=ws)
+    current.wait_stream(capture_stream)
+    torch.cuda.synchronize(device)
+    return _SolveGraph(graph, static_source, static_target, flow, metrics, ws,
+                       time.perf_counter() - started)
+
+
+def prepare_dartel_cuda_graph(source_maps, target_maps, *, loop: int = 6, lmreg: float = 1e-3,
+                              cycles: int = 3, nit: int = 3, its: int = 3, code: int = 1,
+                              kernel: str = "auto", squaring_kernel: str | None = None,
+                              device: str | torch.device = "auto",
+                              dtype: torch.dtype = torch.float64, optimized: bool = True) -> float:
+    """Capture the solve for this shape once (surface_warp.py:169-231)."""
+
+    dev = resolve_device(device)
+    _check_kernel(kernel)
+    source = torch.as_tensor(source_maps, dtype=dtype, device=dev).contiguous()
+    target = torch.as_tensor(target_maps, dtype=dtype, device=dev).contiguous()
+    if tuple(source.shape) != tuple(target.shape):
+        raise ValueError("source_maps and target_maps must have the same shape")
+    key = _solve_key(source, target, loop=loop, lmreg=float(lmreg), cycles=cycles, nit=nit,
+                     its=its, code=code)
+    if key in _DARTEL_CUDA_GRAPH_CACHE:
+        return 0.0
+    entry = _capture(source, target, loop=loop, lmreg=lmreg, cycles=cycles, nit=nit, its=its,
+                     code=code, device=dev)
+    _DARTEL_CUDA_GRAPH_CACHE[key] = entry
+    return entry.capture_seconds
+
+
+def clear_dartel_cuda_graph_cache() -> None:
+    _DARTEL_CUDA_GRAPH_CACHE.clear()
+
+
+def solve_dartel_maps(source_maps, target_maps, *, loop: int = 6, lmreg: float = 1e-3,
+                      cycles: int = 3, nit: int = 3, its: int = 3, code: int = 1,
+                      kernel: str = "auto", squaring_kernel: str | None = None,
+                      device: str | torch.device = "auto", dtype: torch.dtype = torch.float64,
+                      optimized: bool = True, assume_initial_nonzero: bool = False,
+                      cuda_graph: bool = False) -> DartelSolveResult:
+    """Solve DARTEL feature maps with the upstream loop semantics (surface_warp.py:239-367).
+
+    The ``steps`` maps are independent solves (velocity reset per step); they
+    run batched in every kernel launch.  The flow is expdef(v of the last
+    step, 10).
+    """
+
+    if cycles < 0 or nit < 0 or its < 0:
+        raise ValueError("cycles, nit, and its must be non-negative")
+    if squaring_kernel not in {None, "auto", "hip", "torch", "triton"}:
+        raise ValueError("squaring_kernel must be None, 'auto', or 'hip'")
+    _check_kernel(kernel)
+    if squaring_kernel is not None:
+        _check_kernel(squaring_kernel)
+    dev = resolve_device(device)
+    source = torch.as_tensor(source_maps, dtype=dtype, device=dev)
+    target = torch.as_tensor(target_maps, dtype=dtype, device=dev)
+    if source.ndim != 3 or target.ndim != 3:
+        raise ValueError("source_maps and target_maps must have shape [steps, ny, nx]")
+    if tuple(source.shape) != tuple(target.shape):
+        raise ValueError("source_maps and target_maps must have the same shape")
+    if source.shape[0] < 1:
+        raise ValueError("At least one DARTEL step is required")
+    if loop <= 0:
+        raise ValueError(f"loop must be positive, got {loop}")
+    source = source.contiguous()
+    target = target.contiguous()
+    steps, ny, nx = (int(d) for d in source.shape)
+    if cuda_graph:
+        key = _solve_key(source, target, loop=loop, lmreg=float(lmreg), cycles=cycles, nit=nit,
+                         its=its, code=code)
+        entry = _DARTEL_CUDA_GRAPH_CACHE.get(key)
+        if entry is None:
+            entry = _capture(source, target, loop=loop, lmreg=lmreg, cycles=cycles, nit=nit,
+                             its=its, code=code, device=dev)
+            _DARTEL_CUDA_GRAPH_CACHE[key] = entry
+        entry.source_maps.copy_(source)
+        entry.target_maps.copy_(target)
+        entry.graph.replay()
+        return DartelSolveResult(flow=entry.flow.clone(), source_maps=entry.source_maps.clone(),
+                                 target_maps=entry.target_maps.clone(),
+                                 metrics=entry.metrics.to(dtype).clone())
+    flow = torch.empty((2, ny, nx), dtype=dtype, device=dev)
+    metrics = torch.empty((steps, loop * its, 3), dtype=torch.float64, device=dev)
+    ws = solve_workspace(nx, ny, steps, dtype, dev)
+    _launch_solve(source, target, steps_per_job=steps, loop=loop, lmreg=lmreg, cycles=cycles,
+                  nit=nit, its=its, code=code, assume_initial_nonzero=assume_initial_nonzero,
+                  flow=flow, metrics=metrics, workspace=ws)
+    return DartelSolveResult(flow=flow, source_maps=source, target_maps=target,
+                             metrics=metrics.to(dtype))

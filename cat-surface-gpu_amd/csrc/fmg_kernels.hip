@@ -1,0 +1,250 @@
+// Full-multigrid pieces for the membrane system (L_membrane + A) u = b:
+// quadratic B-spline resize (restriction and prolongation), red-black block
+// Gauss-Seidel, residual, and the 2x2 direct solve at the coarsest level.
+//
+// Reference semantics: src/cat_surface_gpu/dartel_grid.py:239-305 (resize),
+// :1018-1070 (membrane system), :1073-1208 (relaxation), :1211-1417 (FMG).
+#include "dartel_kernels.h"
+
+namespace csg {
+
+template <typename T>
+__device__ __forceinline__ T wt2(T d)
+{
+    T a = fabs(d);
+    if (a < (T)0.5)
+        return (T)0.75 - a * a;
+    if (a < (T)1.5) {
+        T o = (T)1.5 - a;
+        return (T)0.5 * o * o;
+    }
+    return (T)0;
+}
+
+// Separable 3-tap resize; one thread per target pixel, all channels.
+// Source location loc = (t + 0.5)·S/T - 0.5, origin floor(loc + 0.5).
+// The y pass is evaluated first for each of the three x taps, as in the
+// reference's two-stage torch evaluation.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_resize(const T *__restrict__ in, int channels, int snx, int sny, int tnx, int tny,
+         size_t istride, size_t ostride, T *__restrict__ out, int accumulate)
+{
+    const int TP = tnx * tny;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= TP)
+        return;
+    in += blockIdx.y * istride;
+    out += blockIdx.y * ostride;
+    int tx = p % tnx, ty = p / tnx;
+    T ly = ((T)ty + (T)0.5) * (T)sny / (T)tny - (T)0.5;
+    int oy = (int)floor(ly + (T)0.5);
+    T wy = wt2<T>((T)oy - ly), wym = wt2<T>((T)oy - (T)1 - ly), wyp = wt2<T>((T)oy + (T)1 - ly);
+    T lx = ((T)tx + (T)0.5) * (T)snx / (T)tnx - (T)0.5;
+    int ox = (int)floor(lx + (T)0.5);
+    T wx = wt2<T>((T)ox - lx), wxm = wt2<T>((T)ox - (T)1 - lx), wxp = wt2<T>((T)ox + (T)1 - lx);
+    int ym = reflect_y(oy - 1, sny) * snx, y0 = reflect_y(oy, sny) * snx,
+        yp = reflect_y(oy + 1, sny) * snx;
+    int xm = wrap_x(ox - 1, snx), x0 = wrap_x(ox, snx), xp = wrap_x(ox + 1, snx);
+    const int SP = snx * sny;
+    for (int c = 0; c < channels; ++c) {
+        const T *f = in + (size_t)c * SP;
+        T im = wym * f[ym + xm] + wy * f[y0 + xm] + wyp * f[yp + xm];
+        T i0 = wym * f[ym + x0] + wy * f[y0 + x0] + wyp * f[yp + x0];
+        T ip = wym * f[ym + xp] + wy * f[y0 + xp] + wyp * f[yp + xp];
+        T r = wxm * im + wx * i0 + wxp * ip;
+        size_t o = (size_t)c * TP + p;
+        out[o] = accumulate ? out[o] + r : r;
+    }
+}
+
+// One colour of the red-black sweep, in place.  Pixels with (x + y) & 1 ==
+// color are updated from their four neighbours (opposite colour for even nx,
+// or the pixel itself through the boundaries).  dartel_triton.py:241-291
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_relax_color(const T *__restrict__ a, const T *__restrict__ b, T *__restrict__ u, T w00, T w01,
+              T w10, int nx, int ny, size_t astride, size_t bstride, int color)
+{
+    const int P = nx * ny;
+    int half = (nx + 1) >> 1;
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= half * ny)
+        return;
+    int y = q / half;
+    int x = 2 * (q - y * half) + ((color + y) & 1);
+    if (x >= nx)
+        return;
+    a += blockIdx.y * astride;
+    b += blockIdx.y * bstride;
+    u += blockIdx.y * bstride;
+    int p = y * nx + x;
+    int xm = y * nx + wrap_x(x - 1, nx), xp = y * nx + wrap_x(x + 1, nx);
+    int ym = reflect_y(y - 1, ny) * nx + x, yp = reflect_y(y + 1, ny) * nx + x;
+    T rx = b[p] - w01 * (u[ym] + u[yp]) - w10 * (u[xm] + u[xp]);
+    T ry = b[P + p] - w01 * (u[P + ym] + u[P + yp]) - w10 * (u[P + xm] + u[P + xp]);
+    T dx = a[p] + w00, dy = a[P + p] + w00, cr = a[2 * P + p];
+    T det = dx * dy * (T)1.0000000001 - cr * cr;
+    u[p] = (dy * rx - cr * ry) / det;
+    u[P + p] = (-cr * rx + dx * ry) / det;
+}
+
+// out = (L + A) u, or out = b - (L + A) u when b_minus is given.
+// dartel_triton.py:189-239, dartel_grid.py:1356-1368
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_apply_membrane(const T *__restrict__ a, const T *__restrict__ u, T w00, T w01, T w10, int nx,
+                 int ny, size_t astride, size_t ustride, T *__restrict__ out,
+                 const T *__restrict__ b_minus)
+{
+    const int P = nx * ny;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P)
+        return;
+    a += blockIdx.y * astride;
+    u += blockIdx.y * ustride;
+    out += blockIdx.y * ustride;
+    int x = p % nx, y = p / nx;
+    int xm = y * nx + wrap_x(x - 1, nx), xp = y * nx + wrap_x(x + 1, nx);
+    int ym = reflect_y(y - 1, ny) * nx + x, yp = reflect_y(y + 1, ny) * nx + x;
+    T vx = u[p], vy = u[P + p];
+    T rx = w00 * vx + w01 * (u[ym] + u[yp]) + w10 * (u[xm] + u[xp]);
+    T ry = w00 * vy + w01 * (u[P + ym] + u[P + yp]) + w10 * (u[P + xm] + u[P + xp]);
+    T ox = rx + a[p] * vx + a[2 * P + p] * vy;
+    T oy = ry + a[2 * P + p] * vx + a[P + p] * vy;
+    if (b_minus) {
+        b_minus += blockIdx.y * ustride;
+        ox = b_minus[p] - ox;
+        oy = b_minus[P + p] - oy;
+    }
+    out[p] = ox;
+    out[P + p] = oy;
+}
+
+// Coarsest level: direct 2x2 block solve in double precision.
+// dartel_grid.py:1301-1315
+template <typename T>
+__global__ void k_coarse_solve(const T *__restrict__ a, const T *__restrict__ b, double id, int P,
+                               size_t astride, size_t bstride, T *__restrict__ u)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P)
+        return;
+    a += blockIdx.y * astride;
+    b += blockIdx.y * bstride;
+    u += blockIdx.y * bstride;
+    double dx = (double)a[p] + id, dy = (double)a[P + p] + id, cr = (double)a[2 * P + p];
+    double det = dx * dy * 1.0000000001 - cr * cr;
+    double bx = (double)b[p], by = (double)b[P + p];
+    u[p] = (T)((dy * bx - cr * by) / det);
+    u[P + p] = (T)((-cr * bx + dx * by) / det);
+}
+
+// flags[b] = any(u[b] != 0); one block per batch element.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_any_nonzero(const T *__restrict__ u, size_t n, size_t stride, int *__restrict__ flags)
+{
+    __shared__ int found;
+    if (threadIdx.x == 0)
+        found = 0;
+    __syncthreads();
+    u += blockIdx.x * stride;
+    int local = 0;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x)
+        local |= (u[i] != (T)0);
+    if (local)
+        found = 1;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        flags[blockIdx.x] = found;
+}
+
+template <typename T>
+int launch_resize(const T *in, int channels, int snx, int sny, int tnx, int tny, int batch,
+                  size_t istride, size_t ostride, T *out, int accumulate, hipStream_t s)
+{
+    dim3 g(grid_for((size_t)tnx * tny), batch);
+    hipLaunchKernelGGL(k_resize<T>, g, dim3(kBlock), 0, s, in, channels, snx, sny, tnx, tny,
+                       istride, ostride, out, accumulate);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+template <typename T>
+int launch_relax_sweep(const T *a, const T *b, T *u, MembraneWeights w, int nx, int ny,
+                       int batch, size_t astride, size_t bstride, int color, hipStream_t s)
+{
+    int half = (nx + 1) / 2;
+    dim3 g(grid_for((size_t)half * ny), batch);
+    hipLaunchKernelGGL(k_relax_color<T>, g, dim3(kBlock), 0, s, a, b, u, (T)w.w00, (T)w.w01,
+                       (T)w.w10, nx, ny, astride, bstride, color);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+template <typename T>
+int launch_relax(const T *a, const T *b, T *u, MembraneWeights w, int nx, int ny, int batch,
+                 size_t astride, size_t bstride, int nit, hipStream_t s)
+{
+    if (nx % 2 == 1 && nx != 1)
+        return fail(kInvalidArgument,
+                    "red-black relaxation on the GPU requires an even x dimension, got nx=%d",
+                    nx);
+    for (int it = 0; it < 2 * nit; ++it) {
+        int rc = launch_relax_sweep<T>(a, b, u, w, nx, ny, batch, astride, bstride,
+                                       (it % 2 == 0) ? 1 : 0, s);
+        if (rc)
+            return rc;
+    }
+    return kOk;
+}
+
+template <typename T>
+int launch_apply_membrane(const T *a, const T *u, MembraneWeights w, int nx, int ny, int batch,
+                          size_t astride, size_t ustride, T *out, const T *b_minus,
+                          hipStream_t s)
+{
+    dim3 g(grid_for((size_t)nx * ny), batch);
+    hipLaunchKernelGGL(k_apply_membrane<T>, g, dim3(kBlock), 0, s, a, u, (T)w.w00, (T)w.w01,
+                       (T)w.w10, nx, ny, astride, ustride, out, b_minus);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+template <typename T>
+int launch_coarse_solve(const T *a, const T *b, double id, int P, int batch, size_t astride,
+                        size_t bstride, T *u, hipStream_t s)
+{
+    dim3 g(grid_for(P, 64), batch);
+    hipLaunchKernelGGL(k_coarse_solve<T>, g, dim3(64), 0, s, a, b, id, P, astride, bstride, u);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+template <typename T>
+int launch_any_nonzero(const T *u, size_t n, int batch, size_t stride, int *flags,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(k_any_nonzero<T>, dim3(batch), dim3(kBlock), 0, s, u, n, stride, flags);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+#define CSG_INSTANTIATE_FMG(T)                                                                 \
+    template int launch_resize<T>(const T *, int, int, int, int, int, int, size_t, size_t, T *, \
+                                  int, hipStream_t);                                           \
+    template int launch_relax_sweep<T>(const T *, const T *, T *, MembraneWeights, int, int,   \
+                                       int, size_t, size_t, int, hipStream_t);                 \
+    template int launch_relax<T>(const T *, const T *, T *, MembraneWeights, int, int, int,    \
+                                 size_t, size_t, int, hipStream_t);                            \
+    template int launch_apply_membrane<T>(const T *, const T *, MembraneWeights, int, int,     \
+                                          int, size_t, size_t, T *, const T *, hipStream_t);   \
+    template int launch_coarse_solve<T>(const T *, const T *, double, int, int, size_t,        \
+                                        size_t, T *, hipStream_t);                             \
+    template int launch_any_nonzero<T>(const T *, size_t, int, size_t, int *, hipStream_t);
+
+CSG_INSTANTIATE_FMG(float)
+CSG_INSTANTIATE_FMG(double)
+
+}  // namespace csg

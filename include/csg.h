@@ -1,0 +1,144 @@
+/*
+ * libcsg_hip — C ABI of the MI355X (gfx950) CAT_SurfWarp registration hot path.
+ *
+ * Conventions for every entry point:
+ *   - all array pointers are device memory unless a comment says "host";
+ *   - grids are channel-major [C][ny][nx], contiguous; maps are one-based;
+ *   - dtype selects the element type of the floating arrays: 0 = float32,
+ *     1 = float64 (reductions are always accumulated in float64);
+ *   - `stream` is a hipStream_t (NULL = the default stream); work is
+ *     stream-ordered and nothing allocates, so callers may capture the calls
+ *     into a hipGraph (except where a comment says the call synchronises);
+ *   - scratch memory comes from the caller as (ws, ws_bytes); the matching
+ *     *_workspace_bytes() call gives the size needed;
+ *   - the return value is 0 on success, -1 for an invalid argument, -2 for a
+ *     runtime (HIP) failure, -3 for an unsupported option; csg_last_error()
+ *     returns the message of the last failure on the calling thread.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the root of ayakacxy/cat-surface-gpu).  The reference binds these seams
+ * in Python (kernel="triton" dispatch); INTEGRATION.md shows the ctypes
+ * binding a maintainer adds for kernel="hip".
+ */
+#ifndef CSG_H
+#define CSG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- library */
+
+const char *csg_last_error(void);
+const char *csg_version(void);
+/* 0 if a HIP device is visible; fills *count. */
+int csg_device_count(int *count);
+
+/* ------------------------------------------------------------ DARTEL grid */
+
+/* Bilinear sample of a scalar [ny][nx] field at zero-based (x, y).
+ * Replaces dartel_grid.sample_field (src/cat_surface_gpu/dartel_grid.py:510-531). */
+int csg_sample_field(const void *field, int nx, int ny, const void *x, const void *y, size_t n,
+                     int dtype, void *out, void *stream);
+
+/* c = b o a.  Replaces dartel_grid.composition (dartel_grid.py:552-575). */
+int csg_composition(const void *a, const void *b, int nx, int ny, int dtype, void *out,
+                    void *stream);
+
+/* (b o a, (Jb o a) Ja).  Replaces dartel_grid.composition_jacobian (dartel_grid.py:578-621). */
+int csg_composition_jacobian(const void *a, const void *ja, const void *b, const void *jb, int nx,
+                             int ny, int dtype, void *map_out, void *jac_out, void *stream);
+
+/* I + scale * grad v (4 channels).  Replaces dartel_grid.jacobian_of_displacement
+ * (dartel_grid.py:624-652). */
+int csg_jacobian_of_displacement(const void *v, double scale, int nx, int ny, int dtype,
+                                 void *jac_out, void *stream);
+
+/* (id + scale v, I + scale grad v).  Replaces dartel_grid.smalldef_jac (dartel_grid.py:655-676). */
+int csg_smalldef_jac(const void *v, double scale, int nx, int ny, int dtype, void *map_out,
+                     void *jac_out, void *stream);
+
+/* J (I + scale grad v)^-1.  Replaces dartel_grid.jac_div_smalldef (dartel_grid.py:679-720). */
+int csg_jac_div_smalldef(const void *jac, const void *v, double scale, int nx, int ny, int dtype,
+                         void *out, void *stream);
+
+/* Scaling and squaring exp(v) with k squarings (+ Jacobian when jac_out != NULL).
+ * Replaces dartel_grid.expdef (dartel_grid.py:1619-1666). */
+size_t csg_expdef_workspace_bytes(int nx, int ny, int dtype);
+int csg_expdef(const void *v, int nx, int ny, int dtype, int k, void *map_out, void *jac_out,
+               void *ws, size_t ws_bytes, void *stream);
+
+/* Objective, gradient (rhs, 2ch) and Gauss-Newton Hessian (3ch); *objective (device
+ * double) receives 0.5 * sum(d^2 w).  dist may be NULL.
+ * Replaces dartel_grid.initialise_objfun (dartel_grid.py:723-775). */
+size_t csg_objfun_workspace_bytes(int nx, int ny);
+int csg_initialise_objfun(const void *src, const void *tgt, const void *map, const void *jac,
+                          const void *dist, int nx, int ny, int dtype, void *rhs, void *hes,
+                          double *objective, void *ws, size_t ws_bytes, void *stream);
+
+/* k fused squaring updates; scratch_out receives the reference's map_buffers[0]
+ * (the FMG initial guess).  Replaces dartel_grid.squaring_update (dartel_grid.py:778-914)
+ * and dartel_triton.squaring_update_triton (src/cat_surface_gpu/dartel_triton.py:644-693). */
+size_t csg_squaring_workspace_bytes(int nx, int ny, int dtype);
+int csg_squaring_update(const void *rhs, const void *hes, const void *map, const void *jac, int k,
+                        int save_transformation, int nx, int ny, int dtype, void *b_out,
+                        void *a_out, void *map_out, void *jac_out, void *scratch_out, void *ws,
+                        size_t ws_bytes, void *stream);
+
+/* L v for rtype 0/1/2; params (host) = (sx, sy, mu, lambda, id).
+ * Replaces dartel_grid.regularization_operator (dartel_grid.py:917-1015). */
+int csg_regularization_operator(const void *v, const double *params, int rtype, int nx, int ny,
+                                int dtype, void *out, void *stream);
+
+/* (L_membrane + A) u.  Replaces dartel_triton.apply_membrane_system_triton
+ * (dartel_triton.py:528-569). */
+int csg_apply_membrane_system(const void *hes, const void *u, double w00, double w01, double w10,
+                              int nx, int ny, int dtype, void *out, void *stream);
+
+/* nit red-black iterations in place on `solution` (nx even or 1).
+ * Replaces dartel_triton.relax_membrane_triton (dartel_triton.py:572-641). */
+int csg_relax_membrane(const void *hes, const void *rhs, void *solution, double w00, double w01,
+                       double w10, int nit, int nx, int ny, int dtype, void *stream);
+
+/* Quadratic B-spline resize of a [channels][sny][snx] field.
+ * Replaces dartel_triton.resize_field_triton (dartel_triton.py:480-525). */
+int csg_resize_field(const void *in, int channels, int snx, int sny, int tnx, int tny, int dtype,
+                     void *out, void *stream);
+
+/* Full multigrid membrane solve; params (host, 5); initial may be NULL;
+ * coarse_init != 0 selects the coarse-initialisation branch.
+ * Replaces dartel_grid.fmg2_membrane (dartel_grid.py:1211-1417). */
+size_t csg_fmg2_workspace_bytes(int nx, int ny, int batch, int dtype);
+int csg_fmg2_membrane(const void *hes, const void *rhs, const double *params, int cycles, int nit,
+                      const void *initial, int coarse_init, int nx, int ny, int batch, int dtype,
+                      void *out, void *ws, size_t ws_bytes, void *stream);
+
+/* One Gauss-Newton DARTEL step for `batch` independent (src, tgt, v) triples;
+ * metrics (device double [batch][3]) = (objective, reg/2, ||rhs||).  dist may be NULL.
+ * With assume_nonzero == 0 the call synchronises the stream once (the
+ * reference's torch.any(initial) check).
+ * Replaces dartel_grid.dartel_step (dartel_grid.py:1420-1616). */
+size_t csg_dartel_step_workspace_bytes(int nx, int ny, int batch, int dtype);
+int csg_dartel_step(const void *src, const void *tgt, const void *v, const void *dist, int k,
+                    const double *params, double lmreg, int cycles, int nit, int code,
+                    int assume_nonzero, int nx, int ny, int batch, int dtype, void *v_out,
+                    double *metrics, void *ws, size_t ws_bytes, void *stream);
+
+/* Whole DARTEL solve: `batch` maps grouped in jobs of steps_per_job consecutive
+ * steps; every step starts from v = 0 and runs loop x its dartel steps; flow
+ * (jobs x 2ch) = expdef(v of each job's last step, 10).  velocity / metrics may
+ * be NULL.  Replaces surface_warp.solve_dartel_maps
+ * (src/cat_surface_gpu/surface_warp.py:239-367). */
+size_t csg_solve_workspace_bytes(int nx, int ny, int batch, int dtype);
+int csg_solve_dartel_maps(const void *src, const void *tgt, int nx, int ny, int batch,
+                          int steps_per_job, int loop, double lmreg, int cycles, int nit, int its,
+                          int code, int assume_nonzero, int dtype, void *flow, void *velocity,
+                          double *metrics, void *ws, size_t ws_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CSG_H */
