@@ -57,6 +57,37 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_solve_workspace_bytes": (_sz, [_i, _i, _i, _i]),
     "csg_solve_dartel_maps": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _d, _i, _i, _i, _i, _i, _i,
                                    _vp, _vp, _vp, _vp, _sz, _vp]),
+    # surface side
+    "csg_resample_vertices": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp]),
+    "csg_edge_mean_workspace_bytes": (_sz, [_i]),
+    "csg_edge_sum": (_i, [_vp, _vp, _i, _vp, ctypes.POINTER(_i), _vp]),
+    "csg_smooth_geometry": (_i, [_vp, _vp, _vp, _i, _i, _vp, _vp, _sz, _vp]),
+    "csg_vertex_normals": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp]),
+    "csg_smooth_values_workspace_bytes": (_sz, [_i, _i, _i]),
+    "csg_smooth_values": (_i, [_vp, _i, _i, _vp, _vp, _vp, _i, _dp, ctypes.POINTER(_i), _i, _vp,
+                               _vp, _sz, _vp]),
+    "csg_minmax_workspace_bytes": (_sz, [_i]),
+    "csg_minmax_normalise": (_i, [_vp, _sz, _i, _d, _i, _vp, _vp, _sz, _vp]),
+    "csg_sheet_gather": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _vp]),
+    "csg_type5_workspace_bytes": (_sz, [_i, _i, _i, _i]),
+    "csg_type5_sheets": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _i, _dp,
+                              ctypes.POINTER(_i), _vp, _vp, _i, _i, _vp, _vp, _sz, _vp]),
+    "csg_depth_workspace_bytes": (_sz, [_i, _i, _i]),
+    "csg_depth_potential": (_i, [_vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i,
+                                 ctypes.POINTER(_i), _vp, _i, _d, _i, _d, _vp, _vp, _vp, _sz,
+                                 _vp]),
+    "csg_apply_flow": (_i, [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp]),
+    "csg_average_xz": (_i, [_vp, _vp, _i, _i, _dp, _vp, _vp]),
+    # point location
+    "csg_grid_index_plan_workspace_bytes": (_sz, [_i]),
+    "csg_grid_index_plan": (_i, [_vp, _i, _vp, _i, _i, _i, _vp, _sz, ctypes.POINTER(_i), _vp]),
+    "csg_grid_index_fill_workspace_bytes": (_sz, [_i]),
+    "csg_grid_index_fill": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _sz, _vp]),
+    "csg_rotation_costs_workspace_bytes": (_sz, [_i, _i]),
+    "csg_rotation_costs": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp,
+                                _vp, _vp, _sz, ctypes.POINTER(_i), _vp]),
+    "csg_closest_triangles": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i, _d, _vp, _vp, _vp,
+                                   ctypes.POINTER(_i), _vp, _sz, _vp]),
 }
 
 

@@ -349,7 +349,8 @@ def relax_membrane(hessian, right_hand, params: Sequence[float], nit: int, initi
                    dtype: torch.dtype = torch.float64) -> torch.Tensor:
     """nit red-black block Gauss-Seidel iterations (dartel_grid.py:1073-1208).
 
-    As on the reference's GPU path, nx must be even (or 1).
+    Odd widths follow the reference's sequential CPU order exactly (the
+    reference's own GPU path rejects them); see k_relax_color.
     """
 
     if nit < 0:
@@ -369,8 +370,6 @@ def relax_membrane(hessian, right_hand, params: Sequence[float], nit: int, initi
     if len(params) != 5:
         raise ValueError(f"regularization parameters must contain 5 values, got {len(params)}")
     _check_kernel(kernel)
-    if spec.nx % 2 == 1 and spec.nx != 1:
-        raise ValueError("GPU red-black relaxation requires an even x dimension")
     w00, w01, w10 = _membrane_weights(params)
     _call("csg_relax_membrane", _lib.ptr(a), _lib.ptr(b), _lib.ptr(solution), w00, w01, w10,
           int(nit), spec.nx, spec.ny, _lib.dtype_code(dtype), _lib.stream_handle(dev))

@@ -1,0 +1,474 @@
+"""Spherical resampling stencils and their device-resident operators (MI355X).
+
+Drop-in for src/cat_surface_gpu/surface_stencil.py of the reference: the
+same stencil v1/v2 file format and the same ``SurfaceStencil`` /
+``SurfaceStencilDevice`` API.  Every device operator runs in libcsg_hip.so;
+topology tables (ascending neighbour CSR, incident-corner CSR, greedy colour
+groups, depth-system CSR layout) are built once per face array on the host
+and cached on the device, as the reference caches its neighbour tables
+(surface_stencil.py:19-33).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from pathlib import Path
+import struct
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_HEADER = struct.Struct("<8i")
+_MAGIC = 0x46534354
+_TOPOLOGY_CACHE: dict[tuple, "Topology"] = {}
+
+
+# --------------------------------------------------------------------- topology
+
+
+def build_neighbour_csr(faces: np.ndarray, n_points: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unique neighbours of each vertex in ascending order (surface_stencil.py:36-64)."""
+
+    f = np.asarray(faces, dtype=np.int64)
+    directed = np.concatenate([f[:, [0, 1]], f[:, [1, 0]], f[:, [1, 2]], f[:, [2, 1]],
+                               f[:, [2, 0]], f[:, [0, 2]]])
+    directed = np.unique(directed, axis=0)
+    counts = np.bincount(directed[:, 0], minlength=n_points)
+    off = np.zeros(n_points + 1, np.int32)
+    off[1:] = np.cumsum(counts)
+    return off, np.ascontiguousarray(directed[:, 1].astype(np.int32))
+
+
+def build_incident_csr(faces: np.ndarray, n_points: int) -> tuple[np.ndarray, np.ndarray]:
+    """face*3+corner per vertex in (corner, face) order: the order of index_add_."""
+
+    f = np.asarray(faces, dtype=np.int64)
+    nf = f.shape[0]
+    corner_major = f.T.reshape(-1)  # entry e = corner * nf + face
+    order = np.argsort(corner_major, kind="stable")
+    corner = order // nf
+    face = order - corner * nf
+    counts = np.bincount(corner_major, minlength=n_points)
+    off = np.zeros(n_points + 1, np.int32)
+    off[1:] = np.cumsum(counts)
+    return off, np.ascontiguousarray((face * 3 + corner).astype(np.int32))
+
+
+def greedy_colours(off: np.ndarray, nbr: np.ndarray) -> np.ndarray:
+    """Smallest colour not used by an already coloured neighbour, in vertex order.
+
+    surface_stencil.py:67-86
+    """
+
+    n = off.shape[0] - 1
+    colours = np.full(n, -1, dtype=np.int32)
+    offl = off.tolist()
+    nbrl = nbr.tolist()
+    col = colours.tolist()
+    for v in range(n):
+        used = {col[u] for u in nbrl[offl[v]:offl[v + 1]] if col[u] >= 0}
+        c = 0
+        while c in used:
+            c += 1
+        col[v] = c
+    return np.asarray(col, dtype=np.int32)
+
+
+@dataclass
+class Topology:
+    """Device tables derived from one face array."""
+
+    n_points: int
+    faces: torch.Tensor  # int32 [F, 3]
+    nbr_off: torch.Tensor
+    nbr: torch.Tensor
+    inc_off: torch.Tensor
+    inc: torch.Tensor
+    colour_off: np.ndarray  # host int32 [C + 1]
+    colour_verts: torch.Tensor
+    sys_row_off: torch.Tensor
+    sys_cols: torch.Tensor
+    sys_perm: torch.Tensor
+    sys_alpha: torch.Tensor
+    faces_np: np.ndarray = field(repr=False, default=None)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.nbr.numel())
+
+
+def topology_for(faces: np.ndarray, n_points: int, device: torch.device) -> Topology:
+    f = np.ascontiguousarray(np.asarray(faces, dtype=np.int32))
+    key = (device.type, device.index, n_points, f.shape, f.tobytes())
+    cached = _TOPOLOGY_CACHE.get(key)
+    if cached is not None:
+        return cached
+    nbr_off, nbr = build_neighbour_csr(f, n_points)
+    inc_off, inc = build_incident_csr(f, n_points)
+    colours = greedy_colours(nbr_off, nbr)
+    order = np.argsort(colours, kind="stable")
+    ccount = np.bincount(colours)
+    colour_off = np.zeros(ccount.size + 1, np.int32)
+    colour_off[1:] = np.cumsum(ccount)
+    # depth-system layout: COO blocks of rotation_feature.py:144-161, then the alpha diagonal
+    nf = f.shape[0]
+    i0, i1, i2 = (f[:, k].astype(np.int64) for k in range(3))
+    rows = np.concatenate([i0, i1, i2, i0, i1, i1, i2, i0, i2, np.arange(n_points)])
+    cols = np.concatenate([i0, i1, i2, i1, i0, i2, i1, i2, i0, np.arange(n_points)])
+    perm = np.argsort(rows, kind="stable")
+    row_off = np.zeros(n_points + 1, np.int32)
+    row_off[1:] = np.cumsum(np.bincount(rows, minlength=n_points))
+
+    def dev(a, dtype=torch.int32):
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
+
+    topo = Topology(
+        n_points=n_points,
+        faces=dev(f),
+        nbr_off=dev(nbr_off),
+        nbr=dev(nbr),
+        inc_off=dev(inc_off),
+        inc=dev(inc),
+        colour_off=colour_off,
+        colour_verts=dev(order.astype(np.int32)),
+        sys_row_off=dev(row_off),
+        sys_cols=dev(cols[perm].astype(np.int32)),
+        sys_perm=dev(perm.astype(np.int32)),
+        sys_alpha=dev((perm >= 9 * nf).astype(np.uint8), torch.uint8),
+        faces_np=f,
+    )
+    _TOPOLOGY_CACHE[key] = topo
+    return topo
+
+
+# --------------------------------------------------------------------- file format
+
+
+@dataclass(frozen=True)
+class SurfaceStencil:
+    """Store CPU indices, barycentric weights, topology, and unit-sphere points."""
+
+    sphere_points: np.ndarray
+    faces: np.ndarray
+    surface_indices: np.ndarray
+    surface_weights: np.ndarray
+    sheet_indices: np.ndarray
+    sheet_weights: np.ndarray
+    nx: int
+    ny: int
+    source_points: int
+    unit_sphere_points: np.ndarray | None = None
+
+    @classmethod
+    def from_file(cls, path: str | Path) -> "SurfaceStencil":
+        """Read a versioned stencil file (surface_stencil.py:127-191)."""
+
+        with Path(path).open("rb") as stream:
+            header = stream.read(_HEADER.size)
+            if len(header) != _HEADER.size:
+                raise ValueError("Stencil file is too short")
+            magic, version, n_points, n_tri, n_sheet, nx, ny, source_points = _HEADER.unpack(header)
+            if magic != _MAGIC or version not in (1, 2):
+                raise ValueError("Stencil file magic or version is invalid")
+            sphere_points = np.fromfile(stream, dtype="<f8", count=3 * n_points)
+            faces = np.fromfile(stream, dtype="<i4", count=3 * n_tri)
+            surface_indices = np.fromfile(stream, dtype="<i4", count=3 * n_points)
+            surface_weights = np.fromfile(stream, dtype="<f8", count=3 * n_points)
+            sheet_indices = np.fromfile(stream, dtype="<i4", count=3 * n_sheet)
+            sheet_weights = np.fromfile(stream, dtype="<f8", count=3 * n_sheet)
+            unit = np.fromfile(stream, dtype="<f4", count=3 * source_points) if version == 2 else None
+        arrays = (sphere_points, faces, surface_indices, surface_weights, sheet_indices,
+                  sheet_weights)
+        expected = (3 * n_points, 3 * n_tri, 3 * n_points, 3 * n_points, 3 * n_sheet, 3 * n_sheet)
+        if any(a.size != e for a, e in zip(arrays, expected)):
+            raise ValueError("Stencil file length is invalid")
+        if unit is not None and unit.size != 3 * source_points:
+            raise ValueError("Stencil unit sphere length is invalid")
+        return cls(
+            sphere_points=sphere_points.reshape(n_points, 3),
+            faces=faces.reshape(n_tri, 3),
+            surface_indices=surface_indices.reshape(n_points, 3),
+            surface_weights=surface_weights.reshape(n_points, 3),
+            sheet_indices=sheet_indices.reshape(ny, nx, 3),
+            sheet_weights=sheet_weights.reshape(ny, nx, 3),
+            nx=nx,
+            ny=ny,
+            source_points=source_points,
+            unit_sphere_points=None if unit is None else unit.reshape(source_points, 3),
+        )
+
+    def to_file(self, path: str | Path) -> None:
+        """Write the v2 layout of native/cat_surface_stencil_builder.c:428-461."""
+
+        version = 2 if self.unit_sphere_points is not None else 1
+        with Path(path).open("wb") as stream:
+            stream.write(_HEADER.pack(_MAGIC, version, self.sphere_points.shape[0],
+                                      self.faces.shape[0], self.nx * self.ny, self.nx, self.ny,
+                                      self.source_points))
+            np.asarray(self.sphere_points, "<f8").tofile(stream)
+            np.asarray(self.faces, "<i4").tofile(stream)
+            np.asarray(self.surface_indices, "<i4").tofile(stream)
+            np.asarray(self.surface_weights, "<f8").tofile(stream)
+            np.asarray(self.sheet_indices, "<i4").tofile(stream)
+            np.asarray(self.sheet_weights, "<f8").tofile(stream)
+            if self.unit_sphere_points is not None:
+                np.asarray(self.unit_sphere_points, "<f4").tofile(stream)
+
+    def to(self, device: str | torch.device, *, geometry_dtype: torch.dtype = torch.float32,
+           weight_dtype: torch.dtype = torch.float64) -> "SurfaceStencilDevice":
+        """Upload the stencil and its cached topology tables (surface_stencil.py:193-261)."""
+
+        from .dartel_grid import resolve_device
+
+        target = resolve_device(device)
+        topo = topology_for(self.faces, int(self.sphere_points.shape[0]), target)
+
+        def dev(a, dtype):
+            return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=target)
+
+        return SurfaceStencilDevice(
+            sphere_points=dev(self.sphere_points, geometry_dtype),
+            faces=topo.faces,
+            surface_indices=dev(self.surface_indices, torch.int32),
+            surface_weights=dev(self.surface_weights, torch.float64),
+            sheet_indices=dev(self.sheet_indices, torch.int32),
+            sheet_weights=dev(self.sheet_weights, torch.float64),
+            nx=self.nx,
+            ny=self.ny,
+            source_points=self.source_points,
+            unit_sphere_points=(None if self.unit_sphere_points is None
+                                else dev(self.unit_sphere_points, torch.float32)),
+            topology=topo,
+        )
+
+
+# --------------------------------------------------------------------- device ops
+
+
+def heat_parameters_from_edge_mean(edge_mean: float, fwhm: float) -> tuple[int, float]:
+    """iterations = ⌈fwhm²·0.541011/ē²⌉, σ = fwhm·0.7355345/√iterations.
+
+    surface_stencil.py:359-370
+    """
+
+    raw = float(fwhm) * float(fwhm) * 0.541011 / (edge_mean * edge_mean)
+    iterations = max(1, int(np.ceil(raw)))
+    sigma = float(fwhm) * 0.7355345 / float(iterations ** 0.5)
+    return iterations, sigma
+
+
+@dataclass
+class SurfaceStencilDevice:
+    """Device-resident stencil and topology tables for repeated use."""
+
+    sphere_points: torch.Tensor
+    faces: torch.Tensor
+    surface_indices: torch.Tensor
+    surface_weights: torch.Tensor
+    sheet_indices: torch.Tensor
+    sheet_weights: torch.Tensor
+    nx: int
+    ny: int
+    source_points: int
+    unit_sphere_points: torch.Tensor | None = None
+    topology: Topology | None = None
+
+    @property
+    def device(self) -> torch.device:
+        return self.sphere_points.device
+
+    def _stream(self) -> int:
+        return _lib.stream_handle(self.device)
+
+    def _call(self, name: str, *args) -> None:
+        _lib.check(getattr(_lib.lib(), name)(*args), name)
+
+    # ---- resampling and sheets
+
+    def resample_vertices(self, vertices: torch.Tensor) -> torch.Tensor:
+        """Resample source vertices onto the stencil sphere (surface_stencil.py:282-306)."""
+
+        value = torch.as_tensor(vertices, device=self.device)
+        if value.ndim != 2 or tuple(value.shape) != (self.source_points, 3):
+            raise ValueError(f"vertices shape must be {(self.source_points, 3)}, "
+                             f"got {tuple(value.shape)}")
+        if value.dtype not in (torch.float32, torch.float64):
+            value = value.to(torch.float32)
+        value = value.contiguous()
+        n = int(self.surface_indices.shape[0])
+        out = torch.empty((n, 3), dtype=value.dtype, device=self.device)
+        self._call("csg_resample_vertices", _lib.ptr(value), _lib.dtype_code(value.dtype),
+                   _lib.ptr(self.surface_indices), _lib.ptr(self.surface_weights), n,
+                   _lib.ptr(out), self._stream())
+        return out
+
+    def map_values_to_sheet(self, values: torch.Tensor) -> torch.Tensor:
+        """sheet = Σ_c values[idx]·w (surface_stencil.py:308-317)."""
+
+        value = torch.as_tensor(values, dtype=torch.float64, device=self.device).reshape(-1)
+        if value.numel() != self.sphere_points.shape[0]:
+            raise ValueError(f"values must contain {self.sphere_points.shape[0]} elements, "
+                             f"got {value.numel()}")
+        value = value.contiguous()
+        out = torch.empty((self.ny, self.nx), dtype=torch.float64, device=self.device)
+        self._call("csg_sheet_gather", _lib.ptr(value), value.numel(), 1,
+                   _lib.ptr(self.sheet_indices), _lib.ptr(self.sheet_weights),
+                   self.nx * self.ny, _lib.ptr(out), self._stream())
+        return out
+
+    # ---- heat kernel
+
+    def edge_mean_distance(self, vertices: torch.Tensor) -> float:
+        """Mean face-edge length (surface_stencil.py:349-357); synchronises."""
+
+        v = torch.as_tensor(vertices, dtype=torch.float32, device=self.device).contiguous()
+        nf = int(self.faces.shape[0])
+        partial = torch.empty(_lib.lib().csg_edge_mean_workspace_bytes(nf) // 8,
+                              dtype=torch.float64, device=self.device)
+        nb = _lib.ctypes.c_int()
+        _lib.check(_lib.lib().csg_edge_sum(_lib.ptr(v), _lib.ptr(self.faces), nf,
+                                           _lib.ptr(partial), _lib.ctypes.byref(nb),
+                                           self._stream()), "csg_edge_sum")
+        return float(partial[: nb.value].sum().item()) / (3.0 * nf)
+
+    def _heat_parameters(self, vertices: torch.Tensor, fwhm: float) -> tuple[int, float]:
+        return heat_parameters_from_edge_mean(self.edge_mean_distance(vertices), fwhm)
+
+    def smooth_geometry(self, vertices: torch.Tensor, fwhm: float,
+                        iterations: int | None = None) -> torch.Tensor:
+        """Iterative heat-kernel smoothing of geometry (surface_stencil.py:372-388)."""
+
+        v = torch.as_tensor(vertices, dtype=torch.float32, device=self.device).contiguous()
+        if iterations is None:
+            iterations, _ = self._heat_parameters(v, fwhm)
+        topo = self.topology
+        n = int(v.shape[0])
+        out = torch.empty_like(v)
+        ws = torch.empty_like(v)
+        self._call("csg_smooth_geometry", _lib.ptr(v), _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr),
+                   n, int(iterations), _lib.ptr(out), _lib.ptr(ws), ws.numel() * 4,
+                   self._stream())
+        return out
+
+    def _smooth_values_batch(self, values: torch.Tensor, geometry: torch.Tensor,
+                             params: list[tuple[int, float]], squared_norm: bool) -> torch.Tensor:
+        topo = self.topology
+        n = int(geometry.shape[0])
+        nb = len(params)
+        vals = values.to(torch.float64).contiguous()
+        geom = geometry.to(torch.float32).contiguous()
+        out = torch.empty((nb, n), dtype=torch.float64, device=self.device)
+        ws = _lib.workspace(_lib.lib().csg_smooth_values_workspace_bytes(n, topo.nnz, nb),
+                            self.device)
+        sig = _lib.doubles([p[1] for p in params])
+        its = (_lib.ctypes.c_int * nb)(*[int(p[0]) for p in params])
+        self._call("csg_smooth_values", _lib.ptr(vals), n, nb, _lib.ptr(geom),
+                   _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), topo.nnz, sig, its,
+                   int(squared_norm), _lib.ptr(out), _lib.ptr(ws), ws.numel(), self._stream())
+        return out
+
+    def smooth_values(self, values: torch.Tensor, vertices: torch.Tensor,
+                      fwhm: float) -> torch.Tensor:
+        """Iterative heat-kernel smoothing of values (surface_stencil.py:390-412)."""
+
+        geom = torch.as_tensor(vertices, device=self.device)
+        params = [self._heat_parameters(geom, fwhm)]
+        vals = torch.as_tensor(values, dtype=torch.float64, device=self.device).reshape(1, -1)
+        return self._smooth_values_batch(vals, geom, params, False)[0]
+
+    def smooth_values_many(self, values: torch.Tensor, vertices: torch.Tensor,
+                           fwhms: tuple[float, ...]) -> tuple[torch.Tensor, ...]:
+        """Several fwhms at once (surface_stencil.py:414-476)."""
+
+        if not fwhms:
+            return ()
+        base = torch.as_tensor(values, dtype=torch.float64, device=self.device)
+        if base.ndim != 1 or base.numel() != self.sphere_points.shape[0]:
+            raise ValueError("smooth_values_many values must have shape [sphere_points]")
+        geom = torch.as_tensor(vertices, device=self.device)
+        if tuple(geom.shape) != tuple(base.shape) + (3,):
+            raise ValueError("smooth_values_many vertices must have shape [sphere_points, 3]")
+        em = self.edge_mean_distance(geom)
+        params = [heat_parameters_from_edge_mean(em, float(f)) for f in fwhms]
+        rows = base.reshape(1, -1).expand(len(fwhms), -1)
+        out = self._smooth_values_batch(rows, geom, params, True)
+        return tuple(out[i] for i in range(len(fwhms)))
+
+    def vertex_normals(self, vertices: torch.Tensor) -> torch.Tensor:
+        """Angle-weighted vertex normals (surface_stencil.py:478-505)."""
+
+        v = torch.as_tensor(vertices, dtype=torch.float32, device=self.device).contiguous()
+        out = torch.empty_like(v)
+        topo = self.topology
+        self._call("csg_vertex_normals", _lib.ptr(v), _lib.ptr(topo.faces),
+                   _lib.ptr(topo.inc_off), _lib.ptr(topo.inc), int(v.shape[0]), _lib.ptr(out),
+                   self._stream())
+        return out
+
+    # ---- curvature sheets
+
+    def curvature_type5(self, vertices: torch.Tensor, fwhm: float) -> torch.Tensor:
+        """Depth along fwhm-50 smoothed normals, smoothed and normalised (:507-519)."""
+
+        geometry = torch.as_tensor(vertices, dtype=torch.float32, device=self.device).contiguous()
+        smoothed = self.smooth_geometry(geometry, 50.0)
+        normals = self.vertex_normals(smoothed)
+        base = ((smoothed - geometry) * normals).sum(dim=-1).to(torch.float64)
+        values = self.smooth_values(base, geometry, fwhm)
+        return _minmax(values)
+
+    def curvature_type5_to_sheet_many(self, vertices: torch.Tensor, fwhms: tuple[float, ...],
+                                      out_dtype: torch.dtype = torch.float64,
+                                      ) -> tuple[torch.Tensor, ...]:
+        """One fused call for all fwhms (surface_stencil.py:521-550)."""
+
+        if not fwhms:
+            return ()
+        geometry = torch.as_tensor(vertices, dtype=torch.float32, device=self.device).contiguous()
+        em = self.edge_mean_distance(geometry)
+        geom_iters, _ = heat_parameters_from_edge_mean(em, 50.0)
+        params = [heat_parameters_from_edge_mean(em, float(f)) for f in fwhms]
+        out = self._type5(geometry, geom_iters, params, out_dtype)
+        return tuple(out[i] for i in range(len(fwhms)))
+
+    def _type5(self, geometry: torch.Tensor, geom_iters: int, params, out_dtype) -> torch.Tensor:
+        topo = self.topology
+        n = int(geometry.shape[0])
+        nb = len(params)
+        m = self.nx * self.ny
+        out = torch.empty((nb, self.ny, self.nx), dtype=out_dtype, device=self.device)
+        ws = _lib.workspace(_lib.lib().csg_type5_workspace_bytes(n, topo.nnz, nb, m), self.device)
+        sig = _lib.doubles([p[1] for p in params])
+        its = (_lib.ctypes.c_int * nb)(*[int(p[0]) for p in params])
+        self._call("csg_type5_sheets", _lib.ptr(geometry), n, _lib.ptr(topo.faces),
+                   _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), topo.nnz, _lib.ptr(topo.inc_off),
+                   _lib.ptr(topo.inc), int(geom_iters), nb, sig, its,
+                   _lib.ptr(self.sheet_indices), _lib.ptr(self.sheet_weights), m,
+                   _lib.dtype_code(out_dtype), _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+                   self._stream())
+        return out
+
+    def curvature_to_sheet(self, vertices: torch.Tensor, fwhm: float, *,
+                           curvtype: int = 5) -> torch.Tensor:
+        """Supported CAT curvature mapped to a normalised sheet (:624-644)."""
+
+        if curvtype != 5:
+            raise NotImplementedError(
+                "this build implements curvtype 5 (used by steps <= 2); curvtype 2 is not built"
+            )
+        return self.curvature_type5_to_sheet_many(vertices, (float(fwhm),))[0]
+
+
+def _minmax(values: torch.Tensor, floor: float = 0.0) -> torch.Tensor:
+    """(x - min) / max(max - min, floor) on the device."""
+
+    x = values.to(torch.float64).contiguous()
+    out = torch.empty_like(x)
+    ws = _lib.workspace(_lib.lib().csg_minmax_workspace_bytes(1), x.device)
+    _lib.check(_lib.lib().csg_minmax_normalise(_lib.ptr(x), x.numel(), 1, float(floor), 1,
+                                               _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+                                               _lib.stream_handle(x.device)),
+               "csg_minmax_normalise")
+    return out

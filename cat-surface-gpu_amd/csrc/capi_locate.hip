@@ -1,0 +1,149 @@
+// C ABI of point location: grid index, rotation costs, closest triangles.
+#include "../../include/csg.h"
+#include "locate_kernels.h"
+
+using namespace csg;
+
+namespace {
+
+inline hipStream_t S(void *s) { return (hipStream_t)s; }
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct PlanLayout {
+    size_t box, counts, offsets, scratch, total;
+};
+
+PlanLayout plan_layout(int nf)
+{
+    PlanLayout l;
+    l.box = 0;
+    l.counts = l.box + align256(sizeof(int) * 6 * (size_t)nf);
+    l.offsets = l.counts + align256(sizeof(int) * ((size_t)nf + 1));
+    l.scratch = l.offsets + align256(sizeof(int) * ((size_t)nf + 1));
+    size_t scan = grid_sort_scratch_bytes(1);
+    l.total = l.scratch + align256(scan);
+    return l;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t csg_grid_index_plan_workspace_bytes(int nf) { return plan_layout(nf).total; }
+
+int csg_grid_index_plan(const void *pts, int pts_dtype, const int *faces, int nf, int grid,
+                        int margin, void *ws, size_t ws_bytes, int *npairs, void *stream)
+{
+    if (nf < 1 || grid < 1 || margin < 0 || npairs == nullptr)
+        return fail(kInvalidArgument, "invalid grid index arguments");
+    if ((long long)grid * grid * grid >= (1LL << 30))
+        return fail(kInvalidArgument, "grid size too large");
+    PlanLayout l = plan_layout(nf);
+    if (ws_bytes < l.total)
+        return fail(kInvalidArgument, "grid index plan workspace too small");
+    char *b = (char *)ws;
+    int *box = (int *)(b + l.box), *counts = (int *)(b + l.counts),
+        *offsets = (int *)(b + l.offsets);
+    // the scan reads nf + 1 counts: the extra one must be zero
+    CSG_HIP_CHECK(hipMemsetAsync(counts + nf, 0, sizeof(int), S(stream)));
+    if (pts_dtype == 1)
+        return grid_index_count<double>((const double *)pts, faces, nf, grid, margin, box, counts,
+                                        offsets, npairs, b + l.scratch, ws_bytes - l.scratch,
+                                        S(stream));
+    return grid_index_count<float>((const float *)pts, faces, nf, grid, margin, box, counts,
+                                   offsets, npairs, b + l.scratch, ws_bytes - l.scratch,
+                                   S(stream));
+}
+
+size_t csg_grid_index_fill_workspace_bytes(int npairs)
+{
+    return 3 * align256(sizeof(int) * (size_t)npairs) + align256(grid_sort_scratch_bytes(npairs));
+}
+
+int csg_grid_index_fill(const void *plan_ws, int nf, int grid, int npairs, int *cell_off,
+                        int *cell_faces, void *ws, size_t ws_bytes, void *stream)
+{
+    if (npairs < 1)
+        return fail(kRuntimeError, "At least one triangle does not cover any spatial grid cell");
+    if (ws_bytes < csg_grid_index_fill_workspace_bytes(npairs))
+        return fail(kInvalidArgument, "grid index fill workspace too small");
+    PlanLayout l = plan_layout(nf);
+    const char *pb = (const char *)plan_ws;
+    char *b = (char *)ws;
+    size_t a = align256(sizeof(int) * (size_t)npairs);
+    int *keys_tmp = (int *)b, *vals_tmp = (int *)(b + a), *keys = (int *)(b + 2 * a);
+    return grid_index_fill((const int *)(pb + l.box), (const int *)(pb + l.offsets), nf, grid,
+                           npairs, keys_tmp, vals_tmp, keys, cell_faces, cell_off, b + 3 * a,
+                           ws_bytes - 3 * a, S(stream));
+}
+
+size_t csg_rotation_costs_workspace_bytes(int ns, int na)
+{
+    return align256(sizeof(double) * (size_t)grid_for(ns) * na) + 256;
+}
+
+int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
+                       const double *tgt_vals, const int *faces, const float *tri,
+                       const int *cell_off, const int *cell_faces, int grid,
+                       const double *angles, int na, double *costs, int *loc_face,
+                       float *loc_bary, void *ws, size_t ws_bytes, int *uncovered_host,
+                       void *stream)
+{
+    if (ns < 1 || na < 1 || grid < 1)
+        return fail(kInvalidArgument, "invalid rotation cost arguments");
+    if (ws_bytes < csg_rotation_costs_workspace_bytes(ns, na))
+        return fail(kInvalidArgument, "rotation cost workspace too small");
+    hipStream_t s = S(stream);
+    char *b = (char *)ws;
+    RotationCostArgs a{};
+    a.src_pts = src_pts;
+    a.src_vals = src_vals;
+    a.ns = ns;
+    a.tgt_vals = tgt_vals;
+    a.faces = faces;
+    a.tri = tri;
+    a.cell_off = cell_off;
+    a.cell_faces = cell_faces;
+    a.grid = grid;
+    a.angles = angles;
+    a.na = na;
+    a.costs = costs;
+    a.partial = (double *)b;
+    a.uncovered = (int *)(b + align256(sizeof(double) * (size_t)grid_for(ns) * na));
+    a.loc_face = loc_face;
+    a.loc_bary = loc_bary;
+    CSG_HIP_CHECK(hipMemsetAsync(a.uncovered, 0, sizeof(int), s));
+    int rc = launch_rotation_costs(a, s);
+    if (rc)
+        return rc;
+    if (uncovered_host) {
+        CSG_HIP_CHECK(hipMemcpyAsync(uncovered_host, a.uncovered, sizeof(int),
+                                     hipMemcpyDeviceToHost, s));
+        CSG_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return kOk;
+}
+
+int csg_closest_triangles(const double *q, int nq, const double *verts, const int *faces,
+                          const int *cell_off, const int *cell_faces, int grid, double reach,
+                          int *face_out, int *idx_out, double *w_out, int *far_host,
+                          void *ws, size_t ws_bytes, void *stream)
+{
+    if (nq < 0 || grid < 1 || ws_bytes < sizeof(int))
+        return fail(kInvalidArgument, "invalid closest-triangle arguments");
+    hipStream_t s = S(stream);
+    int *far_count = (int *)ws;
+    CSG_HIP_CHECK(hipMemsetAsync(far_count, 0, sizeof(int), s));
+    int rc = launch_closest_triangles(q, nq, verts, faces, cell_off, cell_faces, grid,
+                                      reach * reach, face_out, idx_out, w_out, far_count, s);
+    if (rc)
+        return rc;
+    if (far_host) {
+        CSG_HIP_CHECK(hipMemcpyAsync(far_host, far_count, sizeof(int), hipMemcpyDeviceToHost, s));
+        CSG_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return kOk;
+}
+
+}  // extern "C"

@@ -1,0 +1,329 @@
+// C ABI of the surface side of the hot path (declared in include/csg.h).
+#include "../../include/csg.h"
+#include "surface_kernels.h"
+
+using namespace csg;
+
+namespace {
+
+struct Carve {
+    char *base;
+    size_t used;
+    template <typename U>
+    U *take(size_t count)
+    {
+        size_t bytes = (count * sizeof(U) + 255) & ~(size_t)255;
+        U *p = base ? reinterpret_cast<U *>(base + used) : nullptr;
+        used += bytes;
+        return p;
+    }
+};
+
+inline hipStream_t S(void *s) { return (hipStream_t)s; }
+
+#define CSG_TRY(expr)      \
+    do {                   \
+        int _rc = (expr);  \
+        if (_rc != kOk)    \
+            return _rc;    \
+    } while (0)
+
+// Shared pieces of the heat-value smoothing: workspace layout and run.
+struct HeatWs {
+    double *w, *wsum, *buf[2];
+};
+
+HeatWs carve_heat(Carve &c, int n, int nnz, int nb)
+{
+    HeatWs h;
+    h.w = c.take<double>((size_t)nb * nnz);
+    h.wsum = c.take<double>((size_t)nb * n);
+    h.buf[0] = c.take<double>((size_t)nb * n);
+    h.buf[1] = c.take<double>((size_t)nb * n);
+    return h;
+}
+
+int make_batch(int nb, const double *sigma, const int *iterations, HeatBatch &hb)
+{
+    if (nb < 1 || nb > kMaxHeatBatch)
+        return fail(kInvalidArgument, "value smoothing batch must be in [1, %d], got %d",
+                    kMaxHeatBatch, nb);
+    hb.nb = nb;
+    for (int i = 0; i < nb; ++i) {
+        if (iterations[i] < 0)
+            return fail(kInvalidArgument, "heat iteration counts must be non-negative");
+        hb.counts[i] = iterations[i];
+        hb.sigma[i] = sigma[i];
+    }
+    return kOk;
+}
+
+// values (already in h.buf[0], nb rows of n) smoothed in place; returns the buffer
+// holding the result.
+int run_heat(const float *geom, const int *off, const int *nbr, int n, int nnz,
+             const HeatBatch &hb, int squared_norm, HeatWs &h, double **result, hipStream_t s)
+{
+    CSG_TRY(launch_heat_weights(geom, off, nbr, n, hb, squared_norm, (size_t)nnz, h.w, h.wsum, s));
+    int most = 0;
+    for (int i = 0; i < hb.nb; ++i)
+        most = hb.counts[i] > most ? hb.counts[i] : most;
+    int cur = 0;
+    for (int it = 0; it < most; ++it) {
+        CSG_TRY(launch_smooth_values(h.buf[cur], off, nbr, h.w, h.wsum, n, (size_t)nnz, hb, it,
+                                     h.buf[cur ^ 1], s));
+        cur ^= 1;
+    }
+    *result = h.buf[cur];
+    return kOk;
+}
+
+}  // namespace
+
+extern "C" {
+
+int csg_resample_vertices(const void *x, int dtype, const int *idx, const double *w, int n_out,
+                          void *out, void *stream)
+{
+    if (n_out < 0)
+        return fail(kInvalidArgument, "n_out must be non-negative");
+    if (dtype == 0)
+        return launch_resample<float>((const float *)x, idx, w, n_out, (float *)out, S(stream));
+    if (dtype == 1)
+        return launch_resample<double>((const double *)x, idx, w, n_out, (double *)out,
+                                       S(stream));
+    return fail(kInvalidArgument, "dtype must be 0 or 1");
+}
+
+size_t csg_edge_mean_workspace_bytes(int nf) { return (size_t)grid_for(nf) * 8 + 1024; }
+
+int csg_edge_sum(const float *v, const int *faces, int nf, double *partial_out, int *nblocks,
+                 void *stream)
+{
+    if (nf < 1)
+        return fail(kInvalidArgument, "at least one face is required");
+    return launch_edge_sum(v, faces, nf, partial_out, nblocks, S(stream));
+}
+
+int csg_smooth_geometry(const float *v, const int *nbr_off, const int *nbr, int n,
+                        int iterations, float *out, void *ws, size_t ws_bytes, void *stream)
+{
+    if (n < 1 || iterations < 0)
+        return fail(kInvalidArgument, "invalid smoothing arguments");
+    if (ws_bytes < (size_t)3 * n * sizeof(float))
+        return fail(kInvalidArgument, "smooth_geometry workspace too small");
+    hipStream_t s = S(stream);
+    float *bufs[2] = {out, (float *)ws};
+    // choose the starting buffer so that the final iterate lands in `out`
+    int cur = (iterations % 2 == 0) ? 0 : 1;
+    CSG_HIP_CHECK(hipMemcpyAsync(bufs[cur], v, (size_t)3 * n * sizeof(float),
+                                 hipMemcpyDeviceToDevice, s));
+    for (int it = 0; it < iterations; ++it) {
+        CSG_TRY(launch_smooth_geometry(bufs[cur], nbr_off, nbr, n, bufs[cur ^ 1], s));
+        cur ^= 1;
+    }
+    return kOk;
+}
+
+int csg_vertex_normals(const float *v, const int *faces, const int *inc_off, const int *inc,
+                       int n, float *out, void *stream)
+{
+    if (n < 1)
+        return fail(kInvalidArgument, "at least one vertex is required");
+    return launch_vertex_normals(v, faces, inc_off, inc, n, out, S(stream));
+}
+
+size_t csg_smooth_values_workspace_bytes(int n, int nnz, int nb)
+{
+    Carve c{nullptr, 0};
+    carve_heat(c, n, nnz, nb);
+    return c.used;
+}
+
+int csg_smooth_values(const double *values, int n, int nb, const float *geom, const int *nbr_off,
+                      const int *nbr, int nnz, const double *sigma, const int *iterations,
+                      int squared_norm, double *out, void *ws, size_t ws_bytes, void *stream)
+{
+    HeatBatch hb;
+    CSG_TRY(make_batch(nb, sigma, iterations, hb));
+    if (ws_bytes < csg_smooth_values_workspace_bytes(n, nnz, nb))
+        return fail(kInvalidArgument, "smooth_values workspace too small");
+    hipStream_t s = S(stream);
+    Carve c{(char *)ws, 0};
+    HeatWs h = carve_heat(c, n, nnz, nb);
+    CSG_HIP_CHECK(hipMemcpyAsync(h.buf[0], values, sizeof(double) * n * nb,
+                                 hipMemcpyDeviceToDevice, s));
+    double *res = nullptr;
+    CSG_TRY(run_heat(geom, nbr_off, nbr, n, nnz, hb, squared_norm, h, &res, s));
+    CSG_HIP_CHECK(hipMemcpyAsync(out, res, sizeof(double) * n * nb, hipMemcpyDeviceToDevice, s));
+    return kOk;
+}
+
+size_t csg_minmax_workspace_bytes(int nb) { return (size_t)nb * 64 * 2 * 8 + 1024; }
+
+int csg_minmax_normalise(const double *x, size_t n, int nb, double floor_, int out_dtype,
+                         void *out, void *ws, size_t ws_bytes, void *stream)
+{
+    if (n < 1 || nb < 1)
+        return fail(kInvalidArgument, "min/max normalisation needs data");
+    if (ws_bytes < csg_minmax_workspace_bytes(nb))
+        return fail(kInvalidArgument, "min/max workspace too small");
+    if (out_dtype == 0)
+        return launch_minmax_normalise<float>(x, n, nb, n, floor_, (double *)ws, (float *)out, n,
+                                              S(stream));
+    return launch_minmax_normalise<double>(x, n, nb, n, floor_, (double *)ws, (double *)out, n,
+                                           S(stream));
+}
+
+int csg_sheet_gather(const double *values, int n_values, int nb, const int *idx, const double *w,
+                     int m, double *out, void *stream)
+{
+    if (m < 1 || nb < 1)
+        return fail(kInvalidArgument, "sheet gather needs data");
+    return launch_sheet_gather(values, (size_t)n_values, idx, w, m, nb, out, S(stream));
+}
+
+size_t csg_type5_workspace_bytes(int n, int nnz, int nb, int m)
+{
+    Carve c{nullptr, 0};
+    c.take<float>((size_t)3 * n);  // smoothed
+    c.take<float>((size_t)3 * n);  // ping-pong
+    c.take<float>((size_t)3 * n);  // normals
+    c.take<double>((size_t)n);     // base values
+    carve_heat(c, n, nnz, nb);
+    c.take<double>((size_t)nb * n);  // normalised values
+    c.take<double>((size_t)nb * m);  // raw sheets
+    c.take<double>((size_t)nb * 64 * 2 + 64);
+    return c.used;
+}
+
+// curvature_type5_to_sheet_many: fwhm-50 geometry smoothing, normals, depth
+// along normals, per-fwhm value smoothing, min/max, sheet gather, min/max.
+// src/cat_surface_gpu/surface_stencil.py:521-550
+int csg_type5_sheets(const float *geom, int n, const int *faces, const int *nbr_off,
+                     const int *nbr, int nnz, const int *inc_off, const int *inc,
+                     int geom_iterations, int nb, const double *sigma, const int *iterations,
+                     const int *sheet_idx, const double *sheet_w, int m, int out_dtype, void *out,
+                     void *ws, size_t ws_bytes, void *stream)
+{
+    HeatBatch hb;
+    CSG_TRY(make_batch(nb, sigma, iterations, hb));
+    if (ws_bytes < csg_type5_workspace_bytes(n, nnz, nb, m))
+        return fail(kInvalidArgument, "type-5 workspace too small");
+    hipStream_t s = S(stream);
+    Carve c{(char *)ws, 0};
+    float *sm = c.take<float>((size_t)3 * n);
+    float *pp = c.take<float>((size_t)3 * n);
+    float *nr = c.take<float>((size_t)3 * n);
+    double *base = c.take<double>((size_t)n);
+    HeatWs h = carve_heat(c, n, nnz, nb);
+    double *normed = c.take<double>((size_t)nb * n);
+    double *sheets = c.take<double>((size_t)nb * m);
+    double *mm = c.take<double>((size_t)nb * 64 * 2 + 64);
+    CSG_TRY(csg_smooth_geometry(geom, nbr_off, nbr, n, geom_iterations, sm, pp,
+                                (size_t)3 * n * sizeof(float), stream));
+    CSG_TRY(launch_vertex_normals(sm, faces, inc_off, inc, n, nr, s));
+    CSG_TRY(launch_type5_base(geom, sm, nr, n, base, s));
+    for (int b = 0; b < nb; ++b)
+        CSG_HIP_CHECK(hipMemcpyAsync(h.buf[0] + (size_t)b * n, base, sizeof(double) * n,
+                                     hipMemcpyDeviceToDevice, s));
+    double *res = nullptr;
+    CSG_TRY(run_heat(geom, nbr_off, nbr, n, nnz, hb, 1, h, &res, s));
+    CSG_TRY(launch_minmax_normalise<double>(res, (size_t)n, nb, (size_t)n, 0.0, mm, normed,
+                                            (size_t)n, s));
+    CSG_TRY(launch_sheet_gather(normed, (size_t)n, sheet_idx, sheet_w, m, nb, sheets, s));
+    if (out_dtype == 0)
+        return launch_minmax_normalise<float>(sheets, (size_t)m, nb, (size_t)m, 0.0, mm,
+                                              (float *)out, (size_t)m, s);
+    return launch_minmax_normalise<double>(sheets, (size_t)m, nb, (size_t)m, 0.0, mm,
+                                           (double *)out, (size_t)m, s);
+}
+
+size_t csg_depth_workspace_bytes(int n, int nf, int nnz)
+{
+    Carve c{nullptr, 0};
+    c.take<double>((size_t)9 * nf);  // fnormal
+    c.take<double>((size_t)3 * nf);  // farea
+    c.take<double>((size_t)9 * nf);  // coo values
+    c.take<double>((size_t)3 * n);   // normals
+    c.take<double>((size_t)n);       // areas
+    c.take<double>((size_t)nnz);     // csr values
+    c.take<double>((size_t)n);       // mean curvature
+    c.take<double>((size_t)n);       // diagonal
+    c.take<double>((size_t)n);       // rhs
+    c.take<double>((size_t)2 * grid_for(n) + 64);
+    return c.used;
+}
+
+// Depth potential of the rotation feature: (L + alpha·diag(area)) x = rhs by
+// colour-ordered SOR (rotation_feature.py:253-317).  colour_verts lists the
+// vertices of colour c at colour_off[c]..colour_off[c+1].  relres (device
+// double) receives |rhs - A x| / |rhs|.
+int csg_depth_potential(const float *pts, int n, const int *faces, int nf, const int *inc_off,
+                        const int *inc, const int *row_off, const int *cols, const int *perm,
+                        const unsigned char *is_alpha, int nnz, const int *colour_off,
+                        const int *colour_verts, int ncolours, double alpha, int max_iter,
+                        double omega, double *x, double *relres, void *ws, size_t ws_bytes,
+                        void *stream)
+{
+    if (n < 1 || nf < 1 || max_iter < 0)
+        return fail(kInvalidArgument, "invalid depth-potential arguments");
+    if (ws_bytes < csg_depth_workspace_bytes(n, nf, nnz))
+        return fail(kInvalidArgument, "depth-potential workspace too small");
+    hipStream_t s = S(stream);
+    Carve c{(char *)ws, 0};
+    double *fnormal = c.take<double>((size_t)9 * nf);
+    double *farea = c.take<double>((size_t)3 * nf);
+    double *coo = c.take<double>((size_t)9 * nf);
+    double *normals = c.take<double>((size_t)3 * n);
+    double *areas = c.take<double>((size_t)n);
+    double *val = c.take<double>((size_t)nnz);
+    double *mc = c.take<double>((size_t)n);
+    double *diag = c.take<double>((size_t)n);
+    double *rhs = c.take<double>((size_t)n);
+    double *partial = c.take<double>((size_t)2 * grid_for(n) + 64);
+    CSG_TRY(launch_depth_face_terms(pts, faces, nf, fnormal, farea, coo, s));
+    CSG_TRY(launch_depth_vertex_terms(fnormal, farea, inc_off, inc, n, normals, areas, s));
+    CSG_TRY(launch_csr_values(coo, perm, nnz, 9 * nf, areas, alpha, val, s));
+    CSG_TRY(launch_depth_rhs(pts, row_off, cols, val, is_alpha, normals, areas, n, mc, diag,
+                             partial, rhs, s));
+    CSG_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    // colour offsets are host data
+    for (int it = 0; it < max_iter; ++it)
+        for (int col = 0; col < ncolours; ++col)
+            CSG_TRY(launch_sor_color(colour_verts + colour_off[col],
+                                     colour_off[col + 1] - colour_off[col], row_off, cols, val,
+                                     rhs, diag, omega, x, s));
+    if (relres)
+        CSG_TRY(launch_relative_residual(row_off, cols, val, rhs, x, n, partial, relres, s));
+    return kOk;
+}
+
+int csg_apply_flow(const float *unit, int n, const void *flow, int nx, int ny, int dtype,
+                   int inverse, void *out, void *stream)
+{
+    if (nx < 1 || ny < 1 || n < 0)
+        return fail(kInvalidArgument, "invalid warp arguments");
+    if (dtype == 0)
+        return launch_apply_flow<float>(unit, n, (const float *)flow, nx, ny, inverse,
+                                        (float *)out, S(stream));
+    if (dtype == 1)
+        return launch_apply_flow<double>(unit, n, (const double *)flow, nx, ny, inverse,
+                                         (double *)out, S(stream));
+    return fail(kInvalidArgument, "dtype must be 0 or 1");
+}
+
+int csg_average_xz(const void *avg, const void *main_, int n, int dtype, const double *unrotate,
+                   float *out, void *stream)
+{
+    if (n < 0 || unrotate == nullptr)
+        return fail(kInvalidArgument, "invalid average arguments");
+    if (dtype == 0)
+        return launch_average_xz<float>((const float *)avg, (const float *)main_, n, unrotate,
+                                        out, S(stream));
+    if (dtype == 1)
+        return launch_average_xz<double>((const double *)avg, (const double *)main_, n, unrotate,
+                                         out, S(stream));
+    return fail(kInvalidArgument, "dtype must be 0 or 1");
+}
+
+}  // extern "C"

@@ -199,12 +199,6 @@ int fmg_run(const T *hes, const T *rhs, const double *params, int cycles, int ni
     if (c.used > ws_bytes)
         return fail(kInvalidArgument, "FMG workspace too small: need %zu bytes, got %zu", c.used,
                     ws_bytes);
-    for (int l = 0; l < plan.L; ++l)
-        if (plan.nx[l] % 2 == 1 && plan.nx[l] != 1)
-            return fail(kInvalidArgument,
-                        "red-black relaxation on the GPU requires an even x dimension at every "
-                        "multigrid level (level %d has nx=%d)",
-                        l, plan.nx[l]);
     size_t P0 = plan.P(0);
     plan.A[0] = const_cast<T *>(hes);
     plan.B0[0] = plan.B[0] = const_cast<T *>(rhs);

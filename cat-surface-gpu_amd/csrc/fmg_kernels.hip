@@ -61,20 +61,38 @@ k_resize(const T *__restrict__ in, int channels, int snx, int sny, int tnx, int 
 // One colour of the red-black sweep, in place.  Pixels with (x + y) & 1 ==
 // color are updated from their four neighbours (opposite colour for even nx,
 // or the pixel itself through the boundaries).  dartel_triton.py:241-291
+//
+// For odd nx > 1 the reference's CPU path updates row-major in place
+// (dartel_grid.py:1139-1185); the only same-colour neighbours are then
+// x = 0 and x = nx - 1 of a row whose sweep starts at x = 0, and the
+// sequential order updates x = 0 first.  phase 0 updates every active pixel
+// except x = nx - 1 of odd-width rows, phase 1 updates those last pixels, so
+// the result equals the sequential sweep exactly.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_relax_color(const T *__restrict__ a, const T *__restrict__ b, T *__restrict__ u, T w00, T w01,
-              T w10, int nx, int ny, size_t astride, size_t bstride, int color)
+              T w10, int nx, int ny, size_t astride, size_t bstride, int color, int phase)
 {
     const int P = nx * ny;
     int half = (nx + 1) >> 1;
     int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= half * ny)
-        return;
-    int y = q / half;
-    int x = 2 * (q - y * half) + ((color + y) & 1);
-    if (x >= nx)
-        return;
+    int x, y;
+    const bool odd = (nx & 1) && nx > 1;
+    if (phase == 0) {
+        if (q >= half * ny)
+            return;
+        y = q / half;
+        x = 2 * (q - y * half) + ((color + y) & 1);
+        if (x >= nx || (odd && x == nx - 1))
+            return;
+    } else {
+        if (q >= ny)
+            return;
+        y = q;
+        x = nx - 1;
+        if (((x + y) & 1) != color)
+            return;
+    }
     a += blockIdx.y * astride;
     b += blockIdx.y * bstride;
     u += blockIdx.y * bstride;
@@ -178,8 +196,14 @@ int launch_relax_sweep(const T *a, const T *b, T *u, MembraneWeights w, int nx, 
     int half = (nx + 1) / 2;
     dim3 g(grid_for((size_t)half * ny), batch);
     hipLaunchKernelGGL(k_relax_color<T>, g, dim3(kBlock), 0, s, a, b, u, (T)w.w00, (T)w.w01,
-                       (T)w.w10, nx, ny, astride, bstride, color);
+                       (T)w.w10, nx, ny, astride, bstride, color, 0);
     CSG_LAUNCH_CHECK();
+    if ((nx & 1) && nx > 1) {
+        dim3 g1(grid_for((size_t)ny), batch);
+        hipLaunchKernelGGL(k_relax_color<T>, g1, dim3(kBlock), 0, s, a, b, u, (T)w.w00, (T)w.w01,
+                           (T)w.w10, nx, ny, astride, bstride, color, 1);
+        CSG_LAUNCH_CHECK();
+    }
     return kOk;
 }
 
@@ -187,10 +211,6 @@ template <typename T>
 int launch_relax(const T *a, const T *b, T *u, MembraneWeights w, int nx, int ny, int batch,
                  size_t astride, size_t bstride, int nit, hipStream_t s)
 {
-    if (nx % 2 == 1 && nx != 1)
-        return fail(kInvalidArgument,
-                    "red-black relaxation on the GPU requires an even x dimension, got nx=%d",
-                    nx);
     for (int it = 0; it < 2 * nit; ++it) {
         int rc = launch_relax_sweep<T>(a, b, u, w, nx, ny, batch, astride, bstride,
                                        (it % 2 == 0) ? 1 : 0, s);

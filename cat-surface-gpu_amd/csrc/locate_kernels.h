@@ -1,0 +1,42 @@
+// Launch wrappers for the point-location kernels (internal to libcsg_hip).
+#pragma once
+
+#include "csg_common.h"
+
+namespace csg {
+
+template <typename P>
+int grid_index_count(const P *pts, const int *faces, int nf, int g, int margin, int *box,
+                     int *counts, int *offsets, int *total, void *tmp, size_t tmp_bytes,
+                     hipStream_t s);
+size_t grid_sort_scratch_bytes(int npairs);
+int grid_index_fill(const int *box, const int *offsets, int nf, int g, int npairs, int *keys_tmp,
+                    int *vals_tmp, int *keys, int *vals, int *cell_off, void *tmp,
+                    size_t tmp_bytes, hipStream_t s);
+
+struct RotationCostArgs {
+    const float *src_pts;
+    const double *src_vals;
+    int ns;
+    const double *tgt_vals;
+    const int *faces;
+    const float *tri;
+    const int *cell_off, *cell_faces;
+    int grid;
+    const double *angles;
+    int na;
+    double *costs;
+    double *partial;  // [na][ceil(ns / 256)]
+    int *uncovered;
+    int *loc_face;    // [na][ns] or null
+    float *loc_bary;  // [na][ns][3] or null
+};
+
+int launch_rotation_costs(const RotationCostArgs &a, hipStream_t s);
+
+int launch_closest_triangles(const double *q, int nq, const double *verts, const int *faces,
+                             const int *cell_off, const int *cell_faces, int g, double reach2,
+                             int *face_out, int *idx_out, double *w_out, int *far_count,
+                             hipStream_t s);
+
+}  // namespace csg

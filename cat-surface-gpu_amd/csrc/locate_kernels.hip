@@ -1,0 +1,482 @@
+// Point location on triangulated spheres for gfx950: the uniform-grid
+// triangle index, the CAT rotation cost (rotate → locate → barycentric →
+// float64 SSD, batched over candidate rotations), and the exact
+// closest-triangle query used by the stencil builder.
+//
+// Reference semantics: src/cat_surface_gpu/surface_rotation.py:57-120
+// (grid index), :223-313 (rotation matrices), :334-546 (locator and cost);
+// native/cat_surface_stencil_builder.c:107-185 (stencil queries).
+#include <hipcub/hipcub.hpp>
+
+#include "../../include/csg.h"
+#include "locate_kernels.h"
+
+namespace csg {
+
+// ------------------------------------------------------------ grid index
+
+// Per-face cell box (bbox ± margin cells, clipped) and its cell count.
+// surface_rotation.py:80-87
+template <typename P>
+__global__ void __launch_bounds__(kBlock)
+k_face_boxes(const P *__restrict__ pts, const int *__restrict__ faces, int nf, int g, int margin,
+             int *__restrict__ box, int *__restrict__ counts)
+{
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf)
+        return;
+    double cell = 2.0 / (double)g;
+    int lo[3], hi[3], total = 1;
+    for (int d = 0; d < 3; ++d) {
+        double a = (double)pts[3 * faces[3 * f] + d], b = (double)pts[3 * faces[3 * f + 1] + d],
+               c = (double)pts[3 * faces[3 * f + 2] + d];
+        double mn = fmin(fmin(a, b), c), mx = fmax(fmax(a, b), c);
+        long l = (long)floor((mn + 1.0) / cell) - margin;
+        long h = (long)floor((mx + 1.0) / cell) + margin;
+        l = l < 0 ? 0 : (l > g - 1 ? g - 1 : l);
+        h = h < 0 ? 0 : (h > g - 1 ? g - 1 : h);
+        lo[d] = (int)l;
+        hi[d] = (int)h;
+        total *= (hi[d] >= lo[d]) ? (hi[d] - lo[d] + 1) : 0;
+    }
+    for (int d = 0; d < 3; ++d) {
+        box[6 * f + d] = lo[d];
+        box[6 * f + 3 + d] = hi[d];
+    }
+    counts[f] = total;
+}
+
+// (cell, face) pairs, cells of one face in z, y, x order.
+__global__ void __launch_bounds__(kBlock)
+k_emit_pairs(const int *__restrict__ box, const int *__restrict__ offsets, int nf, int g,
+             int *__restrict__ keys, int *__restrict__ vals)
+{
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf)
+        return;
+    int o = offsets[f];
+    const int *b = box + 6 * f;
+    for (int z = b[2]; z <= b[5]; ++z)
+        for (int y = b[1]; y <= b[4]; ++y)
+            for (int x = b[0]; x <= b[3]; ++x) {
+                keys[o] = x + g * (y + g * z);
+                vals[o] = f;
+                ++o;
+            }
+}
+
+// cell_off[c] = first position of key >= c in the sorted keys.
+__global__ void __launch_bounds__(kBlock)
+k_cell_offsets(const int *__restrict__ keys, int npairs, int ncells, int *__restrict__ off)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncells)
+        return;
+    int lo = 0, hi = npairs;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (keys[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    off[c] = lo;
+}
+
+template <typename P>
+int grid_index_count(const P *pts, const int *faces, int nf, int g, int margin, int *box,
+                     int *counts, int *offsets, int *total, void *tmp, size_t tmp_bytes,
+                     hipStream_t s)
+{
+    hipLaunchKernelGGL(k_face_boxes<P>, dim3(grid_for(nf)), dim3(kBlock), 0, s, pts, faces, nf, g,
+                       margin, box, counts);
+    CSG_LAUNCH_CHECK();
+    size_t need = 0;
+    CSG_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, counts, offsets, nf + 1, s));
+    if (need > tmp_bytes)
+        return fail(kInvalidArgument, "grid index scan scratch too small");
+    CSG_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, need, counts, offsets, nf + 1, s));
+    CSG_HIP_CHECK(hipMemcpyAsync(total, offsets + nf, sizeof(int), hipMemcpyDeviceToHost, s));
+    CSG_HIP_CHECK(hipStreamSynchronize(s));
+    return kOk;
+}
+
+size_t grid_sort_scratch_bytes(int npairs)
+{
+    size_t need = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, need, (int *)nullptr, (int *)nullptr,
+                                       (int *)nullptr, (int *)nullptr, npairs, 0, 32);
+    size_t scan = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (int *)nullptr, (int *)nullptr, 1 << 22);
+    return (need > scan ? need : scan) + 1024;
+}
+
+int grid_index_fill(const int *box, const int *offsets, int nf, int g, int npairs, int *keys_tmp,
+                    int *vals_tmp, int *keys, int *vals, int *cell_off, void *tmp,
+                    size_t tmp_bytes, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_emit_pairs, dim3(grid_for(nf)), dim3(kBlock), 0, s, box, offsets, nf, g,
+                       keys_tmp, vals_tmp);
+    CSG_LAUNCH_CHECK();
+    int bits = 1;
+    while ((1LL << bits) < (long long)g * g * g)
+        ++bits;
+    size_t need = 0;
+    CSG_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, keys_tmp, keys, vals_tmp, vals,
+                                                     npairs, 0, bits, s));
+    if (need > tmp_bytes)
+        return fail(kInvalidArgument, "grid index sort scratch too small");
+    CSG_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, need, keys_tmp, keys, vals_tmp, vals,
+                                                     npairs, 0, bits, s));
+    int ncells = g * g * g;
+    hipLaunchKernelGGL(k_cell_offsets, dim3(grid_for((size_t)ncells + 1)), dim3(kBlock), 0, s,
+                       keys, npairs, ncells, cell_off);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+template int grid_index_count<float>(const float *, const int *, int, int, int, int *, int *,
+                                     int *, int *, void *, size_t, hipStream_t);
+template int grid_index_count<double>(const double *, const int *, int, int, int, int *, int *,
+                                      int *, int *, void *, size_t, hipStream_t);
+
+// ------------------------------------------------------------ rotation cost
+
+__device__ __forceinline__ float dot3(const float *a, const float *b)
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+__device__ __forceinline__ void cross3(const float *a, const float *b, float *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__device__ __forceinline__ void seg_dist(const float *q, const float *a, const float *b,
+                                         float *dist, float *pt)
+{
+    float dir[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    float off[3] = {q[0] - a[0], q[1] - a[1], q[2] - a[2]};
+    float den = dot3(dir, dir);
+    float al = den != 0.0f ? dot3(off, dir) / den : 0.0f;
+    al = fminf(fmaxf(al, 0.0f), 1.0f);
+    for (int d = 0; d < 3; ++d)
+        pt[d] = a[d] + al * dir[d];
+    float e0 = q[0] - pt[0], e1 = q[1] - pt[1], e2 = q[2] - pt[2];
+    *dist = e0 * e0 + e1 * e1 + e2 * e2;
+}
+
+// Rotation matrix of (alpha, beta, gamma) in float32, Rz (Ry Rx) assembled
+// column-major and read row-major.  surface_rotation.py:223-292
+__device__ void rotation_matrix_f32(double a0, double a1, double a2, float *m)
+{
+    float al = (float)a0, be = (float)a1, ga = (float)a2;
+    float ca = cosf(al), sa = sinf(al), cb = cosf(be), sb = sinf(be), cg = cosf(ga),
+          sg = sinf(ga);
+    float rx[9] = {1, 0, 0, 0, ca, sa, 0, -sa, ca};
+    float ry[9] = {cb, 0, sb, 0, 1, 0, -sb, 0, cb};
+    float rz[9] = {cg, sg, 0, -sg, cg, 0, 0, 0, 1};
+    float first[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            first[r + 3 * c] = ry[r] * rx[3 * c] + ry[r + 3] * rx[1 + 3 * c] +
+                               ry[r + 6] * rx[2 + 3 * c];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            m[r + 3 * c] = rz[r] * first[3 * c] + rz[r + 3] * first[1 + 3 * c] +
+                           rz[r + 6] * first[2 + 3 * c];
+}
+
+// Locate q among the candidates of its cell; returns the face (or -1) and
+// the barycentric weights of the chosen point.  surface_rotation.py:354-447
+__device__ int locate_point(const float *q, const float *__restrict__ tri,
+                            const int *__restrict__ cand, int ncand, float *bary)
+{
+    float best = INFINITY, bp[3] = {0.0f, 0.0f, 0.0f};
+    int bf = -1;
+    for (int c = 0; c < ncand; ++c) {
+        int face = cand[c];
+        const float *t = tri + 9 * (size_t)face;
+        float p0[3] = {t[0], t[1], t[2]}, p1[3] = {t[3], t[4], t[5]}, p2[3] = {t[6], t[7], t[8]};
+        float ea[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+        float eb[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+        float nrm[3];
+        cross3(ea, eb, nrm);
+        float nsq = dot3(nrm, nrm);
+        float off[3] = {p0[0] - q[0], p0[1] - q[1], p0[2] - q[2]};
+        float pt = nsq != 0.0f ? dot3(off, nrm) / nsq : 0.0f;
+        float proj[3] = {q[0] + pt * nrm[0], q[1] + pt * nrm[1], q[2] + pt * nrm[2]};
+        float po[3] = {proj[0] - p0[0], proj[1] - p0[1], proj[2] - p0[2]};
+        float xx = dot3(ea, ea), xy = dot3(ea, eb), xv = dot3(ea, po), yy = dot3(eb, eb),
+              yv = dot3(eb, po);
+        float den = xx * yy - xy * xy;
+        float xp = den != 0.0f ? (xv * yy - yv * xy) / den : 0.0f;
+        float yp = den != 0.0f ? (yv * xx - xv * xy) / den : 0.0f;
+        bool inside = den != 0.0f && xp >= -1.0e-3f && xp <= 1.0f + 1.0e-3f && yp >= -1.0e-3f &&
+                      yp <= 1.0f + 1.0e-3f && xp + yp >= -1.0e-3f && xp + yp <= 1.0f + 1.0e-3f;
+        float dist, cpt[3];
+        if (inside) {
+            dist = pt * pt * nsq;
+            cpt[0] = proj[0];
+            cpt[1] = proj[1];
+            cpt[2] = proj[2];
+        } else {
+            const float *vs[3] = {p0, p1, p2};
+            float dv[3];
+            for (int k = 0; k < 3; ++k) {
+                float e0 = q[0] - vs[k][0], e1 = q[1] - vs[k][1], e2 = q[2] - vs[k][2];
+                dv[k] = e0 * e0 + e1 * e1 + e2 * e2;
+            }
+            int cv = 0;
+            if (dv[1] < dv[cv])
+                cv = 1;
+            if (dv[2] < dv[cv])
+                cv = 2;
+            float d1, d2, q1[3], q2[3];
+            seg_dist(q, vs[(cv + 2) % 3], vs[cv], &d1, q1);
+            seg_dist(q, vs[cv], vs[(cv + 1) % 3], &d2, q2);
+            if (d1 < d2) {
+                dist = d1;
+                cpt[0] = q1[0];
+                cpt[1] = q1[1];
+                cpt[2] = q1[2];
+            } else {
+                dist = d2;
+                cpt[0] = q2[0];
+                cpt[1] = q2[1];
+                cpt[2] = q2[2];
+            }
+        }
+        if (dist < best) {
+            best = dist;
+            bf = face;
+            bp[0] = cpt[0];
+            bp[1] = cpt[1];
+            bp[2] = cpt[2];
+        }
+    }
+    if (bf < 0)
+        return -1;
+    const float *t = tri + 9 * (size_t)bf;
+    const float *tv[3] = {t, t + 3, t + 6};
+    const int order[3][3] = {{1, 2, 0}, {2, 0, 1}, {0, 1, 2}};
+    for (int k = 0; k < 3; ++k) {
+        const float *fi = tv[order[k][0]], *se = tv[order[k][1]], *th = tv[order[k][2]];
+        float po[3] = {bp[0] - fi[0], bp[1] - fi[1], bp[2] - fi[2]};
+        float ho[3] = {se[0] - fi[0], se[1] - fi[1], se[2] - fi[2]};
+        float up[3] = {th[0] - fi[0], th[1] - fi[1], th[2] - fi[2]};
+        float nn[3], ve[3];
+        cross3(ho, up, nn);
+        cross3(nn, ho, ve);
+        bary[k] = dot3(po, ve) / dot3(up, ve);
+    }
+    return bf;
+}
+
+__device__ __forceinline__ int cell_of(const float *p, int g)
+{
+    int c[3];
+    for (int d = 0; d < 3; ++d) {
+        int k = (int)floorf((p[d] + 1.0f) * ((float)g / 2.0f));
+        c[d] = k < 0 ? 0 : (k > g - 1 ? g - 1 : k);
+    }
+    return c[0] + g * (c[1] + g * c[2]);
+}
+
+// One block of points for one candidate rotation (blockIdx.y); per-block
+// float64 partial SSD.  `uncovered` counts points whose cell has no triangle.
+__global__ void __launch_bounds__(kBlock)
+k_rotation_cost(const float *__restrict__ src_pts, const double *__restrict__ src_vals, int ns,
+                const double *__restrict__ tgt_vals, const int *__restrict__ faces,
+                const float *__restrict__ tri, const int *__restrict__ cell_off,
+                const int *__restrict__ cell_faces, int g, const double *__restrict__ angles,
+                double *__restrict__ partial, int nblocks, int *__restrict__ uncovered,
+                int *__restrict__ loc_face, float *__restrict__ loc_bary)
+{
+    __shared__ double red[kBlock / 64];
+    __shared__ float m[9];
+    const int a = blockIdx.y;
+    if (threadIdx.x == 0)
+        rotation_matrix_f32(angles[3 * a], angles[3 * a + 1], angles[3 * a + 2], m);
+    __syncthreads();
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (i < ns) {
+        const float *p = src_pts + 3 * i;
+        float q[3];
+        for (int r = 0; r < 3; ++r)
+            q[r] = p[0] * m[3 * r] + p[1] * m[3 * r + 1] + p[2] * m[3 * r + 2];
+        int cell = cell_of(q, g);
+        int b = cell_off[cell], nc = cell_off[cell + 1] - b;
+        float w[3];
+        int f = nc > 0 ? locate_point(q, tri, cell_faces + b, nc, w) : -1;
+        if (f < 0) {
+            atomicAdd(uncovered, 1);
+        } else {
+            double sv = tgt_vals[faces[3 * f]] * (double)w[0] +
+                        tgt_vals[faces[3 * f + 1]] * (double)w[1] +
+                        tgt_vals[faces[3 * f + 2]] * (double)w[2];
+            double d = src_vals[i] - sv;
+            e = d * d;
+            if (loc_face) {
+                loc_face[(size_t)a * ns + i] = f;
+                for (int k = 0; k < 3; ++k)
+                    loc_bary[((size_t)a * ns + i) * 3 + k] = w[k];
+            }
+        }
+    }
+    double s = block_sum(e, red);
+    if (threadIdx.x == 0)
+        partial[(size_t)a * nblocks + blockIdx.x] = s;
+}
+
+__global__ void k_sum_rows(const double *__restrict__ partial, int nblocks, int rows,
+                           double *__restrict__ out)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows)
+        return;
+    double s = 0.0;
+    for (int k = 0; k < nblocks; ++k)
+        s += partial[(size_t)r * nblocks + k];
+    out[r] = s;
+}
+
+int launch_rotation_costs(const RotationCostArgs &a, hipStream_t s)
+{
+    int nb = (int)grid_for(a.ns);
+    dim3 grid(nb, a.na);
+    hipLaunchKernelGGL(k_rotation_cost, grid, dim3(kBlock), 0, s, a.src_pts, a.src_vals, a.ns,
+                       a.tgt_vals, a.faces, a.tri, a.cell_off, a.cell_faces, a.grid, a.angles,
+                       a.partial, nb, a.uncovered, a.loc_face, a.loc_bary);
+    CSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_rows, dim3(grid_for(a.na, 64)), dim3(64), 0, s, a.partial, nb, a.na,
+                       a.costs);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+// ------------------------------------------------------------ closest triangle
+
+// Closest point on a triangle with its barycentric weights (float64), by the
+// Voronoi-region method of Ericson, Real-Time Collision Detection, 5.1.5.
+__device__ double closest_on_triangle(const double *p, const double *a, const double *b,
+                                      const double *c, double *w)
+{
+    double ab[3], ac[3], ap[3];
+    for (int d = 0; d < 3; ++d) {
+        ab[d] = b[d] - a[d];
+        ac[d] = c[d] - a[d];
+        ap[d] = p[d] - a[d];
+    }
+    auto dot = [](const double *u, const double *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+    double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) {
+        w[0] = 1; w[1] = 0; w[2] = 0;
+    } else {
+        double bp[3] = {p[0] - b[0], p[1] - b[1], p[2] - b[2]};
+        double d3 = dot(ab, bp), d4 = dot(ac, bp);
+        if (d3 >= 0.0 && d4 <= d3) {
+            w[0] = 0; w[1] = 1; w[2] = 0;
+        } else {
+            double vc = d1 * d4 - d3 * d2;
+            if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+                double v = d1 / (d1 - d3);
+                w[0] = 1 - v; w[1] = v; w[2] = 0;
+            } else {
+                double cp[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
+                double d5 = dot(ab, cp), d6 = dot(ac, cp);
+                if (d6 >= 0.0 && d5 <= d6) {
+                    w[0] = 0; w[1] = 0; w[2] = 1;
+                } else {
+                    double vb = d5 * d2 - d1 * d6;
+                    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+                        double t = d2 / (d2 - d6);
+                        w[0] = 1 - t; w[1] = 0; w[2] = t;
+                    } else {
+                        double va = d3 * d6 - d5 * d4;
+                        if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+                            double t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+                            w[0] = 0; w[1] = 1 - t; w[2] = t;
+                        } else {
+                            double den = 1.0 / (va + vb + vc);
+                            double v = vb * den, t = vc * den;
+                            w[0] = 1 - v - t; w[1] = v; w[2] = t;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    double e = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        double q = w[0] * a[d] + w[1] * b[d] + w[2] * c[d];
+        e += (p[d] - q) * (p[d] - q);
+    }
+    return e;
+}
+
+// For each query: closest triangle among the candidates of its cell (first
+// minimum); `far` counts queries whose best squared distance exceeds
+// reach2, i.e. whose true closest triangle may lie outside the searched cells.
+__global__ void __launch_bounds__(kBlock)
+k_closest_triangles(const double *__restrict__ q, int nq, const double *__restrict__ verts,
+                    const int *__restrict__ faces, const int *__restrict__ cell_off,
+                    const int *__restrict__ cell_faces, int g, double reach2,
+                    int *__restrict__ face_out, int *__restrict__ idx_out,
+                    double *__restrict__ w_out, int *__restrict__ far_count)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq)
+        return;
+    double p[3] = {q[3 * i], q[3 * i + 1], q[3 * i + 2]};
+    double cell = 2.0 / (double)g;
+    int c[3];
+    for (int d = 0; d < 3; ++d) {
+        long k = (long)floor((p[d] + 1.0) / cell);
+        c[d] = (int)(k < 0 ? 0 : (k > g - 1 ? g - 1 : k));
+    }
+    int id = c[0] + g * (c[1] + g * c[2]);
+    double best = INFINITY, bw[3] = {0, 0, 0};
+    int bf = -1;
+    for (int k = cell_off[id]; k < cell_off[id + 1]; ++k) {
+        int f = cell_faces[k];
+        const int *fv = faces + 3 * f;
+        double w[3];
+        double dist = closest_on_triangle(p, verts + 3 * fv[0], verts + 3 * fv[1],
+                                          verts + 3 * fv[2], w);
+        if (dist < best) {
+            best = dist;
+            bf = f;
+            bw[0] = w[0];
+            bw[1] = w[1];
+            bw[2] = w[2];
+        }
+    }
+    if (bf < 0 || best > reach2)
+        atomicAdd(far_count, 1);
+    face_out[i] = bf;
+    for (int d = 0; d < 3; ++d) {
+        w_out[3 * i + d] = bw[d];
+        if (idx_out)
+            idx_out[3 * i + d] = bf >= 0 ? faces[3 * bf + d] : 0;
+    }
+}
+
+int launch_closest_triangles(const double *q, int nq, const double *verts, const int *faces,
+                             const int *cell_off, const int *cell_faces, int g, double reach2,
+                             int *face_out, int *idx_out, double *w_out, int *far_count,
+                             hipStream_t s)
+{
+    if (nq <= 0)
+        return kOk;
+    hipLaunchKernelGGL(k_closest_triangles, dim3(grid_for(nq)), dim3(kBlock), 0, s, q, nq, verts,
+                       faces, cell_off, cell_faces, g, reach2, face_out, idx_out, w_out,
+                       far_count);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+}  // namespace csg

@@ -1,0 +1,62 @@
+// Launch wrappers for the surface-side kernels (internal to libcsg_hip).
+#pragma once
+
+#include "csg_common.h"
+
+namespace csg {
+
+// Up to kMaxHeatBatch value-smoothing problems that share a mesh, each with
+// its own iteration count and Gaussian width.
+constexpr int kMaxHeatBatch = 8;
+struct HeatBatch {
+    int nb;
+    int counts[kMaxHeatBatch];
+    double sigma[kMaxHeatBatch];
+};
+
+template <typename T>
+int launch_resample(const T *x, const int *idx, const double *w, int n, T *out, hipStream_t s);
+int launch_edge_sum(const float *v, const int *faces, int nf, double *partial, int *nblocks,
+                    hipStream_t s);
+int launch_smooth_geometry(const float *cur, const int *off, const int *nbr, int n, float *nxt,
+                           hipStream_t s);
+int launch_vertex_normals(const float *v, const int *faces, const int *inc_off, const int *inc,
+                          int n, float *out, hipStream_t s);
+int launch_type5_base(const float *geom, const float *sm, const float *nr, int n, double *out,
+                      hipStream_t s);
+int launch_heat_weights(const float *geom, const int *off, const int *nbr, int n,
+                        const HeatBatch &hb, int squared_norm, size_t wstride, double *w,
+                        double *wsum, hipStream_t s);
+int launch_smooth_values(const double *cur, const int *off, const int *nbr, const double *w,
+                         const double *wsum, int n, size_t wstride, const HeatBatch &hb,
+                         int iteration, double *nxt, hipStream_t s);
+template <typename T>
+int launch_minmax_normalise(const double *x, size_t n, int nb, size_t stride, double floor_,
+                            double *partial, T *out, size_t ostride, hipStream_t s);
+int launch_sheet_gather(const double *vals, size_t vstride, const int *idx, const double *w,
+                        int m, int nb, double *out, hipStream_t s);
+int launch_depth_face_terms(const float *pts, const int *faces, int nf, double *fnormal,
+                            double *farea, double *coo, hipStream_t s);
+int launch_depth_vertex_terms(const double *fnormal, const double *farea, const int *inc_off,
+                              const int *inc, int n, double *normals, double *areas,
+                              hipStream_t s);
+int launch_csr_values(const double *coo, const int *perm, int nnz, int nface9,
+                      const double *areas, double alpha, double *val, hipStream_t s);
+int launch_depth_rhs(const float *pts, const int *row_off, const int *cols, const double *val,
+                     const unsigned char *is_alpha, const double *normals, const double *areas,
+                     int n, double *mc, double *diag, double *partial, double *rhs,
+                     hipStream_t s);
+int launch_sor_color(const int *verts, int count, const int *row_off, const int *cols,
+                     const double *val, const double *rhs, const double *diag, double omega,
+                     double *x, hipStream_t s);
+int launch_relative_residual(const int *row_off, const int *cols, const double *val,
+                             const double *rhs, const double *x, int n, double *partial,
+                             double *out, hipStream_t s);
+template <typename T>
+int launch_apply_flow(const float *unit, int n, const T *flow, int nx, int ny, int inverse,
+                      T *out, hipStream_t s);
+template <typename T>
+int launch_average_xz(const T *avg, const T *main_, int n, const double *m, float *out,
+                      hipStream_t s);
+
+}  // namespace csg

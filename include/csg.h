@@ -137,6 +137,115 @@ int csg_solve_dartel_maps(const void *src, const void *tgt, int nx, int ny, int 
                           int code, int assume_nonzero, int dtype, void *flow, void *velocity,
                           double *metrics, void *ws, size_t ws_bytes, void *stream);
 
+/* ---------------------------------------------------------------- surface */
+
+/* Stencil resampling of [n_src][3] vertices to [n_out][3] (dtype 0: float32 with the
+ * reference's per-corner float32 accumulation; 1: float64).
+ * Replaces SurfaceStencilDevice.resample_vertices (src/cat_surface_gpu/surface_stencil.py:282-306). */
+int csg_resample_vertices(const void *x, int dtype, const int *idx, const double *w, int n_out,
+                          void *out, void *stream);
+
+/* Per-block partial sums of face edge lengths (partial_out: ceil(nf/256) doubles);
+ * the mean edge length is their sum / (3 nf).
+ * Replaces SurfaceStencilDevice._edge_mean_distance (surface_stencil.py:349-357). */
+size_t csg_edge_mean_workspace_bytes(int nf);
+int csg_edge_sum(const float *v, const int *faces, int nf, double *partial_out, int *nblocks,
+                 void *stream);
+
+/* `iterations` heat iterations of vertex geometry over the ascending CSR neighbour
+ * table.  ws: 3 n floats.  Replaces SurfaceStencilDevice.smooth_geometry
+ * (surface_stencil.py:372-388). */
+int csg_smooth_geometry(const float *v, const int *nbr_off, const int *nbr, int n,
+                        int iterations, float *out, void *ws, size_t ws_bytes, void *stream);
+
+/* Angle-weighted vertex normals; inc lists face*3+corner per vertex in (corner, face)
+ * order.  Replaces SurfaceStencilDevice.vertex_normals (surface_stencil.py:478-505). */
+int csg_vertex_normals(const float *v, const int *faces, const int *inc_off, const int *inc,
+                       int n, float *out, void *stream);
+
+/* nb (<= 8) value-smoothing problems on one mesh; sigma and iterations are host arrays.
+ * squared_norm selects smooth_values_many's distance form.  Replaces
+ * SurfaceStencilDevice.smooth_values / smooth_values_many (surface_stencil.py:390-476). */
+size_t csg_smooth_values_workspace_bytes(int n, int nnz, int nb);
+int csg_smooth_values(const double *values, int n, int nb, const float *geom, const int *nbr_off,
+                      const int *nbr, int nnz, const double *sigma, const int *iterations,
+                      int squared_norm, double *out, void *ws, size_t ws_bytes, void *stream);
+
+/* (x - min) / max(max - min, floor) per row of nb rows of n doubles. */
+size_t csg_minmax_workspace_bytes(int nb);
+int csg_minmax_normalise(const double *x, size_t n, int nb, double floor_, int out_dtype,
+                         void *out, void *ws, size_t ws_bytes, void *stream);
+
+/* sheet[b][p] = sum_c values[b][idx[p,c]] w[p,c].  Replaces
+ * SurfaceStencilDevice.map_values_to_sheet (surface_stencil.py:308-317). */
+int csg_sheet_gather(const double *values, int n_values, int nb, const int *idx, const double *w,
+                     int m, double *out, void *stream);
+
+/* Whole curvature_type5_to_sheet_many for nb fwhms (sigma / iterations host arrays).
+ * Replaces SurfaceStencilDevice.curvature_type5_to_sheet_many (surface_stencil.py:521-550). */
+size_t csg_type5_workspace_bytes(int n, int nnz, int nb, int m);
+int csg_type5_sheets(const float *geom, int n, const int *faces, const int *nbr_off,
+                     const int *nbr, int nnz, const int *inc_off, const int *inc,
+                     int geom_iterations, int nb, const double *sigma, const int *iterations,
+                     const int *sheet_idx, const double *sheet_w, int m, int out_dtype, void *out,
+                     void *ws, size_t ws_bytes, void *stream);
+
+/* Depth potential (L + alpha diag(area)) x = rhs by colour-ordered SOR; colour_off is a
+ * host array of ncolours + 1 offsets into the device array colour_verts.
+ * Replaces rotation_feature._solve_depth_potential_colored_sor
+ * (src/cat_surface_gpu/rotation_feature.py:253-317). */
+size_t csg_depth_workspace_bytes(int n, int nf, int nnz);
+int csg_depth_potential(const float *pts, int n, const int *faces, int nf, const int *inc_off,
+                        const int *inc, const int *row_off, const int *cols, const int *perm,
+                        const unsigned char *is_alpha, int nnz, const int *colour_off,
+                        const int *colour_verts, int ncolours, double alpha, int max_iter,
+                        double omega, double *x, double *relres, void *ws, size_t ws_bytes,
+                        void *stream);
+
+/* Sphere points through a flow (out: [n][3] of dtype).  Replaces
+ * surface_warp.apply_flow_to_sphere (src/cat_surface_gpu/surface_warp.py:500-584). */
+int csg_apply_flow(const float *unit, int n, const void *flow, int nx, int ny, int dtype,
+                   int inverse, void *out, void *stream);
+
+/* Un-rotate the pole-frame warp by `unrotate` (host, row-major 3x3, p m^T) and blend it
+ * with the main warp.  Replaces gpu_pipeline._average_xz_surfaces and its caller
+ * (src/cat_surface_gpu/gpu_pipeline.py:248-271, :1294-1310). */
+int csg_average_xz(const void *avg, const void *main_, int n, int dtype, const double *unrotate,
+                   float *out, void *stream);
+
+/* ---------------------------------------------------------------- location */
+
+/* Uniform-grid triangle index on [-1, 1]^3, two phases: plan (synchronises, returns
+ * the pair count) then fill (cell_off: grid^3 + 1 ints, cell_faces: npairs ints).
+ * Replaces RotationGridIndex.from_geometry (src/cat_surface_gpu/surface_rotation.py:57-120). */
+size_t csg_grid_index_plan_workspace_bytes(int nf);
+int csg_grid_index_plan(const void *pts, int pts_dtype, const int *faces, int nf, int grid,
+                        int margin, void *ws, size_t ws_bytes, int *npairs, void *stream);
+size_t csg_grid_index_fill_workspace_bytes(int npairs);
+int csg_grid_index_fill(const void *plan_ws, int nf, int grid, int npairs, int *cell_off,
+                        int *cell_faces, void *ws, size_t ws_bytes, void *stream);
+
+/* CAT rotation SSD for na candidate angle triples (device); uncovered_host (optional)
+ * receives the count of points outside the index after a synchronisation.
+ * Replaces RotationGridIndexDevice.cost_batch_prepared (surface_rotation.py:512-546). */
+size_t csg_rotation_costs_workspace_bytes(int ns, int na);
+int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
+                       const double *tgt_vals, const int *faces, const float *tri,
+                       const int *cell_off, const int *cell_faces, int grid,
+                       const double *angles, int na, double *costs, int *loc_face,
+                       float *loc_bary, void *ws, size_t ws_bytes, int *uncovered_host,
+                       void *stream);
+
+/* Exact closest triangle (first minimum among the query cell's candidates) with
+ * barycentric weights; far_host counts queries farther than `reach` from their best
+ * triangle (their true closest triangle may lie outside the searched cells).
+ * Replaces the stencil helper's find_closest_polygon_point + weights
+ * (native/cat_surface_stencil_builder.c:107-170). */
+int csg_closest_triangles(const double *q, int nq, const double *verts, const int *faces,
+                          const int *cell_off, const int *cell_faces, int grid, double reach,
+                          int *face_out, int *idx_out, double *w_out, int *far_host,
+                          void *ws, size_t ws_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -342,3 +342,239 @@ def normalise_sphere(vertices) -> np.ndarray:
 def rotate_vertices(vertices, angles) -> np.ndarray:
     m = rotation_matrix_from_angles(angles)
     return (np.asarray(vertices, dtype=np.float64) @ m.T).astype(np.float32)
+
+
+# ---------------------------------------------------------------- surface side
+
+
+def neighbour_csr(faces, n_points: int):
+    """Ascending neighbour lists per vertex (surface_stencil.py:36-64)."""
+
+    f = np.asarray(faces, dtype=np.int64)
+    directed = np.concatenate([f[:, [0, 1]], f[:, [1, 0]], f[:, [1, 2]], f[:, [2, 1]],
+                               f[:, [2, 0]], f[:, [0, 2]]])
+    directed = np.unique(directed, axis=0)  # sorted by (source, target)
+    counts = np.bincount(directed[:, 0], minlength=n_points)
+    off = np.zeros(n_points + 1, np.int32)
+    off[1:] = np.cumsum(counts)
+    return off, np.ascontiguousarray(directed[:, 1].astype(np.int32))
+
+
+def resample_vertices_f32(x, idx, w):
+    x, idx, w = _c32(x), _ci(idx), _c64(w)
+    out = np.empty((idx.shape[0], 3), np.float32)
+    lib().oracle_resample_f32(_f(x), _i(idx), _d(w), ctypes.c_int(idx.shape[0]), _f(out))
+    return out
+
+
+def edge_mean(vertices, faces) -> float:
+    lib().oracle_edge_mean.restype = ctypes.c_double
+    v, f = _c32(vertices), _ci(faces)
+    return float(lib().oracle_edge_mean(_f(v), _i(f), ctypes.c_int(f.shape[0])))
+
+
+def heat_parameters(vertices, faces, fwhm: float):
+    em = edge_mean(vertices, faces)
+    it = ctypes.c_int()
+    sg = ctypes.c_double()
+    lib().oracle_heat_parameters(ctypes.c_double(em), ctypes.c_double(fwhm), ctypes.byref(it),
+                                 ctypes.byref(sg))
+    return it.value, sg.value
+
+
+def smooth_geometry(vertices, faces, fwhm: float):
+    v = _c32(vertices)
+    off, nbr = neighbour_csr(faces, v.shape[0])
+    iters, _ = heat_parameters(v, faces, fwhm)
+    out = np.empty_like(v)
+    lib().oracle_smooth_geometry(_f(v), _i(off), _i(nbr), ctypes.c_int(v.shape[0]), iters,
+                                 _f(out))
+    return out
+
+
+def smooth_values(values, vertices, faces, fwhm: float, squared_norm: bool = False):
+    vals = _c64(values)
+    geom32 = _c32(vertices)
+    geom = _c64(geom32)
+    off, nbr = neighbour_csr(faces, vals.shape[0])
+    iters, sigma = heat_parameters(geom32, faces, fwhm)
+    w = np.empty(nbr.shape[0], np.float64)
+    lib().oracle_heat_weights(_d(geom), _i(off), _i(nbr), ctypes.c_int(vals.shape[0]),
+                              ctypes.c_double(sigma), int(squared_norm), _d(w))
+    out = np.empty_like(vals)
+    lib().oracle_smooth_values(_d(vals), _i(off), _i(nbr), _d(w), ctypes.c_int(vals.shape[0]),
+                               iters, _d(out))
+    return out
+
+
+def vertex_normals(vertices, faces):
+    v, f = _c32(vertices), _ci(faces)
+    out = np.empty_like(v)
+    lib().oracle_vertex_normals(_f(v), _i(f), ctypes.c_int(f.shape[0]), ctypes.c_int(v.shape[0]),
+                                _f(out))
+    return out
+
+
+def sheet_gather(values, idx, w):
+    vals, idx, w = _c64(values), _ci(idx), _c64(w)
+    m = idx.size // 3
+    out = np.empty(m, np.float64)
+    lib().oracle_sheet_gather(_d(vals), _i(idx), _d(w), ctypes.c_int(m), _d(out))
+    return out
+
+
+def minmax_normalise(x):
+    x = _c64(x).copy()
+    lib().oracle_minmax_normalise(_d(x), ctypes.c_int(x.size))
+    return x
+
+
+def type5_base(geometry, faces):
+    g = _c32(geometry)
+    sm = smooth_geometry(g, faces, 50.0)
+    nr = vertex_normals(sm, faces)
+    out = np.empty(g.shape[0], np.float64)
+    lib().oracle_type5_base(_f(g), _f(sm), _f(nr), ctypes.c_int(g.shape[0]), _d(out))
+    return out
+
+
+def curvature_type5_to_sheet_many(geometry, faces, sheet_idx, sheet_w, fwhms):
+    """surface_stencil.py:521-550"""
+
+    g = _c32(geometry)
+    base = type5_base(g, faces)
+    ny, nx = sheet_idx.shape[:2]
+    maps = []
+    for fwhm in fwhms:
+        vals = smooth_values(base, g, faces, fwhm, squared_norm=True)
+        vals = minmax_normalise(vals)
+        sheet = sheet_gather(vals, sheet_idx.reshape(-1, 3), sheet_w.reshape(-1, 3))
+        maps.append(minmax_normalise(sheet).reshape(ny, nx))
+    return maps
+
+
+# ---------------------------------------------------------------- rotation
+
+
+def official_seed_grid_angles() -> np.ndarray:
+    """surface_rotation.py:14-23"""
+
+    seed0 = (-0.6, -0.3, 0.0, 0.3, 0.6)
+    seed12 = (-0.3, 0.0, 0.3)
+    values = [(0.0, 0.0, 0.0)] + [(a, b, c) for a in seed0 for b in seed12 for c in seed12]
+    return np.asarray(values, dtype=np.float64)
+
+
+def grid_index(points, faces, grid_size: int = 128, margin: int = 1):
+    """CSR of triangles per cell of a grid on [-1, 1]^3 (surface_rotation.py:57-120)."""
+
+    points = np.asarray(points, dtype=np.float64)
+    faces = np.asarray(faces, dtype=np.int64)
+    cell = 2.0 / float(grid_size)
+    tri = points[faces]
+    lo = np.floor((tri.min(axis=1) + 1.0) / cell).astype(np.int64)
+    hi = np.floor((tri.max(axis=1) + 1.0) / cell).astype(np.int64)
+    lo = np.clip(lo - margin, 0, grid_size - 1)
+    hi = np.clip(hi + margin, 0, grid_size - 1)
+    cells, owners = [], []
+    for f in range(faces.shape[0]):
+        zs = np.arange(lo[f, 2], hi[f, 2] + 1)
+        ys = np.arange(lo[f, 1], hi[f, 1] + 1)
+        xs = np.arange(lo[f, 0], hi[f, 0] + 1)
+        ids = (xs[None, None, :] + grid_size * (ys[None, :, None] + grid_size * zs[:, None, None]))
+        ids = ids.reshape(-1)
+        cells.append(ids)
+        owners.append(np.full(ids.size, f, np.int64))
+    cells = np.concatenate(cells)
+    owners = np.concatenate(owners)
+    order = np.lexsort((owners, cells))
+    counts = np.bincount(cells, minlength=grid_size ** 3)
+    off = np.zeros(grid_size ** 3 + 1, np.int32)
+    off[1:] = np.cumsum(counts)
+    return off, owners[order].astype(np.int32)
+
+
+def rotation_costs(src_pts, src_vals, tgt_vals, points, faces, angles, grid_size=128, margin=1,
+                   index=None, return_location=False):
+    src_pts, src_vals, tgt_vals = _c32(src_pts), _c64(src_vals), _c64(tgt_vals)
+    faces = _ci(faces)
+    tri = _c32(np.asarray(points, dtype=np.float64)[faces])
+    off, cf = index if index is not None else grid_index(points, faces, grid_size, margin)
+    off, cf = _ci(off), _ci(cf)
+    angles = _c64(np.asarray(angles, dtype=np.float64).reshape(-1, 3))
+    na, ns = angles.shape[0], src_pts.shape[0]
+    costs = np.empty(na, np.float64)
+    lf = np.empty((na, ns), np.int32)
+    lb = np.empty((na, ns, 3), np.float32)
+    rc = lib().oracle_rotation_costs(_f(src_pts), _d(src_vals), ctypes.c_int(ns), _d(tgt_vals),
+                                     _i(faces), _f(tri), _i(off), _i(cf), grid_size, _d(angles),
+                                     ctypes.c_int(na), _d(costs), _i(lf), _f(lb))
+    if rc != 0:
+        raise RuntimeError("Rotation is not covered by target sphere candidate index")
+    if return_location:
+        return costs, lf, lb
+    return costs
+
+
+def rotation_matrix_f32(angles) -> np.ndarray:
+    a = _c64(angles)
+    m = np.empty((3, 3), np.float32)
+    lib().oracle_rotation_matrix_f32(_d(a), _f(m))
+    return m
+
+
+def nelder_mead(cost, seed, max_iter=500, tol=1e-4, simplex_step=0.1):
+    """CAT-ordered Nelder-Mead (surface_rotation.py:594-699); cost(angles[k,3]) -> [k]."""
+
+    seed = np.asarray(seed, dtype=np.float64).reshape(3)
+    simplex = np.stack([seed, seed + (simplex_step, 0, 0), seed + (0, simplex_step, 0),
+                        seed + (0, 0, simplex_step)])
+    f = np.array(cost(simplex), dtype=np.float64)
+    iteration = 0
+    for iteration in range(max_iter):
+        hi, lo, sh = 0, 0, 1
+        for i in range(4):
+            if f[i] > f[hi]:
+                sh, hi = hi, i
+            elif f[i] > f[sh] and i != hi:
+                sh = i
+            if f[i] < f[lo]:
+                lo = i
+        centroid = np.zeros(3)
+        for i in range(4):
+            if i != hi:
+                centroid += simplex[i]
+        centroid /= 3.0
+        refl = centroid + (centroid - simplex[hi])
+        fr = cost(refl[None])[0]
+        if fr < f[lo]:
+            exp = centroid + 2.0 * (refl - centroid)
+            fe = cost(exp[None])[0]
+            if fe < fr:
+                simplex[hi], f[hi] = exp, fe
+            else:
+                simplex[hi], f[hi] = refl, fr
+        elif fr < f[sh]:
+            simplex[hi], f[hi] = refl, fr
+        else:
+            con = centroid + 0.5 * (simplex[hi] - centroid)
+            fc = cost(con[None])[0]
+            if fc < f[hi]:
+                simplex[hi], f[hi] = con, fc
+            else:
+                for i in range(4):
+                    if i != lo:
+                        simplex[i] = simplex[lo] + 0.5 * (simplex[i] - simplex[lo])
+                        f[i] = cost(simplex[i][None])[0]
+        if np.max(np.abs(f - f[lo])) < tol:
+            break
+    return simplex[lo].copy(), float(f[lo]), iteration + 1
+
+
+def closest_triangles(verts, faces, queries):
+    v, f, q = _c64(verts), _ci(faces), _c64(queries)
+    fo = np.empty(q.shape[0], np.int32)
+    w = np.empty((q.shape[0], 3), np.float64)
+    lib().oracle_closest_triangles(_d(v), _i(f), ctypes.c_int(f.shape[0]), _d(q),
+                                   ctypes.c_int(q.shape[0]), _i(fo), _d(w))
+    return fo, w
