@@ -38,3 +38,30 @@ from .surface_warp import (
 )
 
 __version__ = "0.1.0"
+from .surface_warp import (
+    apply_flow_to_sphere,
+    apply_flow_to_stenciled_sphere,
+    solve_dartel_from_surfaces,
+)
+from .surface_stencil import SurfaceStencil, SurfaceStencilDevice
+from .surface_rotation import (
+    RotationCostInputs,
+    RotationGridIndex,
+    RotationGridIndexDevice,
+    RotationSearchResult,
+    official_seed_grid_angles,
+)
+from .rotation_pipeline import (
+    RotationPipeline,
+    read_rotation_points,
+    read_rotation_values,
+    run_rotation_pipeline,
+    run_rotation_pipeline_from_files,
+)
+from .gpu_pipeline import (
+    CatSurfaceGpuResult,
+    GiftiMesh,
+    read_gifti_mesh,
+    run_cat_surface_gpu_pipeline,
+    write_gifti_mesh,
+)

@@ -1,0 +1,355 @@
+"""CAT initial rotation: grid index, SSD cost and search, on MI355X.
+
+Drop-in for src/cat_surface_gpu/surface_rotation.py.  The candidate index is
+built on the GPU; every cost evaluation (rotate in float32, locate in the
+target triangle index, barycentric sample of the target feature, float64
+SSD) is one HIP launch over all points × all candidate angles.  The 46-seed
+grid and the CAT-ordered Nelder-Mead follow the reference step for step; the
+simplex bookkeeping runs on the host with one cost launch per evaluation.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .locate import GridIndex, build_grid_index
+
+
+def official_seed_grid_angles() -> np.ndarray:
+    """The 46 upstream seeds, identity first (surface_rotation.py:14-23)."""
+
+    seed0 = (-0.6, -0.3, 0.0, 0.3, 0.6)
+    seed12 = (-0.3, 0.0, 0.3)
+    values = [(0.0, 0.0, 0.0)]
+    values.extend((a, b, c) for a in seed0 for b in seed12 for c in seed12)
+    return np.asarray(values, dtype=np.float64)
+
+
+def rotation_matrix_np(angles) -> np.ndarray:
+    """Rz (Ry Rx) assembled column-major then read row-major (float64)."""
+
+    a, b, g = (float(x) for x in np.asarray(angles, dtype=np.float64).reshape(3))
+    ca, sa, cb, sb, cg, sg = np.cos(a), np.sin(a), np.cos(b), np.sin(b), np.cos(g), np.sin(g)
+    rx = [1.0, 0.0, 0.0, 0.0, ca, sa, 0.0, -sa, ca]
+    ry = [cb, 0.0, sb, 0.0, 1.0, 0.0, -sb, 0.0, cb]
+    rz = [cg, sg, 0.0, -sg, cg, 0.0, 0.0, 0.0, 1.0]
+    first = [0.0] * 9
+    res = [0.0] * 9
+    for r in range(3):
+        for c in range(3):
+            first[r + 3 * c] = sum(ry[r + 3 * i] * rx[i + 3 * c] for i in range(3))
+    for r in range(3):
+        for c in range(3):
+            res[r + 3 * c] = sum(rz[r + 3 * i] * first[i + 3 * c] for i in range(3))
+    return np.asarray(res, dtype=np.float64).reshape(3, 3)
+
+
+@dataclass(frozen=True)
+class RotationGridIndex:
+    """Host view of the candidate index (surface_rotation.py:26-180)."""
+
+    points: np.ndarray
+    faces: np.ndarray
+    candidate_table: np.ndarray | None
+    grid_size: int
+    margin: int
+    candidate_offsets: np.ndarray | None = None
+    candidate_faces: np.ndarray | None = None
+
+    @classmethod
+    def from_stencil(cls, stencil, *, grid_size: int = 128, margin: int = 1,
+                     include_dense_candidate_table: bool = False) -> "RotationGridIndex":
+        return cls.from_geometry(stencil.sphere_points, stencil.faces, grid_size=grid_size,
+                                 margin=margin,
+                                 include_dense_candidate_table=include_dense_candidate_table)
+
+    @classmethod
+    def from_geometry(cls, points, faces, *, grid_size: int = 128, margin: int = 1,
+                      include_dense_candidate_table: bool = False,
+                      device: str | torch.device = "auto") -> "RotationGridIndex":
+        """Build on the GPU and keep host copies of the CSR arrays."""
+
+        from .dartel_grid import resolve_device
+
+        points = np.asarray(points, dtype=np.float64)
+        faces = np.asarray(faces, dtype=np.int32)
+        if points.ndim != 2 or points.shape[1] != 3:
+            raise ValueError("points must have shape [n, 3]")
+        if faces.ndim != 2 or faces.shape[1] != 3:
+            raise ValueError("faces must have shape [m, 3]")
+        if grid_size < 4 or margin < 0:
+            raise ValueError("grid_size must be at least 4 and margin must be non-negative")
+        dev = resolve_device(device)
+        index = build_grid_index(torch.as_tensor(points, device=dev),
+                                 torch.as_tensor(faces, device=dev), grid_size, margin)
+        offsets = index.cell_off.cpu().numpy()
+        cell_faces = index.cell_faces.cpu().numpy()
+        table = None
+        if include_dense_candidate_table:
+            width = int(np.diff(offsets).max())
+            table = np.full((grid_size ** 3, width), -1, dtype=np.int32)
+            for cell in np.flatnonzero(np.diff(offsets)):
+                vals = cell_faces[offsets[cell]:offsets[cell + 1]]
+                table[cell, : vals.size] = vals
+        return cls(points=np.ascontiguousarray(points), faces=np.ascontiguousarray(faces),
+                   candidate_table=table, grid_size=int(grid_size), margin=int(margin),
+                   candidate_offsets=offsets, candidate_faces=cell_faces)
+
+    @property
+    def max_candidates(self) -> int:
+        if self.candidate_offsets is not None:
+            return int(np.diff(self.candidate_offsets).max())
+        return 0 if self.candidate_table is None else int(self.candidate_table.shape[1])
+
+    def to(self, device, *, geometry_dtype: torch.dtype = torch.float32,
+           candidate_table_dtype: torch.dtype = torch.int32,
+           compressed_candidate_table: bool = True) -> "RotationGridIndexDevice":
+        from .dartel_grid import resolve_device
+
+        dev = resolve_device(device)
+        if candidate_table_dtype not in (torch.int32, torch.int64):
+            raise ValueError("candidate_table_dtype must be int32 or int64")
+        triangles = self.points[self.faces]
+        return RotationGridIndexDevice(
+            target_points=torch.as_tensor(self.points, dtype=geometry_dtype, device=dev),
+            target_faces=torch.as_tensor(self.faces, dtype=torch.int32, device=dev),
+            target_triangles=torch.as_tensor(triangles, dtype=torch.float32, device=dev),
+            grid_size=self.grid_size,
+            candidate_offsets=torch.as_tensor(self.candidate_offsets, device=dev),
+            candidate_faces=torch.as_tensor(self.candidate_faces, device=dev),
+        )
+
+
+@dataclass(frozen=True)
+class RotationCostInputs:
+    source_points: torch.Tensor
+    source_values: torch.Tensor
+    target_values: torch.Tensor
+
+
+@dataclass(frozen=True)
+class RotationSearchResult:
+    seed_angle: torch.Tensor
+    seed_cost: torch.Tensor
+    seed_costs: torch.Tensor
+    angle: torch.Tensor
+    cost: torch.Tensor
+    rotation_matrix: torch.Tensor
+    iterations: int
+
+
+@dataclass
+class RotationGridIndexDevice:
+    """Evaluate CAT rotation candidates against a device-resident index."""
+
+    target_points: torch.Tensor
+    target_faces: torch.Tensor
+    target_triangles: torch.Tensor
+    grid_size: int
+    candidate_offsets: torch.Tensor
+    candidate_faces: torch.Tensor
+    candidate_table: torch.Tensor | None = None
+
+    @classmethod
+    def from_device_geometry(cls, points64: torch.Tensor, faces: torch.Tensor, *,
+                             grid_size: int = 128, margin: int = 1) -> "RotationGridIndexDevice":
+        """Build straight from device tensors (no host round trip)."""
+
+        f = faces.to(torch.int32).contiguous()
+        index: GridIndex = build_grid_index(points64.to(torch.float64), f, grid_size, margin)
+        return cls(target_points=points64.to(torch.float32).contiguous(), target_faces=f,
+                   target_triangles=points64[f.long()].to(torch.float32).contiguous(),
+                   grid_size=grid_size, candidate_offsets=index.cell_off,
+                   candidate_faces=index.cell_faces)
+
+    @property
+    def device(self) -> torch.device:
+        return self.target_points.device
+
+    def rotation_matrix(self, angles) -> torch.Tensor:
+        a = torch.as_tensor(angles, dtype=torch.float64).reshape(-1)
+        if a.numel() != 3:
+            raise ValueError("rotation_matrix accepts exactly one angle vector")
+        return torch.as_tensor(rotation_matrix_np(a.cpu().numpy()), dtype=torch.float32,
+                               device=self.device)
+
+    def rotate_points(self, source_points, angles) -> torch.Tensor:
+        src = torch.as_tensor(source_points, dtype=torch.float32, device=self.device)
+        ang = torch.as_tensor(angles, dtype=torch.float64).reshape(-1, 3).cpu().numpy()
+        mats = torch.as_tensor(np.stack([rotation_matrix_np(a) for a in ang]),
+                               dtype=torch.float32, device=self.device)
+        return torch.matmul(src.unsqueeze(0), mats.transpose(1, 2))
+
+    def prepare_cost_inputs(self, source_points, source_values, target_values) -> RotationCostInputs:
+        src = torch.as_tensor(source_points, dtype=torch.float32, device=self.device).contiguous()
+        sv = torch.as_tensor(source_values, dtype=torch.float64, device=self.device).reshape(-1)
+        tv = torch.as_tensor(target_values, dtype=torch.float64, device=self.device).reshape(-1)
+        if src.ndim != 2 or src.shape[1] != 3:
+            raise ValueError("source_points must have shape [n, 3]")
+        if sv.numel() != src.shape[0]:
+            raise ValueError("source_values length does not match source_points")
+        if tv.numel() != self.target_points.shape[0]:
+            raise ValueError("target_values length does not match the target sphere")
+        return RotationCostInputs(src, sv.contiguous(), tv.contiguous())
+
+    def _costs(self, inputs: RotationCostInputs, angles: torch.Tensor,
+               locate: bool = False):
+        ang = torch.as_tensor(angles, dtype=torch.float64, device=self.device).reshape(-1, 3)
+        ang = ang.contiguous()
+        na, ns = int(ang.shape[0]), int(inputs.source_points.shape[0])
+        costs = torch.empty(na, dtype=torch.float64, device=self.device)
+        loc_f = loc_b = None
+        if locate:
+            loc_f = torch.empty((na, ns), dtype=torch.int32, device=self.device)
+            loc_b = torch.empty((na, ns, 3), dtype=torch.float32, device=self.device)
+        lib = _lib.lib()
+        ws = _lib.workspace(lib.csg_rotation_costs_workspace_bytes(ns, na), self.device)
+        uncovered = ctypes.c_int()
+        _lib.check(lib.csg_rotation_costs(
+            _lib.ptr(inputs.source_points), _lib.ptr(inputs.source_values), ns,
+            _lib.ptr(inputs.target_values), _lib.ptr(self.target_faces),
+            _lib.ptr(self.target_triangles), _lib.ptr(self.candidate_offsets),
+            _lib.ptr(self.candidate_faces), int(self.grid_size), _lib.ptr(ang), na,
+            _lib.ptr(costs), _lib.ptr(loc_f), _lib.ptr(loc_b), _lib.ptr(ws), ws.numel(),
+            ctypes.byref(uncovered), _lib.stream_handle(self.device)), "csg_rotation_costs")
+        if uncovered.value:
+            raise RuntimeError("Rotation is not covered by target sphere candidate index")
+        return costs, loc_f, loc_b
+
+    def cost_batch_prepared(self, inputs: RotationCostInputs, angles, *,
+                            point_chunk: int = 1024) -> torch.Tensor:
+        """SSD per candidate angle (surface_rotation.py:512-546).
+
+        ``point_chunk`` is accepted for API compatibility; all points of all
+        candidates go to the device in one launch.
+        """
+
+        if point_chunk < 1:
+            raise ValueError("point_chunk must be positive")
+        return self._costs(inputs, angles)[0]
+
+    def cost_batch(self, source_points, source_values, target_values, angles, *,
+                   point_chunk: int = 1024) -> torch.Tensor:
+        inputs = self.prepare_cost_inputs(source_points, source_values, target_values)
+        return self.cost_batch_prepared(inputs, angles, point_chunk=point_chunk)
+
+    def locate(self, inputs: RotationCostInputs, angles):
+        """(faces [na, ns], barycentric weights [na, ns, 3]) of the rotated points."""
+
+        _, f, b = self._costs(inputs, angles, locate=True)
+        return f, b
+
+    def coarse_seed_search_prepared(self, inputs: RotationCostInputs, *, point_chunk: int = 1024):
+        """All 46 seeds, first minimum (surface_rotation.py:562-575)."""
+
+        angles = torch.as_tensor(official_seed_grid_angles(), dtype=torch.float64,
+                                 device=self.device)
+        costs = self.cost_batch_prepared(inputs, angles, point_chunk=point_chunk)
+        best = int(torch.argmin(costs).item())
+        return angles[best], costs[best], costs
+
+    def coarse_seed_search(self, source_points, source_values, target_values, *,
+                           point_chunk: int = 1024):
+        inputs = self.prepare_cost_inputs(source_points, source_values, target_values)
+        return self.coarse_seed_search_prepared(inputs, point_chunk=point_chunk)
+
+    def refine_nelder_mead_prepared(self, inputs: RotationCostInputs, initial_angles, *,
+                                    point_chunk: int = 1024, max_iter: int = 500,
+                                    tol: float = 1.0e-4, simplex_step: float = 0.1):
+        """CAT-ordered Nelder-Mead (surface_rotation.py:594-699).
+
+        The simplex bookkeeping is host float64 exactly as the reference's;
+        each evaluation is one device cost launch.
+        """
+
+        if max_iter < 1 or tol < 0.0 or simplex_step <= 0.0:
+            raise ValueError("max_iter, tol, and simplex_step must be positive")
+        seed = np.asarray(torch.as_tensor(initial_angles, dtype=torch.float64).cpu(),
+                          dtype=np.float64).reshape(-1)
+        if seed.size != 3:
+            raise ValueError("initial_angles must contain exactly three angles")
+        simplex = np.stack([seed, seed + (simplex_step, 0.0, 0.0), seed + (0.0, simplex_step, 0.0),
+                            seed + (0.0, 0.0, simplex_step)])
+
+        def cost(points: np.ndarray) -> np.ndarray:
+            return self.cost_batch_prepared(inputs, torch.as_tensor(points)).cpu().numpy()
+
+        f = cost(simplex)
+        iteration = 0
+        for iteration in range(max_iter):
+            hi, lo, sh = 0, 0, 1
+            for i in range(4):
+                if f[i] > f[hi]:
+                    sh, hi = hi, i
+                elif f[i] > f[sh] and i != hi:
+                    sh = i
+                if f[i] < f[lo]:
+                    lo = i
+            centroid = np.zeros(3)
+            for i in range(4):
+                if i != hi:
+                    centroid += simplex[i]
+            centroid /= 3.0
+            refl = centroid + (centroid - simplex[hi])
+            fr = cost(refl[None])[0]
+            if fr < f[lo]:
+                exp = centroid + 2.0 * (refl - centroid)
+                fe = cost(exp[None])[0]
+                if fe < fr:
+                    simplex[hi], f[hi] = exp, fe
+                else:
+                    simplex[hi], f[hi] = refl, fr
+            elif fr < f[sh]:
+                simplex[hi], f[hi] = refl, fr
+            else:
+                con = centroid + 0.5 * (simplex[hi] - centroid)
+                fc = cost(con[None])[0]
+                if fc < f[hi]:
+                    simplex[hi], f[hi] = con, fc
+                else:
+                    for i in range(4):
+                        if i != lo:
+                            simplex[i] = simplex[lo] + 0.5 * (simplex[i] - simplex[lo])
+                            f[i] = cost(simplex[i][None])[0]
+            if np.max(np.abs(f - f[lo])) < tol:
+                break
+        dev = self.device
+        return (torch.as_tensor(simplex[lo].copy(), device=dev),
+                torch.as_tensor(f[lo], dtype=torch.float64, device=dev), iteration + 1)
+
+    def refine_nelder_mead(self, source_points, source_values, target_values, initial_angles, *,
+                           point_chunk: int = 1024, max_iter: int = 500, tol: float = 1.0e-4,
+                           simplex_step: float = 0.1):
+        inputs = self.prepare_cost_inputs(source_points, source_values, target_values)
+        return self.refine_nelder_mead_prepared(inputs, initial_angles, point_chunk=point_chunk,
+                                                max_iter=max_iter, tol=tol,
+                                                simplex_step=simplex_step)
+
+    def search_rotation_prepared(self, inputs: RotationCostInputs, *, point_chunk: int = 1024,
+                                 refine: bool = True, max_iter: int = 500, tol: float = 1.0e-4,
+                                 simplex_step: float = 0.1) -> RotationSearchResult:
+        seed_angle, seed_cost, seed_costs = self.coarse_seed_search_prepared(
+            inputs, point_chunk=point_chunk)
+        if refine:
+            angle, cost, iterations = self.refine_nelder_mead_prepared(
+                inputs, seed_angle, point_chunk=point_chunk, max_iter=max_iter, tol=tol,
+                simplex_step=simplex_step)
+        else:
+            angle, cost, iterations = seed_angle.clone(), seed_cost.clone(), 0
+        return RotationSearchResult(seed_angle=seed_angle.clone(), seed_cost=seed_cost.clone(),
+                                    seed_costs=seed_costs, angle=angle, cost=cost,
+                                    rotation_matrix=self.rotation_matrix(angle),
+                                    iterations=iterations)
+
+    def search_rotation(self, source_points, source_values, target_values, *,
+                        point_chunk: int = 1024, refine: bool = True, max_iter: int = 500,
+                        tol: float = 1.0e-4, simplex_step: float = 0.1) -> RotationSearchResult:
+        inputs = self.prepare_cost_inputs(source_points, source_values, target_values)
+        return self.search_rotation_prepared(inputs, point_chunk=point_chunk, refine=refine,
+                                             max_iter=max_iter, tol=tol,
+                                             simplex_step=simplex_step)

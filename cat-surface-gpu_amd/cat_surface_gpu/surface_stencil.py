@@ -241,6 +241,7 @@ class SurfaceStencil:
             unit_sphere_points=(None if self.unit_sphere_points is None
                                 else dev(self.unit_sphere_points, torch.float32)),
             topology=topo,
+            sphere_points64=dev(self.sphere_points, torch.float64),
         )
 
 
@@ -274,6 +275,7 @@ class SurfaceStencilDevice:
     source_points: int
     unit_sphere_points: torch.Tensor | None = None
     topology: Topology | None = None
+    sphere_points64: torch.Tensor | None = None  # float64 copy for index building
 
     @property
     def device(self) -> torch.device:

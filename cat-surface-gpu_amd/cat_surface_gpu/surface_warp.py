@@ -196,3 +196,98 @@ def solve_dartel_maps(source_maps, target_maps, *, loop: int = 6, lmreg: float =
                   flow=flow, metrics=metrics, workspace=ws)
     return DartelSolveResult(flow=flow, source_maps=source, target_maps=target,
                              metrics=metrics.to(dtype))
+
+
+def solve_dartel_from_surfaces(source_vertices, target_vertices, source_stencil, target_stencil, *,
+                               fwhm: Sequence[float] = (5.0, 5.0 / 3.0, 5.0 / 9.0),
+                               curvtypes: Sequence[int] = (5, 5, 2), loop: int = 6,
+                               n_steps: int = 2, lmreg: float = 1e-3, cycles: int = 3,
+                               nit: int = 3, its: int = 3, code: int = 1, kernel: str = "auto",
+                               squaring_kernel: str | None = None,
+                               device: str | torch.device = "auto",
+                               dtype: torch.dtype = torch.float64, parallel_sides: bool = True,
+                               optimized: bool = True, assume_initial_nonzero: bool = False,
+                               cuda_graph: bool = False) -> DartelSolveResult:
+    """Curvature sheets of both surfaces, then their DARTEL flow (surface_warp.py:370-488).
+
+    Sheets are produced directly in the solve dtype by the fused type-5 kernel
+    chain of each side.
+    """
+
+    from .surface_stencil import SurfaceStencil
+
+    if n_steps < 1 or n_steps > len(fwhm) or n_steps > len(curvtypes):
+        raise ValueError("n_steps must not exceed the lengths of fwhm and curvtypes")
+    if squaring_kernel not in {None, "auto", "hip", "torch", "triton"}:
+        raise ValueError("squaring_kernel must be None, 'auto', or 'hip'")
+    dev = resolve_device(device)
+    src_st = (source_stencil.to(dev) if isinstance(source_stencil, SurfaceStencil)
+              else source_stencil)
+    tgt_st = (target_stencil.to(dev) if isinstance(target_stencil, SurfaceStencil)
+              else target_stencil)
+    for curvtype in curvtypes[:n_steps]:
+        if int(curvtype) not in (2, 5):
+            raise NotImplementedError("only upstream curvtype 2 and 5 exist")
+        if int(curvtype) != 5:
+            raise NotImplementedError("this build implements curvtype 5 (steps <= 2)")
+    source = torch.as_tensor(source_vertices, device=dev).to(torch.float32).contiguous()
+    target = torch.as_tensor(target_vertices, device=dev).to(torch.float32).contiguous()
+    fw = tuple(float(f) for f in fwhm[:n_steps])
+    src_maps = torch.stack(src_st.curvature_type5_to_sheet_many(
+        src_st.resample_vertices(source), fw, out_dtype=dtype))
+    tgt_maps = torch.stack(tgt_st.curvature_type5_to_sheet_many(
+        tgt_st.resample_vertices(target), fw, out_dtype=dtype))
+    return solve_dartel_maps(src_maps, tgt_maps, loop=loop, lmreg=lmreg, cycles=cycles, nit=nit,
+                             its=its, code=code, kernel=kernel, squaring_kernel=squaring_kernel,
+                             device=dev, dtype=dtype, optimized=optimized,
+                             assume_initial_nonzero=assume_initial_nonzero, cuda_graph=cuda_graph)
+
+
+def apply_flow_to_sphere(sphere_vertices, flow, *, unit_sphere_vertices=None,
+                         inverse: bool = True, device: str | torch.device = "auto",
+                         dtype: torch.dtype = torch.float64) -> torch.Tensor:
+    """Warp sphere points through a flow (surface_warp.py:500-584)."""
+
+    dev = resolve_device(device)
+    sphere = torch.as_tensor(sphere_vertices, device=dev)
+    deformation = torch.as_tensor(flow, dtype=dtype, device=dev).contiguous()
+    if sphere.ndim != 2 or sphere.shape[1] != 3:
+        raise ValueError("sphere_vertices must have shape [points, 3]")
+    if deformation.ndim != 3 or deformation.shape[0] != 2:
+        raise ValueError("flow must have shape [2, ny, nx]")
+    if unit_sphere_vertices is None:
+        unit = _normalise_f32(sphere.to(torch.float32).contiguous())
+    else:
+        unit = torch.as_tensor(unit_sphere_vertices, dtype=torch.float32, device=dev).contiguous()
+        if unit.shape != sphere.shape:
+            raise ValueError("unit_sphere_vertices must match sphere_vertices shape")
+    n = int(unit.shape[0])
+    out = torch.empty((n, 3), dtype=dtype, device=dev)
+    ny, nx = int(deformation.shape[1]), int(deformation.shape[2])
+    _lib.check(_lib.lib().csg_apply_flow(_lib.ptr(unit), n, _lib.ptr(deformation), nx, ny,
+                                         _lib.dtype_code(dtype), int(bool(inverse)), _lib.ptr(out),
+                                         _lib.stream_handle(dev)), "csg_apply_flow")
+    return out
+
+
+def _normalise_f32(points: torch.Tensor) -> torch.Tensor:
+    """Per-vertex float32 unit vectors, used only when no stencil unit points are
+    cached (surface_warp.py:518-522); the pipeline always passes cached points."""
+
+    norm = torch.linalg.vector_norm(points, dim=-1, keepdim=True)
+    return (points / norm.clamp_min(torch.finfo(torch.float32).eps)).contiguous()
+
+
+def apply_flow_to_stenciled_sphere(sphere_vertices, flow, stencil, *, inverse: bool = True,
+                                   device: str | torch.device = "auto",
+                                   dtype: torch.dtype = torch.float64) -> torch.Tensor:
+    """Warp with the stencil's cached unit-sphere points (surface_warp.py:587-617)."""
+
+    from .surface_stencil import SurfaceStencil
+
+    dev = resolve_device(device)
+    st = stencil.to(dev) if isinstance(stencil, SurfaceStencil) else stencil
+    if st.unit_sphere_points is None:
+        raise ValueError("stencil does not contain unit-sphere points required by apply_warp")
+    return apply_flow_to_sphere(sphere_vertices, flow, unit_sphere_vertices=st.unit_sphere_points,
+                                inverse=inverse, device=dev, dtype=dtype)

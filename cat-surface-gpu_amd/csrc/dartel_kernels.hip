@@ -358,22 +358,26 @@ k_regularization(const T *__restrict__ v, int rtype, T sx, T sy, T mu, T lam, T 
     }
 }
 
-// Final sum of per-block partials, fixed order, one thread per batch element.
-__global__ void k_reduce_partials(const double *__restrict__ partial, int nblocks, int batch,
-                                  double scale, double *__restrict__ out, int out_stride,
-                                  int accumulate)
+// Final sum of per-block partials: one block per batch element, fixed
+// strided order per thread then a fixed-shape tree, so the result is
+// deterministic.
+__global__ void __launch_bounds__(kBlock)
+k_reduce_partials(const double *__restrict__ partial, int nblocks, int batch, double scale,
+                  double *__restrict__ out, int out_stride, int accumulate)
 {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= batch)
-        return;
+    __shared__ double red[kBlock / 64];
+    const int b = blockIdx.x;
     double s = 0.0;
-    for (int i = 0; i < nblocks; ++i)
+    for (int i = threadIdx.x; i < nblocks; i += blockDim.x)
         s += partial[(size_t)b * nblocks + i];
-    s *= scale;
-    if (accumulate)
-        out[(size_t)b * out_stride] += s;
-    else
-        out[(size_t)b * out_stride] = s;
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) {
+        s *= scale;
+        if (accumulate)
+            out[(size_t)b * out_stride] += s;
+        else
+            out[(size_t)b * out_stride] = s;
+    }
 }
 
 __global__ void k_sqrt_inplace(double *__restrict__ v, int batch, int stride)
@@ -546,8 +550,8 @@ int launch_regularization(const T *v, int rtype, const double *params, int nx, i
 int launch_reduce_partials(const double *partial, int nblocks, int batch, double scale,
                            double *out, int out_stride, int accumulate, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reduce_partials, dim3(grid_for(batch, 64)), dim3(64), 0, s, partial,
-                       nblocks, batch, scale, out, out_stride, accumulate);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(batch), dim3(kBlock), 0, s, partial, nblocks,
+                       batch, scale, out, out_stride, accumulate);
     CSG_LAUNCH_CHECK();
     return kOk;
 }

@@ -445,11 +445,39 @@ def golden_rotation() -> None:
     _save("rotation", **out)
 
 
+def golden_rotation_feature() -> None:
+    """The 'cuda' rotation-feature backend on a small stencil (rotation_feature.py:320-399)."""
+
+    from cat_surface_gpu import rotation_feature as rf
+
+    g = torch.Generator().manual_seed(1618)
+    verts, faces = _icosphere(3)
+    nx, ny = 32, 16
+    stencil = _small_stencil(verts, faces, nx, ny, g)
+    dev = stencil.to("cpu")
+    bump = 1.0 + 0.08 * torch.randn(verts.shape[0], dtype=F64, generator=g)
+    surface = (torch.as_tensor(verts * 30.0, dtype=F64) * bump[:, None]).to(torch.float32)
+    smoothed = dev.smooth_geometry(dev.resample_vertices(surface), 15.0)
+    normals = rf._stable_normals(smoothed, dev.faces)
+    areas = rf._mixed_voronoi_areas(smoothed, dev.faces)
+    depth, iters, resid = rf._solve_depth_potential_colored_sor(
+        smoothed, dev.faces, areas, normals, dev.color_groups(), alpha=1e-3, max_iter=1000)
+    result = rf.compute_rotation_feature(dev, surface, heat_fwhm=15.0)
+    groups = dev.color_groups()
+    colours = np.zeros(verts.shape[0], np.int32)
+    for c, grp in enumerate(groups):
+        colours[grp.numpy()] = c
+    _save("rotation_feature", verts=verts, faces=faces, surface=surface,
+          surf_idx=stencil.surface_indices, surf_w=stencil.surface_weights,
+          smoothed=smoothed, normals=normals, areas=areas, depth=depth,
+          residual=np.asarray(resid), colours=colours, feature=result.values)
+
+
 def main() -> None:
     steps = [
         golden_grid_ops, golden_objfun_squaring, golden_solver,
         golden_dartel_step, golden_solve, golden_warp, golden_surface,
-        golden_rotation,
+        golden_rotation, golden_rotation_feature,
     ]
     only = set(sys.argv[1:])
     for step in steps:

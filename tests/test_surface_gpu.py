@@ -1,0 +1,195 @@
+"""HIP surface-side operators vs the reference golden vectors and the oracle.
+
+Float32 geometry paths are compared within float32 rounding (the reference's
+sums are torch CPU reductions whose order is not specified); float64 value
+paths within 1e-12.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small(golden, cuda):
+    import cat_surface_gpu as csg
+
+    g = golden("surface")
+    st = csg.SurfaceStencil(
+        sphere_points=g["verts"], faces=g["faces"], surface_indices=g["surf_idx"],
+        surface_weights=g["surf_w"], sheet_indices=g["sheet_idx"], sheet_weights=g["sheet_w"],
+        nx=32, ny=16, source_points=g["verts"].shape[0],
+        unit_sphere_points=g["verts"].astype(np.float32))
+    return g, st.to(cuda)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def test_resample_and_sheet_gather(small, cuda):
+    g, dev = small
+    surf = torch.as_tensor(g["surface"], device=cuda)
+    np.testing.assert_array_equal(_np(dev.resample_vertices(surf)), g["resampled"])
+    np.testing.assert_allclose(_np(dev.resample_vertices(surf.double())), g["resampled64"],
+                               rtol=1e-15)
+    vals = torch.as_tensor(g["values"], device=cuda)
+    np.testing.assert_allclose(_np(dev.map_values_to_sheet(vals)), g["sheet_of_values"],
+                               rtol=1e-15, atol=0)
+
+
+def test_heat_smoothing_matches_golden(small, cuda):
+    g, dev = small
+    surf = torch.as_tensor(g["surface"], device=cuda)
+    assert abs(dev.edge_mean_distance(surf) - float(g["edge_mean"])) < 1e-12
+    for fwhm, key in ((50.0, "smooth_geom50"), (15.0, "smooth_geom15")):
+        np.testing.assert_allclose(_np(dev.smooth_geometry(surf, fwhm)), g[key], rtol=2e-7,
+                                   atol=1e-5)
+    vals = torch.as_tensor(g["values"], device=cuda)
+    np.testing.assert_allclose(_np(dev.smooth_values(vals, surf, 5.0)), g["smooth_vals5"],
+                               rtol=1e-12, atol=1e-14)
+    many = dev.smooth_values_many(vals, surf, (5.0, 5.0 / 3.0))
+    np.testing.assert_allclose(_np(many[0]), g["smooth_many0"], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(_np(many[1]), g["smooth_many1"], rtol=1e-12, atol=1e-14)
+
+
+def test_normals_and_type5_sheets(small, cuda):
+    g, dev = small
+    surf = torch.as_tensor(g["surface"], device=cuda)
+    np.testing.assert_allclose(_np(dev.vertex_normals(surf)), g["normals"], atol=2e-6)
+    maps = dev.curvature_type5_to_sheet_many(surf, (5.0 / 9.0, 5.0 / 27.0))
+    np.testing.assert_allclose(_np(maps[0]), g["type5_sheet0"], atol=2e-6)
+    np.testing.assert_allclose(_np(maps[1]), g["type5_sheet1"], atol=2e-6)
+    f32 = dev.curvature_type5_to_sheet_many(surf, (5.0 / 9.0,), out_dtype=torch.float32)
+    np.testing.assert_allclose(_np(f32[0]), g["type5_sheet0"], atol=3e-6)
+
+
+def test_rotation_feature_matches_golden(golden, cuda):
+    import cat_surface_gpu as csg
+    from cat_surface_gpu.rotation_feature import compute_rotation_feature
+
+    g = golden("rotation_feature")
+    n = g["verts"].shape[0]
+    st = csg.SurfaceStencil(
+        sphere_points=g["verts"], faces=g["faces"], surface_indices=g["surf_idx"],
+        surface_weights=g["surf_w"], sheet_indices=np.zeros((1, 1, 3), np.int32),
+        sheet_weights=np.zeros((1, 1, 3)), nx=1, ny=1, source_points=n).to(cuda)
+    res = compute_rotation_feature(st, torch.as_tensor(g["surface"], device=cuda),
+                                   heat_fwhm=15.0)
+    assert res.relative_residual < 1e-9
+    np.testing.assert_allclose(_np(res.values), g["feature"], atol=1e-6)
+
+
+def test_rotation_index_costs_and_search(golden, cuda):
+    import cat_surface_gpu as csg
+
+    g = golden("rotation")
+    idx = csg.RotationGridIndex.from_geometry(g["verts"], g["faces"], grid_size=32, margin=1,
+                                              device=cuda)
+    np.testing.assert_array_equal(idx.candidate_offsets, g["cand_offsets"])
+    np.testing.assert_array_equal(idx.candidate_faces, g["cand_faces"])
+    dev = idx.to(cuda)
+    inputs = dev.prepare_cost_inputs(g["src_pts"], g["src_vals"], g["tgt_vals"])
+    costs = _np(dev.cost_batch_prepared(inputs, torch.as_tensor(g["seeds"])))
+    np.testing.assert_allclose(costs, g["seed_costs"], rtol=1e-6)
+    faces, bary = dev.locate(inputs, torch.as_tensor(g["seeds"][:5]))
+    agree = (_np(faces)[:, :200] == g["loc_faces"]).mean()
+    assert agree > 0.995, agree
+    angle, cost, seeds = dev.coarse_seed_search_prepared(inputs)
+    np.testing.assert_allclose(_np(angle), g["seed_angle"])
+    a, c, it = dev.refine_nelder_mead_prepared(inputs, angle, max_iter=60)
+    np.testing.assert_allclose(_np(a), g["nm_angle"], atol=2e-3)
+    np.testing.assert_allclose(float(c), float(g["nm_cost"]), rtol=5e-3)
+
+
+def test_rotation_costs_match_oracle_on_random_angles(golden, cuda):
+    import cat_surface_gpu as csg
+
+    g = golden("rotation")
+    rng = np.random.default_rng(8)
+    angles = rng.uniform(-0.7, 0.7, size=(64, 3))
+    idx = csg.RotationGridIndex.from_geometry(g["verts"], g["faces"], grid_size=32, margin=1,
+                                              device=cuda).to(cuda)
+    costs = _np(idx.cost_batch(g["src_pts"], g["src_vals"], g["tgt_vals"],
+                               torch.as_tensor(angles)))
+    ref = O.rotation_costs(g["src_pts"], g["src_vals"], g["tgt_vals"], g["verts"], g["faces"],
+                           angles, 32, 1)
+    np.testing.assert_allclose(costs, ref, rtol=1e-6)
+
+
+def test_identity_rotation_cost_is_zero(cuda):
+    """Vertex values sampled at the vertices themselves cost nothing (reference test)."""
+
+    import cat_surface_gpu as csg
+    from cat_surface_gpu.stencil_builder import icosphere
+
+    v, f = icosphere(2)
+    vals = np.arange(v.shape[0], dtype=np.float64)
+    idx = csg.RotationGridIndex.from_geometry(v, f, grid_size=16, margin=2, device=cuda).to(cuda)
+    cost = _np(idx.cost_batch(v.astype(np.float32), vals, vals, torch.zeros((1, 3))))
+    assert cost[0] < 1e-6, cost
+
+
+def test_warp_matches_golden(golden, cuda):
+    import cat_surface_gpu as csg
+
+    g = golden("warp")
+    pts = torch.as_tensor(g["points"], device=cuda)
+    flow = torch.as_tensor(g["flow"], device=cuda)
+    np.testing.assert_allclose(_np(csg.apply_flow_to_sphere(pts, flow, inverse=True,
+                                                            device=cuda)),
+                               g["warp_inverse"], rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(_np(csg.apply_flow_to_sphere(pts, flow, inverse=False,
+                                                            device=cuda)),
+                               g["warp_forward"], rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(_np(csg.apply_flow_to_sphere(pts * 3.0, flow,
+                                                            unit_sphere_vertices=pts,
+                                                            device=cuda)),
+                               g["warp_cached"], rtol=1e-12, atol=1e-13)
+
+
+def test_average_xz_matches_golden(golden, cuda):
+    from cat_surface_gpu import _lib
+
+    g = golden("warp")
+    x = torch.as_tensor(g["avg_x"], dtype=torch.float64, device=cuda)
+    z = torch.as_tensor(g["avg_z"], dtype=torch.float64, device=cuda)
+    out = torch.empty(x.shape, dtype=torch.float32, device=cuda)
+    _lib.check(_lib.lib().csg_average_xz(_lib.ptr(x), _lib.ptr(z), int(x.shape[0]), 1,
+                                         _lib.doubles(np.eye(3).reshape(-1)), _lib.ptr(out),
+                                         _lib.stream_handle(cuda)), "avg")
+    np.testing.assert_array_equal(_np(out), g["avg_out"])
+
+
+def test_stencil_builder_matches_brute_force(cuda):
+    """Closest-triangle stencil entries equal an exhaustive CPU search."""
+
+    from cat_surface_gpu.stencil_builder import build_stencil, icosphere, uv_points
+
+    sv, sf = icosphere(3)
+    rng = np.random.default_rng(3)
+    sphere = (sv + rng.normal(scale=0.01, size=sv.shape)).astype(np.float32)
+    st = build_stencil(sphere, sf, nx=32, ny=16, n_triangles=1280, device=cuda)
+    # the same normalisation the builder applies
+    pts = sphere.astype(np.float64)
+    pts = pts - pts.mean(0)
+    pts /= np.linalg.norm(pts, axis=1, keepdims=True)
+    radius = 0.975 * np.linalg.norm(pts, axis=1).mean()
+    ico, _ = icosphere(3)
+    out_sphere = (pts.min(0) + pts.max(0))[None] + radius * ico
+    face, w = O.closest_triangles(pts, sf, out_sphere)
+    np.testing.assert_array_equal(st.surface_indices, sf[face])
+    np.testing.assert_allclose(st.surface_weights, w, atol=1e-12)
+    assert np.all(st.surface_weights > -1e-12)
+    np.testing.assert_allclose(st.surface_weights.sum(1), 1.0, atol=1e-12)
+    unit = st.sphere_points / np.linalg.norm(st.sphere_points, axis=1, keepdims=True)
+    face2, w2 = O.closest_triangles(unit, st.faces, uv_points(32, 16))
+    np.testing.assert_array_equal(st.sheet_indices.reshape(-1, 3), st.faces[face2])
+    np.testing.assert_allclose(st.sheet_weights.reshape(-1, 3), w2, atol=1e-12)
+    np.testing.assert_allclose(np.linalg.norm(st.unit_sphere_points, axis=1), 1.0, atol=1e-6)
