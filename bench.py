@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark: seconds per hemisphere of CAT_SurfWarp on one MI355X.
+
+Workload (fixed): a 61,442-vertex hemisphere pair built as the convex hull of
+a seeded jittered Fibonacci sphere (jitter 0.2 of spacing), degree-16
+spherical-harmonic features with N(0, 0.05) noise, target rotated (<= 30 deg)
+and smoothly warped, registered with steps=2 runs=2 avg, for both the LH and
+the x-mirrored RH copy.  One timed step = LH + RH.  Inputs are resident in
+memory (no file I/O) when the timed region starts.
+
+Prints one JSON line (rank 0).  Besides the wall-clock value it reports the
+roofline of the dominant DARTEL kernel, measured with HIP events on the
+stream it runs on, and a CPU baseline from the project's C restatement of
+the DARTEL step (oracle/), timed on a bounded sample on the host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "cat-surface-gpu_amd"))
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_inputs(seed: int):
+    from cat_surface_gpu import synthetic
+    from cat_surface_gpu.gpu_pipeline import GiftiMesh
+
+    pairs = []
+    for mirror in (False, True):
+        p = synthetic.make_pair("fibonacci", 61442, lmax=16, noise=0.05, max_rotation_deg=30.0,
+                                warp_deg=3.0, seed=seed, mirror=mirror)
+        pairs.append((GiftiMesh(p.src_surface, p.src_faces), GiftiMesh(p.src_sphere, p.src_faces),
+                      GiftiMesh(p.tgt_surface, p.tgt_faces), GiftiMesh(p.tgt_sphere, p.tgt_faces)))
+    return pairs
+
+
+def run_step(pairs, dtype: str, mode: str, device):
+    from cat_surface_gpu import run_cat_surface_gpu_pipeline
+
+    results = []
+    for meshes in pairs:
+        res = run_cat_surface_gpu_pipeline(None, None, None, None, "unused.gii", device=device,
+                                           dartel_dtype=dtype, steps=2, runs=2, avg=True,
+                                           meshes=meshes, write_output=False)
+        results.append(res)
+    return results
+
+
+def relax_roofline(dtype_name: str, device) -> dict:
+    """Average duration of the finest-level red-black half-sweep (HIP events on
+    the launch stream) against its algorithmic bytes: per half-sweep the active
+    half of the pixels reads A (3 ch), b (2 ch) and its own u (2 ch), reads the
+    other colour's u once (2 ch) and writes u (2 ch): 5.5 values per pixel."""
+
+    import torch
+
+    from cat_surface_gpu import _lib
+
+    dt = torch.float32 if dtype_name == "float32" else torch.float64
+    nx, ny, nit, reps = 512, 256, 3, 200
+    g = torch.Generator(device="cpu").manual_seed(0)
+    hes = (torch.rand((3, ny, nx), generator=g, dtype=torch.float64) + 1.0).to(dt).to(device)
+    rhs = torch.randn((2, ny, nx), generator=g, dtype=torch.float64).to(dt).to(device)
+    sol = torch.zeros_like(rhs)
+    lib = _lib.lib()
+    stream = _lib.stream_handle(device)
+    for _ in range(5):
+        _lib.check(lib.csg_relax_membrane(_lib.ptr(hes), _lib.ptr(rhs), _lib.ptr(sol), 0.5,
+                                          -0.125, -0.125, nit, nx, ny, _lib.dtype_code(dt),
+                                          stream), "relax")
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        lib.csg_relax_membrane(_lib.ptr(hes), _lib.ptr(rhs), _lib.ptr(sol), 0.5, -0.125, -0.125,
+                               nit, nx, ny, _lib.dtype_code(dt), stream)
+    stop.record()
+    torch.cuda.synchronize()
+    launches = reps * 2 * nit
+    seconds = start.elapsed_time(stop) / 1e3 / launches
+    nbytes = 5.5 * nx * ny * (4 if dt == torch.float32 else 8)
+    achieved = nbytes / seconds / 1e9
+    return {"kernel": "k_relax_color (512x256 half-sweep)", "bound": "hbm",
+            "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "avg_launch_us": round(seconds * 1e6, 3), "algorithmic_bytes": nbytes}
+
+
+def cpu_baseline() -> dict:
+    """Project C restatement (oracle) of dartel_step at 512x256 float64, one step
+    per k = 0..5, single thread; scaled to the 108 dartel steps of one hemisphere
+    (3 solves x 2 steps x 6 loops x 3 its).  DARTEL only: a lower bound of a full
+    CPU hemisphere."""
+
+    import numpy as np
+
+    from oracle import oracle as O
+
+    nx, ny = 512, 256
+    x = (np.arange(nx) + 0.5)[None, :] / nx
+    y = (np.arange(ny) + 0.5)[:, None] / ny
+    src = 0.5 + 0.3 * np.sin(2 * np.pi * 3 * x) * np.cos(2 * np.pi * y)
+    tgt = np.roll(src, 3, axis=1)
+    v = np.zeros((2, ny, nx))
+    per_k = []
+    params = O.default_dartel_parameters(6)
+    for k in range(6):
+        t = time.perf_counter()
+        v, _ = O.dartel_step(src, tgt, v, k, params[k])
+        per_k.append(time.perf_counter() - t)
+    seconds = 3 * 2 * 3 * sum(per_k)
+    return {"value": round(seconds, 3), "unit": "s/hemisphere (DARTEL stage only)", "cores": 1,
+            "kind": "port",
+            "sample": "oracle dartel_step 512x256 fp64, one step for each k=0..5 "
+                      f"({sum(per_k):.2f} s), x18 for 3 solves x 2 steps x 3 its"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--dtype", choices=("float32", "float64"), default="float32")
+    ap.add_argument("--mode", choices=("plain",), default="plain")
+    ap.add_argument("--seed", type=int, default=20261205)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: every registration here fits one GPU and this "
+              "build runs no multi-GPU path (--gpus must be 1)", file=sys.stderr)
+        return 2
+    import torch
+
+    device = torch.device("cuda", int(os.environ.get("LOCAL_RANK", "0")))
+    torch.cuda.set_device(device)
+    t = time.perf_counter()
+    pairs = make_inputs(args.seed)
+    log(f"inputs ready in {time.perf_counter() - t:.1f} s")
+    for i in range(args.warmup):
+        run_step(pairs, args.dtype, args.mode, device)
+        torch.cuda.synchronize()
+        log(f"warmup {i} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stage_sums: dict[str, float] = {}
+    for i in range(args.steps):
+        results = run_step(pairs, args.dtype, args.mode, device)
+        for r in results:
+            for key, val in r.timings.items():
+                stage_sums[key] = stage_sums.get(key, 0.0) + val
+        log(f"step {i} done")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    hemispheres = 2
+    value = elapsed / (args.steps * hemispheres)
+    roof = relax_roofline(args.dtype, device)
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    stages = {k: round(v / (args.steps * hemispheres), 4) for k, v in sorted(stage_sums.items())}
+    line = {
+        "metric": "s/hemisphere CAT_SurfWarp 61,442 verts (steps=2 runs=2 avg)",
+        "value": round(value, 4),
+        "unit": "s/hemisphere",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.dtype == "float32" else "f64",
+        "data": "synthetic (seeded Fibonacci-hull hemispheres, SH features)",
+        "config": {"workload": "61,442-vertex hemisphere pair, LH+RH per step, steps=2 runs=2 "
+                               "avg, 512x256 map", "mode": args.mode,
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "stages_s_per_hemisphere": stages,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

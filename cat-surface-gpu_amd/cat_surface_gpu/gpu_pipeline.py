@@ -152,14 +152,16 @@ def run_cat_surface_gpu_pipeline(
     rotation_simplex_step: float = 0.1, rotation_feature_backend: str = "auto",
     rotation_feature_solver: str = "colored-sor", parallel_sides: bool = True,
     optimized_dartel: bool = True, cuda_graph: bool = True,
-    rotation_search=None, write_output: bool = True,
+    rotation_search=None, write_output: bool = True, meshes=None,
 ) -> CatSurfaceGpuResult:
     """Initial rotation, DARTEL runs, averaging, warp and GIFTI output (:468-1326).
 
-    ``rotation_search`` (MI355X addition): a callable
-    (pipeline, source_points, source_values, target_values) -> RotationSearchResult
-    replacing the 46-seed + Nelder-Mead search, e.g. the batched Hopf-grid +
-    adjoint search of batched.py.
+    MI355X additions (keyword-only, defaults keep the reference behaviour):
+    ``rotation_search`` is a callable (pipeline, source_points, source_values,
+    target_values) -> RotationSearchResult replacing the 46-seed + Nelder-Mead
+    search; ``meshes`` passes the four meshes (source surface, source sphere,
+    target surface, target sphere) already in memory instead of reading the
+    paths; ``write_output=False`` skips the output file.
     """
 
     if steps < 1 or steps > 3:
@@ -191,10 +193,13 @@ def run_cat_surface_gpu_pipeline(
 
     t_total = time.perf_counter()
     timings: dict[str, float] = {}
-    src_surface = read_gifti_mesh(source_surface_path)
-    src_sphere = read_gifti_mesh(source_sphere_path)
-    tgt_surface = read_gifti_mesh(target_surface_path)
-    tgt_sphere = read_gifti_mesh(target_sphere_path)
+    if meshes is not None:
+        src_surface, src_sphere, tgt_surface, tgt_sphere = meshes
+    else:
+        src_surface = read_gifti_mesh(source_surface_path)
+        src_sphere = read_gifti_mesh(source_sphere_path)
+        tgt_surface = read_gifti_mesh(target_surface_path)
+        tgt_sphere = read_gifti_mesh(target_sphere_path)
     timings["input_read_seconds"] = time.perf_counter() - t_total
 
     with tempfile.TemporaryDirectory(prefix="cat_surface_gpu_") as tmp_name:
@@ -373,7 +378,8 @@ def run_cat_surface_gpu_pipeline(
                           src_sphere.faces)
         output = Path(output_path)
         if write_output:
-            write_gifti_mesh(output, final, reference_path=source_sphere_path)
+            write_gifti_mesh(output, final,
+                             reference_path=None if meshes is not None else source_sphere_path)
         timings["output_write_seconds"] = time.perf_counter() - t
     timings["total_seconds"] = time.perf_counter() - t_total
     return CatSurfaceGpuResult(vertices=final.vertices, faces=final.faces,

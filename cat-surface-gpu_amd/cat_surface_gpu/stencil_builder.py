@@ -98,7 +98,7 @@ def build_stencil(sphere_vertices, sphere_faces, *, surface_vertices=None, norma
                              normalise=normalise, nx=nx, ny=ny, n_triangles=n_triangles,
                              device=device)
     return SurfaceStencil(
-        sphere_points=d.sphere_points.cpu().numpy(),
+        sphere_points=d.sphere_points64.cpu().numpy(),
         faces=d.topology.faces_np.copy(),
         surface_indices=d.surface_indices.cpu().numpy(),
         surface_weights=d.surface_weights.cpu().numpy(),

@@ -274,8 +274,11 @@ def _normalise_f32(points: torch.Tensor) -> torch.Tensor:
     """Per-vertex float32 unit vectors, used only when no stencil unit points are
     cached (surface_warp.py:518-522); the pipeline always passes cached points."""
 
-    norm = torch.linalg.vector_norm(points, dim=-1, keepdim=True)
-    return (points / norm.clamp_min(torch.finfo(torch.float32).eps)).contiguous()
+    out = torch.empty_like(points)
+    _lib.check(_lib.lib().csg_normalise_points(_lib.ptr(points), int(points.shape[0]),
+                                               _lib.ptr(out), _lib.stream_handle(points.device)),
+               "csg_normalise_points")
+    return out
 
 
 def apply_flow_to_stenciled_sphere(sphere_vertices, flow, stencil, *, inverse: bool = True,

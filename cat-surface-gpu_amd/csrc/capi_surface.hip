@@ -312,6 +312,13 @@ int csg_apply_flow(const float *unit, int n, const void *flow, int nx, int ny, i
     return fail(kInvalidArgument, "dtype must be 0 or 1");
 }
 
+int csg_normalise_points(const float *in, int n, float *out, void *stream)
+{
+    if (n < 0)
+        return fail(kInvalidArgument, "point count must be non-negative");
+    return launch_normalise_f32(in, n, out, S(stream));
+}
+
 int csg_average_xz(const void *avg, const void *main_, int n, int dtype, const double *unrotate,
                    float *out, void *stream)
 {

@@ -121,6 +121,58 @@ __device__ __forceinline__ T wrapped_coordinate(const T *__restrict__ b, const C
     return (k22 * c.dx2 + k12 * c.dx1) * c.dy2 + (k21 * c.dx2 + k11 * c.dx1) * c.dy1 + (T)1;
 }
 
+// ---- displacement form (float32 internal maps)
+//
+// In float32 the internal maps hold d = φ - id instead of the one-based
+// coordinate φ: a coordinate near 512 carries a float32 rounding of ~3e-5
+// pixels, the displacement itself ~1e-7.  The arithmetic below is the
+// reference's coordinate arithmetic rewritten exactly (in real numbers) with
+// the integer parts kept as integers.
+
+// Corners at zero-based position (px + dx, py + dy); also returns the grid
+// columns x0/x1 and rows y0/y1 of the corners (after wrap / reflection).
+template <typename T>
+__device__ __forceinline__ Corners<T> corners_disp(int px, int py, T dx, T dy, int nx, int ny,
+                                                   int &x0, int &x1, int &y0, int &y1)
+{
+    Corners<T> c;
+    T fx = floor(dx), fy = floor(dy);
+    int ix = px + (int)fx, iy = py + (int)fy;
+    c.dx1 = dx - fx;
+    c.dy1 = dy - fy;
+    c.dx2 = (T)1 - c.dx1;
+    c.dy2 = (T)1 - c.dy1;
+    x0 = wrap_x(ix, nx);
+    x1 = wrap_x(ix + 1, nx);
+    y0 = reflect_y(iy, ny);
+    y1 = reflect_y(iy + 1, ny);
+    c.i22 = y0 * nx + x0;
+    c.i12 = y0 * nx + x1;
+    c.i21 = y1 * nx + x0;
+    c.i11 = y1 * nx + x1;
+    return c;
+}
+
+// Displacement of the composed map for one component: corner values are
+// X_c + d_c (X_c the corner's grid index), unwrapped against k22 by `period`
+// exactly as wrapped_coordinate does, interpolated, minus the pixel index p.
+template <typename T>
+__device__ __forceinline__ T wrapped_displacement(const T *__restrict__ d, const Corners<T> &c,
+                                                  int X22, int X12, int X21, int X11, int period,
+                                                  int p)
+{
+    T d22 = d[c.i22], d12 = d[c.i12], d21 = d[c.i21], d11 = d[c.i11];
+    T P = (T)period;
+    int n12 = (int)floor(((T)(X12 - X22) + (d12 - d22)) / P + (T)0.5);
+    int n21 = (int)floor(((T)(X21 - X22) + (d21 - d22)) / P + (T)0.5);
+    int n11 = (int)floor(((T)(X11 - X22) + (d11 - d22)) / P + (T)0.5);
+    T i22 = (T)(X22 - p), i12 = (T)(X12 - n12 * period - p);
+    T i21 = (T)(X21 - n21 * period - p), i11 = (T)(X11 - n11 * period - p);
+    T base = (i22 * c.dx2 + i12 * c.dx1) * c.dy2 + (i21 * c.dx2 + i11 * c.dx1) * c.dy1;
+    T disp = (d22 * c.dx2 + d12 * c.dx1) * c.dy2 + (d21 * c.dx2 + d11 * c.dx1) * c.dy1;
+    return base + disp;
+}
+
 // Deterministic block reduction of doubles (blockDim.x == kBlock, wave64).
 __device__ __forceinline__ double block_sum(double v, double *smem)
 {

@@ -14,16 +14,21 @@ struct ObjectiveArgs {
     T *rhs, *hes, *smap, *sjac;
     size_t rstride, hstride;
     double *partial;  // [batch][nblocks], may be null
+    bool disp = false;  // maps in displacement form (float32 internal path)
 };
 
 inline int objective_blocks(int nx, int ny) { return (int)grid_for((size_t)nx * ny); }
 
 template <typename T>
 int launch_expdef_init(const T *v, double scale, int nx, int ny, int batch, size_t vstride,
-                       T *map, T *jac, size_t mstride, size_t jstride, hipStream_t s);
+                       T *map, T *jac, size_t mstride, size_t jstride, hipStream_t s,
+                       bool disp = false);
 template <typename T>
 int launch_compose_self(const T *map, const T *jac, int nx, int ny, int batch, size_t mstride,
-                        size_t jstride, T *omap, T *ojac, hipStream_t s);
+                        size_t jstride, T *omap, T *ojac, hipStream_t s, bool disp = false);
+template <typename T>
+int launch_disp_to_coord(const T *d, int nx, int ny, int batch, size_t stride, T *out,
+                         hipStream_t s);
 template <typename T>
 int launch_compose(const T *a, const T *ja, const T *b, const T *jb, int nx, int ny, T *omap,
                    T *ojac, hipStream_t s);
@@ -32,7 +37,8 @@ int launch_objective(const ObjectiveArgs<T> &a, hipStream_t s);
 template <typename T>
 int launch_squaring(const T *bin, const T *ain, const T *map, const T *jac, int nx, int ny,
                     int batch, size_t bstride, size_t astride, size_t mstride, size_t jstride,
-                    T *bout, T *aout, T *inc, T *omap, T *ojac, hipStream_t s);
+                    T *bout, T *aout, T *inc, T *omap, T *ojac, hipStream_t s,
+                    bool disp = false);
 template <typename T>
 int launch_combine(const T *v, const T *bf, const T *af, const T *br, const T *ar, double scale,
                    double w00, double w01, double w10, int nx, int ny, int batch, size_t vstride,

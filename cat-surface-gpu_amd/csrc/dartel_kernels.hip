@@ -13,9 +13,9 @@
 
 namespace csg {
 
-// φ0 = id + s·v, J0 = I + s·∇v (central differences).
-// dartel_grid.py:1619-1646, :624-652
-template <typename T>
+// φ0 = id + s·v, J0 = I + s·∇v (central differences).  With D the map holds
+// the displacement s·v.  dartel_grid.py:1619-1646, :624-652
+template <typename T, bool D>
 __global__ void __launch_bounds__(kBlock)
 k_expdef_init(const T *__restrict__ v, T scale, int nx, int ny, size_t vstride,
               T *__restrict__ map, T *__restrict__ jac, size_t mstride, size_t jstride)
@@ -33,8 +33,13 @@ k_expdef_init(const T *__restrict__ v, T scale, int nx, int ny, size_t vstride,
     int ym = reflect_y(y - 1, ny) * nx + x, yp = reflect_y(y + 1, ny) * nx + x;
     T s = scale * (T)0.5;
     if (map) {
-        map[p] = (T)(x + 1) + v[p] * scale;
-        map[P + p] = (T)(y + 1) + v[P + p] * scale;
+        if (D) {
+            map[p] = v[p] * scale;
+            map[P + p] = v[P + p] * scale;
+        } else {
+            map[p] = (T)(x + 1) + v[p] * scale;
+            map[P + p] = (T)(y + 1) + v[P + p] * scale;
+        }
     }
     jac[p] = (v[xm] * (T)-1 + v[xp]) * s + (T)1;
     jac[P + p] = (v[P + xm] * (T)-1 + v[P + xp]) * s;
@@ -42,8 +47,45 @@ k_expdef_init(const T *__restrict__ v, T scale, int nx, int ny, size_t vstride,
     jac[3 * P + p] = (v[P + ym] * (T)-1 + v[P + yp]) * s + (T)1;
 }
 
+// Corners of the map at pixel p, in either representation; for D also the
+// corner grid indices needed by wrapped_displacement.
+template <typename T, bool D>
+struct MapCorners {
+    Corners<T> c;
+    int x0, x1, y0, y1;
+};
+
+template <typename T, bool D>
+__device__ __forceinline__ MapCorners<T, D> map_corners(const T *__restrict__ map, int p, int P,
+                                                        int nx, int ny)
+{
+    MapCorners<T, D> m;
+    if (D) {
+        m.c = corners_disp<T>(p % nx, p / nx, map[p], map[P + p], nx, ny, m.x0, m.x1, m.y0,
+                              m.y1);
+    } else {
+        m.c = corners_at<T>(map[p] - (T)1, map[P + p] - (T)1, nx, ny);
+    }
+    return m;
+}
+
+// The composed map's two components at pixel p (coordinates, or displacements for D).
+template <typename T, bool D>
+__device__ __forceinline__ void composed_map(const T *__restrict__ b, const MapCorners<T, D> &m,
+                                             int p, int nx, int ny, T &cx, T &cy)
+{
+    const int P = nx * ny;
+    if (D) {
+        cx = wrapped_displacement<T>(b, m.c, m.x0, m.x1, m.x0, m.x1, nx, p % nx);
+        cy = wrapped_displacement<T>(b + P, m.c, m.y0, m.y0, m.y1, m.y1, ny, p / nx);
+    } else {
+        cx = wrapped_coordinate<T>(b, m.c, (T)nx);
+        cy = wrapped_coordinate<T>(b + P, m.c, (T)ny);
+    }
+}
+
 // φ' = φ∘φ and J' = (J∘φ)·J.  dartel_grid.py:458-507
-template <typename T>
+template <typename T, bool D>
 __global__ void __launch_bounds__(kBlock)
 k_compose_self(const T *__restrict__ map, const T *__restrict__ jac, int nx, int ny,
                size_t mstride, size_t jstride, T *__restrict__ omap, T *__restrict__ ojac)
@@ -58,9 +100,10 @@ k_compose_self(const T *__restrict__ map, const T *__restrict__ jac, int nx, int
         jac += blockIdx.y * jstride;
         ojac += blockIdx.y * jstride;
     }
-    Corners<T> c = corners_at<T>(map[p] - (T)1, map[P + p] - (T)1, nx, ny);
-    T cx = wrapped_coordinate<T>(map, c, (T)nx);
-    T cy = wrapped_coordinate<T>(map + P, c, (T)ny);
+    MapCorners<T, D> m = map_corners<T, D>(map, p, P, nx, ny);
+    const Corners<T> &c = m.c;
+    T cx, cy;
+    composed_map<T, D>(map, m, p, nx, ny, cx, cy);
     if (jac) {
         T jb00 = bilinear<T>(jac, c), jb01 = bilinear<T>(jac + P, c);
         T jb10 = bilinear<T>(jac + 2 * P, c), jb11 = bilinear<T>(jac + 3 * P, c);
@@ -104,7 +147,7 @@ k_compose(const T *__restrict__ a, const T *__restrict__ ja, const T *__restrict
 // φ (initialise_objfun), and the small-deformation start of the squaring
 // chain, smap = id + s_small·v with its Jacobian (smalldef_jac).
 // dartel_grid.py:679-775, :655-676.  Per-block FP64 partials of Σ d²w.
-template <typename T>
+template <typename T, bool D>
 __global__ void __launch_bounds__(kBlock)
 k_objective(const T *__restrict__ src, const T *__restrict__ tgt, const T *__restrict__ v,
             const T *__restrict__ map, const T *__restrict__ jac, const T *__restrict__ dist,
@@ -155,7 +198,8 @@ k_objective(const T *__restrict__ src, const T *__restrict__ tgt, const T *__res
             j3 = o10 * i01 + o11 * i11;
         }
         // initialise_objfun
-        Corners<T> c = corners_at<T>(map[p] - (T)1, map[P + p] - (T)1, nx, ny);
+        MapCorners<T, D> m = map_corners<T, D>(map, p, P, nx, ny);
+        const Corners<T> &c = m.c;
         T k22 = src[c.i22], k12 = src[c.i12], k21 = src[c.i21], k11 = src[c.i11];
         T sampled = (k22 * c.dx2 + k12 * c.dx1) * c.dy2 + (k21 * c.dx2 + k11 * c.dx1) * c.dy1;
         T diff = sampled - tgt[p];
@@ -173,8 +217,13 @@ k_objective(const T *__restrict__ src, const T *__restrict__ tgt, const T *__res
         if (v) {
             // smalldef_jac(v, s_small)
             T ss = s_small * (T)0.5;
-            smap[p] = (T)(x + 1) + v[p] * s_small;
-            smap[P + p] = (T)(y + 1) + v[P + p] * s_small;
+            if (D) {
+                smap[p] = v[p] * s_small;
+                smap[P + p] = v[P + p] * s_small;
+            } else {
+                smap[p] = (T)(x + 1) + v[p] * s_small;
+                smap[P + p] = (T)(y + 1) + v[P + p] * s_small;
+            }
             sjac[p] = dvx_dx * ss + (T)1;
             sjac[P + p] = dvy_dx * ss;
             sjac[2 * P + p] = dvx_dy * ss;
@@ -189,7 +238,7 @@ k_objective(const T *__restrict__ src, const T *__restrict__ tgt, const T *__res
 // One squaring step: b += det·Jᵀ(b∘φ), A += det·Jᵀ(A∘φ)J with the increment
 // written out; optionally φ'=φ∘φ, J'=(J∘φ)J from the same corners.
 // dartel_grid.py:846-913, dartel_triton.py:344-477
-template <typename T>
+template <typename T, bool D>
 __global__ void __launch_bounds__(kBlock)
 k_squaring(const T *__restrict__ bin, const T *__restrict__ ain, const T *__restrict__ map,
            const T *__restrict__ jac, int nx, int ny, size_t bstride, size_t astride,
@@ -207,7 +256,8 @@ k_squaring(const T *__restrict__ bin, const T *__restrict__ ain, const T *__rest
     aout += bi * astride;
     map += bi * mstride;
     jac += bi * jstride;
-    Corners<T> c = corners_at<T>(map[p] - (T)1, map[P + p] - (T)1, nx, ny);
+    MapCorners<T, D> m = map_corners<T, D>(map, p, P, nx, ny);
+    const Corners<T> &c = m.c;
     T j00 = jac[p], j10 = jac[P + p], j01 = jac[2 * P + p], j11 = jac[3 * P + p];
     T det = j00 * j11 - j01 * j10;
     T sbx = bilinear<T>(bin, c), sby = bilinear<T>(bin + P, c);
@@ -231,8 +281,8 @@ k_squaring(const T *__restrict__ bin, const T *__restrict__ ain, const T *__rest
     if (omap) {
         omap += bi * mstride;
         ojac += bi * jstride;
-        T cx = wrapped_coordinate<T>(map, c, (T)nx);
-        T cy = wrapped_coordinate<T>(map + P, c, (T)ny);
+        T cx, cy;
+        composed_map<T, D>(map, m, p, nx, ny, cx, cy);
         T jb00 = bilinear<T>(jac, c), jb01 = bilinear<T>(jac + P, c);
         T jb10 = bilinear<T>(jac + 2 * P, c), jb11 = bilinear<T>(jac + 3 * P, c);
         ojac[p] = jb00 * j00 + jb10 * j10;
@@ -462,26 +512,59 @@ k_sample_field(const T *__restrict__ field, int nx, int ny, const T *__restrict_
     out[i] = bilinear<T>(field, c);
 }
 
+// φ = id + d: one-based coordinates of a displacement-form map.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_disp_to_coord(const T *__restrict__ d, int nx, int ny, size_t stride, T *__restrict__ out)
+{
+    const int P = nx * ny;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P)
+        return;
+    d += blockIdx.y * stride;
+    out += blockIdx.y * stride;
+    out[p] = (T)(p % nx + 1) + d[p];
+    out[P + p] = (T)(p / nx + 1) + d[P + p];
+}
+
 // ------------------------------------------------------------- launchers
 
 template <typename T>
-int launch_expdef_init(const T *v, double scale, int nx, int ny, int batch, size_t vstride,
-                       T *map, T *jac, size_t mstride, size_t jstride, hipStream_t s)
+int launch_disp_to_coord(const T *d, int nx, int ny, int batch, size_t stride, T *out,
+                         hipStream_t s)
 {
     dim3 g(grid_for((size_t)nx * ny), batch);
-    hipLaunchKernelGGL(k_expdef_init<T>, g, dim3(kBlock), 0, s, v, (T)scale, nx, ny, vstride,
-                       map, jac, mstride, jstride);
+    hipLaunchKernelGGL(k_disp_to_coord<T>, g, dim3(kBlock), 0, s, d, nx, ny, stride, out);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+template <typename T>
+int launch_expdef_init(const T *v, double scale, int nx, int ny, int batch, size_t vstride,
+                       T *map, T *jac, size_t mstride, size_t jstride, hipStream_t s, bool disp)
+{
+    dim3 g(grid_for((size_t)nx * ny), batch);
+    if (disp)
+        hipLaunchKernelGGL((k_expdef_init<T, true>), g, dim3(kBlock), 0, s, v, (T)scale, nx, ny,
+                           vstride, map, jac, mstride, jstride);
+    else
+        hipLaunchKernelGGL((k_expdef_init<T, false>), g, dim3(kBlock), 0, s, v, (T)scale, nx, ny,
+                           vstride, map, jac, mstride, jstride);
     CSG_LAUNCH_CHECK();
     return kOk;
 }
 
 template <typename T>
 int launch_compose_self(const T *map, const T *jac, int nx, int ny, int batch, size_t mstride,
-                        size_t jstride, T *omap, T *ojac, hipStream_t s)
+                        size_t jstride, T *omap, T *ojac, hipStream_t s, bool disp)
 {
     dim3 g(grid_for((size_t)nx * ny), batch);
-    hipLaunchKernelGGL(k_compose_self<T>, g, dim3(kBlock), 0, s, map, jac, nx, ny, mstride,
-                       jstride, omap, ojac);
+    if (disp)
+        hipLaunchKernelGGL((k_compose_self<T, true>), g, dim3(kBlock), 0, s, map, jac, nx, ny,
+                           mstride, jstride, omap, ojac);
+    else
+        hipLaunchKernelGGL((k_compose_self<T, false>), g, dim3(kBlock), 0, s, map, jac, nx, ny,
+                           mstride, jstride, omap, ojac);
     CSG_LAUNCH_CHECK();
     return kOk;
 }
@@ -500,10 +583,16 @@ template <typename T>
 int launch_objective(const ObjectiveArgs<T> &a, hipStream_t s)
 {
     dim3 g(grid_for((size_t)a.nx * a.ny), a.batch);
-    hipLaunchKernelGGL(k_objective<T>, g, dim3(kBlock), 0, s, a.src, a.tgt, a.v, a.map, a.jac,
-                       a.dist, (T)a.s_div, (T)a.s_small, a.nx, a.ny, a.istride, a.vstride,
-                       a.mstride, a.jstride, a.dstride, a.rhs, a.hes, a.smap, a.sjac, a.rstride,
-                       a.hstride, a.partial, (int)g.x);
+    if (a.disp)
+        hipLaunchKernelGGL((k_objective<T, true>), g, dim3(kBlock), 0, s, a.src, a.tgt, a.v,
+                           a.map, a.jac, a.dist, (T)a.s_div, (T)a.s_small, a.nx, a.ny, a.istride,
+                           a.vstride, a.mstride, a.jstride, a.dstride, a.rhs, a.hes, a.smap,
+                           a.sjac, a.rstride, a.hstride, a.partial, (int)g.x);
+    else
+        hipLaunchKernelGGL((k_objective<T, false>), g, dim3(kBlock), 0, s, a.src, a.tgt, a.v,
+                           a.map, a.jac, a.dist, (T)a.s_div, (T)a.s_small, a.nx, a.ny, a.istride,
+                           a.vstride, a.mstride, a.jstride, a.dstride, a.rhs, a.hes, a.smap,
+                           a.sjac, a.rstride, a.hstride, a.partial, (int)g.x);
     CSG_LAUNCH_CHECK();
     return kOk;
 }
@@ -511,11 +600,15 @@ int launch_objective(const ObjectiveArgs<T> &a, hipStream_t s)
 template <typename T>
 int launch_squaring(const T *bin, const T *ain, const T *map, const T *jac, int nx, int ny,
                     int batch, size_t bstride, size_t astride, size_t mstride, size_t jstride,
-                    T *bout, T *aout, T *inc, T *omap, T *ojac, hipStream_t s)
+                    T *bout, T *aout, T *inc, T *omap, T *ojac, hipStream_t s, bool disp)
 {
     dim3 g(grid_for((size_t)nx * ny), batch);
-    hipLaunchKernelGGL(k_squaring<T>, g, dim3(kBlock), 0, s, bin, ain, map, jac, nx, ny, bstride,
-                       astride, mstride, jstride, bout, aout, inc, omap, ojac);
+    if (disp)
+        hipLaunchKernelGGL((k_squaring<T, true>), g, dim3(kBlock), 0, s, bin, ain, map, jac, nx,
+                           ny, bstride, astride, mstride, jstride, bout, aout, inc, omap, ojac);
+    else
+        hipLaunchKernelGGL((k_squaring<T, false>), g, dim3(kBlock), 0, s, bin, ain, map, jac, nx,
+                           ny, bstride, astride, mstride, jstride, bout, aout, inc, omap, ojac);
     CSG_LAUNCH_CHECK();
     return kOk;
 }
@@ -589,15 +682,16 @@ int launch_sample_field(const T *field, int nx, int ny, const T *x, const T *y, 
 
 #define CSG_INSTANTIATE(T)                                                                     \
     template int launch_expdef_init<T>(const T *, double, int, int, int, size_t, T *, T *,     \
-                                       size_t, size_t, hipStream_t);                           \
+                                       size_t, size_t, hipStream_t, bool);                     \
     template int launch_compose_self<T>(const T *, const T *, int, int, int, size_t, size_t,   \
-                                        T *, T *, hipStream_t);                                \
+                                        T *, T *, hipStream_t, bool);                          \
+    template int launch_disp_to_coord<T>(const T *, int, int, int, size_t, T *, hipStream_t);  \
     template int launch_compose<T>(const T *, const T *, const T *, const T *, int, int, T *,  \
                                    T *, hipStream_t);                                          \
     template int launch_objective<T>(const ObjectiveArgs<T> &, hipStream_t);                   \
     template int launch_squaring<T>(const T *, const T *, const T *, const T *, int, int, int, \
                                     size_t, size_t, size_t, size_t, T *, T *, T *, T *, T *,   \
-                                    hipStream_t);                                              \
+                                    hipStream_t, bool);                                        \
     template int launch_combine<T>(const T *, const T *, const T *, const T *, const T *,      \
                                    double, double, double, double, int, int, int, size_t,      \
                                    size_t, size_t, T *, T *, double *, double *, int *,        \

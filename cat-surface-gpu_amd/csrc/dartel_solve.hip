@@ -311,7 +311,7 @@ struct ChainResult {
 
 template <typename T>
 int squaring_chain(T *b0, T *a0, T **maps, T **jacs, T **bbuf, T **abuf, int k, bool save,
-                   int nx, int ny, int batch, ChainResult<T> &res, hipStream_t s)
+                   int nx, int ny, int batch, ChainResult<T> &res, hipStream_t s, bool disp)
 {
     size_t P = (size_t)nx * ny;
     int cur = 0, oth = 1;
@@ -323,7 +323,7 @@ int squaring_chain(T *b0, T *a0, T **maps, T **jacs, T **bbuf, T **abuf, int k, 
         CSG_TRY(launch_squaring<T>(bc, ac, maps[cur], jacs[cur], nx, ny, batch, 2 * P, 3 * P,
                                    2 * P, 4 * P, bn, an, compose ? nullptr : maps[oth],
                                    compose ? maps[oth] : nullptr, compose ? jacs[oth] : nullptr,
-                                   s));
+                                   s, disp));
         bc = bn;
         ac = an;
         pp ^= 1;
@@ -363,14 +363,16 @@ int dartel_step_run(const StepArgs<T> &a, void *ws, size_t ws_bytes, hipStream_t
                     c.used, ws_bytes);
     const size_t P = pl.P;
     const double scale = 1.0 / (double)(1LL << k);
+    // float32 keeps its internal maps as displacements (see csg_common.h)
+    const bool disp = sizeof(T) == 4;
 
     // expdef(v, k) with Jacobian.  dartel_grid.py:1493-1501
     CSG_TRY(launch_expdef_init<T>(a.v, scale, nx, ny, batch, 2 * P, pl.em[0], pl.ej[0], 2 * P,
-                                  4 * P, s));
+                                  4 * P, s, disp));
     int e = 0;
     for (int i = 0; i < k; ++i) {
         CSG_TRY(launch_compose_self<T>(pl.em[e], pl.ej[e], nx, ny, batch, 2 * P, 4 * P,
-                                       pl.em[e ^ 1], pl.ej[e ^ 1], s));
+                                       pl.em[e ^ 1], pl.ej[e ^ 1], s, disp));
         e ^= 1;
     }
     // jac_div_smalldef + initialise_objfun + smalldef_jac(v, -scale).  :1502-1523
@@ -398,6 +400,7 @@ int dartel_step_run(const StepArgs<T> &a, void *ws, size_t ws_bytes, hipStream_t
     o.rstride = 2 * P;
     o.hstride = 3 * P;
     o.partial = pl.partial[0];
+    o.disp = disp;
     CSG_TRY(launch_objective<T>(o, s));
     CSG_TRY(launch_reduce_partials(pl.partial[0], pl.nblocks, batch, 0.5, a.metrics,
                                    a.metrics_stride, 0, s));
@@ -405,7 +408,7 @@ int dartel_step_run(const StepArgs<T> &a, void *ws, size_t ws_bytes, hipStream_t
     ChainResult<T> fwd;
     T *fm[2] = {pl.fm[0], pl.fm[1]}, *fj[2] = {pl.fj[0], pl.fj[1]};
     CSG_TRY(squaring_chain<T>(pl.rb[0], pl.ra[0], fm, fj, pl.fb, pl.fa, k, a.code == 1, nx, ny,
-                              batch, fwd, s));
+                              batch, fwd, s, disp));
     T *bf = fwd.b, *af = fwd.a;
     if (k == 0) {
         // no squaring step ran: the forward result is the objective output itself;
@@ -416,6 +419,8 @@ int dartel_step_run(const StepArgs<T> &a, void *ws, size_t ws_bytes, hipStream_t
         af = pl.fa[0];
     }
     T *scratch = fwd.scratch;
+    // the scratch holds a map unless the chain ended on an increment (k even >= 2, no save)
+    bool scratch_is_map = (a.code == 1) || k == 0 || (k % 2 == 1);
     T *br = nullptr, *ar = nullptr;
     if (a.code == 1) {
         // reverse objective at the saved forward map.  :1544-1570
@@ -437,10 +442,11 @@ int dartel_step_run(const StepArgs<T> &a, void *ws, size_t ws_bytes, hipStream_t
         T *gm[2] = {pl.gm[0], pl.gm[1]}, *gj[2] = {pl.gj[0], pl.gj[1]};
         T *rbb[2] = {pl.rb[1], pl.crhs}, *rab[2] = {pl.ra[1], pl.chess};
         CSG_TRY(squaring_chain<T>(pl.rb[0], pl.ra[0], gm, gj, rbb, rab, k, false, nx, ny, batch,
-                                  rev, s));
+                                  rev, s, disp));
         br = rev.b;
         ar = rev.a;
         scratch = rev.scratch;
+        scratch_is_map = k == 0 || (k % 2 == 1);
         if (br == pl.crhs) {
             // the combine kernel writes crhs/chess: keep the reverse result elsewhere
             CSG_TRY(copy_async<T>(pl.crhs, pl.rb[1], 2 * P * batch, s));
@@ -463,6 +469,11 @@ int dartel_step_run(const StepArgs<T> &a, void *ws, size_t ws_bytes, hipStream_t
     CSG_TRY(launch_sqrt_inplace(a.metrics + 2, batch, a.metrics_stride, s));
     // FMG from the scratch guess.  :1596-1609, :1392-1415
     double sp[5] = {rp[0], rp[1], rp[2], rp[3], rp[4] + a.lmreg};
+    if (disp && scratch_is_map) {
+        // the reference starts the solver from the map's coordinates
+        CSG_TRY(launch_disp_to_coord<T>(scratch, nx, ny, batch, 2 * P, pl.u, s));
+        scratch = pl.u;
+    }
     if (a.assume_nonzero) {
         CSG_TRY(fmg_run<T>(pl.chess, pl.crhs, sp, a.cycles, a.nit, scratch, 0, nx, ny, batch,
                            pl.u, pl.fmg_ws, pl.fmg_bytes, s));
@@ -519,6 +530,7 @@ int solve_run(const SolveArgs<T> &a, void *ws, size_t ws_bytes, hipStream_t s)
                     ws_bytes);
     size_t step_bytes = step_workspace_bytes<T>(nx, ny, batch);
     const size_t P = (size_t)nx * ny;
+    const bool disp = sizeof(T) == 4;
     Carver c{(char *)ws, step_bytes, ws_bytes, false};
     T *v[2] = {c.template take<T>(2 * P * batch), c.template take<T>(2 * P * batch)};
     double *mscratch = c.template take<double>(3 * (size_t)batch);
@@ -577,15 +589,18 @@ int solve_run(const SolveArgs<T> &a, void *ws, size_t ws_bytes, hipStream_t s)
             const T *vl = v[cur] + 2 * P * ((size_t)j * a.steps_per_job + a.steps_per_job - 1);
             T *fm[2] = {pl.em[0] + 2 * P * j, pl.em[1] + 2 * P * j};
             CSG_TRY(launch_expdef_init<T>(vl, 1.0 / 1024.0, nx, ny, 1, 0, fm[0], pl.ej[0], 0, 0,
-                                          s));
+                                          s, disp));
         }
         int e = 0;
         for (int i = 0; i < 10; ++i) {
             CSG_TRY(launch_compose_self<T>(pl.em[e], nullptr, nx, ny, jobs, 2 * P, 0,
-                                           pl.em[e ^ 1], nullptr, s));
+                                           pl.em[e ^ 1], nullptr, s, disp));
             e ^= 1;
         }
-        CSG_TRY(copy_async<T>(pl.em[e], a.flow, 2 * P * jobs, s));
+        if (disp)
+            CSG_TRY(launch_disp_to_coord<T>(pl.em[e], nx, ny, jobs, 2 * P, a.flow, s));
+        else
+            CSG_TRY(copy_async<T>(pl.em[e], a.flow, 2 * P * jobs, s));
     }
     return kOk;
 }

@@ -58,5 +58,6 @@ int launch_apply_flow(const float *unit, int n, const T *flow, int nx, int ny, i
 template <typename T>
 int launch_average_xz(const T *avg, const T *main_, int n, const double *m, float *out,
                       hipStream_t s);
+int launch_normalise_f32(const float *in, int n, float *out, hipStream_t s);
 
 }  // namespace csg

@@ -632,7 +632,31 @@ k_average_xz(const T *__restrict__ avg, const T *__restrict__ main_, int n, doub
         out[3 * i + d] = (float)(r[d] / nrm);
 }
 
+// float32 unit vectors, |p| = sqrt(fma(z, z, fma(y, y, x·x))) clamped by
+// FLT_EPSILON (torch's CPU float32 norm).  surface_warp.py:518-522
+__global__ void __launch_bounds__(kBlock)
+k_normalise_f32(const float *__restrict__ in, int n, float *__restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float x = in[3 * i], y = in[3 * i + 1], z = in[3 * i + 2];
+    float nr = fmaxf(norm3f(x, y, z), 1.1920928955078125e-07f);
+    out[3 * i] = x / nr;
+    out[3 * i + 1] = y / nr;
+    out[3 * i + 2] = z / nr;
+}
+
 // ------------------------------------------------------------- launchers
+
+int launch_normalise_f32(const float *in, int n, float *out, hipStream_t s)
+{
+    if (n <= 0)
+        return kOk;
+    hipLaunchKernelGGL(k_normalise_f32, dim3(grid_for(n)), dim3(kBlock), 0, s, in, n, out);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
 
 template <typename T>
 int launch_resample(const T *x, const int *idx, const double *w, int n, T *out, hipStream_t s)

@@ -207,6 +207,10 @@ int csg_depth_potential(const float *pts, int n, const int *faces, int nf, const
 int csg_apply_flow(const float *unit, int n, const void *flow, int nx, int ny, int dtype,
                    int inverse, void *out, void *stream);
 
+/* float32 unit vectors of [n][3] points, as the warp normalises un-cached points
+ * (surface_warp.py:518-522). */
+int csg_normalise_points(const float *in, int n, float *out, void *stream);
+
 /* Un-rotate the pole-frame warp by `unrotate` (host, row-major 3x3, p m^T) and blend it
  * with the main warp.  Replaces gpu_pipeline._average_xz_surfaces and its caller
  * (src/cat_surface_gpu/gpu_pipeline.py:248-271, :1294-1310). */
