@@ -26,29 +26,88 @@ class RotationFeatureResult:
     relative_residual: float
 
 
-def depth_potential(stencil: SurfaceStencilDevice, smoothed_surface: torch.Tensor, *,
+def depth_potential(stencil: SurfaceStencilDevice, smoothed_surface, *,
                     alpha: float = 1.0 / 1000.0, max_iter: int = 1000,
-                    relaxation: float = 1.90) -> tuple[torch.Tensor, int, float]:
-    """Depth potential by colour-ordered SOR (rotation_feature.py:253-317)."""
+                    relaxation: float = 1.90) -> tuple[torch.Tensor, int, torch.Tensor]:
+    """Depth potential by colour-ordered SOR (rotation_feature.py:253-317).
+
+    ``smoothed_surface`` is one [n, 3] geometry or a [batch, n, 3] stack of
+    geometries on the stencil's topology; all are solved in one persistent
+    launch.  Returns the values ([n] or [batch, n], float64), the iteration
+    count and the relative residuals |rhs - A x| / |rhs| (float64, on the device,
+    shape [] or [batch]).
+    """
 
     topo = stencil.topology
-    pts = smoothed_surface.to(torch.float32).contiguous()
-    n = int(pts.shape[0])
+    pts = torch.as_tensor(smoothed_surface, dtype=torch.float32,
+                          device=stencil.device).contiguous()
+    single = pts.dim() == 2
+    if single:
+        pts = pts.unsqueeze(0)
+    if pts.dim() != 3 or pts.shape[2] != 3 or pts.shape[1] != topo.n_points:
+        raise ValueError(f"depth-potential geometry must have shape [batch, {topo.n_points}, 3]")
+    batch, n = int(pts.shape[0]), int(pts.shape[1])
     nf = int(topo.faces.shape[0])
     nnz = int(topo.sys_cols.numel())
-    x = torch.empty(n, dtype=torch.float64, device=pts.device)
-    rel = torch.empty(1, dtype=torch.float64, device=pts.device)
+    x = torch.empty(batch, n, dtype=torch.float64, device=pts.device)
+    rel = torch.empty(batch, dtype=torch.float64, device=pts.device)
     lib = _lib.lib()
-    ws = _lib.workspace(lib.csg_depth_workspace_bytes(n, nf, nnz), pts.device)
+    ws = _lib.workspace(lib.csg_depth_workspace_bytes(n, nf, nnz, batch), pts.device)
     coff = (ctypes.c_int * topo.colour_off.size)(*topo.colour_off.tolist())
     _lib.check(lib.csg_depth_potential(
-        _lib.ptr(pts), n, _lib.ptr(topo.faces), nf, _lib.ptr(topo.inc_off), _lib.ptr(topo.inc),
-        _lib.ptr(topo.sys_row_off), _lib.ptr(topo.sys_cols), _lib.ptr(topo.sys_perm),
-        _lib.ptr(topo.sys_alpha), nnz, coff, _lib.ptr(topo.colour_verts),
-        int(topo.colour_off.size - 1), float(alpha), int(max_iter), float(relaxation),
-        _lib.ptr(x), _lib.ptr(rel), _lib.ptr(ws), ws.numel(), _lib.stream_handle(pts.device)),
+        _lib.ptr(pts), batch, n, _lib.ptr(topo.faces), nf, _lib.ptr(topo.inc_off),
+        _lib.ptr(topo.inc), _lib.ptr(topo.sys_row_off), _lib.ptr(topo.sys_cols),
+        _lib.ptr(topo.sys_contrib_off), _lib.ptr(topo.sys_contrib), nnz, coff,
+        _lib.ptr(topo.colour_verts), int(topo.colour_off.size - 1), float(alpha), int(max_iter),
+        float(relaxation), _lib.ptr(x), _lib.ptr(rel), _lib.ptr(ws), ws.numel(),
+        _lib.stream_handle(pts.device)),
         "csg_depth_potential")
-    return x, int(max_iter), float(rel.item())
+    if single:
+        return x[0], int(max_iter), rel[0]
+    return x, int(max_iter), rel
+
+
+def _feature_geometry(stencil: SurfaceStencilDevice, surface_vertices, heat_fwhm: float,
+                      smoothed_surface) -> torch.Tensor:
+    """Resampled, heat-smoothed geometry on the stencil sphere (rotation_feature.py:356-370)."""
+
+    if smoothed_surface is None:
+        mapped = stencil.resample_vertices(surface_vertices)
+        return stencil.smooth_geometry(mapped, heat_fwhm)
+    smoothed_surface = torch.as_tensor(smoothed_surface, dtype=torch.float32,
+                                       device=stencil.device).contiguous()
+    expected = (int(stencil.sphere_points.shape[0]), 3)
+    if tuple(smoothed_surface.shape) != expected:
+        raise ValueError(f"coarse-feature geometry must have shape {expected}")
+    return smoothed_surface
+
+
+def _given_depth(stencil: SurfaceStencilDevice, depth_values) -> torch.Tensor:
+    values = torch.as_tensor(depth_values, dtype=torch.float64,
+                             device=stencil.device).reshape(-1).contiguous()
+    if values.numel() != stencil.sphere_points.shape[0]:
+        raise ValueError("Upstream depth-potential length must match the coarse sphere")
+    return values
+
+
+def _finish_feature(stencil: SurfaceStencilDevice, geometry: torch.Tensor, values: torch.Tensor,
+                    iterations: int, residual: float) -> RotationFeatureResult:
+    """fwhm-50 value smoothing and min/max normalisation (rotation_feature.py:384-399)."""
+
+    em = stencil.edge_mean_distance(geometry)
+    params = [heat_parameters_from_edge_mean(em, 50.0)]
+    smoothed = stencil._smooth_values_batch(values.reshape(1, -1), geometry, params, False)[0]
+    return RotationFeatureResult(values=_minmax(smoothed, 2.220446049250313e-16),
+                                 iterations=iterations, relative_residual=residual)
+
+
+def _check_solver(solver: str, need_solve: bool) -> None:
+    if solver not in {"colored-sor", "pcg"}:
+        raise ValueError("rotation feature solver must be 'colored-sor' or 'pcg'")
+    if solver == "pcg" and need_solve:
+        raise NotImplementedError(
+            "this build solves the depth potential with the reference's default "
+            "colour-ordered SOR; solver='pcg' is not built")
 
 
 def compute_rotation_feature(stencil: SurfaceStencilDevice, surface_vertices: torch.Tensor, *,
@@ -59,36 +118,13 @@ def compute_rotation_feature(stencil: SurfaceStencilDevice, surface_vertices: to
                              solver: str = "colored-sor") -> RotationFeatureResult:
     """One CAT-compatible smoothed depth-potential feature (rotation_feature.py:320-399)."""
 
-    if solver not in {"colored-sor", "pcg"}:
-        raise ValueError("rotation feature solver must be 'colored-sor' or 'pcg'")
-    if smoothed_surface is None:
-        mapped = stencil.resample_vertices(surface_vertices)
-        smoothed_surface = stencil.smooth_geometry(mapped, heat_fwhm)
-    else:
-        smoothed_surface = torch.as_tensor(smoothed_surface, dtype=torch.float32,
-                                           device=stencil.device).contiguous()
-        expected = (int(stencil.sphere_points.shape[0]), 3)
-        if tuple(smoothed_surface.shape) != expected:
-            raise ValueError(f"coarse-feature geometry must have shape {expected}")
+    _check_solver(solver, depth_values is None)
+    geometry = _feature_geometry(stencil, surface_vertices, heat_fwhm, smoothed_surface)
     if depth_values is not None:
-        values = torch.as_tensor(depth_values, dtype=torch.float64,
-                                 device=stencil.device).reshape(-1).contiguous()
-        if values.numel() != stencil.sphere_points.shape[0]:
-            raise ValueError("Upstream depth-potential length must match the coarse sphere")
-        iterations, residual = 0, 0.0
-    else:
-        if solver == "pcg":
-            raise NotImplementedError(
-                "this build solves the depth potential with the reference's default "
-                "colour-ordered SOR; solver='pcg' is not built")
-        values, iterations, residual = depth_potential(stencil, smoothed_surface,
-                                                       alpha=depth_alpha, max_iter=max_iter)
-    em = stencil.edge_mean_distance(smoothed_surface)
-    params = [heat_parameters_from_edge_mean(em, 50.0)]
-    smoothed = stencil._smooth_values_batch(values.reshape(1, -1), smoothed_surface, params,
-                                            False)[0]
-    return RotationFeatureResult(values=_minmax(smoothed, 2.220446049250313e-16),
-                                 iterations=iterations, relative_residual=residual)
+        return _finish_feature(stencil, geometry, _given_depth(stencil, depth_values), 0, 0.0)
+    values, iterations, residual = depth_potential(stencil, geometry, alpha=depth_alpha,
+                                                   max_iter=max_iter)
+    return _finish_feature(stencil, geometry, values, iterations, float(residual))
 
 
 def compute_rotation_features(stencils: Sequence[SurfaceStencilDevice],
@@ -98,8 +134,13 @@ def compute_rotation_features(stencils: Sequence[SurfaceStencilDevice],
                               depth_alpha: float = 1.0 / 1000.0, max_iter: int = 1000,
                               tolerance: float = 1.0e-9, solver: str = "colored-sor",
                               parallel_sides: bool = True) -> tuple[RotationFeatureResult, ...]:
-    """Source and target features (rotation_feature.py:402-458), one after the other
-    on the current stream (each side already fills the device)."""
+    """Source and target features (rotation_feature.py:402-458).
+
+    Sides whose stencils share one sphere topology (the usual case: both are
+    built on the same icosphere) have their depth potentials solved together
+    in one persistent SOR launch; ``parallel_sides=False`` solves them one at a
+    time.  The results are identical either way.
+    """
 
     if len(stencils) != len(surfaces) or len(stencils) != len(heat_fwhms):
         raise ValueError("stencils, surfaces, and heat_fwhms must have equal lengths")
@@ -107,10 +148,33 @@ def compute_rotation_features(stencils: Sequence[SurfaceStencilDevice],
         raise ValueError("smoothed_surfaces length must match stencils")
     if depth_values is not None and len(depth_values) != len(stencils):
         raise ValueError("depth_values length must match stencils")
-    return tuple(
-        compute_rotation_feature(
-            stencils[i], surfaces[i], heat_fwhm=float(heat_fwhms[i]),
-            smoothed_surface=None if smoothed_surfaces is None else smoothed_surfaces[i],
-            depth_values=None if depth_values is None else depth_values[i],
-            depth_alpha=depth_alpha, max_iter=max_iter, tolerance=tolerance, solver=solver)
-        for i in range(len(stencils)))
+    _check_solver(solver, depth_values is None)
+    geometries = [
+        _feature_geometry(stencils[i], surfaces[i], float(heat_fwhms[i]),
+                          None if smoothed_surfaces is None else smoothed_surfaces[i])
+        for i in range(len(stencils))]
+    if depth_values is not None:
+        return tuple(_finish_feature(stencils[i], geometries[i],
+                                     _given_depth(stencils[i], depth_values[i]), 0, 0.0)
+                     for i in range(len(stencils)))
+    # group sides by shared topology (the topology cache returns one object per mesh)
+    groups: list[list[int]] = []
+    for i, st in enumerate(stencils):
+        for g in groups:
+            if parallel_sides and stencils[g[0]].topology is st.topology:
+                g.append(i)
+                break
+        else:
+            groups.append([i])
+    values: list[torch.Tensor | None] = [None] * len(stencils)
+    residuals: list[float] = [0.0] * len(stencils)
+    for g in groups:
+        stack = torch.stack([geometries[i] for i in g]) if len(g) > 1 else geometries[g[0]]
+        x, _, rel = depth_potential(stencils[g[0]], stack, alpha=depth_alpha, max_iter=max_iter)
+        rel_host = rel.reshape(-1).tolist()
+        for k, i in enumerate(g):
+            values[i] = x[k] if len(g) > 1 else x
+            residuals[i] = float(rel_host[k])
+    return tuple(_finish_feature(stencils[i], geometries[i], values[i], int(max_iter),
+                                 residuals[i])
+                 for i in range(len(stencils)))

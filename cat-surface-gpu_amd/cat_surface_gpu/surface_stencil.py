@@ -88,10 +88,11 @@ class Topology:
     inc: torch.Tensor
     colour_off: np.ndarray  # host int32 [C + 1]
     colour_verts: torch.Tensor
-    sys_row_off: torch.Tensor
+    sys_row_off: torch.Tensor     # merged (row, col) CSR of the depth system
     sys_cols: torch.Tensor
-    sys_perm: torch.Tensor
-    sys_alpha: torch.Tensor
+    sys_contrib_off: torch.Tensor  # per merged entry: its COO contributions
+    sys_contrib: torch.Tensor      # COO entry indices, COO order within an entry
+    colour_off_dev: torch.Tensor = None
     faces_np: np.ndarray = field(repr=False, default=None)
 
     @property
@@ -112,14 +113,21 @@ def topology_for(faces: np.ndarray, n_points: int, device: torch.device) -> Topo
     ccount = np.bincount(colours)
     colour_off = np.zeros(ccount.size + 1, np.int32)
     colour_off[1:] = np.cumsum(ccount)
-    # depth-system layout: COO blocks of rotation_feature.py:144-161, then the alpha diagonal
-    nf = f.shape[0]
+    # depth-system layout: COO blocks of rotation_feature.py:144-161, then the alpha
+    # diagonal; duplicate (row, col) entries are merged into one CSR entry whose
+    # value is the sum of its COO contributions taken in COO order
     i0, i1, i2 = (f[:, k].astype(np.int64) for k in range(3))
     rows = np.concatenate([i0, i1, i2, i0, i1, i1, i2, i0, i2, np.arange(n_points)])
     cols = np.concatenate([i0, i1, i2, i1, i0, i2, i1, i2, i0, np.arange(n_points)])
-    perm = np.argsort(rows, kind="stable")
+    key = rows * n_points + cols
+    perm = np.argsort(key, kind="stable")
+    uniq, counts = np.unique(key[perm], return_counts=True)
+    m_rows = uniq // n_points
+    m_cols = uniq % n_points
     row_off = np.zeros(n_points + 1, np.int32)
-    row_off[1:] = np.cumsum(np.bincount(rows, minlength=n_points))
+    row_off[1:] = np.cumsum(np.bincount(m_rows, minlength=n_points))
+    contrib_off = np.zeros(uniq.size + 1, np.int32)
+    contrib_off[1:] = np.cumsum(counts)
 
     def dev(a, dtype=torch.int32):
         return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
@@ -134,9 +142,10 @@ def topology_for(faces: np.ndarray, n_points: int, device: torch.device) -> Topo
         colour_off=colour_off,
         colour_verts=dev(order.astype(np.int32)),
         sys_row_off=dev(row_off),
-        sys_cols=dev(cols[perm].astype(np.int32)),
-        sys_perm=dev(perm.astype(np.int32)),
-        sys_alpha=dev((perm >= 9 * nf).astype(np.uint8), torch.uint8),
+        sys_cols=dev(m_cols.astype(np.int32)),
+        sys_contrib_off=dev(contrib_off),
+        sys_contrib=dev(perm.astype(np.int32)),
+        colour_off_dev=dev(colour_off),
         faces_np=f,
     )
     _TOPOLOGY_CACHE[key] = topo

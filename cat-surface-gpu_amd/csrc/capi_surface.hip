@@ -1,5 +1,6 @@
 // C ABI of the surface side of the hot path (declared in include/csg.h).
 #include "../../include/csg.h"
+#include "depth_kernels.h"
 #include "surface_kernels.h"
 
 using namespace csg;
@@ -238,63 +239,83 @@ int csg_type5_sheets(const float *geom, int n, const int *faces, const int *nbr_
                                            (double *)out, (size_t)m, s);
 }
 
-size_t csg_depth_workspace_bytes(int n, int nf, int nnz)
+namespace {
+
+DepthArgs carve_depth(Carve &c, int batch, int n, int nf, int nnz)
+{
+    DepthArgs a{};
+    a.fnormal = c.take<double>((size_t)batch * 9 * nf);
+    a.farea = c.take<double>((size_t)batch * 3 * nf);
+    a.coo = c.take<double>((size_t)batch * 9 * nf);
+    a.normals = c.take<double>((size_t)batch * 3 * n);
+    a.areas = c.take<double>((size_t)batch * n);
+    a.lap = c.take<double>((size_t)batch * nnz);
+    a.sys = c.take<double>((size_t)batch * nnz);
+    a.mc = c.take<double>((size_t)batch * n);
+    a.diag = c.take<double>((size_t)batch * n);
+    a.rhs = c.take<double>((size_t)batch * n);
+    a.partial = c.take<double>((size_t)batch * 2 * grid_for(n) + 64);
+    a.barrier = c.take<unsigned>(4);
+    return a;
+}
+
+}  // namespace
+
+size_t csg_depth_workspace_bytes(int n, int nf, int nnz, int batch)
 {
     Carve c{nullptr, 0};
-    c.take<double>((size_t)9 * nf);  // fnormal
-    c.take<double>((size_t)3 * nf);  // farea
-    c.take<double>((size_t)9 * nf);  // coo values
-    c.take<double>((size_t)3 * n);   // normals
-    c.take<double>((size_t)n);       // areas
-    c.take<double>((size_t)nnz);     // csr values
-    c.take<double>((size_t)n);       // mean curvature
-    c.take<double>((size_t)n);       // diagonal
-    c.take<double>((size_t)n);       // rhs
-    c.take<double>((size_t)2 * grid_for(n) + 64);
+    carve_depth(c, batch, n, nf, nnz);
     return c.used;
 }
 
-// Depth potential of the rotation feature: (L + alpha·diag(area)) x = rhs by
-// colour-ordered SOR (rotation_feature.py:253-317).  colour_verts lists the
-// vertices of colour c at colour_off[c]..colour_off[c+1].  relres (device
-// double) receives |rhs - A x| / |rhs|.
-int csg_depth_potential(const float *pts, int n, const int *faces, int nf, const int *inc_off,
-                        const int *inc, const int *row_off, const int *cols, const int *perm,
-                        const unsigned char *is_alpha, int nnz, const int *colour_off,
-                        const int *colour_verts, int ncolours, double alpha, int max_iter,
-                        double omega, double *x, double *relres, void *ws, size_t ws_bytes,
-                        void *stream)
+// Depth potential of the rotation feature for `batch` meshes of one topology:
+// (L + alpha·diag(area)) x = rhs by colour-ordered SOR (rotation_feature.py:
+// 253-317).  Synchronises once at the end to check the persistent solve.
+int csg_depth_potential(const float *pts, int batch, int n, const int *faces, int nf,
+                        const int *inc_off, const int *inc, const int *row_off, const int *cols,
+                        const int *contrib_off, const int *contrib, int nnz,
+                        const int *colour_off, const int *colour_verts, int ncolours,
+                        double alpha, int max_iter, double omega, double *x, double *relres,
+                        void *ws, size_t ws_bytes, void *stream)
 {
-    if (n < 1 || nf < 1 || max_iter < 0)
+    if (n < 1 || nf < 1 || batch < 1 || max_iter < 0 || colour_off == nullptr)
         return fail(kInvalidArgument, "invalid depth-potential arguments");
-    if (ws_bytes < csg_depth_workspace_bytes(n, nf, nnz))
+    if (ncolours < 1 || ncolours > kMaxColours)
+        return fail(kInvalidArgument, "colour count must be in [1, %d], got %d", kMaxColours,
+                    ncolours);
+    if (ws_bytes < csg_depth_workspace_bytes(n, nf, nnz, batch))
         return fail(kInvalidArgument, "depth-potential workspace too small");
     hipStream_t s = S(stream);
     Carve c{(char *)ws, 0};
-    double *fnormal = c.take<double>((size_t)9 * nf);
-    double *farea = c.take<double>((size_t)3 * nf);
-    double *coo = c.take<double>((size_t)9 * nf);
-    double *normals = c.take<double>((size_t)3 * n);
-    double *areas = c.take<double>((size_t)n);
-    double *val = c.take<double>((size_t)nnz);
-    double *mc = c.take<double>((size_t)n);
-    double *diag = c.take<double>((size_t)n);
-    double *rhs = c.take<double>((size_t)n);
-    double *partial = c.take<double>((size_t)2 * grid_for(n) + 64);
-    CSG_TRY(launch_depth_face_terms(pts, faces, nf, fnormal, farea, coo, s));
-    CSG_TRY(launch_depth_vertex_terms(fnormal, farea, inc_off, inc, n, normals, areas, s));
-    CSG_TRY(launch_csr_values(coo, perm, nnz, 9 * nf, areas, alpha, val, s));
-    CSG_TRY(launch_depth_rhs(pts, row_off, cols, val, is_alpha, normals, areas, n, mc, diag,
-                             partial, rhs, s));
-    CSG_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * n, s));
-    // colour offsets are host data
-    for (int it = 0; it < max_iter; ++it)
-        for (int col = 0; col < ncolours; ++col)
-            CSG_TRY(launch_sor_color(colour_verts + colour_off[col],
-                                     colour_off[col + 1] - colour_off[col], row_off, cols, val,
-                                     rhs, diag, omega, x, s));
+    DepthArgs a = carve_depth(c, batch, n, nf, nnz);
+    a.pts = pts;
+    a.faces = faces;
+    a.inc_off = inc_off;
+    a.inc = inc;
+    a.row_off = row_off;
+    a.cols = cols;
+    a.contrib_off = contrib_off;
+    a.contrib = contrib;
+    a.colour_verts = colour_verts;
+    a.batch = batch;
+    a.n = n;
+    a.nf = nf;
+    a.nnz = nnz;
+    a.alpha = alpha;
+    a.x = x;
+    SorColours co{};
+    co.count = ncolours;
+    for (int i = 0; i <= ncolours; ++i)
+        co.off[i] = colour_off[i];
+    CSG_TRY(depth_setup(a, s));
+    CSG_TRY(depth_sor(a, co, max_iter, omega, s));
     if (relres)
-        CSG_TRY(launch_relative_residual(row_off, cols, val, rhs, x, n, partial, relres, s));
+        CSG_TRY(depth_residual(a, relres, s));
+    int timeout = 0;
+    CSG_HIP_CHECK(hipMemcpyAsync(&timeout, a.barrier + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+    CSG_HIP_CHECK(hipStreamSynchronize(s));
+    if (timeout)
+        return fail(kRuntimeError, "depth-potential SOR grid barrier timed out");
     return kOk;
 }
 

@@ -1,0 +1,37 @@
+// Depth-potential solve for the rotation feature (internal to libcsg_hip).
+#pragma once
+
+#include "csg_common.h"
+
+namespace csg {
+
+constexpr int kSorBlock = 256;
+constexpr int kMaxColours = 32;
+
+// Colour groups: vertices of colour c are colour_verts[off[c] .. off[c+1]).
+struct SorColours {
+    int count;
+    int off[kMaxColours + 1];
+};
+
+struct DepthArgs {
+    const float *pts;  // [batch][n][3]
+    const int *faces;  // [nf][3]
+    const int *inc_off, *inc;
+    const int *row_off, *cols;          // merged CSR, nnz entries
+    const int *contrib_off, *contrib;   // per merged entry, its COO entries
+    const int *colour_verts;
+    int batch, n, nf, nnz;
+    double alpha;
+    // workspace
+    double *fnormal, *farea, *coo, *normals, *areas, *lap, *sys, *mc, *diag, *rhs, *partial;
+    unsigned *barrier;  // 2 counters + 1 timeout flag
+    double *x;          // [batch][n]
+};
+
+int depth_setup(const DepthArgs &a, hipStream_t s);
+int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double omega,
+              hipStream_t s);
+int depth_residual(const DepthArgs &a, double *out, hipStream_t s);
+
+}  // namespace csg

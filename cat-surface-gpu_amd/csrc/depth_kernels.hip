@@ -1,0 +1,422 @@
+// Depth potential of the rotation feature on gfx950, for a batch of meshes
+// that share one topology (source and target stencils):
+//   per face:   stable-normal terms, mixed Voronoi area terms, the 9
+//               cotangent entries (float64);
+//   per vertex: normals and areas gathered from incident (face, corner)
+//               entries in the reference's index_add_ order;
+//   system:     duplicate (row, col) entries merged into one CSR entry (the
+//               sum of its contributions in COO order), Laplacian·points,
+//               mean curvature, right-hand side, diagonal;
+//   solve:      colour-ordered SOR, all max_iter × colours sweeps in ONE
+//               persistent cooperative launch with a grid barrier between
+//               colours (the sweep order of the reference, 1000 × ~7 colours,
+//               would otherwise be ~7000 dependent launches).
+//
+// Reference semantics: src/cat_surface_gpu/rotation_feature.py:24-317.
+#include "depth_kernels.h"
+
+namespace csg {
+
+__global__ void __launch_bounds__(kBlock)
+k_depth_face_terms(const float *__restrict__ pts, const int *__restrict__ faces, int n, int nf,
+                   double *__restrict__ fnormal, double *__restrict__ farea,
+                   double *__restrict__ coo)
+{
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf)
+        return;
+    const int b = blockIdx.y;
+    pts += (size_t)b * 3 * n;
+    fnormal += (size_t)b * 9 * nf;
+    farea += (size_t)b * 3 * nf;
+    coo += (size_t)b * 9 * nf;
+    const double eps = 2.220446049250313e-16;
+    const int *fv = faces + 3 * f;
+    double p[3][3];
+    for (int c = 0; c < 3; ++c)
+        for (int d = 0; d < 3; ++d)
+            p[c][d] = (double)pts[3 * fv[c] + d];
+    double v1[3], v2[3], v3[3];
+    for (int d = 0; d < 3; ++d) {
+        v1[d] = p[1][d] - p[0][d];
+        v2[d] = p[2][d] - p[1][d];
+        v3[d] = p[0][d] - p[2][d];
+    }
+    auto dot = [](const double *a, const double *c) { return a[0] * c[0] + a[1] * c[1] + a[2] * c[2]; };
+    auto cross = [](const double *a, const double *c, double *o) {
+        o[0] = a[1] * c[2] - a[2] * c[1];
+        o[1] = a[2] * c[0] - a[0] * c[2];
+        o[2] = a[0] * c[1] - a[1] * c[0];
+    };
+    auto norm = [&](const double *a) { return sqrt(dot(a, a)); };
+    // _stable_normals (rotation_feature.py:38-73)
+    double u1[3], u2[3], u3[3];
+    double n1 = fmax(norm(v1), eps), n2 = fmax(norm(v2), eps), n3 = fmax(norm(v3), eps);
+    for (int d = 0; d < 3; ++d) {
+        u1[d] = v1[d] / n1;
+        u2[d] = v2[d] / n2;
+        u3[d] = v3[d] / n3;
+    }
+    double c31[3], c23[3], c12[3];
+    cross(u3, u1, c31);
+    cross(u2, u3, c23);
+    cross(u1, u2, c12);
+    double m31 = fmax(norm(c31), eps), m23 = fmax(norm(c23), eps), m12 = fmax(norm(c12), eps);
+    double w0 = (1.0 - dot(u3, u1)) / m31;
+    double w1 = (1.0 - dot(u1, u2)) / m12;
+    double w2 = (1.0 - dot(u2, u3)) / m23;
+    double tn[3] = {c12[0] / m12, c12[1] / m12, c12[2] / m12};
+    for (int d = 0; d < 3; ++d) {
+        fnormal[9 * (size_t)f + d] = tn[d] * w0;
+        fnormal[9 * (size_t)f + 3 + d] = tn[d] * w1;
+        fnormal[9 * (size_t)f + 6 + d] = tn[d] * w2;
+    }
+    // _mixed_voronoi_areas (rotation_feature.py:76-128)
+    double cr[3];
+    cross(v1, v2, cr);
+    double area = fmax(norm(cr), eps);
+    double d31 = dot(v3, v1), d12 = dot(v1, v2), d23 = dot(v2, v3), d13 = dot(v1, v3);
+    bool bad0 = d31 > 0.0, bad1 = d12 > 0.0, bad2 = d23 > 0.0;
+    double l1 = dot(v1, v1), l2 = dot(v2, v2), l3 = dot(v3, v3);
+    double a0, a1, a2;
+    if (bad0 || bad1 || bad2) {
+        a0 = a1 = a2 = 0.125 * area;
+    } else {
+        a0 = -0.125 * (l3 * d12 + l1 * d23) / area;
+        a1 = -0.125 * (l2 * d13 + l1 * d23) / area;
+        a2 = -0.125 * (l2 * d13 + l3 * d12) / area;
+    }
+    farea[3 * (size_t)f] = a0 + (bad0 ? 0.125 * area : 0.0);
+    farea[3 * (size_t)f + 1] = a1 + (bad1 ? 0.125 * area : 0.0);
+    farea[3 * (size_t)f + 2] = a2 + (bad2 ? 0.125 * area : 0.0);
+    // _system_entries (rotation_feature.py:131-186)
+    double e23 = d23 / area, e13 = d13 / area, e12 = d12 / area;
+    double vals[9] = {-0.5 * (e23 + e12), -0.5 * (e13 + e23), -0.5 * (e12 + e13),
+                      0.5 * e23, 0.5 * e23, 0.5 * e13, 0.5 * e13, 0.5 * e12, 0.5 * e12};
+    for (int k = 0; k < 9; ++k)
+        coo[(size_t)k * nf + f] = vals[k];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_depth_vertex_terms(const double *__restrict__ fnormal, const double *__restrict__ farea,
+                     const int *__restrict__ inc_off, const int *__restrict__ inc, int n, int nf,
+                     double *__restrict__ normals, double *__restrict__ areas)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int b = blockIdx.y;
+    fnormal += (size_t)b * 9 * nf;
+    farea += (size_t)b * 3 * nf;
+    const double eps = 2.220446049250313e-16;
+    double nx = 0.0, ny = 0.0, nz = 0.0, a = 0.0;
+    for (int k = inc_off[i]; k < inc_off[i + 1]; ++k) {
+        int fc = inc[k];
+        const double *fn = fnormal + 3 * (size_t)fc;
+        nx += fn[0];
+        ny += fn[1];
+        nz += fn[2];
+        a += farea[fc];
+    }
+    double nr = fmax(sqrt(nx * nx + ny * ny + nz * nz), eps);
+    double *o = normals + (size_t)b * 3 * n;
+    o[3 * i] = nx / nr;
+    o[3 * i + 1] = ny / nr;
+    o[3 * i + 2] = nz / nr;
+    areas[(size_t)b * n + i] = (!isfinite(a) || a == 0.0) ? 1.0 : a;
+}
+
+// Merged entry values: lap = Σ face contributions, sys = lap + Σ alpha·area.
+__global__ void __launch_bounds__(kBlock)
+k_merged_values(const double *__restrict__ coo, const int *__restrict__ contrib_off,
+                const int *__restrict__ contrib, const int *__restrict__ row_of_entry_unused,
+                int nnz, int nf, int n, const double *__restrict__ areas, double alpha,
+                double *__restrict__ lap, double *__restrict__ sys)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nnz)
+        return;
+    const int b = blockIdx.y;
+    coo += (size_t)b * 9 * nf;
+    areas += (size_t)b * n;
+    const int nface9 = 9 * nf;
+    double l = 0.0, s = 0.0;
+    for (int k = contrib_off[m]; k < contrib_off[m + 1]; ++k) {
+        int e = contrib[k];
+        if (e < nface9) {
+            l += coo[e];
+            s += coo[e];
+        } else {
+            s += alpha * areas[e - nface9];
+        }
+    }
+    lap[(size_t)b * nnz + m] = l;
+    sys[(size_t)b * nnz + m] = s;
+}
+
+// Laplacian·points → mean curvature; diagonal of the system.
+__global__ void __launch_bounds__(kBlock)
+k_depth_rhs_parts(const float *__restrict__ pts, const int *__restrict__ row_off,
+                  const int *__restrict__ cols, const double *__restrict__ lap,
+                  const double *__restrict__ sys, const double *__restrict__ normals,
+                  const double *__restrict__ areas, int n, int nnz, double *__restrict__ mc,
+                  double *__restrict__ diag)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int b = blockIdx.y;
+    pts += (size_t)b * 3 * n;
+    lap += (size_t)b * nnz;
+    sys += (size_t)b * nnz;
+    double lx = 0.0, ly = 0.0, lz = 0.0, dg = 0.0;
+    for (int k = row_off[i]; k < row_off[i + 1]; ++k) {
+        int j = cols[k];
+        double v = lap[k];
+        if (j == i)
+            dg += sys[k];
+        lx += v * (double)pts[3 * j];
+        ly += v * (double)pts[3 * j + 1];
+        lz += v * (double)pts[3 * j + 2];
+    }
+    const double *nr = normals + (size_t)b * 3 * n;
+    double a = areas[(size_t)b * n + i];
+    double h = 0.5 * ((2.0 * lx) * nr[3 * i] + (2.0 * ly) * nr[3 * i + 1] +
+                      (2.0 * lz) * nr[3 * i + 2]) / a;
+    mc[(size_t)b * n + i] = isfinite(h) ? h : 0.0;
+    diag[(size_t)b * n + i] = fmax(dg, 2.220446049250313e-16);
+}
+
+// Σ mc·area and Σ area per block.
+__global__ void __launch_bounds__(kBlock)
+k_weighted_sums(const double *__restrict__ mc, const double *__restrict__ areas, int n,
+                double *__restrict__ partial, int nblocks)
+{
+    __shared__ double red[kBlock / 64];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    double a = 0.0, w = 0.0;
+    if (i < n) {
+        a = mc[(size_t)b * n + i] * areas[(size_t)b * n + i];
+        w = areas[(size_t)b * n + i];
+    }
+    double ra = block_sum(a, red);
+    double rw = block_sum(w, red);
+    if (threadIdx.x == 0) {
+        partial[2 * ((size_t)b * nblocks + blockIdx.x)] = ra;
+        partial[2 * ((size_t)b * nblocks + blockIdx.x) + 1] = rw;
+    }
+}
+
+// rhs = (mc - factor)·area, factor = Σ mc·a / Σ a; x = 0.
+__global__ void __launch_bounds__(kBlock)
+k_depth_rhs(const double *__restrict__ mc, const double *__restrict__ areas, int n,
+            const double *__restrict__ partial, int nblocks, double *__restrict__ rhs,
+            double *__restrict__ x)
+{
+    __shared__ double factor;
+    const int b = blockIdx.y;
+    if (threadIdx.x == 0) {
+        double a = 0.0, w = 0.0;
+        for (int k = 0; k < nblocks; ++k) {
+            a += partial[2 * ((size_t)b * nblocks + k)];
+            w += partial[2 * ((size_t)b * nblocks + k) + 1];
+        }
+        factor = a / w;
+    }
+    __syncthreads();
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        size_t o = (size_t)b * n + i;
+        rhs[o] = (mc[o] - factor) * areas[o];
+        x[o] = 0.0;
+    }
+}
+
+// Grid-wide barrier for a co-resident (cooperative) grid.  Returns false if a
+// waiting block gave up after kSpinLimit polls (the caller then exits, so
+// every wave terminates even if the barrier were ever broken).
+constexpr long kSpinLimit = 1L << 26;
+
+__device__ bool grid_barrier(unsigned *bar, unsigned nblocks, int *timeout)
+{
+    __shared__ int ok;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ok = 1;
+        unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned arrived =
+            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == nblocks - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            long spins = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (++spins > kSpinLimit) {
+                    __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+// All SOR sweeps: for it < max_iter, for each colour, every vertex of the
+// colour (in every batch mesh) takes x_i += ω (rhs_i - Σ_j A_ij x_j) / diag_i;
+// a grid barrier separates colours.  rotation_feature.py:296-311
+__global__ void __launch_bounds__(kSorBlock)
+k_sor_persistent(const int *__restrict__ verts, SorColours co, const int *__restrict__ row_off,
+                 const int *__restrict__ cols, const double *__restrict__ val,
+                 const double *__restrict__ rhs, const double *__restrict__ diag, double omega,
+                 int n, int nnz, int batch, int max_iter, double *__restrict__ x, unsigned *bar,
+                 int *timeout)
+{
+    const int nthreads = gridDim.x * blockDim.x;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int it = 0; it < max_iter; ++it) {
+        for (int c = 0; c < co.count; ++c) {
+            const int start = co.off[c], cnt = co.off[c + 1] - start;
+            for (int t = tid; t < cnt * batch; t += nthreads) {
+                const int b = t / cnt;
+                const int i = verts[start + (t - b * cnt)];
+                const double *vb = val + (size_t)b * nnz;
+                double *xb = x + (size_t)b * n;
+                double s = 0.0;
+                for (int k = row_off[i]; k < row_off[i + 1]; ++k)
+                    s += vb[k] * xb[cols[k]];
+                size_t o = (size_t)b * n + i;
+                xb[i] = xb[i] + omega * (rhs[o] - s) / diag[o];
+            }
+            if (!grid_barrier(bar, gridDim.x, timeout))
+                return;
+        }
+    }
+}
+
+// r = rhs - A x; per-block Σ r², Σ rhs².
+__global__ void __launch_bounds__(kBlock)
+k_residual_norms(const int *__restrict__ row_off, const int *__restrict__ cols,
+                 const double *__restrict__ val, const double *__restrict__ rhs,
+                 const double *__restrict__ x, int n, int nnz, double *__restrict__ partial,
+                 int nblocks)
+{
+    __shared__ double red[kBlock / 64];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    double r2 = 0.0, b2 = 0.0;
+    if (i < n) {
+        const double *vb = val + (size_t)b * nnz;
+        const double *xb = x + (size_t)b * n;
+        double s = 0.0;
+        for (int k = row_off[i]; k < row_off[i + 1]; ++k)
+            s += vb[k] * xb[cols[k]];
+        double rr = rhs[(size_t)b * n + i];
+        double r = rr - s;
+        r2 = r * r;
+        b2 = rr * rr;
+    }
+    double a = block_sum(r2, red);
+    double c = block_sum(b2, red);
+    if (threadIdx.x == 0) {
+        partial[2 * ((size_t)b * nblocks + blockIdx.x)] = a;
+        partial[2 * ((size_t)b * nblocks + blockIdx.x) + 1] = c;
+    }
+}
+
+__global__ void k_residual_final(const double *__restrict__ partial, int nblocks, int batch,
+                                 double *__restrict__ out)
+{
+    int b = threadIdx.x;
+    if (b >= batch)
+        return;
+    double a = 0.0, c = 0.0;
+    for (int k = 0; k < nblocks; ++k) {
+        a += partial[2 * ((size_t)b * nblocks + k)];
+        c += partial[2 * ((size_t)b * nblocks + k) + 1];
+    }
+    out[b] = sqrt(a) / fmax(sqrt(c), 2.220446049250313e-16);
+}
+
+int depth_setup(const DepthArgs &a, hipStream_t s)
+{
+    const int B = a.batch;
+    hipLaunchKernelGGL(k_depth_face_terms, dim3(grid_for(a.nf), B), dim3(kBlock), 0, s, a.pts,
+                       a.faces, a.n, a.nf, a.fnormal, a.farea, a.coo);
+    CSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_depth_vertex_terms, dim3(grid_for(a.n), B), dim3(kBlock), 0, s,
+                       a.fnormal, a.farea, a.inc_off, a.inc, a.n, a.nf, a.normals, a.areas);
+    CSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_merged_values, dim3(grid_for(a.nnz), B), dim3(kBlock), 0, s, a.coo,
+                       a.contrib_off, a.contrib, nullptr, a.nnz, a.nf, a.n, a.areas, a.alpha,
+                       a.lap, a.sys);
+    CSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_depth_rhs_parts, dim3(grid_for(a.n), B), dim3(kBlock), 0, s, a.pts,
+                       a.row_off, a.cols, a.lap, a.sys, a.normals, a.areas, a.n, a.nnz, a.mc,
+                       a.diag);
+    CSG_LAUNCH_CHECK();
+    int nb = (int)grid_for(a.n);
+    hipLaunchKernelGGL(k_weighted_sums, dim3(nb, B), dim3(kBlock), 0, s, a.mc, a.areas, a.n,
+                       a.partial, nb);
+    CSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_depth_rhs, dim3(nb, B), dim3(kBlock), 0, s, a.mc, a.areas, a.n,
+                       a.partial, nb, a.rhs, a.x);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double omega,
+              hipStream_t s)
+{
+    if (max_iter <= 0)
+        return kOk;
+    int dev = 0;
+    CSG_HIP_CHECK(hipGetDevice(&dev));
+    int cus = 0, per_cu = 0;
+    CSG_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sor_persistent,
+                                                               kSorBlock, 0));
+    int most = 0;
+    for (int c = 0; c < co.count; ++c)
+        most = (co.off[c + 1] - co.off[c]) > most ? (co.off[c + 1] - co.off[c]) : most;
+    int want = (int)grid_for((size_t)most * a.batch, kSorBlock);
+    int cap = cus * (per_cu > 0 ? per_cu : 1);
+    int grid = want < cap ? want : cap;
+    if (grid < 1)
+        grid = 1;
+    CSG_HIP_CHECK(hipMemsetAsync(a.barrier, 0, 2 * sizeof(unsigned) + sizeof(int), s));
+    int *timeout = reinterpret_cast<int *>(a.barrier + 2);
+    const int *verts = a.colour_verts;
+    const int *row_off = a.row_off, *cols = a.cols;
+    const double *val = a.sys, *rhs = a.rhs, *diag = a.diag;
+    int n = a.n, nnz = a.nnz, batch = a.batch;
+    double *x = a.x;
+    unsigned *bar = a.barrier;
+    SorColours colours = co;
+    void *args[] = {(void *)&verts, (void *)&colours, (void *)&row_off, (void *)&cols,
+                    (void *)&val,   (void *)&rhs,     (void *)&diag,    (void *)&omega,
+                    (void *)&n,     (void *)&nnz,     (void *)&batch,   (void *)&max_iter,
+                    (void *)&x,     (void *)&bar,     (void *)&timeout};
+    CSG_HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_sor_persistent, dim3(grid),
+                                             dim3(kSorBlock), args, 0, s));
+    return kOk;
+}
+
+int depth_residual(const DepthArgs &a, double *out, hipStream_t s)
+{
+    int nb = (int)grid_for(a.n);
+    hipLaunchKernelGGL(k_residual_norms, dim3(nb, a.batch), dim3(kBlock), 0, s, a.row_off,
+                       a.cols, a.sys, a.rhs, a.x, a.n, a.nnz, a.partial, nb);
+    CSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, s, a.partial, nb, a.batch, out);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+}  // namespace csg

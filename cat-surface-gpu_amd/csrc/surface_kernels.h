@@ -35,23 +35,6 @@ int launch_minmax_normalise(const double *x, size_t n, int nb, size_t stride, do
                             double *partial, T *out, size_t ostride, hipStream_t s);
 int launch_sheet_gather(const double *vals, size_t vstride, const int *idx, const double *w,
                         int m, int nb, double *out, hipStream_t s);
-int launch_depth_face_terms(const float *pts, const int *faces, int nf, double *fnormal,
-                            double *farea, double *coo, hipStream_t s);
-int launch_depth_vertex_terms(const double *fnormal, const double *farea, const int *inc_off,
-                              const int *inc, int n, double *normals, double *areas,
-                              hipStream_t s);
-int launch_csr_values(const double *coo, const int *perm, int nnz, int nface9,
-                      const double *areas, double alpha, double *val, hipStream_t s);
-int launch_depth_rhs(const float *pts, const int *row_off, const int *cols, const double *val,
-                     const unsigned char *is_alpha, const double *normals, const double *areas,
-                     int n, double *mc, double *diag, double *partial, double *rhs,
-                     hipStream_t s);
-int launch_sor_color(const int *verts, int count, const int *row_off, const int *cols,
-                     const double *val, const double *rhs, const double *diag, double omega,
-                     double *x, hipStream_t s);
-int launch_relative_residual(const int *row_off, const int *cols, const double *val,
-                             const double *rhs, const double *x, int n, double *partial,
-                             double *out, hipStream_t s);
 template <typename T>
 int launch_apply_flow(const float *unit, int n, const T *flow, int nx, int ny, int inverse,
                       T *out, hipStream_t s);

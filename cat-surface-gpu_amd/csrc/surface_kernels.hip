@@ -274,264 +274,6 @@ k_sheet_gather(const double *__restrict__ vals, size_t vstride, const int *__res
     out[(size_t)blockIdx.y * m + p] = v[ix[0]] * wp[0] + v[ix[1]] * wp[1] + v[ix[2]] * wp[2];
 }
 
-// ------------------------------------------------------------- depth potential
-
-// Per face: the three unit-edge normals terms of _stable_normals, the mixed
-// Voronoi area contributions, and the 9 cotangent entries (float64).
-// rotation_feature.py:24-186
-__global__ void __launch_bounds__(kBlock)
-k_depth_face_terms(const float *__restrict__ pts, const int *__restrict__ faces, int nf,
-                   double *__restrict__ fnormal /* [F][3][3] weighted normal per corner */,
-                   double *__restrict__ farea /* [F][3] */, double *__restrict__ coo /* [9][F] */)
-{
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nf)
-        return;
-    const double eps = 2.220446049250313e-16;
-    const int *fv = faces + 3 * f;
-    double p[3][3];
-    for (int c = 0; c < 3; ++c)
-        for (int d = 0; d < 3; ++d)
-            p[c][d] = (double)pts[3 * fv[c] + d];
-    double v1[3], v2[3], v3[3];
-    for (int d = 0; d < 3; ++d) {
-        v1[d] = p[1][d] - p[0][d];
-        v2[d] = p[2][d] - p[1][d];
-        v3[d] = p[0][d] - p[2][d];
-    }
-    auto dot = [](const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
-    auto cross = [](const double *a, const double *b, double *c) {
-        c[0] = a[1] * b[2] - a[2] * b[1];
-        c[1] = a[2] * b[0] - a[0] * b[2];
-        c[2] = a[0] * b[1] - a[1] * b[0];
-    };
-    auto norm = [&](const double *a) { return sqrt(dot(a, a)); };
-    // _stable_normals
-    double u1[3], u2[3], u3[3];
-    double n1 = fmax(norm(v1), eps), n2 = fmax(norm(v2), eps), n3 = fmax(norm(v3), eps);
-    for (int d = 0; d < 3; ++d) {
-        u1[d] = v1[d] / n1;
-        u2[d] = v2[d] / n2;
-        u3[d] = v3[d] / n3;
-    }
-    double c31[3], c23[3], c12[3];
-    cross(u3, u1, c31);
-    cross(u2, u3, c23);
-    cross(u1, u2, c12);
-    double m31 = fmax(norm(c31), eps), m23 = fmax(norm(c23), eps), m12 = fmax(norm(c12), eps);
-    double w0 = (1.0 - dot(u3, u1)) / m31;
-    double w1 = (1.0 - dot(u1, u2)) / m12;
-    double w2 = (1.0 - dot(u2, u3)) / m23;
-    double tn[3] = {c12[0] / m12, c12[1] / m12, c12[2] / m12};
-    for (int d = 0; d < 3; ++d) {
-        fnormal[9 * (size_t)f + d] = tn[d] * w0;
-        fnormal[9 * (size_t)f + 3 + d] = tn[d] * w1;
-        fnormal[9 * (size_t)f + 6 + d] = tn[d] * w2;
-    }
-    // _mixed_voronoi_areas
-    double cr[3];
-    cross(v1, v2, cr);
-    double area = fmax(norm(cr), eps);
-    double d31 = dot(v3, v1), d12 = dot(v1, v2), d23 = dot(v2, v3);
-    bool bad0 = d31 > 0.0, bad1 = d12 > 0.0, bad2 = d23 > 0.0;
-    bool bad = bad0 || bad1 || bad2;
-    double l1 = dot(v1, v1), l2 = dot(v2, v2), l3 = dot(v3, v3);
-    double d13 = dot(v1, v3);
-    double wt0 = l3 * d12 + l1 * d23;
-    double wt1 = l2 * d13 + l1 * d23;
-    double wt2 = l2 * d13 + l3 * d12;
-    double a0, a1, a2;
-    if (bad) {
-        a0 = a1 = a2 = 0.125 * area;
-    } else {
-        a0 = -0.125 * wt0 / area;
-        a1 = -0.125 * wt1 / area;
-        a2 = -0.125 * wt2 / area;
-    }
-    a0 += bad0 ? 0.125 * area : 0.0;
-    a1 += bad1 ? 0.125 * area : 0.0;
-    a2 += bad2 ? 0.125 * area : 0.0;
-    farea[3 * (size_t)f] = a0;
-    farea[3 * (size_t)f + 1] = a1;
-    farea[3 * (size_t)f + 2] = a2;
-    // _system_entries (cotangent weights)
-    double e23 = d23 / area, e13 = d13 / area, e12 = d12 / area;
-    double vals[9] = {-0.5 * (e23 + e12), -0.5 * (e13 + e23), -0.5 * (e12 + e13),
-                      0.5 * e23, 0.5 * e23, 0.5 * e13, 0.5 * e13, 0.5 * e12, 0.5 * e12};
-    for (int k = 0; k < 9; ++k)
-        coo[(size_t)k * nf + f] = vals[k];
-}
-
-// Per vertex: normal (sum of incident weighted normals, then normalised) and
-// mixed area (sum of incident contributions, 1 where invalid).  Entries of
-// inc[] are face*3+corner in (corner, face) order.  rotation_feature.py:69-73, :125-128
-__global__ void __launch_bounds__(kBlock)
-k_depth_vertex_terms(const double *__restrict__ fnormal, const double *__restrict__ farea,
-                     const int *__restrict__ inc_off, const int *__restrict__ inc, int n,
-                     double *__restrict__ normals, double *__restrict__ areas)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    const double eps = 2.220446049250313e-16;
-    double nx = 0.0, ny = 0.0, nz = 0.0, a = 0.0;
-    for (int k = inc_off[i]; k < inc_off[i + 1]; ++k) {
-        int fc = inc[k];
-        const double *fn = fnormal + 3 * (size_t)fc;  // [F][3 corners][3]
-        nx += fn[0];
-        ny += fn[1];
-        nz += fn[2];
-        a += farea[fc];
-    }
-    double nr = fmax(sqrt(nx * nx + ny * ny + nz * nz), eps);
-    normals[3 * i] = nx / nr;
-    normals[3 * i + 1] = ny / nr;
-    normals[3 * i + 2] = nz / nr;
-    areas[i] = (!isfinite(a) || a == 0.0) ? 1.0 : a;
-}
-
-// CSR matrix values from per-face COO values: val[k] = coo[perm[k]] for the
-// face entries, alpha·area[i] for the diagonal extra entries (perm >= 9F).
-__global__ void __launch_bounds__(kBlock)
-k_csr_values(const double *__restrict__ coo, const int *__restrict__ perm, int nnz, int nface9,
-             const double *__restrict__ areas, double alpha, double *__restrict__ val)
-{
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nnz)
-        return;
-    int e = perm[k];
-    val[k] = e < nface9 ? coo[e] : alpha * areas[e - nface9];
-}
-
-// Per vertex: Laplacian·points (without the alpha terms), mean curvature,
-// and the matrix diagonal.  rotation_feature.py:205-226, :275-281
-__global__ void __launch_bounds__(kBlock)
-k_depth_rhs_parts(const float *__restrict__ pts, const int *__restrict__ row_off,
-                  const int *__restrict__ cols, const double *__restrict__ val,
-                  const unsigned char *__restrict__ is_alpha, const double *__restrict__ normals,
-                  const double *__restrict__ areas, int n, double *__restrict__ mc,
-                  double *__restrict__ diag)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    double lx = 0.0, ly = 0.0, lz = 0.0, dg = 0.0;
-    for (int k = row_off[i]; k < row_off[i + 1]; ++k) {
-        int j = cols[k];
-        double v = val[k];
-        if (j == i)
-            dg += v;
-        if (is_alpha[k])
-            continue;
-        lx += v * (double)pts[3 * j];
-        ly += v * (double)pts[3 * j + 1];
-        lz += v * (double)pts[3 * j + 2];
-    }
-    lx *= 2.0;
-    ly *= 2.0;
-    lz *= 2.0;
-    double h = 0.5 * (lx * normals[3 * i] + ly * normals[3 * i + 1] + lz * normals[3 * i + 2]) /
-               areas[i];
-    mc[i] = isfinite(h) ? h : 0.0;
-    diag[i] = fmax(dg, 2.220446049250313e-16);
-}
-
-// One colour of SOR: x_i += ω (rhs_i - Σ_j A_ij x_j) / diag_i for the
-// vertices of the colour.  rotation_feature.py:296-311
-__global__ void __launch_bounds__(kBlock)
-k_sor_color(const int *__restrict__ verts, int count, const int *__restrict__ row_off,
-            const int *__restrict__ cols, const double *__restrict__ val,
-            const double *__restrict__ rhs, const double *__restrict__ diag, double omega,
-            double *__restrict__ x)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count)
-        return;
-    int i = verts[t];
-    double s = 0.0;
-    for (int k = row_off[i]; k < row_off[i + 1]; ++k)
-        s += val[k] * x[cols[k]];
-    x[i] = x[i] + omega * (rhs[i] - s) / diag[i];
-}
-
-// r_i = rhs_i - Σ_j A_ij x_j and per-block Σ r², Σ rhs².
-__global__ void __launch_bounds__(kBlock)
-k_residual_norms(const int *__restrict__ row_off, const int *__restrict__ cols,
-                 const double *__restrict__ val, const double *__restrict__ rhs,
-                 const double *__restrict__ x, int n, double *__restrict__ partial)
-{
-    __shared__ double red[kBlock / 64];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double r2 = 0.0, b2 = 0.0;
-    if (i < n) {
-        double s = 0.0;
-        for (int k = row_off[i]; k < row_off[i + 1]; ++k)
-            s += val[k] * x[cols[k]];
-        double r = rhs[i] - s;
-        r2 = r * r;
-        b2 = rhs[i] * rhs[i];
-    }
-    double a = block_sum(r2, red);
-    double b = block_sum(b2, red);
-    if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = a;
-        partial[2 * blockIdx.x + 1] = b;
-    }
-}
-
-// Σ mc·area and Σ area per block (the curvature mean factor).
-__global__ void __launch_bounds__(kBlock)
-k_weighted_sums(const double *__restrict__ mc, const double *__restrict__ areas, int n,
-                double *__restrict__ partial)
-{
-    __shared__ double red[kBlock / 64];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double a = 0.0, b = 0.0;
-    if (i < n) {
-        a = mc[i] * areas[i];
-        b = areas[i];
-    }
-    double ra = block_sum(a, red);
-    double rb = block_sum(b, red);
-    if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = ra;
-        partial[2 * blockIdx.x + 1] = rb;
-    }
-}
-
-// rhs_i = (mc_i - factor)·area_i with factor = Σ mc·a / Σ a from the partials.
-__global__ void __launch_bounds__(kBlock)
-k_depth_rhs(const double *__restrict__ mc, const double *__restrict__ areas, int n,
-            const double *__restrict__ partial, int nblocks, double *__restrict__ rhs)
-{
-    __shared__ double factor;
-    if (threadIdx.x == 0) {
-        double a = 0.0, b = 0.0;
-        for (int k = 0; k < nblocks; ++k) {
-            a += partial[2 * k];
-            b += partial[2 * k + 1];
-        }
-        factor = a / b;
-    }
-    __syncthreads();
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        rhs[i] = (mc[i] - factor) * areas[i];
-}
-
-__global__ void k_residual_final(const double *__restrict__ partial, int nblocks,
-                                 double *__restrict__ out)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0)
-        return;
-    double a = 0.0, b = 0.0;
-    for (int k = 0; k < nblocks; ++k) {
-        a += partial[2 * k];
-        b += partial[2 * k + 1];
-    }
-    out[0] = sqrt(a) / fmax(sqrt(b), 2.220446049250313e-16);
-}
-
 // ------------------------------------------------------------------- warp
 
 // Sphere points through a flow.  surface_warp.py:491-584
@@ -756,74 +498,6 @@ int launch_sheet_gather(const double *vals, size_t vstride, const int *idx, cons
     return kOk;
 }
 
-int launch_depth_face_terms(const float *pts, const int *faces, int nf, double *fnormal,
-                            double *farea, double *coo, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_depth_face_terms, dim3(grid_for(nf)), dim3(kBlock), 0, s, pts, faces,
-                       nf, fnormal, farea, coo);
-    CSG_LAUNCH_CHECK();
-    return kOk;
-}
-
-int launch_depth_vertex_terms(const double *fnormal, const double *farea, const int *inc_off,
-                              const int *inc, int n, double *normals, double *areas,
-                              hipStream_t s)
-{
-    hipLaunchKernelGGL(k_depth_vertex_terms, dim3(grid_for(n)), dim3(kBlock), 0, s, fnormal,
-                       farea, inc_off, inc, n, normals, areas);
-    CSG_LAUNCH_CHECK();
-    return kOk;
-}
-
-int launch_csr_values(const double *coo, const int *perm, int nnz, int nface9,
-                      const double *areas, double alpha, double *val, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_csr_values, dim3(grid_for(nnz)), dim3(kBlock), 0, s, coo, perm, nnz,
-                       nface9, areas, alpha, val);
-    CSG_LAUNCH_CHECK();
-    return kOk;
-}
-
-int launch_depth_rhs(const float *pts, const int *row_off, const int *cols, const double *val,
-                     const unsigned char *is_alpha, const double *normals, const double *areas,
-                     int n, double *mc, double *diag, double *partial, double *rhs,
-                     hipStream_t s)
-{
-    int nb = (int)grid_for(n);
-    hipLaunchKernelGGL(k_depth_rhs_parts, dim3(nb), dim3(kBlock), 0, s, pts, row_off, cols, val,
-                       is_alpha, normals, areas, n, mc, diag);
-    CSG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_weighted_sums, dim3(nb), dim3(kBlock), 0, s, mc, areas, n, partial);
-    CSG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_depth_rhs, dim3(nb), dim3(kBlock), 0, s, mc, areas, n, partial, nb, rhs);
-    CSG_LAUNCH_CHECK();
-    return kOk;
-}
-
-int launch_sor_color(const int *verts, int count, const int *row_off, const int *cols,
-                     const double *val, const double *rhs, const double *diag, double omega,
-                     double *x, hipStream_t s)
-{
-    if (count <= 0)
-        return kOk;
-    hipLaunchKernelGGL(k_sor_color, dim3(grid_for(count)), dim3(kBlock), 0, s, verts, count,
-                       row_off, cols, val, rhs, diag, omega, x);
-    CSG_LAUNCH_CHECK();
-    return kOk;
-}
-
-int launch_relative_residual(const int *row_off, const int *cols, const double *val,
-                             const double *rhs, const double *x, int n, double *partial,
-                             double *out, hipStream_t s)
-{
-    int nb = (int)grid_for(n);
-    hipLaunchKernelGGL(k_residual_norms, dim3(nb), dim3(kBlock), 0, s, row_off, cols, val, rhs, x,
-                       n, partial);
-    CSG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, s, partial, nb, out);
-    CSG_LAUNCH_CHECK();
-    return kOk;
-}
 
 template <typename T>
 int launch_apply_flow(const float *unit, int n, const T *flow, int nx, int ny, int inverse,

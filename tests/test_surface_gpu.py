@@ -86,6 +86,36 @@ def test_rotation_feature_matches_golden(golden, cuda):
     np.testing.assert_allclose(_np(res.values), g["feature"], atol=1e-6)
 
 
+def test_depth_potential_batch_matches_single(golden, cuda):
+    """Both sides solved in one persistent launch give exactly the per-side values."""
+
+    import cat_surface_gpu as csg
+    from cat_surface_gpu.rotation_feature import (compute_rotation_feature,
+                                                  compute_rotation_features, depth_potential)
+
+    g = golden("rotation_feature")
+    n = g["verts"].shape[0]
+    st = csg.SurfaceStencil(
+        sphere_points=g["verts"], faces=g["faces"], surface_indices=g["surf_idx"],
+        surface_weights=g["surf_w"], sheet_indices=np.zeros((1, 1, 3), np.int32),
+        sheet_weights=np.zeros((1, 1, 3)), nx=1, ny=1, source_points=n).to(cuda)
+    a = torch.as_tensor(g["smoothed"], dtype=torch.float32, device=cuda)
+    rng = np.random.default_rng(3)
+    b = a * 1.1 + torch.as_tensor(rng.normal(0.0, 0.01, a.shape), dtype=torch.float32,
+                                  device=cuda)
+    xa, _, ra = depth_potential(st, a)
+    xb, _, rb = depth_potential(st, b)
+    xs, _, rs = depth_potential(st, torch.stack([a, b]))
+    assert torch.equal(xs[0], xa) and torch.equal(xs[1], xb)
+    assert float(rs[0]) == float(ra) and float(rs[1]) == float(rb)
+    np.testing.assert_allclose(_np(xa), g["depth"], atol=1e-8 * np.abs(g["depth"]).max())
+    surf = torch.as_tensor(g["surface"], device=cuda)
+    pair = compute_rotation_features((st, st), (surf, surf), heat_fwhms=(15.0, 10.0))
+    solo = compute_rotation_feature(st, surf, heat_fwhm=10.0)
+    assert torch.equal(pair[1].values, solo.values)
+    np.testing.assert_allclose(_np(pair[0].values), g["feature"], atol=1e-6)
+
+
 def test_rotation_index_costs_and_search(golden, cuda):
     import cat_surface_gpu as csg
 
