@@ -102,8 +102,8 @@ class Topology:
 
 def topology_for(faces: np.ndarray, n_points: int, device: torch.device) -> Topology:
     f = np.ascontiguousarray(np.asarray(faces, dtype=np.int32))
-    key = (device.type, device.index, n_points, f.shape, f.tobytes())
-    cached = _TOPOLOGY_CACHE.get(key)
+    cache_key = (device.type, device.index, n_points, f.shape, f.tobytes())
+    cached = _TOPOLOGY_CACHE.get(cache_key)
     if cached is not None:
         return cached
     nbr_off, nbr = build_neighbour_csr(f, n_points)
@@ -119,9 +119,9 @@ def topology_for(faces: np.ndarray, n_points: int, device: torch.device) -> Topo
     i0, i1, i2 = (f[:, k].astype(np.int64) for k in range(3))
     rows = np.concatenate([i0, i1, i2, i0, i1, i1, i2, i0, i2, np.arange(n_points)])
     cols = np.concatenate([i0, i1, i2, i1, i0, i2, i1, i2, i0, np.arange(n_points)])
-    key = rows * n_points + cols
-    perm = np.argsort(key, kind="stable")
-    uniq, counts = np.unique(key[perm], return_counts=True)
+    flat = rows * n_points + cols
+    perm = np.argsort(flat, kind="stable")
+    uniq, counts = np.unique(flat[perm], return_counts=True)
     m_rows = uniq // n_points
     m_cols = uniq % n_points
     row_off = np.zeros(n_points + 1, np.int32)
@@ -148,7 +148,7 @@ def topology_for(faces: np.ndarray, n_points: int, device: torch.device) -> Topo
         colour_off_dev=dev(colour_off),
         faces_np=f,
     )
-    _TOPOLOGY_CACHE[key] = topo
+    _TOPOLOGY_CACHE[cache_key] = topo
     return topo
 
 

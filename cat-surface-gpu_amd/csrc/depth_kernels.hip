@@ -236,7 +236,7 @@ k_depth_rhs(const double *__restrict__ mc, const double *__restrict__ areas, int
 // Grid-wide barrier for a co-resident (cooperative) grid.  Returns false if a
 // waiting block gave up after kSpinLimit polls (the caller then exits, so
 // every wave terminates even if the barrier were ever broken).
-constexpr long kSpinLimit = 1L << 26;
+constexpr long kSpinLimit = 1L << 22;
 
 __device__ bool grid_barrier(unsigned *bar, unsigned nblocks, int *timeout)
 {
