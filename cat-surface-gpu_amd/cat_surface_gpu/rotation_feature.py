@@ -52,13 +52,15 @@ def depth_potential(stencil: SurfaceStencilDevice, smoothed_surface, *,
     x = torch.empty(batch, n, dtype=torch.float64, device=pts.device)
     rel = torch.empty(batch, dtype=torch.float64, device=pts.device)
     lib = _lib.lib()
-    ws = _lib.workspace(lib.csg_depth_workspace_bytes(n, nf, nnz, batch), pts.device)
+    ws = _lib.workspace(lib.csg_depth_workspace_bytes(n, nf, nnz, batch, topo.ell_width),
+                        pts.device)
     coff = (ctypes.c_int * topo.colour_off.size)(*topo.colour_off.tolist())
     _lib.check(lib.csg_depth_potential(
         _lib.ptr(pts), batch, n, _lib.ptr(topo.faces), nf, _lib.ptr(topo.inc_off),
         _lib.ptr(topo.inc), _lib.ptr(topo.sys_row_off), _lib.ptr(topo.sys_cols),
         _lib.ptr(topo.sys_contrib_off), _lib.ptr(topo.sys_contrib), nnz, coff,
-        _lib.ptr(topo.colour_verts), int(topo.colour_off.size - 1), float(alpha), int(max_iter),
+        _lib.ptr(topo.colour_verts), int(topo.colour_off.size - 1), _lib.ptr(topo.ell_cols),
+        _lib.ptr(topo.ell_src), int(topo.ell_width), float(alpha), int(max_iter),
         float(relaxation), _lib.ptr(x), _lib.ptr(rel), _lib.ptr(ws), ws.numel(),
         _lib.stream_handle(pts.device)),
         "csg_depth_potential")

@@ -94,6 +94,11 @@ class Topology:
     sys_contrib: torch.Tensor      # COO entry indices, COO order within an entry
     colour_off_dev: torch.Tensor = None
     faces_np: np.ndarray = field(repr=False, default=None)
+    # the same system as ELL in colour order: slot k of colour-ordered row r is
+    # at k * n + r; ell_src names its merged CSR entry (-1 = padding)
+    ell_width: int = 0
+    ell_cols: torch.Tensor = None
+    ell_src: torch.Tensor = None
 
     @property
     def nnz(self) -> int:
@@ -129,6 +134,17 @@ def topology_for(faces: np.ndarray, n_points: int, device: torch.device) -> Topo
     contrib_off = np.zeros(uniq.size + 1, np.int32)
     contrib_off[1:] = np.cumsum(counts)
 
+    # colour-ordered ELL: row r of the solve order is vertex order[r]; its slots
+    # hold the merged CSR entries of that row in CSR (column) order
+    lengths = np.diff(row_off)
+    width = int(lengths.max()) if lengths.size else 0
+    r_len = lengths[order]
+    r_start = row_off[:-1][order]
+    slot = np.arange(width)[:, None]
+    valid = slot < r_len[None, :]
+    ell_src = np.where(valid, r_start[None, :] + slot, -1).astype(np.int32)
+    ell_cols = np.where(valid, m_cols[np.clip(ell_src, 0, None)], order[None, :]).astype(np.int32)
+
     def dev(a, dtype=torch.int32):
         return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
 
@@ -147,6 +163,9 @@ def topology_for(faces: np.ndarray, n_points: int, device: torch.device) -> Topo
         sys_contrib=dev(perm.astype(np.int32)),
         colour_off_dev=dev(colour_off),
         faces_np=f,
+        ell_width=width,
+        ell_cols=dev(ell_cols.reshape(-1)),
+        ell_src=dev(ell_src.reshape(-1)),
     )
     _TOPOLOGY_CACHE[cache_key] = topo
     return topo

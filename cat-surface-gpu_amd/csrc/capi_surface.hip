@@ -241,7 +241,7 @@ int csg_type5_sheets(const float *geom, int n, const int *faces, const int *nbr_
 
 namespace {
 
-DepthArgs carve_depth(Carve &c, int batch, int n, int nf, int nnz)
+DepthArgs carve_depth(Carve &c, int batch, int n, int nf, int nnz, int width)
 {
     DepthArgs a{};
     a.fnormal = c.take<double>((size_t)batch * 9 * nf);
@@ -255,16 +255,19 @@ DepthArgs carve_depth(Carve &c, int batch, int n, int nf, int nnz)
     a.diag = c.take<double>((size_t)batch * n);
     a.rhs = c.take<double>((size_t)batch * n);
     a.partial = c.take<double>((size_t)batch * 2 * grid_for(n) + 64);
-    a.barrier = c.take<unsigned>(4);
+    a.ell_val = c.take<double>((size_t)batch * width * n);
+    a.rhs_c = c.take<double>((size_t)batch * n);
+    a.diag_c = c.take<double>((size_t)batch * n);
+    a.barrier = c.take<unsigned>((size_t)kBarrierSlots * kBarrierSlotWords);
     return a;
 }
 
 }  // namespace
 
-size_t csg_depth_workspace_bytes(int n, int nf, int nnz, int batch)
+size_t csg_depth_workspace_bytes(int n, int nf, int nnz, int batch, int ell_width)
 {
     Carve c{nullptr, 0};
-    carve_depth(c, batch, n, nf, nnz);
+    carve_depth(c, batch, n, nf, nnz, ell_width);
     return c.used;
 }
 
@@ -275,19 +278,21 @@ int csg_depth_potential(const float *pts, int batch, int n, const int *faces, in
                         const int *inc_off, const int *inc, const int *row_off, const int *cols,
                         const int *contrib_off, const int *contrib, int nnz,
                         const int *colour_off, const int *colour_verts, int ncolours,
-                        double alpha, int max_iter, double omega, double *x, double *relres,
-                        void *ws, size_t ws_bytes, void *stream)
+                        const int *ell_cols, const int *ell_src, int ell_width, double alpha,
+                        int max_iter, double omega, double *x, double *relres, void *ws,
+                        size_t ws_bytes, void *stream)
 {
-    if (n < 1 || nf < 1 || batch < 1 || max_iter < 0 || colour_off == nullptr)
+    if (n < 1 || nf < 1 || batch < 1 || max_iter < 0 || colour_off == nullptr ||
+        ell_width < 1)
         return fail(kInvalidArgument, "invalid depth-potential arguments");
     if (ncolours < 1 || ncolours > kMaxColours)
         return fail(kInvalidArgument, "colour count must be in [1, %d], got %d", kMaxColours,
                     ncolours);
-    if (ws_bytes < csg_depth_workspace_bytes(n, nf, nnz, batch))
+    if (ws_bytes < csg_depth_workspace_bytes(n, nf, nnz, batch, ell_width))
         return fail(kInvalidArgument, "depth-potential workspace too small");
     hipStream_t s = S(stream);
     Carve c{(char *)ws, 0};
-    DepthArgs a = carve_depth(c, batch, n, nf, nnz);
+    DepthArgs a = carve_depth(c, batch, n, nf, nnz, ell_width);
     a.pts = pts;
     a.faces = faces;
     a.inc_off = inc_off;
@@ -297,6 +302,9 @@ int csg_depth_potential(const float *pts, int batch, int n, const int *faces, in
     a.contrib_off = contrib_off;
     a.contrib = contrib;
     a.colour_verts = colour_verts;
+    a.ell_cols = ell_cols;
+    a.ell_src = ell_src;
+    a.ell_width = ell_width;
     a.batch = batch;
     a.n = n;
     a.nf = nf;
@@ -312,7 +320,8 @@ int csg_depth_potential(const float *pts, int batch, int n, const int *faces, in
     if (relres)
         CSG_TRY(depth_residual(a, relres, s));
     int timeout = 0;
-    CSG_HIP_CHECK(hipMemcpyAsync(&timeout, a.barrier + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+    const unsigned *flag = a.barrier + (size_t)(kBarrierSlots - 1) * kBarrierSlotWords;
+    CSG_HIP_CHECK(hipMemcpyAsync(&timeout, flag, sizeof(int), hipMemcpyDeviceToHost, s));
     CSG_HIP_CHECK(hipStreamSynchronize(s));
     if (timeout)
         return fail(kRuntimeError, "depth-potential SOR grid barrier timed out");

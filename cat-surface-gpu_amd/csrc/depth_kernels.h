@@ -5,7 +5,10 @@
 
 namespace csg {
 
-constexpr int kSorBlock = 256;
+constexpr int kSorBlock = 512;
+// grid-barrier state: kBarrierSlots slots of kBarrierSlotWords words (one cache line each)
+constexpr int kBarrierSlots = 128;
+constexpr int kBarrierSlotWords = 32;
 constexpr int kMaxColours = 32;
 
 // Colour groups: vertices of colour c are colour_verts[off[c] .. off[c+1]).
@@ -21,11 +24,14 @@ struct DepthArgs {
     const int *row_off, *cols;          // merged CSR, nnz entries
     const int *contrib_off, *contrib;   // per merged entry, its COO entries
     const int *colour_verts;
+    const int *ell_cols, *ell_src;      // colour-ordered ELL, ell_width slots per row
+    int ell_width;
     int batch, n, nf, nnz;
     double alpha;
     // workspace
     double *fnormal, *farea, *coo, *normals, *areas, *lap, *sys, *mc, *diag, *rhs, *partial;
-    unsigned *barrier;  // 2 counters + 1 timeout flag
+    double *ell_val, *rhs_c, *diag_c;   // [batch][ell_width * n], [batch][n] in colour order
+    unsigned *barrier;  // kBarrierSlots x kBarrierSlotWords; the last slot holds the timeout flag
     double *x;          // [batch][n]
 };
 

@@ -233,27 +233,70 @@ k_depth_rhs(const double *__restrict__ mc, const double *__restrict__ areas, int
     }
 }
 
-// Grid-wide barrier for a co-resident (cooperative) grid.  Returns false if a
-// waiting block gave up after kSpinLimit polls (the caller then exits, so
-// every wave terminates even if the barrier were ever broken).
-constexpr long kSpinLimit = 1L << 22;
+// Colour-ordered copies of the system for the sweeps: ELL values, rhs and
+// diagonal indexed by solve-order position r (vertex colour_verts[r]).
+__global__ void __launch_bounds__(kBlock)
+k_colour_order(const int *__restrict__ verts, const int *__restrict__ ell_src, int W, int n,
+               const double *__restrict__ sys, int nnz, const double *__restrict__ rhs,
+               const double *__restrict__ diag, double *__restrict__ ell_val,
+               double *__restrict__ rhs_c, double *__restrict__ diag_c)
+{
+    const size_t total = (size_t)W * n + n;
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= total)
+        return;
+    const int b = blockIdx.y;
+    if (q < (size_t)W * n) {
+        int e = ell_src[q];
+        ell_val[(size_t)b * W * n + q] = e >= 0 ? sys[(size_t)b * nnz + e] : 0.0;
+    } else {
+        int r = (int)(q - (size_t)W * n);
+        int i = verts[r];
+        rhs_c[(size_t)b * n + r] = rhs[(size_t)b * n + i];
+        diag_c[(size_t)b * n + r] = diag[(size_t)b * n + i];
+    }
+}
 
-__device__ bool grid_barrier(unsigned *bar, unsigned nblocks, int *timeout)
+// Grid-wide barrier for a co-resident (cooperative) grid, as a two-level
+// tree: blocks arrive in groups of kBarrierGroup on their own counter, the
+// last arrival of each group arrives at the root, and the last root arrival
+// advances the generation word that everyone waits on.  Arrivals on one
+// counter serialise, so the tree keeps the critical path to about
+// 2·kBarrierGroup atomic round trips.  Only thread 0 of a block touches the
+// counters; its agent-scope acquire/release (after the block's
+// __syncthreads) publishes the block's stores and invalidates stale cache
+// lines for the whole block.  Returns false if the block gave up after
+// kSpinLimit polls; the caller then exits, so every wave terminates.
+constexpr long kSpinLimit = 1L << 22;
+constexpr int kBarrierGroup = 8;
+
+__device__ bool grid_barrier(unsigned *bar, int nblocks, int *timeout)
 {
     __shared__ int ok;
-    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
         ok = 1;
-        unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned arrived =
-            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblocks - 1) {
-            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
+        const int groups = (nblocks + kBarrierGroup - 1) / kBarrierGroup;
+        unsigned *root = bar + (size_t)groups * kBarrierSlotWords;
+        unsigned *gen_p = root + kBarrierSlotWords;
+        const unsigned gen = __hip_atomic_load(gen_p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const int g = blockIdx.x / kBarrierGroup;
+        const int gsize = min(kBarrierGroup, nblocks - g * kBarrierGroup);
+        unsigned *cnt = bar + (size_t)g * kBarrierSlotWords;
+        bool released = false;
+        unsigned a = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (a == (unsigned)gsize - 1) {
+            __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned r = __hip_atomic_fetch_add(root, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (r == (unsigned)groups - 1) {
+                __hip_atomic_store(root, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(gen_p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                released = true;
+            }
+        }
+        if (!released) {
             long spins = 0;
-            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            while (__hip_atomic_load(gen_p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 if (++spins > kSpinLimit) {
                     __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     ok = 0;
@@ -262,7 +305,6 @@ __device__ bool grid_barrier(unsigned *bar, unsigned nblocks, int *timeout)
                 __builtin_amdgcn_s_sleep(1);
             }
         }
-        __threadfence();
     }
     __syncthreads();
     return ok != 0;
@@ -270,29 +312,32 @@ __device__ bool grid_barrier(unsigned *bar, unsigned nblocks, int *timeout)
 
 // All SOR sweeps: for it < max_iter, for each colour, every vertex of the
 // colour (in every batch mesh) takes x_i += ω (rhs_i - Σ_j A_ij x_j) / diag_i;
-// a grid barrier separates colours.  rotation_feature.py:296-311
+// a grid barrier separates colours.  Row data are read in solve order from
+// the colour-ordered ELL copy, so a sweep's only dependent loads are the x
+// gathers.  rotation_feature.py:296-311
 __global__ void __launch_bounds__(kSorBlock)
-k_sor_persistent(const int *__restrict__ verts, SorColours co, const int *__restrict__ row_off,
-                 const int *__restrict__ cols, const double *__restrict__ val,
-                 const double *__restrict__ rhs, const double *__restrict__ diag, double omega,
-                 int n, int nnz, int batch, int max_iter, double *__restrict__ x, unsigned *bar,
-                 int *timeout)
+k_sor_persistent(const int *__restrict__ verts, SorColours co, const int *__restrict__ ell_cols,
+                 const double *__restrict__ ell_val, const double *__restrict__ rhs_c,
+                 const double *__restrict__ diag_c, int W, double omega, int n, int batch,
+                 int max_iter, double *x, unsigned *bar, int *timeout)
 {
     const int nthreads = gridDim.x * blockDim.x;
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t wn = (size_t)W * n;
     for (int it = 0; it < max_iter; ++it) {
         for (int c = 0; c < co.count; ++c) {
             const int start = co.off[c], cnt = co.off[c + 1] - start;
             for (int t = tid; t < cnt * batch; t += nthreads) {
                 const int b = t / cnt;
-                const int i = verts[start + (t - b * cnt)];
-                const double *vb = val + (size_t)b * nnz;
+                const int r = start + (t - b * cnt);
+                const int i = verts[r];
+                const double *vb = ell_val + (size_t)b * wn;
                 double *xb = x + (size_t)b * n;
                 double s = 0.0;
-                for (int k = row_off[i]; k < row_off[i + 1]; ++k)
-                    s += vb[k] * xb[cols[k]];
-                size_t o = (size_t)b * n + i;
-                xb[i] = xb[i] + omega * (rhs[o] - s) / diag[o];
+                for (int k = 0; k < W; ++k)
+                    s += vb[(size_t)k * n + r] * xb[ell_cols[(size_t)k * n + r]];
+                const size_t o = (size_t)b * n + r;
+                xb[i] = xb[i] + omega * (rhs_c[o] - s) / diag_c[o];
             }
             if (!grid_barrier(bar, gridDim.x, timeout))
                 return;
@@ -374,6 +419,14 @@ int depth_setup(const DepthArgs &a, hipStream_t s)
 int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double omega,
               hipStream_t s)
 {
+    const int W = a.ell_width;
+    const size_t total = (size_t)W * a.n + a.n;
+    hipLaunchKernelGGL(k_colour_order, dim3(grid_for(total), a.batch), dim3(kBlock), 0, s,
+                       a.colour_verts, a.ell_src, W, a.n, a.sys, a.nnz, a.rhs, a.diag, a.ell_val,
+                       a.rhs_c, a.diag_c);
+    CSG_LAUNCH_CHECK();
+    CSG_HIP_CHECK(hipMemsetAsync(a.barrier, 0,
+                                 sizeof(unsigned) * kBarrierSlots * kBarrierSlotWords, s));
     if (max_iter <= 0)
         return kOk;
     int dev = 0;
@@ -387,22 +440,24 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
         most = (co.off[c + 1] - co.off[c]) > most ? (co.off[c + 1] - co.off[c]) : most;
     int want = (int)grid_for((size_t)most * a.batch, kSorBlock);
     int cap = cus * (per_cu > 0 ? per_cu : 1);
+    // the barrier state holds one slot per group, the root and the generation word
+    int slots_cap = (kBarrierSlots - 3) * kBarrierGroup;
+    cap = cap < slots_cap ? cap : slots_cap;
     int grid = want < cap ? want : cap;
     if (grid < 1)
         grid = 1;
-    CSG_HIP_CHECK(hipMemsetAsync(a.barrier, 0, 2 * sizeof(unsigned) + sizeof(int), s));
-    int *timeout = reinterpret_cast<int *>(a.barrier + 2);
+    int *timeout = reinterpret_cast<int *>(a.barrier + (size_t)(kBarrierSlots - 1) * kBarrierSlotWords);
     const int *verts = a.colour_verts;
-    const int *row_off = a.row_off, *cols = a.cols;
-    const double *val = a.sys, *rhs = a.rhs, *diag = a.diag;
-    int n = a.n, nnz = a.nnz, batch = a.batch;
+    const int *ell_cols = a.ell_cols;
+    const double *ell_val = a.ell_val, *rhs_c = a.rhs_c, *diag_c = a.diag_c;
+    int n = a.n, batch = a.batch, width = W;
     double *x = a.x;
     unsigned *bar = a.barrier;
     SorColours colours = co;
-    void *args[] = {(void *)&verts, (void *)&colours, (void *)&row_off, (void *)&cols,
-                    (void *)&val,   (void *)&rhs,     (void *)&diag,    (void *)&omega,
-                    (void *)&n,     (void *)&nnz,     (void *)&batch,   (void *)&max_iter,
-                    (void *)&x,     (void *)&bar,     (void *)&timeout};
+    void *args[] = {(void *)&verts,  (void *)&colours, (void *)&ell_cols, (void *)&ell_val,
+                    (void *)&rhs_c,  (void *)&diag_c,  (void *)&width,    (void *)&omega,
+                    (void *)&n,      (void *)&batch,   (void *)&max_iter, (void *)&x,
+                    (void *)&bar,    (void *)&timeout};
     CSG_HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_sor_persistent, dim3(grid),
                                              dim3(kSorBlock), args, 0, s));
     return kOk;

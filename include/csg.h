@@ -194,17 +194,20 @@ int csg_type5_sheets(const float *geom, int n, const int *faces, const int *nbr_
  * meshes of one topology (pts [batch][n][3], x and relres [batch]).  The system is a
  * CSR with duplicate (row, col) entries merged (contrib_off / contrib list each merged
  * entry's COO contributions); colour_off is a host array of ncolours + 1 offsets into
- * the device array colour_verts.  All max_iter x ncolours sweeps run in one persistent
+ * the device array colour_verts.  ell_cols / ell_src ([ell_width][n], solve order: row r
+ * is vertex colour_verts[r]) give each row's merged entries (column, merged index or -1
+ * for padding) in CSR order.  All max_iter x ncolours sweeps run in one persistent
  * cooperative launch; the call synchronises once at the end.
  * Replaces rotation_feature._solve_depth_potential_colored_sor
  * (src/cat_surface_gpu/rotation_feature.py:253-317). */
-size_t csg_depth_workspace_bytes(int n, int nf, int nnz, int batch);
+size_t csg_depth_workspace_bytes(int n, int nf, int nnz, int batch, int ell_width);
 int csg_depth_potential(const float *pts, int batch, int n, const int *faces, int nf,
                         const int *inc_off, const int *inc, const int *row_off, const int *cols,
                         const int *contrib_off, const int *contrib, int nnz,
                         const int *colour_off, const int *colour_verts, int ncolours,
-                        double alpha, int max_iter, double omega, double *x, double *relres,
-                        void *ws, size_t ws_bytes, void *stream);
+                        const int *ell_cols, const int *ell_src, int ell_width, double alpha,
+                        int max_iter, double omega, double *x, double *relres, void *ws,
+                        size_t ws_bytes, void *stream);
 
 /* Sphere points through a flow (out: [n][3] of dtype).  Replaces
  * surface_warp.apply_flow_to_sphere (src/cat_surface_gpu/surface_warp.py:500-584). */

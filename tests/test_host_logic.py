@@ -127,6 +127,17 @@ def test_depth_system_merge_equals_dense_assembly():
     cverts = topo.colour_verts.numpy()
     assert sorted(cverts.tolist()) == list(range(n))
     assert topo.colour_off[0] == 0 and topo.colour_off[-1] == n
+    # the colour-ordered ELL holds each row's merged entries in CSR order
+    w = topo.ell_width
+    ecols = topo.ell_cols.numpy().reshape(w, n)
+    esrc = topo.ell_src.numpy().reshape(w, n)
+    for r, i in enumerate(cverts):
+        want = list(range(row_off[i], row_off[i + 1]))
+        got = [int(e) for e in esrc[:, r] if e >= 0]
+        assert got == want
+        assert (esrc[len(want):, r] == -1).all()
+        np.testing.assert_array_equal(ecols[:len(want), r], mcols[want])
+        assert (ecols[len(want):, r] == i).all()
 
 
 def test_stencil_file_round_trip(tmp_path):
