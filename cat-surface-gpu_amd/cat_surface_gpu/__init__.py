@@ -15,6 +15,7 @@ from .dartel_grid import (
     dartel_step,
     expdef,
     fmg2_membrane,
+    set_fmg_schedule,
     from_c_layout,
     initialise_objfun,
     jac_div_smalldef,

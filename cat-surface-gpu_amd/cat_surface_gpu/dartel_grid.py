@@ -396,6 +396,17 @@ def resize_field(field, target: GridSpec, *, kernel: str = "auto",
     return out
 
 
+def set_fmg_schedule(fused: bool) -> bool:
+    """Choose the multigrid schedule of every later FMG call; returns the previous one.
+
+    ``True`` (the default) relaxes the fine levels on LDS tiles and the coarse
+    levels in one workgroup per job; ``False`` launches one kernel per reference
+    operator call (dartel_grid.py:1211-1417).  Both compute identical values.
+    """
+
+    return bool(_lib.lib().csg_set_fmg_schedule(1 if fused else 0))
+
+
 def fmg2_membrane(hessian, right_hand, params: Sequence[float], cycles: int = 3, nit: int = 3,
                   initial=None, *, kernel: str = "auto", device: str | torch.device = "auto",
                   dtype: torch.dtype = torch.float64,

@@ -82,6 +82,11 @@ int csg_device_count(int *count)
     return kOk;
 }
 
+int csg_set_fmg_schedule(int fused)
+{
+    return set_fused_fmg_schedule(fused);
+}
+
 int csg_sample_field(const void *field, int nx, int ny, const void *x, const void *y, size_t n,
                      int dtype, void *out, void *stream)
 {

@@ -92,4 +92,47 @@ template <typename T>
 int launch_any_nonzero(const T *u, size_t n, int batch, size_t stride, int *flags,
                        hipStream_t s);
 
+// ---------------------------------------------------------------- fused FMG schedule
+
+constexpr int kFmgMaxLevels = 40;
+constexpr int kTileX = 32, kTileY = 16;  // interior of a fine-level relaxation tile
+constexpr int kTileThreads = 256;
+constexpr int kCoarseThreads = 512;
+
+// Levels c0 .. L-1 of the pyramid, relaxed by one workgroup per job in LDS.
+template <typename T>
+struct CoarseLevels {
+    int L, c0;
+    int nx[kFmgMaxLevels], ny[kFmgMaxLevels];
+    T w00[kFmgMaxLevels], w01[kFmgMaxLevels], w10[kFmgMaxLevels];
+};
+
+// Largest dynamic LDS allocation a block may use on the current device (the
+// kernels' limits are raised to it once per process).
+size_t fmg_lds_budget();
+template <typename T>
+size_t tile_lds_bytes(int sweeps);
+template <typename T>
+size_t coarse_lds_bytes(const int *nx, const int *ny, int c0, int L);
+
+// `sweeps` red-black half-sweeps of one fine level on LDS tiles with a halo of
+// `sweeps` pixels (even nx only); u_out = relaxed(u_in [+ prolong(uc)]).
+template <typename T>
+int launch_relax_tile(const T *a, const T *b, const T *uin, T *uout, const T *uc, int cnx,
+                      int cny, MembraneWeights w, int nx, int ny, int batch, size_t astride,
+                      size_t bstride, size_t cstride, int sweeps, hipStream_t s);
+// bc = restrict(b - (L + A) u); uc = 0.
+template <typename T>
+int launch_restrict_residual(const T *a, const T *b, const T *u, MembraneWeights w, int nx,
+                             int ny, int cnx, int cny, int batch, size_t astride, size_t bstride,
+                             size_t cstride, T *bc, T *uc, hipStream_t s);
+// Levels c0.. of the FMG in one workgroup per job: init != 0 runs the coarse
+// initialisation (coarse solve, then prolong + `cycles` cycles per level down
+// to c0) from b = B0[c0]; otherwise `cycles` cycles from level c0 with b = B[c0]
+// and u as the initial guess.  u (level c0) is overwritten.
+template <typename T>
+int launch_fmg_coarse(const CoarseLevels<T> &lv, const T *a, const T *b, T *u, size_t astride,
+                      size_t bstride, double id, int nit, int cycles, int init, int batch,
+                      size_t lds_bytes, hipStream_t s);
+
 }  // namespace csg

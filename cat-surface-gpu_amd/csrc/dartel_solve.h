@@ -14,6 +14,11 @@ size_t fmg_workspace_bytes(int nx, int ny, int batch);
 
 // coarse_init != 0 runs the reference's coarse-initialisation branch
 // (no or all-zero initial guess), else `cycles` V-cycles from `initial`.
+// Selects the multigrid schedule for later calls: 1 = fused (LDS tiles and a
+// single-workgroup coarse hierarchy, where it applies), 0 = one launch per
+// operator call.  Both give identical values.  Returns the previous setting.
+int set_fused_fmg_schedule(int enable);
+
 template <typename T>
 int fmg_run(const T *hes, const T *rhs, const double *params, int cycles, int nit,
             const T *initial, int coarse_init, int nx, int ny, int batch, T *out, void *ws,

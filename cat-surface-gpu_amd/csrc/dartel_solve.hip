@@ -8,6 +8,7 @@
 // src/cat_surface_gpu/surface_warp.py:76-107, :239-367.
 #include "dartel_solve.h"
 
+#include <atomic>
 #include <vector>
 
 namespace csg {
@@ -100,8 +101,10 @@ struct FmgPlan {
     int L;
     int nx[kMaxLevels], ny[kMaxLevels];
     T *A[kMaxLevels], *B0[kMaxLevels], *B[kMaxLevels], *U[kMaxLevels], *R[kMaxLevels];
+    T *U2[kMaxLevels];  // second correction buffer of the fused schedule (aliases R)
     size_t P(int l) const { return (size_t)nx[l] * ny[l]; }
 };
+static_assert(kMaxLevels == kFmgMaxLevels, "one level limit");
 
 template <typename T>
 static void carve_fmg(Carver &c, int nx, int ny, int batch, FmgPlan<T> &plan)
@@ -110,6 +113,7 @@ static void carve_fmg(Carver &c, int nx, int ny, int batch, FmgPlan<T> &plan)
     for (int l = 0; l < plan.L; ++l) {
         size_t P = plan.P(l);
         plan.R[l] = c.template take<T>(2 * P * batch);
+        plan.U2[l] = plan.R[l];  // the fused schedule needs no residual buffer
         if (l == 0) {
             plan.A[0] = plan.B0[0] = plan.B[0] = plan.U[0] = nullptr;
             continue;
@@ -184,7 +188,111 @@ struct FmgRun {
     }
 };
 
+// The fused schedule: fine levels on LDS tiles, levels c0.. in one
+// workgroup per job (fmg_fused.hip).  Same operations, same order.
+template <typename T>
+struct FusedFmg {
+    FmgPlan<T> &p;
+    const double *params;
+    int nit, batch, c0;
+    hipStream_t s;
+    int cur[kMaxLevels] = {};
+    CoarseLevels<T> lv;
+    size_t lds = 0;
+
+    T *ubuf(int l, int which) { return which ? p.U2[l] : p.U[l]; }
+    T *u(int l) { return ubuf(l, cur[l]); }
+    MembraneWeights weights(int l) const
+    {
+        double lp[5] = {params[0] * p.nx[l] / p.nx[0], params[1] * p.ny[l] / p.ny[0], params[2],
+                        params[3], params[4]};
+        return membrane_weights(lp);
+    }
+    void setup()
+    {
+        lv.L = p.L;
+        lv.c0 = c0;
+        for (int l = 0; l < p.L; ++l) {
+            MembraneWeights w = weights(l);
+            lv.nx[l] = p.nx[l];
+            lv.ny[l] = p.ny[l];
+            lv.w00[l] = (T)w.w00;
+            lv.w01[l] = (T)w.w01;
+            lv.w10[l] = (T)w.w10;
+        }
+        lds = coarse_lds_bytes<T>(p.nx, p.ny, c0, p.L);
+    }
+    // relax_membrane at level l, optionally after u[l] += prolong(u[l+1])
+    int relax(int l, bool prolong)
+    {
+        size_t P = p.P(l), Pc = prolong ? p.P(l + 1) : 0;
+        T *in = u(l), *out = ubuf(l, cur[l] ^ 1);
+        CSG_TRY(launch_relax_tile<T>(p.A[l], p.B[l], in, out, prolong ? u(l + 1) : nullptr,
+                                     prolong ? p.nx[l + 1] : 0, prolong ? p.ny[l + 1] : 0,
+                                     weights(l), p.nx[l], p.ny[l], batch, 3 * P, 2 * P, 2 * Pc,
+                                     2 * nit, s));
+        cur[l] ^= 1;
+        return kOk;
+    }
+    int down(int l)
+    {
+        size_t P = p.P(l), Pc = p.P(l + 1);
+        cur[l + 1] = 0;
+        return launch_restrict_residual<T>(p.A[l], p.B[l], u(l), weights(l), p.nx[l], p.ny[l],
+                                           p.nx[l + 1], p.ny[l + 1], batch, 3 * P, 2 * P, 2 * Pc,
+                                           p.B[l + 1], u(l + 1), s);
+    }
+    int coarse(int init, int ncycles)
+    {
+        size_t P = p.P(c0);
+        return launch_fmg_coarse<T>(lv, p.A[c0], init ? p.B0[c0] : p.B[c0], u(c0), 3 * P, 2 * P,
+                                    params[4], nit, ncycles, init, batch, lds, s);
+    }
+    int cycle(int start)
+    {
+        for (int l = start; l < c0; ++l) {
+            CSG_TRY(relax(l, false));
+            CSG_TRY(down(l));
+        }
+        CSG_TRY(coarse(0, 1));
+        for (int l = c0 - 1; l >= start; --l)
+            CSG_TRY(relax(l, true));
+        return kOk;
+    }
+};
+
+std::atomic<int> g_fused_schedule{1};
+
+// First level whose whole sub-pyramid fits in LDS, or -1 when the fused
+// schedule does not apply (a fine level of odd width, or tiles too large).
+template <typename T>
+int fused_first_coarse_level(const FmgPlan<T> &p, int nit)
+{
+    if (!g_fused_schedule.load(std::memory_order_relaxed))
+        return -1;
+    const size_t budget = fmg_lds_budget();
+    int c0 = -1;
+    for (int l = 0; l < p.L; ++l)
+        if (coarse_lds_bytes<T>(p.nx, p.ny, l, p.L) <= budget) {
+            c0 = l;
+            break;
+        }
+    if (c0 < 0)
+        return -1;
+    for (int l = 0; l < c0; ++l)
+        if ((p.nx[l] & 1) || p.nx[l] < 2)
+            return -1;
+    if (c0 > 0 && tile_lds_bytes<T>(2 * nit) > budget)
+        return -1;
+    return c0;
+}
+
 }  // namespace
+
+int set_fused_fmg_schedule(int enable)
+{
+    return g_fused_schedule.exchange(enable ? 1 : 0);
+}
 
 template <typename T>
 int fmg_run(const T *hes, const T *rhs, const double *params, int cycles, int nit,
@@ -216,6 +324,32 @@ int fmg_run(const T *hes, const T *rhs, const double *params, int cycles, int ni
             CSG_TRY(copy_async<T>(initial, out, 2 * P0 * batch, s));
     } else {
         CSG_TRY(fill_async<T>(out, (T)0, 2 * P0 * batch, s));
+    }
+    const int c0 = fused_first_coarse_level<T>(plan, nit);
+    if (c0 >= 0) {
+        FusedFmg<T> run{plan, params, nit, batch, c0, s};
+        run.setup();
+        if (coarse_init) {
+            CSG_TRY(run.coarse(1, cycles));
+            for (int level = c0 - 1; level >= 0; --level) {
+                size_t P = plan.P(level), Pc = plan.P(level + 1);
+                CSG_TRY(launch_resize<T>(run.u(level + 1), 2, plan.nx[level + 1],
+                                         plan.ny[level + 1], plan.nx[level], plan.ny[level],
+                                         batch, 2 * Pc, 2 * P, run.u(level), 0, s));
+                if (level > 0)
+                    CSG_TRY(copy_async<T>(plan.B0[level], plan.B[level], 2 * P * batch, s));
+                for (int cyc = 0; cyc < cycles; ++cyc)
+                    CSG_TRY(run.cycle(level));
+            }
+        } else if (c0 == 0) {
+            CSG_TRY(run.coarse(0, cycles));
+        } else {
+            for (int cyc = 0; cyc < cycles; ++cyc)
+                CSG_TRY(run.cycle(0));
+        }
+        if (run.cur[0] != 0)
+            CSG_TRY(copy_async<T>(plan.U2[0], out, 2 * P0 * batch, s));
+        return kOk;
     }
     FmgRun<T> run{plan, params, nit, batch, s};
     int last = plan.L - 1;

@@ -1,0 +1,110 @@
+// Per-pixel arithmetic of the membrane multigrid, shared by every kernel that
+// relaxes, applies or resizes (one-launch-per-sweep kernels, the LDS tiles of
+// the fine levels and the single-workgroup coarse-level kernel).  Keeping one
+// definition makes all schedules produce bit-identical values.
+//
+// Reference semantics: src/cat_surface_gpu/dartel_grid.py:229-305 (resize),
+// :1018-1070 (membrane system), :1073-1208 (relaxation).
+#pragma once
+
+#include "csg_common.h"
+
+namespace csg {
+
+// Quadratic B-spline weight.  dartel_grid.py:229-236
+template <typename T>
+__device__ __forceinline__ T wt2(T d)
+{
+    T a = fabs(d);
+    if (a < (T)0.5)
+        return (T)0.75 - a * a;
+    if (a < (T)1.5) {
+        T o = (T)1.5 - a;
+        return (T)0.5 * o * o;
+    }
+    return (T)0;
+}
+
+// The three source taps of target index t when a dimension is resized from S
+// to Tn samples: loc = (t + 0.5)·S/Tn - 0.5, origin o = floor(loc + 0.5),
+// weights of o-1, o, o+1.
+template <typename T>
+struct Taps {
+    int o;
+    T wm, w0, wp;
+};
+
+template <typename T>
+__device__ __forceinline__ Taps<T> resize_taps(int t, int S, int Tn)
+{
+    T l = ((T)t + (T)0.5) * (T)S / (T)Tn - (T)0.5;
+    int o = (int)floor(l + (T)0.5);
+    return {o, wt2<T>((T)o - (T)1 - l), wt2<T>((T)o - l), wt2<T>((T)o + (T)1 - l)};
+}
+
+// Separable resize at one target pixel: the y pass is evaluated first for
+// each of the three x taps, then the x pass, as in the reference's two-stage
+// evaluation.  f(x, y) returns the source value of one channel at a wrapped,
+// reflected source pixel.
+template <typename T, typename F>
+__device__ __forceinline__ T resize_at(const Taps<T> &tx, const Taps<T> &ty, int snx, int sny,
+                                       F f)
+{
+    int ym = reflect_y(ty.o - 1, sny), y0 = reflect_y(ty.o, sny), yp = reflect_y(ty.o + 1, sny);
+    int xm = wrap_x(tx.o - 1, snx), x0 = wrap_x(tx.o, snx), xp = wrap_x(tx.o + 1, snx);
+    T im = ty.wm * f(xm, ym) + ty.w0 * f(xm, y0) + ty.wp * f(xm, yp);
+    T i0 = ty.wm * f(x0, ym) + ty.w0 * f(x0, y0) + ty.wp * f(x0, yp);
+    T ip = ty.wm * f(xp, ym) + ty.w0 * f(xp, y0) + ty.wp * f(xp, yp);
+    return tx.wm * im + tx.w0 * i0 + tx.wp * ip;
+}
+
+template <typename T>
+struct Pair {
+    T x, y;
+};
+
+// Membrane stencil values around one pixel, per channel: u at the pixel and
+// the sums of its two y and two x neighbours.
+template <typename T>
+struct Stencil {
+    T vx, vy;      // u at the pixel
+    T syx, sxx;    // channel 0: u[ym] + u[yp], u[xm] + u[xp]
+    T syy, sxy;    // channel 1
+};
+
+// Gauss-Seidel block update of one pixel: solve the 2x2 system
+// (A + w00) u = b - w01·Σ_y u - w10·Σ_x u.  dartel_triton.py:241-291
+template <typename T>
+__device__ __forceinline__ Pair<T> relax_pixel(T bx, T by, const Stencil<T> &n, T a0, T a1,
+                                               T a2, T w00, T w01, T w10)
+{
+    T rx = bx - w01 * n.syx - w10 * n.sxx;
+    T ry = by - w01 * n.syy - w10 * n.sxy;
+    T dx = a0 + w00, dy = a1 + w00, cr = a2;
+    T det = dx * dy * (T)1.0000000001 - cr * cr;
+    return {(dy * rx - cr * ry) / det, (-cr * rx + dx * ry) / det};
+}
+
+// (L + A) u at one pixel.  dartel_triton.py:189-239
+template <typename T>
+__device__ __forceinline__ Pair<T> membrane_pixel(const Stencil<T> &n, T a0, T a1, T a2, T w00,
+                                                  T w01, T w10)
+{
+    T rx = w00 * n.vx + w01 * n.syx + w10 * n.sxx;
+    T ry = w00 * n.vy + w01 * n.syy + w10 * n.sxy;
+    return {rx + a0 * n.vx + a2 * n.vy, ry + a2 * n.vx + a1 * n.vy};
+}
+
+// Planar [2][P] field accessor with the mixed boundary: the stencil of pixel
+// (x, y) of an nx × ny grid.
+template <typename T>
+__device__ __forceinline__ Stencil<T> stencil_at(const T *u, int P, int nx, int ny, int x, int y)
+{
+    int p = y * nx + x;
+    int xm = y * nx + wrap_x(x - 1, nx), xp = y * nx + wrap_x(x + 1, nx);
+    int ym = reflect_y(y - 1, ny) * nx + x, yp = reflect_y(y + 1, ny) * nx + x;
+    return {u[p], u[P + p], u[ym] + u[yp], u[xm] + u[xp], u[P + ym] + u[P + yp],
+            u[P + xm] + u[P + xp]};
+}
+
+}  // namespace csg

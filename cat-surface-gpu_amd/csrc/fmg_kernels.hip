@@ -5,21 +5,9 @@
 // Reference semantics: src/cat_surface_gpu/dartel_grid.py:239-305 (resize),
 // :1018-1070 (membrane system), :1073-1208 (relaxation), :1211-1417 (FMG).
 #include "dartel_kernels.h"
+#include "fmg_device.h"
 
 namespace csg {
-
-template <typename T>
-__device__ __forceinline__ T wt2(T d)
-{
-    T a = fabs(d);
-    if (a < (T)0.5)
-        return (T)0.75 - a * a;
-    if (a < (T)1.5) {
-        T o = (T)1.5 - a;
-        return (T)0.5 * o * o;
-    }
-    return (T)0;
-}
 
 // Separable 3-tap resize; one thread per target pixel, all channels.
 // Source location loc = (t + 0.5)·S/T - 0.5, origin floor(loc + 0.5).
@@ -37,22 +25,11 @@ k_resize(const T *__restrict__ in, int channels, int snx, int sny, int tnx, int 
     in += blockIdx.y * istride;
     out += blockIdx.y * ostride;
     int tx = p % tnx, ty = p / tnx;
-    T ly = ((T)ty + (T)0.5) * (T)sny / (T)tny - (T)0.5;
-    int oy = (int)floor(ly + (T)0.5);
-    T wy = wt2<T>((T)oy - ly), wym = wt2<T>((T)oy - (T)1 - ly), wyp = wt2<T>((T)oy + (T)1 - ly);
-    T lx = ((T)tx + (T)0.5) * (T)snx / (T)tnx - (T)0.5;
-    int ox = (int)floor(lx + (T)0.5);
-    T wx = wt2<T>((T)ox - lx), wxm = wt2<T>((T)ox - (T)1 - lx), wxp = wt2<T>((T)ox + (T)1 - lx);
-    int ym = reflect_y(oy - 1, sny) * snx, y0 = reflect_y(oy, sny) * snx,
-        yp = reflect_y(oy + 1, sny) * snx;
-    int xm = wrap_x(ox - 1, snx), x0 = wrap_x(ox, snx), xp = wrap_x(ox + 1, snx);
+    const Taps<T> txp = resize_taps<T>(tx, snx, tnx), typ = resize_taps<T>(ty, sny, tny);
     const int SP = snx * sny;
     for (int c = 0; c < channels; ++c) {
         const T *f = in + (size_t)c * SP;
-        T im = wym * f[ym + xm] + wy * f[y0 + xm] + wyp * f[yp + xm];
-        T i0 = wym * f[ym + x0] + wy * f[y0 + x0] + wyp * f[yp + x0];
-        T ip = wym * f[ym + xp] + wy * f[y0 + xp] + wyp * f[yp + xp];
-        T r = wxm * im + wx * i0 + wxp * ip;
+        T r = resize_at<T>(txp, typ, snx, sny, [&](int x, int y) { return f[y * snx + x]; });
         size_t o = (size_t)c * TP + p;
         out[o] = accumulate ? out[o] + r : r;
     }
@@ -97,14 +74,10 @@ k_relax_color(const T *__restrict__ a, const T *__restrict__ b, T *__restrict__ 
     b += blockIdx.y * bstride;
     u += blockIdx.y * bstride;
     int p = y * nx + x;
-    int xm = y * nx + wrap_x(x - 1, nx), xp = y * nx + wrap_x(x + 1, nx);
-    int ym = reflect_y(y - 1, ny) * nx + x, yp = reflect_y(y + 1, ny) * nx + x;
-    T rx = b[p] - w01 * (u[ym] + u[yp]) - w10 * (u[xm] + u[xp]);
-    T ry = b[P + p] - w01 * (u[P + ym] + u[P + yp]) - w10 * (u[P + xm] + u[P + xp]);
-    T dx = a[p] + w00, dy = a[P + p] + w00, cr = a[2 * P + p];
-    T det = dx * dy * (T)1.0000000001 - cr * cr;
-    u[p] = (dy * rx - cr * ry) / det;
-    u[P + p] = (-cr * rx + dx * ry) / det;
+    Pair<T> r = relax_pixel<T>(b[p], b[P + p], stencil_at<T>(u, P, nx, ny, x, y), a[p], a[P + p],
+                               a[2 * P + p], w00, w01, w10);
+    u[p] = r.x;
+    u[P + p] = r.y;
 }
 
 // out = (L + A) u, or out = b - (L + A) u when b_minus is given.
@@ -123,13 +96,9 @@ k_apply_membrane(const T *__restrict__ a, const T *__restrict__ u, T w00, T w01,
     u += blockIdx.y * ustride;
     out += blockIdx.y * ustride;
     int x = p % nx, y = p / nx;
-    int xm = y * nx + wrap_x(x - 1, nx), xp = y * nx + wrap_x(x + 1, nx);
-    int ym = reflect_y(y - 1, ny) * nx + x, yp = reflect_y(y + 1, ny) * nx + x;
-    T vx = u[p], vy = u[P + p];
-    T rx = w00 * vx + w01 * (u[ym] + u[yp]) + w10 * (u[xm] + u[xp]);
-    T ry = w00 * vy + w01 * (u[P + ym] + u[P + yp]) + w10 * (u[P + xm] + u[P + xp]);
-    T ox = rx + a[p] * vx + a[2 * P + p] * vy;
-    T oy = ry + a[2 * P + p] * vx + a[P + p] * vy;
+    Pair<T> m = membrane_pixel<T>(stencil_at<T>(u, P, nx, ny, x, y), a[p], a[P + p], a[2 * P + p],
+                                  w00, w01, w10);
+    T ox = m.x, oy = m.y;
     if (b_minus) {
         b_minus += blockIdx.y * ustride;
         ox = b_minus[p] - ox;

@@ -35,6 +35,12 @@ const char *csg_last_error(void);
 const char *csg_version(void);
 /* 0 if a HIP device is visible; fills *count. */
 int csg_device_count(int *count);
+/* Multigrid schedule of every FMG below (fmg2_membrane, dartel_step, solve): 1 = fused
+ * (fine levels relaxed on LDS tiles, coarse levels in one workgroup per job; the
+ * default), 0 = one launch per reference operator call.  Both compute the same values
+ * in the same order.  Process-wide; returns the previous setting.  No counterpart in
+ * the reference (it has only the per-operator schedule of dartel_grid.py:1211-1417). */
+int csg_set_fmg_schedule(int fused);
 
 /* ------------------------------------------------------------ DARTEL grid */
 
