@@ -96,8 +96,8 @@ int launch_any_nonzero(const T *u, size_t n, int batch, size_t stride, int *flag
 
 constexpr int kFmgMaxLevels = 40;
 constexpr int kTileX = 32, kTileY = 16;  // interior of a fine-level relaxation tile
-constexpr int kTileThreads = 256;
-constexpr int kCoarseThreads = 512;
+constexpr int kTileThreads = 512;
+constexpr int kCoarseThreads = 1024;
 
 // Levels c0 .. L-1 of the pyramid, relaxed by one workgroup per job in LDS.
 template <typename T>

@@ -58,6 +58,27 @@ __device__ __forceinline__ T resize_at(const Taps<T> &tx, const Taps<T> &ty, int
     return tx.wm * im + tx.w0 * i0 + tx.wp * ip;
 }
 
+// n / d and n % d for 0 <= n < 2^22, d >= 1, from inv = 1.0f / d (computed
+// once by the caller): the float quotient is off by at most one and is
+// corrected exactly.  Cheaper than an integer division by a run-time value.
+struct DivMod {
+    int q, r;
+};
+
+__device__ __forceinline__ DivMod divmod_small(int n, int d, float inv)
+{
+    int q = (int)((float)n * inv);
+    int r = n - q * d;
+    if (r < 0) {
+        --q;
+        r += d;
+    } else if (r >= d) {
+        ++q;
+        r -= d;
+    }
+    return {q, r};
+}
+
 template <typename T>
 struct Pair {
     T x, y;

@@ -34,13 +34,16 @@ namespace {
 // inside the grid).  Pixels whose neighbours lie outside the tile are not
 // updated; after s sweeps every pixel at distance >= s from the tile edge
 // holds the value of the global sweep, so the interior is exact after H.
-template <typename T>
+//
+// HC > 0 fixes the halo (= sweeps) at compile time; HC == 0 takes it from H.
+template <typename T, int HC>
 __global__ void __launch_bounds__(kTileThreads)
 k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restrict__ Uin,
              T *__restrict__ Uout, const T *__restrict__ Uc, int cnx, int cny, int nx, int ny,
-             T w00, T w01, T w10, int H, int tiles_x, size_t astride, size_t bstride,
+             T w00, T w01, T w10, int H_arg, int tiles_x, size_t astride, size_t bstride,
              size_t cstride)
 {
+    const int H = HC > 0 ? HC : H_arg;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T *lds = reinterpret_cast<T *>(lds_raw);
     const int P = nx * ny, CP = cnx * cny;
@@ -79,16 +82,20 @@ k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restri
         b1[q] = B[P + p];
     }
     __syncthreads();
+    const int hex = ex / 2;  // pixels of one colour per tile row (ex is even)
     for (int s = 0; s < H; ++s) {
         const int colour = (s & 1) ? 0 : 1;
-        for (int q = threadIdx.x; q < npx; q += blockDim.x) {
-            const int r = q / ex, i = q - r * ex;
+        for (int j = threadIdx.x; j < hex * (yhi - ylo); j += blockDim.x) {
+            const int r = j / hex, k = j - r * hex;
             const int gy = ylo + r;
-            if (i == 0 || i == ex - 1 || (((x0 + i + gy) & 1) != colour))
+            // the column of this row with (x0 + i + gy) & 1 == colour
+            const int i = 2 * k + ((colour - x0 - gy) & 1);
+            if (i == 0 || i == ex - 1)
                 continue;
             const int ym = reflect_y(gy - 1, ny), yp = reflect_y(gy + 1, ny);
             if (ym < ylo || ym >= yhi || yp < ylo || yp >= yhi)
                 continue;
+            const int q = r * ex + i;
             const int qym = (ym - ylo) * ex + i, qyp = (yp - ylo) * ex + i;
             const Stencil<T> n{u0[q],           u1[q],           u0[qym] + u0[qyp],
                                u0[q - 1] + u0[q + 1], u1[qym] + u1[qyp], u1[q - 1] + u1[q + 1]};
@@ -142,74 +149,38 @@ k_restrict_residual(const T *__restrict__ A, const T *__restrict__ B, const T *_
 }
 
 // ---------------------------------------------------------------- coarse levels
+//
+// One workgroup per job holds levels c0 .. L-1 in LDS (per level: A 3 planes,
+// b 2, u 2) plus a two-plane residual buffer of level c0's size.  The schedule
+// is the reference's (dartel_grid.py:1341-1413).  Every step is one call of a
+// single non-inlined function (coarse_op) with plain scalar arguments, so the
+// kernel's code stays small whatever the number of levels and cycles; inside
+// it every access is an LDS offset and every loop strides by the fixed block
+// size, so a step costs no global memory access.
 
-// LDS view of one level: A (3 planes), b (2), u (2).
-template <typename T>
-struct LdsLevel {
-    T *a, *b, *u;
-    int nx, ny;
-    T w00, w01, w10;
-    __device__ int P() const { return nx * ny; }
-};
+extern __shared__ __align__(16) unsigned char g_lds[];
 
-// The level table, copied to LDS at kernel start so that levels can be
-// indexed at run time without spilling the kernel argument to scratch.
+// The level table, copied to LDS at kernel start.
 template <typename T>
 struct LevelTable {
     int nx[kFmgMaxLevels], ny[kFmgMaxLevels];
     unsigned off[kFmgMaxLevels];
+    unsigned roff;  // residual buffer
     T w00[kFmgMaxLevels], w01[kFmgMaxLevels], w10[kFmgMaxLevels];
 };
 
-template <typename T>
-__device__ __forceinline__ LdsLevel<T> lds_level(const LevelTable<T> &t, T *base, int l)
-{
-    const int P = t.nx[l] * t.ny[l];
-    T *a = base + t.off[l];
-    return {a, a + 3 * P, a + 5 * P, t.nx[l], t.ny[l], t.w00[l], t.w01[l], t.w10[l]};
-}
+enum CoarseOp : int { kRelax, kDown, kUpAdd, kProlong, kPyramidA, kPyramidB };
 
-// relax_membrane on one LDS level (including the two-phase odd-width order).
+// LDS resize of `channels` planes: out = resize(in) [+ out].
 template <typename T>
-__device__ __forceinline__ void lds_relax(const LdsLevel<T> &L, int nit)
-{
-    const int nx = L.nx, ny = L.ny, P = L.P();
-    const int half = (nx + 1) >> 1;
-    const bool odd = (nx & 1) && nx > 1;
-    auto update = [&](int x, int y) {
-        const int p = y * nx + x;
-        const Pair<T> v = relax_pixel<T>(L.b[p], L.b[P + p], stencil_at<T>(L.u, P, nx, ny, x, y),
-                                         L.a[p], L.a[P + p], L.a[2 * P + p], L.w00, L.w01, L.w10);
-        L.u[p] = v.x;
-        L.u[P + p] = v.y;
-    };
-    for (int it = 0; it < 2 * nit; ++it) {
-        const int colour = (it % 2 == 0) ? 1 : 0;
-        for (int q = threadIdx.x; q < half * ny; q += blockDim.x) {
-            const int y = q / half;
-            const int x = 2 * (q - y * half) + ((colour + y) & 1);
-            if (x >= nx || (odd && x == nx - 1))
-                continue;
-            update(x, y);
-        }
-        __syncthreads();
-        if (odd) {
-            for (int y = threadIdx.x; y < ny; y += blockDim.x)
-                if (((nx - 1 + y) & 1) == colour)
-                    update(nx - 1, y);
-            __syncthreads();
-        }
-    }
-}
-
-// Resize `channels` planes of level `src` into level `dst` (src → dst sizes).
-template <typename T>
-__device__ __forceinline__ void lds_resize(const T *in, int snx, int sny, T *out, int tnx, int tny, int channels,
-                           bool accumulate)
+__device__ __forceinline__ void lds_resize(const T *in, int snx, int sny, T *out, int tnx,
+                                           int tny, int channels, bool accumulate)
 {
     const int SP = snx * sny, TP = tnx * tny;
-    for (int p = threadIdx.x; p < TP; p += blockDim.x) {
-        const int ty = p / tnx, tx = p - ty * tnx;
+    const float inv = 1.0f / (float)tnx;
+    for (int p = threadIdx.x; p < TP; p += kCoarseThreads) {
+        const DivMod yx = divmod_small(p, tnx, inv);
+        const int ty = yx.q, tx = yx.r;
         const Taps<T> txp = resize_taps<T>(tx, snx, tnx), typ = resize_taps<T>(ty, sny, tny);
         for (int c = 0; c < channels; ++c) {
             const T *f = in + c * SP;
@@ -218,46 +189,91 @@ __device__ __forceinline__ void lds_resize(const T *in, int snx, int sny, T *out
             *o = accumulate ? *o + r : r;
         }
     }
-    __syncthreads();
 }
 
-// b[l+1] = restrict(b[l] - (L + A) u[l]); u[l+1] = 0.
+// One step of the schedule on level (nx, ny) at LDS offset `off`; the other
+// level (onx, ony, ooff) is l+1 for kDown / kUpAdd / kProlong and l-1 for the
+// pyramid steps.  Ends with a workgroup barrier.
 template <typename T>
-__device__ __forceinline__ void lds_down(const LdsLevel<T> &F, const LdsLevel<T> &C)
+__device__ __noinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w00, T w01, T w10,
+                                       int onx, int ony, unsigned ooff, unsigned roff, int nit)
 {
-    const int nx = F.nx, ny = F.ny, P = F.P(), CP = C.P();
-    for (int q = threadIdx.x; q < CP; q += blockDim.x) {
-        const int cy = q / C.nx, cx = q - cy * C.nx;
-        const Taps<T> tx = resize_taps<T>(cx, nx, C.nx), ty = resize_taps<T>(cy, ny, C.ny);
-        auto res = [&](int x, int y, int ch) {
-            const int p = y * nx + x;
-            const Pair<T> m = membrane_pixel<T>(stencil_at<T>(F.u, P, nx, ny, x, y), F.a[p],
-                                                F.a[P + p], F.a[2 * P + p], F.w00, F.w01, F.w10);
-            return ch ? F.b[P + p] - m.y : F.b[p] - m.x;
-        };
-        C.b[q] = resize_at<T>(tx, ty, nx, ny, [&](int x, int y) { return res(x, y, 0); });
-        C.b[CP + q] = resize_at<T>(tx, ty, nx, ny, [&](int x, int y) { return res(x, y, 1); });
-        C.u[q] = (T)0;
-        C.u[CP + q] = (T)0;
+    T *base = reinterpret_cast<T *>(g_lds);
+    const int P = nx * ny, OP = onx * ony;
+    T *a = base + off, *b = a + 3 * P, *u = a + 5 * P;
+    T *oa = base + ooff, *ob = oa + 3 * OP, *ou = oa + 5 * OP;
+    if (op == kRelax) {
+        // relax_membrane: 2·nit half-sweeps, two-phase order for odd widths
+        const int half = (nx + 1) >> 1;
+        const float inv_half = 1.0f / (float)half;
+        const bool odd = (nx & 1) && nx > 1;
+        for (int it = 0; it < 2 * nit; ++it) {
+            const int colour = (it % 2 == 0) ? 1 : 0;
+            for (int phase = 0; phase < (odd ? 2 : 1); ++phase) {
+                const int count = phase == 0 ? half * ny : ny;
+                for (int q = threadIdx.x; q < count; q += kCoarseThreads) {
+                    int x, y;
+                    if (phase == 0) {
+                        const DivMod yk = divmod_small(q, half, inv_half);
+                        y = yk.q;
+                        x = 2 * yk.r + ((colour + y) & 1);
+                        if (x >= nx || (odd && x == nx - 1))
+                            continue;
+                    } else {
+                        y = q;
+                        x = nx - 1;
+                        if (((x + y) & 1) != colour)
+                            continue;
+                    }
+                    const int p = y * nx + x;
+                    const Pair<T> v =
+                        relax_pixel<T>(b[p], b[P + p], stencil_at<T>(u, P, nx, ny, x, y), a[p],
+                                       a[P + p], a[2 * P + p], w00, w01, w10);
+                    u[p] = v.x;
+                    u[P + p] = v.y;
+                }
+                __syncthreads();
+            }
+        }
+    } else if (op == kDown) {
+        // r = b - (L + A) u; b[l+1] = restrict(r); u[l+1] = 0.  dartel_grid.py:1327-1368
+        T *r = base + roff;
+        const float inv_nx = 1.0f / (float)nx;
+        for (int p = threadIdx.x; p < P; p += kCoarseThreads) {
+            const DivMod yx = divmod_small(p, nx, inv_nx);
+            const int y = yx.q, x = yx.r;
+            const Pair<T> m = membrane_pixel<T>(stencil_at<T>(u, P, nx, ny, x, y), a[p],
+                                                a[P + p], a[2 * P + p], w00, w01, w10);
+            r[p] = b[p] - m.x;
+            r[P + p] = b[P + p] - m.y;
+        }
+        __syncthreads();
+        lds_resize<T>(r, nx, ny, ob, onx, ony, 2, false);
+        for (int p = threadIdx.x; p < 2 * OP; p += kCoarseThreads)
+            ou[p] = (T)0;
+        __syncthreads();
+    } else if (op == kUpAdd || op == kProlong) {
+        lds_resize<T>(ou, onx, ony, u, nx, ny, 2, op == kUpAdd);
+        __syncthreads();
+    } else {
+        // pyramid: A (or the original b) of this level from the finer one
+        if (op == kPyramidA)
+            lds_resize<T>(oa, onx, ony, a, nx, ny, 3, false);
+        else
+            lds_resize<T>(ob, onx, ony, b, nx, ny, 2, false);
+        __syncthreads();
     }
-    __syncthreads();
 }
 
+// coarse_op at level l of the table (the table is in LDS, the call takes scalars).
 template <typename T>
-__device__ __forceinline__ void lds_cycle(const LevelTable<T> &lv, int L, T *base, int start, int nit)
+__device__ __forceinline__ void run_op(int op, int l, const LevelTable<T> &t, int nit)
 {
-    const int last = L - 1;
-    for (int l = start; l < last; ++l) {
-        LdsLevel<T> F = lds_level(lv, base, l);
-        lds_relax(F, nit);
-        lds_down(F, lds_level(lv, base, l + 1));
-    }
-    lds_relax(lds_level(lv, base, last), nit);
-    for (int l = last - 1; l >= start; --l) {
-        LdsLevel<T> F = lds_level(lv, base, l), C = lds_level(lv, base, l + 1);
-        lds_resize(C.u, C.nx, C.ny, F.u, F.nx, F.ny, 2, true);
-        lds_relax(F, nit);
-    }
+    // the other level: none for a relaxation (pass l itself), l - 1 for the
+    // pyramid steps, l + 1 otherwise
+    const int o = op == kRelax ? l : ((op == kPyramidA || op == kPyramidB) ? l - 1 : l + 1);
+    coarse_op<T>(op, t.nx[l], t.ny[l], t.off[l], t.w00[l], t.w01[l], t.w10[l], t.nx[o], t.ny[o],
+                 t.off[o], t.roff, nit);
 }
 
 template <typename T>
@@ -266,9 +282,8 @@ k_fmg_coarse(CoarseLevels<T> lv, const T *__restrict__ A, const T *__restrict__ 
              T *__restrict__ U, size_t astride, size_t bstride, double id, int nit, int cycles,
              int init)
 {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ LevelTable<T> tab;
-    T *base = reinterpret_cast<T *>(lds_raw);
+    T *base = reinterpret_cast<T *>(g_lds);
     const int job = blockIdx.x;
     A += job * astride;
     B += job * bstride;
@@ -290,63 +305,81 @@ k_fmg_coarse(CoarseLevels<T> lv, const T *__restrict__ A, const T *__restrict__ 
                 off += 7u * (unsigned)(lv.nx[l] * lv.ny[l]);
             }
         }
+        tab.roff = off;
     }
     __syncthreads();
-    LdsLevel<T> top = lds_level(tab, base, c0);
-    const int P0 = top.P();
-    for (int p = threadIdx.x; p < P0; p += blockDim.x) {
-        top.a[p] = A[p];
-        top.a[P0 + p] = A[P0 + p];
-        top.a[2 * P0 + p] = A[2 * P0 + p];
-        top.b[p] = B[p];
-        top.b[P0 + p] = B[P0 + p];
+    const int P0 = tab.nx[c0] * tab.ny[c0];
+    T *ta = base + tab.off[c0], *tb = ta + 3 * P0, *tu = ta + 5 * P0;
+    for (int p = threadIdx.x; p < P0; p += kCoarseThreads) {
+        ta[p] = A[p];
+        ta[P0 + p] = A[P0 + p];
+        ta[2 * P0 + p] = A[2 * P0 + p];
+        tb[p] = B[p];
+        tb[P0 + p] = B[P0 + p];
         if (!init) {
-            top.u[p] = U[p];
-            top.u[P0 + p] = U[P0 + p];
+            tu[p] = U[p];
+            tu[P0 + p] = U[P0 + p];
         }
     }
     __syncthreads();
     // pyramids of A (and, for the coarse initialisation, of the original b)
     for (int l = c0 + 1; l <= last; ++l) {
-        LdsLevel<T> F = lds_level(tab, base, l - 1), C = lds_level(tab, base, l);
-        lds_resize(F.a, F.nx, F.ny, C.a, C.nx, C.ny, 3, false);
+        run_op<T>(kPyramidA, l, tab, nit);
         if (init)
-            lds_resize(F.b, F.nx, F.ny, C.b, C.nx, C.ny, 2, false);
+            run_op<T>(kPyramidB, l, tab, nit);
     }
     if (init) {
         // dartel_grid.py:1301-1326: direct 2x2 solve against the original b of
         // the coarsest level, then relaxation against that level's (zero) b
-        LdsLevel<T> C = lds_level(tab, base, last);
-        const int PC = C.P();
-        for (int p = threadIdx.x; p < PC; p += blockDim.x) {
-            double dx = (double)C.a[p] + id, dy = (double)C.a[PC + p] + id,
-                   cr = (double)C.a[2 * PC + p];
+        const int PC = tab.nx[last] * tab.ny[last];
+        T *ca = base + tab.off[last], *cb = ca + 3 * PC, *cu = ca + 5 * PC;
+        for (int p = threadIdx.x; p < PC; p += kCoarseThreads) {
+            double dx = (double)ca[p] + id, dy = (double)ca[PC + p] + id,
+                   cr = (double)ca[2 * PC + p];
             double det = dx * dy * 1.0000000001 - cr * cr;
-            double bx = (double)C.b[p], by = (double)C.b[PC + p];
-            C.u[p] = (T)((dy * bx - cr * by) / det);
-            C.u[PC + p] = (T)((-cr * bx + dx * by) / det);
+            double bx = (double)cb[p], by = (double)cb[PC + p];
+            cu[p] = (T)((dy * bx - cr * by) / det);
+            cu[PC + p] = (T)((-cr * bx + dx * by) / det);
         }
         __syncthreads();
-        for (int p = threadIdx.x; p < 2 * PC; p += blockDim.x)
-            C.b[p] = (T)0;
+        for (int p = threadIdx.x; p < 2 * PC; p += kCoarseThreads)
+            cb[p] = (T)0;
         __syncthreads();
-        lds_relax(C, nit);
+        run_op<T>(kRelax, last, tab, nit);
         // dartel_grid.py:1402-1413; b of each level still holds its original
         // pyramid value when the level is reached (only coarser levels have
         // been overwritten by then)
         for (int level = last - 1; level >= c0; --level) {
-            LdsLevel<T> F = lds_level(tab, base, level), Cl = lds_level(tab, base, level + 1);
-            lds_resize(Cl.u, Cl.nx, Cl.ny, F.u, F.nx, F.ny, 2, false);
-            for (int cyc = 0; cyc < cycles; ++cyc)
-                lds_cycle(tab, L, base, level, nit);
+            run_op<T>(kProlong, level, tab, nit);
+            for (int cyc = 0; cyc < cycles; ++cyc) {
+                // cycle(level), dartel_grid.py:1341-1391
+                for (int l = level; l < last; ++l) {
+                    run_op<T>(kRelax, l, tab, nit);
+                    run_op<T>(kDown, l, tab, nit);
+                }
+                run_op<T>(kRelax, last, tab, nit);
+                for (int l = last - 1; l >= level; --l) {
+                    run_op<T>(kUpAdd, l, tab, nit);
+                    run_op<T>(kRelax, l, tab, nit);
+                }
+            }
         }
     } else {
-        for (int cyc = 0; cyc < cycles; ++cyc)
-            lds_cycle(tab, L, base, c0, nit);
+        for (int cyc = 0; cyc < cycles; ++cyc) {
+            for (int l = c0; l < last; ++l) {
+                run_op<T>(kRelax, l, tab, nit);
+                run_op<T>(kDown, l, tab, nit);
+            }
+            run_op<T>(kRelax, last, tab, nit);
+            for (int l = last - 1; l >= c0; --l) {
+                run_op<T>(kUpAdd, l, tab, nit);
+                run_op<T>(kRelax, l, tab, nit);
+            }
+        }
     }
-    for (int p = threadIdx.x; p < P0; p += blockDim.x) {
-        U[p] = top.u[p];
-        U[P0 + p] = top.u[P0 + p];
+    for (int p = threadIdx.x; p < P0; p += kCoarseThreads) {
+        U[p] = tu[p];
+        U[P0 + p] = tu[P0 + p];
     }
 }
 
@@ -378,8 +411,10 @@ void init_lds_budget()
     }
     // leave room for the kernels' static shared variables
     int budget = per_block - 4096;
-    raise_limit(k_relax_tile<float>, budget);
-    raise_limit(k_relax_tile<double>, budget);
+    raise_limit(k_relax_tile<float, 0>, budget);
+    raise_limit(k_relax_tile<float, 6>, budget);
+    raise_limit(k_relax_tile<double, 0>, budget);
+    raise_limit(k_relax_tile<double, 6>, budget);
     raise_limit(k_fmg_coarse<float>, budget);
     raise_limit(k_fmg_coarse<double>, budget);
     if (hipGetLastError() == hipSuccess)
@@ -406,7 +441,8 @@ size_t coarse_lds_bytes(const int *nx, const int *ny, int c0, int L)
     size_t px = 0;
     for (int l = c0; l < L; ++l)
         px += (size_t)nx[l] * ny[l];
-    return px * 7 * sizeof(T);
+    // A, b, u of every level and a residual buffer of level c0's size
+    return (px * 7 + (size_t)2 * nx[c0] * ny[c0]) * sizeof(T);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -422,9 +458,15 @@ int launch_relax_tile(const T *a, const T *b, const T *uin, T *uout, const T *uc
     if (lds > fmg_lds_budget())
         return fail(kInvalidArgument, "tiled relaxation needs %zu bytes of LDS", lds);
     const int tiles_x = (nx + kTileX - 1) / kTileX, tiles_y = (ny + kTileY - 1) / kTileY;
-    hipLaunchKernelGGL(k_relax_tile<T>, dim3(tiles_x * tiles_y, batch), dim3(kTileThreads), lds,
-                       s, a, b, uin, uout, uc, cnx, cny, nx, ny, (T)w.w00, (T)w.w01, (T)w.w10,
-                       sweeps, tiles_x, astride, bstride, cstride);
+    dim3 grid(tiles_x * tiles_y, batch);
+    if (sweeps == 6)  // nit = 3, the reference default
+        hipLaunchKernelGGL((k_relax_tile<T, 6>), grid, dim3(kTileThreads), lds, s, a, b, uin, uout,
+                           uc, cnx, cny, nx, ny, (T)w.w00, (T)w.w01, (T)w.w10, sweeps, tiles_x,
+                           astride, bstride, cstride);
+    else
+        hipLaunchKernelGGL((k_relax_tile<T, 0>), grid, dim3(kTileThreads), lds, s, a, b, uin, uout,
+                           uc, cnx, cny, nx, ny, (T)w.w00, (T)w.w01, (T)w.w10, sweeps, tiles_x,
+                           astride, bstride, cstride);
     CSG_LAUNCH_CHECK();
     return kOk;
 }
