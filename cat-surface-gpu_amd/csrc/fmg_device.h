@@ -42,20 +42,50 @@ __device__ __forceinline__ Taps<T> resize_taps(int t, int S, int Tn)
     return {o, wt2<T>((T)o - (T)1 - l), wt2<T>((T)o - l), wt2<T>((T)o + (T)1 - l)};
 }
 
-// Separable resize at one target pixel: the y pass is evaluated first for
-// each of the three x taps, then the x pass, as in the reference's two-stage
-// evaluation.  f(x, y) returns the source value of one channel at a wrapped,
-// reflected source pixel.
+// Source indices (wrapped / reflected) and weights of one target index.
+template <typename T>
+struct TapIdx {
+    int im, i0, ip;
+    T wm, w0, wp;
+};
+
+template <typename T>
+__device__ __forceinline__ TapIdx<T> taps_x(int t, int S, int Tn)
+{
+    const Taps<T> k = resize_taps<T>(t, S, Tn);
+    return {wrap_x(k.o - 1, S), wrap_x(k.o, S), wrap_x(k.o + 1, S), k.wm, k.w0, k.wp};
+}
+
+template <typename T>
+__device__ __forceinline__ TapIdx<T> taps_y(int t, int S, int Tn)
+{
+    const Taps<T> k = resize_taps<T>(t, S, Tn);
+    return {reflect_y(k.o - 1, S), reflect_y(k.o, S), reflect_y(k.o + 1, S), k.wm, k.w0, k.wp};
+}
+
+// Separable resize at one target pixel from resolved taps: the y pass is
+// evaluated first for each of the three x taps, then the x pass, as in the
+// reference's two-stage evaluation.  f(x, y) returns the source value of one
+// channel at source pixel (x, y).
+template <typename T, typename F>
+__device__ __forceinline__ T resize_from(const TapIdx<T> &tx, const TapIdx<T> &ty, F f)
+{
+    T im = ty.wm * f(tx.im, ty.im) + ty.w0 * f(tx.im, ty.i0) + ty.wp * f(tx.im, ty.ip);
+    T i0 = ty.wm * f(tx.i0, ty.im) + ty.w0 * f(tx.i0, ty.i0) + ty.wp * f(tx.i0, ty.ip);
+    T ip = ty.wm * f(tx.ip, ty.im) + ty.w0 * f(tx.ip, ty.i0) + ty.wp * f(tx.ip, ty.ip);
+    return tx.wm * im + tx.w0 * i0 + tx.wp * ip;
+}
+
+// The same from unresolved taps (indices wrapped and reflected here).
 template <typename T, typename F>
 __device__ __forceinline__ T resize_at(const Taps<T> &tx, const Taps<T> &ty, int snx, int sny,
                                        F f)
 {
-    int ym = reflect_y(ty.o - 1, sny), y0 = reflect_y(ty.o, sny), yp = reflect_y(ty.o + 1, sny);
-    int xm = wrap_x(tx.o - 1, snx), x0 = wrap_x(tx.o, snx), xp = wrap_x(tx.o + 1, snx);
-    T im = ty.wm * f(xm, ym) + ty.w0 * f(xm, y0) + ty.wp * f(xm, yp);
-    T i0 = ty.wm * f(x0, ym) + ty.w0 * f(x0, y0) + ty.wp * f(x0, yp);
-    T ip = ty.wm * f(xp, ym) + ty.w0 * f(xp, y0) + ty.wp * f(xp, yp);
-    return tx.wm * im + tx.w0 * i0 + tx.wp * ip;
+    const TapIdx<T> x{wrap_x(tx.o - 1, snx), wrap_x(tx.o, snx), wrap_x(tx.o + 1, snx),
+                      tx.wm, tx.w0, tx.wp};
+    const TapIdx<T> y{reflect_y(ty.o - 1, sny), reflect_y(ty.o, sny), reflect_y(ty.o + 1, sny),
+                      ty.wm, ty.w0, ty.wp};
+    return resize_from<T>(x, y, f);
 }
 
 // n / d and n % d for 0 <= n < 2^22, d >= 1, from inv = 1.0f / d (computed
@@ -117,13 +147,14 @@ __device__ __forceinline__ Pair<T> membrane_pixel(const Stencil<T> &n, T a0, T a
 }
 
 // Planar [2][P] field accessor with the mixed boundary: the stencil of pixel
-// (x, y) of an nx × ny grid.
+// (x, y) of an nx × ny grid.  For offsets of one pixel the circular x and the
+// reflected y boundaries reduce to a comparison each (nx or ny == 1 included).
 template <typename T>
 __device__ __forceinline__ Stencil<T> stencil_at(const T *u, int P, int nx, int ny, int x, int y)
 {
-    int p = y * nx + x;
-    int xm = y * nx + wrap_x(x - 1, nx), xp = y * nx + wrap_x(x + 1, nx);
-    int ym = reflect_y(y - 1, ny) * nx + x, yp = reflect_y(y + 1, ny) * nx + x;
+    const int row = y * nx, p = row + x;
+    const int xm = row + (x == 0 ? nx - 1 : x - 1), xp = row + (x == nx - 1 ? 0 : x + 1);
+    const int ym = (y == 0 ? 0 : y - 1) * nx + x, yp = (y == ny - 1 ? y : y + 1) * nx + x;
     return {u[p], u[P + p], u[ym] + u[yp], u[xm] + u[xp], u[P + ym] + u[P + yp],
             u[P + xm] + u[P + xp]};
 }

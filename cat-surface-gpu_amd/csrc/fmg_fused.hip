@@ -166,28 +166,33 @@ struct LevelTable {
     int nx[kFmgMaxLevels], ny[kFmgMaxLevels];
     unsigned off[kFmgMaxLevels];
     unsigned roff;  // residual buffer
+    // resolved resize taps (TapIdx entries from the tap region): down[l] maps
+    // level l to l + 1 (one x entry per column of l + 1, one y entry per row),
+    // up[l] maps l + 1 to l
+    unsigned down_x[kFmgMaxLevels], down_y[kFmgMaxLevels], up_x[kFmgMaxLevels],
+        up_y[kFmgMaxLevels];
     T w00[kFmgMaxLevels], w01[kFmgMaxLevels], w10[kFmgMaxLevels];
 };
 
 enum CoarseOp : int { kRelax, kDown, kUpAdd, kProlong, kPyramidA, kPyramidB };
 
-// LDS resize of `channels` planes: out = resize(in) [+ out].
+// LDS resize of `channels` planes from resolved taps: out = resize(in) [+ out].
+// Channels are spread over threads as well as pixels.
 template <typename T>
 __device__ __forceinline__ void lds_resize(const T *in, int snx, int sny, T *out, int tnx,
-                                           int tny, int channels, bool accumulate)
+                                           int tny, int channels, bool accumulate,
+                                           const TapIdx<T> *tabx, const TapIdx<T> *taby)
 {
     const int SP = snx * sny, TP = tnx * tny;
-    const float inv = 1.0f / (float)tnx;
-    for (int p = threadIdx.x; p < TP; p += kCoarseThreads) {
-        const DivMod yx = divmod_small(p, tnx, inv);
-        const int ty = yx.q, tx = yx.r;
-        const Taps<T> txp = resize_taps<T>(tx, snx, tnx), typ = resize_taps<T>(ty, sny, tny);
-        for (int c = 0; c < channels; ++c) {
-            const T *f = in + c * SP;
-            T r = resize_at<T>(txp, typ, snx, sny, [&](int x, int y) { return f[y * snx + x]; });
-            T *o = out + c * TP + p;
-            *o = accumulate ? *o + r : r;
-        }
+    const float inv_tp = 1.0f / (float)TP, inv_tnx = 1.0f / (float)tnx;
+    for (int q = threadIdx.x; q < channels * TP; q += kCoarseThreads) {
+        const DivMod cp = divmod_small(q, TP, inv_tp);
+        const DivMod yx = divmod_small(cp.r, tnx, inv_tnx);
+        const T *f = in + cp.q * SP;
+        T r = resize_from<T>(tabx[yx.r], taby[yx.q],
+                             [&](int x, int y) { return f[y * snx + x]; });
+        T *o = out + q;
+        *o = accumulate ? *o + r : r;
     }
 }
 
@@ -196,9 +201,12 @@ __device__ __forceinline__ void lds_resize(const T *in, int snx, int sny, T *out
 // pyramid steps.  Ends with a workgroup barrier.
 template <typename T>
 __device__ __noinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w00, T w01, T w10,
-                                       int onx, int ony, unsigned ooff, unsigned roff, int nit)
+                                       int onx, int ony, unsigned ooff, unsigned roff,
+                                       unsigned tx_off, unsigned ty_off, int nit)
 {
     T *base = reinterpret_cast<T *>(g_lds);
+    const TapIdx<T> *tabx = reinterpret_cast<const TapIdx<T> *>(g_lds) + tx_off;
+    const TapIdx<T> *taby = reinterpret_cast<const TapIdx<T> *>(g_lds) + ty_off;
     const int P = nx * ny, OP = onx * ony;
     T *a = base + off, *b = a + 3 * P, *u = a + 5 * P;
     T *oa = base + ooff, *ob = oa + 3 * OP, *ou = oa + 5 * OP;
@@ -248,19 +256,19 @@ __device__ __noinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w
             r[P + p] = b[P + p] - m.y;
         }
         __syncthreads();
-        lds_resize<T>(r, nx, ny, ob, onx, ony, 2, false);
+        lds_resize<T>(r, nx, ny, ob, onx, ony, 2, false, tabx, taby);
         for (int p = threadIdx.x; p < 2 * OP; p += kCoarseThreads)
             ou[p] = (T)0;
         __syncthreads();
     } else if (op == kUpAdd || op == kProlong) {
-        lds_resize<T>(ou, onx, ony, u, nx, ny, 2, op == kUpAdd);
+        lds_resize<T>(ou, onx, ony, u, nx, ny, 2, op == kUpAdd, tabx, taby);
         __syncthreads();
     } else {
         // pyramid: A (or the original b) of this level from the finer one
         if (op == kPyramidA)
-            lds_resize<T>(oa, onx, ony, a, nx, ny, 3, false);
+            lds_resize<T>(oa, onx, ony, a, nx, ny, 3, false, tabx, taby);
         else
-            lds_resize<T>(ob, onx, ony, b, nx, ny, 2, false);
+            lds_resize<T>(ob, onx, ony, b, nx, ny, 2, false, tabx, taby);
         __syncthreads();
     }
 }
@@ -271,9 +279,23 @@ __device__ __forceinline__ void run_op(int op, int l, const LevelTable<T> &t, in
 {
     // the other level: none for a relaxation (pass l itself), l - 1 for the
     // pyramid steps, l + 1 otherwise
-    const int o = op == kRelax ? l : ((op == kPyramidA || op == kPyramidB) ? l - 1 : l + 1);
+    const bool pyramid = op == kPyramidA || op == kPyramidB;
+    const int o = op == kRelax ? l : (pyramid ? l - 1 : l + 1);
+    // taps: restriction l -> l + 1 (kDown), prolongation l + 1 -> l (kUpAdd,
+    // kProlong), restriction l - 1 -> l (pyramid)
+    unsigned tx = 0, ty = 0;
+    if (op == kDown) {
+        tx = t.down_x[l];
+        ty = t.down_y[l];
+    } else if (op == kUpAdd || op == kProlong) {
+        tx = t.up_x[l];
+        ty = t.up_y[l];
+    } else if (pyramid) {
+        tx = t.down_x[l - 1];
+        ty = t.down_y[l - 1];
+    }
     coarse_op<T>(op, t.nx[l], t.ny[l], t.off[l], t.w00[l], t.w01[l], t.w10[l], t.nx[o], t.ny[o],
-                 t.off[o], t.roff, nit);
+                 t.off[o], t.roff, tx, ty, nit);
 }
 
 template <typename T>
@@ -306,6 +328,38 @@ k_fmg_coarse(CoarseLevels<T> lv, const T *__restrict__ A, const T *__restrict__ 
             }
         }
         tab.roff = off;
+        off += 2u * (unsigned)(lv.nx[c0] * lv.ny[c0]);
+        // tap entries start at the first TapIdx-aligned position after the planes
+        unsigned e = (unsigned)((off * sizeof(T) + sizeof(TapIdx<T>) - 1) / sizeof(TapIdx<T>));
+#pragma unroll
+        for (int l = 0; l < kFmgMaxLevels - 1; ++l) {
+            if (l >= c0 && l < L - 1) {
+                tab.down_x[l] = e;
+                e += (unsigned)lv.nx[l + 1];
+                tab.down_y[l] = e;
+                e += (unsigned)lv.ny[l + 1];
+                tab.up_x[l] = e;
+                e += (unsigned)lv.nx[l];
+                tab.up_y[l] = e;
+                e += (unsigned)lv.ny[l];
+            }
+        }
+    }
+    __syncthreads();
+    // resolve every resize tap of the sub-pyramid once
+    TapIdx<T> *taps = reinterpret_cast<TapIdx<T> *>(g_lds);
+    for (int l = c0; l < last; ++l) {
+        const int fx = tab.nx[l], fy = tab.ny[l], cx = tab.nx[l + 1], cy = tab.ny[l + 1];
+        for (int t = threadIdx.x; t < cx + cy + fx + fy; t += kCoarseThreads) {
+            if (t < cx)
+                taps[tab.down_x[l] + t] = taps_x<T>(t, fx, cx);
+            else if (t < cx + cy)
+                taps[tab.down_y[l] + t - cx] = taps_y<T>(t - cx, fy, cy);
+            else if (t < cx + cy + fx)
+                taps[tab.up_x[l] + t - cx - cy] = taps_x<T>(t - cx - cy, cx, fx);
+            else
+                taps[tab.up_y[l] + t - cx - cy - fx] = taps_y<T>(t - cx - cy - fx, cy, fy);
+        }
     }
     __syncthreads();
     const int P0 = tab.nx[c0] * tab.ny[c0];
@@ -441,8 +495,14 @@ size_t coarse_lds_bytes(const int *nx, const int *ny, int c0, int L)
     size_t px = 0;
     for (int l = c0; l < L; ++l)
         px += (size_t)nx[l] * ny[l];
-    // A, b, u of every level and a residual buffer of level c0's size
-    return (px * 7 + (size_t)2 * nx[c0] * ny[c0]) * sizeof(T);
+    // A, b, u of every level, a residual buffer of level c0's size, then the
+    // resolved resize taps (four tables per level pair)
+    size_t planes = (px * 7 + (size_t)2 * nx[c0] * ny[c0]) * sizeof(T);
+    size_t entries = 0;
+    for (int l = c0; l + 1 < L; ++l)
+        entries += (size_t)nx[l + 1] + ny[l + 1] + nx[l] + ny[l];
+    size_t tap = sizeof(TapIdx<T>);
+    return (planes + tap - 1) / tap * tap + entries * tap;
 }
 
 // ---------------------------------------------------------------- launchers
