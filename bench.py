@@ -9,9 +9,11 @@ the x-mirrored RH copy.  One timed step = LH + RH.  Inputs are resident in
 memory (no file I/O) when the timed region starts.
 
 Prints one JSON line (rank 0).  Besides the wall-clock value it reports the
-roofline of the dominant DARTEL kernel, measured with HIP events on the
-stream it runs on, and a CPU baseline from the project's C restatement of
-the DARTEL step (oracle/), timed on a bounded sample on the host.
+roofline of the dominant hot-path kernel (the timed kernel with the largest
+total time): the library brackets each of its launches inside the timed
+region with HIP events on the stream it runs on and accumulates the launch's
+algorithmic bytes.  A CPU baseline from the project's C restatement of the
+DARTEL step (oracle/) is timed on a bounded sample on the host.
 """
 
 from __future__ import annotations
@@ -47,57 +49,49 @@ def make_inputs(seed: int):
     return pairs
 
 
-def run_step(pairs, dtype: str, mode: str, device):
+def run_step(pairs, dtype: str, mode: str, device, cuda_graph: bool):
     from cat_surface_gpu import run_cat_surface_gpu_pipeline
 
     results = []
     for meshes in pairs:
         res = run_cat_surface_gpu_pipeline(None, None, None, None, "unused.gii", device=device,
                                            dartel_dtype=dtype, steps=2, runs=2, avg=True,
-                                           meshes=meshes, write_output=False)
+                                           meshes=meshes, write_output=False,
+                                           cuda_graph=cuda_graph)
         results.append(res)
     return results
 
 
-def relax_roofline(dtype_name: str, device) -> dict:
-    """Average duration of the finest-level red-black half-sweep (HIP events on
-    the launch stream) against its algorithmic bytes: per half-sweep the active
-    half of the pixels reads A (3 ch), b (2 ch) and its own u (2 ch), reads the
-    other colour's u once (2 ch) and writes u (2 ch): 5.5 values per pixel."""
+def kernel_rooflines() -> list[dict]:
+    """Totals of the library's HIP-event kernel timers since they were enabled:
+    for each timed kernel, its launches, average launch duration and the
+    algorithmic bytes of those launches (what the reference's operator sequence
+    reads and writes once for the same work) over the measured time."""
 
-    import torch
+    import ctypes
 
     from cat_surface_gpu import _lib
 
-    dt = torch.float32 if dtype_name == "float32" else torch.float64
-    nx, ny, nit, reps = 512, 256, 3, 200
-    g = torch.Generator(device="cpu").manual_seed(0)
-    hes = (torch.rand((3, ny, nx), generator=g, dtype=torch.float64) + 1.0).to(dt).to(device)
-    rhs = torch.randn((2, ny, nx), generator=g, dtype=torch.float64).to(dt).to(device)
-    sol = torch.zeros_like(rhs)
     lib = _lib.lib()
-    stream = _lib.stream_handle(device)
-    for _ in range(5):
-        _lib.check(lib.csg_relax_membrane(_lib.ptr(hes), _lib.ptr(rhs), _lib.ptr(sol), 0.5,
-                                          -0.125, -0.125, nit, nx, ny, _lib.dtype_code(dt),
-                                          stream), "relax")
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record()
-    for _ in range(reps):
-        lib.csg_relax_membrane(_lib.ptr(hes), _lib.ptr(rhs), _lib.ptr(sol), 0.5, -0.125, -0.125,
-                               nit, nx, ny, _lib.dtype_code(dt), stream)
-    stop.record()
-    torch.cuda.synchronize()
-    launches = reps * 2 * nit
-    seconds = start.elapsed_time(stop) / 1e3 / launches
-    nbytes = 5.5 * nx * ny * (4 if dt == torch.float32 else 8)
-    achieved = nbytes / seconds / 1e9
-    return {"kernel": "k_relax_color (512x256 half-sweep)", "bound": "hbm",
-            "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-            "avg_launch_us": round(seconds * 1e6, 3), "algorithmic_bytes": nbytes}
+    out = []
+    k = 0
+    while True:
+        name = lib.csg_kernel_timing_name(k)
+        if not name:
+            break
+        ms, nbytes, launches = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        _lib.check(lib.csg_kernel_timing_read(k, ctypes.byref(ms), ctypes.byref(nbytes),
+                                              ctypes.byref(launches)), "kernel timing")
+        if launches.value:
+            achieved = nbytes.value / (ms.value / 1e3) / 1e9
+            out.append({"kernel": name.decode(), "bound": "hbm", "achieved": round(achieved, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                        "launches": launches.value, "total_ms": round(ms.value, 3),
+                        "avg_launch_us": round(ms.value * 1e3 / launches.value, 3),
+                        "algorithmic_bytes_per_launch": round(nbytes.value / launches.value)})
+        k += 1
+    return sorted(out, key=lambda r: -r["total_ms"])
 
 
 def cpu_baseline() -> dict:
@@ -138,6 +132,9 @@ def main() -> int:
     ap.add_argument("--mode", choices=("plain",), default="plain")
     ap.add_argument("--seed", type=int, default=20261205)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cuda-graph", action=argparse.BooleanOptionalAction, default=False,
+                    help="replay the DARTEL solves from captured graphs (the kernels inside a "
+                         "replay are not seen by the kernel timers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: every registration here fits one GPU and this "
@@ -151,24 +148,30 @@ def main() -> int:
     pairs = make_inputs(args.seed)
     log(f"inputs ready in {time.perf_counter() - t:.1f} s")
     for i in range(args.warmup):
-        run_step(pairs, args.dtype, args.mode, device)
+        run_step(pairs, args.dtype, args.mode, device, args.cuda_graph)
         torch.cuda.synchronize()
         log(f"warmup {i} done")
+    from cat_surface_gpu import _lib
+
+    kernel_rooflines()  # clear
     torch.cuda.synchronize()
+    _lib.lib().csg_kernel_timing(1)
     t0 = time.perf_counter()
     stage_sums: dict[str, float] = {}
     for i in range(args.steps):
-        results = run_step(pairs, args.dtype, args.mode, device)
+        results = run_step(pairs, args.dtype, args.mode, device, args.cuda_graph)
         for r in results:
             for key, val in r.timings.items():
                 stage_sums[key] = stage_sums.get(key, 0.0) + val
         log(f"step {i} done")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    _lib.lib().csg_kernel_timing(0)
     ms_per_step = elapsed / args.steps * 1e3
     hemispheres = 2
     value = elapsed / (args.steps * hemispheres)
-    roof = relax_roofline(args.dtype, device)
+    kernels = kernel_rooflines()
+    roof = kernels[0] if kernels else None
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     stages = {k: round(v / (args.steps * hemispheres), 4) for k, v in sorted(stage_sums.items())}
     line = {
@@ -186,8 +189,9 @@ def main() -> int:
         "data": "synthetic (seeded Fibonacci-hull hemispheres, SH features)",
         "config": {"workload": "61,442-vertex hemisphere pair, LH+RH per step, steps=2 runs=2 "
                                "avg, 512x256 map", "mode": args.mode,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "cuda_graph": args.cuda_graph},
         "roofline": roof,
+        "roofline_kernels": kernels,
         "cpu_baseline": cpu,
         "stages_s_per_hemisphere": stages,
     }

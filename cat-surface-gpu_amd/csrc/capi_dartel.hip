@@ -5,6 +5,7 @@
 
 #include "../../include/csg.h"
 #include "dartel_solve.h"
+#include "kernel_timing.h"
 
 namespace csg {
 
@@ -85,6 +86,21 @@ int csg_device_count(int *count)
 int csg_set_fmg_schedule(int fused)
 {
     return set_fused_fmg_schedule(fused);
+}
+
+int csg_kernel_timing(int enable)
+{
+    return timing_enable(enable);
+}
+
+int csg_kernel_timing_read(int kernel, double *ms, double *bytes, long long *launches)
+{
+    return timing_read(kernel, ms, bytes, launches);
+}
+
+const char *csg_kernel_timing_name(int kernel)
+{
+    return timed_kernel_name(kernel);
 }
 
 int csg_sample_field(const void *field, int nx, int ny, const void *x, const void *y, size_t n,

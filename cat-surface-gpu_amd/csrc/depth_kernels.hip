@@ -14,6 +14,7 @@
 //
 // Reference semantics: src/cat_surface_gpu/rotation_feature.py:24-317.
 #include "depth_kernels.h"
+#include "kernel_timing.h"
 
 namespace csg {
 
@@ -458,8 +459,13 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
                     (void *)&rhs_c,  (void *)&diag_c,  (void *)&width,    (void *)&omega,
                     (void *)&n,      (void *)&batch,   (void *)&max_iter, (void *)&x,
                     (void *)&bar,    (void *)&timeout};
+    // per row and sweep: the row's ELL values and columns (12 B per slot), its
+    // x gathers (8 B per slot), rhs, diagonal and the x update (24 B)
+    TimedLaunch timer(kTimedSor, s,
+                      (double)max_iter * batch * n * (20.0 * W + 24.0));
     CSG_HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_sor_persistent, dim3(grid),
                                              dim3(kSorBlock), args, 0, s));
+    timer.done();
     return kOk;
 }
 

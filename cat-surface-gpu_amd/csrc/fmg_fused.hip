@@ -17,6 +17,7 @@
 //     workgroup barriers between steps.
 #include "dartel_kernels.h"
 #include "fmg_device.h"
+#include "kernel_timing.h"
 
 #include <mutex>
 
@@ -505,6 +506,34 @@ size_t coarse_lds_bytes(const int *nx, const int *ny, int c0, int L)
     return (planes + tap - 1) / tap * tap + entries * tap;
 }
 
+// Values the reference's operator sequence reads and writes once for the
+// coarse levels of one call (per job): level c0 in and out, the pyramids,
+// the coarse solve, then per operator: relax_membrane 9·nit, residual and
+// restriction 12, prolongation-add 4.5, prolongation 2.5 (per fine pixel).
+template <typename T>
+static double coarse_reference_values(const CoarseLevels<T> &lv, int nit, int cycles, int init)
+{
+    auto M = [&](int l) { return (double)lv.nx[l] * lv.ny[l]; };
+    const int c0 = lv.c0, last = lv.L - 1;
+    double v = 7.0 * M(c0) + 2.0 * M(c0);
+    for (int l = c0 + 1; l <= last; ++l)
+        v += (init ? 5.0 : 3.0) * (M(l - 1) + M(l));
+    auto cycle = [&](int start) {
+        double c = 9.0 * nit * M(last);
+        for (int l = start; l < last; ++l)
+            c += 2.0 * 9.0 * nit * M(l) + 12.0 * M(l) + 4.5 * M(l);
+        return c;
+    };
+    if (init) {
+        v += 7.0 * M(last) + 9.0 * nit * M(last);
+        for (int level = last - 1; level >= c0; --level)
+            v += 2.5 * M(level) + cycles * cycle(level);
+    } else {
+        v += cycles * cycle(c0);
+    }
+    return v;
+}
+
 // ---------------------------------------------------------------- launchers
 
 template <typename T>
@@ -519,6 +548,11 @@ int launch_relax_tile(const T *a, const T *b, const T *uin, T *uout, const T *uc
         return fail(kInvalidArgument, "tiled relaxation needs %zu bytes of LDS", lds);
     const int tiles_x = (nx + kTileX - 1) / kTileX, tiles_y = (ny + kTileY - 1) / kTileY;
     dim3 grid(tiles_x * tiles_y, batch);
+    // reference bytes: 9 values per pixel per red+black iteration, plus the
+    // prolongation u += resize(u_c) (reads u_c and u, writes u)
+    const double P = (double)nx * ny;
+    TimedLaunch timer(kTimedRelaxTile, s,
+                      (9.0 * (sweeps / 2.0) + (uc ? 4.5 : 0.0)) * P * sizeof(T) * batch);
     if (sweeps == 6)  // nit = 3, the reference default
         hipLaunchKernelGGL((k_relax_tile<T, 6>), grid, dim3(kTileThreads), lds, s, a, b, uin, uout,
                            uc, cnx, cny, nx, ny, (T)w.w00, (T)w.w01, (T)w.w10, sweeps, tiles_x,
@@ -528,6 +562,7 @@ int launch_relax_tile(const T *a, const T *b, const T *uin, T *uout, const T *uc
                            uc, cnx, cny, nx, ny, (T)w.w00, (T)w.w01, (T)w.w10, sweeps, tiles_x,
                            astride, bstride, cstride);
     CSG_LAUNCH_CHECK();
+    timer.done();
     return kOk;
 }
 
@@ -537,9 +572,13 @@ int launch_restrict_residual(const T *a, const T *b, const T *u, MembraneWeights
                              size_t cstride, T *bc, T *uc, hipStream_t s)
 {
     dim3 g(grid_for((size_t)cnx * cny), batch);
+    // reference bytes: apply_membrane (A, u in, R out: 7), b (2), restriction
+    // (R in: 2, b_c out: 0.5) and the zeroed u_c (0.5) per fine pixel
+    TimedLaunch timer(kTimedRestrict, s, 12.0 * nx * ny * sizeof(T) * batch);
     hipLaunchKernelGGL(k_restrict_residual<T>, g, dim3(kBlock), 0, s, a, b, u, nx, ny, cnx, cny,
                        (T)w.w00, (T)w.w01, (T)w.w10, astride, bstride, cstride, bc, uc);
     CSG_LAUNCH_CHECK();
+    timer.done();
     return kOk;
 }
 
@@ -550,9 +589,12 @@ int launch_fmg_coarse(const CoarseLevels<T> &lv, const T *a, const T *b, T *u, s
 {
     if (lds_bytes > fmg_lds_budget())
         return fail(kInvalidArgument, "coarse multigrid levels need %zu bytes of LDS", lds_bytes);
+    TimedLaunch timer(kTimedFmgCoarse, s,
+                      coarse_reference_values(lv, nit, cycles, init) * sizeof(T) * batch);
     hipLaunchKernelGGL(k_fmg_coarse<T>, dim3(batch), dim3(kCoarseThreads), lds_bytes, s, lv, a,
                        b, u, astride, bstride, id, nit, cycles, init);
     CSG_LAUNCH_CHECK();
+    timer.done();
     return kOk;
 }
 

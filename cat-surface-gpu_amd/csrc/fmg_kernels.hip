@@ -6,6 +6,7 @@
 // :1018-1070 (membrane system), :1073-1208 (relaxation), :1211-1417 (FMG).
 #include "dartel_kernels.h"
 #include "fmg_device.h"
+#include "kernel_timing.h"
 
 namespace csg {
 
@@ -164,6 +165,8 @@ int launch_relax_sweep(const T *a, const T *b, T *u, MembraneWeights w, int nx, 
 {
     int half = (nx + 1) / 2;
     dim3 g(grid_for((size_t)half * ny), batch);
+    // reference bytes: half of the 9 values per pixel of a red+black iteration
+    TimedLaunch timer(kTimedRelaxSweep, s, 4.5 * nx * ny * sizeof(T) * batch);
     hipLaunchKernelGGL(k_relax_color<T>, g, dim3(kBlock), 0, s, a, b, u, (T)w.w00, (T)w.w01,
                        (T)w.w10, nx, ny, astride, bstride, color, 0);
     CSG_LAUNCH_CHECK();
@@ -173,6 +176,7 @@ int launch_relax_sweep(const T *a, const T *b, T *u, MembraneWeights w, int nx, 
                            (T)w.w10, nx, ny, astride, bstride, color, 1);
         CSG_LAUNCH_CHECK();
     }
+    timer.done();
     return kOk;
 }
 

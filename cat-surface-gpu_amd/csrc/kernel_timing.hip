@@ -1,0 +1,134 @@
+// HIP-event timing of the hot kernels; see kernel_timing.h.
+#include "kernel_timing.h"
+
+#include <atomic>
+#include <mutex>
+#include <vector>
+
+namespace csg {
+
+namespace {
+
+struct Pending {
+    hipEvent_t a, b;
+    double bytes;
+};
+
+struct TimerState {
+    std::mutex m;
+    std::vector<hipEvent_t> pool;
+    std::vector<Pending> pending[kTimedCount];
+};
+
+std::atomic<int> g_on{0};
+TimerState &state()
+{
+    static TimerState st;
+    return st;
+}
+
+hipEvent_t take_event()
+{
+    TimerState &st = state();
+    {
+        std::lock_guard<std::mutex> lock(st.m);
+        if (!st.pool.empty()) {
+            hipEvent_t e = st.pool.back();
+            st.pool.pop_back();
+            return e;
+        }
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return e;
+}
+
+bool capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return true;  // unknown: do not record
+    }
+    return st != hipStreamCaptureStatusNone;
+}
+
+const char *kNames[kTimedCount] = {"k_relax_tile", "k_fmg_coarse", "k_restrict_residual",
+                                   "k_relax_color", "k_sor_persistent"};
+
+}  // namespace
+
+const char *timed_kernel_name(int id)
+{
+    return (id >= 0 && id < kTimedCount) ? kNames[id] : nullptr;
+}
+
+int timing_enable(int on)
+{
+    return g_on.exchange(on ? 1 : 0);
+}
+
+TimedLaunch::TimedLaunch(int id, hipStream_t s, double bytes) : id_(id), s_(s), bytes_(bytes)
+{
+    if (!g_on.load(std::memory_order_relaxed) || capturing(s))
+        return;
+    a_ = take_event();
+    if (a_ && hipEventRecord(a_, s) != hipSuccess) {
+        (void)hipGetLastError();
+        a_ = nullptr;
+    }
+}
+
+void TimedLaunch::done()
+{
+    if (!a_)
+        return;
+    hipEvent_t b = take_event();
+    if (!b || hipEventRecord(b, s_) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    TimerState &st = state();
+    std::lock_guard<std::mutex> lock(st.m);
+    st.pending[id_].push_back({a_, b, bytes_});
+    a_ = nullptr;
+}
+
+int timing_read(int id, double *ms, double *bytes, long long *launches)
+{
+    if (id < 0 || id >= kTimedCount)
+        return fail(kInvalidArgument, "no timed kernel %d", id);
+    std::vector<Pending> list;
+    TimerState &st = state();
+    {
+        std::lock_guard<std::mutex> lock(st.m);
+        list.swap(st.pending[id]);
+    }
+    double t = 0.0, by = 0.0;
+    for (const Pending &p : list) {
+        CSG_HIP_CHECK(hipEventSynchronize(p.b));
+        float e = 0.0f;
+        CSG_HIP_CHECK(hipEventElapsedTime(&e, p.a, p.b));
+        t += e;
+        by += p.bytes;
+    }
+    {
+        std::lock_guard<std::mutex> lock(st.m);
+        for (const Pending &p : list) {
+            st.pool.push_back(p.a);
+            st.pool.push_back(p.b);
+        }
+    }
+    if (ms)
+        *ms = t;
+    if (bytes)
+        *bytes = by;
+    if (launches)
+        *launches = (long long)list.size();
+    return kOk;
+}
+
+}  // namespace csg

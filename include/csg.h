@@ -41,6 +41,16 @@ int csg_device_count(int *count);
  * in the same order.  Process-wide; returns the previous setting.  No counterpart in
  * the reference (it has only the per-operator schedule of dartel_grid.py:1211-1417). */
 int csg_set_fmg_schedule(int fused);
+/* Optional HIP-event timing of the hot kernels for roofline reports (no counterpart in the
+ * reference).  csg_kernel_timing(1) brackets every later launch of a timed kernel made
+ * outside stream capture with two events on its stream and accumulates the launch's
+ * algorithmic bytes (what the reference's operator sequence reads and writes once for it);
+ * returns the previous setting.  csg_kernel_timing_read synchronises on the recorded events
+ * and returns (then clears) the totals of kernel `kernel` (0 .. while csg_kernel_timing_name
+ * returns non-NULL). */
+int csg_kernel_timing(int enable);
+int csg_kernel_timing_read(int kernel, double *ms, double *bytes, long long *launches);
+const char *csg_kernel_timing_name(int kernel);
 
 /* ------------------------------------------------------------ DARTEL grid */
 
