@@ -8,9 +8,10 @@
 //               sum of its contributions in COO order), Laplacian·points,
 //               mean curvature, right-hand side, diagonal;
 //   solve:      colour-ordered SOR, all max_iter × colours sweeps in ONE
-//               persistent cooperative launch with a grid barrier between
-//               colours (the sweep order of the reference, 1000 × ~7 colours,
-//               would otherwise be ~7000 dependent launches).
+//               persistent launch (grid capped at what the device holds at
+//               once) with a grid barrier between colours (the sweep order of
+//               the reference, 1000 × ~7 colours, would otherwise be ~7000
+//               dependent launches).
 //
 // Reference semantics: src/cat_surface_gpu/rotation_feature.py:24-317.
 #include "depth_kernels.h"
@@ -258,7 +259,7 @@ k_colour_order(const int *__restrict__ verts, const int *__restrict__ ell_src, i
     }
 }
 
-// Grid-wide barrier for a co-resident (cooperative) grid, as a two-level
+// Grid-wide barrier for a co-resident grid, as a two-level
 // tree: blocks arrive in groups of kBarrierGroup on their own counter, the
 // last arrival of each group arrives at the root, and the last root arrival
 // advances the generation word that everyone waits on.  Arrivals on one
@@ -345,6 +346,78 @@ k_sor_persistent(const int *__restrict__ verts, SorColours co, const int *__rest
         }
     }
 }
+
+// The same sweeps with every row a thread owns held in registers: when the
+// grid has at least as many threads as the largest colour has rows (times the
+// batch), thread t owns row t of every colour, so its ELL values, columns,
+// rhs and diagonal are loaded once and a sweep's only memory traffic is the x
+// gathers and the x update.  Identical arithmetic and order as
+// k_sor_persistent.  CMAX / WMAX bound the colours and the ELL width; the
+// block is two waves per SIMD so that the rows fit in registers.
+constexpr int kSorRegBlock = 512;
+constexpr int kSorRegColours = 7, kSorRegWidth = 7;
+template <int CMAX, int WMAX>
+__global__ void __launch_bounds__(kSorRegBlock)
+k_sor_registers(const int *__restrict__ verts, SorColours co, const int *__restrict__ ell_cols,
+                const double *__restrict__ ell_val, const double *__restrict__ rhs_c,
+                const double *__restrict__ diag_c, int W, double omega, int n, int batch,
+                int max_iter, double *x, unsigned *bar, int *timeout)
+{
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t wn = (size_t)W * n;
+    bool act[CMAX];
+    int vi[CMAX];
+    int col[CMAX][WMAX];
+    double val[CMAX][WMAX], rh[CMAX], dg[CMAX];
+#pragma unroll
+    for (int c = 0; c < CMAX; ++c) {
+        act[c] = false;
+        vi[c] = 0;
+        rh[c] = 0.0;
+        dg[c] = 1.0;
+#pragma unroll
+        for (int k = 0; k < WMAX; ++k) {
+            col[c][k] = 0;
+            val[c][k] = 0.0;
+        }
+        if (c < co.count) {
+            const int start = co.off[c], cnt = co.off[c + 1] - start;
+            if (tid < cnt * batch) {
+                const int b = tid / cnt;
+                const int r = start + (tid - b * cnt);
+                act[c] = true;
+                vi[c] = b * n + verts[r];
+                rh[c] = rhs_c[(size_t)b * n + r];
+                dg[c] = diag_c[(size_t)b * n + r];
+#pragma unroll
+                for (int k = 0; k < WMAX; ++k) {
+                    if (k < W) {
+                        val[c][k] = ell_val[(size_t)b * wn + (size_t)k * n + r];
+                        col[c][k] = b * n + ell_cols[(size_t)k * n + r];
+                    }
+                }
+            }
+        }
+    }
+    for (int it = 0; it < max_iter; ++it) {
+#pragma unroll
+        for (int c = 0; c < CMAX; ++c) {
+            if (c >= co.count)
+                break;
+            if (act[c]) {
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k < WMAX; ++k)
+                    if (k < W)
+                        s += val[c][k] * x[col[c][k]];
+                x[vi[c]] = x[vi[c]] + omega * (rh[c] - s) / dg[c];
+            }
+            if (!grid_barrier(bar, gridDim.x, timeout))
+                return;
+        }
+    }
+}
+
 
 // r = rhs - A x; per-block Σ r², Σ rhs².
 __global__ void __launch_bounds__(kBlock)
@@ -440,6 +513,13 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     for (int c = 0; c < co.count; ++c)
         most = (co.off[c + 1] - co.off[c]) > most ? (co.off[c + 1] - co.off[c]) : most;
     int want = (int)grid_for((size_t)most * a.batch, kSorBlock);
+    int reg_per_cu = 0;
+    CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &reg_per_cu, k_sor_registers<kSorRegColours, kSorRegWidth>, kSorRegBlock, 0));
+    const int reg_grid = (int)grid_for((size_t)most * a.batch, kSorRegBlock);
+    const bool registers = co.count <= kSorRegColours && W <= kSorRegWidth &&
+                           reg_grid <= cus * reg_per_cu &&
+                           reg_grid <= (kBarrierSlots - 3) * kBarrierGroup;
     int cap = cus * (per_cu > 0 ? per_cu : 1);
     // the barrier state holds one slot per group, the root and the generation word
     int slots_cap = (kBarrierSlots - 3) * kBarrierGroup;
@@ -455,16 +535,22 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     double *x = a.x;
     unsigned *bar = a.barrier;
     SorColours colours = co;
-    void *args[] = {(void *)&verts,  (void *)&colours, (void *)&ell_cols, (void *)&ell_val,
-                    (void *)&rhs_c,  (void *)&diag_c,  (void *)&width,    (void *)&omega,
-                    (void *)&n,      (void *)&batch,   (void *)&max_iter, (void *)&x,
-                    (void *)&bar,    (void *)&timeout};
     // per row and sweep: the row's ELL values and columns (12 B per slot), its
     // x gathers (8 B per slot), rhs, diagonal and the x update (24 B)
     TimedLaunch timer(kTimedSor, s,
                       (double)max_iter * batch * n * (20.0 * W + 24.0));
-    CSG_HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_sor_persistent, dim3(grid),
-                                             dim3(kSorBlock), args, 0, s));
+    // A regular launch: the grid is capped at what the device holds at once
+    // (occupancy x CUs), and every barrier wait is bounded, so blocks that were
+    // not resident turn into a timeout error rather than a hang.
+    if (registers)
+        hipLaunchKernelGGL((k_sor_registers<kSorRegColours, kSorRegWidth>), dim3(reg_grid),
+                           dim3(kSorRegBlock), 0, s, verts, colours, ell_cols, ell_val, rhs_c,
+                           diag_c, width, omega, n, batch, max_iter, x, bar, timeout);
+    else
+        hipLaunchKernelGGL(k_sor_persistent, dim3(grid), dim3(kSorBlock), 0, s, verts, colours,
+                           ell_cols, ell_val, rhs_c, diag_c, width, omega, n, batch, max_iter,
+                           x, bar, timeout);
+    CSG_LAUNCH_CHECK();
     timer.done();
     return kOk;
 }

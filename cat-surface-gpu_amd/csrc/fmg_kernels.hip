@@ -128,7 +128,18 @@ __global__ void k_coarse_solve(const T *__restrict__ a, const T *__restrict__ b,
     u[P + p] = (T)((-cr * bx + dx * by) / det);
 }
 
-// flags[b] = any(u[b] != 0); one block per batch element.
+// flags[b] = any(u[b] != 0): flags are cleared first, then every block that
+// finds a non-zero value in its chunk stores 1 (all writers store the same
+// value).
+__global__ void k_clear_flags(int *__restrict__ flags, int batch)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < batch)
+        flags[b] = 0;
+}
+
+constexpr int kNonzeroChunk = kBlock * 8;
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_any_nonzero(const T *__restrict__ u, size_t n, size_t stride, int *__restrict__ flags)
@@ -137,15 +148,17 @@ k_any_nonzero(const T *__restrict__ u, size_t n, size_t stride, int *__restrict_
     if (threadIdx.x == 0)
         found = 0;
     __syncthreads();
-    u += blockIdx.x * stride;
+    u += blockIdx.y * stride;
+    const size_t begin = (size_t)blockIdx.x * kNonzeroChunk;
+    const size_t end = begin + kNonzeroChunk < n ? begin + kNonzeroChunk : n;
     int local = 0;
-    for (size_t i = threadIdx.x; i < n; i += blockDim.x)
+    for (size_t i = begin + threadIdx.x; i < end; i += blockDim.x)
         local |= (u[i] != (T)0);
     if (local)
         found = 1;
     __syncthreads();
-    if (threadIdx.x == 0)
-        flags[blockIdx.x] = found;
+    if (threadIdx.x == 0 && found)
+        flags[blockIdx.y] = 1;
 }
 
 template <typename T>
@@ -219,7 +232,12 @@ template <typename T>
 int launch_any_nonzero(const T *u, size_t n, int batch, size_t stride, int *flags,
                        hipStream_t s)
 {
-    hipLaunchKernelGGL(k_any_nonzero<T>, dim3(batch), dim3(kBlock), 0, s, u, n, stride, flags);
+    hipLaunchKernelGGL(k_clear_flags, dim3(grid_for((size_t)batch)), dim3(kBlock), 0, s, flags,
+                       batch);
+    CSG_LAUNCH_CHECK();
+    const unsigned chunks = (unsigned)((n + kNonzeroChunk - 1) / kNonzeroChunk);
+    hipLaunchKernelGGL(k_any_nonzero<T>, dim3(chunks > 0 ? chunks : 1, batch), dim3(kBlock), 0,
+                       s, u, n, stride, flags);
     CSG_LAUNCH_CHECK();
     return kOk;
 }

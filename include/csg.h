@@ -213,7 +213,7 @@ int csg_type5_sheets(const float *geom, int n, const int *faces, const int *nbr_
  * the device array colour_verts.  ell_cols / ell_src ([ell_width][n], solve order: row r
  * is vertex colour_verts[r]) give each row's merged entries (column, merged index or -1
  * for padding) in CSR order.  All max_iter x ncolours sweeps run in one persistent
- * cooperative launch; the call synchronises once at the end.
+ * launch; the call synchronises once at the end.
  * Replaces rotation_feature._solve_depth_potential_colored_sor
  * (src/cat_surface_gpu/rotation_feature.py:253-317). */
 size_t csg_depth_workspace_bytes(int n, int nf, int nnz, int batch, int ell_width);
