@@ -50,16 +50,20 @@ def make_inputs(seed: int):
 
 
 def run_step(pairs, dtype: str, mode: str, device, cuda_graph: bool):
-    from cat_surface_gpu import run_cat_surface_gpu_pipeline
+    """One step: LH and RH.  "plain" registers them one after the other (the
+    reference's pipeline, one job per call); "batched" schedules both jobs
+    together (run_cat_surface_gpu_pipelines: shared feature solve and batched
+    DARTEL rounds, identical results)."""
 
-    results = []
-    for meshes in pairs:
-        res = run_cat_surface_gpu_pipeline(None, None, None, None, "unused.gii", device=device,
-                                           dartel_dtype=dtype, steps=2, runs=2, avg=True,
-                                           meshes=meshes, write_output=False,
-                                           cuda_graph=cuda_graph)
-        results.append(res)
-    return results
+    from cat_surface_gpu import (PipelineJob, run_cat_surface_gpu_pipeline,
+                                 run_cat_surface_gpu_pipelines)
+
+    kw = dict(device=device, dartel_dtype=dtype, steps=2, runs=2, avg=True, write_output=False,
+              cuda_graph=cuda_graph)
+    if mode == "batched":
+        return run_cat_surface_gpu_pipelines([PipelineJob(meshes=m) for m in pairs], **kw)
+    return [run_cat_surface_gpu_pipeline(None, None, None, None, "unused.gii", meshes=m, **kw)
+            for m in pairs]
 
 
 def kernel_rooflines() -> list[dict]:
@@ -129,10 +133,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--dtype", choices=("float32", "float64"), default="float32")
-    ap.add_argument("--mode", choices=("plain",), default="plain")
+    ap.add_argument("--mode", choices=("plain", "batched"), default="batched")
     ap.add_argument("--seed", type=int, default=20261205)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cuda-graph", action=argparse.BooleanOptionalAction, default=False,
+    ap.add_argument("--cuda-graph", action=argparse.BooleanOptionalAction, default=True,
                     help="replay the DARTEL solves from captured graphs (the kernels inside a "
                          "replay are not seen by the kernel timers)")
     args = ap.parse_args()
@@ -160,7 +164,8 @@ def main() -> int:
     stage_sums: dict[str, float] = {}
     for i in range(args.steps):
         results = run_step(pairs, args.dtype, args.mode, device, args.cuda_graph)
-        for r in results:
+        # batched jobs share their stage timings: count them once
+        for r in (results[:1] if args.mode == "batched" else results):
             for key, val in r.timings.items():
                 stage_sums[key] = stage_sums.get(key, 0.0) + val
         log(f"step {i} done")

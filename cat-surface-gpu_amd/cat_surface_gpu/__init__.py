@@ -43,6 +43,8 @@ from .surface_warp import (
     apply_flow_to_sphere,
     apply_flow_to_stenciled_sphere,
     solve_dartel_from_surfaces,
+    solve_dartel_from_surfaces_jobs,
+    solve_dartel_maps_jobs,
 )
 from .surface_stencil import SurfaceStencil, SurfaceStencilDevice
 from .surface_rotation import (
@@ -62,7 +64,9 @@ from .rotation_pipeline import (
 from .gpu_pipeline import (
     CatSurfaceGpuResult,
     GiftiMesh,
+    PipelineJob,
     read_gifti_mesh,
     run_cat_surface_gpu_pipeline,
+    run_cat_surface_gpu_pipelines,
     write_gifti_mesh,
 )

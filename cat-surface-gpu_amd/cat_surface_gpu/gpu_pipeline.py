@@ -31,7 +31,7 @@ from .rotation_pipeline import RotationPipeline, read_rotation_values
 from .stencil_builder import build_device_stencil
 from .surface_rotation import rotation_matrix_np
 from .surface_stencil import SurfaceStencil
-from .surface_warp import apply_flow_to_stenciled_sphere, solve_dartel_from_surfaces
+from .surface_warp import apply_flow_to_stenciled_sphere, solve_dartel_from_surfaces_jobs
 
 
 @dataclass(frozen=True)
@@ -137,6 +137,24 @@ def _external_stencil(builder, sphere_path, out_path: Path, options: Sequence[st
     return SurfaceStencil.from_file(out_path).to(device)
 
 
+@dataclass
+class PipelineJob:
+    """One registration of run_cat_surface_gpu_pipelines: the four input surfaces
+    (GIFTI paths, or ``meshes`` = (source surface, source sphere, target
+    surface, target sphere) already in memory), the output path and optional
+    per-job stencil / rotation-value files."""
+
+    source_surface: str | Path | None = None
+    source_sphere: str | Path | None = None
+    target_surface: str | Path | None = None
+    target_sphere: str | Path | None = None
+    output: str | Path | None = None
+    meshes: tuple | None = None
+    source_stencil: str | Path | None = None
+    target_stencil: str | Path | None = None
+    rotation_values: str | Path | None = None
+
+
 def run_cat_surface_gpu_pipeline(
     source_surface_path, source_sphere_path, target_surface_path, target_sphere_path,
     output_path, *, source_stencil_path=None, target_stencil_path=None,
@@ -164,6 +182,82 @@ def run_cat_surface_gpu_pipeline(
     paths; ``write_output=False`` skips the output file.
     """
 
+    job = PipelineJob(source_surface_path, source_sphere_path, target_surface_path,
+                      target_sphere_path, output_path, meshes, source_stencil_path,
+                      target_stencil_path, rotation_values_path)
+    return run_cat_surface_gpu_pipelines(
+        [job], rotation_values_probe=rotation_values_probe,
+        rotation_geometry_probe=rotation_geometry_probe,
+        rotation_depth_probe=rotation_depth_probe, stencil_builder=stencil_builder,
+        rotated_stencil_builder=rotated_stencil_builder, stencil_threads=stencil_threads,
+        device=device, kernel=kernel, dartel_dtype=dartel_dtype,
+        squaring_kernel=squaring_kernel, steps=steps, runs=runs, avg=avg, curvtypes=curvtypes,
+        fwhm=fwhm, loop=loop, cycles=cycles, nit=nit, its=its, code=code,
+        point_chunk=point_chunk, rotation_grid_size=rotation_grid_size,
+        rotation_margin=rotation_margin, rotation_refine=rotation_refine,
+        rotation_max_iter=rotation_max_iter, rotation_tol=rotation_tol,
+        rotation_simplex_step=rotation_simplex_step,
+        rotation_feature_backend=rotation_feature_backend,
+        rotation_feature_solver=rotation_feature_solver, parallel_sides=parallel_sides,
+        optimized_dartel=optimized_dartel, cuda_graph=cuda_graph,
+        rotation_search=rotation_search, write_output=write_output)[0]
+
+
+@dataclass
+class _JobState:
+    job: PipelineJob
+    src_surface: GiftiMesh
+    src_sphere: GiftiMesh
+    tgt_surface: GiftiMesh
+    tgt_sphere: GiftiMesh
+    timings: dict
+    src_stencil: object = None
+    tgt_stencil: object = None
+    src_vals: object = None
+    tgt_vals: object = None
+    angle: np.ndarray | None = None
+    rotation_cost: float = 0.0
+    current_sphere: np.ndarray | None = None
+    cur_stencil: object = None
+    avg_tgt_surface: np.ndarray | None = None
+    avg_tgt_stencil: object = None
+    avg_src_surface: np.ndarray | None = None
+    avg_src_sphere: np.ndarray | None = None
+    avg_src_stencil: object = None
+    final_prewarp: np.ndarray | None = None
+    last_flow: object = None
+    avg_flow: object = None
+    src_dev: object = None
+    tgt_dev: object = None
+
+
+def run_cat_surface_gpu_pipelines(
+    jobs: Sequence[PipelineJob], *, rotation_values_probe=None, rotation_geometry_probe=None,
+    rotation_depth_probe=None, stencil_builder=None, rotated_stencil_builder=None,
+    stencil_threads: int = 8, device: str | torch.device = "cuda", kernel: str = "hip",
+    dartel_dtype: str | torch.dtype = torch.float64, squaring_kernel: str | None = None,
+    steps: int = 2, runs: int = 1, avg: bool = False, curvtypes: Sequence[int] = (5, 5, 2),
+    fwhm: Sequence[float] = (5.0, 5.0 / 3.0, 5.0 / 9.0), loop: int = 6, cycles: int = 3,
+    nit: int = 3, its: int = 3, code: int = 1, point_chunk: int = 4096,
+    rotation_grid_size: int = 128, rotation_margin: int = 1, rotation_refine: bool = True,
+    rotation_max_iter: int = 500, rotation_tol: float = 1.0e-4,
+    rotation_simplex_step: float = 0.1, rotation_feature_backend: str = "auto",
+    rotation_feature_solver: str = "colored-sor", parallel_sides: bool = True,
+    optimized_dartel: bool = True, cuda_graph: bool = True,
+    rotation_search=None, write_output: bool = True,
+) -> list[CatSurfaceGpuResult]:
+    """Several independent CAT_SurfWarp registrations, stage by stage.
+
+    Each job follows run_cat_surface_gpu_pipeline exactly (same stages, same
+    results); what is shared is the schedule: the rotation features of every
+    job and side are solved together, and every DARTEL round solves all jobs
+    (and, in the last run, each job's main and average solves, which are
+    independent: gpu_pipeline.py:1133-1224) in one batched solve.  Stage
+    timings are those of the shared stages (the same for every job).
+    """
+
+    if not jobs:
+        return []
     if steps < 1 or steps > 3:
         raise ValueError("steps must be between 1 and 3")
     if runs < 1:
@@ -172,6 +266,11 @@ def run_cat_surface_gpu_pipeline(
         raise ValueError("stencil_threads must be between 1 and 256")
     if len(curvtypes) < steps or len(fwhm) < steps:
         raise ValueError("curvtypes and fwhm must contain at least 'steps' entries")
+    for curvtype in curvtypes[:steps]:
+        if int(curvtype) not in (2, 5):
+            raise NotImplementedError("only upstream curvtype 2 and 5 exist")
+        if int(curvtype) != 5:
+            raise NotImplementedError("this build implements curvtype 5 (steps <= 2)")
     if rotation_max_iter < 1 or rotation_tol < 0.0 or rotation_simplex_step <= 0.0:
         raise ValueError("rotation_max_iter, rotation_tol, and rotation_simplex_step must be "
                          "positive")
@@ -184,69 +283,84 @@ def run_cat_surface_gpu_pipeline(
     solve_dtype = _resolve_dartel_dtype(dartel_dtype)
     if squaring_kernel not in {None, "auto", "hip", "torch", "triton"}:
         raise ValueError("squaring_kernel must be None, 'auto', or 'hip'")
+    from .dartel_grid import _check_kernel
+
+    _check_kernel(kernel)
+    if squaring_kernel is not None:
+        _check_kernel(squaring_kernel)
     if rotation_feature_backend == "auto":
         rotation_feature_backend = "official-cpu" if rotation_values_probe is not None else "cuda"
     if device == "cpu" or str(device) == "cpu":
         raise ValueError("The explicit GPU pipeline requires a HIP device")
     dev = resolve_device(device)
     graph_requested = bool(cuda_graph and (runs + int(avg) >= 3))
+    pole = np.asarray((0.0, np.pi / 2.0, 0.0), dtype=np.float64)
 
     t_total = time.perf_counter()
-    timings: dict[str, float] = {}
-    if meshes is not None:
-        src_surface, src_sphere, tgt_surface, tgt_sphere = meshes
-    else:
-        src_surface = read_gifti_mesh(source_surface_path)
-        src_sphere = read_gifti_mesh(source_sphere_path)
-        tgt_surface = read_gifti_mesh(target_surface_path)
-        tgt_sphere = read_gifti_mesh(target_sphere_path)
-    timings["input_read_seconds"] = time.perf_counter() - t_total
+    shared: dict[str, float] = {}
+    states: list[_JobState] = []
+    for job in jobs:
+        if job.meshes is not None:
+            meshes = job.meshes
+        else:
+            meshes = (read_gifti_mesh(job.source_surface), read_gifti_mesh(job.source_sphere),
+                      read_gifti_mesh(job.target_surface), read_gifti_mesh(job.target_sphere))
+        states.append(_JobState(job, *meshes, timings={}))
+    shared["input_read_seconds"] = time.perf_counter() - t_total
 
     with tempfile.TemporaryDirectory(prefix="cat_surface_gpu_") as tmp_name:
         tmp = Path(tmp_name)
 
-        def stencil(sphere_vertices, sphere_path, surface, label, normalise, path=None,
-                    builder=None):
+        def stencil(st: _JobState, index: int, sphere_vertices, sphere_path, label, normalise,
+                    path=None, builder=None):
+            source_side = label.startswith(("source", "rotated", "avg-source"))
+            faces = st.src_sphere.faces if source_side else st.tgt_sphere.faces
             if path is not None:
                 if not Path(path).is_file():
                     raise FileNotFoundError(f"{label} does not exist: {path}")
                 return SurfaceStencil.from_file(path).to(dev)
             if builder is not None:
                 if sphere_path is None:
-                    sphere_path = tmp / f"{label}.gii"
-                    write_gifti_surface(sphere_path, sphere_vertices, src_sphere.faces
-                                        if label.startswith(("source", "rotated", "avg-source"))
-                                        else tgt_sphere.faces)
+                    sphere_path = tmp / f"job{index}-{label}.gii"
+                    write_gifti_surface(sphere_path, sphere_vertices, faces)
                 opts = ([] if normalise else ["--no-normalize"]) + [
                     "--threads", str(stencil_threads)]
-                return _external_stencil(builder, sphere_path, tmp / f"{label}.stencil", opts,
-                                         dev)
-            faces = (src_sphere.faces if label.startswith(("source", "rotated", "avg-source"))
-                     else tgt_sphere.faces)
+                return _external_stencil(builder, sphere_path,
+                                         tmp / f"job{index}-{label}.stencil", opts, dev)
+            surface = st.src_surface.vertices if source_side else st.tgt_surface.vertices
             return build_device_stencil(sphere_vertices, faces, surface_vertices=surface,
                                         normalise=normalise, device=dev)
 
         t = time.perf_counter()
-        src_stencil = stencil(src_sphere.vertices, source_sphere_path, src_surface.vertices,
-                              "source", True, source_stencil_path, stencil_builder)
-        tgt_stencil = stencil(tgt_sphere.vertices, target_sphere_path, tgt_surface.vertices,
-                              "target", True, target_stencil_path, stencil_builder)
+        for i, st in enumerate(states):
+            st.src_stencil = stencil(st, i, st.src_sphere.vertices, st.job.source_sphere,
+                                     "source", True, st.job.source_stencil, stencil_builder)
+            st.tgt_stencil = stencil(st, i, st.tgt_sphere.vertices, st.job.target_sphere,
+                                     "target", True, st.job.target_stencil, stencil_builder)
+            st.src_dev = torch.as_tensor(st.src_surface.vertices, device=dev)
+            st.tgt_dev = torch.as_tensor(st.tgt_surface.vertices, device=dev)
         torch.cuda.synchronize(dev)
-        timings["initial_stencil_seconds"] = time.perf_counter() - t
+        shared["initial_stencil_seconds"] = time.perf_counter() - t
 
-        # rotation features (rotation_feature.py; gpu_pipeline.py:720-895)
+        # rotation features (rotation_feature.py; gpu_pipeline.py:720-895): every
+        # job and side solved together where the topologies agree
         t = time.perf_counter()
-        if rotation_values_path is not None:
-            src_vals, tgt_vals = read_rotation_values(rotation_values_path)
-        elif rotation_feature_backend == "official-cpu":
-            if rotation_values_probe is None:
-                raise ValueError("official-cpu rotation features require rotation_values_probe")
-            out = tmp / "rotation-values.bin"
-            _run_checked([rotation_values_probe, source_surface_path, source_sphere_path,
-                          target_surface_path, target_sphere_path, out], "Rotation feature")
-            src_vals, tgt_vals = read_rotation_values(out)
-        else:
-            depth = None
+        gpu_jobs = []
+        depth = []
+        for i, st in enumerate(states):
+            job = st.job
+            if job.rotation_values is not None:
+                st.src_vals, st.tgt_vals = read_rotation_values(job.rotation_values)
+                continue
+            if rotation_feature_backend == "official-cpu":
+                if rotation_values_probe is None:
+                    raise ValueError("official-cpu rotation features require "
+                                     "rotation_values_probe")
+                out = tmp / f"job{i}-rotation-values.bin"
+                _run_checked([rotation_values_probe, job.source_surface, job.source_sphere,
+                              job.target_surface, job.target_sphere, out], "Rotation feature")
+                st.src_vals, st.tgt_vals = read_rotation_values(out)
+                continue
             if rotation_feature_backend == "cuda-official-depth":
                 if rotation_depth_probe is None:
                     raise ValueError("cuda-official-depth requires rotation_depth_probe")
@@ -255,133 +369,153 @@ def run_cat_surface_gpu_pipeline(
                 env = os.environ.copy()
                 env["CAT_SURFACE_GPU_ROTATION_RAW_DEPTH"] = "1"
                 env["CAT_SURFACE_GPU_ROTATION_RAW_DEPTH_GEOMETRY"] = "1"
-                out = tmp / "rotation-depth-values.bin"
-                _run_checked([rotation_depth_probe, source_surface_path, source_sphere_path,
-                              target_surface_path, target_sphere_path, out], "Rotation depth",
+                out = tmp / f"job{i}-rotation-depth-values.bin"
+                _run_checked([rotation_depth_probe, job.source_surface, job.source_sphere,
+                              job.target_surface, job.target_sphere, out], "Rotation depth",
                              env)
-                depth = read_rotation_values(out)
+                d = read_rotation_values(out)
+                depth += [torch.as_tensor(d[0], device=dev), torch.as_tensor(d[1], device=dev)]
             elif rotation_feature_backend == "cuda-official-geometry":
                 raise ValueError("cuda-official-geometry requires rotation_geometry_probe "
                                  "outputs, which this build does not produce")
+            gpu_jobs.append(st)
+        if gpu_jobs:
             feats = compute_rotation_features(
-                (src_stencil, tgt_stencil),
-                (torch.as_tensor(src_surface.vertices, device=dev),
-                 torch.as_tensor(tgt_surface.vertices, device=dev)),
-                heat_fwhms=(15.0, 10.0),
-                depth_values=None if depth is None else tuple(
-                    torch.as_tensor(d, device=dev) for d in depth),
-                solver=rotation_feature_solver)
-            src_vals, tgt_vals = feats[0].values, feats[1].values
+                [s for st in gpu_jobs for s in (st.src_stencil, st.tgt_stencil)],
+                [v for st in gpu_jobs for v in (st.src_dev, st.tgt_dev)],
+                heat_fwhms=tuple(f for _ in gpu_jobs for f in (15.0, 10.0)),
+                depth_values=depth if depth else None, solver=rotation_feature_solver,
+                parallel_sides=parallel_sides)
+            for k, st in enumerate(gpu_jobs):
+                st.src_vals, st.tgt_vals = feats[2 * k].values, feats[2 * k + 1].values
         torch.cuda.synchronize(dev)
-        timings["rotation_feature_seconds"] = time.perf_counter() - t
+        shared["rotation_feature_seconds"] = time.perf_counter() - t
 
-        t = time.perf_counter()
-        rot = RotationPipeline.from_stencil(tgt_stencil, device=dev, grid_size=rotation_grid_size,
-                                            margin=rotation_margin)
-        torch.cuda.synchronize(dev)
-        timings["rotation_index_upload_seconds"] = time.perf_counter() - t
-
-        t = time.perf_counter()
-        src_points = src_stencil.sphere_points64 if src_stencil.sphere_points64 is not None \
-            else src_stencil.sphere_points
-        if rotation_search is not None:
-            result = rotation_search(rot, src_points, src_vals, tgt_vals)
-        else:
-            result = rot.search(src_points, src_vals, tgt_vals, point_chunk=point_chunk,
-                                refine=rotation_refine, max_iter=rotation_max_iter,
-                                tol=rotation_tol, simplex_step=rotation_simplex_step)
-        timings["rotation_search_seconds"] = time.perf_counter() - t
-        angle = np.asarray(result.angle.detach().cpu(), dtype=np.float64).reshape(3)
-        rotation_cost = float(result.cost.detach().cpu())
-
-        current_sphere = _rotate_vertices(_normalise_sphere(src_sphere.vertices), angle)
-        pole = np.asarray((0.0, np.pi / 2.0, 0.0), dtype=np.float64)
-        avg_tgt_surface = _rotate_vertices(tgt_surface.vertices, pole)
-        avg_tgt_sphere = _rotate_vertices(_normalise_sphere(tgt_sphere.vertices), pole)
+        t_idx = 0.0
+        t_search = 0.0
+        for st in states:
+            t = time.perf_counter()
+            rot = RotationPipeline.from_stencil(st.tgt_stencil, device=dev,
+                                                grid_size=rotation_grid_size,
+                                                margin=rotation_margin)
+            torch.cuda.synchronize(dev)
+            t_idx += time.perf_counter() - t
+            t = time.perf_counter()
+            src_points = (st.src_stencil.sphere_points64
+                          if st.src_stencil.sphere_points64 is not None
+                          else st.src_stencil.sphere_points)
+            if rotation_search is not None:
+                result = rotation_search(rot, src_points, st.src_vals, st.tgt_vals)
+            else:
+                result = rot.search(src_points, st.src_vals, st.tgt_vals, point_chunk=point_chunk,
+                                    refine=rotation_refine, max_iter=rotation_max_iter,
+                                    tol=rotation_tol, simplex_step=rotation_simplex_step)
+            t_search += time.perf_counter() - t
+            st.angle = np.asarray(result.angle.detach().cpu(), dtype=np.float64).reshape(3)
+            st.rotation_cost = float(result.cost.detach().cpu())
+        shared["rotation_index_upload_seconds"] = t_idx
+        shared["rotation_search_seconds"] = t_search
 
         t = time.perf_counter()
         builder = rotated_stencil_builder
-        cur_stencil = stencil(current_sphere, None, src_surface.vertices, "rotated-000", False,
-                              builder=builder)
-        avg_tgt_stencil = None
-        if avg:
-            avg_tgt_stencil = stencil(avg_tgt_sphere, None, tgt_surface.vertices, "avg-target",
-                                      False, builder=builder)
+        for i, st in enumerate(states):
+            st.current_sphere = _rotate_vertices(_normalise_sphere(st.src_sphere.vertices),
+                                                 st.angle)
+            st.avg_tgt_surface = _rotate_vertices(st.tgt_surface.vertices, pole)
+            avg_tgt_sphere = _rotate_vertices(_normalise_sphere(st.tgt_sphere.vertices), pole)
+            st.cur_stencil = stencil(st, i, st.current_sphere, None, "rotated-000", False,
+                                     builder=builder)
+            if avg:
+                st.avg_tgt_stencil = stencil(st, i, avg_tgt_sphere, None, "avg-target", False,
+                                             builder=builder)
         torch.cuda.synchronize(dev)
-        timings["rotated_stencil_seconds"] = time.perf_counter() - t
+        shared["rotated_stencil_seconds"] = time.perf_counter() - t
 
-        src_surface_dev = torch.as_tensor(src_surface.vertices, device=dev)
-        tgt_surface_dev = torch.as_tensor(tgt_surface.vertices, device=dev)
-
-        def solve_one(sv, tv, ss, ts, solve_fwhm):
-            return solve_dartel_from_surfaces(
-                sv, tv, ss, ts, fwhm=solve_fwhm, curvtypes=tuple(curvtypes), n_steps=steps,
-                loop=loop, cycles=cycles, nit=nit, its=its, code=code, kernel=kernel,
-                device=dev, dtype=solve_dtype, squaring_kernel=squaring_kernel,
-                parallel_sides=parallel_sides, optimized=optimized_dartel,
-                cuda_graph=graph_requested)
-
+        solve_kw = dict(n_steps=steps, loop=loop, cycles=cycles, nit=nit, its=its, code=code,
+                        device=dev, dtype=solve_dtype, cuda_graph=graph_requested)
         t_dartel = time.perf_counter()
-        last = None
-        final_prewarp = None
-        avg_src_stencil = avg_src_surface = avg_src_sphere = None
         for run in range(runs):
             t = time.perf_counter()
             run_fwhm = tuple(float(fwhm[0]) / (3.0 ** (steps * (run + 1) + i))
                              for i in range(steps))
-            if avg and run == runs - 1:
-                final_prewarp = current_sphere.copy()
-                avg_src_surface = _rotate_vertices(src_surface.vertices, pole)
-                avg_src_sphere = _rotate_vertices(final_prewarp, pole)
-                avg_src_stencil = stencil(avg_src_sphere, None, src_surface.vertices,
-                                          "avg-source", False, builder=builder)
-            solve = solve_one(src_surface_dev, tgt_surface_dev, cur_stencil, tgt_stencil,
-                              run_fwhm)
-            if not (avg and run == runs - 1):
-                warped = apply_flow_to_stenciled_sphere(current_sphere, solve.flow, cur_stencil,
-                                                        inverse=True, device=dev,
-                                                        dtype=solve_dtype)
-                current_sphere = warped.to(torch.float32).cpu().numpy()
-            last = solve
+            final_run = avg and run == runs - 1
+            requests, owners = [], []
+            for i, st in enumerate(states):
+                if final_run:
+                    st.final_prewarp = st.current_sphere.copy()
+                    st.avg_src_surface = _rotate_vertices(st.src_surface.vertices, pole)
+                    st.avg_src_sphere = _rotate_vertices(st.final_prewarp, pole)
+                    st.avg_src_stencil = stencil(st, i, st.avg_src_sphere, None, "avg-source",
+                                                 False, builder=builder)
+                requests.append((st.src_dev, st.tgt_dev, st.cur_stencil, st.tgt_stencil,
+                                 run_fwhm))
+                owners.append((st, "main"))
+            if final_run:
+                # the average solve depends only on the pre-warp sphere of this run
+                avg_fwhm = tuple(float(fwhm[0]) / (3.0 ** (steps * (runs + 1) + i))
+                                 for i in range(steps))
+                for st in states:
+                    requests.append((torch.as_tensor(st.avg_src_surface, device=dev),
+                                     torch.as_tensor(st.avg_tgt_surface, device=dev),
+                                     st.avg_src_stencil, st.avg_tgt_stencil, avg_fwhm))
+                    owners.append((st, "avg"))
+            results = solve_dartel_from_surfaces_jobs(requests, **solve_kw)
+            for (st, role), res in zip(owners, results):
+                if role == "avg":
+                    st.avg_flow = res.flow
+                    continue
+                st.last_flow = res.flow
+                if not final_run:
+                    warped = apply_flow_to_stenciled_sphere(st.current_sphere, res.flow,
+                                                            st.cur_stencil, inverse=True,
+                                                            device=dev, dtype=solve_dtype)
+                    st.current_sphere = warped.to(torch.float32).cpu().numpy()
             if run + 1 < runs:
-                cur_stencil = stencil(current_sphere, None, src_surface.vertices,
-                                      f"rotated-{run + 1:03d}", False, builder=builder)
+                for i, st in enumerate(states):
+                    st.cur_stencil = stencil(st, i, st.current_sphere, None,
+                                             f"rotated-{run + 1:03d}", False, builder=builder)
             torch.cuda.synchronize(dev)
-            timings[f"dartel_run_{run}_seconds"] = time.perf_counter() - t
-        timings["source_dartel_total_seconds"] = time.perf_counter() - t_dartel
+            shared[f"dartel_run_{run}_seconds"] = time.perf_counter() - t
+        shared["source_dartel_total_seconds"] = time.perf_counter() - t_dartel
 
         if avg:
             t = time.perf_counter()
-            avg_fwhm = tuple(float(fwhm[0]) / (3.0 ** (steps * (runs + 1) + i))
-                             for i in range(steps))
-            avg_result = solve_one(torch.as_tensor(avg_src_surface, device=dev),
-                                   torch.as_tensor(avg_tgt_surface, device=dev),
-                                   avg_src_stencil, avg_tgt_stencil, avg_fwhm)
-            main_warped = apply_flow_to_stenciled_sphere(final_prewarp, last.flow, cur_stencil,
-                                                         inverse=True, device=dev,
-                                                         dtype=solve_dtype)
-            avg_warped = apply_flow_to_stenciled_sphere(avg_src_sphere, avg_result.flow,
-                                                        avg_src_stencil, inverse=True,
-                                                        device=dev, dtype=solve_dtype)
             unrotate = rotation_matrix_np(-pole)
-            combined = torch.empty((main_warped.shape[0], 3), dtype=torch.float32, device=dev)
-            _lib.check(_lib.lib().csg_average_xz(
-                _lib.ptr(avg_warped), _lib.ptr(main_warped), int(main_warped.shape[0]),
-                _lib.dtype_code(solve_dtype), _lib.doubles(unrotate.reshape(-1)),
-                _lib.ptr(combined), _lib.stream_handle(dev)), "csg_average_xz")
-            current_sphere = combined.cpu().numpy()
-            timings["avg_seconds"] = time.perf_counter() - t
-        timings["dartel_total_seconds"] = time.perf_counter() - t_dartel
+            for st in states:
+                main_warped = apply_flow_to_stenciled_sphere(st.final_prewarp, st.last_flow,
+                                                             st.cur_stencil, inverse=True,
+                                                             device=dev, dtype=solve_dtype)
+                avg_warped = apply_flow_to_stenciled_sphere(st.avg_src_sphere, st.avg_flow,
+                                                            st.avg_src_stencil, inverse=True,
+                                                            device=dev, dtype=solve_dtype)
+                combined = torch.empty((main_warped.shape[0], 3), dtype=torch.float32,
+                                       device=dev)
+                _lib.check(_lib.lib().csg_average_xz(
+                    _lib.ptr(avg_warped), _lib.ptr(main_warped), int(main_warped.shape[0]),
+                    _lib.dtype_code(solve_dtype), _lib.doubles(unrotate.reshape(-1)),
+                    _lib.ptr(combined), _lib.stream_handle(dev)), "csg_average_xz")
+                st.current_sphere = combined.cpu().numpy()
+            shared["avg_seconds"] = time.perf_counter() - t
+        shared["dartel_total_seconds"] = time.perf_counter() - t_dartel
 
         t = time.perf_counter()
-        final = GiftiMesh(np.ascontiguousarray(current_sphere, dtype=np.float32),
-                          src_sphere.faces)
-        output = Path(output_path)
-        if write_output:
-            write_gifti_mesh(output, final,
-                             reference_path=None if meshes is not None else source_sphere_path)
-        timings["output_write_seconds"] = time.perf_counter() - t
-    timings["total_seconds"] = time.perf_counter() - t_total
-    return CatSurfaceGpuResult(vertices=final.vertices, faces=final.faces,
-                               rotation_angle=angle, rotation_cost=rotation_cost,
-                               timings=timings, output_path=output)
+        results_out = []
+        for st in states:
+            final = GiftiMesh(np.ascontiguousarray(st.current_sphere, dtype=np.float32),
+                              st.src_sphere.faces)
+            output = Path(st.job.output) if st.job.output is not None else Path("")
+            if write_output:
+                if st.job.output is None:
+                    raise ValueError("an output path is required when write_output is set")
+                write_gifti_mesh(output, final, reference_path=None if st.job.meshes is not None
+                                 else st.job.source_sphere)
+            results_out.append((st, final, output))
+        shared["output_write_seconds"] = time.perf_counter() - t
+    shared["total_seconds"] = time.perf_counter() - t_total
+    out = []
+    for st, final, output in results_out:
+        timings = dict(shared)
+        out.append(CatSurfaceGpuResult(vertices=final.vertices, faces=final.faces,
+                                       rotation_angle=st.angle, rotation_cost=st.rotation_cost,
+                                       timings=timings, output_path=output))
+    return out

@@ -88,11 +88,13 @@ def solve_workspace(nx: int, ny: int, batch: int, dtype: torch.dtype,
     return _lib.workspace(nbytes, device)
 
 
-def _capture(source, target, *, loop, lmreg, cycles, nit, its, code, device) -> _SolveGraph:
+def _capture(source, target, *, loop, lmreg, cycles, nit, its, code, device,
+             steps_per_job: int | None = None) -> _SolveGraph:
     static_source = source.detach().clone().contiguous()
     static_target = target.detach().clone().contiguous()
     steps, ny, nx = (int(d) for d in source.shape)
-    flow = torch.empty((2, ny, nx), dtype=source.dtype, device=device)
+    per_job = steps if steps_per_job is None else int(steps_per_job)
+    flow = torch.empty((steps // per_job, 2, ny, nx), dtype=source.dtype, device=device)
     metrics = torch.empty((steps, loop * its, 3), dtype=torch.float64, device=device)
     ws = solve_workspace(nx, ny, steps, source.dtype, device)
     graph = torch.cuda.CUDAGraph()
@@ -103,7 +105,7 @@ def _capture(source, target, *, loop, lmreg, cycles, nit, its, code, device) -> 
     capture_stream.wait_stream(current)
     with torch.cuda.stream(capture_stream):
         with torch.cuda.graph(graph, stream=capture_stream):
-            _launch_solve(static_source, static_target, steps_per_job=steps, loop=loop,
+            _launch_solve(static_source, static_target, steps_per_job=per_job, loop=loop,
                           lmreg=lmreg, cycles=cycles, nit=nit, its=its, code=code,
                           assume_initial_nonzero=True, flow=flow, metrics=metrics,
                           workspace=ws)
@@ -185,7 +187,8 @@ def solve_dartel_maps(source_maps, target_maps, *, loop: int = 6, lmreg: float =
         entry.source_maps.copy_(source)
         entry.target_maps.copy_(target)
         entry.graph.replay()
-        return DartelSolveResult(flow=entry.flow.clone(), source_maps=entry.source_maps.clone(),
+        return DartelSolveResult(flow=entry.flow[0].clone(),
+                                 source_maps=entry.source_maps.clone(),
                                  target_maps=entry.target_maps.clone(),
                                  metrics=entry.metrics.to(dtype).clone())
     flow = torch.empty((2, ny, nx), dtype=dtype, device=dev)
@@ -196,6 +199,100 @@ def solve_dartel_maps(source_maps, target_maps, *, loop: int = 6, lmreg: float =
                   flow=flow, metrics=metrics, workspace=ws)
     return DartelSolveResult(flow=flow, source_maps=source, target_maps=target,
                              metrics=metrics.to(dtype))
+
+
+def solve_dartel_maps_jobs(source_maps, target_maps, steps_per_job: int, *, loop: int = 6,
+                           lmreg: float = 1e-3, cycles: int = 3, nit: int = 3, its: int = 3,
+                           code: int = 1, device: str | torch.device = "auto",
+                           dtype: torch.dtype = torch.float64,
+                           assume_initial_nonzero: bool = False,
+                           cuda_graph: bool = False) -> list[DartelSolveResult]:
+    """Several independent solves in one batched launch.
+
+    ``source_maps`` / ``target_maps`` are [jobs · steps_per_job, ny, nx]; each run
+    of ``steps_per_job`` consecutive maps is one solve_dartel_maps job.  Every
+    kernel of the solve processes all jobs at once; the results equal solving
+    each job alone.  With ``cuda_graph`` the whole batch is captured once per
+    shape and replayed.
+    """
+
+    dev = resolve_device(device)
+    source = torch.as_tensor(source_maps, dtype=dtype, device=dev).contiguous()
+    target = torch.as_tensor(target_maps, dtype=dtype, device=dev).contiguous()
+    if source.ndim != 3 or tuple(source.shape) != tuple(target.shape):
+        raise ValueError("source_maps and target_maps must have the same [batch, ny, nx] shape")
+    if steps_per_job < 1 or source.shape[0] % steps_per_job:
+        raise ValueError("the batch must be a whole number of jobs of steps_per_job maps")
+    if cycles < 0 or nit < 0 or its < 0 or loop <= 0:
+        raise ValueError("loop must be positive; cycles, nit and its non-negative")
+    batch, ny, nx = (int(d) for d in source.shape)
+    jobs = batch // steps_per_job
+    if cuda_graph:
+        key = _solve_key(source, target, loop=loop, lmreg=float(lmreg), cycles=cycles, nit=nit,
+                         its=its, code=code, steps_per_job=int(steps_per_job))
+        entry = _DARTEL_CUDA_GRAPH_CACHE.get(key)
+        if entry is None:
+            entry = _capture(source, target, loop=loop, lmreg=lmreg, cycles=cycles, nit=nit,
+                             its=its, code=code, device=dev, steps_per_job=steps_per_job)
+            _DARTEL_CUDA_GRAPH_CACHE[key] = entry
+        entry.source_maps.copy_(source)
+        entry.target_maps.copy_(target)
+        entry.graph.replay()
+        flow, metrics = entry.flow.clone(), entry.metrics.clone()
+    else:
+        flow = torch.empty((jobs, 2, ny, nx), dtype=dtype, device=dev)
+        metrics = torch.empty((batch, loop * its, 3), dtype=torch.float64, device=dev)
+        ws = solve_workspace(nx, ny, batch, dtype, dev)
+        _launch_solve(source, target, steps_per_job=steps_per_job, loop=loop, lmreg=lmreg,
+                      cycles=cycles, nit=nit, its=its, code=code,
+                      assume_initial_nonzero=assume_initial_nonzero, flow=flow,
+                      metrics=metrics, workspace=ws)
+    out = []
+    for j in range(jobs):
+        sl = slice(j * steps_per_job, (j + 1) * steps_per_job)
+        out.append(DartelSolveResult(flow=flow[j], source_maps=source[sl],
+                                     target_maps=target[sl], metrics=metrics[sl].to(dtype)))
+    return out
+
+
+def type5_sheet_maps(vertices, stencil, fwhm: Sequence[float], dtype: torch.dtype,
+                     device: torch.device) -> torch.Tensor:
+    """[steps, ny, nx] type-5 curvature sheets of one surface (surface_warp.py:426-448)."""
+
+    from .surface_stencil import SurfaceStencil
+
+    st = stencil.to(device) if isinstance(stencil, SurfaceStencil) else stencil
+    v = torch.as_tensor(vertices, device=device).to(torch.float32).contiguous()
+    return torch.stack(st.curvature_type5_to_sheet_many(st.resample_vertices(v),
+                                                        tuple(float(f) for f in fwhm),
+                                                        out_dtype=dtype))
+
+
+def solve_dartel_from_surfaces_jobs(requests, *, n_steps: int = 2, loop: int = 6,
+                                    lmreg: float = 1e-3, cycles: int = 3, nit: int = 3,
+                                    its: int = 3, code: int = 1,
+                                    device: str | torch.device = "auto",
+                                    dtype: torch.dtype = torch.float64,
+                                    assume_initial_nonzero: bool = False,
+                                    cuda_graph: bool = False) -> list[DartelSolveResult]:
+    """solve_dartel_from_surfaces for several independent registrations at once.
+
+    ``requests`` holds (source_vertices, target_vertices, source_stencil,
+    target_stencil, fwhm) tuples; their curvature sheets are computed per side
+    and all solves run as one batched solve (solve_dartel_maps_jobs).
+    """
+
+    dev = resolve_device(device)
+    src, tgt = [], []
+    for sv, tv, ss, ts, fw in requests:
+        if len(fw) < n_steps:
+            raise ValueError("each request needs at least n_steps fwhm values")
+        src.append(type5_sheet_maps(sv, ss, fw[:n_steps], dtype, dev))
+        tgt.append(type5_sheet_maps(tv, ts, fw[:n_steps], dtype, dev))
+    return solve_dartel_maps_jobs(torch.cat(src), torch.cat(tgt), n_steps, loop=loop, lmreg=lmreg,
+                                  cycles=cycles, nit=nit, its=its, code=code, device=dev,
+                                  dtype=dtype, assume_initial_nonzero=assume_initial_nonzero,
+                                  cuda_graph=cuda_graph)
 
 
 def solve_dartel_from_surfaces(source_vertices, target_vertices, source_stencil, target_stencil, *,

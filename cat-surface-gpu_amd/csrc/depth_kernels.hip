@@ -537,7 +537,7 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     SorColours colours = co;
     // per row and sweep: the row's ELL values and columns (12 B per slot), its
     // x gathers (8 B per slot), rhs, diagonal and the x update (24 B)
-    TimedLaunch timer(kTimedSor, s,
+    TimedLaunch timer(registers ? kTimedSorRegisters : kTimedSor, s,
                       (double)max_iter * batch * n * (20.0 * W + 24.0));
     // A regular launch: the grid is capped at what the device holds at once
     // (occupancy x CUs), and every barrier wait is bounded, so blocks that were

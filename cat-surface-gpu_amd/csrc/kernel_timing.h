@@ -16,7 +16,8 @@ enum TimedKernel : int {
     kTimedFmgCoarse,       // coarse levels of one FMG cycle (or initialisation)
     kTimedRestrict,        // residual + restriction of one fine level
     kTimedRelaxSweep,      // one half-sweep of the per-operator schedule
-    kTimedSor,             // all SOR sweeps of one depth-potential solve
+    kTimedSor,             // all SOR sweeps of one depth-potential solve (rows from memory)
+    kTimedSorRegisters,    // the same with the rows held in registers
     kTimedCount
 };
 

@@ -56,8 +56,8 @@ bool capturing(hipStream_t s)
     return st != hipStreamCaptureStatusNone;
 }
 
-const char *kNames[kTimedCount] = {"k_relax_tile", "k_fmg_coarse", "k_restrict_residual",
-                                   "k_relax_color", "k_sor_persistent"};
+const char *kNames[kTimedCount] = {"k_relax_tile",  "k_fmg_coarse",     "k_restrict_residual",
+                                   "k_relax_color", "k_sor_persistent", "k_sor_registers"};
 
 }  // namespace
 

@@ -47,3 +47,29 @@ def test_pipeline_end_to_end(pair_paths, cuda, dtype):
     assert res.rotation_cost >= 0.0
     for key in ("rotation_search_seconds", "dartel_total_seconds", "total_seconds"):
         assert key in res.timings
+
+
+def test_batched_jobs_equal_single_runs(pair_paths, cuda, tmp_path):
+    """Two registrations scheduled together give exactly the single-job results."""
+
+    from cat_surface_gpu import (GiftiMesh, PipelineJob, run_cat_surface_gpu_pipeline,
+                                 run_cat_surface_gpu_pipelines, synthetic)
+
+    pair, paths, folder = pair_paths
+    mirrored = synthetic.make_pair("icosphere", subdivisions=4, lmax=6, noise=0.0,
+                                   max_rotation_deg=15.0, seed=99, mirror=True)
+    jobs = []
+    for p in (pair, mirrored):
+        jobs.append(PipelineJob(meshes=(GiftiMesh(p.src_surface, p.src_faces),
+                                        GiftiMesh(p.src_sphere, p.src_faces),
+                                        GiftiMesh(p.tgt_surface, p.tgt_faces),
+                                        GiftiMesh(p.tgt_sphere, p.tgt_faces))))
+    kw = dict(device=cuda, dartel_dtype="float32", steps=2, runs=2, avg=True,
+              write_output=False)
+    together = run_cat_surface_gpu_pipelines(jobs, **kw)
+    for job, res in zip(jobs, together):
+        alone = run_cat_surface_gpu_pipeline(None, None, None, None, tmp_path / "x.gii",
+                                             meshes=job.meshes, **kw)
+        np.testing.assert_array_equal(res.vertices, alone.vertices)
+        np.testing.assert_array_equal(res.rotation_angle, alone.rotation_angle)
+        np.testing.assert_array_equal(res.faces, alone.faces)
