@@ -66,6 +66,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_edge_mean_workspace_bytes": (_sz, [_i]),
     "csg_edge_sum": (_i, [_vp, _vp, _i, _vp, ctypes.POINTER(_i), _vp]),
     "csg_smooth_geometry": (_i, [_vp, _vp, _vp, _i, _i, _vp, _vp, _sz, _vp]),
+    "csg_smooth_geometry_batch": (_i, [_vp, _i, _vp, _vp, _i, ctypes.POINTER(_i), _vp, _vp, _sz,
+                                       _vp]),
     "csg_vertex_normals": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp]),
     "csg_smooth_values_workspace_bytes": (_sz, [_i, _i, _i]),
     "csg_smooth_values": (_i, [_vp, _i, _i, _vp, _vp, _vp, _i, _dp, ctypes.POINTER(_i), _i, _vp,
@@ -74,7 +76,7 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_minmax_normalise": (_i, [_vp, _sz, _i, _d, _i, _vp, _vp, _sz, _vp]),
     "csg_sheet_gather": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _vp]),
     "csg_type5_workspace_bytes": (_sz, [_i, _i, _i, _i]),
-    "csg_type5_sheets": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _i, _dp,
+    "csg_type5_sheets": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _vp, _i, _dp,
                               ctypes.POINTER(_i), _vp, _vp, _i, _i, _vp, _vp, _sz, _vp]),
     "csg_depth_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
     "csg_depth_potential": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i,

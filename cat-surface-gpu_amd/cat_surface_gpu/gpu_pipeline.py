@@ -229,6 +229,14 @@ class _JobState:
     avg_flow: object = None
     src_dev: object = None
     tgt_dev: object = None
+    src_faces_dev: object = None
+    tgt_faces_dev: object = None
+    current_dev: object = None  # device copy of current_sphere (float32)
+
+    def current_host(self) -> np.ndarray:
+        if self.current_sphere is None:
+            self.current_sphere = self.current_dev.cpu().numpy()
+        return self.current_sphere
 
 
 def run_cat_surface_gpu_pipelines(
@@ -328,11 +336,14 @@ def run_cat_surface_gpu_pipelines(
                 return _external_stencil(builder, sphere_path,
                                          tmp / f"job{index}-{label}.stencil", opts, dev)
             surface = st.src_surface.vertices if source_side else st.tgt_surface.vertices
-            return build_device_stencil(sphere_vertices, faces, surface_vertices=surface,
+            faces_dev = st.src_faces_dev if source_side else st.tgt_faces_dev
+            return build_device_stencil(sphere_vertices, faces_dev, surface_vertices=surface,
                                         normalise=normalise, device=dev)
 
         t = time.perf_counter()
         for i, st in enumerate(states):
+            st.src_faces_dev = torch.as_tensor(st.src_sphere.faces, device=dev)
+            st.tgt_faces_dev = torch.as_tensor(st.tgt_sphere.faces, device=dev)
             st.src_stencil = stencil(st, i, st.src_sphere.vertices, st.job.source_sphere,
                                      "source", True, st.job.source_stencil, stencil_builder)
             st.tgt_stencil = stencil(st, i, st.tgt_sphere.vertices, st.job.target_sphere,
@@ -442,7 +453,7 @@ def run_cat_surface_gpu_pipelines(
             requests, owners = [], []
             for i, st in enumerate(states):
                 if final_run:
-                    st.final_prewarp = st.current_sphere.copy()
+                    st.final_prewarp = st.current_host().copy()
                     st.avg_src_surface = _rotate_vertices(st.src_surface.vertices, pole)
                     st.avg_src_sphere = _rotate_vertices(st.final_prewarp, pole)
                     st.avg_src_stencil = stencil(st, i, st.avg_src_sphere, None, "avg-source",
@@ -466,14 +477,17 @@ def run_cat_surface_gpu_pipelines(
                     continue
                 st.last_flow = res.flow
                 if not final_run:
-                    warped = apply_flow_to_stenciled_sphere(st.current_sphere, res.flow,
-                                                            st.cur_stencil, inverse=True,
-                                                            device=dev, dtype=solve_dtype)
-                    st.current_sphere = warped.to(torch.float32).cpu().numpy()
+                    warped = apply_flow_to_stenciled_sphere(
+                        st.current_host() if st.current_dev is None else st.current_dev,
+                        res.flow, st.cur_stencil, inverse=True, device=dev, dtype=solve_dtype)
+                    st.current_dev = warped.to(torch.float32)
+                    st.current_sphere = None
             if run + 1 < runs:
                 for i, st in enumerate(states):
-                    st.cur_stencil = stencil(st, i, st.current_sphere, None,
-                                             f"rotated-{run + 1:03d}", False, builder=builder)
+                    sphere = (st.current_host() if builder is not None or st.current_dev is None
+                              else st.current_dev)
+                    st.cur_stencil = stencil(st, i, sphere, None, f"rotated-{run + 1:03d}",
+                                             False, builder=builder)
             torch.cuda.synchronize(dev)
             shared[f"dartel_run_{run}_seconds"] = time.perf_counter() - t
         shared["source_dartel_total_seconds"] = time.perf_counter() - t_dartel
@@ -501,7 +515,7 @@ def run_cat_surface_gpu_pipelines(
         t = time.perf_counter()
         results_out = []
         for st in states:
-            final = GiftiMesh(np.ascontiguousarray(st.current_sphere, dtype=np.float32),
+            final = GiftiMesh(np.ascontiguousarray(st.current_host(), dtype=np.float32),
                               st.src_sphere.faces)
             output = Path(st.job.output) if st.job.output is not None else Path("")
             if write_output:

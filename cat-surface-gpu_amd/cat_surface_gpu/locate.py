@@ -73,14 +73,13 @@ def closest_triangles(queries: torch.Tensor, verts: torch.Tensor, faces: torch.T
 
     q = queries.to(torch.float64)
     v = verts.to(torch.float64)
-    # the grid spans [-1, 1]^3: scale both point sets into it (argmin and weights
-    # are invariant under a common scale)
-    extent = float(torch.maximum(q.abs().max(), v.abs().max()).item())
-    if extent > 1.0:
-        q = q / (extent * 1.0001)
-        v = v / (extent * 1.0001)
-    q = q.contiguous()
-    v = v.contiguous()
+    # the grid spans [-1, 1]^3: scale both point sets into it when they reach
+    # outside (argmin and weights are invariant under a common scale); decided
+    # on the device, without a host round trip
+    extent = torch.maximum(q.abs().max(), v.abs().max())
+    scale = torch.where(extent > 1.0, extent * 1.0001, torch.ones_like(extent))
+    q = (q / scale).contiguous()
+    v = (v / scale).contiguous()
     f = faces.to(torch.int32).contiguous()
     nq = int(q.shape[0])
     dev = q.device

@@ -16,7 +16,8 @@ from typing import Sequence
 import torch
 
 from . import _lib
-from .surface_stencil import SurfaceStencilDevice, _minmax, heat_parameters_from_edge_mean
+from .surface_stencil import (SurfaceStencilDevice, _minmax, edge_means,
+                              heat_parameters_from_edge_mean, smooth_geometry_many)
 
 
 @dataclass(frozen=True)
@@ -93,10 +94,11 @@ def _given_depth(stencil: SurfaceStencilDevice, depth_values) -> torch.Tensor:
 
 
 def _finish_feature(stencil: SurfaceStencilDevice, geometry: torch.Tensor, values: torch.Tensor,
-                    iterations: int, residual: float) -> RotationFeatureResult:
+                    iterations: int, residual: float,
+                    edge_mean: float | None = None) -> RotationFeatureResult:
     """fwhm-50 value smoothing and min/max normalisation (rotation_feature.py:384-399)."""
 
-    em = stencil.edge_mean_distance(geometry)
+    em = stencil.edge_mean_distance(geometry) if edge_mean is None else edge_mean
     params = [heat_parameters_from_edge_mean(em, 50.0)]
     smoothed = stencil._smooth_values_batch(values.reshape(1, -1), geometry, params, False)[0]
     return RotationFeatureResult(values=_minmax(smoothed, 2.220446049250313e-16),
@@ -151,13 +153,23 @@ def compute_rotation_features(stencils: Sequence[SurfaceStencilDevice],
     if depth_values is not None and len(depth_values) != len(stencils):
         raise ValueError("depth_values length must match stencils")
     _check_solver(solver, depth_values is None)
-    geometries = [
-        _feature_geometry(stencils[i], surfaces[i], float(heat_fwhms[i]),
-                          None if smoothed_surfaces is None else smoothed_surfaces[i])
-        for i in range(len(stencils))]
+    if smoothed_surfaces is None:
+        # resample every side, then one host synchronisation for the edge means
+        # and shared smoothing launches (the same values as per-side calls)
+        mapped = [st.resample_vertices(torch.as_tensor(v, device=st.device))
+                  for st, v in zip(stencils, surfaces)]
+        ems = edge_means(stencils, mapped)
+        its = [heat_parameters_from_edge_mean(em, float(f))[0]
+               for em, f in zip(ems, heat_fwhms)]
+        geometries = smooth_geometry_many(stencils, mapped, its)
+    else:
+        geometries = [_feature_geometry(stencils[i], surfaces[i], float(heat_fwhms[i]),
+                                        smoothed_surfaces[i]) for i in range(len(stencils))]
+    final_ems = edge_means(stencils, geometries)
     if depth_values is not None:
         return tuple(_finish_feature(stencils[i], geometries[i],
-                                     _given_depth(stencils[i], depth_values[i]), 0, 0.0)
+                                     _given_depth(stencils[i], depth_values[i]), 0, 0.0,
+                                     final_ems[i])
                      for i in range(len(stencils)))
     # group sides by shared topology (the topology cache returns one object per mesh)
     groups: list[list[int]] = []
@@ -178,5 +190,5 @@ def compute_rotation_features(stencils: Sequence[SurfaceStencilDevice],
             values[i] = x[k] if len(g) > 1 else x
             residuals[i] = float(rel_host[k])
     return tuple(_finish_feature(stencils[i], geometries[i], values[i], int(max_iter),
-                                 residuals[i])
+                                 residuals[i], final_ems[i])
                  for i in range(len(stencils)))

@@ -111,43 +111,72 @@ def build_stencil(sphere_vertices, sphere_faces, *, surface_vertices=None, norma
     )
 
 
+_DEVICE_SPHERES: dict = {}
+_DEVICE_UV: dict = {}
+
+
+def _device_icosphere(subdivisions: int, dev: torch.device):
+    """Icosphere vertices (float64), faces and topology on a device, built once."""
+
+    from .surface_stencil import topology_for
+
+    key = (dev.type, dev.index, subdivisions)
+    entry = _DEVICE_SPHERES.get(key)
+    if entry is None:
+        ico_v, ico_f = icosphere(subdivisions)
+        entry = (torch.as_tensor(ico_v, device=dev), torch.as_tensor(ico_f, device=dev),
+                 topology_for(ico_f, int(ico_v.shape[0]), dev))
+        _DEVICE_SPHERES[key] = entry
+    return entry
+
+
+def _device_uv(nx: int, ny: int, dev: torch.device) -> torch.Tensor:
+    key = (dev.type, dev.index, nx, ny)
+    uv = _DEVICE_UV.get(key)
+    if uv is None:
+        uv = torch.as_tensor(uv_points(nx, ny), device=dev)
+        _DEVICE_UV[key] = uv
+    return uv
+
+
 def build_device_stencil(sphere_vertices, sphere_faces, *, surface_vertices=None,
                          normalise: bool = True, nx: int = 512, ny: int = 256,
                          n_triangles: int = 81920, device: torch.device | None = None):
-    """Build the stencil directly as a SurfaceStencilDevice (no host round trip)."""
+    """Build the stencil directly as a SurfaceStencilDevice (no host round trip).
+
+    ``sphere_vertices`` and ``sphere_faces`` may be host arrays or device
+    tensors; the icosphere, its topology and the map's pixel directions are
+    cached per device.
+    """
 
     from .dartel_grid import resolve_device
-    from .surface_stencil import SurfaceStencilDevice, topology_for
+    from .surface_stencil import SurfaceStencilDevice
 
     dev = resolve_device("auto" if device is None else device)
-    sv = np.asarray(sphere_vertices, dtype=np.float32)
-    sf = np.ascontiguousarray(np.asarray(sphere_faces, dtype=np.int32))
+    sv = torch.as_tensor(sphere_vertices, device=dev).to(torch.float32)
+    sf = torch.as_tensor(sphere_faces, device=dev).to(torch.int32).contiguous()
     if sv.ndim != 2 or sv.shape[1] != 3 or sf.ndim != 2 or sf.shape[1] != 3:
         raise ValueError("sphere vertices and faces must have shape [n, 3]")
-    if surface_vertices is not None and np.asarray(surface_vertices).shape != sv.shape:
+    if surface_vertices is not None and tuple(np.shape(surface_vertices)) != tuple(sv.shape):
         raise ValueError("Surface and sphere vertex counts do not match")
-    pts = torch.as_tensor(sv, dtype=torch.float64, device=dev)
+    pts = sv.to(torch.float64)
     if normalise:
         pts = pts - pts.mean(dim=0, keepdim=True)
         pts = pts / torch.linalg.vector_norm(pts, dim=1, keepdim=True)
-    radius = 0.975 * float(torch.linalg.vector_norm(pts, dim=1).mean().item())
+    radius = 0.975 * torch.linalg.vector_norm(pts, dim=1).mean()
     lo = pts.min(dim=0).values
     hi = pts.max(dim=0).values
     centre = lo + hi  # the helper passes the bound sums as the centre
-    ico_v, ico_f = icosphere(subdivisions_for(n_triangles))
-    out_sphere = centre[None, :] + radius * torch.as_tensor(ico_v, device=dev)
-    faces_dev = torch.as_tensor(sf, device=dev)
+    ico_v, ico_f, topo = _device_icosphere(subdivisions_for(n_triangles), dev)
+    out_sphere = centre[None, :] + radius * ico_v
     # surface stencil: closest input-sphere triangle per resampled vertex
-    _, surf_idx, surf_w = closest_triangles(out_sphere, pts, faces_dev, grid_size=32, margin=1)
+    _, surf_idx, surf_w = closest_triangles(out_sphere, pts, sf, grid_size=32, margin=1)
     resampled = (pts[surf_idx.long()] * surf_w[..., None]).sum(dim=1)
     # sheet stencil on the unit resampled sphere
     unit_res = resampled / torch.linalg.vector_norm(resampled, dim=1, keepdim=True)
-    queries = torch.as_tensor(uv_points(nx, ny), device=dev)
-    ico_f_dev = torch.as_tensor(ico_f, device=dev)
-    _, sheet_idx, sheet_w = closest_triangles(queries, unit_res, ico_f_dev, grid_size=64,
-                                              margin=1)
+    _, sheet_idx, sheet_w = closest_triangles(_device_uv(nx, ny, dev), unit_res, ico_f,
+                                              grid_size=64, margin=1)
     unit_pts = (pts / torch.linalg.vector_norm(pts, dim=1, keepdim=True)).to(torch.float32)
-    topo = topology_for(ico_f, int(ico_v.shape[0]), dev)
     return SurfaceStencilDevice(
         sphere_points=resampled.to(torch.float32).contiguous(),
         faces=topo.faces,

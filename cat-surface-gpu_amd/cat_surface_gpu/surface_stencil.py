@@ -463,7 +463,8 @@ class SurfaceStencilDevice:
         out = self._type5(geometry, geom_iters, params, out_dtype)
         return tuple(out[i] for i in range(len(fwhms)))
 
-    def _type5(self, geometry: torch.Tensor, geom_iters: int, params, out_dtype) -> torch.Tensor:
+    def _type5(self, geometry: torch.Tensor, geom_iters: int, params, out_dtype,
+               smoothed: torch.Tensor | None = None) -> torch.Tensor:
         topo = self.topology
         n = int(geometry.shape[0])
         nb = len(params)
@@ -474,7 +475,7 @@ class SurfaceStencilDevice:
         its = (_lib.ctypes.c_int * nb)(*[int(p[0]) for p in params])
         self._call("csg_type5_sheets", _lib.ptr(geometry), n, _lib.ptr(topo.faces),
                    _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), topo.nnz, _lib.ptr(topo.inc_off),
-                   _lib.ptr(topo.inc), int(geom_iters), nb, sig, its,
+                   _lib.ptr(topo.inc), int(geom_iters), _lib.ptr(smoothed), nb, sig, its,
                    _lib.ptr(self.sheet_indices), _lib.ptr(self.sheet_weights), m,
                    _lib.dtype_code(out_dtype), _lib.ptr(out), _lib.ptr(ws), ws.numel(),
                    self._stream())
@@ -489,6 +490,75 @@ class SurfaceStencilDevice:
                 "this build implements curvtype 5 (used by steps <= 2); curvtype 2 is not built"
             )
         return self.curvature_type5_to_sheet_many(vertices, (float(fwhm),))[0]
+
+
+def edge_means(stencils, geometries) -> list[float]:
+    """edge_mean_distance of several meshes with a single host synchronisation
+    (surface_stencil.py:349-357): the same per-mesh partial sums and totals."""
+
+    totals, nfs = [], []
+    for st, g in zip(stencils, geometries):
+        v = torch.as_tensor(g, dtype=torch.float32, device=st.device).contiguous()
+        nf = int(st.faces.shape[0])
+        partial = torch.empty(_lib.lib().csg_edge_mean_workspace_bytes(nf) // 8,
+                              dtype=torch.float64, device=st.device)
+        nb = _lib.ctypes.c_int()
+        _lib.check(_lib.lib().csg_edge_sum(_lib.ptr(v), _lib.ptr(st.faces), nf, _lib.ptr(partial),
+                                           _lib.ctypes.byref(nb), st._stream()), "csg_edge_sum")
+        totals.append(partial[: nb.value].sum())
+        nfs.append(nf)
+    if not totals:
+        return []
+    host = torch.stack(totals).cpu().tolist()
+    return [float(t) / (3.0 * nf) for t, nf in zip(host, nfs)]
+
+
+def smooth_geometry_many(stencils, geometries, iterations) -> list[torch.Tensor]:
+    """SurfaceStencilDevice.smooth_geometry of several meshes with given
+    iteration counts; meshes that share a topology are smoothed in the same
+    launches (csg_smooth_geometry_batch).  Results equal one call per mesh."""
+
+    out: list[torch.Tensor | None] = [None] * len(stencils)
+    groups: dict[int, list[int]] = {}
+    for i, st in enumerate(stencils):
+        groups.setdefault(id(st.topology), []).append(i)
+    for members in groups.values():
+        for start in range(0, len(members), 32):
+            chunk = members[start:start + 32]
+            topo = stencils[chunk[0]].topology
+            dev = stencils[chunk[0]].device
+            v = torch.stack([torch.as_tensor(geometries[i], dtype=torch.float32, device=dev)
+                             for i in chunk]).contiguous()
+            n = int(v.shape[1])
+            res = torch.empty_like(v)
+            ws = torch.empty_like(v)
+            its = (_lib.ctypes.c_int * len(chunk))(*[int(iterations[i]) for i in chunk])
+            _lib.check(_lib.lib().csg_smooth_geometry_batch(
+                _lib.ptr(v), len(chunk), _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), n, its,
+                _lib.ptr(res), _lib.ptr(ws), ws.numel() * 4, stencils[chunk[0]]._stream()),
+                "csg_smooth_geometry_batch")
+            for k, i in enumerate(chunk):
+                out[i] = res[k]
+    return out
+
+
+def type5_sheets_many(stencils, geometries, fwhms, out_dtype=torch.float64) -> list[torch.Tensor]:
+    """curvature_type5_to_sheet_many for several (stencil, resampled geometry)
+    pairs: one host synchronisation for all edge means and the fwhm-50 geometry
+    smoothing of every mesh in shared launches.  Returns [len(fwhms), ny, nx]
+    sheets per pair, equal to the per-pair calls."""
+
+    fw = tuple(float(f) for f in fwhms)
+    geoms = [torch.as_tensor(g, dtype=torch.float32, device=st.device).contiguous()
+             for st, g in zip(stencils, geometries)]
+    ems = edge_means(stencils, geoms)
+    geom_iters = [heat_parameters_from_edge_mean(em, 50.0)[0] for em in ems]
+    smoothed = smooth_geometry_many(stencils, geoms, geom_iters)
+    out = []
+    for st, g, em, gi, sm in zip(stencils, geoms, ems, geom_iters, smoothed):
+        params = [heat_parameters_from_edge_mean(em, f) for f in fw]
+        out.append(st._type5(g, gi, params, out_dtype, smoothed=sm.contiguous()))
+    return out
 
 
 def _minmax(values: torch.Tensor, floor: float = 0.0) -> torch.Tensor:

@@ -282,14 +282,32 @@ def solve_dartel_from_surfaces_jobs(requests, *, n_steps: int = 2, loop: int = 6
     and all solves run as one batched solve (solve_dartel_maps_jobs).
     """
 
+    from .surface_stencil import SurfaceStencil, type5_sheets_many
+
     dev = resolve_device(device)
-    src, tgt = [], []
+    stencils, geoms, fws = [], [], []
     for sv, tv, ss, ts, fw in requests:
         if len(fw) < n_steps:
             raise ValueError("each request needs at least n_steps fwhm values")
-        src.append(type5_sheet_maps(sv, ss, fw[:n_steps], dtype, dev))
-        tgt.append(type5_sheet_maps(tv, ts, fw[:n_steps], dtype, dev))
-    return solve_dartel_maps_jobs(torch.cat(src), torch.cat(tgt), n_steps, loop=loop, lmreg=lmreg,
+        for st, vert in ((ss, sv), (ts, tv)):
+            st = st.to(dev) if isinstance(st, SurfaceStencil) else st
+            v = torch.as_tensor(vert, device=dev).to(torch.float32).contiguous()
+            stencils.append(st)
+            geoms.append(st.resample_vertices(v))
+            fws.append(tuple(float(f) for f in fw[:n_steps]))
+    # all sides that share the fwhm list are computed together
+    sheets: list[torch.Tensor | None] = [None] * len(stencils)
+    by_fwhm: dict[tuple, list[int]] = {}
+    for i, f in enumerate(fws):
+        by_fwhm.setdefault(f, []).append(i)
+    for f, members in by_fwhm.items():
+        res = type5_sheets_many([stencils[i] for i in members], [geoms[i] for i in members], f,
+                                dtype)
+        for i, r in zip(members, res):
+            sheets[i] = r
+    src = torch.cat([sheets[2 * k] for k in range(len(requests))])
+    tgt = torch.cat([sheets[2 * k + 1] for k in range(len(requests))])
+    return solve_dartel_maps_jobs(src, tgt, n_steps, loop=loop, lmreg=lmreg,
                                   cycles=cycles, nit=nit, its=its, code=code, device=dev,
                                   dtype=dtype, assume_initial_nonzero=assume_initial_nonzero,
                                   cuda_graph=cuda_graph)

@@ -125,6 +125,38 @@ int csg_smooth_geometry(const float *v, const int *nbr_off, const int *nbr, int 
     return kOk;
 }
 
+// batch meshes [batch][n][3] of one topology, mesh b smoothed iterations[b]
+// (host array) times; ws: 3 n batch floats.
+int csg_smooth_geometry_batch(const float *v, int batch, const int *nbr_off, const int *nbr, int n,
+                              const int *iterations, float *out, void *ws, size_t ws_bytes,
+                              void *stream)
+{
+    if (n < 1 || batch < 1 || batch > kMaxSmoothBatch || iterations == nullptr)
+        return fail(kInvalidArgument, "invalid batched smoothing arguments (batch <= %d)",
+                    kMaxSmoothBatch);
+    if (ws_bytes < (size_t)3 * n * batch * sizeof(float))
+        return fail(kInvalidArgument, "smooth_geometry_batch workspace too small");
+    SmoothBatch sb{};
+    sb.count = batch;
+    int most = 0;
+    for (int b = 0; b < batch; ++b) {
+        if (iterations[b] < 0)
+            return fail(kInvalidArgument, "negative smoothing iterations");
+        sb.iters[b] = iterations[b];
+        most = iterations[b] > most ? iterations[b] : most;
+    }
+    hipStream_t s = S(stream);
+    float *bufs[2] = {out, (float *)ws};
+    const size_t mesh = (size_t)3 * n;
+    // start each mesh in the buffer from which its last step writes `out`
+    for (int b = 0; b < batch; ++b)
+        CSG_HIP_CHECK(hipMemcpyAsync(bufs[iterations[b] & 1] + b * mesh, v + b * mesh,
+                                     mesh * sizeof(float), hipMemcpyDeviceToDevice, s));
+    for (int it = 0; it < most; ++it)
+        CSG_TRY(launch_smooth_geometry_batch(bufs[0], bufs[1], nbr_off, nbr, n, sb, it, s));
+    return kOk;
+}
+
 int csg_vertex_normals(const float *v, const int *faces, const int *inc_off, const int *inc,
                        int n, float *out, void *stream)
 {
@@ -202,9 +234,9 @@ size_t csg_type5_workspace_bytes(int n, int nnz, int nb, int m)
 // src/cat_surface_gpu/surface_stencil.py:521-550
 int csg_type5_sheets(const float *geom, int n, const int *faces, const int *nbr_off,
                      const int *nbr, int nnz, const int *inc_off, const int *inc,
-                     int geom_iterations, int nb, const double *sigma, const int *iterations,
-                     const int *sheet_idx, const double *sheet_w, int m, int out_dtype, void *out,
-                     void *ws, size_t ws_bytes, void *stream)
+                     int geom_iterations, const float *smoothed, int nb, const double *sigma,
+                     const int *iterations, const int *sheet_idx, const double *sheet_w, int m,
+                     int out_dtype, void *out, void *ws, size_t ws_bytes, void *stream)
 {
     HeatBatch hb;
     CSG_TRY(make_batch(nb, sigma, iterations, hb));
@@ -220,8 +252,11 @@ int csg_type5_sheets(const float *geom, int n, const int *faces, const int *nbr_
     double *normed = c.take<double>((size_t)nb * n);
     double *sheets = c.take<double>((size_t)nb * m);
     double *mm = c.take<double>((size_t)nb * 64 * 2 + 64);
-    CSG_TRY(csg_smooth_geometry(geom, nbr_off, nbr, n, geom_iterations, sm, pp,
-                                (size_t)3 * n * sizeof(float), stream));
+    if (smoothed)
+        sm = const_cast<float *>(smoothed);
+    else
+        CSG_TRY(csg_smooth_geometry(geom, nbr_off, nbr, n, geom_iterations, sm, pp,
+                                    (size_t)3 * n * sizeof(float), stream));
     CSG_TRY(launch_vertex_normals(sm, faces, inc_off, inc, n, nr, s));
     CSG_TRY(launch_type5_base(geom, sm, nr, n, base, s));
     for (int b = 0; b < nb; ++b)

@@ -14,12 +14,22 @@ struct HeatBatch {
     double sigma[kMaxHeatBatch];
 };
 
+// Up to kMaxSmoothBatch meshes of one topology smoothed together, each with
+// its own number of steps.
+constexpr int kMaxSmoothBatch = 32;
+struct SmoothBatch {
+    int count;
+    int iters[kMaxSmoothBatch];
+};
+
 template <typename T>
 int launch_resample(const T *x, const int *idx, const double *w, int n, T *out, hipStream_t s);
 int launch_edge_sum(const float *v, const int *faces, int nf, double *partial, int *nblocks,
                     hipStream_t s);
 int launch_smooth_geometry(const float *cur, const int *off, const int *nbr, int n, float *nxt,
                            hipStream_t s);
+int launch_smooth_geometry_batch(float *buf0, float *buf1, const int *off, const int *nbr,
+                                 int n, const SmoothBatch &sb, int it, hipStream_t s);
 int launch_vertex_normals(const float *v, const int *faces, const int *inc_off, const int *inc,
                           int n, float *out, hipStream_t s);
 int launch_type5_base(const float *geom, const float *sm, const float *nr, int n, double *out,

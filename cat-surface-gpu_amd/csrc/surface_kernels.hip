@@ -93,6 +93,36 @@ k_smooth_geometry(const float *__restrict__ cur, const int *__restrict__ off,
     nxt[3 * i + 2] = (float)(((double)cur[3 * i + 2] + sz / 3.0) / den);
 }
 
+// One smoothing step for a batch of meshes of one topology, each with its own
+// number of steps: mesh b reads buffer ((start_b + it) & 1) and writes the
+// other; meshes whose steps are done are left alone.  Same arithmetic as
+// k_smooth_geometry.
+__global__ void __launch_bounds__(kBlock)
+k_smooth_geometry_batch(float *__restrict__ buf0, float *__restrict__ buf1,
+                        const int *__restrict__ off, const int *__restrict__ nbr, int n,
+                        SmoothBatch sb, int it)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (i >= n || it >= sb.iters[b])
+        return;
+    const bool from1 = ((sb.iters[b] + it) & 1) != 0;
+    const float *cur = (from1 ? buf1 : buf0) + (size_t)b * 3 * n;
+    float *nxt = (from1 ? buf0 : buf1) + (size_t)b * 3 * n;
+    int bo = off[i], e = off[i + 1];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int k = bo; k < e; ++k) {
+        const float *p = cur + 3 * nbr[k];
+        sx += (double)p[0];
+        sy += (double)p[1];
+        sz += (double)p[2];
+    }
+    double den = 1.0 + (double)(e - bo) / 3.0;
+    nxt[3 * i] = (float)(((double)cur[3 * i] + sx / 3.0) / den);
+    nxt[3 * i + 1] = (float)(((double)cur[3 * i + 1] + sy / 3.0) / den);
+    nxt[3 * i + 2] = (float)(((double)cur[3 * i + 2] + sz / 3.0) / den);
+}
+
 // Angle-weighted vertex normals (float32).  inc[] lists face*3+corner per
 // vertex in (corner, face) order, the order index_add_ applies them.
 // surface_stencil.py:478-505
@@ -429,6 +459,15 @@ int launch_smooth_geometry(const float *cur, const int *off, const int *nbr, int
 {
     hipLaunchKernelGGL(k_smooth_geometry, dim3(grid_for(n)), dim3(kBlock), 0, s, cur, off, nbr, n,
                        nxt);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+int launch_smooth_geometry_batch(float *buf0, float *buf1, const int *off, const int *nbr,
+                                 int n, const SmoothBatch &sb, int it, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_smooth_geometry_batch, dim3(grid_for(n), sb.count), dim3(kBlock), 0, s,
+                       buf0, buf1, off, nbr, n, sb, it);
     CSG_LAUNCH_CHECK();
     return kOk;
 }

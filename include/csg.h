@@ -173,6 +173,12 @@ int csg_edge_sum(const float *v, const int *faces, int nf, double *partial_out, 
  * (surface_stencil.py:372-388). */
 int csg_smooth_geometry(const float *v, const int *nbr_off, const int *nbr, int n,
                         int iterations, float *out, void *ws, size_t ws_bytes, void *stream);
+/* `batch` meshes [batch][n][3] of one topology, mesh b smoothed iterations[b] (host array)
+ * times, all in the same launches (batch <= 32; ws: 3 n batch floats).  Replaces repeated
+ * SurfaceStencilDevice.smooth_geometry calls (surface_stencil.py:372-388). */
+int csg_smooth_geometry_batch(const float *v, int batch, const int *nbr_off, const int *nbr, int n,
+                              const int *iterations, float *out, void *ws, size_t ws_bytes,
+                              void *stream);
 
 /* Angle-weighted vertex normals; inc lists face*3+corner per vertex in (corner, face)
  * order.  Replaces SurfaceStencilDevice.vertex_normals (surface_stencil.py:478-505). */
@@ -198,13 +204,15 @@ int csg_sheet_gather(const double *values, int n_values, int nb, const int *idx,
                      int m, double *out, void *stream);
 
 /* Whole curvature_type5_to_sheet_many for nb fwhms (sigma / iterations host arrays).
+ * `smoothed` may give the fwhm-50 smoothed geometry already computed (e.g. by
+ * csg_smooth_geometry_batch); NULL smooths `geom` geom_iterations times here.
  * Replaces SurfaceStencilDevice.curvature_type5_to_sheet_many (surface_stencil.py:521-550). */
 size_t csg_type5_workspace_bytes(int n, int nnz, int nb, int m);
 int csg_type5_sheets(const float *geom, int n, const int *faces, const int *nbr_off,
                      const int *nbr, int nnz, const int *inc_off, const int *inc,
-                     int geom_iterations, int nb, const double *sigma, const int *iterations,
-                     const int *sheet_idx, const double *sheet_w, int m, int out_dtype, void *out,
-                     void *ws, size_t ws_bytes, void *stream);
+                     int geom_iterations, const float *smoothed, int nb, const double *sigma,
+                     const int *iterations, const int *sheet_idx, const double *sheet_w, int m,
+                     int out_dtype, void *out, void *ws, size_t ws_bytes, void *stream);
 
 /* Depth potential (L + alpha diag(area)) x = rhs by colour-ordered SOR for `batch`
  * meshes of one topology (pts [batch][n][3], x and relres [batch]).  The system is a
