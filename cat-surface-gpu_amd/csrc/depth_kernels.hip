@@ -259,16 +259,20 @@ k_colour_order(const int *__restrict__ verts, const int *__restrict__ ell_src, i
     }
 }
 
-// Grid-wide barrier for a co-resident grid, as a two-level
-// tree: blocks arrive in groups of kBarrierGroup on their own counter, the
-// last arrival of each group arrives at the root, and the last root arrival
-// advances the generation word that everyone waits on.  Arrivals on one
-// counter serialise, so the tree keeps the critical path to about
-// 2·kBarrierGroup atomic round trips.  Only thread 0 of a block touches the
-// counters; its agent-scope acquire/release (after the block's
-// __syncthreads) publishes the block's stores and invalidates stale cache
-// lines for the whole block.  Returns false if the block gave up after
-// kSpinLimit polls; the caller then exits, so every wave terminates.
+// Grid-wide barrier for a co-resident grid, as a two-level tree: blocks
+// arrive in groups of kBarrierGroup on their own counter, the last arrival of
+// each group arrives at the root, and the last root arrival advances the
+// generation word that everyone polls.  Arrivals on one counter serialise, so
+// the tree keeps the critical path to about 2·kBarrierGroup atomic round
+// trips.  Only thread 0 of a block touches the counters, after the block's
+// __syncthreads.  Memory ordering uses as few agent-scope cache operations as
+// the chain needs: each arrival is a release (publishes the block's stores);
+// a group's last arrival and the root's last arrival take an acquire fence
+// before their own release, so the chain arrival -> group -> root ->
+// generation carries every block's stores; waiters poll with relaxed loads and
+// take one acquire fence after the generation changes.  Returns false if the
+// block gave up after kSpinLimit polls; the caller then exits, so every wave
+// terminates.
 constexpr long kSpinLimit = 1L << 22;
 constexpr int kBarrierGroup = 8;
 
@@ -281,24 +285,27 @@ __device__ bool grid_barrier(unsigned *bar, int nblocks, int *timeout)
         const int groups = (nblocks + kBarrierGroup - 1) / kBarrierGroup;
         unsigned *root = bar + (size_t)groups * kBarrierSlotWords;
         unsigned *gen_p = root + kBarrierSlotWords;
-        const unsigned gen = __hip_atomic_load(gen_p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned gen = __hip_atomic_load(gen_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int g = blockIdx.x / kBarrierGroup;
         const int gsize = min(kBarrierGroup, nblocks - g * kBarrierGroup);
         unsigned *cnt = bar + (size_t)g * kBarrierSlotWords;
         bool released = false;
-        unsigned a = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned a = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         if (a == (unsigned)gsize - 1) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            unsigned r = __hip_atomic_fetch_add(root, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned r =
+                __hip_atomic_fetch_add(root, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             if (r == (unsigned)groups - 1) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 __hip_atomic_store(root, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add(gen_p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(gen_p, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 released = true;
             }
         }
         if (!released) {
             long spins = 0;
-            while (__hip_atomic_load(gen_p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            while (__hip_atomic_load(gen_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 if (++spins > kSpinLimit) {
                     __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     ok = 0;
@@ -306,6 +313,7 @@ __device__ bool grid_barrier(unsigned *bar, int nblocks, int *timeout)
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
     }
     __syncthreads();
