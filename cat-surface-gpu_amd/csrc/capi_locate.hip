@@ -80,7 +80,7 @@ int csg_grid_index_fill(const void *plan_ws, int nf, int grid, int npairs, int *
 
 size_t csg_rotation_costs_workspace_bytes(int ns, int na)
 {
-    return align256(sizeof(double) * (size_t)grid_for(ns) * na) + 256;
+    return align256(sizeof(double) * (size_t)rotation_cost_blocks(ns) * na) + 256;
 }
 
 int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
@@ -110,7 +110,7 @@ int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
     a.na = na;
     a.costs = costs;
     a.partial = (double *)b;
-    a.uncovered = (int *)(b + align256(sizeof(double) * (size_t)grid_for(ns) * na));
+    a.uncovered = (int *)(b + align256(sizeof(double) * (size_t)rotation_cost_blocks(ns) * na));
     a.loc_face = loc_face;
     a.loc_bary = loc_bary;
     CSG_HIP_CHECK(hipMemsetAsync(a.uncovered, 0, sizeof(int), s));

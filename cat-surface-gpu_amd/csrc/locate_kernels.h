@@ -14,6 +14,13 @@ int grid_index_fill(const int *box, const int *offsets, int nf, int g, int npair
                     int *vals_tmp, int *keys, int *vals, int *cell_off, void *tmp,
                     size_t tmp_bytes, hipStream_t s);
 
+// Lanes that share one query point in the location kernels, each scanning
+// every kLocateLanes-th candidate of the point's cell.
+constexpr int kLocateLanes = 4;
+
+// Blocks of the rotation-cost kernel per angle.
+inline int rotation_cost_blocks(int ns) { return (int)grid_for((size_t)ns * kLocateLanes); }
+
 struct RotationCostArgs {
     const float *src_pts;
     const double *src_vals;
@@ -26,7 +33,7 @@ struct RotationCostArgs {
     const double *angles;
     int na;
     double *costs;
-    double *partial;  // [na][ceil(ns / 256)]
+    double *partial;  // [na][rotation_cost_blocks(ns)]
     int *uncovered;
     int *loc_face;    // [na][ns] or null
     float *loc_bary;  // [na][ns][3] or null

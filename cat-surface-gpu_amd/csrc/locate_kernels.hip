@@ -189,14 +189,22 @@ __device__ void rotation_matrix_f32(double a0, double a1, double a2, float *m)
                            rz[r + 6] * first[2 + 3 * c];
 }
 
-// Locate q among the candidates of its cell; returns the face (or -1) and
-// the barycentric weights of the chosen point.  surface_rotation.py:354-447
-__device__ int locate_point(const float *q, const float *__restrict__ tri,
-                            const int *__restrict__ cand, int ncand, float *bary)
+// Best candidate of one lane: candidates c = lane, lane + lanes, ... of q's
+// cell, in increasing order; the first minimum of the squared distance wins.
+// Returns the candidate position (or a value >= ncand if none); *face and
+// bp[] receive the face and its closest point.  surface_rotation.py:354-447
+struct LocateBest {
+    float dist;
+    int pos, face;
+    float bp[3];
+};
+
+__device__ __forceinline__ LocateBest scan_candidates(const float *q, const float *__restrict__ tri,
+                                                     const int *__restrict__ cand, int ncand,
+                                                     int lane, int lanes)
 {
-    float best = INFINITY, bp[3] = {0.0f, 0.0f, 0.0f};
-    int bf = -1;
-    for (int c = 0; c < ncand; ++c) {
+    LocateBest r{INFINITY, 0x7fffffff, -1, {0.0f, 0.0f, 0.0f}};
+    for (int c = lane; c < ncand; c += lanes) {
         int face = cand[c];
         const float *t = tri + 9 * (size_t)face;
         float p0[3] = {t[0], t[1], t[2]}, p1[3] = {t[3], t[4], t[5]}, p2[3] = {t[6], t[7], t[8]};
@@ -249,16 +257,41 @@ __device__ int locate_point(const float *q, const float *__restrict__ tri,
                 cpt[2] = q2[2];
             }
         }
-        if (dist < best) {
-            best = dist;
-            bf = face;
-            bp[0] = cpt[0];
-            bp[1] = cpt[1];
-            bp[2] = cpt[2];
+        if (dist < r.dist) {
+            r.dist = dist;
+            r.pos = c;
+            r.face = face;
+            r.bp[0] = cpt[0];
+            r.bp[1] = cpt[1];
+            r.bp[2] = cpt[2];
         }
     }
-    if (bf < 0)
-        return -1;
+    return r;
+}
+
+// Combine the lanes' bests of one query (lanes are consecutive lanes of a
+// wave, `lanes` a power of two): smaller distance first, then the earlier
+// candidate, which is the sequential loop's first minimum.
+__device__ __forceinline__ LocateBest reduce_lanes(LocateBest r, int lanes)
+{
+    for (int m = 1; m < lanes; m <<= 1) {
+        LocateBest o;
+        o.dist = __shfl_xor(r.dist, m);
+        o.pos = __shfl_xor(r.pos, m);
+        o.face = __shfl_xor(r.face, m);
+        o.bp[0] = __shfl_xor(r.bp[0], m);
+        o.bp[1] = __shfl_xor(r.bp[1], m);
+        o.bp[2] = __shfl_xor(r.bp[2], m);
+        if (o.dist < r.dist || (o.dist == r.dist && o.pos < r.pos))
+            r = o;
+    }
+    return r;
+}
+
+// Barycentric weights of the chosen point on face bf.
+__device__ __forceinline__ void locate_weights(const float *__restrict__ tri, int bf,
+                                               const float *bp, float *bary)
+{
     const float *t = tri + 9 * (size_t)bf;
     const float *tv[3] = {t, t + 3, t + 6};
     const int order[3][3] = {{1, 2, 0}, {2, 0, 1}, {0, 1, 2}};
@@ -272,7 +305,6 @@ __device__ int locate_point(const float *q, const float *__restrict__ tri,
         cross3(nn, ho, ve);
         bary[k] = dot3(po, ve) / dot3(up, ve);
     }
-    return bf;
 }
 
 __device__ __forceinline__ int cell_of(const float *p, int g)
@@ -285,8 +317,10 @@ __device__ __forceinline__ int cell_of(const float *p, int g)
     return c[0] + g * (c[1] + g * c[2]);
 }
 
-// One block of points for one candidate rotation (blockIdx.y); per-block
+// kLocateLanes consecutive lanes share one point, each scanning every
+// kLocateLanes-th candidate; blockIdx.y is the candidate rotation; per-block
 // float64 partial SSD.  `uncovered` counts points whose cell has no triangle.
+
 __global__ void __launch_bounds__(kBlock)
 k_rotation_cost(const float *__restrict__ src_pts, const double *__restrict__ src_vals, int ns,
                 const double *__restrict__ tgt_vals, const int *__restrict__ faces,
@@ -301,20 +335,29 @@ k_rotation_cost(const float *__restrict__ src_pts, const double *__restrict__ sr
     if (threadIdx.x == 0)
         rotation_matrix_f32(angles[3 * a], angles[3 * a + 1], angles[3 * a + 2], m);
     __syncthreads();
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / kLocateLanes, lane = t % kLocateLanes;
     double e = 0.0;
+    // every lane of a point takes part in the shuffles, including padding lanes
+    float q[3] = {0.0f, 0.0f, 0.0f};
+    int b = 0, nc = 0;
     if (i < ns) {
         const float *p = src_pts + 3 * i;
-        float q[3];
         for (int r = 0; r < 3; ++r)
             q[r] = p[0] * m[3 * r] + p[1] * m[3 * r + 1] + p[2] * m[3 * r + 2];
         int cell = cell_of(q, g);
-        int b = cell_off[cell], nc = cell_off[cell + 1] - b;
-        float w[3];
-        int f = nc > 0 ? locate_point(q, tri, cell_faces + b, nc, w) : -1;
-        if (f < 0) {
+        b = cell_off[cell];
+        nc = cell_off[cell + 1] - b;
+    }
+    LocateBest best = scan_candidates(q, tri, cell_faces + b, nc, lane, kLocateLanes);
+    best = reduce_lanes(best, kLocateLanes);
+    if (i < ns && lane == 0) {
+        if (best.face < 0) {
             atomicAdd(uncovered, 1);
         } else {
+            const int f = best.face;
+            float w[3];
+            locate_weights(tri, f, best.bp, w);
             double sv = tgt_vals[faces[3 * f]] * (double)w[0] +
                         tgt_vals[faces[3 * f + 1]] * (double)w[1] +
                         tgt_vals[faces[3 * f + 2]] * (double)w[2];
@@ -327,9 +370,9 @@ k_rotation_cost(const float *__restrict__ src_pts, const double *__restrict__ sr
             }
         }
     }
-    double s = block_sum(e, red);
+    double sum = block_sum(e, red);
     if (threadIdx.x == 0)
-        partial[(size_t)a * nblocks + blockIdx.x] = s;
+        partial[(size_t)a * nblocks + blockIdx.x] = sum;
 }
 
 __global__ void k_sum_rows(const double *__restrict__ partial, int nblocks, int rows,
@@ -346,7 +389,7 @@ __global__ void k_sum_rows(const double *__restrict__ partial, int nblocks, int 
 
 int launch_rotation_costs(const RotationCostArgs &a, hipStream_t s)
 {
-    int nb = (int)grid_for(a.ns);
+    int nb = rotation_cost_blocks(a.ns);
     dim3 grid(nb, a.na);
     hipLaunchKernelGGL(k_rotation_cost, grid, dim3(kBlock), 0, s, a.src_pts, a.src_vals, a.ns,
                        a.tgt_vals, a.faces, a.tri, a.cell_off, a.cell_faces, a.grid, a.angles,
@@ -421,6 +464,9 @@ __device__ double closest_on_triangle(const double *p, const double *a, const do
 // For each query: closest triangle among the candidates of its cell (first
 // minimum); `far` counts queries whose best squared distance exceeds
 // reach2, i.e. whose true closest triangle may lie outside the searched cells.
+// kLocateLanes consecutive lanes share one query, each scanning every
+// kLocateLanes-th candidate; the lanes' bests are combined as in the
+// sequential scan (smaller distance, then earlier candidate).
 __global__ void __launch_bounds__(kBlock)
 k_closest_triangles(const double *__restrict__ q, int nq, const double *__restrict__ verts,
                     const int *__restrict__ faces, const int *__restrict__ cell_off,
@@ -428,20 +474,27 @@ k_closest_triangles(const double *__restrict__ q, int nq, const double *__restri
                     int *__restrict__ face_out, int *__restrict__ idx_out,
                     double *__restrict__ w_out, int *__restrict__ far_count)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq)
-        return;
-    double p[3] = {q[3 * i], q[3 * i + 1], q[3 * i + 2]};
-    double cell = 2.0 / (double)g;
-    int c[3];
-    for (int d = 0; d < 3; ++d) {
-        long k = (long)floor((p[d] + 1.0) / cell);
-        c[d] = (int)(k < 0 ? 0 : (k > g - 1 ? g - 1 : k));
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / kLocateLanes, lane = t % kLocateLanes;
+    double p[3] = {0.0, 0.0, 0.0};
+    int begin = 0, end = 0;
+    if (i < nq) {
+        p[0] = q[3 * i];
+        p[1] = q[3 * i + 1];
+        p[2] = q[3 * i + 2];
+        double cell = 2.0 / (double)g;
+        int c[3];
+        for (int d = 0; d < 3; ++d) {
+            long k = (long)floor((p[d] + 1.0) / cell);
+            c[d] = (int)(k < 0 ? 0 : (k > g - 1 ? g - 1 : k));
+        }
+        int id = c[0] + g * (c[1] + g * c[2]);
+        begin = cell_off[id];
+        end = cell_off[id + 1];
     }
-    int id = c[0] + g * (c[1] + g * c[2]);
     double best = INFINITY, bw[3] = {0, 0, 0};
-    int bf = -1;
-    for (int k = cell_off[id]; k < cell_off[id + 1]; ++k) {
+    int bf = -1, bpos = 0x7fffffff;
+    for (int k = begin + lane; k < end; k += kLocateLanes) {
         int f = cell_faces[k];
         const int *fv = faces + 3 * f;
         double w[3];
@@ -450,11 +503,27 @@ k_closest_triangles(const double *__restrict__ q, int nq, const double *__restri
         if (dist < best) {
             best = dist;
             bf = f;
+            bpos = k;
             bw[0] = w[0];
             bw[1] = w[1];
             bw[2] = w[2];
         }
     }
+    for (int m = 1; m < kLocateLanes; m <<= 1) {
+        double ob = __shfl_xor(best, m);
+        int of = __shfl_xor(bf, m), op = __shfl_xor(bpos, m);
+        double o0 = __shfl_xor(bw[0], m), o1 = __shfl_xor(bw[1], m), o2 = __shfl_xor(bw[2], m);
+        if (ob < best || (ob == best && op < bpos)) {
+            best = ob;
+            bf = of;
+            bpos = op;
+            bw[0] = o0;
+            bw[1] = o1;
+            bw[2] = o2;
+        }
+    }
+    if (i >= nq || lane != 0)
+        return;
     if (bf < 0 || best > reach2)
         atomicAdd(far_count, 1);
     face_out[i] = bf;
@@ -472,7 +541,8 @@ int launch_closest_triangles(const double *q, int nq, const double *verts, const
 {
     if (nq <= 0)
         return kOk;
-    hipLaunchKernelGGL(k_closest_triangles, dim3(grid_for(nq)), dim3(kBlock), 0, s, q, nq, verts,
+    hipLaunchKernelGGL(k_closest_triangles, dim3(grid_for((size_t)nq * kLocateLanes)),
+                       dim3(kBlock), 0, s, q, nq, verts,
                        faces, cell_off, cell_faces, g, reach2, face_out, idx_out, w_out,
                        far_count);
     CSG_LAUNCH_CHECK();
