@@ -29,7 +29,7 @@ from .gifti import read_gifti, write_gifti_surface, INTENT_POINTSET, INTENT_TRIA
 from .rotation_feature import compute_rotation_features
 from .rotation_pipeline import RotationPipeline, read_rotation_values
 from .stencil_builder import build_device_stencil
-from .surface_rotation import rotation_matrix_np
+from .surface_rotation import rotation_matrix_np, search_rotations
 from .surface_stencil import SurfaceStencil
 from .surface_warp import apply_flow_to_stenciled_sphere, solve_dartel_from_surfaces_jobs
 
@@ -402,30 +402,33 @@ def run_cat_surface_gpu_pipelines(
         torch.cuda.synchronize(dev)
         shared["rotation_feature_seconds"] = time.perf_counter() - t
 
-        t_idx = 0.0
-        t_search = 0.0
+        t = time.perf_counter()
+        rots, problems = [], []
         for st in states:
-            t = time.perf_counter()
             rot = RotationPipeline.from_stencil(st.tgt_stencil, device=dev,
                                                 grid_size=rotation_grid_size,
                                                 margin=rotation_margin)
-            torch.cuda.synchronize(dev)
-            t_idx += time.perf_counter() - t
-            t = time.perf_counter()
             src_points = (st.src_stencil.sphere_points64
                           if st.src_stencil.sphere_points64 is not None
                           else st.src_stencil.sphere_points)
-            if rotation_search is not None:
-                result = rotation_search(rot, src_points, st.src_vals, st.tgt_vals)
-            else:
-                result = rot.search(src_points, st.src_vals, st.tgt_vals, point_chunk=point_chunk,
-                                    refine=rotation_refine, max_iter=rotation_max_iter,
-                                    tol=rotation_tol, simplex_step=rotation_simplex_step)
-            t_search += time.perf_counter() - t
+            rots.append((rot, src_points))
+        torch.cuda.synchronize(dev)
+        shared["rotation_index_upload_seconds"] = time.perf_counter() - t
+        t = time.perf_counter()
+        if rotation_search is not None:
+            results = [rotation_search(rot, pts, st.src_vals, st.tgt_vals)
+                       for (rot, pts), st in zip(rots, states)]
+        else:
+            # every job's seed scan and Nelder-Mead in lockstep (same results)
+            problems = [(rot.index, rot.prepare_inputs(pts, st.src_vals, st.tgt_vals))
+                        for (rot, pts), st in zip(rots, states)]
+            results = search_rotations(problems, refine=rotation_refine,
+                                       max_iter=rotation_max_iter, tol=rotation_tol,
+                                       simplex_step=rotation_simplex_step)
+        for st, result in zip(states, results):
             st.angle = np.asarray(result.angle.detach().cpu(), dtype=np.float64).reshape(3)
             st.rotation_cost = float(result.cost.detach().cpu())
-        shared["rotation_index_upload_seconds"] = t_idx
-        shared["rotation_search_seconds"] = t_search
+        shared["rotation_search_seconds"] = time.perf_counter() - t
 
         t = time.perf_counter()
         builder = rotated_stencil_builder

@@ -197,8 +197,10 @@ class RotationGridIndexDevice:
             raise ValueError("target_values length does not match the target sphere")
         return RotationCostInputs(src, sv.contiguous(), tv.contiguous())
 
-    def _costs(self, inputs: RotationCostInputs, angles: torch.Tensor,
-               locate: bool = False):
+    def _launch_costs(self, inputs: RotationCostInputs, angles, locate: bool = False,
+                      uncovered_dev: torch.Tensor | None = None):
+        """Enqueue one cost launch; with ``uncovered_dev`` nothing synchronises."""
+
         ang = torch.as_tensor(angles, dtype=torch.float64, device=self.device).reshape(-1, 3)
         ang = ang.contiguous()
         na, ns = int(ang.shape[0]), int(inputs.source_points.shape[0])
@@ -216,8 +218,14 @@ class RotationGridIndexDevice:
             _lib.ptr(self.target_triangles), _lib.ptr(self.candidate_offsets),
             _lib.ptr(self.candidate_faces), int(self.grid_size), _lib.ptr(ang), na,
             _lib.ptr(costs), _lib.ptr(loc_f), _lib.ptr(loc_b), _lib.ptr(ws), ws.numel(),
-            ctypes.byref(uncovered), _lib.stream_handle(self.device)), "csg_rotation_costs")
-        if uncovered.value:
+            None if uncovered_dev is not None else ctypes.byref(uncovered),
+            _lib.ptr(uncovered_dev), _lib.stream_handle(self.device)), "csg_rotation_costs")
+        return costs, loc_f, loc_b, (None if uncovered_dev is not None else uncovered.value)
+
+    def _costs(self, inputs: RotationCostInputs, angles: torch.Tensor,
+               locate: bool = False):
+        costs, loc_f, loc_b, uncovered = self._launch_costs(inputs, angles, locate)
+        if uncovered:
             raise RuntimeError("Rotation is not covered by target sphere candidate index")
         return costs, loc_f, loc_b
 
@@ -267,60 +275,8 @@ class RotationGridIndexDevice:
         each evaluation is one device cost launch.
         """
 
-        if max_iter < 1 or tol < 0.0 or simplex_step <= 0.0:
-            raise ValueError("max_iter, tol, and simplex_step must be positive")
-        seed = np.asarray(torch.as_tensor(initial_angles, dtype=torch.float64).cpu(),
-                          dtype=np.float64).reshape(-1)
-        if seed.size != 3:
-            raise ValueError("initial_angles must contain exactly three angles")
-        simplex = np.stack([seed, seed + (simplex_step, 0.0, 0.0), seed + (0.0, simplex_step, 0.0),
-                            seed + (0.0, 0.0, simplex_step)])
-
-        def cost(points: np.ndarray) -> np.ndarray:
-            return self.cost_batch_prepared(inputs, torch.as_tensor(points)).cpu().numpy()
-
-        f = cost(simplex)
-        iteration = 0
-        for iteration in range(max_iter):
-            hi, lo, sh = 0, 0, 1
-            for i in range(4):
-                if f[i] > f[hi]:
-                    sh, hi = hi, i
-                elif f[i] > f[sh] and i != hi:
-                    sh = i
-                if f[i] < f[lo]:
-                    lo = i
-            centroid = np.zeros(3)
-            for i in range(4):
-                if i != hi:
-                    centroid += simplex[i]
-            centroid /= 3.0
-            refl = centroid + (centroid - simplex[hi])
-            fr = cost(refl[None])[0]
-            if fr < f[lo]:
-                exp = centroid + 2.0 * (refl - centroid)
-                fe = cost(exp[None])[0]
-                if fe < fr:
-                    simplex[hi], f[hi] = exp, fe
-                else:
-                    simplex[hi], f[hi] = refl, fr
-            elif fr < f[sh]:
-                simplex[hi], f[hi] = refl, fr
-            else:
-                con = centroid + 0.5 * (simplex[hi] - centroid)
-                fc = cost(con[None])[0]
-                if fc < f[hi]:
-                    simplex[hi], f[hi] = con, fc
-                else:
-                    for i in range(4):
-                        if i != lo:
-                            simplex[i] = simplex[lo] + 0.5 * (simplex[i] - simplex[lo])
-                            f[i] = cost(simplex[i][None])[0]
-            if np.max(np.abs(f - f[lo])) < tol:
-                break
-        dev = self.device
-        return (torch.as_tensor(simplex[lo].copy(), device=dev),
-                torch.as_tensor(f[lo], dtype=torch.float64, device=dev), iteration + 1)
+        return refine_nelder_mead_many([(self, inputs, initial_angles)], max_iter=max_iter,
+                                       tol=tol, simplex_step=simplex_step)[0]
 
     def refine_nelder_mead(self, source_points, source_values, target_values, initial_angles, *,
                            point_chunk: int = 1024, max_iter: int = 500, tol: float = 1.0e-4,
@@ -353,3 +309,139 @@ class RotationGridIndexDevice:
         return self.search_rotation_prepared(inputs, point_chunk=point_chunk, refine=refine,
                                              max_iter=max_iter, tol=tol,
                                              simplex_step=simplex_step)
+
+
+def _nelder_mead_steps(seed, max_iter: int, tol: float, simplex_step: float):
+    """CAT-ordered Nelder-Mead (surface_rotation.py:594-699) as a generator: it
+    yields the angle triples it needs evaluated (an [k, 3] array) and receives
+    their costs; it returns (angle, cost, iterations)."""
+
+    simplex = np.stack([seed, seed + (simplex_step, 0.0, 0.0), seed + (0.0, simplex_step, 0.0),
+                        seed + (0.0, 0.0, simplex_step)])
+    f = np.array((yield simplex.copy()), dtype=np.float64)
+    iteration = 0
+    for iteration in range(max_iter):
+        hi, lo, sh = 0, 0, 1
+        for i in range(4):
+            if f[i] > f[hi]:
+                sh, hi = hi, i
+            elif f[i] > f[sh] and i != hi:
+                sh = i
+            if f[i] < f[lo]:
+                lo = i
+        centroid = np.zeros(3)
+        for i in range(4):
+            if i != hi:
+                centroid += simplex[i]
+        centroid /= 3.0
+        refl = centroid + (centroid - simplex[hi])
+        fr = (yield refl[None])[0]
+        if fr < f[lo]:
+            exp = centroid + 2.0 * (refl - centroid)
+            fe = (yield exp[None])[0]
+            if fe < fr:
+                simplex[hi], f[hi] = exp, fe
+            else:
+                simplex[hi], f[hi] = refl, fr
+        elif fr < f[sh]:
+            simplex[hi], f[hi] = refl, fr
+        else:
+            con = centroid + 0.5 * (simplex[hi] - centroid)
+            fc = (yield con[None])[0]
+            if fc < f[hi]:
+                simplex[hi], f[hi] = con, fc
+            else:
+                for i in range(4):
+                    if i != lo:
+                        simplex[i] = simplex[lo] + 0.5 * (simplex[i] - simplex[lo])
+                        f[i] = (yield simplex[i][None])[0]
+        if np.max(np.abs(f - f[lo])) < tol:
+            break
+    return simplex[lo].copy(), float(f[lo]), iteration + 1
+
+
+def refine_nelder_mead_many(problems, *, max_iter: int = 500, tol: float = 1.0e-4,
+                            simplex_step: float = 0.1):
+    """Nelder-Mead refinement of several independent rotation problems in
+    lockstep.  ``problems`` holds (index, prepared inputs, seed angles); each
+    round launches the cost evaluations every problem's simplex needs and reads
+    all costs back with one synchronisation.  Each problem follows exactly the
+    sequential algorithm; returns [(angle, cost, iterations)] as tensors."""
+
+    if max_iter < 1 or tol < 0.0 or simplex_step <= 0.0:
+        raise ValueError("max_iter, tol, and simplex_step must be positive")
+    gens, pending, results = [], [], [None] * len(problems)
+    for k, (index, inputs, initial) in enumerate(problems):
+        seed = np.asarray(torch.as_tensor(initial, dtype=torch.float64).cpu(),
+                          dtype=np.float64).reshape(-1)
+        if seed.size != 3:
+            raise ValueError("initial_angles must contain exactly three angles")
+        gen = _nelder_mead_steps(seed, max_iter, tol, simplex_step)
+        gens.append(gen)
+        pending.append(next(gen))
+    active = list(range(len(problems)))
+    while active:
+        launched = []
+        for k in active:
+            index, inputs, _ = problems[k]
+            unc = torch.zeros(1, dtype=torch.int32, device=index.device)
+            costs, _, _, _ = index._launch_costs(inputs, torch.as_tensor(pending[k]),
+                                                 uncovered_dev=unc)
+            launched.append((k, costs, unc))
+        host = torch.cat([torch.cat([c.to(torch.float64),
+                                     u.to(torch.float64).to(c.device)]) for _, c, u in launched])
+        host = host.cpu().numpy()
+        still, pos = [], 0
+        for k, costs, _ in launched:
+            n = int(costs.numel())
+            vals, unc = host[pos:pos + n], host[pos + n]
+            pos += n + 1
+            if unc:
+                raise RuntimeError("Rotation is not covered by target sphere candidate index")
+            try:
+                pending[k] = gens[k].send(vals)
+                still.append(k)
+            except StopIteration as done:
+                angle, cost, iterations = done.value
+                dev = problems[k][0].device
+                results[k] = (torch.as_tensor(angle, device=dev),
+                              torch.as_tensor(cost, dtype=torch.float64, device=dev), iterations)
+        active = still
+    return results
+
+
+def search_rotations(problems, *, refine: bool = True, max_iter: int = 500, tol: float = 1.0e-4,
+                     simplex_step: float = 0.1) -> list[RotationSearchResult]:
+    """search_rotation_prepared for several independent problems ((index,
+    prepared inputs) pairs): the 46-seed scans are launched together and read
+    with one synchronisation, then the Nelder-Mead refinements run in lockstep
+    (refine_nelder_mead_many).  Each result equals the single-problem search."""
+
+    seeds_np = official_seed_grid_angles()
+    launched = []
+    for index, inputs in problems:
+        seeds = torch.as_tensor(seeds_np, dtype=torch.float64, device=index.device)
+        unc = torch.zeros(1, dtype=torch.int32, device=index.device)
+        costs, _, _, _ = index._launch_costs(inputs, seeds, uncovered_dev=unc)
+        launched.append((seeds, costs, unc))
+    uncovered = torch.cat([u for _, _, u in launched]).cpu()
+    if bool(uncovered.any()):
+        raise RuntimeError("Rotation is not covered by target sphere candidate index")
+    firsts = []
+    for seeds, costs, _ in launched:
+        best = int(torch.argmin(costs).item())
+        firsts.append((seeds[best], costs[best], costs))
+    if refine:
+        refined = refine_nelder_mead_many(
+            [(index, inputs, first[0]) for (index, inputs), first in zip(problems, firsts)],
+            max_iter=max_iter, tol=tol, simplex_step=simplex_step)
+    else:
+        refined = [(a.clone(), c.clone(), 0) for a, c, _ in firsts]
+    out = []
+    for (index, _), (seed_angle, seed_cost, seed_costs), (angle, cost, iterations) in zip(
+            problems, firsts, refined):
+        out.append(RotationSearchResult(seed_angle=seed_angle.clone(), seed_cost=seed_cost.clone(),
+                                        seed_costs=seed_costs, angle=angle, cost=cost,
+                                        rotation_matrix=index.rotation_matrix(angle),
+                                        iterations=iterations))
+    return out

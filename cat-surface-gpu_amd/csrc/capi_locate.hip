@@ -88,7 +88,7 @@ int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
                        const int *cell_off, const int *cell_faces, int grid,
                        const double *angles, int na, double *costs, int *loc_face,
                        float *loc_bary, void *ws, size_t ws_bytes, int *uncovered_host,
-                       void *stream)
+                       int *uncovered_dev, void *stream)
 {
     if (ns < 1 || na < 1 || grid < 1)
         return fail(kInvalidArgument, "invalid rotation cost arguments");
@@ -110,7 +110,9 @@ int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
     a.na = na;
     a.costs = costs;
     a.partial = (double *)b;
-    a.uncovered = (int *)(b + align256(sizeof(double) * (size_t)rotation_cost_blocks(ns) * na));
+    a.uncovered = uncovered_dev
+                      ? uncovered_dev
+                      : (int *)(b + align256(sizeof(double) * (size_t)rotation_cost_blocks(ns) * na));
     a.loc_face = loc_face;
     a.loc_bary = loc_bary;
     CSG_HIP_CHECK(hipMemsetAsync(a.uncovered, 0, sizeof(int), s));

@@ -375,16 +375,19 @@ k_rotation_cost(const float *__restrict__ src_pts, const double *__restrict__ sr
         partial[(size_t)a * nblocks + blockIdx.x] = sum;
 }
 
-__global__ void k_sum_rows(const double *__restrict__ partial, int nblocks, int rows,
-                           double *__restrict__ out)
+// out[r] = sum of row r of the block partials; one block per row, a fixed
+// (deterministic) summation order.
+__global__ void __launch_bounds__(kBlock)
+k_sum_rows(const double *__restrict__ partial, int nblocks, int rows, double *__restrict__ out)
 {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows)
-        return;
+    __shared__ double red[kBlock / 64];
+    const int r = blockIdx.x;
     double s = 0.0;
-    for (int k = 0; k < nblocks; ++k)
+    for (int k = threadIdx.x; k < nblocks; k += blockDim.x)
         s += partial[(size_t)r * nblocks + k];
-    out[r] = s;
+    s = block_sum(s, red);
+    if (threadIdx.x == 0)
+        out[r] = s;
 }
 
 int launch_rotation_costs(const RotationCostArgs &a, hipStream_t s)
@@ -395,8 +398,7 @@ int launch_rotation_costs(const RotationCostArgs &a, hipStream_t s)
                        a.tgt_vals, a.faces, a.tri, a.cell_off, a.cell_faces, a.grid, a.angles,
                        a.partial, nb, a.uncovered, a.loc_face, a.loc_bary);
     CSG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sum_rows, dim3(grid_for(a.na, 64)), dim3(64), 0, s, a.partial, nb, a.na,
-                       a.costs);
+    hipLaunchKernelGGL(k_sum_rows, dim3(a.na), dim3(kBlock), 0, s, a.partial, nb, a.na, a.costs);
     CSG_LAUNCH_CHECK();
     return kOk;
 }

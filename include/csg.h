@@ -260,8 +260,10 @@ size_t csg_grid_index_fill_workspace_bytes(int npairs);
 int csg_grid_index_fill(const void *plan_ws, int nf, int grid, int npairs, int *cell_off,
                         int *cell_faces, void *ws, size_t ws_bytes, void *stream);
 
-/* CAT rotation SSD for na candidate angle triples (device); uncovered_host (optional)
- * receives the count of points outside the index after a synchronisation.
+/* CAT rotation SSD for na candidate angle triples (device).  The count of points outside
+ * the index is kept in a device int: uncovered_dev if given (the call then does not
+ * synchronise), else in the workspace; uncovered_host (optional) receives it after a
+ * synchronisation.
  * Replaces RotationGridIndexDevice.cost_batch_prepared (surface_rotation.py:512-546). */
 size_t csg_rotation_costs_workspace_bytes(int ns, int na);
 int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
@@ -269,7 +271,7 @@ int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
                        const int *cell_off, const int *cell_faces, int grid,
                        const double *angles, int na, double *costs, int *loc_face,
                        float *loc_bary, void *ws, size_t ws_bytes, int *uncovered_host,
-                       void *stream);
+                       int *uncovered_dev, void *stream);
 
 /* Exact closest triangle (first minimum among the query cell's candidates) with
  * barycentric weights; far_host counts queries farther than `reach` from their best
