@@ -105,6 +105,42 @@ def _finish_feature(stencil: SurfaceStencilDevice, geometry: torch.Tensor, value
                                  iterations=iterations, relative_residual=residual)
 
 
+def _finish_features(stencils: Sequence[SurfaceStencilDevice], geometries, values, iterations,
+                     residuals, edge_means_, groups) -> tuple[RotationFeatureResult, ...]:
+    """_finish_feature for many sides: one smoothing and one normalisation per topology group."""
+
+    out: list[RotationFeatureResult | None] = [None] * len(stencils)
+    for g in groups:
+        params = [heat_parameters_from_edge_mean(edge_means_[i], 50.0) for i in g]
+        if len(g) > 1:
+            geom = torch.stack([geometries[i] for i in g])
+            vals = torch.stack([values[i].reshape(-1) for i in g])
+            smoothed = stencils[g[0]]._smooth_values_batch(vals, geom, params, False,
+                                                           mesh_of=range(len(g)))
+        else:
+            smoothed = stencils[g[0]]._smooth_values_batch(values[g[0]].reshape(1, -1),
+                                                           geometries[g[0]], params, False)
+        normed = _minmax(smoothed, 2.220446049250313e-16)
+        for k, i in enumerate(g):
+            out[i] = RotationFeatureResult(values=normed[k], iterations=iterations[i],
+                                           relative_residual=residuals[i])
+    return tuple(out)
+
+
+def _topology_groups(stencils: Sequence[SurfaceStencilDevice], together: bool) -> list[list[int]]:
+    """Sides grouped by shared sphere topology (the cache returns one object per mesh)."""
+
+    groups: list[list[int]] = []
+    for i, st in enumerate(stencils):
+        for g in groups:
+            if together and len(g) < 32 and stencils[g[0]].topology is st.topology:
+                g.append(i)
+                break
+        else:
+            groups.append([i])
+    return groups
+
+
 def _check_solver(solver: str, need_solve: bool) -> None:
     if solver not in {"colored-sor", "pcg"}:
         raise ValueError("rotation feature solver must be 'colored-sor' or 'pcg'")
@@ -166,20 +202,12 @@ def compute_rotation_features(stencils: Sequence[SurfaceStencilDevice],
         geometries = [_feature_geometry(stencils[i], surfaces[i], float(heat_fwhms[i]),
                                         smoothed_surfaces[i]) for i in range(len(stencils))]
     final_ems = edge_means(stencils, geometries)
+    groups = _topology_groups(stencils, parallel_sides)
     if depth_values is not None:
-        return tuple(_finish_feature(stencils[i], geometries[i],
-                                     _given_depth(stencils[i], depth_values[i]), 0, 0.0,
-                                     final_ems[i])
-                     for i in range(len(stencils)))
-    # group sides by shared topology (the topology cache returns one object per mesh)
-    groups: list[list[int]] = []
-    for i, st in enumerate(stencils):
-        for g in groups:
-            if parallel_sides and stencils[g[0]].topology is st.topology:
-                g.append(i)
-                break
-        else:
-            groups.append([i])
+        given = [_given_depth(stencils[i], depth_values[i]) for i in range(len(stencils))]
+        zeros = [0] * len(stencils)
+        return _finish_features(stencils, geometries, given, zeros, [0.0] * len(stencils),
+                                final_ems, groups)
     values: list[torch.Tensor | None] = [None] * len(stencils)
     residuals: list[float] = [0.0] * len(stencils)
     for g in groups:
@@ -189,6 +217,5 @@ def compute_rotation_features(stencils: Sequence[SurfaceStencilDevice],
         for k, i in enumerate(g):
             values[i] = x[k] if len(g) > 1 else x
             residuals[i] = float(rel_host[k])
-    return tuple(_finish_feature(stencils[i], geometries[i], values[i], int(max_iter),
-                                 residuals[i], final_ems[i])
-                 for i in range(len(stencils)))
+    return _finish_features(stencils, geometries, values, [int(max_iter)] * len(stencils),
+                            residuals, final_ems, groups)

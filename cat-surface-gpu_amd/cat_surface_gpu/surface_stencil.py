@@ -14,6 +14,7 @@ from __future__ import annotations
 from dataclasses import dataclass, field
 from pathlib import Path
 import struct
+from typing import Sequence
 
 import numpy as np
 import torch
@@ -383,18 +384,27 @@ class SurfaceStencilDevice:
         return out
 
     def _smooth_values_batch(self, values: torch.Tensor, geometry: torch.Tensor,
-                             params: list[tuple[int, float]], squared_norm: bool) -> torch.Tensor:
+                             params: list[tuple[int, float]], squared_norm: bool,
+                             mesh_of: Sequence[int] | None = None) -> torch.Tensor:
+        """Rows of ``values`` smoothed with (iterations, sigma) ``params``.
+
+        ``geometry`` is one mesh [n, 3] or a stack [meshes, n, 3] of meshes of
+        this stencil's topology; row b uses mesh ``mesh_of[b]`` (default 0).
+        """
+
         topo = self.topology
-        n = int(geometry.shape[0])
+        geom = geometry.to(torch.float32).contiguous()
+        meshes = 1 if geom.ndim == 2 else int(geom.shape[0])
+        n = int(geom.shape[-2])
         nb = len(params)
         vals = values.to(torch.float64).contiguous()
-        geom = geometry.to(torch.float32).contiguous()
         out = torch.empty((nb, n), dtype=torch.float64, device=self.device)
         ws = _lib.workspace(_lib.lib().csg_smooth_values_workspace_bytes(n, topo.nnz, nb),
                             self.device)
         sig = _lib.doubles([p[1] for p in params])
         its = (_lib.ctypes.c_int * nb)(*[int(p[0]) for p in params])
-        self._call("csg_smooth_values", _lib.ptr(vals), n, nb, _lib.ptr(geom),
+        mesh = None if mesh_of is None else (_lib.ctypes.c_int * nb)(*[int(m) for m in mesh_of])
+        self._call("csg_smooth_values", _lib.ptr(vals), n, nb, _lib.ptr(geom), meshes, mesh,
                    _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), topo.nnz, sig, its,
                    int(squared_norm), _lib.ptr(out), _lib.ptr(ws), ws.numel(), self._stream())
         return out
@@ -562,13 +572,14 @@ def type5_sheets_many(stencils, geometries, fwhms, out_dtype=torch.float64) -> l
 
 
 def _minmax(values: torch.Tensor, floor: float = 0.0) -> torch.Tensor:
-    """(x - min) / max(max - min, floor) on the device."""
+    """(x - min) / max(max - min, floor) on the device; a 2-D input per row."""
 
     x = values.to(torch.float64).contiguous()
+    rows = 1 if x.ndim < 2 else int(x.shape[0])
     out = torch.empty_like(x)
-    ws = _lib.workspace(_lib.lib().csg_minmax_workspace_bytes(1), x.device)
-    _lib.check(_lib.lib().csg_minmax_normalise(_lib.ptr(x), x.numel(), 1, float(floor), 1,
-                                               _lib.ptr(out), _lib.ptr(ws), ws.numel(),
-                                               _lib.stream_handle(x.device)),
+    ws = _lib.workspace(_lib.lib().csg_minmax_workspace_bytes(rows), x.device)
+    _lib.check(_lib.lib().csg_minmax_normalise(_lib.ptr(x), x.numel() // rows, rows,
+                                               float(floor), 1, _lib.ptr(out), _lib.ptr(ws),
+                                               ws.numel(), _lib.stream_handle(x.device)),
                "csg_minmax_normalise")
     return out

@@ -44,7 +44,8 @@ HeatWs carve_heat(Carve &c, int n, int nnz, int nb)
     return h;
 }
 
-int make_batch(int nb, const double *sigma, const int *iterations, HeatBatch &hb)
+int make_batch(int nb, const double *sigma, const int *iterations, HeatBatch &hb,
+               const int *mesh = nullptr, int meshes = 1)
 {
     if (nb < 1 || nb > kMaxHeatBatch)
         return fail(kInvalidArgument, "value smoothing batch must be in [1, %d], got %d",
@@ -55,6 +56,10 @@ int make_batch(int nb, const double *sigma, const int *iterations, HeatBatch &hb
             return fail(kInvalidArgument, "heat iteration counts must be non-negative");
         hb.counts[i] = iterations[i];
         hb.sigma[i] = sigma[i];
+        hb.mesh[i] = mesh ? mesh[i] : 0;
+        if (hb.mesh[i] < 0 || hb.mesh[i] >= meshes)
+            return fail(kInvalidArgument, "value smoothing mesh index %d out of range",
+                        hb.mesh[i]);
     }
     return kOk;
 }
@@ -172,12 +177,13 @@ size_t csg_smooth_values_workspace_bytes(int n, int nnz, int nb)
     return c.used;
 }
 
-int csg_smooth_values(const double *values, int n, int nb, const float *geom, const int *nbr_off,
-                      const int *nbr, int nnz, const double *sigma, const int *iterations,
-                      int squared_norm, double *out, void *ws, size_t ws_bytes, void *stream)
+int csg_smooth_values(const double *values, int n, int nb, const float *geom, int meshes,
+                      const int *mesh_of, const int *nbr_off, const int *nbr, int nnz,
+                      const double *sigma, const int *iterations, int squared_norm, double *out,
+                      void *ws, size_t ws_bytes, void *stream)
 {
     HeatBatch hb;
-    CSG_TRY(make_batch(nb, sigma, iterations, hb));
+    CSG_TRY(make_batch(nb, sigma, iterations, hb, mesh_of, meshes < 1 ? 1 : meshes));
     if (ws_bytes < csg_smooth_values_workspace_bytes(n, nnz, nb))
         return fail(kInvalidArgument, "smooth_values workspace too small");
     hipStream_t s = S(stream);

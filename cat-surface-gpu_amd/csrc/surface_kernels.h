@@ -5,13 +5,14 @@
 
 namespace csg {
 
-// Up to kMaxHeatBatch value-smoothing problems that share a mesh, each with
-// its own iteration count and Gaussian width.
-constexpr int kMaxHeatBatch = 8;
+// Up to kMaxHeatBatch value-smoothing problems on meshes of one topology,
+// each with its own iteration count, Gaussian width and geometry.
+constexpr int kMaxHeatBatch = 32;
 struct HeatBatch {
     int nb;
     int counts[kMaxHeatBatch];
     double sigma[kMaxHeatBatch];
+    int mesh[kMaxHeatBatch];  // geometry of each problem: mesh[b] * 3 n floats in
 };
 
 // Up to kMaxSmoothBatch meshes of one topology smoothed together, each with

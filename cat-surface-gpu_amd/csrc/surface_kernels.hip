@@ -192,6 +192,7 @@ k_heat_weights(const float *__restrict__ geom, const int *__restrict__ off,
         return;
     int b = blockIdx.y;
     double sg = hb.sigma[b];
+    geom += (size_t)hb.mesh[b] * 3 * n;
     double gx = geom[3 * i], gy = geom[3 * i + 1], gz = geom[3 * i + 2];
     double ws = 0.0;
     for (int k = off[i]; k < off[i + 1]; ++k) {

@@ -185,13 +185,16 @@ int csg_smooth_geometry_batch(const float *v, int batch, const int *nbr_off, con
 int csg_vertex_normals(const float *v, const int *faces, const int *inc_off, const int *inc,
                        int n, float *out, void *stream);
 
-/* nb (<= 8) value-smoothing problems on one mesh; sigma and iterations are host arrays.
- * squared_norm selects smooth_values_many's distance form.  Replaces
+/* nb (<= 32) value-smoothing problems on meshes of one topology; sigma, iterations and
+ * mesh_of (problem b smooths on geometry geom + mesh_of[b] * 3 n, each < meshes; NULL
+ * means every problem uses mesh 0) are host arrays.  squared_norm selects
+ * smooth_values_many's distance form.  Replaces
  * SurfaceStencilDevice.smooth_values / smooth_values_many (surface_stencil.py:390-476). */
 size_t csg_smooth_values_workspace_bytes(int n, int nnz, int nb);
-int csg_smooth_values(const double *values, int n, int nb, const float *geom, const int *nbr_off,
-                      const int *nbr, int nnz, const double *sigma, const int *iterations,
-                      int squared_norm, double *out, void *ws, size_t ws_bytes, void *stream);
+int csg_smooth_values(const double *values, int n, int nb, const float *geom, int meshes,
+                      const int *mesh_of, const int *nbr_off, const int *nbr, int nnz,
+                      const double *sigma, const int *iterations, int squared_norm, double *out,
+                      void *ws, size_t ws_bytes, void *stream);
 
 /* (x - min) / max(max - min, floor) per row of nb rows of n doubles. */
 size_t csg_minmax_workspace_bytes(int nb);

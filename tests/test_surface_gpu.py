@@ -57,6 +57,15 @@ def test_heat_smoothing_matches_golden(small, cuda):
     many = dev.smooth_values_many(vals, surf, (5.0, 5.0 / 3.0))
     np.testing.assert_allclose(_np(many[0]), g["smooth_many0"], rtol=1e-12, atol=1e-14)
     np.testing.assert_allclose(_np(many[1]), g["smooth_many1"], rtol=1e-12, atol=1e-14)
+    # problems on different meshes of one topology in one call equal separate calls
+    other = surf * 1.25
+    params = [(7, 1.5), (3, 0.5), (5, 2.0)]
+    rows = torch.stack([vals, vals * 2.0, vals - 1.0]).double()
+    batched = dev._smooth_values_batch(rows, torch.stack([surf, other]), params, False,
+                                       mesh_of=(1, 0, 1))
+    for b, (mesh, p) in enumerate(zip((other, surf, other), params)):
+        single = dev._smooth_values_batch(rows[b:b + 1], mesh, [p], False)[0]
+        assert torch.equal(batched[b], single)
 
 
 def test_normals_and_type5_sheets(small, cuda):
