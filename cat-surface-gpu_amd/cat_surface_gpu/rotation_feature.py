@@ -27,6 +27,17 @@ class RotationFeatureResult:
     relative_residual: float
 
 
+def set_sor_placement(xcd_local: bool) -> bool:
+    """Choose where the depth-potential SOR sweeps run; returns the previous setting.
+
+    ``True`` (the default) keeps each mesh's rows on the workgroups of one XCD
+    so that colours synchronise through that XCD's L2; ``False`` uses one
+    device-wide grid barrier per colour.  Both compute identical values.
+    """
+
+    return bool(_lib.lib().csg_set_sor_placement(1 if xcd_local else 0))
+
+
 def depth_potential(stencil: SurfaceStencilDevice, smoothed_surface, *,
                     alpha: float = 1.0 / 1000.0, max_iter: int = 1000,
                     relaxation: float = 1.90) -> tuple[torch.Tensor, int, torch.Tensor]:

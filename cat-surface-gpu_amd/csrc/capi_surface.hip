@@ -312,6 +312,11 @@ size_t csg_depth_workspace_bytes(int n, int nf, int nnz, int batch, int ell_widt
     return c.used;
 }
 
+int csg_set_sor_placement(int xcd_local)
+{
+    return set_sor_xcd_local(xcd_local);
+}
+
 // Depth potential of the rotation feature for `batch` meshes of one topology:
 // (L + alpha·diag(area)) x = rhs by colour-ordered SOR (rotation_feature.py:
 // 253-317).  Synchronises once at the end to check the persistent solve.

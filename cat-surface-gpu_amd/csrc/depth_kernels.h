@@ -17,6 +17,11 @@ struct SorColours {
     int off[kMaxColours + 1];
 };
 
+// Workgroups of each XCD's group in the XCD-local SOR (0: the XCD holds no mesh).
+struct SorXcdPlan {
+    int blocks[8];
+};
+
 struct DepthArgs {
     const float *pts;  // [batch][n][3]
     const int *faces;  // [nf][3]
@@ -39,5 +44,9 @@ int depth_setup(const DepthArgs &a, hipStream_t s);
 int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double omega,
               hipStream_t s);
 int depth_residual(const DepthArgs &a, double *out, hipStream_t s);
+// Process-wide: 1 (default) runs the SOR with each mesh's rows on one XCD when
+// the meshes fit, 0 always uses the device-wide grid barrier.  Returns the
+// previous setting.  Both compute identical values.
+int set_sor_xcd_local(int on);
 
 }  // namespace csg

@@ -17,6 +17,8 @@
 #include "depth_kernels.h"
 #include "kernel_timing.h"
 
+#include <atomic>
+
 namespace csg {
 
 __global__ void __launch_bounds__(kBlock)
@@ -427,6 +429,154 @@ k_sor_registers(const int *__restrict__ verts, SorColours co, const int *__restr
 }
 
 
+// ---- XCD-local SOR
+//
+// MI355X has eight XCDs, each with its own L2; the L2s are not coherent with
+// one another, so a grid barrier across XCDs needs agent-scope fences that
+// write back and invalidate L2 on every colour.  The sweeps of one mesh only
+// read and write that mesh's x, so when the rows of each mesh are handled by
+// workgroups that all run on ONE XCD, the L2 they share is the coherence
+// point: x is written with L1-bypassing (sc1) stores, each wave waits for its
+// stores to be acknowledged (s_waitcnt 0) before the workgroup barrier,
+// thread 0 then bumps the group's arrival counter in that L2, and readers
+// poll the counter and gather x with sc1 loads, which miss L1 and hit the
+// shared L2.  No L2 writeback or invalidate is needed between colours; the
+// kernel's end writes the L2 back as every kernel's does.
+//
+// Placement is not assumed from the dispatch order: thread 0 of every
+// workgroup reads its XCD from the XCC_ID hardware register and claims the
+// next slot of that XCD's group.  Mesh b belongs to XCD b % 8; workgroups
+// beyond their XCD's group size exit at once.  The launch carries spare
+// workgroups per XCD, and every wait is bounded, so a placement that leaves a
+// group short ends in the timeout flag rather than a hang.
+constexpr int kXcds = 8;
+constexpr int kXcdSpare = 4;
+
+__device__ __forceinline__ int xcd_id()
+{
+    unsigned v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return (int)(v & (kXcds - 1));
+}
+
+__device__ __forceinline__ double load_l2(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_l2(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Barrier of the workgroups of one XCD group.  The arrival counter only grows:
+// phase p is complete when it reaches p * nblocks (modular arithmetic), so it
+// needs no reset and no generation word.
+__device__ bool xcd_barrier(unsigned *cnt, unsigned target, int *timeout)
+{
+    __shared__ int ok;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's x stores are in L2
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ok = 1;
+        __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        long spins = 0;
+        while ((int)(__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+                     target) < 0) {
+            if (++spins > kSpinLimit) {
+                __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+// k_sor_registers with each mesh's rows on one XCD.  blocks[x] workgroups of
+// XCD x handle meshes x, x + 8, ... (thread t of the group owns row t of every
+// colour, meshes side by side as in k_sor_registers).  Barrier words: slot x
+// holds XCD x's claim counter, slot 8 + x its arrival counter.
+template <int CMAX, int WMAX>
+__global__ void __launch_bounds__(kSorRegBlock)
+k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restrict__ ell_cols,
+                const double *__restrict__ ell_val, const double *__restrict__ rhs_c,
+                const double *__restrict__ diag_c, int W, double omega, int n, int batch,
+                int max_iter, SorXcdPlan plan, double *x, unsigned *bar, int *timeout)
+{
+    __shared__ int s_xcd, s_slot;
+    if (threadIdx.x == 0) {
+        const int xcd = xcd_id();
+        s_xcd = xcd;
+        s_slot = (int)__hip_atomic_fetch_add(bar + (size_t)xcd * kBarrierSlotWords, 1u,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const int xcd = s_xcd, slot = s_slot;
+    const int nblocks = plan.blocks[xcd];
+    if (slot >= nblocks)
+        return;
+    unsigned *cnt = bar + (size_t)(kXcds + xcd) * kBarrierSlotWords;
+    const int meshes = batch > xcd ? (batch - xcd + kXcds - 1) / kXcds : 0;
+    const int tid = slot * blockDim.x + threadIdx.x;
+    const size_t wn = (size_t)W * n;
+    bool act[CMAX];
+    int vi[CMAX];
+    int col[CMAX][WMAX];
+    double val[CMAX][WMAX], rh[CMAX], dg[CMAX];
+#pragma unroll
+    for (int c = 0; c < CMAX; ++c) {
+        act[c] = false;
+        vi[c] = 0;
+        rh[c] = 0.0;
+        dg[c] = 1.0;
+#pragma unroll
+        for (int k = 0; k < WMAX; ++k) {
+            col[c][k] = 0;
+            val[c][k] = 0.0;
+        }
+        if (c < co.count) {
+            const int start = co.off[c], count = co.off[c + 1] - start;
+            if (tid < count * meshes) {
+                const int b = xcd + kXcds * (tid / count);
+                const int r = start + tid % count;
+                act[c] = true;
+                vi[c] = b * n + verts[r];
+                rh[c] = rhs_c[(size_t)b * n + r];
+                dg[c] = diag_c[(size_t)b * n + r];
+#pragma unroll
+                for (int k = 0; k < WMAX; ++k) {
+                    if (k < W) {
+                        val[c][k] = ell_val[(size_t)b * wn + (size_t)k * n + r];
+                        col[c][k] = b * n + ell_cols[(size_t)k * n + r];
+                    }
+                }
+            }
+        }
+    }
+    unsigned target = 0;
+    for (int it = 0; it < max_iter; ++it) {
+#pragma unroll
+        for (int c = 0; c < CMAX; ++c) {
+            if (c >= co.count)
+                break;
+            if (act[c]) {
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k < WMAX; ++k)
+                    if (k < W)
+                        s += val[c][k] * load_l2(x + col[c][k]);
+                store_l2(x + vi[c], load_l2(x + vi[c]) + omega * (rh[c] - s) / dg[c]);
+            }
+            target += (unsigned)nblocks;
+            if (!xcd_barrier(cnt, target, timeout))
+                return;
+        }
+    }
+}
+
 // r = rhs - A x; per-block Σ r², Σ rhs².
 __global__ void __launch_bounds__(kBlock)
 k_residual_norms(const int *__restrict__ row_off, const int *__restrict__ cols,
@@ -498,6 +648,15 @@ int depth_setup(const DepthArgs &a, hipStream_t s)
     return kOk;
 }
 
+namespace {
+std::atomic<int> g_sor_xcd_local{1};
+}
+
+int set_sor_xcd_local(int on)
+{
+    return g_sor_xcd_local.exchange(on ? 1 : 0);
+}
+
 int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double omega,
               hipStream_t s)
 {
@@ -524,6 +683,22 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     int reg_per_cu = 0;
     CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &reg_per_cu, k_sor_registers<kSorRegColours, kSorRegWidth>, kSorRegBlock, 0));
+    // XCD-local placement: mesh b on XCD b % 8; every XCD's group must fit on
+    // its share of the CUs at the register kernel's occupancy
+    SorXcdPlan plan{};
+    int local_per_cu = 0;
+    CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &local_per_cu, k_sor_xcd_local<kSorRegColours, kSorRegWidth>, kSorRegBlock, 0));
+    bool local = g_sor_xcd_local.load() != 0 && co.count <= kSorRegColours &&
+                 W <= kSorRegWidth && cus % kXcds == 0;
+    int local_grid = 0;
+    for (int xcd = 0; xcd < kXcds && local; ++xcd) {
+        const int meshes = a.batch > xcd ? (a.batch - xcd + kXcds - 1) / kXcds : 0;
+        plan.blocks[xcd] = (int)((size_t)most * meshes + kSorRegBlock - 1) / kSorRegBlock;
+        local = plan.blocks[xcd] <= (cus / kXcds) * local_per_cu;
+        local_grid = plan.blocks[xcd] > local_grid ? plan.blocks[xcd] : local_grid;
+    }
+    local_grid = (local_grid + kXcdSpare) * kXcds;
     const int reg_grid = (int)grid_for((size_t)most * a.batch, kSorRegBlock);
     const bool registers = co.count <= kSorRegColours && W <= kSorRegWidth &&
                            reg_grid <= cus * reg_per_cu &&
@@ -545,12 +720,16 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     SorColours colours = co;
     // per row and sweep: the row's ELL values and columns (12 B per slot), its
     // x gathers (8 B per slot), rhs, diagonal and the x update (24 B)
-    TimedLaunch timer(registers ? kTimedSorRegisters : kTimedSor, s,
+    TimedLaunch timer(local ? kTimedSorXcdLocal : registers ? kTimedSorRegisters : kTimedSor, s,
                       (double)max_iter * batch * n * (20.0 * W + 24.0));
     // A regular launch: the grid is capped at what the device holds at once
     // (occupancy x CUs), and every barrier wait is bounded, so blocks that were
     // not resident turn into a timeout error rather than a hang.
-    if (registers)
+    if (local)
+        hipLaunchKernelGGL((k_sor_xcd_local<kSorRegColours, kSorRegWidth>), dim3(local_grid),
+                           dim3(kSorRegBlock), 0, s, verts, colours, ell_cols, ell_val, rhs_c,
+                           diag_c, width, omega, n, batch, max_iter, plan, x, bar, timeout);
+    else if (registers)
         hipLaunchKernelGGL((k_sor_registers<kSorRegColours, kSorRegWidth>), dim3(reg_grid),
                            dim3(kSorRegBlock), 0, s, verts, colours, ell_cols, ell_val, rhs_c,
                            diag_c, width, omega, n, batch, max_iter, x, bar, timeout);

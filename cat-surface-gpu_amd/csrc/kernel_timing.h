@@ -18,6 +18,7 @@ enum TimedKernel : int {
     kTimedRelaxSweep,      // one half-sweep of the per-operator schedule
     kTimedSor,             // all SOR sweeps of one depth-potential solve (rows from memory)
     kTimedSorRegisters,    // the same with the rows held in registers
+    kTimedSorXcdLocal,     // the same with each mesh's rows on one XCD
     kTimedCount
 };
 

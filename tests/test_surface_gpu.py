@@ -125,6 +125,37 @@ def test_depth_potential_batch_matches_single(golden, cuda):
     np.testing.assert_allclose(_np(pair[0].values), g["feature"], atol=1e-6)
 
 
+@pytest.mark.parametrize("level,batch", [(3, 1), (6, 1), (6, 4), (5, 9)])
+def test_sor_xcd_local_equals_grid_barrier(level, batch, cuda):
+    """Each mesh's sweeps on one XCD give exactly the device-wide-barrier values."""
+
+    import cat_surface_gpu as csg
+    from cat_surface_gpu.rotation_feature import depth_potential, set_sor_placement
+    from cat_surface_gpu.stencil_builder import icosphere
+
+    v, f = icosphere(level)
+    n = v.shape[0]
+    idx = np.repeat(np.arange(n, dtype=np.int32)[:, None], 3, axis=1)
+    w = np.tile(np.array([1.0, 0.0, 0.0]), (n, 1))
+    st = csg.SurfaceStencil(
+        sphere_points=v, faces=f.astype(np.int32), surface_indices=idx, surface_weights=w,
+        sheet_indices=np.zeros((1, 1, 3), np.int32), sheet_weights=np.zeros((1, 1, 3)),
+        nx=1, ny=1, source_points=n).to(cuda)
+    rng = np.random.default_rng(level * 100 + batch)
+    bumps = 1.0 + 0.05 * rng.standard_normal((batch, n, 1))
+    pts = torch.as_tensor(v[None] * 60.0 * bumps, dtype=torch.float32, device=cuda)
+    prev = set_sor_placement(True)
+    try:
+        xl, _, rl = depth_potential(st, pts, max_iter=200)
+        set_sor_placement(False)
+        xg, _, rg = depth_potential(st, pts, max_iter=200)
+    finally:
+        set_sor_placement(prev)
+    assert torch.isfinite(xl).all()
+    assert torch.equal(xl, xg)
+    assert torch.equal(rl, rg)
+
+
 def test_rotation_index_costs_and_search(golden, cuda):
     import cat_surface_gpu as csg
 
