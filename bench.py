@@ -53,14 +53,18 @@ def run_step(pairs, dtype: str, mode: str, device, cuda_graph: bool):
     """One step: LH and RH.  "plain" registers them one after the other (the
     reference's pipeline, one job per call); "batched" schedules both jobs
     together (run_cat_surface_gpu_pipelines: shared feature solve and batched
-    DARTEL rounds, identical results)."""
+    DARTEL rounds, identical results); "batched+adjoint" also replaces the
+    Nelder-Mead rotation refinement by gradient (Levenberg-Marquardt) steps on
+    the 8 best seeds."""
 
     from cat_surface_gpu import (PipelineJob, run_cat_surface_gpu_pipeline,
                                  run_cat_surface_gpu_pipelines)
 
     kw = dict(device=device, dartel_dtype=dtype, steps=2, runs=2, avg=True, write_output=False,
               cuda_graph=cuda_graph)
-    if mode == "batched":
+    if mode == "batched+adjoint":
+        kw.update(rotation_method="adjoint")
+    if mode in ("batched", "batched+adjoint"):
         return run_cat_surface_gpu_pipelines([PipelineJob(meshes=m) for m in pairs], **kw)
     return [run_cat_surface_gpu_pipeline(None, None, None, None, "unused.gii", meshes=m, **kw)
             for m in pairs]
@@ -133,9 +137,13 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--dtype", choices=("float32", "float64"), default="float32")
-    ap.add_argument("--mode", choices=("plain", "batched"), default="batched")
+    ap.add_argument("--mode", choices=("plain", "batched", "batched+adjoint"),
+                    default="batched+adjoint")
     ap.add_argument("--seed", type=int, default=20261205)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--compare-plain", action=argparse.BooleanOptionalAction, default=True,
+                    help="after the timed mode, time the plain path on the same inputs and "
+                         "report it with the largest vertex difference between the two")
     ap.add_argument("--cuda-graph", action=argparse.BooleanOptionalAction, default=True,
                     help="replay the DARTEL solves from captured graphs (the kernels inside a "
                          "replay are not seen by the kernel timers)")
@@ -165,7 +173,7 @@ def main() -> int:
     for i in range(args.steps):
         results = run_step(pairs, args.dtype, args.mode, device, args.cuda_graph)
         # batched jobs share their stage timings: count them once
-        for r in (results[:1] if args.mode == "batched" else results):
+        for r in (results[:1] if args.mode != "plain" else results):
             for key, val in r.timings.items():
                 stage_sums[key] = stage_sums.get(key, 0.0) + val
         log(f"step {i} done")
@@ -177,6 +185,22 @@ def main() -> int:
     value = elapsed / (args.steps * hemispheres)
     kernels = kernel_rooflines()
     roof = kernels[0] if kernels else None
+    plain = None
+    if args.compare_plain and args.mode != "plain":
+        for i in range(args.warmup):
+            run_step(pairs, args.dtype, "plain", device, args.cuda_graph)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        for i in range(args.steps):
+            plain_results = run_step(pairs, args.dtype, "plain", device, args.cuda_graph)
+        torch.cuda.synchronize()
+        plain_value = (time.perf_counter() - tp) / (args.steps * hemispheres)
+        delta = max(float(abs(a.vertices.astype("float64") - b.vertices).max())
+                    for a, b in zip(results, plain_results))
+        plain = {"value": round(plain_value, 4), "unit": "s/hemisphere",
+                 "speedup": round(plain_value / value, 3),
+                 "max_vertex_delta": delta}
+        log("plain comparison done")
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     stages = {k: round(v / (args.steps * hemispheres), 4) for k, v in sorted(stage_sums.items())}
     line = {
@@ -195,6 +219,7 @@ def main() -> int:
         "config": {"workload": "61,442-vertex hemisphere pair, LH+RH per step, steps=2 runs=2 "
                                "avg, 512x256 map", "mode": args.mode,
                    "parallelism": "single GPU", "cuda_graph": args.cuda_graph},
+        "plain": plain,
         "roofline": roof,
         "roofline_kernels": kernels,
         "cpu_baseline": cpu,

@@ -92,6 +92,9 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_grid_index_fill_workspace_bytes": (_sz, [_i]),
     "csg_grid_index_fill": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _sz, _vp]),
     "csg_rotation_costs_workspace_bytes": (_sz, [_i, _i]),
+    "csg_rotation_cost_matrices_workspace_bytes": (_sz, [_i, _i, _i]),
+    "csg_rotation_cost_matrices": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _i,
+                                        _vp, _vp, _sz, _vp, _vp]),
     "csg_rotation_costs": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp,
                                 _vp, _vp, _sz, ctypes.POINTER(_i), _vp, _vp]),
     "csg_closest_triangles": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i, _d, _vp, _vp, _vp,

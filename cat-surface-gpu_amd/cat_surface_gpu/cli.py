@@ -69,6 +69,12 @@ def surfwarp_main(argv: list[str] | None = None) -> int:
     p.add_argument("--rotation-max-iter", type=int, default=500)
     p.add_argument("--rotation-tol", type=float, default=1.0e-4)
     p.add_argument("--rotation-simplex-step", type=float, default=0.1)
+    # MI355X additions: gradient refinement of the best rotation candidates
+    p.add_argument("--rotation-method", choices=("nelder-mead", "adjoint"),
+                   default="nelder-mead")
+    p.add_argument("--rotation-candidates", choices=("seeds", "hopf"), default="seeds")
+    p.add_argument("--rotation-top", type=int, default=8)
+    p.add_argument("--rotation-steps", type=int, default=20)
     args = p.parse_args(argv)
     result = run_cat_surface_gpu_pipeline(
         args.source_surface, args.source_sphere, args.target_surface, args.target_sphere,
@@ -89,7 +95,9 @@ def surfwarp_main(argv: list[str] | None = None) -> int:
         rotation_refine=not args.no_rotation_refine, rotation_max_iter=args.rotation_max_iter,
         rotation_tol=args.rotation_tol, rotation_simplex_step=args.rotation_simplex_step,
         parallel_sides=args.parallel_sides, optimized_dartel=args.optimized_dartel,
-        cuda_graph=args.cuda_graph)
+        cuda_graph=args.cuda_graph, rotation_method=args.rotation_method,
+        rotation_candidates=args.rotation_candidates, rotation_top=args.rotation_top,
+        rotation_steps=args.rotation_steps)
     device = torch.device(args.device)
     print(json.dumps({
         "backend": "hip_gfx950_cat_surface_gpu_pipeline",

@@ -171,6 +171,8 @@ def run_cat_surface_gpu_pipeline(
     rotation_feature_solver: str = "colored-sor", parallel_sides: bool = True,
     optimized_dartel: bool = True, cuda_graph: bool = True,
     rotation_search=None, write_output: bool = True, meshes=None,
+    rotation_method: str = "nelder-mead", rotation_candidates: str = "seeds",
+    rotation_top: int = 8, rotation_steps: int = 20,
 ) -> CatSurfaceGpuResult:
     """Initial rotation, DARTEL runs, averaging, warp and GIFTI output (:468-1326).
 
@@ -179,7 +181,11 @@ def run_cat_surface_gpu_pipeline(
     target_values) -> RotationSearchResult replacing the 46-seed + Nelder-Mead
     search; ``meshes`` passes the four meshes (source surface, source sphere,
     target surface, target sphere) already in memory instead of reading the
-    paths; ``write_output=False`` skips the output file.
+    paths; ``write_output=False`` skips the output file.  ``rotation_method=
+    "adjoint"`` refines the ``rotation_top`` best of the seeds (or of the
+    8,192-rotation Hopf grid, ``rotation_candidates="hopf"``) by
+    ``rotation_steps`` gradient steps instead of Nelder-Mead
+    (surface_rotation.search_rotations).
     """
 
     job = PipelineJob(source_surface_path, source_sphere_path, target_surface_path,
@@ -200,7 +206,9 @@ def run_cat_surface_gpu_pipeline(
         rotation_feature_backend=rotation_feature_backend,
         rotation_feature_solver=rotation_feature_solver, parallel_sides=parallel_sides,
         optimized_dartel=optimized_dartel, cuda_graph=cuda_graph,
-        rotation_search=rotation_search, write_output=write_output)[0]
+        rotation_search=rotation_search, write_output=write_output,
+        rotation_method=rotation_method, rotation_candidates=rotation_candidates,
+        rotation_top=rotation_top, rotation_steps=rotation_steps)[0]
 
 
 @dataclass
@@ -253,6 +261,8 @@ def run_cat_surface_gpu_pipelines(
     rotation_feature_solver: str = "colored-sor", parallel_sides: bool = True,
     optimized_dartel: bool = True, cuda_graph: bool = True,
     rotation_search=None, write_output: bool = True,
+    rotation_method: str = "nelder-mead", rotation_candidates: str = "seeds",
+    rotation_top: int = 8, rotation_steps: int = 20,
 ) -> list[CatSurfaceGpuResult]:
     """Several independent CAT_SurfWarp registrations, stage by stage.
 
@@ -424,7 +434,10 @@ def run_cat_surface_gpu_pipelines(
                         for (rot, pts), st in zip(rots, states)]
             results = search_rotations(problems, refine=rotation_refine,
                                        max_iter=rotation_max_iter, tol=rotation_tol,
-                                       simplex_step=rotation_simplex_step)
+                                       simplex_step=rotation_simplex_step,
+                                       method=rotation_method,
+                                       candidates=rotation_candidates, top=rotation_top,
+                                       steps=rotation_steps)
         for st, result in zip(states, results):
             st.angle = np.asarray(result.angle.detach().cpu(), dtype=np.float64).reshape(3)
             st.rotation_cost = float(result.cost.detach().cpu())

@@ -49,6 +49,75 @@ def rotation_matrix_np(angles) -> np.ndarray:
     return np.asarray(res, dtype=np.float64).reshape(3, 3)
 
 
+def angles_from_matrix(matrix) -> np.ndarray:
+    """Euler angles (alpha, beta, gamma) with rotation_matrix_np(angles) == matrix.
+
+    rotation_matrix_np returns M = (Rz(gamma) Ry(-beta) Rx(alpha))ᵀ, so
+    M[0, 2] = sin(beta), M[1, 2] / M[2, 2] = tan(alpha) and
+    M[0, 1] / M[0, 0] = tan(gamma) (cos(beta) > 0 branch).
+    """
+
+    m = np.asarray(matrix, dtype=np.float64).reshape(3, 3)
+    beta = float(np.arcsin(np.clip(m[0, 2], -1.0, 1.0)))
+    alpha = float(np.arctan2(m[1, 2], m[2, 2]))
+    gamma = float(np.arctan2(m[0, 1], m[0, 0]))
+    return np.array([alpha, beta, gamma], dtype=np.float64)
+
+
+def quaternion_matrix(q) -> np.ndarray:
+    """Rotation matrices [..., 3, 3] of unit quaternions [..., 4] (w, x, y, z)."""
+
+    q = np.asarray(q, dtype=np.float64)
+    q = q / np.linalg.norm(q, axis=-1, keepdims=True)
+    w, x, y, z = q[..., 0], q[..., 1], q[..., 2], q[..., 3]
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1),
+    ], -2)
+
+
+def hopf_grid(n_sphere: int = 256, n_circle: int = 32) -> np.ndarray:
+    """n_sphere x n_circle rotation matrices spread uniformly over SO(3).
+
+    Hopf-fibration sampling (Yershova, Jain, LaValle and Mitchell, "Generating
+    uniform incremental grids on SO(3) using the Hopf fibration", IJRR 2010):
+    a near-uniform set of base points (theta, phi) on S² (a Fibonacci lattice
+    here) times n_circle equally spaced fibre angles psi in [0, 2 pi), mapped to
+    the unit quaternion (cos(theta/2) cos(psi/2), cos(theta/2) sin(psi/2),
+    sin(theta/2) cos(phi + psi/2), sin(theta/2) sin(phi + psi/2)).  The map
+    carries the product measure on S² x S¹ to the uniform measure on SO(3).
+    Candidate c is base point c // n_circle, fibre angle c % n_circle.
+    """
+
+    if n_sphere < 1 or n_circle < 1:
+        raise ValueError("hopf_grid needs at least one base point and one fibre angle")
+    k = np.arange(n_sphere, dtype=np.float64)
+    z = 1.0 - (2.0 * k + 1.0) / n_sphere
+    theta = np.arccos(np.clip(z, -1.0, 1.0))
+    phi = np.mod(k * np.pi * (3.0 - np.sqrt(5.0)), 2.0 * np.pi)
+    psi = (np.arange(n_circle, dtype=np.float64) + 0.5) * (2.0 * np.pi / n_circle)
+    th = np.repeat(theta, n_circle)
+    ph = np.repeat(phi, n_circle)
+    ps = np.tile(psi, n_sphere)
+    q = np.stack([np.cos(th / 2) * np.cos(ps / 2), np.cos(th / 2) * np.sin(ps / 2),
+                  np.sin(th / 2) * np.cos(ph + ps / 2), np.sin(th / 2) * np.sin(ph + ps / 2)],
+                 axis=-1)
+    return quaternion_matrix(q)
+
+
+def axis_angle_matrix(delta) -> np.ndarray:
+    """exp([delta]x) by Rodrigues' formula."""
+
+    d = np.asarray(delta, dtype=np.float64).reshape(3)
+    theta = float(np.linalg.norm(d))
+    k = np.array([[0.0, -d[2], d[1]], [d[2], 0.0, -d[0]], [-d[1], d[0], 0.0]])
+    if theta < 1e-12:
+        return np.eye(3) + k
+    return (np.eye(3) + (np.sin(theta) / theta) * k
+            + ((1.0 - np.cos(theta)) / (theta * theta)) * (k @ k))
+
+
 @dataclass(frozen=True)
 class RotationGridIndex:
     """Host view of the candidate index (surface_rotation.py:26-180)."""
@@ -221,6 +290,41 @@ class RotationGridIndexDevice:
             None if uncovered_dev is not None else ctypes.byref(uncovered),
             _lib.ptr(uncovered_dev), _lib.stream_handle(self.device)), "csg_rotation_costs")
         return costs, loc_f, loc_b, (None if uncovered_dev is not None else uncovered.value)
+
+    def _launch_matrix_costs(self, inputs: RotationCostInputs, matrices, gradient: bool,
+                             uncovered_dev: torch.Tensor) -> torch.Tensor:
+        """Enqueue the matrix-input cost ([na]) or cost + gradient + Gauss-Newton
+        terms ([na, 10]); the uncovered count goes to ``uncovered_dev``."""
+
+        mats = torch.as_tensor(np.asarray(matrices, dtype=np.float64).reshape(-1, 9),
+                               dtype=torch.float32).to(self.device).contiguous()
+        na, ns = int(mats.shape[0]), int(inputs.source_points.shape[0])
+        out = torch.empty((na, 10) if gradient else (na,), dtype=torch.float64,
+                          device=self.device)
+        lib = _lib.lib()
+        ws = _lib.workspace(lib.csg_rotation_cost_matrices_workspace_bytes(ns, na,
+                                                                           int(gradient)),
+                            self.device)
+        _lib.check(lib.csg_rotation_cost_matrices(
+            _lib.ptr(inputs.source_points), _lib.ptr(inputs.source_values), ns,
+            _lib.ptr(inputs.target_values), _lib.ptr(self.target_faces),
+            _lib.ptr(self.target_triangles), _lib.ptr(self.candidate_offsets),
+            _lib.ptr(self.candidate_faces), int(self.grid_size), _lib.ptr(mats), na,
+            int(gradient), _lib.ptr(out), _lib.ptr(ws), ws.numel(), _lib.ptr(uncovered_dev),
+            _lib.stream_handle(self.device)), "csg_rotation_cost_matrices")
+        return out
+
+    def cost_matrices(self, inputs: RotationCostInputs, matrices, *, gradient: bool = False):
+        """SSD per candidate rotation matrix; with ``gradient`` also the axis-angle
+        gradient [na, 3] and Gauss-Newton matrix [na, 3, 3] (float64 tensors)."""
+
+        unc = torch.zeros(1, dtype=torch.int32, device=self.device)
+        out = self._launch_matrix_costs(inputs, matrices, gradient, unc)
+        if int(unc.item()):
+            raise RuntimeError("Rotation is not covered by target sphere candidate index")
+        if not gradient:
+            return out
+        return out[:, 0], out[:, 1:4], _gn_matrix(out[:, 4:10])
 
     def _costs(self, inputs: RotationCostInputs, angles: torch.Tensor,
                locate: bool = False):
@@ -410,13 +514,114 @@ def refine_nelder_mead_many(problems, *, max_iter: int = 500, tol: float = 1.0e-
     return results
 
 
+def _gn_matrix(terms):
+    """[..., 6] (xx, yy, zz, xy, xz, yz) -> symmetric [..., 3, 3]."""
+
+    t = terms
+    if isinstance(t, torch.Tensor):
+        rows = [torch.stack([t[..., 0], t[..., 3], t[..., 4]], -1),
+                torch.stack([t[..., 3], t[..., 1], t[..., 5]], -1),
+                torch.stack([t[..., 4], t[..., 5], t[..., 2]], -1)]
+        return torch.stack(rows, -2)
+    t = np.asarray(t)
+    return np.stack([np.stack([t[..., 0], t[..., 3], t[..., 4]], -1),
+                     np.stack([t[..., 3], t[..., 1], t[..., 5]], -1),
+                     np.stack([t[..., 4], t[..., 5], t[..., 2]], -1)], -2)
+
+
+def _lm_step(grad, hess, lam, max_step):
+    """Levenberg-Marquardt step (H + lam diag(H)) d = -g, at most max_step radians."""
+
+    diag = np.maximum(np.diag(hess), 1e-12 * max(1.0, float(np.max(np.abs(np.diag(hess))))))
+    a = hess + lam * np.diag(diag)
+    try:
+        d = -np.linalg.solve(a, grad)
+    except np.linalg.LinAlgError:
+        d = -grad / (lam * diag + 1e-300)
+    n = float(np.linalg.norm(d))
+    if n > max_step:
+        d *= max_step / n
+    return d
+
+
+def refine_adjoint_many(problems, starts, *, steps: int = 20, max_step: float = 0.05,
+                        damping: float = 1e-3):
+    """Gradient refinement of candidate rotations for several problems in lockstep.
+
+    ``problems`` holds (index, prepared inputs); ``starts[k]`` is an [m, 3, 3]
+    array of starting matrices for problem k.  Every candidate takes ``steps``
+    Levenberg-Marquardt steps on a left axis-angle perturbation, driven by the
+    cost gradient and Gauss-Newton matrix of csg_rotation_cost_matrices; a
+    step is kept only if it lowers the cost (else the damping grows).  Each
+    step is one launch per problem and one synchronisation for all.  Returns,
+    per problem, the candidates' final matrices [m, 3, 3] and costs [m].
+    """
+
+    if steps < 0 or max_step <= 0.0 or damping <= 0.0:
+        raise ValueError("steps must be non-negative; max_step and damping positive")
+    mats = [np.asarray(m, dtype=np.float64).reshape(-1, 3, 3).copy() for m in starts]
+    lams = [np.full(len(m), damping) for m in mats]
+
+    def evaluate(candidates):
+        outs, uncs = [], []
+        for (index, inputs), cm in zip(problems, candidates):
+            unc = torch.zeros(1, dtype=torch.int32, device=index.device)
+            outs.append(index._launch_matrix_costs(inputs, cm, True, unc))
+            uncs.append(unc)
+        host = torch.cat([torch.cat([o.reshape(-1), u.to(torch.float64)])
+                          for o, u in zip(outs, uncs)]).cpu().numpy()
+        res, pos = [], 0
+        for cm in candidates:
+            n = len(cm) * 10
+            vals, unc = host[pos:pos + n].reshape(-1, 10), host[pos + n]
+            pos += n + 1
+            if unc:
+                raise RuntimeError("Rotation is not covered by target sphere candidate index")
+            res.append((vals[:, 0].copy(), vals[:, 1:4].copy(), _gn_matrix(vals[:, 4:10])))
+        return res
+
+    state = evaluate(mats)
+    for _ in range(steps):
+        trials = []
+        for k, (cost, grad, hess) in enumerate(state):
+            trials.append(np.stack([axis_angle_matrix(_lm_step(grad[j], hess[j], lams[k][j],
+                                                               max_step)) @ mats[k][j]
+                                    for j in range(len(mats[k]))]))
+        new = evaluate(trials)
+        for k in range(len(problems)):
+            cost, grad, hess = state[k]
+            tcost, tgrad, thess = new[k]
+            better = tcost < cost
+            mats[k][better] = trials[k][better]
+            cost[better], grad[better], hess[better] = (tcost[better], tgrad[better],
+                                                        thess[better])
+            lams[k] = np.where(better, np.maximum(lams[k] / 3.0, 1e-7), lams[k] * 4.0)
+    return [(m, st[0]) for m, st in zip(mats, state)]
+
+
 def search_rotations(problems, *, refine: bool = True, max_iter: int = 500, tol: float = 1.0e-4,
-                     simplex_step: float = 0.1) -> list[RotationSearchResult]:
+                     simplex_step: float = 0.1, method: str = "nelder-mead",
+                     candidates: str = "seeds", top: int = 8,
+                     steps: int = 20) -> list[RotationSearchResult]:
     """search_rotation_prepared for several independent problems ((index,
     prepared inputs) pairs): the 46-seed scans are launched together and read
     with one synchronisation, then the Nelder-Mead refinements run in lockstep
-    (refine_nelder_mead_many).  Each result equals the single-problem search."""
+    (refine_nelder_mead_many).  Each result equals the single-problem search.
 
+    ``method="adjoint"`` replaces Nelder-Mead: the ``top`` best candidates
+    (of the 46 seeds, or of the 8,192-rotation Hopf grid with
+    ``candidates="hopf"``) take ``steps`` gradient (Levenberg-Marquardt) steps
+    each (refine_adjoint_many), and the result is the best refined rotation,
+    expressed as Euler angles and costed in the reference's angle formulation.
+    """
+
+    if method not in {"nelder-mead", "adjoint"}:
+        raise ValueError("rotation search method must be 'nelder-mead' or 'adjoint'")
+    if candidates not in {"seeds", "hopf"}:
+        raise ValueError("rotation candidates must be 'seeds' or 'hopf'")
+    if method == "adjoint":
+        return _search_rotations_adjoint(problems, refine=refine, candidates=candidates,
+                                         top=top, steps=steps)
     seeds_np = official_seed_grid_angles()
     launched = []
     for index, inputs in problems:
@@ -444,4 +649,77 @@ def search_rotations(problems, *, refine: bool = True, max_iter: int = 500, tol:
                                         seed_costs=seed_costs, angle=angle, cost=cost,
                                         rotation_matrix=index.rotation_matrix(angle),
                                         iterations=iterations))
+    return out
+
+
+def _search_rotations_adjoint(problems, *, refine: bool, candidates: str, top: int,
+                              steps: int) -> list[RotationSearchResult]:
+    if top < 1:
+        raise ValueError("top must be positive")
+    firsts, starts = [], []
+    if candidates == "seeds":
+        seeds_np = official_seed_grid_angles()
+        launched = []
+        for index, inputs in problems:
+            seeds = torch.as_tensor(seeds_np, dtype=torch.float64, device=index.device)
+            unc = torch.zeros(1, dtype=torch.int32, device=index.device)
+            costs, _, _, _ = index._launch_costs(inputs, seeds, uncovered_dev=unc)
+            launched.append((costs, unc))
+        host = torch.cat([torch.cat([c, u.to(torch.float64)]) for c, u in launched]).cpu()
+        host = host.numpy()
+        pos = 0
+        for costs, _ in launched:
+            vals, unc = host[pos:pos + len(seeds_np)], host[pos + len(seeds_np)]
+            pos += len(seeds_np) + 1
+            if unc:
+                raise RuntimeError("Rotation is not covered by target sphere candidate index")
+            order = np.argsort(vals, kind="stable")
+            firsts.append((seeds_np[order[0]], vals[order[0]], costs))
+            starts.append(np.stack([rotation_matrix_np(seeds_np[j]) for j in order[:top]]))
+    else:
+        grid = hopf_grid()
+        launched = []
+        for index, inputs in problems:
+            unc = torch.zeros(1, dtype=torch.int32, device=index.device)
+            launched.append((index._launch_matrix_costs(inputs, grid, False, unc), unc))
+        host = torch.cat([torch.cat([c, u.to(torch.float64)]) for c, u in launched]).cpu()
+        host = host.numpy()
+        pos = 0
+        for costs, _ in launched:
+            vals, unc = host[pos:pos + len(grid)], host[pos + len(grid)]
+            pos += len(grid) + 1
+            if unc:
+                raise RuntimeError("Rotation is not covered by target sphere candidate index")
+            order = np.argsort(vals, kind="stable")
+            firsts.append((angles_from_matrix(grid[order[0]]), vals[order[0]], costs))
+            starts.append(grid[order[:top]])
+    if refine and steps > 0:
+        refined = refine_adjoint_many(problems, starts, steps=steps)
+    else:
+        refined = [(st, None) for st in starts]
+    # the final choice is costed in the reference's angle formulation
+    finals = []
+    for (index, inputs), (mats, _) in zip(problems, refined):
+        ang = np.stack([angles_from_matrix(m) for m in mats])
+        unc = torch.zeros(1, dtype=torch.int32, device=index.device)
+        costs, _, _, _ = index._launch_costs(inputs, torch.as_tensor(ang), uncovered_dev=unc)
+        finals.append((ang, costs, unc))
+    host = torch.cat([torch.cat([c, u.to(torch.float64)]) for _, c, u in finals]).cpu().numpy()
+    out, pos = [], 0
+    for (index, _), (seed_angle, seed_cost, seed_costs), (ang, costs, _) in zip(
+            problems, firsts, finals):
+        vals, unc = host[pos:pos + len(ang)], host[pos + len(ang)]
+        pos += len(ang) + 1
+        if unc:
+            raise RuntimeError("Rotation is not covered by target sphere candidate index")
+        best = int(np.argmin(vals))
+        dev = index.device
+        angle = torch.as_tensor(ang[best], dtype=torch.float64, device=dev)
+        out.append(RotationSearchResult(
+            seed_angle=torch.as_tensor(seed_angle, dtype=torch.float64, device=dev),
+            seed_cost=torch.as_tensor(seed_cost, dtype=torch.float64, device=dev),
+            seed_costs=seed_costs, angle=angle,
+            cost=torch.as_tensor(vals[best], dtype=torch.float64, device=dev),
+            rotation_matrix=index.rotation_matrix(angle),
+            iterations=steps if refine else 0))
     return out

@@ -127,6 +127,43 @@ int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
     return kOk;
 }
 
+size_t csg_rotation_cost_matrices_workspace_bytes(int ns, int na, int with_gradient)
+{
+    const size_t rows = (size_t)na * (with_gradient ? kRotationGradTerms : 1);
+    return align256(sizeof(double) * (size_t)rotation_cost_blocks(ns) * rows) + 256;
+}
+
+int csg_rotation_cost_matrices(const float *src_pts, const double *src_vals, int ns,
+                               const double *tgt_vals, const int *faces, const float *tri,
+                               const int *cell_off, const int *cell_faces, int grid,
+                               const float *mats, int na, int with_gradient, double *out,
+                               void *ws, size_t ws_bytes, int *uncovered_dev, void *stream)
+{
+    if (ns < 1 || na < 1 || grid < 1 || mats == nullptr || out == nullptr ||
+        uncovered_dev == nullptr)
+        return fail(kInvalidArgument, "invalid rotation cost arguments");
+    if (na > 65535)
+        return fail(kInvalidArgument, "at most 65535 candidate rotations per call, got %d", na);
+    if (ws_bytes < csg_rotation_cost_matrices_workspace_bytes(ns, na, with_gradient))
+        return fail(kInvalidArgument, "rotation cost workspace too small");
+    hipStream_t s = S(stream);
+    RotationCostArgs a{};
+    a.src_pts = src_pts;
+    a.src_vals = src_vals;
+    a.ns = ns;
+    a.tgt_vals = tgt_vals;
+    a.faces = faces;
+    a.tri = tri;
+    a.cell_off = cell_off;
+    a.cell_faces = cell_faces;
+    a.grid = grid;
+    a.na = na;
+    a.costs = out;
+    a.partial = (double *)ws;
+    a.uncovered = uncovered_dev;
+    return launch_rotation_cost_mats(a, mats, with_gradient != 0, s);
+}
+
 int csg_closest_triangles(const double *q, int nq, const double *verts, const int *faces,
                           const int *cell_off, const int *cell_faces, int grid, double reach,
                           int *face_out, int *idx_out, double *w_out, int *far_host,

@@ -41,6 +41,14 @@ struct RotationCostArgs {
 
 int launch_rotation_costs(const RotationCostArgs &a, hipStream_t s);
 
+// Float64 sums per candidate of the matrix-input cost with gradient: cost,
+// gradient (3) and Gauss-Newton matrix (xx, yy, zz, xy, xz, yz).
+constexpr int kRotationGradTerms = 10;
+// a.angles unused; a.costs receives [na] or [na][kRotationGradTerms];
+// a.partial needs rows x rotation_cost_blocks(ns) doubles.
+int launch_rotation_cost_mats(const RotationCostArgs &a, const float *mats, bool gradient,
+                              hipStream_t s);
+
 int launch_closest_triangles(const double *q, int nq, const double *verts, const int *faces,
                              const int *cell_off, const int *cell_faces, int g, double reach2,
                              int *face_out, int *idx_out, double *w_out, int *far_count,

@@ -375,6 +375,93 @@ k_rotation_cost(const float *__restrict__ src_pts, const double *__restrict__ sr
         partial[(size_t)a * nblocks + blockIdx.x] = sum;
 }
 
+// The rotation SSD for candidate matrices (q = M p, M row-major [na][9]) and,
+// with GRAD, its gradient and Gauss-Newton matrix with respect to a left
+// axis-angle perturbation exp([δ]x)·M.  With r = s - t(q) and g the gradient
+// of the located face's linear interpolant of the target values (the face held
+// fixed), ∂r/∂δ = -(q x g); per point the kernel adds r², 2 r ∂r/∂δ and
+// 2 ∂r/∂δ ∂r/∂δᵀ.  Location and sampling are k_rotation_cost's, so the cost
+// equals k_rotation_cost's for the same float32 matrix.  Per block and
+// candidate: 1 or 10 float64 partials (cost, gradient xyz, H xx yy zz xy xz yz).
+template <bool GRAD>
+__global__ void __launch_bounds__(kBlock)
+k_rotation_cost_mats(const float *__restrict__ src_pts, const double *__restrict__ src_vals,
+                     int ns, const double *__restrict__ tgt_vals, const int *__restrict__ faces,
+                     const float *__restrict__ tri, const int *__restrict__ cell_off,
+                     const int *__restrict__ cell_faces, int g, const float *__restrict__ mats,
+                     double *__restrict__ partial, int nblocks, int *__restrict__ uncovered)
+{
+    constexpr int kTerms = GRAD ? kRotationGradTerms : 1;
+    __shared__ double red[kBlock / 64];
+    __shared__ float m[9];
+    const int a = blockIdx.y;
+    if (threadIdx.x < 9)
+        m[threadIdx.x] = mats[9 * (size_t)a + threadIdx.x];
+    __syncthreads();
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / kLocateLanes, lane = t % kLocateLanes;
+    double term[kTerms];
+#pragma unroll
+    for (int k = 0; k < kTerms; ++k)
+        term[k] = 0.0;
+    float q[3] = {0.0f, 0.0f, 0.0f};
+    int b = 0, nc = 0;
+    if (i < ns) {
+        const float *p = src_pts + 3 * i;
+        for (int r = 0; r < 3; ++r)
+            q[r] = p[0] * m[3 * r] + p[1] * m[3 * r + 1] + p[2] * m[3 * r + 2];
+        int cell = cell_of(q, g);
+        b = cell_off[cell];
+        nc = cell_off[cell + 1] - b;
+    }
+    LocateBest best = scan_candidates(q, tri, cell_faces + b, nc, lane, kLocateLanes);
+    best = reduce_lanes(best, kLocateLanes);
+    if (i < ns && lane == 0) {
+        if (best.face < 0) {
+            atomicAdd(uncovered, 1);
+        } else {
+            const int f = best.face;
+            float w[3];
+            locate_weights(tri, f, best.bp, w);
+            const double t0 = tgt_vals[faces[3 * f]], t1 = tgt_vals[faces[3 * f + 1]],
+                         t2 = tgt_vals[faces[3 * f + 2]];
+            const double d = src_vals[i] - (t0 * (double)w[0] + t1 * (double)w[1] +
+                                            t2 * (double)w[2]);
+            term[0] = d * d;
+            if constexpr (GRAD) {
+                const float *tv = tri + 9 * (size_t)f;
+                float e1[3] = {tv[3] - tv[0], tv[4] - tv[1], tv[5] - tv[2]};
+                float e2[3] = {tv[6] - tv[0], tv[7] - tv[1], tv[8] - tv[2]};
+                float xx = dot3(e1, e1), xy = dot3(e1, e2), yy = dot3(e2, e2);
+                float den = xx * yy - xy * xy;
+                float d1 = (float)(t1 - t0), d2 = (float)(t2 - t0);
+                float ca = den != 0.0f ? (d1 * yy - d2 * xy) / den : 0.0f;
+                float cb = den != 0.0f ? (d2 * xx - d1 * xy) / den : 0.0f;
+                float gv[3] = {ca * e1[0] + cb * e2[0], ca * e1[1] + cb * e2[1],
+                               ca * e1[2] + cb * e2[2]};
+                float jr[3];
+                cross3(q, gv, jr);
+                const double j0 = -(double)jr[0], j1 = -(double)jr[1], j2 = -(double)jr[2];
+                term[1] = 2.0 * d * j0;
+                term[2] = 2.0 * d * j1;
+                term[3] = 2.0 * d * j2;
+                term[4] = 2.0 * j0 * j0;
+                term[5] = 2.0 * j1 * j1;
+                term[6] = 2.0 * j2 * j2;
+                term[7] = 2.0 * j0 * j1;
+                term[8] = 2.0 * j0 * j2;
+                term[9] = 2.0 * j1 * j2;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kTerms; ++k) {
+        double sum = block_sum(term[k], red);
+        if (threadIdx.x == 0)
+            partial[((size_t)a * kTerms + k) * nblocks + blockIdx.x] = sum;
+    }
+}
+
 // out[r] = sum of row r of the block partials; one block per row, a fixed
 // (deterministic) summation order.
 __global__ void __launch_bounds__(kBlock)
@@ -399,6 +486,26 @@ int launch_rotation_costs(const RotationCostArgs &a, hipStream_t s)
                        a.partial, nb, a.uncovered, a.loc_face, a.loc_bary);
     CSG_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sum_rows, dim3(a.na), dim3(kBlock), 0, s, a.partial, nb, a.na, a.costs);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+int launch_rotation_cost_mats(const RotationCostArgs &a, const float *mats, bool gradient,
+                              hipStream_t s)
+{
+    int nb = rotation_cost_blocks(a.ns);
+    dim3 grid(nb, a.na);
+    if (gradient)
+        hipLaunchKernelGGL(k_rotation_cost_mats<true>, grid, dim3(kBlock), 0, s, a.src_pts,
+                           a.src_vals, a.ns, a.tgt_vals, a.faces, a.tri, a.cell_off,
+                           a.cell_faces, a.grid, mats, a.partial, nb, a.uncovered);
+    else
+        hipLaunchKernelGGL(k_rotation_cost_mats<false>, grid, dim3(kBlock), 0, s, a.src_pts,
+                           a.src_vals, a.ns, a.tgt_vals, a.faces, a.tri, a.cell_off,
+                           a.cell_faces, a.grid, mats, a.partial, nb, a.uncovered);
+    CSG_LAUNCH_CHECK();
+    const int rows = a.na * (gradient ? kRotationGradTerms : 1);
+    hipLaunchKernelGGL(k_sum_rows, dim3(rows), dim3(kBlock), 0, s, a.partial, nb, rows, a.costs);
     CSG_LAUNCH_CHECK();
     return kOk;
 }

@@ -283,6 +283,22 @@ int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
                        float *loc_bary, void *ws, size_t ws_bytes, int *uncovered_host,
                        int *uncovered_dev, void *stream);
 
+/* The same SSD for na candidate rotation matrices (float32 [na][9], row-major, q = M p;
+ * na <= 65535) instead of angle triples: cost_batch_prepared's formulation
+ * (surface_rotation.py:512-546) for rotations that come as matrices (the Hopf grid of
+ * candidates).  With with_gradient, out[a] holds 10 float64 sums: the cost, its gradient
+ * with respect to a left axis-angle perturbation exp([d]x) M (3), and the Gauss-Newton
+ * matrix (xx, yy, zz, xy, xz, yz), taken with each point's located face held fixed; the
+ * gradient has no counterpart in the reference (which refines by Nelder-Mead).  Points
+ * outside the index are counted in *uncovered_dev, which the caller clears.  Does not
+ * synchronise. */
+size_t csg_rotation_cost_matrices_workspace_bytes(int ns, int na, int with_gradient);
+int csg_rotation_cost_matrices(const float *src_pts, const double *src_vals, int ns,
+                               const double *tgt_vals, const int *faces, const float *tri,
+                               const int *cell_off, const int *cell_faces, int grid,
+                               const float *mats, int na, int with_gradient, double *out,
+                               void *ws, size_t ws_bytes, int *uncovered_dev, void *stream);
+
 /* Exact closest triangle (first minimum among the query cell's candidates) with
  * barycentric weights; far_host counts queries farther than `reach` from their best
  * triangle (their true closest triangle may lie outside the searched cells).

@@ -240,3 +240,53 @@ def test_fibonacci_hull_is_a_closed_outward_mesh(seed):
     tri = v[f]
     normal = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
     assert (np.einsum("ij,ij->i", normal, tri.mean(axis=1)) > 0).all()
+
+
+def test_euler_angles_invert_the_reference_rotation_matrix():
+    """angles_from_matrix inverts the oracle's Rz(Ry Rx) column-major layout."""
+
+    from oracle import oracle as O
+    from cat_surface_gpu.surface_rotation import angles_from_matrix
+
+    rng = np.random.default_rng(11)
+    for a in rng.uniform(-1.5, 1.5, size=(50, 3)):
+        m = O.rotation_matrix_from_angles(a)
+        b = angles_from_matrix(m)
+        np.testing.assert_allclose(O.rotation_matrix_from_angles(b), m, atol=1e-13)
+        np.testing.assert_allclose(b, a, atol=1e-12)   # |beta| < pi/2: the same triple
+
+
+def test_hopf_grid_is_a_uniform_rotation_grid():
+    """8,192 proper rotations, all distinct, with the spacing of a uniform grid."""
+
+    from cat_surface_gpu.surface_rotation import hopf_grid, quaternion_matrix
+
+    g = hopf_grid()
+    assert g.shape == (8192, 3, 3)
+    np.testing.assert_allclose(np.einsum("nij,nkj->nik", g, g), np.broadcast_to(np.eye(3),
+                                                                                 g.shape),
+                               atol=1e-12)
+    np.testing.assert_allclose(np.linalg.det(g), 1.0, atol=1e-12)
+    # geodesic distance between rotations: arccos((trace(A Bᵀ) - 1) / 2)
+    tr = np.einsum("aij,bij->ab", g[:256], g)
+    dist = np.arccos(np.clip((tr - 1.0) / 2.0, -1.0, 1.0))
+    dist[np.arange(256), np.arange(256)] = np.inf
+    nearest = dist.min(axis=1)
+    assert nearest.min() > 0.15          # no duplicates (q and -q are one rotation)
+    assert nearest.max() < 0.25          # ~ (8 pi^2 / 8192)^(1/3) spacing
+    # covering: random rotations lie within ~0.17 rad of a grid rotation; a
+    # uniform grid of 8192 covers SO(3) at about that radius
+    rng = np.random.default_rng(2)
+    r = quaternion_matrix(rng.normal(size=(500, 4)))
+    cover = np.arccos(np.clip((np.einsum("aij,bij->ab", r, g) - 1.0) / 2.0, -1, 1)).min(axis=1)
+    assert cover.max() < 0.2
+
+
+def test_axis_angle_matrix_is_the_matrix_exponential():
+    import scipy.linalg
+
+    from cat_surface_gpu.surface_rotation import axis_angle_matrix
+
+    for d in (np.array([0.3, -0.2, 0.5]), np.array([1e-9, 0.0, 2e-9]), np.zeros(3)):
+        k = np.array([[0.0, -d[2], d[1]], [d[2], 0.0, -d[0]], [-d[1], d[0], 0.0]])
+        np.testing.assert_allclose(axis_angle_matrix(d), scipy.linalg.expm(k), atol=1e-14)

@@ -73,3 +73,25 @@ def test_batched_jobs_equal_single_runs(pair_paths, cuda, tmp_path):
         np.testing.assert_array_equal(res.vertices, alone.vertices)
         np.testing.assert_array_equal(res.rotation_angle, alone.rotation_angle)
         np.testing.assert_array_equal(res.faces, alone.faces)
+
+
+def test_adjoint_rotation_pipeline_matches_nelder_mead(pair_paths, cuda):
+    """Gradient-refined rotation: no higher SSD than Nelder-Mead's choice (1e-6
+    relative) and a registration within the 1e-3 vertex tolerance of it."""
+
+    from cat_surface_gpu import GiftiMesh, PipelineJob, run_cat_surface_gpu_pipelines
+
+    pair, _, _ = pair_paths
+    job = PipelineJob(meshes=(GiftiMesh(pair.src_surface, pair.src_faces),
+                              GiftiMesh(pair.src_sphere, pair.src_faces),
+                              GiftiMesh(pair.tgt_surface, pair.tgt_faces),
+                              GiftiMesh(pair.tgt_sphere, pair.tgt_faces)))
+    kw = dict(device=cuda, dartel_dtype="float32", steps=2, runs=2, avg=True,
+              write_output=False)
+    nm = run_cat_surface_gpu_pipelines([job], **kw)[0]
+    for candidates in ("seeds", "hopf"):
+        adj = run_cat_surface_gpu_pipelines([job], rotation_method="adjoint",
+                                            rotation_candidates=candidates, **kw)[0]
+        assert adj.rotation_cost <= nm.rotation_cost * (1.0 + 1e-6)
+        np.testing.assert_array_equal(adj.faces, nm.faces)
+        assert np.abs(adj.vertices - nm.vertices).max() <= 1e-3

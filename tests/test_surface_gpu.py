@@ -263,3 +263,76 @@ def test_stencil_builder_matches_brute_force(cuda):
     np.testing.assert_array_equal(st.sheet_indices.reshape(-1, 3), st.faces[face2])
     np.testing.assert_allclose(st.sheet_weights.reshape(-1, 3), w2, atol=1e-12)
     np.testing.assert_allclose(np.linalg.norm(st.unit_sphere_points, axis=1), 1.0, atol=1e-6)
+
+
+def _smooth_rotation_problem(cuda, angle=(0.2, -0.1, 0.15), tgt_level=4, src_level=3):
+    """Target icosphere carrying a smooth field; source points whose values are
+    the field at their image under ``angle``'s rotation."""
+
+    import cat_surface_gpu as csg
+    from cat_surface_gpu.stencil_builder import icosphere
+    from cat_surface_gpu.surface_rotation import rotation_matrix_np
+
+    def field(p):
+        return np.sin(2.0 * p[:, 0]) + np.cos(3.0 * p[:, 1]) * p[:, 2] + 0.5 * p[:, 0] * p[:, 1]
+
+    v, f = icosphere(tgt_level)
+    src, _ = icosphere(src_level)
+    m = rotation_matrix_np(angle)
+    idx = csg.RotationGridIndex.from_geometry(v, f, grid_size=32, margin=1,
+                                              device=cuda).to(cuda)
+    inputs = idx.prepare_cost_inputs(src.astype(np.float32), field(src @ m.T), field(v))
+    return idx, inputs, m
+
+
+def test_rotation_cost_gradient_matches_finite_differences(cuda):
+    from cat_surface_gpu.surface_rotation import angles_from_matrix, axis_angle_matrix
+
+    idx, inputs, m = _smooth_rotation_problem(cuda)
+    m0 = axis_angle_matrix([0.05, -0.03, 0.04]) @ m
+    cost, grad, hess = idx.cost_matrices(inputs, m0[None], gradient=True)
+    cost, grad, hess = float(cost[0]), _np(grad[0]), _np(hess[0])
+    # the matrix-input cost is the angle-input cost of the same rotation
+    ref = float(idx.cost_batch_prepared(inputs, torch.as_tensor(angles_from_matrix(m0)[None]))[0])
+    assert abs(cost - ref) <= 1e-4 * ref
+    assert float(idx.cost_matrices(inputs, m0[None])[0]) == cost
+    h = 1e-3
+    fd = np.empty(3)
+    for k in range(3):
+        e = np.zeros(3)
+        e[k] = h
+        plus = float(idx.cost_matrices(inputs, (axis_angle_matrix(e) @ m0)[None])[0])
+        minus = float(idx.cost_matrices(inputs, (axis_angle_matrix(-e) @ m0)[None])[0])
+        fd[k] = (plus - minus) / (2.0 * h)
+    np.testing.assert_allclose(grad, fd, rtol=0, atol=2e-2 * np.abs(fd).max())
+    # Gauss-Newton matrix: symmetric positive semi-definite
+    np.testing.assert_allclose(hess, hess.T)
+    assert np.linalg.eigvalsh(hess).min() >= -1e-9 * np.abs(hess).max()
+
+
+@pytest.mark.parametrize("candidates", ["seeds", "hopf"])
+def test_adjoint_rotation_search_reaches_nelder_mead_cost(cuda, candidates):
+    """Top candidates refined by gradient steps cost no more than the reference's
+    seed + Nelder-Mead choice (within 1e-6 relative) and find the true rotation."""
+
+    from cat_surface_gpu.surface_rotation import search_rotations
+
+    idx, inputs, m = _smooth_rotation_problem(cuda)
+    nm = search_rotations([(idx, inputs)])[0]
+    adj = search_rotations([(idx, inputs)], method="adjoint", candidates=candidates)[0]
+    assert float(adj.cost) <= float(nm.cost) * (1.0 + 1e-6)
+    rel = _np(adj.rotation_matrix).astype(np.float64) @ m.T
+    assert np.arccos(np.clip((np.trace(rel) - 1.0) / 2.0, -1.0, 1.0)) < 5e-3
+
+
+def test_adjoint_rotation_search_on_reference_problem(golden, cuda):
+    import cat_surface_gpu as csg
+    from cat_surface_gpu.surface_rotation import search_rotations
+
+    g = golden("rotation")
+    idx = csg.RotationGridIndex.from_geometry(g["verts"], g["faces"], grid_size=32, margin=1,
+                                              device=cuda).to(cuda)
+    inputs = idx.prepare_cost_inputs(g["src_pts"], g["src_vals"], g["tgt_vals"])
+    adj = search_rotations([(idx, inputs)], method="adjoint")[0]
+    assert float(adj.cost) <= float(g["nm_cost"]) * (1.0 + 1e-6)
+    np.testing.assert_allclose(_np(adj.seed_angle), g["seed_angle"])
