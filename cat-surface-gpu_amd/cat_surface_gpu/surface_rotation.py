@@ -11,6 +11,7 @@ simplex bookkeeping runs on the host with one cost launch per evaluation.
 from __future__ import annotations
 
 import ctypes
+import functools
 from dataclasses import dataclass
 
 import numpy as np
@@ -296,8 +297,12 @@ class RotationGridIndexDevice:
         """Enqueue the matrix-input cost ([na]) or cost + gradient + Gauss-Newton
         terms ([na, 10]); the uncovered count goes to ``uncovered_dev``."""
 
-        mats = torch.as_tensor(np.asarray(matrices, dtype=np.float64).reshape(-1, 9),
-                               dtype=torch.float32).to(self.device).contiguous()
+        if isinstance(matrices, torch.Tensor):
+            mats = matrices.to(device=self.device, dtype=torch.float32).reshape(-1, 9)
+        else:
+            mats = torch.as_tensor(np.asarray(matrices, dtype=np.float64).reshape(-1, 9),
+                                   dtype=torch.float32).to(self.device)
+        mats = mats.contiguous()
         na, ns = int(mats.shape[0]), int(inputs.source_points.shape[0])
         out = torch.empty((na, 10) if gradient else (na,), dtype=torch.float64,
                           device=self.device)
@@ -529,74 +534,89 @@ def _gn_matrix(terms):
                      np.stack([t[..., 4], t[..., 5], t[..., 2]], -1)], -2)
 
 
-def _lm_step(grad, hess, lam, max_step):
-    """Levenberg-Marquardt step (H + lam diag(H)) d = -g, at most max_step radians."""
+def refine_adjoint_launch(index, inputs, start: torch.Tensor, *, start_is_angles: bool,
+                          steps: int = 20, max_step: float = 0.05, damping: float = 1e-3):
+    """Enqueue csg_rotation_refine for one problem (nothing synchronises).
 
-    diag = np.maximum(np.diag(hess), 1e-12 * max(1.0, float(np.max(np.abs(np.diag(hess))))))
-    a = hess + lam * np.diag(diag)
-    try:
-        d = -np.linalg.solve(a, grad)
-    except np.linalg.LinAlgError:
-        d = -grad / (lam * diag + 1e-300)
-    n = float(np.linalg.norm(d))
-    if n > max_step:
-        d *= max_step / n
-    return d
+    ``start`` is a device tensor of m Euler-angle triples ([m, 3],
+    ``start_is_angles``) or m row-major matrices ([m, 9] / [m, 3, 3]).
+    Returns device tensors (matrices [m, 3, 3], angles [m, 3], costs [m] in the
+    reference's angle formulation, uncovered-point count [1]).
+    """
+
+    st = start.to(device=index.device, dtype=torch.float64).contiguous()
+    m = int(st.shape[0])
+    ns = int(inputs.source_points.shape[0])
+    mats = torch.empty((m, 3, 3), dtype=torch.float64, device=index.device)
+    angles = torch.empty((m, 3), dtype=torch.float64, device=index.device)
+    costs = torch.empty(m, dtype=torch.float64, device=index.device)
+    unc = torch.empty(1, dtype=torch.int32, device=index.device)
+    lib = _lib.lib()
+    ws = _lib.workspace(lib.csg_rotation_refine_workspace_bytes(ns, m), index.device)
+    _lib.check(lib.csg_rotation_refine(
+        _lib.ptr(inputs.source_points), _lib.ptr(inputs.source_values), ns,
+        _lib.ptr(inputs.target_values), _lib.ptr(index.target_faces),
+        _lib.ptr(index.target_triangles), _lib.ptr(index.candidate_offsets),
+        _lib.ptr(index.candidate_faces), int(index.grid_size), _lib.ptr(st),
+        int(start_is_angles), m, int(steps), float(max_step), float(damping), _lib.ptr(mats),
+        _lib.ptr(angles), _lib.ptr(costs), _lib.ptr(ws), ws.numel(), _lib.ptr(unc),
+        _lib.stream_handle(index.device)), "csg_rotation_refine")
+    return mats, angles, costs, unc
 
 
 def refine_adjoint_many(problems, starts, *, steps: int = 20, max_step: float = 0.05,
-                        damping: float = 1e-3):
-    """Gradient refinement of candidate rotations for several problems in lockstep.
+                        damping: float = 1e-3, start_is_angles: bool = False):
+    """Gradient refinement of candidate rotations for several problems.
 
-    ``problems`` holds (index, prepared inputs); ``starts[k]`` is an [m, 3, 3]
-    array of starting matrices for problem k.  Every candidate takes ``steps``
-    Levenberg-Marquardt steps on a left axis-angle perturbation, driven by the
-    cost gradient and Gauss-Newton matrix of csg_rotation_cost_matrices; a
-    step is kept only if it lowers the cost (else the damping grows).  Each
-    step is one launch per problem and one synchronisation for all.  Returns,
-    per problem, the candidates' final matrices [m, 3, 3] and costs [m].
+    ``problems`` holds (index, prepared inputs); ``starts[k]`` holds problem
+    k's starting candidates (matrices, or Euler angles with
+    ``start_is_angles``).  Every candidate takes ``steps`` Levenberg-Marquardt
+    steps on the device (csg_rotation_refine); all problems are enqueued
+    before one synchronisation.  Returns per problem (matrices [m, 3, 3],
+    Euler angles [m, 3], costs [m] in the reference's angle formulation) as
+    host arrays.
     """
 
     if steps < 0 or max_step <= 0.0 or damping <= 0.0:
         raise ValueError("steps must be non-negative; max_step and damping positive")
-    mats = [np.asarray(m, dtype=np.float64).reshape(-1, 3, 3).copy() for m in starts]
-    lams = [np.full(len(m), damping) for m in mats]
+    launched = []
+    for (index, inputs), start in zip(problems, starts):
+        if not isinstance(start, torch.Tensor):
+            start = torch.as_tensor(np.asarray(start, dtype=np.float64))
+        launched.append(refine_adjoint_launch(index, inputs, start,
+                                              start_is_angles=start_is_angles, steps=steps,
+                                              max_step=max_step, damping=damping))
+    host = torch.cat([torch.cat([m.reshape(-1), a.reshape(-1), c, u.to(torch.float64)])
+                      for m, a, c, u in launched]).cpu().numpy()
+    out, pos = [], 0
+    for m, _, _, _ in launched:
+        k = int(m.shape[0])
+        mats = host[pos:pos + 9 * k].reshape(k, 3, 3)
+        angles = host[pos + 9 * k:pos + 12 * k].reshape(k, 3)
+        costs = host[pos + 12 * k:pos + 13 * k]
+        if host[pos + 13 * k]:
+            raise RuntimeError("Rotation is not covered by target sphere candidate index")
+        pos += 13 * k + 1
+        out.append((mats, angles, costs))
+    return out
 
-    def evaluate(candidates):
-        outs, uncs = [], []
-        for (index, inputs), cm in zip(problems, candidates):
-            unc = torch.zeros(1, dtype=torch.int32, device=index.device)
-            outs.append(index._launch_matrix_costs(inputs, cm, True, unc))
-            uncs.append(unc)
-        host = torch.cat([torch.cat([o.reshape(-1), u.to(torch.float64)])
-                          for o, u in zip(outs, uncs)]).cpu().numpy()
-        res, pos = [], 0
-        for cm in candidates:
-            n = len(cm) * 10
-            vals, unc = host[pos:pos + n].reshape(-1, 10), host[pos + n]
-            pos += n + 1
-            if unc:
-                raise RuntimeError("Rotation is not covered by target sphere candidate index")
-            res.append((vals[:, 0].copy(), vals[:, 1:4].copy(), _gn_matrix(vals[:, 4:10])))
-        return res
 
-    state = evaluate(mats)
-    for _ in range(steps):
-        trials = []
-        for k, (cost, grad, hess) in enumerate(state):
-            trials.append(np.stack([axis_angle_matrix(_lm_step(grad[j], hess[j], lams[k][j],
-                                                               max_step)) @ mats[k][j]
-                                    for j in range(len(mats[k]))]))
-        new = evaluate(trials)
-        for k in range(len(problems)):
-            cost, grad, hess = state[k]
-            tcost, tgrad, thess = new[k]
-            better = tcost < cost
-            mats[k][better] = trials[k][better]
-            cost[better], grad[better], hess[better] = (tcost[better], tgrad[better],
-                                                        thess[better])
-            lams[k] = np.where(better, np.maximum(lams[k] / 3.0, 1e-7), lams[k] * 4.0)
-    return [(m, st[0]) for m, st in zip(mats, state)]
+_HOPF_DEVICE: dict[str, torch.Tensor] = {}
+
+
+@functools.lru_cache(maxsize=1)
+def _hopf_host() -> np.ndarray:
+    return hopf_grid()
+
+
+def _hopf_device(device) -> torch.Tensor:
+    """The 8,192-rotation Hopf grid as a [8192, 9] float64 device tensor (cached)."""
+
+    key = str(device)
+    if key not in _HOPF_DEVICE:
+        _HOPF_DEVICE[key] = torch.as_tensor(_hopf_host().reshape(-1, 9), dtype=torch.float64,
+                                            device=device)
+    return _HOPF_DEVICE[key]
 
 
 def search_rotations(problems, *, refine: bool = True, max_iter: int = 500, tol: float = 1.0e-4,
@@ -654,71 +674,52 @@ def search_rotations(problems, *, refine: bool = True, max_iter: int = 500, tol:
 
 def _search_rotations_adjoint(problems, *, refine: bool, candidates: str, top: int,
                               steps: int) -> list[RotationSearchResult]:
+    """Candidate scan, top-k and gradient refinement, enqueued for every problem
+    and read back with one synchronisation."""
+
     if top < 1:
         raise ValueError("top must be positive")
-    firsts, starts = [], []
-    if candidates == "seeds":
-        seeds_np = official_seed_grid_angles()
-        launched = []
-        for index, inputs in problems:
-            seeds = torch.as_tensor(seeds_np, dtype=torch.float64, device=index.device)
-            unc = torch.zeros(1, dtype=torch.int32, device=index.device)
-            costs, _, _, _ = index._launch_costs(inputs, seeds, uncovered_dev=unc)
-            launched.append((costs, unc))
-        host = torch.cat([torch.cat([c, u.to(torch.float64)]) for c, u in launched]).cpu()
-        host = host.numpy()
-        pos = 0
-        for costs, _ in launched:
-            vals, unc = host[pos:pos + len(seeds_np)], host[pos + len(seeds_np)]
-            pos += len(seeds_np) + 1
-            if unc:
-                raise RuntimeError("Rotation is not covered by target sphere candidate index")
-            order = np.argsort(vals, kind="stable")
-            firsts.append((seeds_np[order[0]], vals[order[0]], costs))
-            starts.append(np.stack([rotation_matrix_np(seeds_np[j]) for j in order[:top]]))
-    else:
-        grid = hopf_grid()
-        launched = []
-        for index, inputs in problems:
-            unc = torch.zeros(1, dtype=torch.int32, device=index.device)
-            launched.append((index._launch_matrix_costs(inputs, grid, False, unc), unc))
-        host = torch.cat([torch.cat([c, u.to(torch.float64)]) for c, u in launched]).cpu()
-        host = host.numpy()
-        pos = 0
-        for costs, _ in launched:
-            vals, unc = host[pos:pos + len(grid)], host[pos + len(grid)]
-            pos += len(grid) + 1
-            if unc:
-                raise RuntimeError("Rotation is not covered by target sphere candidate index")
-            order = np.argsort(vals, kind="stable")
-            firsts.append((angles_from_matrix(grid[order[0]]), vals[order[0]], costs))
-            starts.append(grid[order[:top]])
-    if refine and steps > 0:
-        refined = refine_adjoint_many(problems, starts, steps=steps)
-    else:
-        refined = [(st, None) for st in starts]
-    # the final choice is costed in the reference's angle formulation
-    finals = []
-    for (index, inputs), (mats, _) in zip(problems, refined):
-        ang = np.stack([angles_from_matrix(m) for m in mats])
-        unc = torch.zeros(1, dtype=torch.int32, device=index.device)
-        costs, _, _, _ = index._launch_costs(inputs, torch.as_tensor(ang), uncovered_dev=unc)
-        finals.append((ang, costs, unc))
-    host = torch.cat([torch.cat([c, u.to(torch.float64)]) for _, c, u in finals]).cpu().numpy()
-    out, pos = [], 0
-    for (index, _), (seed_angle, seed_cost, seed_costs), (ang, costs, _) in zip(
-            problems, firsts, finals):
-        vals, unc = host[pos:pos + len(ang)], host[pos + len(ang)]
-        pos += len(ang) + 1
-        if unc:
-            raise RuntimeError("Rotation is not covered by target sphere candidate index")
-        best = int(np.argmin(vals))
+    staged = []
+    for index, inputs in problems:
         dev = index.device
+        unc = torch.zeros(1, dtype=torch.int32, device=dev)
+        if candidates == "seeds":
+            pool = torch.as_tensor(official_seed_grid_angles(), dtype=torch.float64, device=dev)
+            costs, _, _, _ = index._launch_costs(inputs, pool, uncovered_dev=unc)
+        else:
+            pool = _hopf_device(dev)
+            costs = index._launch_matrix_costs(inputs, pool, False, unc)
+        order = torch.sort(costs, stable=True).indices[:top]
+        start = pool[order]
+        if refine and steps > 0:
+            _, angles, final, unc2 = refine_adjoint_launch(
+                index, inputs, start, start_is_angles=candidates == "seeds", steps=steps)
+        else:
+            _, angles, final, unc2 = refine_adjoint_launch(
+                index, inputs, start, start_is_angles=candidates == "seeds", steps=0)
+        staged.append((costs, order, angles, final, unc, unc2))
+    host = torch.cat([torch.cat([order[:1].to(torch.float64), angles.reshape(-1), final,
+                                 unc.to(torch.float64), unc2.to(torch.float64)])
+                      for _, order, angles, final, unc, unc2 in staged]).cpu().numpy()
+    out, pos = [], 0
+    for (index, _), (costs, order, angles, final, _, _) in zip(problems, staged):
+        k = int(final.numel())
+        first = int(host[pos])
+        ang = host[pos + 1:pos + 1 + 3 * k].reshape(k, 3)
+        vals = host[pos + 1 + 3 * k:pos + 1 + 4 * k]
+        if host[pos + 1 + 4 * k] or host[pos + 2 + 4 * k]:
+            raise RuntimeError("Rotation is not covered by target sphere candidate index")
+        pos += 3 + 4 * k
+        dev = index.device
+        if candidates == "seeds":
+            seed_angle = official_seed_grid_angles()[first]
+        else:
+            seed_angle = angles_from_matrix(_hopf_host()[first])
+        best = int(np.argmin(vals))
         angle = torch.as_tensor(ang[best], dtype=torch.float64, device=dev)
         out.append(RotationSearchResult(
             seed_angle=torch.as_tensor(seed_angle, dtype=torch.float64, device=dev),
-            seed_cost=torch.as_tensor(seed_cost, dtype=torch.float64, device=dev),
-            seed_costs=seed_costs, angle=angle,
+            seed_cost=costs[first], seed_costs=costs, angle=angle,
             cost=torch.as_tensor(vals[best], dtype=torch.float64, device=dev),
             rotation_matrix=index.rotation_matrix(angle),
             iterations=steps if refine else 0))

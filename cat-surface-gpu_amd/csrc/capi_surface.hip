@@ -22,13 +22,6 @@ struct Carve {
 
 inline hipStream_t S(void *s) { return (hipStream_t)s; }
 
-#define CSG_TRY(expr)      \
-    do {                   \
-        int _rc = (expr);  \
-        if (_rc != kOk)    \
-            return _rc;    \
-    } while (0)
-
 // Shared pieces of the heat-value smoothing: workspace layout and run.
 struct HeatWs {
     double *w, *wsum, *buf[2];

@@ -37,6 +37,14 @@ int fail(int code, const char *fmt, ...);
 
 #define CSG_LAUNCH_CHECK() CSG_HIP_CHECK(hipGetLastError())
 
+// Propagate a non-zero status code of a library call.
+#define CSG_TRY(expr)                      \
+    do {                                   \
+        int _rc = (expr);                  \
+        if (_rc != ::csg::kOk)             \
+            return _rc;                    \
+    } while (0)
+
 inline int dtype_size(int dtype) { return dtype == 1 ? 8 : 4; }
 
 // ---------------------------------------------------------------- grid maths

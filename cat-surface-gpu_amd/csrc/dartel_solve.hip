@@ -68,13 +68,6 @@ int fill_async(T *out, T value, size_t n, hipStream_t s)
     return kOk;
 }
 
-#define CSG_TRY(expr)           \
-    do {                        \
-        int _rc = (expr);       \
-        if (_rc != kOk)         \
-            return _rc;         \
-    } while (0)
-
 }  // namespace
 
 int fmg_levels(int nx, int ny, int *lnx, int *lny)

@@ -299,6 +299,23 @@ int csg_rotation_cost_matrices(const float *src_pts, const double *src_vals, int
                                const float *mats, int na, int with_gradient, double *out,
                                void *ws, size_t ws_bytes, int *uncovered_dev, void *stream);
 
+/* Gradient refinement of m <= 256 candidate rotations (no counterpart in the reference,
+ * which refines its best seed by Nelder-Mead, surface_rotation.py:594-699).  start holds
+ * m Euler-angle triples (start_is_angles = 1, the reference's angle convention) or m
+ * row-major float64 matrices.  Each candidate takes `steps` Levenberg-Marquardt steps on
+ * a left axis-angle perturbation driven by csg_rotation_cost_matrices' gradient and
+ * Gauss-Newton terms (step clamped to max_step radians, initial damping `damping`; a
+ * step is kept only if it lowers the cost).  Writes the final matrices [m][9], their
+ * Euler angles [m][3] and those angles' costs in the reference's angle formulation
+ * (csg_rotation_costs) [m]; clears and fills *uncovered_dev.  Nothing synchronises. */
+size_t csg_rotation_refine_workspace_bytes(int ns, int m);
+int csg_rotation_refine(const float *src_pts, const double *src_vals, int ns,
+                        const double *tgt_vals, const int *faces, const float *tri,
+                        const int *cell_off, const int *cell_faces, int grid, const double *start,
+                        int start_is_angles, int m, int steps, double max_step, double damping,
+                        double *mats_out, double *angles_out, double *costs_out, void *ws,
+                        size_t ws_bytes, int *uncovered_dev, void *stream);
+
 /* Exact closest triangle (first minimum among the query cell's candidates) with
  * barycentric weights; far_host counts queries farther than `reach` from their best
  * triangle (their true closest triangle may lie outside the searched cells).
