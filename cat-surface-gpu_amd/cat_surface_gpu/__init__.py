@@ -70,3 +70,4 @@ from .gpu_pipeline import (
     run_cat_surface_gpu_pipelines,
     write_gifti_mesh,
 )
+from .dartel_adjoint import ExpWarpSSD, dartel_rhs, exp_warp_ssd

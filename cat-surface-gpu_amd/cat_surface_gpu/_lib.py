@@ -62,6 +62,11 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_solve_workspace_bytes": (_sz, [_i, _i, _i, _i]),
     "csg_solve_dartel_maps": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _d, _i, _i, _i, _i, _i, _i,
                                    _vp, _vp, _vp, _vp, _sz, _vp]),
+    "csg_exp_warp_ssd_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
+    "csg_exp_warp_ssd_forward": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i, _i, _vp, _vp, _vp, _sz,
+                                      _vp]),
+    "csg_exp_warp_ssd_backward": (_i, [_vp, _vp, _vp, _i, _i, _i, _d, _i, _i, _vp, _vp, _vp,
+                                       _sz, _vp]),
     # surface side
     "csg_resample_vertices": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp]),
     "csg_edge_mean_workspace_bytes": (_sz, [_i]),

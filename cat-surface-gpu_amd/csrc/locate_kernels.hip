@@ -231,20 +231,32 @@ __device__ __forceinline__ LocateBest scan_candidates(const float *q, const floa
             cpt[1] = proj[1];
             cpt[2] = proj[2];
         } else {
-            const float *vs[3] = {p0, p1, p2};
-            float dv[3];
-            for (int k = 0; k < 3; ++k) {
-                float e0 = q[0] - vs[k][0], e1 = q[1] - vs[k][1], e2 = q[2] - vs[k][2];
-                dv[k] = e0 * e0 + e1 * e1 + e2 * e2;
-            }
+            // nearest vertex (first minimum), then the two edges through it;
+            // selects instead of indexed pointer tables keep this in registers
+            float e0 = q[0] - p0[0], e1 = q[1] - p0[1], e2 = q[2] - p0[2];
+            const float dv0 = e0 * e0 + e1 * e1 + e2 * e2;
+            e0 = q[0] - p1[0], e1 = q[1] - p1[1], e2 = q[2] - p1[2];
+            const float dv1 = e0 * e0 + e1 * e1 + e2 * e2;
+            e0 = q[0] - p2[0], e1 = q[1] - p2[1], e2 = q[2] - p2[2];
+            const float dv2 = e0 * e0 + e1 * e1 + e2 * e2;
             int cv = 0;
-            if (dv[1] < dv[cv])
+            float dbest = dv0;
+            if (dv1 < dbest) {
                 cv = 1;
-            if (dv[2] < dv[cv])
+                dbest = dv1;
+            }
+            if (dv2 < dbest)
                 cv = 2;
+            float prev[3], cur[3], next[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                cur[d] = cv == 0 ? p0[d] : (cv == 1 ? p1[d] : p2[d]);
+                prev[d] = cv == 0 ? p2[d] : (cv == 1 ? p0[d] : p1[d]);
+                next[d] = cv == 0 ? p1[d] : (cv == 1 ? p2[d] : p0[d]);
+            }
             float d1, d2, q1[3], q2[3];
-            seg_dist(q, vs[(cv + 2) % 3], vs[cv], &d1, q1);
-            seg_dist(q, vs[cv], vs[(cv + 1) % 3], &d2, q2);
+            seg_dist(q, prev, cur, &d1, q1);
+            seg_dist(q, cur, next, &d2, q2);
             if (d1 < d2) {
                 dist = d1;
                 cpt[0] = q1[0];
@@ -293,10 +305,11 @@ __device__ __forceinline__ void locate_weights(const float *__restrict__ tri, in
                                                const float *bp, float *bary)
 {
     const float *t = tri + 9 * (size_t)bf;
-    const float *tv[3] = {t, t + 3, t + 6};
-    const int order[3][3] = {{1, 2, 0}, {2, 0, 1}, {0, 1, 2}};
+    // weight k uses the vertex order (k+1, k+2, k): offsets of the three vertices
+    const int first[3] = {3, 6, 0}, second[3] = {6, 0, 3}, third[3] = {0, 3, 6};
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float *fi = tv[order[k][0]], *se = tv[order[k][1]], *th = tv[order[k][2]];
+        const float *fi = t + first[k], *se = t + second[k], *th = t + third[k];
         float po[3] = {bp[0] - fi[0], bp[1] - fi[1], bp[2] - fi[2]};
         float ho[3] = {se[0] - fi[0], se[1] - fi[1], se[2] - fi[2]};
         float up[3] = {th[0] - fi[0], th[1] - fi[1], th[2] - fi[2]};

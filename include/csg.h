@@ -153,6 +153,28 @@ int csg_solve_dartel_maps(const void *src, const void *tgt, int nx, int ny, int 
                           int code, int assume_nonzero, int dtype, void *flow, void *velocity,
                           double *metrics, void *ws, size_t ws_bytes, void *stream);
 
+/* Exp-warp-SSD objective and its hand-written adjoint (no counterpart in the reference,
+ * whose DARTEL step uses the push-forward gradient of dartel_grid.py:1509-1592):
+ *   E(v) = 1/2 sum (f(phi) - t)^2 + 1/2 sum v . (L v),  phi = exp(v),
+ * with K squarings of phi0 = id + v 2^-K (expdef, dartel_grid.py:1619-1666), the bilinear
+ * sample of initialise_objfun (:723-775) and L the rtype-1 regulariser with parameters
+ * (1, 1, membrane, 0, 0) (:917-955).  v is [2][ny][nx], feature and target [ny][nx], all
+ * of dtype (0 float32, 1 float64).  forward writes E to energy[0] (device double) and,
+ * if parts != NULL, (SSD, membrane) to parts[0..1].  backward writes dE/dv (times
+ * *grad_scale when given) to grad_v.  store = 1 keeps the K + 1 maps of the forward pass
+ * in the workspace for backward (pass the same workspace, forward first); store = 0
+ * rebuilds them by recomputation (two maps of workspace).  The adjoint accumulates with
+ * float atomics: gradients vary in their last bits from run to run. */
+size_t csg_exp_warp_ssd_workspace_bytes(int nx, int ny, int K, int dtype, int store);
+int csg_exp_warp_ssd_forward(const void *v, const void *feature, const void *target, int nx,
+                             int ny, int K, double membrane, int dtype, int store,
+                             double *energy, double *parts, void *ws, size_t ws_bytes,
+                             void *stream);
+int csg_exp_warp_ssd_backward(const void *v, const void *feature, const void *target, int nx,
+                              int ny, int K, double membrane, int dtype, int store,
+                              const double *grad_scale, void *grad_v, void *ws,
+                              size_t ws_bytes, void *stream);
+
 /* ---------------------------------------------------------------- surface */
 
 /* Stencil resampling of [n_src][3] vertices to [n_out][3] (dtype 0: float32 with the

@@ -1,0 +1,104 @@
+"""The hand-written adjoint of exp + bilinear warp + SSD (+ membrane).
+
+The forward energy is pinned to the oracle's expdef / initialise_objfun /
+regularisation (themselves pinned to the reference's golden vectors); the
+adjoint is pinned by finite differences of that forward
+(torch.autograd.gradcheck in float64 at 64x32, eps 1e-6, rtol 1e-5), float32
+against float64 (relative L2 <= 1e-4), recompute against store, and compared
+with the reference DARTEL step's code-0 right-hand side (a different
+discretisation of the same continuous gradient, so only close, not equal).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(nx, ny, cuda, dtype=torch.float64, seed=0):
+    from cat_surface_gpu.dartel_adjoint import synthetic_problem
+
+    return synthetic_problem(nx, ny, seed=seed, dtype=dtype, device=cuda)
+
+
+@pytest.mark.parametrize("dtype,rtol", [(torch.float64, 1e-10), (torch.float32, 2e-5)])
+def test_forward_energy_matches_oracle(cuda, dtype, rtol):
+    from cat_surface_gpu.dartel_adjoint import exp_warp_ssd, exp_warp_ssd_parts
+
+    v, f, t = _problem(64, 32, cuda, dtype)
+    e, ssd, mem = exp_warp_ssd_parts(v, f, t, squarings=6, membrane=0.01)
+    vn, fn, tn = (x.double().cpu().numpy() for x in (v, f, t))
+    phi = O.expdef(vn, 6)
+    ident = np.zeros((4,) + fn.shape)
+    ident[0] = ident[3] = 1.0
+    obj, _, _ = O.initialise_objfun(fn, tn, phi, ident)
+    reg = 0.5 * float((vn * O.regularization_operator(vn, (1.0, 1.0, 0.01, 0.0, 0.0))).sum())
+    assert abs(ssd - obj) <= rtol * obj
+    assert abs(mem - reg) <= rtol * abs(reg)
+    assert abs(e - (ssd + mem)) <= 1e-12 * e
+    assert float(exp_warp_ssd(v, f, t)) == e
+
+
+@pytest.mark.parametrize("recompute", [True, False])
+def test_gradcheck_float64_64x32(cuda, recompute):
+    from cat_surface_gpu.dartel_adjoint import exp_warp_ssd
+
+    v, f, t = _problem(64, 32, cuda)
+    w = v.clone().requires_grad_(True)
+    assert torch.autograd.gradcheck(
+        lambda x: exp_warp_ssd(x, f, t, squarings=6, membrane=0.01, recompute=recompute),
+        (w,), eps=1e-6, rtol=1e-5, atol=1e-6, fast_mode=False)
+
+
+def test_recompute_equals_store(cuda):
+    from cat_surface_gpu.dartel_adjoint import exp_warp_ssd, relative_l2
+
+    v, f, t = _problem(128, 64, cuda)
+    grads = []
+    for recompute in (True, False):
+        w = v.clone().requires_grad_(True)
+        exp_warp_ssd(w, f, t, recompute=recompute).backward()
+        grads.append(w.grad)
+    # the same arithmetic; only the order of the atomic accumulations differs
+    assert relative_l2(grads[0], grads[1]) < 1e-13
+
+
+@pytest.mark.parametrize("nx,ny", [(256, 128), (512, 256)])
+def test_float32_adjoint_against_float64_and_dartel_rhs(cuda, nx, ny):
+    from cat_surface_gpu.dartel_adjoint import cosine, dartel_rhs, exp_warp_ssd, relative_l2
+
+    v64, f64, t64 = _problem(nx, ny, cuda)
+    grads = {}
+    for dtype in (torch.float64, torch.float32):
+        w = v64.to(dtype).clone().requires_grad_(True)
+        exp_warp_ssd(w, f64.to(dtype), t64.to(dtype)).backward()
+        grads[dtype] = w.grad
+    assert relative_l2(grads[torch.float32], grads[torch.float64]) <= 1e-4
+    rhs = dartel_rhs(f64, t64, v64, squarings=6, membrane=0.01)
+    # DARTEL's push-forward gradient discretises the same continuous gradient
+    # differently: close in direction, not equal
+    rel = relative_l2(grads[torch.float32], rhs)
+    cos = cosine(grads[torch.float32], rhs)
+    print(f"\n{nx}x{ny}: rel L2 adjoint vs DARTEL rhs {rel:.3e}, cosine {cos:.6f}")
+    assert cos > 0.9
+
+
+def test_gradient_descends_the_energy(cuda):
+    """A small step against the adjoint gradient lowers E by about |g|^2 step."""
+
+    from cat_surface_gpu.dartel_adjoint import exp_warp_ssd
+
+    v, f, t = _problem(128, 64, cuda)
+    w = v.clone().requires_grad_(True)
+    e0 = exp_warp_ssd(w, f, t)
+    e0.backward()
+    g = w.grad
+    step = 1e-3 / float(g.abs().max())
+    e1 = float(exp_warp_ssd(v - step * g, f, t))
+    predicted = step * float((g * g).sum())
+    assert abs((float(e0) - e1) - predicted) <= 0.05 * predicted
