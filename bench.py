@@ -63,7 +63,7 @@ def run_step(pairs, dtype: str, mode: str, device, cuda_graph: bool):
     kw = dict(device=device, dartel_dtype=dtype, steps=2, runs=2, avg=True, write_output=False,
               cuda_graph=cuda_graph)
     if mode == "batched+adjoint":
-        kw.update(rotation_method="adjoint")
+        kw.update(rotation_method="adjoint", autotune=True)
     if mode in ("batched", "batched+adjoint"):
         return run_cat_surface_gpu_pipelines([PipelineJob(meshes=m) for m in pairs], **kw)
     return [run_cat_surface_gpu_pipeline(None, None, None, None, "unused.gii", meshes=m, **kw)
@@ -131,6 +131,138 @@ def cpu_baseline() -> dict:
                       f"({sum(per_k):.2f} s), x18 for 3 solves x 2 steps x 3 its"}
 
 
+def _pair_meshes(p):
+    from cat_surface_gpu.gpu_pipeline import GiftiMesh
+
+    return (GiftiMesh(p.src_surface, p.src_faces), GiftiMesh(p.src_sphere, p.src_faces),
+            GiftiMesh(p.tgt_surface, p.tgt_faces), GiftiMesh(p.tgt_sphere, p.tgt_faces))
+
+
+def uneven_report(seed: int, device, jobs_count: int = 32) -> dict:
+    """Uneven multi-job schedule: jobs_count full registrations (steps=2 runs=2
+    avg) with meshes of 10,242 / 40,962 / 163,842 (icosphere) or 61,442
+    (Fibonacci hull) vertices and 128x64 / 256x128 / 512x256 maps, in one
+    batched+adjoint schedule with tuned relaxation tiles.  Reports throughput,
+    the per-job latency of each job run alone, and each job's largest vertex
+    difference from the float64 plain path."""
+
+    import numpy as np
+    import torch
+
+    from cat_surface_gpu import PipelineJob, run_cat_surface_gpu_pipelines, synthetic
+
+    rng = np.random.default_rng(seed)
+    meshes = (("icosphere", 10242, 5), ("icosphere", 40962, 6), ("fibonacci", 61442, 6),
+              ("icosphere", 163842, 7))
+    maps = ((128, 64), (256, 128), (512, 256))
+    jobs, shapes = [], []
+    t = time.perf_counter()
+    for k in range(jobs_count):
+        kind, n, sub = meshes[int(rng.integers(len(meshes)))]
+        size = maps[int(rng.integers(len(maps)))]
+        p = synthetic.make_pair(kind, n, subdivisions=sub, lmax=16, noise=0.05,
+                                max_rotation_deg=30.0, warp_deg=3.0, seed=seed + 1000 + k)
+        jobs.append(PipelineJob(meshes=_pair_meshes(p), map_size=size))
+        shapes.append((n, size))
+    log(f"{jobs_count} uneven jobs built in {time.perf_counter() - t:.1f} s")
+    kw = dict(device=device, dartel_dtype="float32", steps=2, runs=2, avg=True,
+              write_output=False, cuda_graph=True, rotation_method="adjoint")
+    run_cat_surface_gpu_pipelines(jobs, autotune=True, **kw)  # capture graphs, tune tiles
+    torch.cuda.synchronize()
+    log("uneven warmup done")
+    t0 = time.perf_counter()
+    batched = run_cat_surface_gpu_pipelines(jobs, autotune=True, **kw)
+    torch.cuda.synchronize()
+    makespan = time.perf_counter() - t0
+    log("uneven batched run done")
+    alone, errors = [], []
+    for job, res in zip(jobs, batched):
+        run_cat_surface_gpu_pipelines([job], **kw)  # warm this job's graph shapes
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        run_cat_surface_gpu_pipelines([job], **kw)
+        torch.cuda.synchronize()
+        alone.append(time.perf_counter() - t1)
+        ref = run_cat_surface_gpu_pipelines([job], **dict(kw, dartel_dtype="float64"))[0]
+        errors.append(float(np.abs(res.vertices.astype(np.float64) - ref.vertices).max()))
+        log(f"uneven job {len(alone)}/{len(jobs)}: alone {alone[-1]:.3f} s, "
+            f"max vertex delta vs float64 {errors[-1]:.2e}")
+    lat = np.asarray(alone)
+    return {
+        "metric": "uneven multi-job CAT_SurfWarp throughput",
+        "value": round(jobs_count / makespan, 3), "unit": "jobs/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f32",
+        "config": {"workload": f"{jobs_count} jobs, meshes 10,242-163,842 vertices, maps "
+                               "128x64-512x256, steps=2 runs=2 avg, batched+adjoint, tuned "
+                               "tiles", "jobs": [{"vertices": n, "map": f"{a}x{b}"}
+                                                 for n, (a, b) in shapes]},
+        "makespan_s": round(makespan, 4),
+        "alone_latency_s": {"p50": round(float(np.percentile(lat, 50)), 4),
+                            "p99": round(float(np.percentile(lat, 99)), 4),
+                            "sum": round(float(lat.sum()), 4)},
+        "batched_speedup_vs_alone_sum": round(float(lat.sum()) / makespan, 3),
+        "max_vertex_delta_vs_float64_plain": {"max": max(errors),
+                                              "p50": float(np.median(errors))},
+    }
+
+
+def hopf_report(seed: int, device) -> dict:
+    """Rotation search on the 61,442-vertex LH/RH pair: the reference's 46 seeds
+    + Nelder-Mead against 8,192 Hopf-grid candidates scored in one launch and
+    the best 8 refined by 20 gradient steps each."""
+
+    import torch
+
+    from cat_surface_gpu import PipelineJob, run_cat_surface_gpu_pipelines
+
+    pairs = make_inputs(seed)
+    jobs = [PipelineJob(meshes=m) for m in pairs]
+    kw = dict(device=device, dartel_dtype="float32", steps=2, runs=2, avg=True,
+              write_output=False, cuda_graph=True)
+    out = {}
+    for name, extra in (("seeds+nelder-mead", {}),
+                        ("hopf8192+adjoint", dict(rotation_method="adjoint",
+                                                  rotation_candidates="hopf"))):
+        run_cat_surface_gpu_pipelines(jobs, **kw, **extra)
+        torch.cuda.synchronize()
+        res = run_cat_surface_gpu_pipelines(jobs, **kw, **extra)
+        out[name] = {"rotation_search_s": round(res[0].timings["rotation_search_seconds"], 4),
+                     "costs": [r.rotation_cost for r in res],
+                     "angles": [[round(float(a), 6) for a in r.rotation_angle] for r in res]}
+    nm, hopf = out["seeds+nelder-mead"]["costs"], out["hopf8192+adjoint"]["costs"]
+    return {"metric": "rotation search, 61,442-vertex LH+RH", "dtype": "f32 locate, f64 SSD",
+            "n_gpus": 1, "results": out,
+            "cost_ratio_hopf_over_nm": [h / c for h, c in zip(hopf, nm)],
+            "within_1e-6_or_lower": all(h <= c * (1 + 1e-6) for h, c in zip(hopf, nm))}
+
+
+def adjoint_report(device) -> dict:
+    """The exp-warp-SSD adjoint: recompute vs store time and memory, float32
+    against float64, and against the reference DARTEL code-0 rhs."""
+
+    import torch
+
+    from cat_surface_gpu.dartel_adjoint import (cosine, dartel_rhs, exp_warp_ssd,
+                                                recompute_vs_store, relative_l2,
+                                                synthetic_problem)
+
+    rows = []
+    for nx, ny in ((256, 128), (512, 256)):
+        row = recompute_vs_store(nx, ny, device=device)
+        v, f, t = synthetic_problem(nx, ny, device=device)
+        grads = {}
+        for dtype in (torch.float64, torch.float32):
+            w = v.to(dtype).clone().requires_grad_(True)
+            exp_warp_ssd(w, f.to(dtype), t.to(dtype)).backward()
+            grads[dtype] = w.grad
+        rhs = dartel_rhs(f, t, v)
+        row["rel_l2_f32_vs_f64"] = relative_l2(grads[torch.float32], grads[torch.float64])
+        row["rel_l2_vs_dartel_rhs"] = relative_l2(grads[torch.float32], rhs)
+        row["cosine_vs_dartel_rhs"] = cosine(grads[torch.float32], rhs)
+        rows.append(row)
+    return {"metric": "exp-warp-SSD adjoint (K=6, membrane 0.01)", "n_gpus": 1, "rows": rows}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +276,11 @@ def main() -> int:
     ap.add_argument("--compare-plain", action=argparse.BooleanOptionalAction, default=True,
                     help="after the timed mode, time the plain path on the same inputs and "
                          "report it with the largest vertex difference between the two")
+    ap.add_argument("--workload", choices=("hemisphere-pair", "uneven-32", "hopf-8192",
+                                           "adjoint"), default="hemisphere-pair",
+                    help="hemisphere-pair is the headline metric; the others print reports "
+                         "of the uneven multi-job schedule, the Hopf-grid rotation search and "
+                         "the exp-warp-SSD adjoint")
     ap.add_argument("--cuda-graph", action=argparse.BooleanOptionalAction, default=True,
                     help="replay the DARTEL solves from captured graphs (the kernels inside a "
                          "replay are not seen by the kernel timers)")
@@ -156,6 +293,12 @@ def main() -> int:
 
     device = torch.device("cuda", int(os.environ.get("LOCAL_RANK", "0")))
     torch.cuda.set_device(device)
+    if args.workload != "hemisphere-pair":
+        report = {"uneven-32": lambda: uneven_report(args.seed, device),
+                  "hopf-8192": lambda: hopf_report(args.seed, device),
+                  "adjoint": lambda: adjoint_report(device)}[args.workload]()
+        print(json.dumps(report), flush=True)
+        return 0
     t = time.perf_counter()
     pairs = make_inputs(args.seed)
     log(f"inputs ready in {time.perf_counter() - t:.1f} s")

@@ -153,6 +153,7 @@ class PipelineJob:
     source_stencil: str | Path | None = None
     target_stencil: str | Path | None = None
     rotation_values: str | Path | None = None
+    map_size: tuple[int, int] | None = None  # (nx, ny) of the DARTEL maps; None: 512 x 256
 
 
 def run_cat_surface_gpu_pipeline(
@@ -262,7 +263,7 @@ def run_cat_surface_gpu_pipelines(
     optimized_dartel: bool = True, cuda_graph: bool = True,
     rotation_search=None, write_output: bool = True,
     rotation_method: str = "nelder-mead", rotation_candidates: str = "seeds",
-    rotation_top: int = 8, rotation_steps: int = 20,
+    rotation_top: int = 8, rotation_steps: int = 20, autotune: bool = False,
 ) -> list[CatSurfaceGpuResult]:
     """Several independent CAT_SurfWarp registrations, stage by stage.
 
@@ -347,8 +348,9 @@ def run_cat_surface_gpu_pipelines(
                                          tmp / f"job{index}-{label}.stencil", opts, dev)
             surface = st.src_surface.vertices if source_side else st.tgt_surface.vertices
             faces_dev = st.src_faces_dev if source_side else st.tgt_faces_dev
+            nx, ny = st.job.map_size or (512, 256)
             return build_device_stencil(sphere_vertices, faces_dev, surface_vertices=surface,
-                                        normalise=normalise, device=dev)
+                                        normalise=normalise, nx=int(nx), ny=int(ny), device=dev)
 
         t = time.perf_counter()
         for i, st in enumerate(states):
@@ -460,6 +462,19 @@ def run_cat_surface_gpu_pipelines(
 
         solve_kw = dict(n_steps=steps, loop=loop, cycles=cycles, nit=nit, its=its, code=code,
                         device=dev, dtype=solve_dtype, cuda_graph=graph_requested)
+        if autotune:
+            # relaxation tiles per map size, tuned (or read from the cache)
+            # before any solve is captured; the batch is the largest solve's
+            from .autotune import ensure_tuned
+
+            t = time.perf_counter()
+            sizes: dict[tuple[int, int], int] = {}
+            for st in states:
+                shape = (int(st.cur_stencil.nx), int(st.cur_stencil.ny))
+                sizes[shape] = sizes.get(shape, 0) + steps * (2 if avg else 1)
+            for shape, batch in sizes.items():
+                ensure_tuned([shape], solve_dtype, batch, device=dev)
+            shared["autotune_seconds"] = time.perf_counter() - t
         t_dartel = time.perf_counter()
         for run in range(runs):
             t = time.perf_counter()

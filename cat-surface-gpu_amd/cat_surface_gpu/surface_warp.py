@@ -268,6 +268,18 @@ def type5_sheet_maps(vertices, stencil, fwhm: Sequence[float], dtype: torch.dtyp
                                                         out_dtype=dtype))
 
 
+_SIDE_STREAMS: dict[tuple, torch.cuda.Stream] = {}
+
+
+def _side_stream(device: torch.device, index: int) -> torch.cuda.Stream:
+    """A reusable default-priority stream per (device, index)."""
+
+    key = (device.type, device.index, index)
+    if key not in _SIDE_STREAMS:
+        _SIDE_STREAMS[key] = torch.cuda.Stream(device=device)
+    return _SIDE_STREAMS[key]
+
+
 def solve_dartel_from_surfaces_jobs(requests, *, n_steps: int = 2, loop: int = 6,
                                     lmreg: float = 1e-3, cycles: int = 3, nit: int = 3,
                                     its: int = 3, code: int = 1,
@@ -305,12 +317,40 @@ def solve_dartel_from_surfaces_jobs(requests, *, n_steps: int = 2, loop: int = 6
                                 dtype)
         for i, r in zip(members, res):
             sheets[i] = r
-    src = torch.cat([sheets[2 * k] for k in range(len(requests))])
-    tgt = torch.cat([sheets[2 * k + 1] for k in range(len(requests))])
-    return solve_dartel_maps_jobs(src, tgt, n_steps, loop=loop, lmreg=lmreg,
-                                  cycles=cycles, nit=nit, its=its, code=code, device=dev,
-                                  dtype=dtype, assume_initial_nonzero=assume_initial_nonzero,
-                                  cuda_graph=cuda_graph)
+    # one batched solve per map shape; shapes run on streams of their own so
+    # that small maps overlap large ones
+    groups: dict[tuple, list[int]] = {}
+    for k in range(len(requests)):
+        groups.setdefault(tuple(sheets[2 * k].shape[1:]), []).append(k)
+    results: list[DartelSolveResult | None] = [None] * len(requests)
+    kw = dict(loop=loop, lmreg=lmreg, cycles=cycles, nit=nit, its=its, code=code, device=dev,
+              dtype=dtype, assume_initial_nonzero=assume_initial_nonzero, cuda_graph=cuda_graph)
+    if len(groups) == 1:
+        members = next(iter(groups.values()))
+        src = torch.cat([sheets[2 * k] for k in members])
+        tgt = torch.cat([sheets[2 * k + 1] for k in members])
+        return solve_dartel_maps_jobs(src, tgt, n_steps, **kw)
+    current = torch.cuda.current_stream(dev)
+    streams = []
+    for members in groups.values():
+        src = torch.cat([sheets[2 * k] for k in members])
+        tgt = torch.cat([sheets[2 * k + 1] for k in members])
+        side = _side_stream(dev, len(streams))
+        side.wait_stream(current)
+        with torch.cuda.stream(side):
+            out = solve_dartel_maps_jobs(src, tgt, n_steps, **kw)
+        for k, r in zip(members, out):
+            # the results are read on the caller's stream: keep their memory
+            # from being reused on the side stream before that
+            for tensor in (r.flow, r.source_maps, r.target_maps, r.metrics):
+                tensor.record_stream(current)
+            results[k] = r
+        src.record_stream(side)
+        tgt.record_stream(side)
+        streams.append(side)
+    for side in streams:
+        current.wait_stream(side)
+    return results
 
 
 def solve_dartel_from_surfaces(source_vertices, target_vertices, source_stencil, target_stencil, *,

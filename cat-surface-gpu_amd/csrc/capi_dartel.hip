@@ -88,6 +88,30 @@ int csg_set_fmg_schedule(int fused)
     return set_fused_fmg_schedule(fused);
 }
 
+int csg_set_relax_tile(int nx, int ny, int dtype, int tile_x, int tile_y)
+{
+    if (dtype != 0 && dtype != 1)
+        return fail(kInvalidArgument, "dtype must be 0 or 1");
+    return set_relax_tile_shape(nx, ny, dtype == 1 ? 8 : 4, tile_x, tile_y);
+}
+
+int csg_fmg_tiled_levels(int nx, int ny, int nit, int dtype)
+{
+    if (nx < 1 || ny < 1 || nit < 0 || (dtype != 0 && dtype != 1))
+        return fail(kInvalidArgument, "invalid FMG shape");
+    return fused_tiled_levels(nx, ny, nit, dtype == 1 ? 8 : 4);
+}
+
+int csg_get_relax_tile(int nx, int ny, int dtype, int *tile_x, int *tile_y)
+{
+    if ((dtype != 0 && dtype != 1) || tile_x == nullptr || tile_y == nullptr)
+        return fail(kInvalidArgument, "dtype must be 0 or 1 and both outputs given");
+    TileShape t = relax_tile_shape(nx, ny, dtype == 1 ? 8 : 4);
+    *tile_x = t.x;
+    *tile_y = t.y;
+    return kOk;
+}
+
 int csg_kernel_timing(int enable)
 {
     return timing_enable(enable);

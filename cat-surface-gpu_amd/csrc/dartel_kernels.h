@@ -95,8 +95,20 @@ int launch_any_nonzero(const T *u, size_t n, int batch, size_t stride, int *flag
 // ---------------------------------------------------------------- fused FMG schedule
 
 constexpr int kFmgMaxLevels = 40;
-constexpr int kTileX = 32, kTileY = 16;  // interior of a fine-level relaxation tile
 constexpr int kTileThreads = 512;
+
+// Interior of the fine-level relaxation tiles.  Any shape with an even width
+// gives the same values (the halo makes every tile's interior exact); the
+// shape only changes the speed, so it is a tuning knob: a process-wide table
+// maps (level width, level height, element size) to a shape, default 32 x 16.
+struct TileShape {
+    int x, y;
+};
+constexpr TileShape kTileDefault{32, 16};
+TileShape relax_tile_shape(int nx, int ny, int elem);
+// tx = ty = 0 removes the entry.  Fails unless tx is even, ty >= 1 and a tile
+// with the default nit = 3 halo fits the LDS budget for that element size.
+int set_relax_tile_shape(int nx, int ny, int elem, int tx, int ty);
 constexpr int kCoarseThreads = 1024;
 
 // Levels c0 .. L-1 of the pyramid, relaxed by one workgroup per job in LDS.
@@ -111,7 +123,7 @@ struct CoarseLevels {
 // kernels' limits are raised to it once per process).
 size_t fmg_lds_budget();
 template <typename T>
-size_t tile_lds_bytes(int sweeps);
+size_t tile_lds_bytes(int sweeps, TileShape tile);
 template <typename T>
 size_t coarse_lds_bytes(const int *nx, const int *ny, int c0, int L);
 

@@ -272,15 +272,33 @@ int fused_first_coarse_level(const FmgPlan<T> &p, int nit)
         }
     if (c0 < 0)
         return -1;
-    for (int l = 0; l < c0; ++l)
+    for (int l = 0; l < c0; ++l) {
         if ((p.nx[l] & 1) || p.nx[l] < 2)
             return -1;
-    if (c0 > 0 && tile_lds_bytes<T>(2 * nit) > budget)
-        return -1;
+        if (tile_lds_bytes<T>(2 * nit, relax_tile_shape(p.nx[l], p.ny[l], (int)sizeof(T))) >
+            budget)
+            return -1;
+    }
     return c0;
 }
 
 }  // namespace
+
+int fused_tiled_levels(int nx, int ny, int nit, int elem)
+{
+    FmgPlan<float> pf{};
+    FmgPlan<double> pd{};
+    int L = fmg_levels(nx, ny, pf.nx, pf.ny);
+    pf.L = L;
+    for (int l = 0; l < L; ++l) {
+        pd.nx[l] = pf.nx[l];
+        pd.ny[l] = pf.ny[l];
+    }
+    pd.L = L;
+    const int c0 = elem == 8 ? fused_first_coarse_level<double>(pd, nit)
+                             : fused_first_coarse_level<float>(pf, nit);
+    return c0 < 0 ? 0 : c0;
+}
 
 int set_fused_fmg_schedule(int enable)
 {

@@ -19,6 +19,7 @@
 #include "fmg_device.h"
 #include "kernel_timing.h"
 
+#include <map>
 #include <mutex>
 
 namespace csg {
@@ -28,10 +29,10 @@ namespace csg {
 namespace {
 
 // One relax_membrane call (`sweeps` = 2·nit half-sweeps, first colour
-// (x + y) & 1 == 1) on a kTileX x kTileY interior.  The tile holds columns
-// x0 - H .. x0 + kTileX + H - 1 in unwrapped coordinates (nx is even, so a
+// (x + y) & 1 == 1) on a TX x TY interior (TX even).  The tile holds columns
+// x0 - H .. x0 + TX + H - 1 in unwrapped coordinates (nx is even, so a
 // pixel's colour is the same unwrapped or wrapped) and the rows of
-// y0 - H .. y0 + kTileY + H - 1 that exist (the y boundary reflects onto rows
+// y0 - H .. y0 + TY + H - 1 that exist (the y boundary reflects onto rows
 // inside the grid).  Pixels whose neighbours lie outside the tile are not
 // updated; after s sweeps every pixel at distance >= s from the tile edge
 // holds the value of the global sweep, so the interior is exact after H.
@@ -41,8 +42,8 @@ template <typename T, int HC>
 __global__ void __launch_bounds__(kTileThreads)
 k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restrict__ Uin,
              T *__restrict__ Uout, const T *__restrict__ Uc, int cnx, int cny, int nx, int ny,
-             T w00, T w01, T w10, int H_arg, int tiles_x, size_t astride, size_t bstride,
-             size_t cstride)
+             T w00, T w01, T w10, int H_arg, int TX, int TY, int tiles_x, size_t astride,
+             size_t bstride, size_t cstride)
 {
     const int H = HC > 0 ? HC : H_arg;
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -56,9 +57,9 @@ k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restri
     if (Uc)
         Uc += job * cstride;
     const int tyi = blockIdx.x / tiles_x, txi = blockIdx.x - tyi * tiles_x;
-    const int ex = kTileX + 2 * H;
-    const int x0 = txi * kTileX - H;
-    const int ylo = max(0, tyi * kTileY - H), yhi = min(ny, tyi * kTileY + kTileY + H);
+    const int ex = TX + 2 * H;
+    const int x0 = txi * TX - H;
+    const int ylo = max(0, tyi * TY - H), yhi = min(ny, tyi * TY + TY + H);
     const int npx = ex * (yhi - ylo);
     T *u0 = lds, *u1 = lds + npx, *a0 = lds + 2 * npx, *a1 = lds + 3 * npx, *a2 = lds + 4 * npx,
       *b0 = lds + 5 * npx, *b1 = lds + 6 * npx;
@@ -106,9 +107,9 @@ k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restri
         }
         __syncthreads();
     }
-    for (int q = threadIdx.x; q < kTileX * kTileY; q += blockDim.x) {
-        const int ly = q / kTileX, lx = q - ly * kTileX;
-        const int gx = txi * kTileX + lx, gy = tyi * kTileY + ly;
+    for (int q = threadIdx.x; q < TX * TY; q += blockDim.x) {
+        const int ly = q / TX, lx = q - ly * TX;
+        const int gx = txi * TX + lx, gy = tyi * TY + ly;
         if (gx >= nx || gy >= ny)
             continue;
         const int t = (gy - ylo) * ex + H + lx;
@@ -485,9 +486,52 @@ size_t fmg_lds_budget()
 }
 
 template <typename T>
-size_t tile_lds_bytes(int sweeps)
+size_t tile_lds_bytes(int sweeps, TileShape tile)
 {
-    return (size_t)(kTileX + 2 * sweeps) * (kTileY + 2 * sweeps) * 7 * sizeof(T);
+    return (size_t)(tile.x + 2 * sweeps) * (tile.y + 2 * sweeps) * 7 * sizeof(T);
+}
+
+namespace {
+
+struct TileKey {
+    int nx, ny, elem;
+    bool operator<(const TileKey &o) const
+    {
+        return nx != o.nx ? nx < o.nx : (ny != o.ny ? ny < o.ny : elem < o.elem);
+    }
+};
+std::mutex g_tile_mutex;
+std::map<TileKey, TileShape> g_tile_table;
+
+}  // namespace
+
+TileShape relax_tile_shape(int nx, int ny, int elem)
+{
+    std::lock_guard<std::mutex> lock(g_tile_mutex);
+    auto it = g_tile_table.find(TileKey{nx, ny, elem});
+    return it == g_tile_table.end() ? kTileDefault : it->second;
+}
+
+int set_relax_tile_shape(int nx, int ny, int elem, int tx, int ty)
+{
+    if (nx < 1 || ny < 1 || (elem != 4 && elem != 8))
+        return fail(kInvalidArgument, "relaxation tile shapes are set per level size and "
+                                      "element size 4 or 8");
+    if (tx == 0 && ty == 0) {
+        std::lock_guard<std::mutex> lock(g_tile_mutex);
+        g_tile_table.erase(TileKey{nx, ny, elem});
+        return kOk;
+    }
+    if (tx < 2 || (tx & 1) || ty < 1)
+        return fail(kInvalidArgument, "tile width must be even and >= 2, height >= 1 (got %d x %d)",
+                    tx, ty);
+    const size_t need = (size_t)(tx + 12) * (ty + 12) * 7 * elem;
+    if (need > fmg_lds_budget())
+        return fail(kInvalidArgument, "a %d x %d tile needs %zu bytes of LDS, over the budget",
+                    tx, ty, need);
+    std::lock_guard<std::mutex> lock(g_tile_mutex);
+    g_tile_table[TileKey{nx, ny, elem}] = TileShape{tx, ty};
+    return kOk;
 }
 
 template <typename T>
@@ -543,10 +587,11 @@ int launch_relax_tile(const T *a, const T *b, const T *uin, T *uout, const T *uc
 {
     if ((nx & 1) || nx < 2 || sweeps < 0)
         return fail(kInvalidArgument, "tiled relaxation needs an even width, got %d", nx);
-    size_t lds = tile_lds_bytes<T>(sweeps);
+    const TileShape tile = relax_tile_shape(nx, ny, (int)sizeof(T));
+    size_t lds = tile_lds_bytes<T>(sweeps, tile);
     if (lds > fmg_lds_budget())
         return fail(kInvalidArgument, "tiled relaxation needs %zu bytes of LDS", lds);
-    const int tiles_x = (nx + kTileX - 1) / kTileX, tiles_y = (ny + kTileY - 1) / kTileY;
+    const int tiles_x = (nx + tile.x - 1) / tile.x, tiles_y = (ny + tile.y - 1) / tile.y;
     dim3 grid(tiles_x * tiles_y, batch);
     // reference bytes: 9 values per pixel per red+black iteration, plus the
     // prolongation u += resize(u_c) (reads u_c and u, writes u)
@@ -555,12 +600,12 @@ int launch_relax_tile(const T *a, const T *b, const T *uin, T *uout, const T *uc
                       (9.0 * (sweeps / 2.0) + (uc ? 4.5 : 0.0)) * P * sizeof(T) * batch);
     if (sweeps == 6)  // nit = 3, the reference default
         hipLaunchKernelGGL((k_relax_tile<T, 6>), grid, dim3(kTileThreads), lds, s, a, b, uin, uout,
-                           uc, cnx, cny, nx, ny, (T)w.w00, (T)w.w01, (T)w.w10, sweeps, tiles_x,
-                           astride, bstride, cstride);
+                           uc, cnx, cny, nx, ny, (T)w.w00, (T)w.w01, (T)w.w10, sweeps, tile.x,
+                           tile.y, tiles_x, astride, bstride, cstride);
     else
         hipLaunchKernelGGL((k_relax_tile<T, 0>), grid, dim3(kTileThreads), lds, s, a, b, uin, uout,
-                           uc, cnx, cny, nx, ny, (T)w.w00, (T)w.w01, (T)w.w10, sweeps, tiles_x,
-                           astride, bstride, cstride);
+                           uc, cnx, cny, nx, ny, (T)w.w00, (T)w.w01, (T)w.w10, sweeps, tile.x,
+                           tile.y, tiles_x, astride, bstride, cstride);
     CSG_LAUNCH_CHECK();
     timer.done();
     return kOk;
@@ -599,7 +644,7 @@ int launch_fmg_coarse(const CoarseLevels<T> &lv, const T *a, const T *b, T *u, s
 }
 
 #define CSG_INSTANTIATE_FUSED(T)                                                                 \
-    template size_t tile_lds_bytes<T>(int);                                                      \
+    template size_t tile_lds_bytes<T>(int, TileShape);                                                      \
     template size_t coarse_lds_bytes<T>(const int *, const int *, int, int);                     \
     template int launch_relax_tile<T>(const T *, const T *, const T *, T *, const T *, int, int, \
                                       MembraneWeights, int, int, int, size_t, size_t, size_t,    \

@@ -41,6 +41,17 @@ int csg_device_count(int *count);
  * in the same order.  Process-wide; returns the previous setting.  No counterpart in
  * the reference (it has only the per-operator schedule of dartel_grid.py:1211-1417). */
 int csg_set_fmg_schedule(int fused);
+/* Tile interior (tile_x even, tile_y) of the fused schedule's fine-level relaxation for
+ * levels of nx x ny in dtype; 0 x 0 restores the default 32 x 16.  Every shape computes
+ * the same values; the shape only changes the speed (set by an autotuner).  Process-wide;
+ * launches made (or graphs captured) afterwards use it.  No counterpart in the
+ * reference. */
+int csg_set_relax_tile(int nx, int ny, int dtype, int tile_x, int tile_y);
+int csg_get_relax_tile(int nx, int ny, int dtype, int *tile_x, int *tile_y);
+/* Number of levels of an nx x ny FMG (nit relaxation iterations) that the fused schedule
+ * relaxes on tiles, i.e. the levels whose tile shape matters; 0 if the fused schedule
+ * does not apply; negative on invalid arguments. */
+int csg_fmg_tiled_levels(int nx, int ny, int nit, int dtype);
 /* Optional HIP-event timing of the hot kernels for roofline reports (no counterpart in the
  * reference).  csg_kernel_timing(1) brackets every later launch of a timed kernel made
  * outside stream capture with two events on its stream and accumulates the launch's

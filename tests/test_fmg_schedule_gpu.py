@@ -92,3 +92,61 @@ def test_fused_schedule_solve_is_bit_identical(csg, cuda, dtype):
     ref, fused = _both(csg, lambda: csg.solve_dartel_maps(s, t, loop=3, device=cuda,
                                                           dtype=dtype).flow)
     assert torch.equal(ref, fused)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("tile", [(64, 8), (16, 16), (128, 4), (2, 1), (34, 5)])
+def test_relaxation_tile_shape_does_not_change_values(csg, cuda, dtype, tile):
+    """Every fine level relaxed on tiles of another shape gives the same bits."""
+
+    from cat_surface_gpu.autotune import fine_levels, set_tile
+
+    nx, ny = 512, 256
+    rng = np.random.default_rng(77)
+    a, b, u0 = _system(nx, ny, rng, cuda, dtype)
+    params = (1.0, 1.0, 0.125, 0.0, 1e-3)
+    ref = csg.fmg2_membrane(a, b, params, device=cuda, dtype=dtype)
+    levels = fine_levels(nx, ny, dtype)
+    assert levels and levels[0] == (nx, ny)
+    try:
+        for lx, ly in levels:
+            set_tile(lx, ly, dtype, tile)
+        other = csg.fmg2_membrane(a, b, params, device=cuda, dtype=dtype)
+        with_initial = csg.fmg2_membrane(a, b, params, initial=u0, device=cuda, dtype=dtype)
+    finally:
+        for lx, ly in levels:
+            set_tile(lx, ly, dtype, None)
+    assert torch.equal(ref, other)
+    assert torch.equal(with_initial, csg.fmg2_membrane(a, b, params, initial=u0, device=cuda,
+                                                       dtype=dtype))
+
+
+def test_relaxation_tile_rejects_bad_shapes(csg, cuda):
+    from cat_surface_gpu.autotune import set_tile
+
+    for bad in ((3, 8), (0, 4), (4, 0), (512, 512)):
+        with pytest.raises(ValueError):
+            set_tile(512, 256, torch.float32, bad)
+
+
+def test_autotuner_installs_valid_shapes(csg, cuda, tmp_path, monkeypatch):
+    from cat_surface_gpu import autotune
+
+    monkeypatch.setenv("CSG_AUTOTUNE_CACHE", str(tmp_path / "tiles.json"))
+    autotune._TUNED.clear()
+    try:
+        chosen = autotune.ensure_tuned([(128, 64)], torch.float32, 4, device=cuda)
+        levels = chosen[(128, 64)]
+        assert set(levels) == set(autotune.fine_levels(128, 64, torch.float32))
+        for (lx, ly), shape in levels.items():
+            assert autotune.current_tile(lx, ly, torch.float32) == tuple(shape)
+        assert (tmp_path / "tiles.json").is_file()
+        # a second process-level call reads the cache instead of tuning
+        autotune._TUNED.clear()
+        again = autotune.ensure_tuned([(128, 64)], torch.float32, 4, device=cuda)
+        assert {k: tuple(v) for k, v in again[(128, 64)].items()} == \
+            {k: tuple(v) for k, v in levels.items()}
+    finally:
+        for lx, ly in autotune.fine_levels(128, 64, torch.float32):
+            autotune.set_tile(lx, ly, torch.float32, None)
+        autotune._TUNED.clear()

@@ -95,3 +95,28 @@ def test_adjoint_rotation_pipeline_matches_nelder_mead(pair_paths, cuda):
         assert adj.rotation_cost <= nm.rotation_cost * (1.0 + 1e-6)
         np.testing.assert_array_equal(adj.faces, nm.faces)
         assert np.abs(adj.vertices - nm.vertices).max() <= 1e-3
+
+
+def test_uneven_jobs_equal_single_runs(cuda, tmp_path):
+    """Jobs of different mesh and map sizes in one schedule (one batched solve per
+    map size, on streams of their own, with tuned tiles) give the single-job results."""
+
+    from cat_surface_gpu import (GiftiMesh, PipelineJob, run_cat_surface_gpu_pipeline,
+                                 run_cat_surface_gpu_pipelines, synthetic)
+
+    jobs = []
+    for k, (sub, size) in enumerate(((3, (128, 64)), (4, (256, 128)), (3, (256, 128)))):
+        p = synthetic.make_pair("icosphere", subdivisions=sub, lmax=6, noise=0.0,
+                                max_rotation_deg=15.0, seed=300 + k)
+        jobs.append(PipelineJob(meshes=(GiftiMesh(p.src_surface, p.src_faces),
+                                        GiftiMesh(p.src_sphere, p.src_faces),
+                                        GiftiMesh(p.tgt_surface, p.tgt_faces),
+                                        GiftiMesh(p.tgt_sphere, p.tgt_faces)), map_size=size))
+    kw = dict(device=cuda, dartel_dtype="float32", steps=2, runs=2, avg=True,
+              write_output=False)
+    together = run_cat_surface_gpu_pipelines(jobs, autotune=True, **kw)
+    for job, res in zip(jobs, together):
+        alone = run_cat_surface_gpu_pipelines([job], **kw)[0]
+        np.testing.assert_array_equal(res.vertices, alone.vertices)
+        np.testing.assert_array_equal(res.faces, alone.faces)
+        assert np.all(np.isfinite(res.vertices))
