@@ -164,6 +164,7 @@ def uneven_report(seed: int, device, jobs_count: int = 32) -> dict:
                                 max_rotation_deg=30.0, warp_deg=3.0, seed=seed + 1000 + k)
         jobs.append(PipelineJob(meshes=_pair_meshes(p), map_size=size))
         shapes.append((n, size))
+        log(f"uneven job {k + 1}/{jobs_count} built: {n} vertices, {size[0]}x{size[1]} map")
     log(f"{jobs_count} uneven jobs built in {time.perf_counter() - t:.1f} s")
     kw = dict(device=device, dartel_dtype="float32", steps=2, runs=2, avg=True,
               write_output=False, cuda_graph=True, rotation_method="adjoint")

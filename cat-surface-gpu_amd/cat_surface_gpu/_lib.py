@@ -103,6 +103,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_rotation_cost_matrices_workspace_bytes": (_sz, [_i, _i, _i]),
     "csg_rotation_cost_matrices": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _i,
                                         _vp, _vp, _sz, _vp, _vp]),
+    "csg_rotation_map_costs_workspace_bytes": (_sz, [_i, _i, _i, _i]),
+    "csg_rotation_map_costs": (_i, [_vp, _vp, _i, _i, _i, _vp, _i, _vp, _vp, _sz, _vp]),
     "csg_rotation_refine_workspace_bytes": (_sz, [_i, _i]),
     "csg_rotation_refine": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _i, _d,
                                  _d, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),

@@ -434,6 +434,11 @@ def run_cat_surface_gpu_pipelines(
             # every job's seed scan and Nelder-Mead in lockstep (same results)
             problems = [(rot.index, rot.prepare_inputs(pts, st.src_vals, st.tgt_vals))
                         for (rot, pts), st in zip(rots, states)]
+            if rotation_method == "adjoint" and rotation_candidates == "hopf":
+                # the grid is scored on the features' maps (the stencil sheets)
+                problems = [p + ((st.src_stencil.map_values_to_sheet(st.src_vals),
+                                  st.tgt_stencil.map_values_to_sheet(st.tgt_vals)),)
+                            for p, st in zip(problems, states)]
             results = search_rotations(problems, refine=rotation_refine,
                                        max_iter=rotation_max_iter, tol=rotation_tol,
                                        simplex_step=rotation_simplex_step,

@@ -672,20 +672,50 @@ def search_rotations(problems, *, refine: bool = True, max_iter: int = 500, tol:
     return out
 
 
+def map_costs(source_map: torch.Tensor, target_map: torch.Tensor, matrices: torch.Tensor, *,
+              stride: int = 4) -> torch.Tensor:
+    """Area-weighted SSD of candidate rotations on two [ny, nx] feature maps
+    (csg_rotation_map_costs): a coarse score without point location."""
+
+    src = source_map.to(torch.float64).contiguous()
+    tgt = target_map.to(torch.float64).contiguous()
+    if src.ndim != 2 or tuple(src.shape) != tuple(tgt.shape):
+        raise ValueError("feature maps must be two [ny, nx] arrays of one shape")
+    ny, nx = int(src.shape[0]), int(src.shape[1])
+    mats = matrices.to(device=src.device, dtype=torch.float32).reshape(-1, 9).contiguous()
+    na = int(mats.shape[0])
+    out = torch.empty(na, dtype=torch.float64, device=src.device)
+    lib = _lib.lib()
+    ws = _lib.workspace(lib.csg_rotation_map_costs_workspace_bytes(nx, ny, stride, na),
+                        src.device)
+    _lib.check(lib.csg_rotation_map_costs(_lib.ptr(src), _lib.ptr(tgt), nx, ny, int(stride),
+                                          _lib.ptr(mats), na, _lib.ptr(out), _lib.ptr(ws),
+                                          ws.numel(), _lib.stream_handle(src.device)),
+               "csg_rotation_map_costs")
+    return out
+
+
 def _search_rotations_adjoint(problems, *, refine: bool, candidates: str, top: int,
                               steps: int) -> list[RotationSearchResult]:
     """Candidate scan, top-k and gradient refinement, enqueued for every problem
-    and read back with one synchronisation."""
+    and read back with one synchronisation.  A problem given with feature maps
+    ((index, inputs, (source map, target map))) has the Hopf grid scored on the
+    maps; otherwise every candidate is costed by point location."""
 
     if top < 1:
         raise ValueError("top must be positive")
     staged = []
-    for index, inputs in problems:
+    for problem in problems:
+        index, inputs = problem[0], problem[1]
+        maps = problem[2] if len(problem) > 2 else None
         dev = index.device
         unc = torch.zeros(1, dtype=torch.int32, device=dev)
         if candidates == "seeds":
             pool = torch.as_tensor(official_seed_grid_angles(), dtype=torch.float64, device=dev)
             costs, _, _, _ = index._launch_costs(inputs, pool, uncovered_dev=unc)
+        elif maps is not None:
+            pool = _hopf_device(dev)
+            costs = map_costs(maps[0], maps[1], pool)
         else:
             pool = _hopf_device(dev)
             costs = index._launch_matrix_costs(inputs, pool, False, unc)
@@ -702,7 +732,8 @@ def _search_rotations_adjoint(problems, *, refine: bool, candidates: str, top: i
                                  unc.to(torch.float64), unc2.to(torch.float64)])
                       for _, order, angles, final, unc, unc2 in staged]).cpu().numpy()
     out, pos = [], 0
-    for (index, _), (costs, order, angles, final, _, _) in zip(problems, staged):
+    for problem, (costs, order, angles, final, _, _) in zip(problems, staged):
+        index = problem[0]
         k = int(final.numel())
         first = int(host[pos])
         ang = host[pos + 1:pos + 1 + 3 * k].reshape(k, 3)

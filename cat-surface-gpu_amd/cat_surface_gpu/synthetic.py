@@ -21,7 +21,6 @@ import math
 import numpy as np
 from scipy.spatial import ConvexHull
 from scipy.spatial.transform import Rotation
-from scipy.special import sph_harm_y
 
 from .stencil_builder import icosphere
 
@@ -59,21 +58,40 @@ def fibonacci_hull(n: int, jitter: float, seed: int) -> tuple[np.ndarray, np.nda
 
 
 def real_sh_basis(points: np.ndarray, lmax: int) -> np.ndarray:
-    """[n, (lmax+1)^2 - 1] real spherical harmonics of degree 1..lmax."""
+    """[n, (lmax+1)^2 - 1] orthonormal real spherical harmonics of degree 1..lmax,
+    columns l = 1..lmax, m = -l..l: sqrt(2) P_l^|m|(cos t) sin(|m| p) for m < 0,
+    P_l^0(cos t) for m = 0 and sqrt(2) P_l^m(cos t) cos(m p) for m > 0, with the
+    fully normalised associated Legendre functions (no Condon-Shortley phase)
+    from the standard three-term recurrences."""
 
     x, y, z = points[:, 0], points[:, 1], points[:, 2]
-    theta = np.arccos(np.clip(z, -1.0, 1.0))
+    ct = np.clip(z, -1.0, 1.0)
+    st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
     phi = np.arctan2(y, x)
+    # pbar[l][m] for 0 <= m <= l <= lmax
+    pbar = [[None] * (l + 1) for l in range(lmax + 1)]
+    pbar[0][0] = np.full_like(ct, math.sqrt(1.0 / (4.0 * math.pi)))
+    for m in range(1, lmax + 1):
+        pbar[m][m] = math.sqrt((2.0 * m + 1.0) / (2.0 * m)) * st * pbar[m - 1][m - 1]
+    for m in range(0, lmax):
+        pbar[m + 1][m] = math.sqrt(2.0 * m + 3.0) * ct * pbar[m][m]
+    for m in range(0, lmax + 1):
+        for l in range(m + 2, lmax + 1):
+            a = math.sqrt((4.0 * l * l - 1.0) / (l * l - m * m))
+            b = math.sqrt(((l - 1.0) ** 2 - m * m) / (4.0 * (l - 1.0) ** 2 - 1.0))
+            pbar[l][m] = a * (ct * pbar[l - 1][m] - b * pbar[l - 2][m])
+    cos_m = [np.cos(m * phi) for m in range(lmax + 1)]
+    sin_m = [np.sin(m * phi) for m in range(lmax + 1)]
     cols = []
+    root2 = math.sqrt(2.0)
     for l in range(1, lmax + 1):
         for m in range(-l, l + 1):
-            y_lm = sph_harm_y(l, abs(m), theta, phi)
             if m < 0:
-                cols.append(math.sqrt(2.0) * (-1) ** m * y_lm.imag)
+                cols.append(root2 * pbar[l][-m] * sin_m[-m])
             elif m == 0:
-                cols.append(y_lm.real)
+                cols.append(pbar[l][0])
             else:
-                cols.append(math.sqrt(2.0) * (-1) ** m * y_lm.real)
+                cols.append(root2 * pbar[l][m] * cos_m[m])
     return np.stack(cols, axis=1)
 
 

@@ -164,6 +164,26 @@ int csg_rotation_cost_matrices(const float *src_pts, const double *src_vals, int
     return launch_rotation_cost_mats(a, mats, with_gradient != 0, s);
 }
 
+size_t csg_rotation_map_costs_workspace_bytes(int nx, int ny, int stride, int na)
+{
+    if (nx < 1 || ny < 1 || stride < 1 || na < 1)
+        return 0;
+    return align256(sizeof(double) * (size_t)map_cost_blocks(nx, ny, stride) * na) + 256;
+}
+
+int csg_rotation_map_costs(const double *src_map, const double *tgt_map, int nx, int ny,
+                           int stride, const float *mats, int na, double *costs, void *ws,
+                           size_t ws_bytes, void *stream)
+{
+    if (nx < 2 || ny < 2 || stride < 1 || na < 1 || na > 65535 || src_map == nullptr ||
+        tgt_map == nullptr || mats == nullptr || costs == nullptr)
+        return fail(kInvalidArgument, "invalid map rotation cost arguments");
+    if (ws_bytes < csg_rotation_map_costs_workspace_bytes(nx, ny, stride, na))
+        return fail(kInvalidArgument, "map rotation cost workspace too small");
+    return launch_rotation_map_costs(src_map, tgt_map, nx, ny, stride, mats, na, (double *)ws,
+                                     costs, S(stream));
+}
+
 int csg_closest_triangles(const double *q, int nq, const double *verts, const int *faces,
                           const int *cell_off, const int *cell_faces, int grid, double reach,
                           int *face_out, int *idx_out, double *w_out, int *far_host,

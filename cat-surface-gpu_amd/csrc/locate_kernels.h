@@ -46,6 +46,11 @@ int launch_rotation_costs(const RotationCostArgs &a, hipStream_t s);
 constexpr int kRotationGradTerms = 10;
 // a.angles unused; a.costs receives [na] or [na][kRotationGradTerms];
 // a.partial needs rows x rotation_cost_blocks(ns) doubles.
+// Area-weighted map SSD of candidate rotations (see k_rotation_map_cost).
+int map_cost_blocks(int nx, int ny, int stride);
+int launch_rotation_map_costs(const double *smap, const double *tmap, int nx, int ny, int stride,
+                              const float *mats, int na, double *partial, double *costs,
+                              hipStream_t s);
 int launch_rotation_cost_mats(const RotationCostArgs &a, const float *mats, bool gradient,
                               hipStream_t s);
 

@@ -475,6 +475,63 @@ k_rotation_cost_mats(const float *__restrict__ src_pts, const double *__restrict
     }
 }
 
+// Rotation SSD on feature maps: for every `stride`-th pixel x of the target
+// map (equirectangular, pixel centre (x+½, y+½) at azimuth 2π(x+½)/nx and
+// polar angle π(y+½)/ny), the source map is sampled bilinearly at Mᵀx (x
+// circular, y clamped) and w(x) (S(Mᵀx) - T(x))² is summed in float64 with
+// w = sin(polar angle), the pixel's area weight.  The continuous form equals
+// the point cost's ∫ (s(p) - t(M p))² dp (a rotation keeps the measure), with
+// no point location.  blockIdx.y is the candidate.
+__global__ void __launch_bounds__(kBlock)
+k_rotation_map_cost(const double *__restrict__ smap, const double *__restrict__ tmap, int nx,
+                    int ny, int stride, int sx, int sy, const float *__restrict__ mats,
+                    double *__restrict__ partial, int nblocks)
+{
+    __shared__ double red[kBlock / 64];
+    __shared__ float m[9];
+    const int a = blockIdx.y;
+    if (threadIdx.x < 9)
+        m[threadIdx.x] = mats[9 * (size_t)a + threadIdx.x];
+    __syncthreads();
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (k < sx * sy) {
+        const int yi = (k / sx) * stride, xi = (k % sx) * stride;
+        const float kPi = 3.14159265358979f;
+        float st, ct, sp, cp;
+        __sincosf(kPi * ((float)yi + 0.5f) / (float)ny, &st, &ct);
+        __sincosf(2.0f * kPi * ((float)xi + 0.5f) / (float)nx, &sp, &cp);
+        const float px = st * cp, py = st * sp, pz = ct;
+        // Mᵀ x
+        const float qx = m[0] * px + m[3] * py + m[6] * pz;
+        const float qy = m[1] * px + m[4] * py + m[7] * pz;
+        const float qz = m[2] * px + m[5] * py + m[8] * pz;
+        float phi = atan2f(qy, qx);
+        if (phi < 0.0f)
+            phi += 2.0f * kPi;
+        const float theta = acosf(fminf(fmaxf(qz, -1.0f), 1.0f));
+        const float u = phi * (0.5f / kPi) * (float)nx - 0.5f;
+        const float v = theta * (1.0f / kPi) * (float)ny - 0.5f;
+        const float fu = floorf(u), fv = floorf(v);
+        const float du = u - fu, dv = v - fv;
+        int x0 = (int)fu, y0 = (int)fv;
+        int x1 = x0 + 1, y1 = y0 + 1;
+        x0 = x0 < 0 ? x0 + nx : (x0 >= nx ? x0 - nx : x0);
+        x1 = x1 < 0 ? x1 + nx : (x1 >= nx ? x1 - nx : x1);
+        y0 = y0 < 0 ? 0 : (y0 >= ny ? ny - 1 : y0);
+        y1 = y1 < 0 ? 0 : (y1 >= ny ? ny - 1 : y1);
+        const double s00 = smap[(size_t)y0 * nx + x0], s01 = smap[(size_t)y0 * nx + x1];
+        const double s10 = smap[(size_t)y1 * nx + x0], s11 = smap[(size_t)y1 * nx + x1];
+        const double sv = (s00 * (1.0 - du) + s01 * du) * (1.0 - dv) +
+                          (s10 * (1.0 - du) + s11 * du) * dv;
+        const double d = sv - tmap[(size_t)yi * nx + xi];
+        e = (double)st * d * d;
+    }
+    double sum = block_sum(e, red);
+    if (threadIdx.x == 0)
+        partial[(size_t)a * nblocks + blockIdx.x] = sum;
+}
+
 // out[r] = sum of row r of the block partials; one block per row, a fixed
 // (deterministic) summation order.
 __global__ void __launch_bounds__(kBlock)
@@ -519,6 +576,26 @@ int launch_rotation_cost_mats(const RotationCostArgs &a, const float *mats, bool
     CSG_LAUNCH_CHECK();
     const int rows = a.na * (gradient ? kRotationGradTerms : 1);
     hipLaunchKernelGGL(k_sum_rows, dim3(rows), dim3(kBlock), 0, s, a.partial, nb, rows, a.costs);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+int map_cost_blocks(int nx, int ny, int stride)
+{
+    const int sx = (nx + stride - 1) / stride, sy = (ny + stride - 1) / stride;
+    return (int)grid_for((size_t)sx * sy);
+}
+
+int launch_rotation_map_costs(const double *smap, const double *tmap, int nx, int ny, int stride,
+                              const float *mats, int na, double *partial, double *costs,
+                              hipStream_t s)
+{
+    const int sx = (nx + stride - 1) / stride, sy = (ny + stride - 1) / stride;
+    const int nb = map_cost_blocks(nx, ny, stride);
+    hipLaunchKernelGGL(k_rotation_map_cost, dim3(nb, na), dim3(kBlock), 0, s, smap, tmap, nx, ny,
+                       stride, sx, sy, mats, partial, nb);
+    CSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_rows, dim3(na), dim3(kBlock), 0, s, partial, nb, na, costs);
     CSG_LAUNCH_CHECK();
     return kOk;
 }

@@ -332,6 +332,19 @@ int csg_rotation_cost_matrices(const float *src_pts, const double *src_vals, int
                                const float *mats, int na, int with_gradient, double *out,
                                void *ws, size_t ws_bytes, int *uncovered_dev, void *stream);
 
+/* Rotation SSD of na <= 65535 candidate matrices (float32 [na][9], row-major) on feature
+ * maps: src_map and tgt_map are float64 [ny][nx] equirectangular maps (pixel centre
+ * (x+1/2, y+1/2) at azimuth 2 pi (x+1/2)/nx and polar angle pi (y+1/2)/ny, the layout of
+ * the stencil sheets); for every stride-th pixel x in each direction, costs[a] sums
+ * sin(polar) (S(M^T x) - T(x))^2 with S sampled bilinearly (x circular, y clamped).  The
+ * continuous form of cost_batch_prepared's SSD (surface_rotation.py:512-546) without point
+ * location: a coarse score for many candidates (the Hopf grid) before refinement.  No
+ * counterpart in the reference.  Does not synchronise. */
+size_t csg_rotation_map_costs_workspace_bytes(int nx, int ny, int stride, int na);
+int csg_rotation_map_costs(const double *src_map, const double *tgt_map, int nx, int ny,
+                           int stride, const float *mats, int na, double *costs, void *ws,
+                           size_t ws_bytes, void *stream);
+
 /* Gradient refinement of m <= 256 candidate rotations (no counterpart in the reference,
  * which refines its best seed by Nelder-Mead, surface_rotation.py:594-699).  start holds
  * m Euler-angle triples (start_is_angles = 1, the reference's angle convention) or m

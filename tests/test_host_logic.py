@@ -290,3 +290,23 @@ def test_axis_angle_matrix_is_the_matrix_exponential():
     for d in (np.array([0.3, -0.2, 0.5]), np.array([1e-9, 0.0, 2e-9]), np.zeros(3)):
         k = np.array([[0.0, -d[2], d[1]], [d[2], 0.0, -d[0]], [-d[1], d[0], 0.0]])
         np.testing.assert_allclose(axis_angle_matrix(d), scipy.linalg.expm(k), atol=1e-14)
+
+
+def test_real_sh_basis_matches_scipy_spherical_harmonics():
+    import math
+
+    from scipy.special import sph_harm_y
+
+    from cat_surface_gpu.synthetic import real_sh_basis
+
+    rng = np.random.default_rng(4)
+    p = rng.normal(size=(300, 3))
+    p /= np.linalg.norm(p, axis=1, keepdims=True)
+    theta, phi = np.arccos(np.clip(p[:, 2], -1, 1)), np.arctan2(p[:, 1], p[:, 0])
+    cols = []
+    for l in range(1, 9):
+        for m in range(-l, l + 1):
+            y = sph_harm_y(l, abs(m), theta, phi)
+            cols.append(math.sqrt(2) * (-1) ** m * y.imag if m < 0 else
+                        (y.real if m == 0 else math.sqrt(2) * (-1) ** m * y.real))
+    np.testing.assert_allclose(real_sh_basis(p, 8), np.stack(cols, axis=1), atol=1e-13)

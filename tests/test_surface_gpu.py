@@ -310,7 +310,36 @@ def test_rotation_cost_gradient_matches_finite_differences(cuda):
     assert np.linalg.eigvalsh(hess).min() >= -1e-9 * np.abs(hess).max()
 
 
-@pytest.mark.parametrize("candidates", ["seeds", "hopf"])
+def _field_maps(m, cuda, nx=512, ny=256):
+    """Equirectangular maps of the smooth field: target T(x) = f(x), source
+    S(x) = f(M x), so that S(Mᵀ x) = T(x)."""
+
+    from cat_surface_gpu.stencil_builder import uv_points
+
+    def field(p):
+        return np.sin(2.0 * p[:, 0]) + np.cos(3.0 * p[:, 1]) * p[:, 2] + 0.5 * p[:, 0] * p[:, 1]
+
+    x = uv_points(nx, ny)
+    to = dict(dtype=torch.float64, device=cuda)
+    return (torch.as_tensor(field(x @ m.T).reshape(ny, nx), **to),
+            torch.as_tensor(field(x).reshape(ny, nx), **to))
+
+
+def test_map_rotation_cost_is_minimal_at_the_true_rotation(cuda):
+    from cat_surface_gpu.surface_rotation import axis_angle_matrix, map_costs, rotation_matrix_np
+
+    m = rotation_matrix_np((0.2, -0.1, 0.15))
+    smap, tmap = _field_maps(m, cuda)
+    others = [axis_angle_matrix(d) @ m for d in ([0.05, 0, 0], [0, -0.05, 0], [0, 0, 0.05])]
+    mats = torch.as_tensor(np.stack([m, np.eye(3)] + others), device=cuda)
+    costs = _np(map_costs(smap, tmap, mats))
+    assert costs[0] < 1e-3 * costs[1:].min()
+    # the stride only subsamples the target pixels
+    full = _np(map_costs(smap, tmap, mats, stride=1))
+    assert int(np.argmin(full)) == 0
+
+
+@pytest.mark.parametrize("candidates", ["seeds", "hopf", "hopf-maps"])
 def test_adjoint_rotation_search_reaches_nelder_mead_cost(cuda, candidates):
     """Top candidates refined by gradient steps cost no more than the reference's
     seed + Nelder-Mead choice (within 1e-6 relative) and find the true rotation."""
@@ -319,7 +348,11 @@ def test_adjoint_rotation_search_reaches_nelder_mead_cost(cuda, candidates):
 
     idx, inputs, m = _smooth_rotation_problem(cuda)
     nm = search_rotations([(idx, inputs)])[0]
-    adj = search_rotations([(idx, inputs)], method="adjoint", candidates=candidates)[0]
+    problem = (idx, inputs)
+    if candidates == "hopf-maps":
+        problem = (idx, inputs, _field_maps(m, cuda))
+        candidates = "hopf"
+    adj = search_rotations([problem], method="adjoint", candidates=candidates)[0]
     assert float(adj.cost) <= float(nm.cost) * (1.0 + 1e-6)
     rel = _np(adj.rotation_matrix).astype(np.float64) @ m.T
     assert np.arccos(np.clip((np.trace(rel) - 1.0) / 2.0, -1.0, 1.0)) < 5e-3
