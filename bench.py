@@ -49,16 +49,26 @@ def make_inputs(seed: int):
     return pairs
 
 
+PLAIN_DEFINITION = ("one registration per call (the reference's pipeline), 46 seeds + "
+                    "Nelder-Mead, one launch per reference multigrid operator (per-operator "
+                    "FMG schedule), solves launched eagerly (no graph capture)")
+
+
 def run_step(pairs, dtype: str, mode: str, device, cuda_graph: bool):
-    """One step: LH and RH.  "plain" registers them one after the other (the
-    reference's pipeline, one job per call); "batched" schedules both jobs
-    together (run_cat_surface_gpu_pipelines: shared feature solve and batched
-    DARTEL rounds, identical results); "batched+adjoint" also replaces the
-    Nelder-Mead rotation refinement by gradient (Levenberg-Marquardt) steps on
+    """One step: LH and RH.
+
+    "plain" is the plain method: the reference's pipeline one registration at a
+    time, Nelder-Mead rotation refinement, and the DARTEL solves as one launch
+    per reference operator (per-operator multigrid schedule), launched eagerly.
+    "plain-fused" is the same with the fused multigrid schedule and graph replay.
+    "batched" schedules both jobs together (run_cat_surface_gpu_pipelines:
+    shared feature solve and batched DARTEL rounds on the fused schedule, graph
+    replay, identical results); "batched+adjoint" also tunes the relaxation
+    tiles and replaces Nelder-Mead by gradient (Levenberg-Marquardt) steps on
     the 8 best seeds."""
 
     from cat_surface_gpu import (PipelineJob, run_cat_surface_gpu_pipeline,
-                                 run_cat_surface_gpu_pipelines)
+                                 run_cat_surface_gpu_pipelines, set_fmg_schedule)
 
     kw = dict(device=device, dartel_dtype=dtype, steps=2, runs=2, avg=True, write_output=False,
               cuda_graph=cuda_graph)
@@ -66,6 +76,14 @@ def run_step(pairs, dtype: str, mode: str, device, cuda_graph: bool):
         kw.update(rotation_method="adjoint", autotune=True)
     if mode in ("batched", "batched+adjoint"):
         return run_cat_surface_gpu_pipelines([PipelineJob(meshes=m) for m in pairs], **kw)
+    if mode == "plain":
+        kw.update(cuda_graph=False)
+        previous = set_fmg_schedule(False)
+        try:
+            return [run_cat_surface_gpu_pipeline(None, None, None, None, "unused.gii",
+                                                 meshes=m, **kw) for m in pairs]
+        finally:
+            set_fmg_schedule(previous)
     return [run_cat_surface_gpu_pipeline(None, None, None, None, "unused.gii", meshes=m, **kw)
             for m in pairs]
 
@@ -264,13 +282,54 @@ def adjoint_report(device) -> dict:
     return {"metric": "exp-warp-SSD adjoint (K=6, membrane 0.01)", "n_gpus": 1, "rows": rows}
 
 
+def sor_traffic_run(device, copies: int = 3) -> dict:
+    """The dominant kernel alone, for hardware-counter runs: one depth-potential
+    solve of four 40,962-vertex meshes (the bench's sides: icosphere-6
+    resampling of each surface, 1000 SOR sweeps, the XCD-local kernel), then
+    `copies` plain device copies of 64 MiB (torch) whose counters give the
+    bytes-per-counter-unit scale."""
+
+    import numpy as np
+    import torch
+
+    import cat_surface_gpu as csg
+    from cat_surface_gpu.rotation_feature import depth_potential
+    from cat_surface_gpu.stencil_builder import icosphere
+
+    v, f = icosphere(6)
+    n = v.shape[0]
+    idx = np.repeat(np.arange(n, dtype=np.int32)[:, None], 3, axis=1)
+    w = np.tile(np.array([1.0, 0.0, 0.0]), (n, 1))
+    st = csg.SurfaceStencil(sphere_points=v, faces=f.astype(np.int32), surface_indices=idx,
+                            surface_weights=w, sheet_indices=np.zeros((1, 1, 3), np.int32),
+                            sheet_weights=np.zeros((1, 1, 3)), nx=1, ny=1,
+                            source_points=n).to(device)
+    rng = np.random.default_rng(7)
+    pts = torch.as_tensor(v[None] * 60.0 * (1.0 + 0.05 * rng.standard_normal((4, n, 1))),
+                          dtype=torch.float32, device=device)
+    depth_potential(st, pts)  # warm
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    depth_potential(st, pts)
+    torch.cuda.synchronize()
+    solve_s = time.perf_counter() - t
+    nbytes = 64 << 20
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=device)
+    dst = torch.empty_like(src)
+    for _ in range(copies):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"workload": "depth potential, 4 x 40,962 vertices, 1000 SOR sweeps",
+            "solve_s": round(solve_s, 4), "copy_bytes": nbytes, "copies": copies}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--dtype", choices=("float32", "float64"), default="float32")
-    ap.add_argument("--mode", choices=("plain", "batched", "batched+adjoint"),
+    ap.add_argument("--mode", choices=("plain", "plain-fused", "batched", "batched+adjoint"),
                     default="batched+adjoint")
     ap.add_argument("--seed", type=int, default=20261205)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -278,7 +337,7 @@ def main() -> int:
                     help="after the timed mode, time the plain path on the same inputs and "
                          "report it with the largest vertex difference between the two")
     ap.add_argument("--workload", choices=("hemisphere-pair", "uneven-32", "hopf-8192",
-                                           "adjoint"), default="hemisphere-pair",
+                                           "adjoint", "sor-traffic"), default="hemisphere-pair",
                     help="hemisphere-pair is the headline metric; the others print reports "
                          "of the uneven multi-job schedule, the Hopf-grid rotation search and "
                          "the exp-warp-SSD adjoint")
@@ -297,7 +356,8 @@ def main() -> int:
     if args.workload != "hemisphere-pair":
         report = {"uneven-32": lambda: uneven_report(args.seed, device),
                   "hopf-8192": lambda: hopf_report(args.seed, device),
-                  "adjoint": lambda: adjoint_report(device)}[args.workload]()
+                  "adjoint": lambda: adjoint_report(device),
+                  "sor-traffic": lambda: sor_traffic_run(device)}[args.workload]()
         print(json.dumps(report), flush=True)
         return 0
     t = time.perf_counter()
@@ -330,21 +390,24 @@ def main() -> int:
     kernels = kernel_rooflines()
     roof = kernels[0] if kernels else None
     plain = None
-    if args.compare_plain and args.mode != "plain":
-        for i in range(args.warmup):
-            run_step(pairs, args.dtype, "plain", device, args.cuda_graph)
-        torch.cuda.synchronize()
-        tp = time.perf_counter()
-        for i in range(args.steps):
-            plain_results = run_step(pairs, args.dtype, "plain", device, args.cuda_graph)
-        torch.cuda.synchronize()
-        plain_value = (time.perf_counter() - tp) / (args.steps * hemispheres)
-        delta = max(float(abs(a.vertices.astype("float64") - b.vertices).max())
-                    for a, b in zip(results, plain_results))
-        plain = {"value": round(plain_value, 4), "unit": "s/hemisphere",
-                 "speedup": round(plain_value / value, 3),
-                 "max_vertex_delta": delta}
-        log("plain comparison done")
+    if args.compare_plain and args.mode not in ("plain", "plain-fused"):
+        plain = {}
+        for leg in ("plain", "plain-fused"):
+            for i in range(args.warmup):
+                run_step(pairs, args.dtype, leg, device, args.cuda_graph)
+            torch.cuda.synchronize()
+            tp = time.perf_counter()
+            for i in range(args.steps):
+                plain_results = run_step(pairs, args.dtype, leg, device, args.cuda_graph)
+            torch.cuda.synchronize()
+            plain_value = (time.perf_counter() - tp) / (args.steps * hemispheres)
+            delta = max(float(abs(a.vertices.astype("float64") - b.vertices).max())
+                        for a, b in zip(results, plain_results))
+            plain[leg] = {"value": round(plain_value, 4), "unit": "s/hemisphere",
+                          "speedup": round(plain_value / value, 3),
+                          "max_vertex_delta": delta}
+            log(f"{leg} comparison done")
+        plain["definition"] = PLAIN_DEFINITION
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     stages = {k: round(v / (args.steps * hemispheres), 4) for k, v in sorted(stage_sums.items())}
     line = {
