@@ -44,11 +44,9 @@ int depth_setup(const DepthArgs &a, hipStream_t s);
 int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double omega,
               hipStream_t s);
 int depth_residual(const DepthArgs &a, double *out, hipStream_t s);
-// Process-wide placement of the SOR sweeps: 0 = device-wide grid barrier;
-// 1 = each mesh's rows on one XCD, x accessed with L1-bypassing device-scope
-// loads and stores; 2 = on one XCD, x cached in L1 and L2 with the L1
-// invalidated after every barrier.  1 and 2 apply when the meshes fit.
-// Returns the previous setting.  All compute identical values.
-int set_sor_xcd_local(int mode);
+// Process-wide: 1 (default) runs the SOR with each mesh's rows on one XCD when
+// the meshes fit, 0 always uses the device-wide grid barrier.  Returns the
+// previous setting.  Both compute identical values.
+int set_sor_xcd_local(int on);
 
 }  // namespace csg

@@ -436,13 +436,12 @@ k_sor_registers(const int *__restrict__ verts, SorColours co, const int *__restr
 // write back and invalidate L2 on every colour.  The sweeps of one mesh only
 // read and write that mesh's x, so when the rows of each mesh are handled by
 // workgroups that all run on ONE XCD, the L2 they share is the coherence
-// point: each wave waits for its x stores to be acknowledged (s_waitcnt 0)
-// before the workgroup barrier, thread 0 then bumps the group's arrival
-// counter in that L2, and readers poll the counter and gather x from the
-// shared L2: either with L1-bypassing device-scope (sc1) accesses, or with
-// plain cached accesses after invalidating their CU's L1.  No agent-scope
-// L2 writeback or invalidate is needed between colours; the kernel's end
-// writes the L2 back as every kernel's does.
+// point: x is written with L1-bypassing (sc1) stores, each wave waits for its
+// stores to be acknowledged (s_waitcnt 0) before the workgroup barrier,
+// thread 0 then bumps the group's arrival counter in that L2, and readers
+// poll the counter and gather x with sc1 loads, which miss L1 and hit the
+// shared L2.  No L2 writeback or invalidate is needed between colours; the
+// kernel's end writes the L2 back as every kernel's does.
 //
 // Placement is not assumed from the dispatch order: thread 0 of every
 // workgroup reads its XCD from the XCC_ID hardware register and claims the
@@ -460,32 +459,19 @@ __device__ __forceinline__ int xcd_id()
     return (int)(v & (kXcds - 1));
 }
 
-// x accesses between colours.  BYPASS: device-scope (sc1) loads and stores,
-// which miss the CU's L1 and write through the L2.  Otherwise plain accesses,
-// cached in L1 and L2 (L2 write-back); the barrier then invalidates the CU's
-// L1 (buffer_inv sc0) so that the next colour reads every x from the XCD's L2.
-template <bool BYPASS>
-__device__ __forceinline__ double load_x(const double *p)
+__device__ __forceinline__ double load_l2(const double *p)
 {
-    if (BYPASS)
-        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return *p;
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <bool BYPASS>
-__device__ __forceinline__ void store_x(double *p, double v)
+__device__ __forceinline__ void store_l2(double *p, double v)
 {
-    if (BYPASS)
-        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else
-        *p = v;
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Barrier of the workgroups of one XCD group.  The arrival counter only grows:
 // phase p is complete when it reaches p * nblocks (modular arithmetic), so it
-// needs no reset and no generation word.  With INVALIDATE every wave drops
-// its CU's L1 contents after the barrier (no outputs: nothing in flight).
-template <bool INVALIDATE>
+// needs no reset and no generation word.
 __device__ bool xcd_barrier(unsigned *cnt, unsigned target, int *timeout)
 {
     __shared__ int ok;
@@ -506,8 +492,6 @@ __device__ bool xcd_barrier(unsigned *cnt, unsigned target, int *timeout)
         }
     }
     __syncthreads();
-    if (INVALIDATE)
-        asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
     return ok != 0;
 }
 
@@ -515,7 +499,7 @@ __device__ bool xcd_barrier(unsigned *cnt, unsigned target, int *timeout)
 // XCD x handle meshes x, x + 8, ... (thread t of the group owns row t of every
 // colour, meshes side by side as in k_sor_registers).  Barrier words: slot x
 // holds XCD x's claim counter, slot 8 + x its arrival counter.
-template <int CMAX, int WMAX, bool BYPASS>
+template <int CMAX, int WMAX>
 __global__ void __launch_bounds__(kSorRegBlock)
 k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restrict__ ell_cols,
                 const double *__restrict__ ell_val, const double *__restrict__ rhs_c,
@@ -583,12 +567,11 @@ k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restr
 #pragma unroll
                 for (int k = 0; k < WMAX; ++k)
                     if (k < W)
-                        s += val[c][k] * load_x<BYPASS>(x + col[c][k]);
-                store_x<BYPASS>(x + vi[c],
-                                load_x<BYPASS>(x + vi[c]) + omega * (rh[c] - s) / dg[c]);
+                        s += val[c][k] * load_l2(x + col[c][k]);
+                store_l2(x + vi[c], load_l2(x + vi[c]) + omega * (rh[c] - s) / dg[c]);
             }
             target += (unsigned)nblocks;
-            if (!xcd_barrier<!BYPASS>(cnt, target, timeout))
+            if (!xcd_barrier(cnt, target, timeout))
                 return;
         }
     }
@@ -669,9 +652,9 @@ namespace {
 std::atomic<int> g_sor_xcd_local{1};
 }
 
-int set_sor_xcd_local(int mode)
+int set_sor_xcd_local(int on)
 {
-    return g_sor_xcd_local.exchange(mode < 0 ? 0 : (mode > 2 ? 2 : mode));
+    return g_sor_xcd_local.exchange(on ? 1 : 0);
 }
 
 int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double omega,
@@ -704,16 +687,9 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     // its share of the CUs at the register kernel's occupancy
     SorXcdPlan plan{};
     int local_per_cu = 0;
-    const int placement = g_sor_xcd_local.load();
-    if (placement == 1)
-        CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &local_per_cu, k_sor_xcd_local<kSorRegColours, kSorRegWidth, true>, kSorRegBlock,
-            0));
-    else
-        CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &local_per_cu, k_sor_xcd_local<kSorRegColours, kSorRegWidth, false>, kSorRegBlock,
-            0));
-    bool local = placement != 0 && co.count <= kSorRegColours &&
+    CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &local_per_cu, k_sor_xcd_local<kSorRegColours, kSorRegWidth>, kSorRegBlock, 0));
+    bool local = g_sor_xcd_local.load() != 0 && co.count <= kSorRegColours &&
                  W <= kSorRegWidth && cus % kXcds == 0;
     int local_grid = 0;
     for (int xcd = 0; xcd < kXcds && local; ++xcd) {
@@ -749,16 +725,10 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     // A regular launch: the grid is capped at what the device holds at once
     // (occupancy x CUs), and every barrier wait is bounded, so blocks that were
     // not resident turn into a timeout error rather than a hang.
-    if (local && placement == 1)
-        hipLaunchKernelGGL((k_sor_xcd_local<kSorRegColours, kSorRegWidth, true>),
-                           dim3(local_grid), dim3(kSorRegBlock), 0, s, verts, colours, ell_cols,
-                           ell_val, rhs_c, diag_c, width, omega, n, batch, max_iter, plan, x, bar,
-                           timeout);
-    else if (local)
-        hipLaunchKernelGGL((k_sor_xcd_local<kSorRegColours, kSorRegWidth, false>),
-                           dim3(local_grid), dim3(kSorRegBlock), 0, s, verts, colours, ell_cols,
-                           ell_val, rhs_c, diag_c, width, omega, n, batch, max_iter, plan, x, bar,
-                           timeout);
+    if (local)
+        hipLaunchKernelGGL((k_sor_xcd_local<kSorRegColours, kSorRegWidth>), dim3(local_grid),
+                           dim3(kSorRegBlock), 0, s, verts, colours, ell_cols, ell_val, rhs_c,
+                           diag_c, width, omega, n, batch, max_iter, plan, x, bar, timeout);
     else if (registers)
         hipLaunchKernelGGL((k_sor_registers<kSorRegColours, kSorRegWidth>), dim3(reg_grid),
                            dim3(kSorRegBlock), 0, s, verts, colours, ell_cols, ell_val, rhs_c,
