@@ -432,16 +432,20 @@ k_sor_registers(const int *__restrict__ verts, SorColours co, const int *__restr
 // ---- XCD-local SOR
 //
 // MI355X has eight XCDs, each with its own L2; the L2s are not coherent with
-// one another, so a grid barrier across XCDs needs agent-scope fences that
-// write back and invalidate L2 on every colour.  The sweeps of one mesh only
-// read and write that mesh's x, so when the rows of each mesh are handled by
-// workgroups that all run on ONE XCD, the L2 they share is the coherence
-// point: x is written with L1-bypassing (sc1) stores, each wave waits for its
-// stores to be acknowledged (s_waitcnt 0) before the workgroup barrier,
-// thread 0 then bumps the group's arrival counter in that L2, and readers
-// poll the counter and gather x with sc1 loads, which miss L1 and hit the
-// shared L2.  No L2 writeback or invalidate is needed between colours; the
-// kernel's end writes the L2 back as every kernel's does.
+// one another, so the grid barrier above needs agent-scope release/acquire
+// fences that write back and invalidate L2 on every colour, ~7000 times per
+// solve.  This kernel needs none.  x is only ever read and written with
+// device-scope relaxed atomic loads and stores (sc1: they bypass L1 and are
+// coherent across the device, the property that lets a relaxed poll see
+// another XCD's update); each wave waits for its x stores to complete
+// (s_waitcnt 0) before the workgroup barrier, so they are visible before
+// thread 0 bumps the arrival counter, and readers gather x only after their
+// poll has seen every arrival.  The sweeps of one mesh only touch that mesh's
+// x, so each mesh has its own barrier among the workgroups that own its rows,
+// all placed on one XCD: a short arrival chain (~13 workgroups for 40,962
+// vertices) on a counter in that XCD's L2.  (The same arrangement with cached
+// x accesses and an L1 invalidation per colour does not give the grid
+// barrier's values on MI355X, so x stays device-coherent.)
 //
 // Placement is not assumed from the dispatch order: thread 0 of every
 // workgroup reads its XCD from the XCC_ID hardware register and claims the
