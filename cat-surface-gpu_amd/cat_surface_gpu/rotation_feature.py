@@ -27,19 +27,15 @@ class RotationFeatureResult:
     relative_residual: float
 
 
-def set_sor_placement(mode) -> int:
-    """Choose how the depth-potential SOR sweeps synchronise; returns the previous mode.
+def set_sor_placement(xcd_local: bool) -> bool:
+    """Choose where the depth-potential SOR sweeps run; returns the previous setting.
 
-    ``1`` / ``True`` (the default) keeps each mesh's rows on the workgroups of
-    one XCD with one barrier per mesh on an arrival counter; ``2`` the same
-    with per-workgroup flags; ``0`` / ``False`` one device-wide grid barrier
-    per colour.  All compute identical values.
+    ``True`` (the default) keeps each mesh's rows on the workgroups of one XCD
+    so that colours synchronise through that XCD's L2; ``False`` uses one
+    device-wide grid barrier per colour.  Both compute identical values.
     """
 
-    code = int(mode)
-    if code not in (0, 1, 2):
-        raise ValueError("SOR placement must be 0, 1 or 2")
-    return int(_lib.lib().csg_set_sor_placement(code))
+    return bool(_lib.lib().csg_set_sor_placement(1 if xcd_local else 0))
 
 
 def depth_potential(stencil: SurfaceStencilDevice, smoothed_surface, *,

@@ -472,32 +472,6 @@ __device__ __forceinline__ void store_l2(double *p, double v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Barrier of the workgroups of one XCD group on one arrival counter, which
-// only grows: phase p is complete when it reaches p * nblocks (modulo 2^32),
-// so it needs no reset and no generation word.
-__device__ bool counter_barrier(unsigned *cnt, unsigned target, int *timeout)
-{
-    __shared__ int ok;
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's x stores are complete
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        ok = 1;
-        __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        long spins = 0;
-        while ((int)(__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
-                     target) < 0) {
-            if (++spins > kSpinLimit) {
-                __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    __syncthreads();
-    return ok != 0;
-}
-
 // Barrier of the workgroups of one XCD group, by per-workgroup flags: the
 // group's flags share one cache line (kBarrierSlotWords words, so at most that
 // many workgroups), a workgroup arrives by storing the phase number into its
@@ -544,7 +518,7 @@ __device__ bool flag_barrier(unsigned *flags, int slot, int nblocks, unsigned ph
 // XCD x handle meshes x, x + 8, ... (thread t of the group owns row t of every
 // colour, meshes side by side as in k_sor_registers).  Barrier words: slot x
 // holds XCD x's claim counter, slot 8 + x its group's barrier flags.
-template <int CMAX, int WMAX, bool FLAGS>
+template <int CMAX, int WMAX>
 __global__ void __launch_bounds__(kSorRegBlock)
 k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restrict__ ell_cols,
                 const double *__restrict__ ell_val, const double *__restrict__ rhs_c,
@@ -615,9 +589,7 @@ k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restr
                         s += val[c][k] * load_l2(x + col[c][k]);
                 store_l2(x + vi[c], load_l2(x + vi[c]) + omega * (rh[c] - s) / dg[c]);
             }
-            ++phase;
-            if (FLAGS ? !flag_barrier(flags, slot, nblocks, phase, timeout)
-                      : !counter_barrier(flags, phase * (unsigned)nblocks, timeout))
+            if (!flag_barrier(flags, slot, nblocks, ++phase, timeout))
                 return;
         }
     }
@@ -698,9 +670,9 @@ namespace {
 std::atomic<int> g_sor_xcd_local{1};
 }
 
-int set_sor_xcd_local(int mode)
+int set_sor_xcd_local(int on)
 {
-    return g_sor_xcd_local.exchange(mode < 0 ? 0 : (mode > 2 ? 2 : mode));
+    return g_sor_xcd_local.exchange(on ? 1 : 0);
 }
 
 int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double omega,
@@ -734,9 +706,8 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     SorXcdPlan plan{};
     int local_per_cu = 0;
     CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &local_per_cu, k_sor_xcd_local<kSorRegColours, kSorRegWidth, true>, kSorRegBlock, 0));
-    const int placement = g_sor_xcd_local.load();
-    bool local = placement != 0 && co.count <= kSorRegColours &&
+        &local_per_cu, k_sor_xcd_local<kSorRegColours, kSorRegWidth>, kSorRegBlock, 0));
+    bool local = g_sor_xcd_local.load() != 0 && co.count <= kSorRegColours &&
                  W <= kSorRegWidth && cus % kXcds == 0;
     int local_grid = 0;
     for (int xcd = 0; xcd < kXcds && local; ++xcd) {
@@ -773,16 +744,10 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     // A regular launch: the grid is capped at what the device holds at once
     // (occupancy x CUs), and every barrier wait is bounded, so blocks that were
     // not resident turn into a timeout error rather than a hang.
-    if (local && placement == 2)
-        hipLaunchKernelGGL((k_sor_xcd_local<kSorRegColours, kSorRegWidth, true>),
-                           dim3(local_grid), dim3(kSorRegBlock), 0, s, verts, colours, ell_cols,
-                           ell_val, rhs_c, diag_c, width, omega, n, batch, max_iter, plan, x, bar,
-                           timeout);
-    else if (local)
-        hipLaunchKernelGGL((k_sor_xcd_local<kSorRegColours, kSorRegWidth, false>),
-                           dim3(local_grid), dim3(kSorRegBlock), 0, s, verts, colours, ell_cols,
-                           ell_val, rhs_c, diag_c, width, omega, n, batch, max_iter, plan, x, bar,
-                           timeout);
+    if (local)
+        hipLaunchKernelGGL((k_sor_xcd_local<kSorRegColours, kSorRegWidth>), dim3(local_grid),
+                           dim3(kSorRegBlock), 0, s, verts, colours, ell_cols, ell_val, rhs_c,
+                           diag_c, width, omega, n, batch, max_iter, plan, x, bar, timeout);
     else if (registers)
         hipLaunchKernelGGL((k_sor_registers<kSorRegColours, kSorRegWidth>), dim3(reg_grid),
                            dim3(kSorRegBlock), 0, s, verts, colours, ell_cols, ell_val, rhs_c,
