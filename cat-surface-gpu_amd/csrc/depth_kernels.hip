@@ -441,8 +441,9 @@ k_sor_registers(const int *__restrict__ verts, SorColours co, const int *__restr
 // (s_waitcnt 0) before the workgroup barrier, so they are visible before
 // thread 0 bumps the arrival counter, and readers gather x only after their
 // poll has seen every arrival.  The sweeps of one mesh only touch that mesh's
-// x, so each mesh has its own barrier among the workgroups that own its rows
-// (~13 for 40,962 vertices), all placed on one XCD (flag_barrier).  (The same arrangement with cached
+// x, so each mesh has its own barrier among the workgroups that own its rows,
+// all placed on one XCD: a short arrival chain (~13 workgroups for 40,962
+// vertices) on a counter in that XCD's L2.  (The same arrangement with cached
 // x accesses and an L1 invalidation per colour does not give the grid
 // barrier's values on MI355X, so x stays device-coherent.)
 //
@@ -472,42 +473,26 @@ __device__ __forceinline__ void store_l2(double *p, double v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Barrier of the workgroups of one XCD group, by per-workgroup flags: the
-// group's flags share one cache line (kBarrierSlotWords words, so at most that
-// many workgroups), a workgroup arrives by storing the phase number into its
-// own word, and wave 0 waits until every word of the group holds at least that
-// phase, reading all of them with one coalesced load per poll.  No arrival
-// waits for another (an arrival counter serialises its atomic adds).  Phase
-// numbers only grow (compared modulo 2^32), so nothing is reset.
-__device__ bool flag_barrier(unsigned *flags, int slot, int nblocks, unsigned phase,
-                             int *timeout)
+// Barrier of the workgroups of one XCD group.  The arrival counter only grows:
+// phase p is complete when it reaches p * nblocks (modular arithmetic), so it
+// needs no reset and no generation word.
+__device__ bool xcd_barrier(unsigned *cnt, unsigned target, int *timeout)
 {
     __shared__ int ok;
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's x stores are complete
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's x stores are in L2
     __syncthreads();
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        if (lane == 0)
-            __hip_atomic_store(flags + slot, phase, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+        ok = 1;
+        __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         long spins = 0;
-        int good = 1;
-        while (true) {
-            bool done = lane >= nblocks ||
-                        (int)(__hip_atomic_load(flags + lane, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT) -
-                              phase) >= 0;
-            if (__all(done))
-                break;
+        while ((int)(__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+                     target) < 0) {
             if (++spins > kSpinLimit) {
-                good = 0;
+                __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
-        }
-        if (lane == 0) {
-            ok = good;
-            if (!good)
-                __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     __syncthreads();
@@ -517,7 +502,7 @@ __device__ bool flag_barrier(unsigned *flags, int slot, int nblocks, unsigned ph
 // k_sor_registers with each mesh's rows on one XCD.  blocks[x] workgroups of
 // XCD x handle meshes x, x + 8, ... (thread t of the group owns row t of every
 // colour, meshes side by side as in k_sor_registers).  Barrier words: slot x
-// holds XCD x's claim counter, slot 8 + x its group's barrier flags.
+// holds XCD x's claim counter, slot 8 + x its arrival counter.
 template <int CMAX, int WMAX>
 __global__ void __launch_bounds__(kSorRegBlock)
 k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restrict__ ell_cols,
@@ -537,7 +522,7 @@ k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restr
     const int nblocks = plan.blocks[xcd];
     if (slot >= nblocks)
         return;
-    unsigned *flags = bar + (size_t)(kXcds + xcd) * kBarrierSlotWords;
+    unsigned *cnt = bar + (size_t)(kXcds + xcd) * kBarrierSlotWords;
     const int meshes = batch > xcd ? (batch - xcd + kXcds - 1) / kXcds : 0;
     const int tid = slot * blockDim.x + threadIdx.x;
     const size_t wn = (size_t)W * n;
@@ -575,7 +560,7 @@ k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restr
             }
         }
     }
-    unsigned phase = 0;
+    unsigned target = 0;
     for (int it = 0; it < max_iter; ++it) {
 #pragma unroll
         for (int c = 0; c < CMAX; ++c) {
@@ -589,7 +574,8 @@ k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restr
                         s += val[c][k] * load_l2(x + col[c][k]);
                 store_l2(x + vi[c], load_l2(x + vi[c]) + omega * (rh[c] - s) / dg[c]);
             }
-            if (!flag_barrier(flags, slot, nblocks, ++phase, timeout))
+            target += (unsigned)nblocks;
+            if (!xcd_barrier(cnt, target, timeout))
                 return;
         }
     }
@@ -713,8 +699,7 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     for (int xcd = 0; xcd < kXcds && local; ++xcd) {
         const int meshes = a.batch > xcd ? (a.batch - xcd + kXcds - 1) / kXcds : 0;
         plan.blocks[xcd] = (int)((size_t)most * meshes + kSorRegBlock - 1) / kSorRegBlock;
-        local = plan.blocks[xcd] <= (cus / kXcds) * local_per_cu &&
-                plan.blocks[xcd] <= kBarrierSlotWords;
+        local = plan.blocks[xcd] <= (cus / kXcds) * local_per_cu;
         local_grid = plan.blocks[xcd] > local_grid ? plan.blocks[xcd] : local_grid;
     }
     local_grid = (local_grid + kXcdSpare) * kXcds;
