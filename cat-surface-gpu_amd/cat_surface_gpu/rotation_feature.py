@@ -35,7 +35,7 @@ def set_sor_placement(xcd_local: bool) -> bool:
     device-wide grid barrier per colour.  Both compute identical values.
     """
 
-    return bool(_lib.lib().csg_set_sor_placement(1 if xcd_local else 0))
+    return int(_lib.lib().csg_set_sor_placement(int(xcd_local)))
 
 
 def depth_potential(stencil: SurfaceStencilDevice, smoothed_surface, *,

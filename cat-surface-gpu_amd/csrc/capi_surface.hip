@@ -305,9 +305,11 @@ size_t csg_depth_workspace_bytes(int n, int nf, int nnz, int batch, int ell_widt
     return c.used;
 }
 
-int csg_set_sor_placement(int xcd_local)
+int csg_set_sor_placement(int mode)
 {
-    return set_sor_xcd_local(xcd_local);
+    if (mode < 0 || mode > 2)
+        return fail(kInvalidArgument, "SOR placement must be 0, 1 or 2, got %d", mode);
+    return set_sor_xcd_local(mode);
 }
 
 // Depth potential of the rotation feature for `batch` meshes of one topology:

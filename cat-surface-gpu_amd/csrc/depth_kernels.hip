@@ -468,9 +468,13 @@ __device__ __forceinline__ double load_l2(const double *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+template <bool WRITE_THROUGH>
 __device__ __forceinline__ void store_l2(double *p, double v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (WRITE_THROUGH)
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        *p = v;
 }
 
 // Barrier of the workgroups of one XCD group.  The arrival counter only grows:
@@ -503,7 +507,7 @@ __device__ bool xcd_barrier(unsigned *cnt, unsigned target, int *timeout)
 // XCD x handle meshes x, x + 8, ... (thread t of the group owns row t of every
 // colour, meshes side by side as in k_sor_registers).  Barrier words: slot x
 // holds XCD x's claim counter, slot 8 + x its arrival counter.
-template <int CMAX, int WMAX>
+template <int CMAX, int WMAX, bool WT>
 __global__ void __launch_bounds__(kSorRegBlock)
 k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restrict__ ell_cols,
                 const double *__restrict__ ell_val, const double *__restrict__ rhs_c,
@@ -572,7 +576,7 @@ k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restr
                 for (int k = 0; k < WMAX; ++k)
                     if (k < W)
                         s += val[c][k] * load_l2(x + col[c][k]);
-                store_l2(x + vi[c], load_l2(x + vi[c]) + omega * (rh[c] - s) / dg[c]);
+                store_l2<WT>(x + vi[c], load_l2(x + vi[c]) + omega * (rh[c] - s) / dg[c]);
             }
             target += (unsigned)nblocks;
             if (!xcd_barrier(cnt, target, timeout))
@@ -656,9 +660,9 @@ namespace {
 std::atomic<int> g_sor_xcd_local{1};
 }
 
-int set_sor_xcd_local(int on)
+int set_sor_xcd_local(int mode)
 {
-    return g_sor_xcd_local.exchange(on ? 1 : 0);
+    return g_sor_xcd_local.exchange(mode < 0 ? 0 : (mode > 2 ? 2 : mode));
 }
 
 int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double omega,
@@ -692,8 +696,9 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     SorXcdPlan plan{};
     int local_per_cu = 0;
     CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &local_per_cu, k_sor_xcd_local<kSorRegColours, kSorRegWidth>, kSorRegBlock, 0));
-    bool local = g_sor_xcd_local.load() != 0 && co.count <= kSorRegColours &&
+        &local_per_cu, k_sor_xcd_local<kSorRegColours, kSorRegWidth, true>, kSorRegBlock, 0));
+    const int placement = g_sor_xcd_local.load();
+    bool local = placement != 0 && co.count <= kSorRegColours &&
                  W <= kSorRegWidth && cus % kXcds == 0;
     int local_grid = 0;
     for (int xcd = 0; xcd < kXcds && local; ++xcd) {
@@ -729,10 +734,16 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     // A regular launch: the grid is capped at what the device holds at once
     // (occupancy x CUs), and every barrier wait is bounded, so blocks that were
     // not resident turn into a timeout error rather than a hang.
-    if (local)
-        hipLaunchKernelGGL((k_sor_xcd_local<kSorRegColours, kSorRegWidth>), dim3(local_grid),
-                           dim3(kSorRegBlock), 0, s, verts, colours, ell_cols, ell_val, rhs_c,
-                           diag_c, width, omega, n, batch, max_iter, plan, x, bar, timeout);
+    if (local && placement == 2)
+        hipLaunchKernelGGL((k_sor_xcd_local<kSorRegColours, kSorRegWidth, false>),
+                           dim3(local_grid), dim3(kSorRegBlock), 0, s, verts, colours, ell_cols,
+                           ell_val, rhs_c, diag_c, width, omega, n, batch, max_iter, plan, x, bar,
+                           timeout);
+    else if (local)
+        hipLaunchKernelGGL((k_sor_xcd_local<kSorRegColours, kSorRegWidth, true>),
+                           dim3(local_grid), dim3(kSorRegBlock), 0, s, verts, colours, ell_cols,
+                           ell_val, rhs_c, diag_c, width, omega, n, batch, max_iter, plan, x, bar,
+                           timeout);
     else if (registers)
         hipLaunchKernelGGL((k_sor_registers<kSorRegColours, kSorRegWidth>), dim3(reg_grid),
                            dim3(kSorRegBlock), 0, s, verts, colours, ell_cols, ell_val, rhs_c,

@@ -125,8 +125,9 @@ def test_depth_potential_batch_matches_single(golden, cuda):
     np.testing.assert_allclose(_np(pair[0].values), g["feature"], atol=1e-6)
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("level,batch", [(3, 1), (6, 1), (6, 4), (5, 9)])
-def test_sor_xcd_local_equals_grid_barrier(level, batch, cuda):
+def test_sor_xcd_local_equals_grid_barrier(level, batch, mode, cuda):
     """Each mesh's sweeps on one XCD give exactly the device-wide-barrier values."""
 
     import cat_surface_gpu as csg
@@ -144,10 +145,10 @@ def test_sor_xcd_local_equals_grid_barrier(level, batch, cuda):
     rng = np.random.default_rng(level * 100 + batch)
     bumps = 1.0 + 0.05 * rng.standard_normal((batch, n, 1))
     pts = torch.as_tensor(v[None] * 60.0 * bumps, dtype=torch.float32, device=cuda)
-    prev = set_sor_placement(True)
+    prev = set_sor_placement(mode)
     try:
         xl, _, rl = depth_potential(st, pts, max_iter=200)
-        set_sor_placement(False)
+        set_sor_placement(0)
         xg, _, rg = depth_potential(st, pts, max_iter=200)
     finally:
         set_sor_placement(prev)
