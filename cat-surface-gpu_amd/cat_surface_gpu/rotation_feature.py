@@ -27,15 +27,19 @@ class RotationFeatureResult:
     relative_residual: float
 
 
-def set_sor_placement(xcd_local: bool) -> bool:
-    """Choose where the depth-potential SOR sweeps run; returns the previous setting.
+def set_sor_placement(mode) -> int:
+    """Choose where the depth-potential SOR sweeps run; returns the previous mode.
 
-    ``True`` (the default) keeps each mesh's rows on the workgroups of one XCD
-    so that colours synchronise through that XCD's L2; ``False`` uses one
-    device-wide grid barrier per colour.  Both compute identical values.
+    ``2`` (the default) keeps each mesh's rows on the workgroups of one XCD so
+    that colours synchronise through that XCD's L2; ``1`` does the same with x
+    stores written through to memory; ``0`` uses one device-wide grid barrier
+    per colour.  All compute identical values.
     """
 
-    return int(_lib.lib().csg_set_sor_placement(int(xcd_local)))
+    code = int(mode)
+    if code not in (0, 1, 2):
+        raise ValueError("SOR placement must be 0, 1 or 2")
+    return int(_lib.lib().csg_set_sor_placement(code))
 
 
 def depth_potential(stencil: SurfaceStencilDevice, smoothed_surface, *,

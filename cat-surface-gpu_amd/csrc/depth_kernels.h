@@ -44,9 +44,10 @@ int depth_setup(const DepthArgs &a, hipStream_t s);
 int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double omega,
               hipStream_t s);
 int depth_residual(const DepthArgs &a, double *out, hipStream_t s);
-// Process-wide: 1 (default) runs the SOR with each mesh's rows on one XCD when
-// the meshes fit, 0 always uses the device-wide grid barrier.  Returns the
-// previous setting.  Both compute identical values.
-int set_sor_xcd_local(int on);
+// Process-wide placement of the SOR sweeps: 2 (default) each mesh's rows on
+// one XCD, x stored through that XCD's L2; 1 the same with x stores written
+// through to memory; 0 the device-wide grid barrier.  1 and 2 apply when the
+// meshes fit.  Returns the previous setting.  All compute identical values.
+int set_sor_xcd_local(int mode);
 
 }  // namespace csg

@@ -434,18 +434,19 @@ k_sor_registers(const int *__restrict__ verts, SorColours co, const int *__restr
 // MI355X has eight XCDs, each with its own L2; the L2s are not coherent with
 // one another, so the grid barrier above needs agent-scope release/acquire
 // fences that write back and invalidate L2 on every colour, ~7000 times per
-// solve.  This kernel needs none.  x is only ever read and written with
-// device-scope relaxed atomic loads and stores (sc1: they bypass L1 and are
-// coherent across the device, the property that lets a relaxed poll see
-// another XCD's update); each wave waits for its x stores to complete
-// (s_waitcnt 0) before the workgroup barrier, so they are visible before
-// thread 0 bumps the arrival counter, and readers gather x only after their
-// poll has seen every arrival.  The sweeps of one mesh only touch that mesh's
-// x, so each mesh has its own barrier among the workgroups that own its rows,
-// all placed on one XCD: a short arrival chain (~13 workgroups for 40,962
-// vertices) on a counter in that XCD's L2.  (The same arrangement with cached
-// x accesses and an L1 invalidation per colour does not give the grid
-// barrier's values on MI355X, so x stays device-coherent.)
+// solve.  This kernel needs none.  The sweeps of one mesh only touch that
+// mesh's x, so each mesh's rows go to workgroups that all run on ONE XCD and
+// synchronise among themselves; within an XCD the shared L2 is the point of
+// coherence.  x is read with device-scope relaxed atomic loads (sc1), which
+// bypass the CU's L1 (which is not coherent between CUs) and are served by
+// the L2.  x is written either with plain stores (WT = false, the default:
+// L1 is write-through, the line stays in the XCD's L2) or with device-scope
+// atomic stores (WT = true: written through to memory as well).  Each wave
+// waits for its stores to complete (s_waitcnt 0) before the workgroup barrier,
+// so they are in the L2 before thread 0 bumps the mesh's arrival counter, and
+// readers gather x only after their poll has seen every arrival.  (Plain
+// loads with an L1 invalidation per colour do not give the grid barrier's
+// values on MI355X, so loads stay sc1.)
 //
 // Placement is not assumed from the dispatch order: thread 0 of every
 // workgroup reads its XCD from the XCC_ID hardware register and claims the
@@ -657,7 +658,7 @@ int depth_setup(const DepthArgs &a, hipStream_t s)
 }
 
 namespace {
-std::atomic<int> g_sor_xcd_local{1};
+std::atomic<int> g_sor_xcd_local{2};
 }
 
 int set_sor_xcd_local(int mode)

@@ -260,13 +260,15 @@ int csg_type5_sheets(const float *geom, int n, const int *faces, const int *nbr_
  * launch; the call synchronises once at the end.
  * Replaces rotation_feature._solve_depth_potential_colored_sor
  * (src/cat_surface_gpu/rotation_feature.py:253-317). */
-/* Placement of csg_depth_potential's SOR sweeps: 1 (default) puts each mesh's rows on
- * workgroups of one XCD (mesh b on XCD b % 8, claimed through the XCC_ID register) so that
- * colours synchronise through that XCD's L2 without agent-scope cache writeback and
- * invalidation, when every XCD's share fits on its CUs; 0 always uses the device-wide grid
- * barrier.  Both compute the same values in the same order.  Process-wide; returns the
+/* Placement of csg_depth_potential's SOR sweeps.  2 (default) and 1 put each mesh's rows
+ * on workgroups of one XCD (mesh b on XCD b % 8, claimed through the XCC_ID register) with
+ * one barrier per mesh, so that colours synchronise through that XCD's L2 without
+ * agent-scope cache writeback and invalidation, when every XCD's share fits on its CUs;
+ * x is read with L1-bypassing device-scope loads and written with plain stores (2) or
+ * device-scope stores written through to memory (1).  0 always uses the device-wide grid
+ * barrier.  All compute the same values in the same order.  Process-wide; returns the
  * previous setting.  No counterpart in the reference. */
-int csg_set_sor_placement(int xcd_local);
+int csg_set_sor_placement(int mode);
 size_t csg_depth_workspace_bytes(int n, int nf, int nnz, int batch, int ell_width);
 int csg_depth_potential(const float *pts, int batch, int n, const int *faces, int nf,
                         const int *inc_off, const int *inc, const int *row_off, const int *cols,
