@@ -16,6 +16,7 @@
 // Reference semantics: src/cat_surface_gpu/rotation_feature.py:24-317.
 #include "depth_kernels.h"
 #include "kernel_timing.h"
+#include "xcd_sync.h"
 
 #include <atomic>
 
@@ -431,77 +432,21 @@ k_sor_registers(const int *__restrict__ verts, SorColours co, const int *__restr
 
 // ---- XCD-local SOR
 //
-// MI355X has eight XCDs, each with its own L2; the L2s are not coherent with
-// one another, so the grid barrier above needs agent-scope release/acquire
-// fences that write back and invalidate L2 on every colour, ~7000 times per
-// solve.  This kernel needs none.  The sweeps of one mesh only touch that
-// mesh's x, so each mesh's rows go to workgroups that all run on ONE XCD and
-// synchronise among themselves; within an XCD the shared L2 is the point of
-// coherence.  x is read with device-scope relaxed atomic loads (sc1), which
-// bypass the CU's L1 (which is not coherent between CUs) and are served by
-// the L2.  x is written either with plain stores (WT = false, the default:
-// L1 is write-through, the line stays in the XCD's L2) or with device-scope
-// atomic stores (WT = true: written through to memory as well).  Each wave
-// waits for its stores to complete (s_waitcnt 0) before the workgroup barrier,
-// so they are in the L2 before thread 0 bumps the mesh's arrival counter, and
-// readers gather x only after their poll has seen every arrival.  (Plain
-// loads with an L1 invalidation per colour do not give the grid barrier's
-// values on MI355X, so loads stay sc1.)
-//
-// Placement is not assumed from the dispatch order: thread 0 of every
-// workgroup reads its XCD from the XCC_ID hardware register and claims the
-// next slot of that XCD's group.  Mesh b belongs to XCD b % 8; workgroups
-// beyond their XCD's group size exit at once.  The launch carries spare
-// workgroups per XCD, and every wait is bounded, so a placement that leaves a
-// group short ends in the timeout flag rather than a hang.
-constexpr int kXcds = 8;
-constexpr int kXcdSpare = 4;
-
-__device__ __forceinline__ int xcd_id()
-{
-    unsigned v;
-    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
-    return (int)(v & (kXcds - 1));
-}
-
-__device__ __forceinline__ double load_l2(const double *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// The grid barrier above needs agent-scope release/acquire fences that write
+// back and invalidate every L2 on each colour, ~7000 times per solve.  The
+// sweeps of one mesh only touch that mesh's x, so here each mesh's rows go to
+// a group of workgroups on one XCD that synchronise through its L2
+// (xcd_sync.h): x is gathered with L1-bypassing device-scope loads and written
+// with plain stores (WT = false, the default) or with device-scope stores that
+// also write through to memory (WT = true).
 
 template <bool WRITE_THROUGH>
-__device__ __forceinline__ void store_l2(double *p, double v)
+__device__ __forceinline__ void store_x(double *p, double v)
 {
     if (WRITE_THROUGH)
         __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else
         *p = v;
-}
-
-// Barrier of the workgroups of one XCD group.  The arrival counter only grows:
-// phase p is complete when it reaches p * nblocks (modular arithmetic), so it
-// needs no reset and no generation word.
-__device__ bool xcd_barrier(unsigned *cnt, unsigned target, int *timeout)
-{
-    __shared__ int ok;
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's x stores are in L2
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        ok = 1;
-        __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        long spins = 0;
-        while ((int)(__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
-                     target) < 0) {
-            if (++spins > kSpinLimit) {
-                __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    __syncthreads();
-    return ok != 0;
 }
 
 // k_sor_registers with each mesh's rows on one XCD.  blocks[x] workgroups of
@@ -515,15 +460,8 @@ k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restr
                 const double *__restrict__ diag_c, int W, double omega, int n, int batch,
                 int max_iter, SorXcdPlan plan, double *x, unsigned *bar, int *timeout)
 {
-    __shared__ int s_xcd, s_slot;
-    if (threadIdx.x == 0) {
-        const int xcd = xcd_id();
-        s_xcd = xcd;
-        s_slot = (int)__hip_atomic_fetch_add(bar + (size_t)xcd * kBarrierSlotWords, 1u,
-                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    const int xcd = s_xcd, slot = s_slot;
+    int xcd, slot;
+    xcd_claim(bar, kBarrierSlotWords, xcd, slot);
     const int nblocks = plan.blocks[xcd];
     if (slot >= nblocks)
         return;
@@ -577,10 +515,10 @@ k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restr
                 for (int k = 0; k < WMAX; ++k)
                     if (k < W)
                         s += val[c][k] * load_l2(x + col[c][k]);
-                store_l2<WT>(x + vi[c], load_l2(x + vi[c]) + omega * (rh[c] - s) / dg[c]);
+                store_x<WT>(x + vi[c], load_l2(x + vi[c]) + omega * (rh[c] - s) / dg[c]);
             }
             target += (unsigned)nblocks;
-            if (!xcd_barrier(cnt, target, timeout))
+            if (!group_barrier(cnt, target, timeout))
                 return;
         }
     }
