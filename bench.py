@@ -88,6 +88,12 @@ def run_step(pairs, dtype: str, mode: str, device, cuda_graph: bool):
             for m in pairs]
 
 
+def _installed_tiles() -> list[dict]:
+    from cat_surface_gpu import autotune
+
+    return autotune.installed()
+
+
 def kernel_rooflines() -> list[dict]:
     """Totals of the library's HIP-event kernel timers since they were enabled:
     for each timed kernel, its launches, average launch duration and the
@@ -427,6 +433,7 @@ def main() -> int:
                                "avg, 512x256 map", "mode": args.mode,
                    "parallelism": "single GPU", "cuda_graph": args.cuda_graph},
         "plain": plain,
+        "relax_tiles": _installed_tiles(),
         "roofline": roof,
         "roofline_kernels": kernels,
         "cpu_baseline": cpu,

@@ -33,6 +33,7 @@ CANDIDATES: tuple[tuple[int, int], ...] = (
     (32, 16), (64, 8), (16, 16), (32, 8), (64, 16), (16, 32), (128, 4), (32, 32), (16, 8))
 
 _TUNED: dict[tuple, dict[tuple[int, int], tuple[int, int]]] = {}
+_SOURCE: dict[tuple, str] = {}
 
 
 def fine_levels(nx: int, ny: int, dtype: torch.dtype, nit: int = 3) -> list[tuple[int, int]]:
@@ -169,14 +170,25 @@ def ensure_tuned(shapes: Iterable[tuple[int, int]], dtype: torch.dtype, batch: i
                           for k, s in entry["levels"].items()}
                 for (lx, ly), shape in levels.items():
                     set_tile(lx, ly, dtype, shape)
+                _SOURCE[key] = "cache"
             else:
                 res = tune_relax_tiles(nx, ny, dtype, batch, device=dev)
                 levels = res["levels"]
                 cache[ckey] = {"levels": {f"{a}x{b}": list(s) for (a, b), s in levels.items()},
                                "ms": res["ms"], "default_ms": res["default_ms"]}
                 changed = True
+                _SOURCE[key] = "tuned"
             _TUNED[key] = levels
         out[(nx, ny)] = _TUNED[key]
     if changed and use_cache:
         _store_cache(cache)
     return out
+
+
+def installed() -> list[dict]:
+    """The tile shapes ensure_tuned installed in this process, with their source."""
+
+    return [{"map": f"{k[1]}x{k[2]}", "dtype": k[3].replace("torch.", ""), "batch": k[4],
+             "source": _SOURCE.get(k, "?"),
+             "levels": {f"{a}x{b}": list(v) for (a, b), v in levels.items()}}
+            for k, levels in _TUNED.items()]

@@ -125,14 +125,6 @@ int csg_smooth_geometry(const float *v, const int *nbr_off, const int *nbr, int 
 
 // batch meshes [batch][n][3] of one topology, mesh b smoothed iterations[b]
 // (host array) times; ws: 3 n batch floats.
-size_t csg_smooth_geometry_batch_workspace_bytes(int n, int batch)
-{
-    if (n < 1 || batch < 1)
-        return 0;
-    return ((size_t)3 * n * batch * sizeof(float) + 255) / 256 * 256 +
-           smooth_xcd_barrier_words() * sizeof(unsigned);
-}
-
 int csg_smooth_geometry_batch(const float *v, int batch, const int *nbr_off, const int *nbr, int n,
                               const int *iterations, float *out, void *ws, size_t ws_bytes,
                               void *stream)
@@ -142,8 +134,6 @@ int csg_smooth_geometry_batch(const float *v, int batch, const int *nbr_off, con
                     kMaxSmoothBatch);
     if (ws_bytes < (size_t)3 * n * batch * sizeof(float))
         return fail(kInvalidArgument, "smooth_geometry_batch workspace too small");
-    // with room for the barrier words, all iterations run in one launch
-    const bool xcd_room = ws_bytes >= csg_smooth_geometry_batch_workspace_bytes(n, batch);
     SmoothBatch sb{};
     sb.count = batch;
     int most = 0;
@@ -160,21 +150,6 @@ int csg_smooth_geometry_batch(const float *v, int batch, const int *nbr_off, con
     for (int b = 0; b < batch; ++b)
         CSG_HIP_CHECK(hipMemcpyAsync(bufs[iterations[b] & 1] + b * mesh, v + b * mesh,
                                      mesh * sizeof(float), hipMemcpyDeviceToDevice, s));
-    if (xcd_room && most > 1) {
-        unsigned *bar = (unsigned *)((char *)ws + ((size_t)3 * n * batch * sizeof(float) + 255) /
-                                                      256 * 256);
-        bool used = false;
-        CSG_TRY(launch_smooth_geometry_xcd(bufs[0], bufs[1], nbr_off, nbr, n, sb, bar, s, &used));
-        if (used) {
-            int timeout = 0;
-            const unsigned *flag = bar + smooth_xcd_barrier_words() - 32;
-            CSG_HIP_CHECK(hipMemcpyAsync(&timeout, flag, sizeof(int), hipMemcpyDeviceToHost, s));
-            CSG_HIP_CHECK(hipStreamSynchronize(s));
-            if (timeout)
-                return fail(kRuntimeError, "geometry smoothing group barrier timed out");
-            return kOk;
-        }
-    }
     for (int it = 0; it < most; ++it)
         CSG_TRY(launch_smooth_geometry_batch(bufs[0], bufs[1], nbr_off, nbr, n, sb, it, s));
     return kOk;

@@ -31,13 +31,6 @@ int launch_smooth_geometry(const float *cur, const int *off, const int *nbr, int
                            hipStream_t s);
 int launch_smooth_geometry_batch(float *buf0, float *buf1, const int *off, const int *nbr,
                                  int n, const SmoothBatch &sb, int it, hipStream_t s);
-// All iterations of a batched geometry smoothing in one launch, each mesh on
-// one XCD (xcd_sync.h); bar holds smooth_xcd_barrier_words() words (cleared
-// here); the last word group holds the timeout flag.  *used is false when the
-// device does not allow this placement (then nothing was launched).
-size_t smooth_xcd_barrier_words();
-int launch_smooth_geometry_xcd(float *buf0, float *buf1, const int *off, const int *nbr, int n,
-                               const SmoothBatch &sb, unsigned *bar, hipStream_t s, bool *used);
 int launch_vertex_normals(const float *v, const int *faces, const int *inc_off, const int *inc,
                           int n, float *out, hipStream_t s);
 int launch_type5_base(const float *geom, const float *sm, const float *nr, int n, double *out,

@@ -13,7 +13,6 @@
 // per-vertex list of incident (face, corner) entries sorted in the order the
 // reference adds them, so every sum is deterministic and atomic-free.
 #include "surface_kernels.h"
-#include "xcd_sync.h"
 
 namespace csg {
 
@@ -122,62 +121,6 @@ k_smooth_geometry_batch(float *__restrict__ buf0, float *__restrict__ buf1,
     nxt[3 * i] = (float)(((double)cur[3 * i] + sx / 3.0) / den);
     nxt[3 * i + 1] = (float)(((double)cur[3 * i + 1] + sy / 3.0) / den);
     nxt[3 * i + 2] = (float)(((double)cur[3 * i + 2] + sz / 3.0) / den);
-}
-
-// All iterations of k_smooth_geometry_batch in one launch: mesh b is smoothed
-// by a group of `group_blocks` workgroups on XCD b % 8 that synchronise
-// through its L2 between iterations (xcd_sync.h); workgroup slot s of an XCD
-// is member s % group_blocks of its (s / group_blocks)-th mesh.  Same
-// arithmetic per vertex and iteration as k_smooth_geometry_batch.  Barrier
-// words: slot x (kBarrierStride words each) holds XCD x's claim counter, slot
-// 8 + b mesh b's arrival counter; `timeout` is set if a wait gives up.
-constexpr int kXcdSmoothBlock = 1024;
-constexpr int kBarrierStride = 32;
-
-__global__ void __launch_bounds__(kXcdSmoothBlock)
-k_smooth_geometry_xcd(float *__restrict__ buf0, float *__restrict__ buf1,
-                      const int *__restrict__ off, const int *__restrict__ nbr, int n,
-                      SmoothBatch sb, int group_blocks, unsigned *bar, int *timeout)
-{
-    int xcd, slot;
-    xcd_claim(bar, kBarrierStride, xcd, slot);
-    const int j = slot / group_blocks, member = slot - j * group_blocks;
-    const int b = xcd + kXcds * j;
-    if (b >= sb.count)
-        return;
-    unsigned *cnt = bar + (size_t)(kXcds + b) * kBarrierStride;
-    const int iters = sb.iters[b];
-    const int stride = group_blocks * blockDim.x;
-    unsigned target = 0;
-    for (int it = 0; it < iters; ++it) {
-        const bool from1 = ((iters + it) & 1) != 0;
-        const float *cur = (from1 ? buf1 : buf0) + (size_t)b * 3 * n;
-        float *nxt = (from1 ? buf0 : buf1) + (size_t)b * 3 * n;
-        for (int i = member * blockDim.x + threadIdx.x; i < n; i += stride) {
-            const int bo = off[i], e = off[i + 1];
-            double sx = 0.0, sy = 0.0, sz = 0.0;
-            for (int k = bo; k < e; ++k) {
-                const float *p = cur + 3 * nbr[k];
-                sx += (double)load_l2(p);
-                sy += (double)load_l2(p + 1);
-                sz += (double)load_l2(p + 2);
-            }
-            const double den = 1.0 + (double)(e - bo) / 3.0;
-            nxt[3 * i] = (float)(((double)load_l2(cur + 3 * i) + sx / 3.0) / den);
-            nxt[3 * i + 1] = (float)(((double)load_l2(cur + 3 * i + 1) + sy / 3.0) / den);
-            nxt[3 * i + 2] = (float)(((double)load_l2(cur + 3 * i + 2) + sz / 3.0) / den);
-        }
-        target += (unsigned)group_blocks;
-        if (!group_barrier(cnt, target, timeout))
-            return;
-    }
-}
-
-__global__ void k_clear_words(unsigned *w, int n)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        w[i] = 0u;
 }
 
 // Angle-weighted vertex normals (float32).  inc[] lists face*3+corner per
@@ -527,40 +470,6 @@ int launch_smooth_geometry_batch(float *buf0, float *buf1, const int *off, const
     hipLaunchKernelGGL(k_smooth_geometry_batch, dim3(grid_for(n), sb.count), dim3(kBlock), 0, s,
                        buf0, buf1, off, nbr, n, sb, it);
     CSG_LAUNCH_CHECK();
-    return kOk;
-}
-
-size_t smooth_xcd_barrier_words() { return (size_t)(kXcds + kMaxSmoothBatch + 1) * kBarrierStride; }
-
-int launch_smooth_geometry_xcd(float *buf0, float *buf1, const int *off, const int *nbr, int n,
-                               const SmoothBatch &sb, unsigned *bar, hipStream_t s, bool *used)
-{
-    *used = false;
-    int dev = 0, cus = 0, per_cu = 0;
-    CSG_HIP_CHECK(hipGetDevice(&dev));
-    CSG_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_smooth_geometry_xcd,
-                                                               kXcdSmoothBlock, 0));
-    if (cus % kXcds != 0 || per_cu < 1)
-        return kOk;
-    const int meshes_per_xcd = (sb.count + kXcds - 1) / kXcds;
-    const int capacity = (cus / kXcds) * per_cu;  // resident workgroups per XCD
-    // about four vertices per thread, within what one XCD holds at once
-    int group = (int)((n + 4 * kXcdSmoothBlock - 1) / (4 * kXcdSmoothBlock));
-    group = group < 1 ? 1 : group;
-    if (group * meshes_per_xcd > capacity)
-        group = capacity / meshes_per_xcd;
-    if (group < 1)
-        return kOk;
-    const int words = (int)smooth_xcd_barrier_words();
-    hipLaunchKernelGGL(k_clear_words, dim3(grid_for(words)), dim3(kBlock), 0, s, bar, words);
-    CSG_LAUNCH_CHECK();
-    int *timeout = reinterpret_cast<int *>(bar + (size_t)(kXcds + kMaxSmoothBatch) * kBarrierStride);
-    const int grid = (group * meshes_per_xcd + kXcdSpare) * kXcds;
-    hipLaunchKernelGGL(k_smooth_geometry_xcd, dim3(grid), dim3(kXcdSmoothBlock), 0, s, buf0, buf1,
-                       off, nbr, n, sb, group, bar, timeout);
-    CSG_LAUNCH_CHECK();
-    *used = true;
     return kOk;
 }
 

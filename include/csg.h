@@ -207,13 +207,8 @@ int csg_edge_sum(const float *v, const int *faces, int nf, double *partial_out, 
 int csg_smooth_geometry(const float *v, const int *nbr_off, const int *nbr, int n,
                         int iterations, float *out, void *ws, size_t ws_bytes, void *stream);
 /* `batch` meshes [batch][n][3] of one topology, mesh b smoothed iterations[b] (host array)
- * times (batch <= 32).  ws needs 3 n batch floats; given
- * csg_smooth_geometry_batch_workspace_bytes, all iterations run in one launch with each
- * mesh on one XCD, synchronising through its L2, and the call synchronises the stream
- * once to check that launch; otherwise one launch per iteration, asynchronously.  Same
- * values either way.  Replaces repeated SurfaceStencilDevice.smooth_geometry calls
- * (surface_stencil.py:372-388). */
-size_t csg_smooth_geometry_batch_workspace_bytes(int n, int batch);
+ * times, all in the same launches (batch <= 32; ws: 3 n batch floats).  Replaces repeated
+ * SurfaceStencilDevice.smooth_geometry calls (surface_stencil.py:372-388). */
 int csg_smooth_geometry_batch(const float *v, int batch, const int *nbr_off, const int *nbr, int n,
                               const int *iterations, float *out, void *ws, size_t ws_bytes,
                               void *stream);
