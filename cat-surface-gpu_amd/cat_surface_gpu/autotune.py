@@ -156,7 +156,8 @@ def ensure_tuned(shapes: Iterable[tuple[int, int]], dtype: torch.dtype, batch: i
     """Tile shapes for every (nx, ny) map size, tuned at most once per process."""
 
     dev = torch.device(device)
-    name = torch.cuda.get_device_name(dev)
+    props = torch.cuda.get_device_properties(dev)
+    name = f"{getattr(props, 'gcnArchName', props.name)}:{props.multi_processor_count}cu"
     cache = _load_cache() if use_cache else {}
     changed = False
     out = {}
