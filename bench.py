@@ -31,6 +31,14 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of kernels whose traffic counters were collected at the
+# bench's shape: two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE), each scaled
+# by what the counter reads for a 64 MiB copy.  Source: the named profile.
+MEASURED_TRAFFIC = {
+    "k_sor_xcd_local": (56.63e6, "profiles/k_sor_xcd_local_depth4x40962.md "
+                                 "(four 40,962-vertex meshes, 1000 sweeps)"),
+}
+
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
@@ -116,9 +124,12 @@ def kernel_rooflines() -> list[dict]:
                                               ctypes.byref(launches)), "kernel timing")
         if launches.value:
             achieved = nbytes.value / (ms.value / 1e3) / 1e9
+            traffic = MEASURED_TRAFFIC.get(name.decode())
             out.append({"kernel": name.decode(), "bound": "hbm", "achieved": round(achieved, 2),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                        "frac": round(achieved / HBM_PEAK_GBS, 5),
+                        "traffic": traffic[0] if traffic else None,
+                        "traffic_source": traffic[1] if traffic else None,
                         "launches": launches.value, "total_ms": round(ms.value, 3),
                         "avg_launch_us": round(ms.value * 1e3 / launches.value, 3),
                         "algorithmic_bytes_per_launch": round(nbytes.value / launches.value)})
