@@ -75,6 +75,7 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_edge_mean_workspace_bytes": (_sz, [_i]),
     "csg_edge_sum": (_i, [_vp, _vp, _i, _vp, ctypes.POINTER(_i), _vp]),
     "csg_smooth_geometry": (_i, [_vp, _vp, _vp, _i, _i, _vp, _vp, _sz, _vp]),
+    "csg_smooth_geometry_batch_workspace_bytes": (_sz, [_i, _i]),
     "csg_smooth_geometry_batch": (_i, [_vp, _i, _vp, _vp, _i, ctypes.POINTER(_i), _vp, _vp, _sz,
                                        _vp]),
     "csg_vertex_normals": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp]),

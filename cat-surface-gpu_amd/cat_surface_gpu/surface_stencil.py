@@ -541,11 +541,12 @@ def smooth_geometry_many(stencils, geometries, iterations) -> list[torch.Tensor]
                              for i in chunk]).contiguous()
             n = int(v.shape[1])
             res = torch.empty_like(v)
-            ws = torch.empty_like(v)
+            ws = _lib.workspace(_lib.lib().csg_smooth_geometry_batch_workspace_bytes(
+                n, len(chunk)), dev)
             its = (_lib.ctypes.c_int * len(chunk))(*[int(iterations[i]) for i in chunk])
             _lib.check(_lib.lib().csg_smooth_geometry_batch(
                 _lib.ptr(v), len(chunk), _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), n, its,
-                _lib.ptr(res), _lib.ptr(ws), ws.numel() * 4, stencils[chunk[0]]._stream()),
+                _lib.ptr(res), _lib.ptr(ws), ws.numel(), stencils[chunk[0]]._stream()),
                 "csg_smooth_geometry_batch")
             for k, i in enumerate(chunk):
                 out[i] = res[k]

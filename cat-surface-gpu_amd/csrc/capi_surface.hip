@@ -125,6 +125,13 @@ int csg_smooth_geometry(const float *v, const int *nbr_off, const int *nbr, int 
 
 // batch meshes [batch][n][3] of one topology, mesh b smoothed iterations[b]
 // (host array) times; ws: 3 n batch floats.
+size_t csg_smooth_geometry_batch_workspace_bytes(int n, int batch)
+{
+    if (n < 1 || batch < 1)
+        return 0;
+    return (size_t)2 * n * batch * sizeof(float4);
+}
+
 int csg_smooth_geometry_batch(const float *v, int batch, const int *nbr_off, const int *nbr, int n,
                               const int *iterations, float *out, void *ws, size_t ws_bytes,
                               void *stream)
@@ -144,6 +151,11 @@ int csg_smooth_geometry_batch(const float *v, int batch, const int *nbr_off, con
         most = iterations[b] > most ? iterations[b] : most;
     }
     hipStream_t s = S(stream);
+    if (ws_bytes >= csg_smooth_geometry_batch_workspace_bytes(n, batch)) {
+        float4 *b0 = (float4 *)ws;
+        return launch_smooth_geometry4(v, b0, b0 + (size_t)n * batch, nbr_off, nbr, n, sb, most,
+                                       out, s);
+    }
     float *bufs[2] = {out, (float *)ws};
     const size_t mesh = (size_t)3 * n;
     // start each mesh in the buffer from which its last step writes `out`

@@ -31,6 +31,11 @@ int launch_smooth_geometry(const float *cur, const int *off, const int *nbr, int
                            hipStream_t s);
 int launch_smooth_geometry_batch(float *buf0, float *buf1, const int *off, const int *nbr,
                                  int n, const SmoothBatch &sb, int it, hipStream_t s);
+// The batched smoothing on padded float4 copies (buf0, buf1: batch x n each):
+// pack v, `most` steps, unpack each mesh's last iterate into out.
+int launch_smooth_geometry4(const float *v, float4 *buf0, float4 *buf1, const int *off,
+                            const int *nbr, int n, const SmoothBatch &sb, int most, float *out,
+                            hipStream_t s);
 int launch_vertex_normals(const float *v, const int *faces, const int *inc_off, const int *inc,
                           int n, float *out, hipStream_t s);
 int launch_type5_base(const float *geom, const float *sm, const float *nr, int n, double *out,

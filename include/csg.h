@@ -207,8 +207,11 @@ int csg_edge_sum(const float *v, const int *faces, int nf, double *partial_out, 
 int csg_smooth_geometry(const float *v, const int *nbr_off, const int *nbr, int n,
                         int iterations, float *out, void *ws, size_t ws_bytes, void *stream);
 /* `batch` meshes [batch][n][3] of one topology, mesh b smoothed iterations[b] (host array)
- * times, all in the same launches (batch <= 32; ws: 3 n batch floats).  Replaces repeated
+ * times, all in the same launches (batch <= 32).  ws needs at least 3 n batch floats;
+ * with csg_smooth_geometry_batch_workspace_bytes it runs on padded float4 copies (one
+ * 16-byte load per neighbour), same values.  Replaces repeated
  * SurfaceStencilDevice.smooth_geometry calls (surface_stencil.py:372-388). */
+size_t csg_smooth_geometry_batch_workspace_bytes(int n, int batch);
 int csg_smooth_geometry_batch(const float *v, int batch, const int *nbr_off, const int *nbr, int n,
                               const int *iterations, float *out, void *ws, size_t ws_bytes,
                               void *stream);

@@ -370,3 +370,36 @@ def test_adjoint_rotation_search_on_reference_problem(golden, cuda):
     adj = search_rotations([(idx, inputs)], method="adjoint")[0]
     assert float(adj.cost) <= float(g["nm_cost"]) * (1.0 + 1e-6)
     np.testing.assert_allclose(_np(adj.seed_angle), g["seed_angle"])
+
+
+@pytest.mark.parametrize("level,iterations", [(6, (50, 13, 0, 7)), (3, (5, 2)), (5, tuple(range(9)))])
+def test_geometry_smoothing_padded_equals_packed(level, iterations, cuda):
+    """The padded float4 schedule (full workspace) gives the bits of the packed one."""
+
+    from cat_surface_gpu import _lib
+    from cat_surface_gpu.stencil_builder import icosphere
+    from cat_surface_gpu.surface_stencil import topology_for
+
+    v, f = icosphere(level)
+    n = v.shape[0]
+    topo = topology_for(f.astype(np.int32), n, cuda)
+    rng = np.random.default_rng(level)
+    batch = len(iterations)
+    geom = torch.as_tensor(v[None] * 50.0 * (1.0 + 0.05 * rng.standard_normal((batch, n, 1))),
+                           dtype=torch.float32, device=cuda).contiguous()
+    its = (_lib.ctypes.c_int * batch)(*iterations)
+    lib = _lib.lib()
+    outs = []
+    for nbytes in (lib.csg_smooth_geometry_batch_workspace_bytes(n, batch), 3 * n * batch * 4):
+        ws = _lib.workspace(nbytes, cuda)
+        out = torch.empty_like(geom)
+        _lib.check(lib.csg_smooth_geometry_batch(
+            _lib.ptr(geom), batch, _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), n, its,
+            _lib.ptr(out), _lib.ptr(ws), nbytes, _lib.stream_handle(cuda)),
+            "csg_smooth_geometry_batch")
+        outs.append(out)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1])
+    for b, k in enumerate(iterations):
+        if k == 0:
+            assert torch.equal(outs[0][b], geom[b])
