@@ -405,7 +405,6 @@ k_rotation_cost_mats(const float *__restrict__ src_pts, const double *__restrict
                      double *__restrict__ partial, int nblocks, int *__restrict__ uncovered)
 {
     constexpr int kTerms = GRAD ? kRotationGradTerms : 1;
-    __shared__ double red[kBlock / 64];
     __shared__ float m[9];
     const int a = blockIdx.y;
     if (threadIdx.x < 9)
@@ -467,11 +466,24 @@ k_rotation_cost_mats(const float *__restrict__ src_pts, const double *__restrict
             }
         }
     }
+    // all terms reduced together, in block_sum's order: a shuffle tree per
+    // wave, then the waves' sums in order (one barrier instead of one per term)
+    __shared__ double wave_sums[kBlock / 64][kTerms];
+    const int wlane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < kTerms; ++k) {
-        double sum = block_sum(term[k], red);
-        if (threadIdx.x == 0)
-            partial[((size_t)a * kTerms + k) * nblocks + blockIdx.x] = sum;
+        double v = term[k];
+        for (int off = 32; off > 0; off >>= 1)
+            v += __shfl_down(v, off, 64);
+        if (wlane == 0)
+            wave_sums[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kTerms) {
+        double sum = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+            sum += wave_sums[w][threadIdx.x];
+        partial[((size_t)a * kTerms + threadIdx.x) * nblocks + blockIdx.x] = sum;
     }
 }
 
