@@ -80,30 +80,30 @@ def write_gifti_mesh(path: str | Path, mesh: GiftiMesh, *,
                         triangle_intent=triangle_intent)
 
 
-def _normalise_sphere(vertices: np.ndarray) -> np.ndarray:
-    """Sequential float64 centroid, per-vertex float32 unit vectors (:274-305)."""
+def _prepare_sphere(vertices: torch.Tensor, *, normalise: bool, angles=None,
+                    bad: torch.Tensor | None = None) -> torch.Tensor:
+    """The pipeline's sphere preparation (gpu_pipeline.py:274-305, :373-378) on the
+    device (csg_prepare_sphere): with ``normalise``, centre on the sequential
+    float64 centroid and scale every vertex to unit length in float32 (a vertex
+    that cannot be normalised sets ``bad``); with ``angles``, then rotate by
+    p · Mᵀ in float64, stored float32.  Asynchronous."""
 
-    value = np.ascontiguousarray(np.asarray(vertices, dtype=np.float32))
-    if value.ndim != 2 or value.shape[1] != 3:
-        raise ValueError(f"Sphere vertices must have shape [n, 3], got {value.shape}")
-    centre = np.cumsum(value.astype(np.float64), axis=0)[-1]
-    centred = (value.astype(np.float64) - centre / value.shape[0]).astype(np.float32)
-    c64 = centred.astype(np.float64)
-    length = np.sqrt((c64 * c64).sum(1))
-    if not np.all(np.isfinite(length)) or np.any(length == 0.0):
+    if vertices.ndim != 2 or vertices.shape[1] != 3:
+        raise ValueError(f"Sphere vertices must have shape [n, 3], got {tuple(vertices.shape)}")
+    v = vertices.to(torch.float32).contiguous()
+    out = torch.empty_like(v)
+    ws = torch.empty(4, dtype=torch.float64, device=v.device)
+    rot = None if angles is None else _lib.doubles(rotation_matrix_np(angles).reshape(-1))
+    _lib.check(_lib.lib().csg_prepare_sphere(_lib.ptr(v), int(v.shape[0]), int(normalise), rot,
+                                             _lib.ptr(out), _lib.ptr(ws), _lib.ptr(bad),
+                                             _lib.stream_handle(v.device)),
+               "csg_prepare_sphere")
+    return out
+
+
+def _check_normalised(bad: torch.Tensor) -> None:
+    if int(bad.item()):
         raise ValueError("Sphere contains a vertex that cannot be normalized")
-    return np.ascontiguousarray((c64 * (1.0 / length)[:, None]).astype(np.float32))
-
-
-def _rotation_matrix_from_angles(angles) -> np.ndarray:
-    return rotation_matrix_np(angles)
-
-
-def _rotate_vertices(vertices: np.ndarray, angles) -> np.ndarray:
-    """p · Mᵀ in float64, stored float32 (:373-378)."""
-
-    m = rotation_matrix_np(angles)
-    return np.ascontiguousarray((np.asarray(vertices, np.float64) @ m.T).astype(np.float32))
 
 
 def _resolve_dartel_dtype(value) -> torch.dtype:
@@ -228,12 +228,12 @@ class _JobState:
     rotation_cost: float = 0.0
     current_sphere: np.ndarray | None = None
     cur_stencil: object = None
-    avg_tgt_surface: np.ndarray | None = None
+    avg_tgt_surface: object = None  # device, float32
     avg_tgt_stencil: object = None
-    avg_src_surface: np.ndarray | None = None
-    avg_src_sphere: np.ndarray | None = None
+    avg_src_surface: object = None  # device, float32
+    avg_src_sphere: object = None  # device, float32
     avg_src_stencil: object = None
-    final_prewarp: np.ndarray | None = None
+    final_prewarp: object = None  # device, float32
     last_flow: object = None
     avg_flow: object = None
     src_dev: object = None
@@ -241,6 +241,8 @@ class _JobState:
     src_faces_dev: object = None
     tgt_faces_dev: object = None
     current_dev: object = None  # device copy of current_sphere (float32)
+    src_sphere_dev: object = None
+    tgt_sphere_dev: object = None
 
     def current_host(self) -> np.ndarray:
         if self.current_sphere is None:
@@ -339,6 +341,8 @@ def run_cat_surface_gpu_pipelines(
                     raise FileNotFoundError(f"{label} does not exist: {path}")
                 return SurfaceStencil.from_file(path).to(dev)
             if builder is not None:
+                if isinstance(sphere_vertices, torch.Tensor):
+                    sphere_vertices = sphere_vertices.cpu().numpy()
                 if sphere_path is None:
                     sphere_path = tmp / f"job{index}-{label}.gii"
                     write_gifti_surface(sphere_path, sphere_vertices, faces)
@@ -362,6 +366,10 @@ def run_cat_surface_gpu_pipelines(
                                      "target", True, st.job.target_stencil, stencil_builder)
             st.src_dev = torch.as_tensor(st.src_surface.vertices, device=dev)
             st.tgt_dev = torch.as_tensor(st.tgt_surface.vertices, device=dev)
+            st.src_sphere_dev = torch.as_tensor(st.src_sphere.vertices, dtype=torch.float32,
+                                                device=dev)
+            st.tgt_sphere_dev = torch.as_tensor(st.tgt_sphere.vertices, dtype=torch.float32,
+                                                device=dev)
         torch.cuda.synchronize(dev)
         shared["initial_stencil_seconds"] = time.perf_counter() - t
 
@@ -452,17 +460,22 @@ def run_cat_surface_gpu_pipelines(
 
         t = time.perf_counter()
         builder = rotated_stencil_builder
+        # normalise and rotate on the device (gpu_pipeline.py:274-305, :373-378)
+        bad = torch.zeros(1, dtype=torch.int32, device=dev)
         for i, st in enumerate(states):
-            st.current_sphere = _rotate_vertices(_normalise_sphere(st.src_sphere.vertices),
-                                                 st.angle)
-            st.avg_tgt_surface = _rotate_vertices(st.tgt_surface.vertices, pole)
-            avg_tgt_sphere = _rotate_vertices(_normalise_sphere(st.tgt_sphere.vertices), pole)
-            st.cur_stencil = stencil(st, i, st.current_sphere, None, "rotated-000", False,
+            st.current_dev = _prepare_sphere(st.src_sphere_dev, normalise=True, angles=st.angle,
+                                             bad=bad)
+            st.current_sphere = None
+            st.avg_tgt_surface = _prepare_sphere(st.tgt_dev, normalise=False, angles=pole)
+            avg_tgt_sphere = _prepare_sphere(st.tgt_sphere_dev, normalise=True, angles=pole,
+                                             bad=bad)
+            st.cur_stencil = stencil(st, i, st.current_dev, None, "rotated-000", False,
                                      builder=builder)
             if avg:
                 st.avg_tgt_stencil = stencil(st, i, avg_tgt_sphere, None, "avg-target", False,
                                              builder=builder)
         torch.cuda.synchronize(dev)
+        _check_normalised(bad)
         shared["rotated_stencil_seconds"] = time.perf_counter() - t
 
         solve_kw = dict(n_steps=steps, loop=loop, cycles=cycles, nit=nit, its=its, code=code,
@@ -489,9 +502,12 @@ def run_cat_surface_gpu_pipelines(
             requests, owners = [], []
             for i, st in enumerate(states):
                 if final_run:
-                    st.final_prewarp = st.current_host().copy()
-                    st.avg_src_surface = _rotate_vertices(st.src_surface.vertices, pole)
-                    st.avg_src_sphere = _rotate_vertices(st.final_prewarp, pole)
+                    st.final_prewarp = (st.current_dev.clone() if st.current_dev is not None
+                                        else torch.as_tensor(st.current_host(), device=dev))
+                    st.avg_src_surface = _prepare_sphere(st.src_dev, normalise=False,
+                                                         angles=pole)
+                    st.avg_src_sphere = _prepare_sphere(st.final_prewarp, normalise=False,
+                                                        angles=pole)
                     st.avg_src_stencil = stencil(st, i, st.avg_src_sphere, None, "avg-source",
                                                  False, builder=builder)
                 requests.append((st.src_dev, st.tgt_dev, st.cur_stencil, st.tgt_stencil,

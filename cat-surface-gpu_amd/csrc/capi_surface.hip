@@ -402,6 +402,14 @@ int csg_normalise_points(const float *in, int n, float *out, void *stream)
     return launch_normalise_f32(in, n, out, S(stream));
 }
 
+int csg_prepare_sphere(const float *in, int n, int normalise, const double *rotation,
+                       float *out, double *ws, int *bad, void *stream)
+{
+    if (n < 1 || in == nullptr || out == nullptr || (normalise && (ws == nullptr || !bad)))
+        return fail(kInvalidArgument, "invalid sphere preparation arguments");
+    return launch_sphere_prepare(in, n, normalise, rotation, out, ws, bad, S(stream));
+}
+
 int csg_average_xz(const void *avg, const void *main_, int n, int dtype, const double *unrotate,
                    float *out, void *stream)
 {

@@ -58,5 +58,14 @@ template <typename T>
 int launch_average_xz(const T *avg, const T *main_, int n, const double *m, float *out,
                       hipStream_t s);
 int launch_normalise_f32(const float *in, int n, float *out, hipStream_t s);
+struct Mat3 {
+    double a[9];
+};
+// normalise: sequential float64 centroid (centre: 3 doubles of scratch), then
+// float32 unit vectors; m_host (row-major, may be null): then p · mᵀ in
+// float64, stored float32.  *bad is set to 1 for a vertex of zero or
+// non-finite length (the caller clears it).
+int launch_sphere_prepare(const float *v, int n, int normalise, const double *m_host, float *out,
+                          double *centre, int *bad, hipStream_t s);
 
 }  // namespace csg

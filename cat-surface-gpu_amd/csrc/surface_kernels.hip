@@ -477,7 +477,80 @@ k_normalise_f32(const float *__restrict__ in, int n, float *__restrict__ out)
     out[3 * i + 2] = z / nr;
 }
 
+// Sphere preparation of the pipeline (gpu_pipeline.py:274-305, :373-378).
+// The centroid is a strictly sequential float64 sum in vertex order (the
+// reference's loop; one lane per coordinate): centre[c] = Σ_i f64(v[i][c]).
+__global__ void k_sequential_centroid(const float *__restrict__ v, int n,
+                                      double *__restrict__ centre)
+{
+    const int c = threadIdx.x;
+    if (c >= 3)
+        return;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i)
+        s += (double)v[3 * i + c];
+    centre[c] = s;
+}
+
+// Per vertex: with `centre`, p = f32(f64(v) - centre / n), then
+// f32(f64(p) / |f64(p)|) (flagging zero or non-finite lengths in *bad);
+// with `m` (row-major 3x3), f32(f64(p) · mᵀ).
+__global__ void __launch_bounds__(kBlock)
+k_normalise_rotate(const float *__restrict__ v, int n, const double *__restrict__ centre,
+                   Mat3 m, int rotate, float *__restrict__ out, int *__restrict__ bad)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float x = v[3 * i], y = v[3 * i + 1], z = v[3 * i + 2];
+    if (centre) {
+        const double dn = (double)n;
+        const float ex = (float)((double)x - centre[0] / dn);
+        const float ey = (float)((double)y - centre[1] / dn);
+        const float ez = (float)((double)z - centre[2] / dn);
+        const double c0 = ex, c1 = ey, c2 = ez;
+        const double len = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+        if (!(len > 0.0) || !isfinite(len))
+            *bad = 1;
+        const double inv = 1.0 / len;
+        x = (float)(c0 * inv);
+        y = (float)(c1 * inv);
+        z = (float)(c2 * inv);
+    }
+    if (rotate) {
+        const double px = x, py = y, pz = z;
+        const double rx = px * m.a[0] + py * m.a[1] + pz * m.a[2];
+        const double ry = px * m.a[3] + py * m.a[4] + pz * m.a[5];
+        const double rz = px * m.a[6] + py * m.a[7] + pz * m.a[8];
+        x = (float)rx;
+        y = (float)ry;
+        z = (float)rz;
+    }
+    out[3 * i] = x;
+    out[3 * i + 1] = y;
+    out[3 * i + 2] = z;
+}
+
 // ------------------------------------------------------------- launchers
+
+int launch_sphere_prepare(const float *v, int n, int normalise, const double *m_host, float *out,
+                          double *centre, int *bad, hipStream_t s)
+{
+    if (n <= 0)
+        return kOk;
+    Mat3 m{};
+    if (m_host)
+        for (int k = 0; k < 9; ++k)
+            m.a[k] = m_host[k];
+    if (normalise) {
+        hipLaunchKernelGGL(k_sequential_centroid, dim3(1), dim3(64), 0, s, v, n, centre);
+        CSG_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_normalise_rotate, dim3(grid_for(n)), dim3(kBlock), 0, s, v, n,
+                       normalise ? centre : nullptr, m, m_host ? 1 : 0, out, bad);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
 
 int launch_normalise_f32(const float *in, int n, float *out, hipStream_t s)
 {

@@ -290,6 +290,15 @@ int csg_apply_flow(const float *unit, int n, const void *flow, int nx, int ny, i
  * (surface_warp.py:518-522). */
 int csg_normalise_points(const float *in, int n, float *out, void *stream);
 
+/* Sphere preparation of the pipeline on the device.  normalise = 1: the centroid as a
+ * strictly sequential float64 sum (ws: 3 doubles), p = f32(f64(v) - centroid / n), then
+ * f32(f64(p) / |f64(p)|), setting *bad (device int, cleared by the caller) for a zero or
+ * non-finite length (the reference raises ValueError); replaces _normalise_sphere
+ * (gpu_pipeline.py:274-305).  rotation (host, row-major 3x3, may be NULL): then
+ * f32(f64(p) m^T), replacing _rotate_vertices (:373-378).  Does not synchronise. */
+int csg_prepare_sphere(const float *in, int n, int normalise, const double *rotation,
+                       float *out, double *ws, int *bad, void *stream);
+
 /* Un-rotate the pole-frame warp by `unrotate` (host, row-major 3x3, p m^T) and blend it
  * with the main warp.  Replaces gpu_pipeline._average_xz_surfaces and its caller
  * (src/cat_surface_gpu/gpu_pipeline.py:248-271, :1294-1310). */

@@ -238,6 +238,59 @@ def test_average_xz_matches_golden(golden, cuda):
     np.testing.assert_array_equal(_np(out), g["avg_out"])
 
 
+def _ulps(a, b):
+    """Distance in float32 units in the last place."""
+
+    ia = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    ib = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    ia = np.where(ia < 0, np.int64(-2**31) - ia, ia)
+    ib = np.where(ib < 0, np.int64(-2**31) - ib, ib)
+    return np.abs(ia - ib)
+
+
+def test_prepare_sphere_matches_golden_and_oracle(golden, cuda):
+    """csg_prepare_sphere: the reference's sequential-centroid normalisation and
+    float64 rotation (golden vectors), and the same on a 61,442-vertex sphere
+    against the oracle.  The centroid is the same strictly sequential sum, so the
+    normalisation is bit-exact; the rotation's 3-term float64 dot products may
+    round differently from numpy's matmul, which at most moves the float32
+    result by one unit in the last place."""
+
+    from cat_surface_gpu.gpu_pipeline import _check_normalised, _prepare_sphere
+
+    g = golden("warp")
+    bad = torch.zeros(1, dtype=torch.int32, device=cuda)
+    v = torch.as_tensor(g["norm_in"], dtype=torch.float32, device=cuda)
+    norm = _prepare_sphere(v, normalise=True, bad=bad)
+    np.testing.assert_array_equal(_np(norm), g["norm_out"])
+    rot = _prepare_sphere(norm, normalise=False, angles=g["rot_angles"][1])
+    assert _ulps(_np(rot), g["rotv_out"]).max() <= 1
+    both = _prepare_sphere(v, normalise=True, angles=g["rot_angles"][1], bad=bad)
+    np.testing.assert_array_equal(_np(both), _np(rot))
+    _check_normalised(bad)
+
+    rng = np.random.default_rng(11)
+    big = rng.normal(size=(61442, 3))
+    big = (big / np.linalg.norm(big, axis=1, keepdims=True) * 100.0
+           + np.array([3.0, -7.0, 11.0])).astype(np.float32)
+    angles = np.array([0.7, -1.1, 2.3])
+    dv = torch.as_tensor(big, device=cuda)
+    got = _prepare_sphere(dv, normalise=True, bad=bad)
+    np.testing.assert_array_equal(_np(got), O.normalise_sphere(big))
+    got = _prepare_sphere(dv, normalise=True, angles=angles, bad=bad)
+    ulp = _ulps(_np(got), O.rotate_vertices(O.normalise_sphere(big), angles))
+    assert ulp.max() <= 1 and (ulp > 0).mean() < 1e-3
+    _check_normalised(bad)
+
+    # a vertex at the centroid cannot be normalised (the reference raises ValueError)
+    degenerate = np.array([[1, 0, 0], [-1, 0, 0], [0, 0, 0]], np.float32)
+    _prepare_sphere(torch.as_tensor(degenerate, device=cuda), normalise=True, bad=bad)
+    with pytest.raises(ValueError, match="cannot be normalized"):
+        _check_normalised(bad)
+    with pytest.raises(ValueError, match="cannot be normalized"):
+        O.normalise_sphere(degenerate)
+
+
 def test_stencil_builder_matches_brute_force(cuda):
     """Closest-triangle stencil entries equal an exhaustive CPU search."""
 
