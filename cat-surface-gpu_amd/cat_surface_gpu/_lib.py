@@ -95,7 +95,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_apply_flow": (_i, [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp]),
     "csg_average_xz": (_i, [_vp, _vp, _i, _i, _dp, _vp, _vp]),
     "csg_normalise_points": (_i, [_vp, _i, _vp, _vp]),
-    "csg_prepare_sphere": (_i, [_vp, _i, _i, _dp, _vp, _vp, _vp, _vp]),
+    "csg_sphere_centroids": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_i), _i, _vp, _vp]),
+    "csg_prepare_sphere": (_i, [_vp, _i, _vp, _dp, _vp, _vp, _vp]),
     # point location
     "csg_grid_index_plan_workspace_bytes": (_sz, [_i]),
     "csg_grid_index_plan": (_i, [_vp, _i, _vp, _i, _i, _i, _vp, _sz, ctypes.POINTER(_i), _vp]),

@@ -13,6 +13,7 @@ runs it.
 from __future__ import annotations
 
 from dataclasses import dataclass
+import ctypes
 import math
 import subprocess
 import tempfile
@@ -80,22 +81,41 @@ def write_gifti_mesh(path: str | Path, mesh: GiftiMesh, *,
                         triangle_intent=triangle_intent)
 
 
-def _prepare_sphere(vertices: torch.Tensor, *, normalise: bool, angles=None,
+def _sphere_centroids(spheres: Sequence[torch.Tensor]) -> torch.Tensor:
+    """[len(spheres), 3] float64 centroids, each the strictly sequential float64 sum
+    of the vertices divided by n (the centring of gpu_pipeline.py:274-305), all in
+    one launch (csg_sphere_centroids).  Asynchronous."""
+
+    vs = [v.contiguous() for v in spheres]
+    for v in vs:
+        if v.ndim != 2 or v.shape[1] != 3 or v.dtype != torch.float32 or v.shape[0] < 1:
+            raise ValueError("Sphere vertices must be a non-empty float32 [n, 3] tensor")
+    dev = vs[0].device
+    out = torch.empty((len(vs), 3), dtype=torch.float64, device=dev)
+    ptrs = (ctypes.c_void_p * len(vs))(*[v.data_ptr() for v in vs])
+    counts = (ctypes.c_int * len(vs))(*[int(v.shape[0]) for v in vs])
+    _lib.check(_lib.lib().csg_sphere_centroids(ptrs, counts, len(vs), _lib.ptr(out),
+                                               _lib.stream_handle(dev)), "csg_sphere_centroids")
+    return out
+
+
+def _prepare_sphere(vertices: torch.Tensor, *, centroid: torch.Tensor | None = None, angles=None,
                     bad: torch.Tensor | None = None) -> torch.Tensor:
     """The pipeline's sphere preparation (gpu_pipeline.py:274-305, :373-378) on the
-    device (csg_prepare_sphere): with ``normalise``, centre on the sequential
-    float64 centroid and scale every vertex to unit length in float32 (a vertex
-    that cannot be normalised sets ``bad``); with ``angles``, then rotate by
-    p · Mᵀ in float64, stored float32.  Asynchronous."""
+    device (csg_prepare_sphere): with ``centroid`` (from _sphere_centroids), centre
+    the vertices and scale each to unit length in float32 (a vertex that cannot be
+    normalised sets ``bad``); with ``angles``, then rotate by p · Mᵀ in float64,
+    stored float32.  Asynchronous."""
 
     if vertices.ndim != 2 or vertices.shape[1] != 3:
         raise ValueError(f"Sphere vertices must have shape [n, 3], got {tuple(vertices.shape)}")
+    if centroid is not None and bad is None:
+        raise ValueError("normalisation needs a device flag for bad vertices")
     v = vertices.to(torch.float32).contiguous()
     out = torch.empty_like(v)
-    ws = torch.empty(4, dtype=torch.float64, device=v.device)
     rot = None if angles is None else _lib.doubles(rotation_matrix_np(angles).reshape(-1))
-    _lib.check(_lib.lib().csg_prepare_sphere(_lib.ptr(v), int(v.shape[0]), int(normalise), rot,
-                                             _lib.ptr(out), _lib.ptr(ws), _lib.ptr(bad),
+    _lib.check(_lib.lib().csg_prepare_sphere(_lib.ptr(v), int(v.shape[0]), _lib.ptr(centroid),
+                                             rot, _lib.ptr(out), _lib.ptr(bad),
                                              _lib.stream_handle(v.device)),
                "csg_prepare_sphere")
     return out
@@ -243,6 +263,8 @@ class _JobState:
     current_dev: object = None  # device copy of current_sphere (float32)
     src_sphere_dev: object = None
     tgt_sphere_dev: object = None
+    src_centroid: object = None  # device float64 [3]
+    tgt_centroid: object = None
 
     def current_host(self) -> np.ndarray:
         if self.current_sphere is None:
@@ -370,6 +392,11 @@ def run_cat_surface_gpu_pipelines(
                                                 device=dev)
             st.tgt_sphere_dev = torch.as_tensor(st.tgt_sphere.vertices, dtype=torch.float32,
                                                 device=dev)
+        # every input sphere's centroid, for the normalisations after the rotation search
+        centroids = _sphere_centroids([v for st in states
+                                       for v in (st.src_sphere_dev, st.tgt_sphere_dev)])
+        for i, st in enumerate(states):
+            st.src_centroid, st.tgt_centroid = centroids[2 * i], centroids[2 * i + 1]
         torch.cuda.synchronize(dev)
         shared["initial_stencil_seconds"] = time.perf_counter() - t
 
@@ -463,12 +490,12 @@ def run_cat_surface_gpu_pipelines(
         # normalise and rotate on the device (gpu_pipeline.py:274-305, :373-378)
         bad = torch.zeros(1, dtype=torch.int32, device=dev)
         for i, st in enumerate(states):
-            st.current_dev = _prepare_sphere(st.src_sphere_dev, normalise=True, angles=st.angle,
-                                             bad=bad)
+            st.current_dev = _prepare_sphere(st.src_sphere_dev, centroid=st.src_centroid,
+                                             angles=st.angle, bad=bad)
             st.current_sphere = None
-            st.avg_tgt_surface = _prepare_sphere(st.tgt_dev, normalise=False, angles=pole)
-            avg_tgt_sphere = _prepare_sphere(st.tgt_sphere_dev, normalise=True, angles=pole,
-                                             bad=bad)
+            st.avg_tgt_surface = _prepare_sphere(st.tgt_dev, angles=pole)
+            avg_tgt_sphere = _prepare_sphere(st.tgt_sphere_dev, centroid=st.tgt_centroid,
+                                             angles=pole, bad=bad)
             st.cur_stencil = stencil(st, i, st.current_dev, None, "rotated-000", False,
                                      builder=builder)
             if avg:
@@ -504,10 +531,8 @@ def run_cat_surface_gpu_pipelines(
                 if final_run:
                     st.final_prewarp = (st.current_dev.clone() if st.current_dev is not None
                                         else torch.as_tensor(st.current_host(), device=dev))
-                    st.avg_src_surface = _prepare_sphere(st.src_dev, normalise=False,
-                                                         angles=pole)
-                    st.avg_src_sphere = _prepare_sphere(st.final_prewarp, normalise=False,
-                                                        angles=pole)
+                    st.avg_src_surface = _prepare_sphere(st.src_dev, angles=pole)
+                    st.avg_src_sphere = _prepare_sphere(st.final_prewarp, angles=pole)
                     st.avg_src_stencil = stencil(st, i, st.avg_src_sphere, None, "avg-source",
                                                  False, builder=builder)
                 requests.append((st.src_dev, st.tgt_dev, st.cur_stencil, st.tgt_stencil,

@@ -3,6 +3,8 @@
 #include "depth_kernels.h"
 #include "surface_kernels.h"
 
+#include <algorithm>
+
 using namespace csg;
 
 namespace {
@@ -402,12 +404,33 @@ int csg_normalise_points(const float *in, int n, float *out, void *stream)
     return launch_normalise_f32(in, n, out, S(stream));
 }
 
-int csg_prepare_sphere(const float *in, int n, int normalise, const double *rotation,
-                       float *out, double *ws, int *bad, void *stream)
+int csg_sphere_centroids(const float *const *spheres, const int *counts, int count,
+                         double *centroids, void *stream)
 {
-    if (n < 1 || in == nullptr || out == nullptr || (normalise && (ws == nullptr || !bad)))
+    if (count < 0 || (count > 0 && (!spheres || !counts || !centroids)))
+        return fail(kInvalidArgument, "invalid sphere centroid arguments");
+    for (int first = 0; first < count; first += kMaxSpheres) {
+        SphereList list{};
+        list.count = std::min(kMaxSpheres, count - first);
+        for (int k = 0; k < list.count; ++k) {
+            if (!spheres[first + k] || counts[first + k] < 1)
+                return fail(kInvalidArgument, "every sphere needs vertices");
+            list.ptr[k] = spheres[first + k];
+            list.n[k] = counts[first + k];
+        }
+        const int rc = launch_sphere_centroids(list, centroids + 3 * first, S(stream));
+        if (rc != kOk)
+            return rc;
+    }
+    return kOk;
+}
+
+int csg_prepare_sphere(const float *in, int n, const double *centroid, const double *rotation,
+                       float *out, int *bad, void *stream)
+{
+    if (n < 1 || in == nullptr || out == nullptr || (centroid && !bad))
         return fail(kInvalidArgument, "invalid sphere preparation arguments");
-    return launch_sphere_prepare(in, n, normalise, rotation, out, ws, bad, S(stream));
+    return launch_sphere_prepare(in, n, centroid, rotation, out, bad, S(stream));
 }
 
 int csg_average_xz(const void *avg, const void *main_, int n, int dtype, const double *unrotate,

@@ -61,11 +61,20 @@ int launch_normalise_f32(const float *in, int n, float *out, hipStream_t s);
 struct Mat3 {
     double a[9];
 };
-// normalise: sequential float64 centroid (centre: 3 doubles of scratch), then
-// float32 unit vectors; m_host (row-major, may be null): then p · mᵀ in
-// float64, stored float32.  *bad is set to 1 for a vertex of zero or
-// non-finite length (the caller clears it).
-int launch_sphere_prepare(const float *v, int n, int normalise, const double *m_host, float *out,
-                          double *centre, int *bad, hipStream_t s);
+// Up to kMaxSpheres vertex arrays ([n][3] float32, device) per centroid launch.
+constexpr int kMaxSpheres = 64;
+struct SphereList {
+    int count;
+    const float *ptr[kMaxSpheres];
+    int n[kMaxSpheres];
+};
+// centroid[3 s + c] = (strictly sequential float64 sum over sphere s's vertices) / n.
+int launch_sphere_centroids(const SphereList &list, double *centroid, hipStream_t s);
+// centroid (device, may be null): p = f32(f64(v) - centroid), then float32 unit
+// vectors; m_host (row-major, may be null): then p · mᵀ in float64, stored
+// float32.  *bad is set to 1 for a vertex of zero or non-finite length (the
+// caller clears it).
+int launch_sphere_prepare(const float *v, int n, const double *centroid, const double *m_host,
+                          float *out, int *bad, hipStream_t s);
 
 }  // namespace csg

@@ -479,35 +479,66 @@ k_normalise_f32(const float *__restrict__ in, int n, float *__restrict__ out)
 
 // Sphere preparation of the pipeline (gpu_pipeline.py:274-305, :373-378).
 // The centroid is a strictly sequential float64 sum in vertex order (the
-// reference's loop; one lane per coordinate): centre[c] = Σ_i f64(v[i][c]).
-__global__ void k_sequential_centroid(const float *__restrict__ v, int n,
-                                      double *__restrict__ centre)
+// reference's loop), divided by n: centroid[c] = (Σ_i f64(v[i][c])) / n.  One
+// workgroup per sphere: waves 1-3 stream the next chunk of kCentroidChunk
+// vertices (coalesced) into one LDS buffer while lanes 0-2 of wave 0 add the
+// current chunk, one coordinate per lane, from the other.
+constexpr int kCentroidThreads = 256;
+constexpr int kCentroidChunk = 256;  // vertices per LDS buffer
+
+__global__ void __launch_bounds__(kCentroidThreads)
+k_sequential_centroids(SphereList list, double *__restrict__ centroid)
 {
-    const int c = threadIdx.x;
-    if (c >= 3)
-        return;
+    __shared__ float tile[2][3 * kCentroidChunk];
+    const int sphere = blockIdx.x;
+    const float *__restrict__ v = list.ptr[sphere];
+    const int n = list.n[sphere];
+    const long long total = 3LL * n;
+    const int tid = threadIdx.x;
+    const int chunks = (n + kCentroidChunk - 1) / kCentroidChunk;
+    for (int j = tid; j < 3 * kCentroidChunk; j += kCentroidThreads)
+        tile[0][j] = j < total ? v[j] : 0.0f;
+    __syncthreads();
     double s = 0.0;
-    for (int i = 0; i < n; ++i)
-        s += (double)v[3 * i + c];
-    centre[c] = s;
+    for (int k = 0; k < chunks; ++k) {
+        const int p = k & 1;
+        if (tid >= 64 && k + 1 < chunks) {
+            const long long base = (long long)(k + 1) * 3 * kCentroidChunk;
+            for (int j = tid - 64; j < 3 * kCentroidChunk; j += kCentroidThreads - 64)
+                tile[p ^ 1][j] = base + j < total ? v[base + j] : 0.0f;
+        }
+        if (tid < 3) {
+            const int m = min(kCentroidChunk, n - k * kCentroidChunk);
+            if (m == kCentroidChunk) {
+#pragma unroll 16
+                for (int q = 0; q < kCentroidChunk; ++q)
+                    s += (double)tile[p][3 * q + tid];
+            } else {
+                for (int q = 0; q < m; ++q)
+                    s += (double)tile[p][3 * q + tid];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < 3)
+        centroid[3 * sphere + tid] = s / (double)n;
 }
 
-// Per vertex: with `centre`, p = f32(f64(v) - centre / n), then
+// Per vertex: with `centroid`, p = f32(f64(v) - centroid), then
 // f32(f64(p) / |f64(p)|) (flagging zero or non-finite lengths in *bad);
 // with `m` (row-major 3x3), f32(f64(p) · mᵀ).
 __global__ void __launch_bounds__(kBlock)
-k_normalise_rotate(const float *__restrict__ v, int n, const double *__restrict__ centre,
+k_normalise_rotate(const float *__restrict__ v, int n, const double *__restrict__ centroid,
                    Mat3 m, int rotate, float *__restrict__ out, int *__restrict__ bad)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     float x = v[3 * i], y = v[3 * i + 1], z = v[3 * i + 2];
-    if (centre) {
-        const double dn = (double)n;
-        const float ex = (float)((double)x - centre[0] / dn);
-        const float ey = (float)((double)y - centre[1] / dn);
-        const float ez = (float)((double)z - centre[2] / dn);
+    if (centroid) {
+        const float ex = (float)((double)x - centroid[0]);
+        const float ey = (float)((double)y - centroid[1]);
+        const float ez = (float)((double)z - centroid[2]);
         const double c0 = ex, c1 = ey, c2 = ez;
         const double len = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
         if (!(len > 0.0) || !isfinite(len))
@@ -533,8 +564,18 @@ k_normalise_rotate(const float *__restrict__ v, int n, const double *__restrict_
 
 // ------------------------------------------------------------- launchers
 
-int launch_sphere_prepare(const float *v, int n, int normalise, const double *m_host, float *out,
-                          double *centre, int *bad, hipStream_t s)
+int launch_sphere_centroids(const SphereList &list, double *centroid, hipStream_t s)
+{
+    if (list.count <= 0)
+        return kOk;
+    hipLaunchKernelGGL(k_sequential_centroids, dim3(list.count), dim3(kCentroidThreads), 0, s,
+                       list, centroid);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+int launch_sphere_prepare(const float *v, int n, const double *centroid, const double *m_host,
+                          float *out, int *bad, hipStream_t s)
 {
     if (n <= 0)
         return kOk;
@@ -542,12 +583,8 @@ int launch_sphere_prepare(const float *v, int n, int normalise, const double *m_
     if (m_host)
         for (int k = 0; k < 9; ++k)
             m.a[k] = m_host[k];
-    if (normalise) {
-        hipLaunchKernelGGL(k_sequential_centroid, dim3(1), dim3(64), 0, s, v, n, centre);
-        CSG_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(k_normalise_rotate, dim3(grid_for(n)), dim3(kBlock), 0, s, v, n,
-                       normalise ? centre : nullptr, m, m_host ? 1 : 0, out, bad);
+    hipLaunchKernelGGL(k_normalise_rotate, dim3(grid_for(n)), dim3(kBlock), 0, s, v, n, centroid,
+                       m, m_host ? 1 : 0, out, bad);
     CSG_LAUNCH_CHECK();
     return kOk;
 }

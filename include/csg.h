@@ -290,14 +290,21 @@ int csg_apply_flow(const float *unit, int n, const void *flow, int nx, int ny, i
  * (surface_warp.py:518-522). */
 int csg_normalise_points(const float *in, int n, float *out, void *stream);
 
-/* Sphere preparation of the pipeline on the device.  normalise = 1: the centroid as a
- * strictly sequential float64 sum (ws: 3 doubles), p = f32(f64(v) - centroid / n), then
+/* Centroids of `count` spheres (host arrays of device pointers to [n][3] float32 vertices
+ * and of vertex counts): centroids[3 s + c] (device, float64) = the strictly sequential
+ * float64 sum of sphere s's coordinate c in vertex order, divided by n — the centring of
+ * _normalise_sphere (src/cat_surface_gpu/gpu_pipeline.py:274-305).  Does not synchronise. */
+int csg_sphere_centroids(const float *const *spheres, const int *counts, int count,
+                         double *centroids, void *stream);
+
+/* Sphere preparation of the pipeline on the device.  centroid (device, 3 doubles from
+ * csg_sphere_centroids, or NULL to skip): p = f32(f64(v) - centroid), then
  * f32(f64(p) / |f64(p)|), setting *bad (device int, cleared by the caller) for a zero or
  * non-finite length (the reference raises ValueError); replaces _normalise_sphere
  * (gpu_pipeline.py:274-305).  rotation (host, row-major 3x3, may be NULL): then
  * f32(f64(p) m^T), replacing _rotate_vertices (:373-378).  Does not synchronise. */
-int csg_prepare_sphere(const float *in, int n, int normalise, const double *rotation,
-                       float *out, double *ws, int *bad, void *stream);
+int csg_prepare_sphere(const float *in, int n, const double *centroid, const double *rotation,
+                       float *out, int *bad, void *stream);
 
 /* Un-rotate the pole-frame warp by `unrotate` (host, row-major 3x3, p m^T) and blend it
  * with the main warp.  Replaces gpu_pipeline._average_xz_surfaces and its caller

@@ -256,39 +256,51 @@ def test_prepare_sphere_matches_golden_and_oracle(golden, cuda):
     round differently from numpy's matmul, which at most moves the float32
     result by one unit in the last place."""
 
-    from cat_surface_gpu.gpu_pipeline import _check_normalised, _prepare_sphere
+    from cat_surface_gpu.gpu_pipeline import _check_normalised, _prepare_sphere, _sphere_centroids
 
     g = golden("warp")
     bad = torch.zeros(1, dtype=torch.int32, device=cuda)
     v = torch.as_tensor(g["norm_in"], dtype=torch.float32, device=cuda)
-    norm = _prepare_sphere(v, normalise=True, bad=bad)
+    c = _sphere_centroids([v])[0]
+    norm = _prepare_sphere(v, centroid=c, bad=bad)
     np.testing.assert_array_equal(_np(norm), g["norm_out"])
-    rot = _prepare_sphere(norm, normalise=False, angles=g["rot_angles"][1])
+    rot = _prepare_sphere(norm, angles=g["rot_angles"][1])
     assert _ulps(_np(rot), g["rotv_out"]).max() <= 1
-    both = _prepare_sphere(v, normalise=True, angles=g["rot_angles"][1], bad=bad)
+    both = _prepare_sphere(v, centroid=c, angles=g["rot_angles"][1], bad=bad)
     np.testing.assert_array_equal(_np(both), _np(rot))
     _check_normalised(bad)
 
+    # spheres of several sizes in one centroid launch (chunk tails, more than one
+    # launch's worth of spheres), each against the oracle's sequential sum
     rng = np.random.default_rng(11)
-    big = rng.normal(size=(61442, 3))
-    big = (big / np.linalg.norm(big, axis=1, keepdims=True) * 100.0
-           + np.array([3.0, -7.0, 11.0])).astype(np.float32)
+    sizes = [61442, 1, 255, 256, 257, 1000] + [int(k) for k in rng.integers(2, 3000, 64)]
+    spheres = []
+    for n in sizes:
+        p = rng.normal(size=(n, 3))
+        p = p / np.linalg.norm(p, axis=1, keepdims=True) * 100.0 + rng.normal(size=3) * 10
+        spheres.append(p.astype(np.float32))
+    dev = [torch.as_tensor(p, device=cuda) for p in spheres]
+    cs = _np(_sphere_centroids(dev))
+    for p, got in zip(spheres, cs):
+        np.testing.assert_array_equal(got, np.cumsum(p.astype(np.float64), axis=0)[-1]
+                                      / p.shape[0])
+    big, dv = spheres[0], dev[0]
     angles = np.array([0.7, -1.1, 2.3])
-    dv = torch.as_tensor(big, device=cuda)
-    got = _prepare_sphere(dv, normalise=True, bad=bad)
+    got = _prepare_sphere(dv, centroid=_sphere_centroids([dv])[0], bad=bad)
     np.testing.assert_array_equal(_np(got), O.normalise_sphere(big))
-    got = _prepare_sphere(dv, normalise=True, angles=angles, bad=bad)
+    got = _prepare_sphere(dv, centroid=_sphere_centroids([dv])[0], angles=angles, bad=bad)
     ulp = _ulps(_np(got), O.rotate_vertices(O.normalise_sphere(big), angles))
     assert ulp.max() <= 1 and (ulp > 0).mean() < 1e-3
     _check_normalised(bad)
 
     # a vertex at the centroid cannot be normalised (the reference raises ValueError)
-    degenerate = np.array([[1, 0, 0], [-1, 0, 0], [0, 0, 0]], np.float32)
-    _prepare_sphere(torch.as_tensor(degenerate, device=cuda), normalise=True, bad=bad)
+    degenerate = torch.as_tensor(np.array([[1, 0, 0], [-1, 0, 0], [0, 0, 0]], np.float32),
+                                 device=cuda)
+    _prepare_sphere(degenerate, centroid=_sphere_centroids([degenerate])[0], bad=bad)
     with pytest.raises(ValueError, match="cannot be normalized"):
         _check_normalised(bad)
     with pytest.raises(ValueError, match="cannot be normalized"):
-        O.normalise_sphere(degenerate)
+        O.normalise_sphere(_np(degenerate))
 
 
 def test_stencil_builder_matches_brute_force(cuda):
