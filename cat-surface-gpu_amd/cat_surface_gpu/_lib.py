@@ -99,6 +99,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_prepare_sphere": (_i, [_vp, _i, _vp, _dp, _vp, _vp, _vp]),
     # point location
     "csg_grid_index_plan_workspace_bytes": (_sz, [_i]),
+    "csg_grid_index_build_workspace_bytes": (_sz, [_i, _i]),
+    "csg_grid_index_build": (_i, [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
     "csg_grid_index_plan": (_i, [_vp, _i, _vp, _i, _i, _i, _vp, _sz, ctypes.POINTER(_i), _vp]),
     "csg_grid_index_fill_workspace_bytes": (_sz, [_i]),
     "csg_grid_index_fill": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _sz, _vp]),

@@ -29,6 +29,7 @@ from .dartel_grid import resolve_device
 from .gifti import read_gifti, write_gifti_surface, INTENT_POINTSET, INTENT_TRIANGLE
 from .rotation_feature import compute_rotation_features
 from .rotation_pipeline import RotationPipeline, read_rotation_values
+from .locate import check_all
 from .stencil_builder import build_device_stencil
 from .surface_rotation import rotation_matrix_np, search_rotations
 from .surface_stencil import SurfaceStencil
@@ -372,20 +373,54 @@ def run_cat_surface_gpu_pipelines(
                     "--threads", str(stencil_threads)]
                 return _external_stencil(builder, sphere_path,
                                          tmp / f"job{index}-{label}.stencil", opts, dev)
+            return device_stencil(st, sphere_vertices, label, normalise)
+
+        def device_stencil(st: _JobState, sphere_vertices, label, normalise, deferred=False):
+            source_side = label.startswith(("source", "rotated", "avg-source"))
             surface = st.src_surface.vertices if source_side else st.tgt_surface.vertices
             faces_dev = st.src_faces_dev if source_side else st.tgt_faces_dev
             nx, ny = st.job.map_size or (512, 256)
             return build_device_stencil(sphere_vertices, faces_dev, surface_vertices=surface,
-                                        normalise=normalise, nx=int(nx), ny=int(ny), device=dev)
+                                        normalise=normalise, nx=int(nx), ny=int(ny), device=dev,
+                                        deferred=deferred)
+
+        # In-process stencil builds of a stage are enqueued without host round trips
+        # and confirmed together after the stage's synchronisation (settle); one
+        # that fails its check (index capacity, search reach) is built again
+        # synchronously before anything reads it.
+        pending: list = []
+
+        def build_into(st: _JobState, attr: str, index: int, sphere_vertices, label: str,
+                       normalise: bool, builder, sphere_path=None, path=None):
+            if builder is None and path is None:
+                value, checks = device_stencil(st, sphere_vertices, label, normalise,
+                                               deferred=True)
+                pending.append((checks, st, attr, sphere_vertices, label, normalise))
+            else:
+                value = stencil(st, index, sphere_vertices, sphere_path, label, normalise, path,
+                                builder)
+            setattr(st, attr, value)
+
+        def settle() -> None:
+            if not pending:
+                return
+            verdict = check_all([c for entry in pending for c in entry[0]])
+            k = 0
+            for checks, st, attr, sphere_vertices, label, normalise in pending:
+                ok = all(verdict[k:k + len(checks)])
+                k += len(checks)
+                if not ok:
+                    setattr(st, attr, device_stencil(st, sphere_vertices, label, normalise))
+            pending.clear()
 
         t = time.perf_counter()
         for i, st in enumerate(states):
             st.src_faces_dev = torch.as_tensor(st.src_sphere.faces, device=dev)
             st.tgt_faces_dev = torch.as_tensor(st.tgt_sphere.faces, device=dev)
-            st.src_stencil = stencil(st, i, st.src_sphere.vertices, st.job.source_sphere,
-                                     "source", True, st.job.source_stencil, stencil_builder)
-            st.tgt_stencil = stencil(st, i, st.tgt_sphere.vertices, st.job.target_sphere,
-                                     "target", True, st.job.target_stencil, stencil_builder)
+            build_into(st, "src_stencil", i, st.src_sphere.vertices, "source", True,
+                       stencil_builder, st.job.source_sphere, st.job.source_stencil)
+            build_into(st, "tgt_stencil", i, st.tgt_sphere.vertices, "target", True,
+                       stencil_builder, st.job.target_sphere, st.job.target_stencil)
             st.src_dev = torch.as_tensor(st.src_surface.vertices, device=dev)
             st.tgt_dev = torch.as_tensor(st.tgt_surface.vertices, device=dev)
             st.src_sphere_dev = torch.as_tensor(st.src_sphere.vertices, dtype=torch.float32,
@@ -398,6 +433,7 @@ def run_cat_surface_gpu_pipelines(
         for i, st in enumerate(states):
             st.src_centroid, st.tgt_centroid = centroids[2 * i], centroids[2 * i + 1]
         torch.cuda.synchronize(dev)
+        settle()
         shared["initial_stencil_seconds"] = time.perf_counter() - t
 
         # rotation features (rotation_feature.py; gpu_pipeline.py:720-895): every
@@ -496,13 +532,13 @@ def run_cat_surface_gpu_pipelines(
             st.avg_tgt_surface = _prepare_sphere(st.tgt_dev, angles=pole)
             avg_tgt_sphere = _prepare_sphere(st.tgt_sphere_dev, centroid=st.tgt_centroid,
                                              angles=pole, bad=bad)
-            st.cur_stencil = stencil(st, i, st.current_dev, None, "rotated-000", False,
-                                     builder=builder)
+            build_into(st, "cur_stencil", i, st.current_dev, "rotated-000", False, builder)
             if avg:
-                st.avg_tgt_stencil = stencil(st, i, avg_tgt_sphere, None, "avg-target", False,
-                                             builder=builder)
+                build_into(st, "avg_tgt_stencil", i, avg_tgt_sphere, "avg-target", False,
+                           builder)
         torch.cuda.synchronize(dev)
         _check_normalised(bad)
+        settle()
         shared["rotated_stencil_seconds"] = time.perf_counter() - t
 
         solve_kw = dict(n_steps=steps, loop=loop, cycles=cycles, nit=nit, its=its, code=code,
@@ -533,8 +569,8 @@ def run_cat_surface_gpu_pipelines(
                                         else torch.as_tensor(st.current_host(), device=dev))
                     st.avg_src_surface = _prepare_sphere(st.src_dev, angles=pole)
                     st.avg_src_sphere = _prepare_sphere(st.final_prewarp, angles=pole)
-                    st.avg_src_stencil = stencil(st, i, st.avg_src_sphere, None, "avg-source",
-                                                 False, builder=builder)
+                    build_into(st, "avg_src_stencil", i, st.avg_src_sphere, "avg-source",
+                               False, builder)
                 requests.append((st.src_dev, st.tgt_dev, st.cur_stencil, st.tgt_stencil,
                                  run_fwhm))
                 owners.append((st, "main"))
@@ -547,6 +583,7 @@ def run_cat_surface_gpu_pipelines(
                                      torch.as_tensor(st.avg_tgt_surface, device=dev),
                                      st.avg_src_stencil, st.avg_tgt_stencil, avg_fwhm))
                     owners.append((st, "avg"))
+            settle()  # the average solves read the avg-source stencils
             results = solve_dartel_from_surfaces_jobs(requests, **solve_kw)
             for (st, role), res in zip(owners, results):
                 if role == "avg":
@@ -563,9 +600,10 @@ def run_cat_surface_gpu_pipelines(
                 for i, st in enumerate(states):
                     sphere = (st.current_host() if builder is not None or st.current_dev is None
                               else st.current_dev)
-                    st.cur_stencil = stencil(st, i, sphere, None, f"rotated-{run + 1:03d}",
-                                             False, builder=builder)
+                    build_into(st, "cur_stencil", i, sphere, f"rotated-{run + 1:03d}", False,
+                               builder)
             torch.cuda.synchronize(dev)
+            settle()
             shared[f"dartel_run_{run}_seconds"] = time.perf_counter() - t
         shared["source_dartel_total_seconds"] = time.perf_counter() - t_dartel
 

@@ -17,6 +17,16 @@ import torch
 
 from . import _lib
 
+# Pair capacities for asynchronous index builds, learned from the exact counts
+# of earlier builds: (faces, grid size, margin) -> capacity.
+_CAPACITY: dict[tuple[int, int, int], int] = {}
+
+
+def _learn_capacity(key: tuple[int, int, int], pairs: int) -> None:
+    want = int(pairs * 1.08) + 1024
+    if want > _CAPACITY.get(key, 0):
+        _CAPACITY[key] = want
+
 
 @dataclass
 class GridIndex:
@@ -52,6 +62,7 @@ def build_grid_index(points: torch.Tensor, faces: torch.Tensor, grid_size: int,
     n = npairs.value
     if n < 1:
         raise ValueError("At least one triangle does not cover any spatial grid cell")
+    _learn_capacity((nf, int(grid_size), int(margin)), n)
     cell_off = torch.empty(grid_size ** 3 + 1, dtype=torch.int32, device=dev)
     cell_faces = torch.empty(n, dtype=torch.int32, device=dev)
     ws = _lib.workspace(lib.csg_grid_index_fill_workspace_bytes(n), dev)
@@ -61,14 +72,62 @@ def build_grid_index(points: torch.Tensor, faces: torch.Tensor, grid_size: int,
     return GridIndex(int(grid_size), int(margin), cell_off, cell_faces)
 
 
+def build_grid_index_async(points: torch.Tensor, faces: torch.Tensor, grid_size: int,
+                           margin: int, capacity: int, required: torch.Tensor) -> GridIndex:
+    """The index in one asynchronous call (csg_grid_index_build): at most
+    ``capacity`` pairs; ``required`` (a device int) receives the count the index
+    needs, and the index is complete iff that count is at most ``capacity``."""
+
+    dev = points.device
+    pts = points.contiguous()
+    if pts.dtype not in (torch.float32, torch.float64):
+        pts = pts.to(torch.float64)
+    f = faces.to(torch.int32).contiguous()
+    nf = int(f.shape[0])
+    lib = _lib.lib()
+    cell_off = torch.empty(grid_size ** 3 + 1, dtype=torch.int32, device=dev)
+    cell_faces = torch.empty(capacity, dtype=torch.int32, device=dev)
+    ws = _lib.workspace(lib.csg_grid_index_build_workspace_bytes(nf, capacity), dev)
+    _lib.check(lib.csg_grid_index_build(_lib.ptr(pts), _lib.dtype_code(pts.dtype), _lib.ptr(f), nf,
+                                        int(grid_size), int(margin), int(capacity),
+                                        _lib.ptr(cell_off), _lib.ptr(cell_faces),
+                                        _lib.ptr(required), _lib.ptr(ws), ws.numel(),
+                                        _lib.stream_handle(dev)), "csg_grid_index_build")
+    return GridIndex(int(grid_size), int(margin), cell_off, cell_faces)
+
+
+@dataclass
+class LocateCheck:
+    """What an asynchronous closest_triangles still has to confirm: status (device
+    int32 [2]) = (pairs the index needed, queries beyond the searched reach)."""
+
+    status: torch.Tensor
+    capacity: int
+    key: tuple[int, int, int]
+
+    def passed(self, host_status) -> bool:
+        required, far = int(host_status[0]), int(host_status[1])
+        if required > self.capacity:
+            _learn_capacity(self.key, required)
+        return required <= self.capacity and far == 0
+
+
 def closest_triangles(queries: torch.Tensor, verts: torch.Tensor, faces: torch.Tensor, *,
-                      grid_size: int, margin: int = 1, max_margin: int = 8):
+                      grid_size: int, margin: int = 1, max_margin: int = 8,
+                      deferred: bool = False):
     """Exact closest triangle and barycentric weights for each query (float64).
 
     The search covers the query's cell; a query farther than ``margin`` cells
     from its best triangle could have its true closest triangle elsewhere, so
     the index is rebuilt with a larger margin until no query is that far.
     Returns (face [n] int32, vertex indices [n, 3] int32, weights [n, 3] f64).
+
+    ``deferred=True`` returns a fourth item: None when the result is final, else
+    a LocateCheck; then nothing synchronises (the index is built with a pair
+    capacity learned from earlier builds of the same face count, grid and
+    margin, and only ``margin`` is searched) and the result is exact iff the
+    check passes, which the caller confirms after its next synchronisation
+    (check_all) and otherwise recomputes with ``deferred=False``.
     """
 
     q = queries.to(torch.float64)
@@ -88,6 +147,18 @@ def closest_triangles(queries: torch.Tensor, verts: torch.Tensor, faces: torch.T
     idx = torch.empty((nq, 3), dtype=torch.int32, device=dev)
     w = torch.empty((nq, 3), dtype=torch.float64, device=dev)
     ws = _lib.workspace(256, dev)
+    key = (int(f.shape[0]), int(grid_size), int(margin))
+    if deferred and key in _CAPACITY:
+        capacity = _CAPACITY[key]
+        status = torch.zeros(2, dtype=torch.int32, device=dev)
+        index = build_grid_index_async(v, f, grid_size, margin, capacity, status[0:1])
+        reach = margin * 2.0 / grid_size
+        _lib.check(lib.csg_closest_triangles(_lib.ptr(q), nq, _lib.ptr(v), _lib.ptr(f),
+                                             _lib.ptr(index.cell_off), _lib.ptr(index.cell_faces),
+                                             grid_size, reach, _lib.ptr(face), _lib.ptr(idx),
+                                             _lib.ptr(w), None, _lib.ptr(status[1:]), 4,
+                                             _lib.stream_handle(dev)), "csg_closest_triangles")
+        return face, idx, w, LocateCheck(status, capacity, key)
     while True:
         index = build_grid_index(v, f, grid_size, margin)
         far = ctypes.c_int()
@@ -99,7 +170,18 @@ def closest_triangles(queries: torch.Tensor, verts: torch.Tensor, faces: torch.T
                                              ws.numel(), _lib.stream_handle(dev)),
                    "csg_closest_triangles")
         if far.value == 0:
-            return face, idx, w
+            return (face, idx, w, None) if deferred else (face, idx, w)
         if margin >= max_margin:
             raise RuntimeError(f"{far.value} points lie farther than {reach} from the mesh")
         margin += 1
+
+
+def check_all(checks) -> list[bool]:
+    """Confirm LocateChecks (None counts as passed) with one device-to-host copy."""
+
+    live = [c for c in checks if c is not None]
+    if not live:
+        return [True] * len(checks)
+    host = torch.stack([c.status for c in live]).cpu().tolist()
+    verdict = iter([c.passed(h) for c, h in zip(live, host)])
+    return [True if c is None else next(verdict) for c in checks]

@@ -141,12 +141,16 @@ def _device_uv(nx: int, ny: int, dev: torch.device) -> torch.Tensor:
 
 def build_device_stencil(sphere_vertices, sphere_faces, *, surface_vertices=None,
                          normalise: bool = True, nx: int = 512, ny: int = 256,
-                         n_triangles: int = 81920, device: torch.device | None = None):
+                         n_triangles: int = 81920, device: torch.device | None = None,
+                         deferred: bool = False):
     """Build the stencil directly as a SurfaceStencilDevice (no host round trip).
 
     ``sphere_vertices`` and ``sphere_faces`` may be host arrays or device
     tensors; the icosphere, its topology and the map's pixel directions are
-    cached per device.
+    cached per device.  ``deferred=True`` returns (stencil, checks): nothing
+    synchronises, and the stencil is exact iff ``locate.check_all(checks)``
+    passes after the caller's next synchronisation (else build it again
+    without ``deferred``).
     """
 
     from .dartel_grid import resolve_device
@@ -170,14 +174,16 @@ def build_device_stencil(sphere_vertices, sphere_faces, *, surface_vertices=None
     ico_v, ico_f, topo = _device_icosphere(subdivisions_for(n_triangles), dev)
     out_sphere = centre[None, :] + radius * ico_v
     # surface stencil: closest input-sphere triangle per resampled vertex
-    _, surf_idx, surf_w = closest_triangles(out_sphere, pts, sf, grid_size=32, margin=1)
+    first = closest_triangles(out_sphere, pts, sf, grid_size=32, margin=1, deferred=deferred)
+    surf_idx, surf_w = first[1], first[2]
     resampled = (pts[surf_idx.long()] * surf_w[..., None]).sum(dim=1)
     # sheet stencil on the unit resampled sphere
     unit_res = resampled / torch.linalg.vector_norm(resampled, dim=1, keepdim=True)
-    _, sheet_idx, sheet_w = closest_triangles(_device_uv(nx, ny, dev), unit_res, ico_f,
-                                              grid_size=64, margin=1)
+    second = closest_triangles(_device_uv(nx, ny, dev), unit_res, ico_f, grid_size=64,
+                               margin=1, deferred=deferred)
+    sheet_idx, sheet_w = second[1], second[2]
     unit_pts = (pts / torch.linalg.vector_norm(pts, dim=1, keepdim=True)).to(torch.float32)
-    return SurfaceStencilDevice(
+    stencil = SurfaceStencilDevice(
         sphere_points=resampled.to(torch.float32).contiguous(),
         faces=topo.faces,
         surface_indices=surf_idx.contiguous(),
@@ -191,3 +197,4 @@ def build_device_stencil(sphere_vertices, sphere_faces, *, surface_vertices=None
         topology=topo,
         sphere_points64=resampled.contiguous(),
     )
+    return (stencil, [first[3], second[3]]) if deferred else stencil

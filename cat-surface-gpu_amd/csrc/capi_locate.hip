@@ -45,8 +45,6 @@ int csg_grid_index_plan(const void *pts, int pts_dtype, const int *faces, int nf
     char *b = (char *)ws;
     int *box = (int *)(b + l.box), *counts = (int *)(b + l.counts),
         *offsets = (int *)(b + l.offsets);
-    // the scan reads nf + 1 counts: the extra one must be zero
-    CSG_HIP_CHECK(hipMemsetAsync(counts + nf, 0, sizeof(int), S(stream)));
     if (pts_dtype == 1)
         return grid_index_count<double>((const double *)pts, faces, nf, grid, margin, box, counts,
                                         offsets, npairs, b + l.scratch, ws_bytes - l.scratch,
@@ -76,6 +74,44 @@ int csg_grid_index_fill(const void *plan_ws, int nf, int grid, int npairs, int *
     return grid_index_fill((const int *)(pb + l.box), (const int *)(pb + l.offsets), nf, grid,
                            npairs, keys_tmp, vals_tmp, keys, cell_faces, cell_off, b + 3 * a,
                            ws_bytes - 3 * a, S(stream));
+}
+
+size_t csg_grid_index_build_workspace_bytes(int nf, int capacity)
+{
+    if (nf < 1 || capacity < 1)
+        return 0;
+    const size_t a = align256(sizeof(int) * (size_t)capacity);
+    const size_t sort = grid_sort_scratch_bytes(capacity);
+    return plan_layout(nf).scratch + 3 * a + align256(sort);
+}
+
+int csg_grid_index_build(const void *pts, int pts_dtype, const int *faces, int nf, int grid,
+                         int margin, int capacity, int *cell_off, int *cell_faces, int *required,
+                         void *ws, size_t ws_bytes, void *stream)
+{
+    if (nf < 1 || grid < 1 || margin < 0 || capacity < 1 || !required || !cell_off ||
+        !cell_faces)
+        return fail(kInvalidArgument, "invalid grid index arguments");
+    if ((long long)grid * grid * grid >= (1LL << 30))
+        return fail(kInvalidArgument, "grid size too large");
+    if (ws_bytes < csg_grid_index_build_workspace_bytes(nf, capacity))
+        return fail(kInvalidArgument, "grid index build workspace too small");
+    PlanLayout l = plan_layout(nf);
+    char *b = (char *)ws;
+    const size_t a = align256(sizeof(int) * (size_t)capacity);
+    int *box = (int *)(b + l.box), *counts = (int *)(b + l.counts),
+        *offsets = (int *)(b + l.offsets);
+    int *keys_tmp = (int *)(b + l.scratch), *vals_tmp = (int *)(b + l.scratch + a),
+        *keys = (int *)(b + l.scratch + 2 * a);
+    char *tmp = b + l.scratch + 3 * a;
+    const size_t tmp_bytes = ws_bytes - (l.scratch + 3 * a);
+    if (pts_dtype == 1)
+        return grid_index_build<double>((const double *)pts, faces, nf, grid, margin, capacity,
+                                        box, counts, offsets, keys_tmp, vals_tmp, keys, cell_faces,
+                                        cell_off, required, tmp, tmp_bytes, S(stream));
+    return grid_index_build<float>((const float *)pts, faces, nf, grid, margin, capacity, box,
+                                   counts, offsets, keys_tmp, vals_tmp, keys, cell_faces, cell_off,
+                                   required, tmp, tmp_bytes, S(stream));
 }
 
 size_t csg_rotation_costs_workspace_bytes(int ns, int na)

@@ -10,6 +10,14 @@ int grid_index_count(const P *pts, const int *faces, int nf, int g, int margin, 
                      int *counts, int *offsets, int *total, void *tmp, size_t tmp_bytes,
                      hipStream_t s);
 size_t grid_sort_scratch_bytes(int npairs);
+// The whole index without a host round trip: at most `capacity` pairs are
+// listed (cell_faces holds capacity ints) and *required (device) receives the
+// count the index needs; the index is complete iff *required <= capacity.
+template <typename P>
+int grid_index_build(const P *pts, const int *faces, int nf, int g, int margin, int capacity,
+                     int *box, int *counts, int *offsets, int *keys_tmp, int *vals_tmp, int *keys,
+                     int *cell_faces, int *cell_off, int *required, void *tmp, size_t tmp_bytes,
+                     hipStream_t s);
 int grid_index_fill(const int *box, const int *offsets, int nf, int g, int npairs, int *keys_tmp,
                     int *vals_tmp, int *keys, int *vals, int *cell_off, void *tmp,
                     size_t tmp_bytes, hipStream_t s);

@@ -23,8 +23,11 @@ k_face_boxes(const P *__restrict__ pts, const int *__restrict__ faces, int nf, i
              int *__restrict__ box, int *__restrict__ counts)
 {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nf)
+    if (f >= nf) {
+        if (f == nf)
+            counts[nf] = 0;  // the exclusive scan reads nf + 1 counts
         return;
+    }
     double cell = 2.0 / (double)g;
     int lo[3], hi[3], total = 1;
     for (int d = 0; d < 3; ++d) {
@@ -65,6 +68,42 @@ k_emit_pairs(const int *__restrict__ box, const int *__restrict__ offsets, int n
             }
 }
 
+// Capacity-bounded emission: pairs past `capacity` are dropped (the caller
+// sees the required count in *required and rebuilds with more room).
+__global__ void __launch_bounds__(kBlock)
+k_emit_pairs_capped(const int *__restrict__ box, const int *__restrict__ offsets, int nf, int g,
+                    int capacity, int *__restrict__ keys, int *__restrict__ vals)
+{
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf)
+        return;
+    int o = offsets[f];
+    const int *b = box + 6 * f;
+    for (int z = b[2]; z <= b[5]; ++z)
+        for (int y = b[1]; y <= b[4]; ++y)
+            for (int x = b[0]; x <= b[3]; ++x) {
+                if (o >= capacity)
+                    return;
+                keys[o] = x + g * (y + g * z);
+                vals[o] = f;
+                ++o;
+            }
+}
+
+// Slots [required, capacity) get the key `sentinel` (= the cell count), which
+// sorts after every cell; *required = the pair count the index needs.
+__global__ void __launch_bounds__(kBlock)
+k_pad_keys(const int *__restrict__ total, int capacity, int sentinel, int *__restrict__ keys,
+           int *__restrict__ required)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = *total;
+    if (i == 0)
+        *required = n;
+    if (i < capacity && i >= n)
+        keys[i] = sentinel;
+}
+
 // cell_off[c] = first position of key >= c in the sorted keys.
 __global__ void __launch_bounds__(kBlock)
 k_cell_offsets(const int *__restrict__ keys, int npairs, int ncells, int *__restrict__ off)
@@ -88,8 +127,8 @@ int grid_index_count(const P *pts, const int *faces, int nf, int g, int margin, 
                      int *counts, int *offsets, int *total, void *tmp, size_t tmp_bytes,
                      hipStream_t s)
 {
-    hipLaunchKernelGGL(k_face_boxes<P>, dim3(grid_for(nf)), dim3(kBlock), 0, s, pts, faces, nf, g,
-                       margin, box, counts);
+    hipLaunchKernelGGL(k_face_boxes<P>, dim3(grid_for((size_t)nf + 1)), dim3(kBlock), 0, s, pts,
+                       faces, nf, g, margin, box, counts);
     CSG_LAUNCH_CHECK();
     size_t need = 0;
     CSG_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, counts, offsets, nf + 1, s));
@@ -135,6 +174,49 @@ int grid_index_fill(const int *box, const int *offsets, int nf, int g, int npair
     return kOk;
 }
 
+template <typename P>
+int grid_index_build(const P *pts, const int *faces, int nf, int g, int margin, int capacity,
+                     int *box, int *counts, int *offsets, int *keys_tmp, int *vals_tmp, int *keys,
+                     int *cell_faces, int *cell_off, int *required, void *tmp, size_t tmp_bytes,
+                     hipStream_t s)
+{
+    hipLaunchKernelGGL(k_face_boxes<P>, dim3(grid_for((size_t)nf + 1)), dim3(kBlock), 0, s, pts,
+                       faces, nf, g, margin, box, counts);
+    CSG_LAUNCH_CHECK();
+    size_t need = 0;
+    CSG_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, counts, offsets, nf + 1, s));
+    if (need > tmp_bytes)
+        return fail(kInvalidArgument, "grid index scan scratch too small");
+    CSG_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, need, counts, offsets, nf + 1, s));
+    const int ncells = g * g * g;
+    hipLaunchKernelGGL(k_pad_keys, dim3(grid_for(capacity)), dim3(kBlock), 0, s, offsets + nf,
+                       capacity, ncells, keys_tmp, required);
+    CSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_emit_pairs_capped, dim3(grid_for(nf)), dim3(kBlock), 0, s, box, offsets,
+                       nf, g, capacity, keys_tmp, vals_tmp);
+    CSG_LAUNCH_CHECK();
+    int bits = 1;
+    while ((1LL << bits) <= (long long)ncells)  // keys run to the sentinel ncells
+        ++bits;
+    need = 0;
+    CSG_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, keys_tmp, keys, vals_tmp,
+                                                     cell_faces, capacity, 0, bits, s));
+    if (need > tmp_bytes)
+        return fail(kInvalidArgument, "grid index sort scratch too small");
+    CSG_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, need, keys_tmp, keys, vals_tmp,
+                                                     cell_faces, capacity, 0, bits, s));
+    hipLaunchKernelGGL(k_cell_offsets, dim3(grid_for((size_t)ncells + 1)), dim3(kBlock), 0, s,
+                       keys, capacity, ncells, cell_off);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+template int grid_index_build<float>(const float *, const int *, int, int, int, int, int *, int *,
+                                     int *, int *, int *, int *, int *, int *, int *, void *,
+                                     size_t, hipStream_t);
+template int grid_index_build<double>(const double *, const int *, int, int, int, int, int *,
+                                      int *, int *, int *, int *, int *, int *, int *, int *,
+                                      void *, size_t, hipStream_t);
 template int grid_index_count<float>(const float *, const int *, int, int, int, int *, int *,
                                      int *, int *, void *, size_t, hipStream_t);
 template int grid_index_count<double>(const double *, const int *, int, int, int, int *, int *,

@@ -324,6 +324,16 @@ size_t csg_grid_index_fill_workspace_bytes(int npairs);
 int csg_grid_index_fill(const void *plan_ws, int nf, int grid, int npairs, int *cell_off,
                         int *cell_faces, void *ws, size_t ws_bytes, void *stream);
 
+/* The same index in one asynchronous call (no host round trip): at most `capacity` pairs
+ * are listed (cell_faces: capacity ints, cell_off: grid^3 + 1 ints) and *required (device
+ * int) receives the pair count the index needs.  The index equals the two-phase one iff
+ * *required <= capacity; otherwise it lists a subset (every access stays in bounds) and
+ * must be rebuilt with capacity >= *required. */
+size_t csg_grid_index_build_workspace_bytes(int nf, int capacity);
+int csg_grid_index_build(const void *pts, int pts_dtype, const int *faces, int nf, int grid,
+                         int margin, int capacity, int *cell_off, int *cell_faces, int *required,
+                         void *ws, size_t ws_bytes, void *stream);
+
 /* CAT rotation SSD for na candidate angle triples (device).  The count of points outside
  * the index is kept in a device int: uncovered_dev if given (the call then does not
  * synchronise), else in the workspace; uncovered_host (optional) receives it after a
@@ -384,8 +394,10 @@ int csg_rotation_refine(const float *src_pts, const double *src_vals, int ns,
                         size_t ws_bytes, int *uncovered_dev, void *stream);
 
 /* Exact closest triangle (first minimum among the query cell's candidates) with
- * barycentric weights; far_host counts queries farther than `reach` from their best
- * triangle (their true closest triangle may lie outside the searched cells).
+ * barycentric weights.  The count of queries farther than `reach` from their best
+ * triangle (their true closest triangle may lie outside the searched cells) is left in
+ * the first int of ws (device); far_host, if given, receives it after a synchronisation,
+ * else the call does not synchronise.
  * Replaces the stencil helper's find_closest_polygon_point + weights
  * (native/cat_surface_stencil_builder.c:107-170). */
 int csg_closest_triangles(const double *q, int nq, const double *verts, const int *faces,

@@ -303,6 +303,62 @@ def test_prepare_sphere_matches_golden_and_oracle(golden, cuda):
         O.normalise_sphere(_np(degenerate))
 
 
+def test_grid_index_build_equals_two_phase(cuda):
+    """csg_grid_index_build (one asynchronous call, bounded pair capacity) lists the
+    same cells as the two-phase plan/fill index when the capacity suffices, and
+    reports the required count (staying in bounds) when it does not."""
+
+    from cat_surface_gpu.locate import build_grid_index, build_grid_index_async
+    from cat_surface_gpu.stencil_builder import icosphere
+
+    sv, sf = icosphere(4)
+    rng = np.random.default_rng(5)
+    pts = torch.as_tensor(0.9 * sv + rng.normal(scale=0.01, size=sv.shape), device=cuda)
+    faces = torch.as_tensor(sf, device=cuda)
+    for grid, margin in ((32, 1), (16, 2)):
+        ref = build_grid_index(pts, faces, grid, margin)
+        n = int(ref.cell_faces.numel())
+        for capacity in (n, n + 777):
+            req = torch.zeros(1, dtype=torch.int32, device=cuda)
+            got = build_grid_index_async(pts, faces, grid, margin, capacity, req)
+            assert int(req.item()) == n
+            np.testing.assert_array_equal(_np(got.cell_off), _np(ref.cell_off))
+            np.testing.assert_array_equal(_np(got.cell_faces)[:n], _np(ref.cell_faces))
+        req = torch.zeros(1, dtype=torch.int32, device=cuda)
+        small = build_grid_index_async(pts, faces, grid, margin, n // 3, req)
+        assert int(req.item()) == n
+        off = _np(small.cell_off)
+        assert off[-1] == n // 3 and np.all(np.diff(off) >= 0)
+        listed = _np(small.cell_faces)
+        assert listed.min() >= 0 and listed.max() < sf.shape[0]
+
+
+def test_deferred_stencil_equals_synchronous(cuda):
+    """A stencil built without host round trips (deferred checks) equals the one
+    built with them, and its checks pass."""
+
+    from cat_surface_gpu.locate import check_all, closest_triangles
+    from cat_surface_gpu.stencil_builder import build_device_stencil, icosphere
+
+    sv, sf = icosphere(5)
+    rng = np.random.default_rng(9)
+    sphere = torch.as_tensor((sv + rng.normal(scale=0.003, size=sv.shape)).astype(np.float32),
+                             device=cuda)
+    faces = torch.as_tensor(sf, device=cuda)
+    ref = build_device_stencil(sphere, faces, nx=64, ny=32, device=cuda)  # learns capacities
+    got, checks = build_device_stencil(sphere, faces, nx=64, ny=32, device=cuda, deferred=True)
+    assert all(c is not None for c in checks)
+    assert check_all(checks) == [True, True]
+    for name in ("sphere_points", "surface_indices", "surface_weights", "sheet_indices",
+                 "sheet_weights", "unit_sphere_points"):
+        np.testing.assert_array_equal(_np(getattr(got, name)), _np(getattr(ref, name)), name)
+    # a query set whose closest triangles lie beyond one cell fails its check
+    far_q = torch.as_tensor(np.array([[0.0, 0.0, 0.0], [0.05, 0.0, 0.0]]), device=cuda)
+    face, idx, w, check = closest_triangles(far_q, sphere.double(), faces, grid_size=32,
+                                            margin=1, deferred=True)
+    assert check is not None and check_all([check]) == [False]
+
+
 def test_stencil_builder_matches_brute_force(cuda):
     """Closest-triangle stencil entries equal an exhaustive CPU search."""
 
