@@ -78,6 +78,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_smooth_geometry_batch_workspace_bytes": (_sz, [_i, _i]),
     "csg_smooth_geometry_batch": (_i, [_vp, _i, _vp, _vp, _i, ctypes.POINTER(_i), _vp, _vp, _sz,
                                        _vp]),
+    "csg_smooth_geometry_patched": (_i, [_vp, _i, _i, ctypes.POINTER(_i), _i, _i, _vp, _vp, _vp,
+                                         _vp, _vp, _vp, _vp, _sz, _vp, _vp, _sz, _vp]),
     "csg_vertex_normals": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp]),
     "csg_smooth_values_workspace_bytes": (_sz, [_i, _i, _i]),
     "csg_smooth_values": (_i, [_vp, _i, _i, _vp, _i, ctypes.POINTER(_i), _vp, _vp, _i, _dp,

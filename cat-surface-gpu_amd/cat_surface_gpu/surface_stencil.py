@@ -100,10 +100,142 @@ class Topology:
     ell_width: int = 0
     ell_cols: torch.Tensor = None
     ell_src: torch.Tensor = None
+    nbr_off_np: np.ndarray = field(repr=False, default=None)
+    nbr_np: np.ndarray = field(repr=False, default=None)
+    patch_plans: dict = field(repr=False, default_factory=dict)
 
     @property
     def nnz(self) -> int:
         return int(self.nbr.numel())
+
+
+@dataclass
+class SmoothingPatches:
+    """A partition of a mesh for csg_smooth_geometry_patched (device arrays)."""
+
+    count: int
+    rings: int
+    vstart: torch.Tensor
+    ring_end: torch.Tensor
+    lverts: torch.Tensor
+    aoff: torch.Tensor
+    astart: torch.Tensor
+    ladj: torch.Tensor  # int16 storage of the uint16 local indices
+    perm: torch.Tensor  # working order: working index i holds vertex perm[i]
+    lds_bytes: int
+    largest: int  # local vertices of the largest patch
+
+
+def _morton_order(points: np.ndarray) -> np.ndarray:
+    """Vertices in Morton (Z-curve) order of their positions, 10 bits per axis."""
+
+    p = np.asarray(points, dtype=np.float64)
+    lo, hi = p.min(0), p.max(0)
+    q = np.clip(((p - lo) / np.maximum(hi - lo, 1e-30) * 1023.0).astype(np.int64), 0, 1023)
+    code = np.zeros(p.shape[0], np.int64)
+    for bit in range(10):
+        for d in range(3):
+            code |= ((q[:, d] >> bit) & 1) << (3 * bit + d)
+    return np.argsort(code, kind="stable")
+
+
+def smoothing_patches(topology: Topology, positions, *, core: int = 256,
+                      rings: int = 8) -> SmoothingPatches:
+    """Patches of ``core`` vertices (consecutive in Morton order of ``positions``),
+    each with the ``rings`` rings of its graph neighbourhood, for iterating the
+    geometry smoothing ``rings`` times per launch; cached on the topology per
+    (core, rings) (the partition only affects speed, never values)."""
+
+    import scipy.sparse as sp
+
+    key = (int(core), int(rings))
+    plan = topology.patch_plans.get(key)
+    if plan is not None:
+        return plan
+    if core < 1 or not 1 <= rings <= 64:
+        raise ValueError("core must be positive and rings in [1, 64]")
+    n = topology.n_points
+    off = topology.nbr_off_np.astype(np.int64)
+    nbr = topology.nbr_np.astype(np.int64)
+    pos = positions.detach().cpu().numpy() if isinstance(positions, torch.Tensor) else positions
+    order = _morton_order(np.asarray(pos).reshape(n, 3))
+    count = (n + core - 1) // core
+    if count > 65535:
+        raise ValueError("too many smoothing patches; use a larger core")
+    pid = np.empty(n, np.int64)
+    pid[order] = np.arange(n) // core
+    adj = sp.csr_matrix((np.ones(nbr.size, np.int32), nbr, off), shape=(n, n))
+    # ring 0: each patch's core in Morton order; ring r + 1: unvisited neighbours
+    # of ring r, ascending
+    ring_pairs = [(pid[order], order)]
+    member = sp.csr_matrix((np.ones(n, np.int32), (pid, np.arange(n))), shape=(count, n))
+    visited, frontier = member.copy(), member
+    for _ in range(rings):
+        nxt = frontier @ adj
+        nxt.data[:] = 1
+        nxt = nxt - nxt.multiply(visited)
+        nxt.eliminate_zeros()
+        nxt.data[:] = 1
+        nxt.sort_indices()
+        visited = visited + nxt
+        frontier = nxt
+        coo = nxt.tocoo()
+        ring_pairs.append((coo.row.astype(np.int64), coo.col.astype(np.int64)))
+    # local layout: per patch, rings in order
+    patch = np.concatenate([rp[0] for rp in ring_pairs])
+    vert = np.concatenate([rp[1] for rp in ring_pairs])
+    ring = np.concatenate([np.full(rp[0].size, r, np.int64) for r, rp in enumerate(ring_pairs)])
+    layout = np.lexsort((np.arange(patch.size), ring, patch))  # stable within a ring
+    patch, vert, ring = patch[layout], vert[layout], ring[layout]
+    sizes = np.bincount(patch, minlength=count)
+    vstart = np.zeros(count + 1, np.int64)
+    vstart[1:] = np.cumsum(sizes)
+    if sizes.max() >= 65536:
+        raise ValueError("a smoothing patch has more than 65535 local vertices")
+    local = np.arange(patch.size) - vstart[patch]
+    per_ring = np.zeros((count, rings + 1), np.int64)
+    np.add.at(per_ring, (patch, ring), 1)
+    ring_end = np.cumsum(per_ring, axis=1)
+    # local adjacency of every vertex in rings < rings, in the mesh's CSR order
+    deg = np.where(ring < rings, off[vert + 1] - off[vert], 0)
+    lookup_key = patch * n + vert
+    sort = np.argsort(lookup_key, kind="stable")
+    sorted_key = lookup_key[sort]
+    src = np.repeat(np.arange(patch.size), deg)
+    first = np.repeat(off[vert], deg)
+    within = np.arange(src.size) - np.repeat(np.cumsum(deg) - deg, deg)
+    nb = nbr[first + within]
+    want = patch[src] * n + nb
+    at = np.searchsorted(sorted_key, want)
+    if at.size and (at.max() >= sorted_key.size or np.any(sorted_key[at] != want)):
+        raise AssertionError("a smoothed vertex has a neighbour outside its patch")
+    ladj = local[sort[at]]
+    # per patch CSR offsets relative to the patch's first adjacency entry
+    edges = np.bincount(patch, weights=deg, minlength=count).astype(np.int64)
+    astart = np.zeros(count + 1, np.int64)
+    astart[1:] = np.cumsum(edges)
+    aoff = np.zeros(patch.size + count, np.int64)
+    cum = np.cumsum(deg) - np.repeat(astart[:-1], sizes)  # end of each vertex's list
+    slot = np.arange(patch.size) + patch + 1
+    aoff[slot] = cum
+    if edges.max() >= 65536:
+        raise ValueError("a smoothing patch has more than 65535 adjacency entries")
+    lds = 24 * sizes + 2 * (sizes + 1) + 2 * edges
+    lds_bytes = int((lds.max() + 15) // 16 * 16)
+
+    def dev(a, dtype=torch.int32):
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=topology.nbr.device)
+
+    # working order = Morton order: cores are contiguous, halos lie in nearby cores
+    rank = np.empty(n, np.int64)
+    rank[order] = np.arange(n)
+    plan = SmoothingPatches(
+        count=count, rings=int(rings), vstart=dev(vstart), ring_end=dev(ring_end.reshape(-1)),
+        lverts=dev(rank[vert]), aoff=dev(aoff), astart=dev(astart), perm=dev(order),
+        ladj=dev(ladj.astype(np.uint16).view(np.int16), torch.int16), lds_bytes=lds_bytes,
+        largest=int(sizes.max()))
+    topology.patch_plans[key] = plan
+    return plan
 
 
 def topology_for(faces: np.ndarray, n_points: int, device: torch.device) -> Topology:
@@ -164,6 +296,8 @@ def topology_for(faces: np.ndarray, n_points: int, device: torch.device) -> Topo
         sys_contrib=dev(perm.astype(np.int32)),
         colour_off_dev=dev(colour_off),
         faces_np=f,
+        nbr_off_np=np.asarray(nbr_off),
+        nbr_np=np.asarray(nbr),
         ell_width=width,
         ell_cols=dev(ell_cols.reshape(-1)),
         ell_src=dev(ell_src.reshape(-1)),
@@ -523,10 +657,33 @@ def edge_means(stencils, geometries) -> list[float]:
     return [float(t) / (3.0 * nf) for t, nf in zip(host, nfs)]
 
 
+# Patch shape (core vertices, rings = iterations per launch) of the batched
+# geometry smoothing; None: one launch per iteration.
+_GEOMETRY_PATCHES: tuple[int, int] | None = (320, 4)
+
+
+def set_geometry_smoothing(patches: tuple[int, int] | None) -> tuple[int, int] | None:
+    """Select the batched geometry smoothing's schedule: (core, rings) patches
+    iterated in LDS (csg_smooth_geometry_patched), or None for one launch per
+    iteration (csg_smooth_geometry_batch).  Both give identical values.
+    Returns the previous setting."""
+
+    global _GEOMETRY_PATCHES
+    previous = _GEOMETRY_PATCHES
+    if patches is not None:
+        core, rings = (int(v) for v in patches)
+        if core < 1 or not 1 <= rings <= 64:
+            raise ValueError("patches must be (core >= 1, 1 <= rings <= 64)")
+        patches = (core, rings)
+    _GEOMETRY_PATCHES = patches
+    return previous
+
+
 def smooth_geometry_many(stencils, geometries, iterations) -> list[torch.Tensor]:
     """SurfaceStencilDevice.smooth_geometry of several meshes with given
     iteration counts; meshes that share a topology are smoothed in the same
-    launches (csg_smooth_geometry_batch).  Results equal one call per mesh."""
+    launches (csg_smooth_geometry_patched, or csg_smooth_geometry_batch when
+    patches are off).  Results equal one call per mesh."""
 
     out: list[torch.Tensor | None] = [None] * len(stencils)
     groups: dict[int, list[int]] = {}
@@ -544,10 +701,21 @@ def smooth_geometry_many(stencils, geometries, iterations) -> list[torch.Tensor]
             ws = _lib.workspace(_lib.lib().csg_smooth_geometry_batch_workspace_bytes(
                 n, len(chunk)), dev)
             its = (_lib.ctypes.c_int * len(chunk))(*[int(iterations[i]) for i in chunk])
-            _lib.check(_lib.lib().csg_smooth_geometry_batch(
-                _lib.ptr(v), len(chunk), _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), n, its,
-                _lib.ptr(res), _lib.ptr(ws), ws.numel(), stencils[chunk[0]]._stream()),
-                "csg_smooth_geometry_batch")
+            if _GEOMETRY_PATCHES is not None and topo.nbr_off_np is not None:
+                pp = smoothing_patches(topo, v[0], core=_GEOMETRY_PATCHES[0],
+                                       rings=_GEOMETRY_PATCHES[1])
+                _lib.check(_lib.lib().csg_smooth_geometry_patched(
+                    _lib.ptr(v), len(chunk), n, its, pp.count, pp.rings, _lib.ptr(pp.perm),
+                    _lib.ptr(pp.vstart),
+                    _lib.ptr(pp.ring_end), _lib.ptr(pp.lverts), _lib.ptr(pp.aoff),
+                    _lib.ptr(pp.astart), _lib.ptr(pp.ladj), pp.lds_bytes, _lib.ptr(res),
+                    _lib.ptr(ws), ws.numel(), stencils[chunk[0]]._stream()),
+                    "csg_smooth_geometry_patched")
+            else:
+                _lib.check(_lib.lib().csg_smooth_geometry_batch(
+                    _lib.ptr(v), len(chunk), _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), n, its,
+                    _lib.ptr(res), _lib.ptr(ws), ws.numel(), stencils[chunk[0]]._stream()),
+                    "csg_smooth_geometry_batch")
             for k, i in enumerate(chunk):
                 out[i] = res[k]
     return out

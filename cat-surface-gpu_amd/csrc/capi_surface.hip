@@ -169,6 +169,37 @@ int csg_smooth_geometry_batch(const float *v, int batch, const int *nbr_off, con
     return kOk;
 }
 
+int csg_smooth_geometry_patched(const float *v, int batch, int n, const int *iterations,
+                                int npatches, int rings, const int *perm, const int *vstart,
+                                const int *ring_end, const int *lverts, const int *aoff,
+                                const int *astart, const unsigned short *ladj, size_t lds_bytes,
+                                float *out, void *ws, size_t ws_bytes, void *stream)
+{
+    if (n < 1 || batch < 1 || batch > kMaxSmoothBatch || iterations == nullptr)
+        return fail(kInvalidArgument, "invalid patched smoothing arguments (batch <= %d)",
+                    kMaxSmoothBatch);
+    if (npatches < 1 || npatches > 65535 || rings < 1 || rings > 64 || !perm || !vstart ||
+        !ring_end || !lverts || !aoff || !astart || !ladj)
+        return fail(kInvalidArgument, "invalid smoothing patches");
+    if (lds_bytes < sizeof(float4) || lds_bytes > 160 * 1024)
+        return fail(kInvalidArgument, "patch LDS size must be in (0, 160 KiB]");
+    if (ws_bytes < csg_smooth_geometry_batch_workspace_bytes(n, batch))
+        return fail(kInvalidArgument, "patched smoothing workspace too small");
+    SmoothBatch sb{};
+    sb.count = batch;
+    int most = 0;
+    for (int b = 0; b < batch; ++b) {
+        if (iterations[b] < 0)
+            return fail(kInvalidArgument, "negative smoothing iterations");
+        sb.iters[b] = iterations[b];
+        most = iterations[b] > most ? iterations[b] : most;
+    }
+    SmoothPatches pp{npatches, rings, vstart, ring_end, lverts, aoff, astart, perm, ladj};
+    float4 *b0 = (float4 *)ws;
+    return launch_smooth_geometry_patched(v, b0, b0 + (size_t)n * batch, n, pp, sb, most,
+                                          lds_bytes, out, S(stream));
+}
+
 int csg_vertex_normals(const float *v, const int *faces, const int *inc_off, const int *inc,
                        int n, float *out, void *stream)
 {

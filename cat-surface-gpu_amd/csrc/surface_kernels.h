@@ -23,6 +23,23 @@ struct SmoothBatch {
     int iters[kMaxSmoothBatch];
 };
 
+// Patches for k_smooth_geometry_patch (device arrays).  The working buffers
+// hold the vertices in the order perm (working index i = vertex perm[i]), the
+// patches' cores one after another; patch p's local vertices are working
+// indices lverts[vstart[p] .. vstart[p + 1]) (core first, then rings
+// 1..rings); ring_end[p * (rings + 1) + r] = local count of rings 0..r; the
+// local adjacency of local vertex j is ladj[astart[p] + aoff[vstart[p] + p + j]
+// .. astart[p] + aoff[vstart[p] + p + j + 1]) (local indices, neighbour order
+// of the mesh's CSR; empty for the outermost ring).
+struct SmoothPatches {
+    int count, rings;
+    const int *vstart, *ring_end, *lverts, *aoff, *astart, *perm;
+    const unsigned short *ladj;
+};
+int launch_smooth_geometry_patched(const float *v, float4 *buf0, float4 *buf1, int n,
+                                   const SmoothPatches &pp, const SmoothBatch &sb, int most,
+                                   size_t lds_bytes, float *out, hipStream_t s);
+
 template <typename T>
 int launch_resample(const T *x, const int *idx, const double *w, int n, T *out, hipStream_t s);
 int launch_edge_sum(const float *v, const int *faces, int nf, double *partial, int *nblocks,

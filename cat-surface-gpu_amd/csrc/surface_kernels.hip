@@ -14,6 +14,8 @@
 // reference adds them, so every sum is deterministic and atomic-free.
 #include "surface_kernels.h"
 
+#include <mutex>
+
 namespace csg {
 
 __device__ __forceinline__ float norm3f(float x, float y, float z)
@@ -153,6 +155,100 @@ k_smooth_geometry4(float4 *__restrict__ buf0, float4 *__restrict__ buf1,
                          (float)(((double)c.z + sz / 3.0) / den), 0.0f);
 }
 
+// Up to `rings` iterations of k_smooth_geometry4 in one launch, one
+// workgroup per (patch, mesh).  A patch lists its core vertices (ring 0) and
+// the rings 1..rings of vertices at growing graph distance; every neighbour of
+// a vertex in rings < `rings` is in the patch.  The workgroup stages the
+// patch's positions and local adjacency in LDS, and iteration t of m updates
+// rings 0..m-1-t (whose neighbours then hold iterate t), so after m
+// iterations the core holds exactly the per-iteration kernel's values (same
+// arithmetic, neighbours in the same order).  Meshes with fewer than m
+// iterations left (m = 0: a copy) are advanced by what they have left.
+constexpr int kPatchThreads = 1024;
+constexpr int kPatchGather = 6;  // neighbours gathered together (icosphere degrees 5, 6)
+
+__global__ void __launch_bounds__(kPatchThreads)
+k_smooth_geometry_patch(const float4 *__restrict__ in, float4 *__restrict__ out, int n,
+                        SmoothPatches pp, SmoothBatch sb, int it0)
+{
+    extern __shared__ float g_patch_lds[];
+    const int p = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+    const int s0 = pp.vstart[p], L = pp.vstart[p + 1] - s0;
+    const int a0 = pp.astart[p], E = pp.astart[p + 1] - a0;
+    const int *lv = pp.lverts + s0;
+    const int *ring_end = pp.ring_end + p * (pp.rings + 1);
+    const float4 *src = in + (size_t)b * n;
+    float4 *dst = out + (size_t)b * n;
+    int m = sb.iters[b] - it0;
+    m = m < 0 ? 0 : (m > pp.rings ? pp.rings : m);
+    const int core = ring_end[0];
+    if (m == 0) {
+        for (int j = tid; j < core; j += kPatchThreads)
+            dst[lv[j]] = src[lv[j]];
+        return;
+    }
+    // LDS: two position buffers as x, y, z planes of L floats, then the local
+    // CSR offsets (L + 1) and neighbours (E) as 16-bit indices
+    float *buf[2] = {g_patch_lds, g_patch_lds + 3 * L};
+    unsigned short *off = reinterpret_cast<unsigned short *>(g_patch_lds + 6 * L);
+    unsigned short *adj = off + L + 1;
+    const int *aoff = pp.aoff + s0 + p;
+    for (int j = tid; j < L; j += kPatchThreads) {
+        const float4 v = src[lv[j]];
+        buf[0][j] = v.x;
+        buf[0][L + j] = v.y;
+        buf[0][2 * L + j] = v.z;
+    }
+    for (int j = tid; j <= L; j += kPatchThreads)
+        off[j] = (unsigned short)aoff[j];
+    for (int k = tid; k < E; k += kPatchThreads)
+        adj[k] = pp.ladj[a0 + k];
+    __syncthreads();
+    for (int t = 0; t < m; ++t) {
+        const float *cx = buf[t & 1], *cy = cx + L, *cz = cy + L;
+        float *nx = buf[(t + 1) & 1], *ny = nx + L, *nz = ny + L;
+        const int count = ring_end[m - 1 - t];
+        for (int j = tid; j < count; j += kPatchThreads) {
+            const int bo = off[j], e = off[j + 1];
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            // the first kPatchGather neighbours' loads are issued together; the
+            // sums still run in neighbour order
+            int q[kPatchGather];
+#pragma unroll
+            for (int k = 0; k < kPatchGather; ++k)
+                q[k] = bo + k < e ? adj[bo + k] : j;
+            float qx[kPatchGather], qy[kPatchGather], qz[kPatchGather];
+#pragma unroll
+            for (int k = 0; k < kPatchGather; ++k) {
+                qx[k] = cx[q[k]];
+                qy[k] = cy[q[k]];
+                qz[k] = cz[q[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < kPatchGather; ++k)
+                if (bo + k < e) {
+                    sx += (double)qx[k];
+                    sy += (double)qy[k];
+                    sz += (double)qz[k];
+                }
+            for (int k = bo + kPatchGather; k < e; ++k) {
+                const int r = adj[k];
+                sx += (double)cx[r];
+                sy += (double)cy[r];
+                sz += (double)cz[r];
+            }
+            const double den = 1.0 + (double)(e - bo) / 3.0;
+            nx[j] = (float)(((double)cx[j] + sx / 3.0) / den);
+            ny[j] = (float)(((double)cy[j] + sy / 3.0) / den);
+            nz[j] = (float)(((double)cz[j] + sz / 3.0) / den);
+        }
+        __syncthreads();
+    }
+    const float *fx = buf[m & 1], *fy = fx + L, *fz = fy + L;
+    for (int j = tid; j < core; j += kPatchThreads)
+        dst[lv[j]] = make_float4(fx[j], fy[j], fz[j], 0.0f);
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_pack4(const float *__restrict__ v, int n, float4 *__restrict__ out)
 {
@@ -175,6 +271,35 @@ k_unpack4(const float4 *__restrict__ buf0, const float4 *__restrict__ buf1, int 
         return;
     const float4 q = ((sb.iters[b] & 1) ? buf1 : buf0)[b * n + i];
     float *o = out + (b * n + i) * 3;
+    o[0] = q.x;
+    o[1] = q.y;
+    o[2] = q.z;
+}
+
+// k_pack4 / k_unpack4 through a vertex permutation: working position i of
+// mesh b holds vertex perm[i].
+__global__ void __launch_bounds__(kBlock)
+k_pack4_perm(const float *__restrict__ v, int n, const int *__restrict__ perm,
+             float4 *__restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t b = blockIdx.y;
+    if (i >= n)
+        return;
+    const float *p = v + (b * n + perm[i]) * 3;
+    out[b * n + i] = make_float4(p[0], p[1], p[2], 0.0f);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_unpack4_perm(const float4 *__restrict__ buf, int n, const int *__restrict__ perm,
+               float *__restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t b = blockIdx.y;
+    if (i >= n)
+        return;
+    const float4 q = buf[b * n + i];
+    float *o = out + (b * n + perm[i]) * 3;
     o[0] = q.x;
     o[1] = q.y;
     o[2] = q.z;
@@ -653,6 +778,45 @@ int launch_smooth_geometry4(const float *v, float4 *buf0, float4 *buf1, const in
         CSG_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_unpack4, g, dim3(kBlock), 0, s, buf0, buf1, n, sb, out);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+int launch_smooth_geometry_patched(const float *v, float4 *buf0, float4 *buf1, int n,
+                                   const SmoothPatches &pp, const SmoothBatch &sb, int most,
+                                   size_t lds_bytes, float *out, hipStream_t s)
+{
+    static std::once_flag once;
+    static size_t limit = 65536;
+    std::call_once(once, [] {
+        int dev = 0, per_block = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
+                hipSuccess &&
+            per_block > 0 &&
+            hipFuncSetAttribute((const void *)k_smooth_geometry_patch,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                per_block) == hipSuccess)
+            limit = (size_t)per_block;
+        (void)hipGetLastError();
+    });
+    if (lds_bytes > limit)
+        return fail(kInvalidArgument, "smoothing patches need %zu bytes of LDS, the device has %zu",
+                    lds_bytes, limit);
+    dim3 g(grid_for(n), sb.count);
+    hipLaunchKernelGGL(k_pack4_perm, g, dim3(kBlock), 0, s, v, n, pp.perm, buf0);
+    CSG_LAUNCH_CHECK();
+    int launches = 0;
+    for (int it0 = 0; it0 < most; it0 += pp.rings, ++launches) {
+        const float4 *in = (launches & 1) ? buf1 : buf0;
+        float4 *o = (launches & 1) ? buf0 : buf1;
+        hipLaunchKernelGGL(k_smooth_geometry_patch, dim3(pp.count, sb.count),
+                           dim3(kPatchThreads), lds_bytes, s, in, o, n, pp, sb, it0);
+        CSG_LAUNCH_CHECK();
+    }
+    // every mesh's result is in the buffer the last launch wrote
+    hipLaunchKernelGGL(k_unpack4_perm, g, dim3(kBlock), 0, s, (launches & 1) ? buf1 : buf0, n,
+                       pp.perm, out);
     CSG_LAUNCH_CHECK();
     return kOk;
 }

@@ -216,6 +216,24 @@ int csg_smooth_geometry_batch(const float *v, int batch, const int *nbr_off, con
                               const int *iterations, float *out, void *ws, size_t ws_bytes,
                               void *stream);
 
+/* csg_smooth_geometry_batch in launches of `rings` iterations each: one workgroup per
+ * (patch, mesh) iterates in LDS on a patch of core vertices and the `rings` rings of
+ * their graph neighbourhood (a partition of the mesh computed once per topology, see
+ * surface_stencil.smoothing_patches), writing the core.  Identical values to
+ * csg_smooth_geometry_batch.  Patch arrays (device): perm [n] (the working order: working
+ * index i holds vertex perm[i], the patches' cores one after another), vstart
+ * [npatches + 1], ring_end [npatches (rings + 1)] (local counts of rings 0..r), lverts
+ * (working indices, core first),
+ * aoff [vstart[npatches] + npatches] (per patch, local CSR offsets relative to astart[p]),
+ * astart [npatches + 1], ladj (uint16 local neighbour indices in the mesh's CSR order);
+ * lds_bytes = the largest patch's 24 L + 2 (L + 1) + 2 E bytes (E < 65536).  Workspace as
+ * csg_smooth_geometry_batch_workspace_bytes. */
+int csg_smooth_geometry_patched(const float *v, int batch, int n, const int *iterations,
+                                int npatches, int rings, const int *perm, const int *vstart,
+                                const int *ring_end, const int *lverts, const int *aoff,
+                                const int *astart, const unsigned short *ladj, size_t lds_bytes,
+                                float *out, void *ws, size_t ws_bytes, void *stream);
+
 /* Angle-weighted vertex normals; inc lists face*3+corner per vertex in (corner, face)
  * order.  Replaces SurfaceStencilDevice.vertex_normals (surface_stencil.py:478-505). */
 int csg_vertex_normals(const float *v, const int *faces, const int *inc_off, const int *inc,

@@ -524,3 +524,42 @@ def test_geometry_smoothing_padded_equals_packed(level, iterations, cuda):
     for b, k in enumerate(iterations):
         if k == 0:
             assert torch.equal(outs[0][b], geom[b])
+
+
+@pytest.mark.parametrize("level,iterations,patches", [
+    (6, (50, 13, 0, 7), (256, 8)), (6, (283, 291, 17, 300), (512, 4)),
+    (3, (5, 2), (64, 3)), (5, tuple(range(9)), (256, 8)), (4, (20, 21), (100000, 6))])
+def test_geometry_smoothing_patched_equals_per_iteration(level, iterations, patches, cuda):
+    """Patch-local iterations in LDS (csg_smooth_geometry_patched) give the bits of
+    one launch per iteration (csg_smooth_geometry_batch), for iteration counts
+    below, at and between multiples of the rings per launch, and for one patch
+    covering the whole mesh."""
+
+    from cat_surface_gpu import _lib
+    from cat_surface_gpu.stencil_builder import icosphere
+    from cat_surface_gpu.surface_stencil import smoothing_patches, topology_for
+
+    v, f = icosphere(level)
+    n = v.shape[0]
+    topo = topology_for(f.astype(np.int32), n, cuda)
+    rng = np.random.default_rng(level + 100)
+    batch = len(iterations)
+    geom = torch.as_tensor(v[None] * 50.0 * (1.0 + 0.05 * rng.standard_normal((batch, n, 1))),
+                           dtype=torch.float32, device=cuda).contiguous()
+    its = (_lib.ctypes.c_int * batch)(*iterations)
+    lib = _lib.lib()
+    nbytes = lib.csg_smooth_geometry_batch_workspace_bytes(n, batch)
+    ws = _lib.workspace(nbytes, cuda)
+    ref = torch.empty_like(geom)
+    _lib.check(lib.csg_smooth_geometry_batch(
+        _lib.ptr(geom), batch, _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), n, its,
+        _lib.ptr(ref), _lib.ptr(ws), nbytes, _lib.stream_handle(cuda)), "batch")
+    pp = smoothing_patches(topo, geom[0], core=patches[0], rings=patches[1])
+    out = torch.empty_like(geom)
+    _lib.check(lib.csg_smooth_geometry_patched(
+        _lib.ptr(geom), batch, n, its, pp.count, pp.rings, _lib.ptr(pp.perm), _lib.ptr(pp.vstart),
+        _lib.ptr(pp.ring_end), _lib.ptr(pp.lverts), _lib.ptr(pp.aoff), _lib.ptr(pp.astart),
+        _lib.ptr(pp.ladj), pp.lds_bytes, _lib.ptr(out), _lib.ptr(ws), nbytes,
+        _lib.stream_handle(cuda)), "patched")
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
