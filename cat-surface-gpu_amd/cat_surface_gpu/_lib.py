@@ -94,6 +94,9 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_depth_potential": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i,
                                  ctypes.POINTER(_i), _vp, _i, _vp, _vp, _i, _d, _i, _d, _vp, _vp,
                                  _vp, _sz, _vp]),
+    "csg_depth_potential_patched": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp,
+                                         _i, ctypes.POINTER(_i), _vp, _i, _vp, _vp, _i, _d, _i,
+                                         _d, _vp, _vp, _vp, _vp, _sz, _vp]),
     "csg_apply_flow": (_i, [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp]),
     "csg_average_xz": (_i, [_vp, _vp, _i, _i, _dp, _vp, _vp]),
     "csg_normalise_points": (_i, [_vp, _i, _vp, _vp]),

@@ -17,7 +17,7 @@ import torch
 
 from . import _lib
 from .surface_stencil import (SurfaceStencilDevice, _minmax, edge_means,
-                              heat_parameters_from_edge_mean, smooth_geometry_many)
+                              heat_parameters_from_edge_mean, smooth_geometry_many, sor_patches)
 
 
 @dataclass(frozen=True)
@@ -40,6 +40,21 @@ def set_sor_placement(mode) -> int:
     if code not in (0, 1, 2):
         raise ValueError("SOR placement must be 0, 1 or 2")
     return int(_lib.lib().csg_set_sor_placement(code))
+
+
+# Depth-potential sweeps blocked in time on mesh patches (csg_depth_potential_patched);
+# False: one synchronisation per colour (csg_depth_potential).  Identical values.
+_SOR_PATCHES = True
+
+
+def set_sor_patches(enabled: bool) -> bool:
+    """Select the patch-blocked SOR schedule (default) or the per-colour one;
+    returns the previous setting."""
+
+    global _SOR_PATCHES
+    previous = _SOR_PATCHES
+    _SOR_PATCHES = bool(enabled)
+    return previous
 
 
 def depth_potential(stencil: SurfaceStencilDevice, smoothed_surface, *,
@@ -71,14 +86,20 @@ def depth_potential(stencil: SurfaceStencilDevice, smoothed_surface, *,
     ws = _lib.workspace(lib.csg_depth_workspace_bytes(n, nf, nnz, batch, topo.ell_width),
                         pts.device)
     coff = (ctypes.c_int * topo.colour_off.size)(*topo.colour_off.tolist())
-    _lib.check(lib.csg_depth_potential(
+    patches = None
+    if _SOR_PATCHES and topo.colour_order_np is not None:
+        # one patch per CU of an XCD: all of a mesh's patches run on its XCD at once
+        per_xcd = max(1, torch.cuda.get_device_properties(pts.device).multi_processor_count // 8)
+        # the partition follows the stencil's sphere (compact patches)
+        patches = sor_patches(topo, stencil.sphere_points, core=-(-n // per_xcd)).abi()
+    _lib.check(lib.csg_depth_potential_patched(
         _lib.ptr(pts), batch, n, _lib.ptr(topo.faces), nf, _lib.ptr(topo.inc_off),
         _lib.ptr(topo.inc), _lib.ptr(topo.sys_row_off), _lib.ptr(topo.sys_cols),
         _lib.ptr(topo.sys_contrib_off), _lib.ptr(topo.sys_contrib), nnz, coff,
         _lib.ptr(topo.colour_verts), int(topo.colour_off.size - 1), _lib.ptr(topo.ell_cols),
         _lib.ptr(topo.ell_src), int(topo.ell_width), float(alpha), int(max_iter),
-        float(relaxation), _lib.ptr(x), _lib.ptr(rel), _lib.ptr(ws), ws.numel(),
-        _lib.stream_handle(pts.device)),
+        float(relaxation), None if patches is None else ctypes.byref(patches), _lib.ptr(x),
+        _lib.ptr(rel), _lib.ptr(ws), ws.numel(), _lib.stream_handle(pts.device)),
         "csg_depth_potential")
     if single:
         return x[0], int(max_iter), rel[0]

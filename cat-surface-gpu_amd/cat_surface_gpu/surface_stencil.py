@@ -102,6 +102,8 @@ class Topology:
     ell_src: torch.Tensor = None
     nbr_off_np: np.ndarray = field(repr=False, default=None)
     nbr_np: np.ndarray = field(repr=False, default=None)
+    colour_order_np: np.ndarray = field(repr=False, default=None)  # solve order -> vertex
+    ell_cols_np: np.ndarray = field(repr=False, default=None)      # [ell_width, n]
     patch_plans: dict = field(repr=False, default_factory=dict)
 
     @property
@@ -139,34 +141,29 @@ def _morton_order(points: np.ndarray) -> np.ndarray:
     return np.argsort(code, kind="stable")
 
 
-def smoothing_patches(topology: Topology, positions, *, core: int = 256,
-                      rings: int = 8) -> SmoothingPatches:
-    """Patches of ``core`` vertices (consecutive in Morton order of ``positions``),
-    each with the ``rings`` rings of its graph neighbourhood, for iterating the
-    geometry smoothing ``rings`` times per launch; cached on the topology per
-    (core, rings) (the partition only affects speed, never values)."""
+def _host_points(positions, n: int) -> np.ndarray:
+    pos = positions.detach().cpu().numpy() if isinstance(positions, torch.Tensor) else positions
+    return np.asarray(pos).reshape(n, 3)
+
+
+def _patch_layout(topology: Topology, order: np.ndarray, core: int, rings: int):
+    """Patches of ``core`` consecutive vertices of ``order`` and the ``rings`` rings
+    of their graph neighbourhood: (count, patch, vertex, ring, sizes, vstart) with
+    the (patch, vertex, ring) triples sorted by patch, then ring (cores in
+    ``order``, each ring by ascending vertex)."""
 
     import scipy.sparse as sp
 
-    key = (int(core), int(rings))
-    plan = topology.patch_plans.get(key)
-    if plan is not None:
-        return plan
-    if core < 1 or not 1 <= rings <= 64:
-        raise ValueError("core must be positive and rings in [1, 64]")
     n = topology.n_points
     off = topology.nbr_off_np.astype(np.int64)
     nbr = topology.nbr_np.astype(np.int64)
-    pos = positions.detach().cpu().numpy() if isinstance(positions, torch.Tensor) else positions
-    order = _morton_order(np.asarray(pos).reshape(n, 3))
     count = (n + core - 1) // core
     if count > 65535:
-        raise ValueError("too many smoothing patches; use a larger core")
+        raise ValueError("too many patches; use a larger core")
     pid = np.empty(n, np.int64)
     pid[order] = np.arange(n) // core
     adj = sp.csr_matrix((np.ones(nbr.size, np.int32), nbr, off), shape=(n, n))
-    # ring 0: each patch's core in Morton order; ring r + 1: unvisited neighbours
-    # of ring r, ascending
+    # ring 0: each patch's core in `order`; ring r + 1: unvisited neighbours of ring r
     ring_pairs = [(pid[order], order)]
     member = sp.csr_matrix((np.ones(n, np.int32), (pid, np.arange(n))), shape=(count, n))
     visited, frontier = member.copy(), member
@@ -181,7 +178,6 @@ def smoothing_patches(topology: Topology, positions, *, core: int = 256,
         frontier = nxt
         coo = nxt.tocoo()
         ring_pairs.append((coo.row.astype(np.int64), coo.col.astype(np.int64)))
-    # local layout: per patch, rings in order
     patch = np.concatenate([rp[0] for rp in ring_pairs])
     vert = np.concatenate([rp[1] for rp in ring_pairs])
     ring = np.concatenate([np.full(rp[0].size, r, np.int64) for r, rp in enumerate(ring_pairs)])
@@ -190,26 +186,57 @@ def smoothing_patches(topology: Topology, positions, *, core: int = 256,
     sizes = np.bincount(patch, minlength=count)
     vstart = np.zeros(count + 1, np.int64)
     vstart[1:] = np.cumsum(sizes)
+    return count, patch, vert, ring, sizes, vstart
+
+
+def _local_lookup(patch: np.ndarray, vert: np.ndarray, n: int, vstart: np.ndarray):
+    """A function mapping (patch, vertex) arrays to local indices in the patch."""
+
+    key = patch * n + vert
+    sort = np.argsort(key, kind="stable")
+    sorted_key = key[sort]
+    local = np.arange(patch.size) - vstart[patch]
+
+    def find(p, v):
+        want = p * n + v
+        at = np.searchsorted(sorted_key, want)
+        if at.size and (at.max() >= sorted_key.size or np.any(sorted_key[at] != want)):
+            raise AssertionError("a patch row reaches outside its patch")
+        return local[sort[at]]
+
+    return find
+
+
+def smoothing_patches(topology: Topology, positions, *, core: int = 256,
+                      rings: int = 8) -> SmoothingPatches:
+    """Patches of ``core`` vertices (consecutive in Morton order of ``positions``),
+    each with the ``rings`` rings of its graph neighbourhood, for iterating the
+    geometry smoothing ``rings`` times per launch; cached on the topology per
+    (core, rings) (the partition only affects speed, never values)."""
+
+    key = (int(core), int(rings))
+    plan = topology.patch_plans.get(key)
+    if plan is not None:
+        return plan
+    if core < 1 or not 1 <= rings <= 64:
+        raise ValueError("core must be positive and rings in [1, 64]")
+    n = topology.n_points
+    off = topology.nbr_off_np.astype(np.int64)
+    nbr = topology.nbr_np.astype(np.int64)
+    order = _morton_order(_host_points(positions, n))
+    count, patch, vert, ring, sizes, vstart = _patch_layout(topology, order, core, rings)
     if sizes.max() >= 65536:
         raise ValueError("a smoothing patch has more than 65535 local vertices")
-    local = np.arange(patch.size) - vstart[patch]
     per_ring = np.zeros((count, rings + 1), np.int64)
     np.add.at(per_ring, (patch, ring), 1)
     ring_end = np.cumsum(per_ring, axis=1)
     # local adjacency of every vertex in rings < rings, in the mesh's CSR order
     deg = np.where(ring < rings, off[vert + 1] - off[vert], 0)
-    lookup_key = patch * n + vert
-    sort = np.argsort(lookup_key, kind="stable")
-    sorted_key = lookup_key[sort]
     src = np.repeat(np.arange(patch.size), deg)
     first = np.repeat(off[vert], deg)
     within = np.arange(src.size) - np.repeat(np.cumsum(deg) - deg, deg)
     nb = nbr[first + within]
-    want = patch[src] * n + nb
-    at = np.searchsorted(sorted_key, want)
-    if at.size and (at.max() >= sorted_key.size or np.any(sorted_key[at] != want)):
-        raise AssertionError("a smoothed vertex has a neighbour outside its patch")
-    ladj = local[sort[at]]
+    ladj = _local_lookup(patch, vert, n, vstart)(patch[src], nb)
     # per patch CSR offsets relative to the patch's first adjacency entry
     edges = np.bincount(patch, weights=deg, minlength=count).astype(np.int64)
     astart = np.zeros(count + 1, np.int64)
@@ -298,12 +325,97 @@ def topology_for(faces: np.ndarray, n_points: int, device: torch.device) -> Topo
         faces_np=f,
         nbr_off_np=np.asarray(nbr_off),
         nbr_np=np.asarray(nbr),
+        colour_order_np=order.astype(np.int64),
+        ell_cols_np=ell_cols,
         ell_width=width,
         ell_cols=dev(ell_cols.reshape(-1)),
         ell_src=dev(ell_src.reshape(-1)),
     )
     _TOPOLOGY_CACHE[cache_key] = topo
     return topo
+
+
+class SorPatchesABI(_lib.ctypes.Structure):
+    """csg_sor_patches of include/csg.h."""
+
+    _fields_ = [("count", _lib.ctypes.c_int), ("rings", _lib.ctypes.c_int),
+                ("max_local", _lib.ctypes.c_int), ("max_rows", _lib.ctypes.c_int)] + [
+        (name, _lib.ctypes.c_void_p) for name in
+        ("vstart", "ncore", "lverts", "estart", "e_row", "e_local", "cstart", "active",
+         "e_lcol")]
+
+
+@dataclass
+class SorPatches:
+    """A partition of a mesh for csg_depth_potential_patched (device arrays)."""
+
+    count: int
+    rings: int
+    max_local: int
+    max_rows: int
+    arrays: dict  # name -> device tensor
+
+    def abi(self) -> SorPatchesABI:
+        a = self.arrays
+        return SorPatchesABI(self.count, self.rings, self.max_local, self.max_rows,
+                             *[_lib.ptr(a[k]) for k in ("vstart", "ncore", "lverts", "estart",
+                                                        "e_row", "e_local", "cstart", "active",
+                                                        "e_lcol")])
+
+
+def sor_patches(topology: Topology, positions, *, core: int) -> SorPatches:
+    """Patches of ``core`` vertices (consecutive in Morton order of ``positions``)
+    with as many rings as the topology has colours, and their SOR row tables
+    (csg_sor_patches); cached on the topology per core size (the partition only
+    affects speed, never values)."""
+
+    key = ("sor", int(core))
+    plan = topology.patch_plans.get(key)
+    if plan is not None:
+        return plan
+    n = topology.n_points
+    colour_off = np.asarray(topology.colour_off, np.int64)
+    ncolours = int(colour_off.size - 1)
+    rings = ncolours
+    order = _morton_order(_host_points(positions, n))
+    count, patch, vert, ring, sizes, vstart = _patch_layout(topology, order, int(core), rings)
+    find = _local_lookup(patch, vert, n, vstart)
+    row_of = np.empty(n, np.int64)
+    row_of[topology.colour_order_np] = np.arange(n)
+    colour_of = np.searchsorted(colour_off, row_of, side="right") - 1
+    local = np.arange(patch.size) - vstart[patch]
+    sel = ring < rings
+    pe, ve, re_, le = patch[sel], vert[sel], ring[sel], local[sel]
+    ce, rowe = colour_of[ve], row_of[ve]
+    srt = np.lexsort((le, re_, ce, pe))
+    pe, ve, re_, le, ce, rowe = pe[srt], ve[srt], re_[srt], le[srt], ce[srt], rowe[srt]
+    rows = np.bincount(pe, minlength=count)
+    estart = np.zeros(count + 1, np.int64)
+    estart[1:] = np.cumsum(rows)
+    per_colour = np.zeros((count, ncolours), np.int64)
+    np.add.at(per_colour, (pe, ce), 1)
+    cstart = np.zeros((count, ncolours + 1), np.int64)
+    cstart[:, 1:] = np.cumsum(per_colour, axis=1)
+    active = np.zeros((count, rings), np.int64)
+    live = re_ <= rings - 1 - ce  # colour t's entries updated in phase t
+    np.add.at(active, (pe[live], ce[live]), 1)
+    width = int(topology.ell_width)
+    lcol = np.stack([find(pe, topology.ell_cols_np[k][rowe]) for k in range(width)], axis=1)
+    if sizes.max() >= 65536:
+        raise ValueError("an SOR patch has more than 65535 local vertices")
+    ncore = np.bincount(patch[ring == 0], minlength=count)
+
+    def dev(a, dtype=torch.int32):
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=topology.nbr.device)
+
+    plan = SorPatches(
+        count=count, rings=rings, max_local=int(sizes.max()), max_rows=int(rows.max()),
+        arrays={"vstart": dev(vstart), "ncore": dev(ncore), "lverts": dev(vert),
+                "estart": dev(estart), "e_row": dev(rowe), "e_local": dev(le),
+                "cstart": dev(cstart.reshape(-1)), "active": dev(active.reshape(-1)),
+                "e_lcol": dev(lcol.astype(np.uint16).view(np.int16), torch.int16)})
+    topology.patch_plans[key] = plan
+    return plan
 
 
 # --------------------------------------------------------------------- file format
@@ -702,8 +814,9 @@ def smooth_geometry_many(stencils, geometries, iterations) -> list[torch.Tensor]
                 n, len(chunk)), dev)
             its = (_lib.ctypes.c_int * len(chunk))(*[int(iterations[i]) for i in chunk])
             if _GEOMETRY_PATCHES is not None and topo.nbr_off_np is not None:
-                pp = smoothing_patches(topo, v[0], core=_GEOMETRY_PATCHES[0],
-                                       rings=_GEOMETRY_PATCHES[1])
+                # the partition follows the stencil's sphere (compact patches)
+                pp = smoothing_patches(topo, stencils[chunk[0]].sphere_points,
+                                       core=_GEOMETRY_PATCHES[0], rings=_GEOMETRY_PATCHES[1])
                 _lib.check(_lib.lib().csg_smooth_geometry_patched(
                     _lib.ptr(v), len(chunk), n, its, pp.count, pp.rings, _lib.ptr(pp.perm),
                     _lib.ptr(pp.vstart),

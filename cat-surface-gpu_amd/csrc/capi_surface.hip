@@ -338,6 +338,7 @@ DepthArgs carve_depth(Carve &c, int batch, int n, int nf, int nnz, int width)
     a.rhs_c = c.take<double>((size_t)batch * n);
     a.diag_c = c.take<double>((size_t)batch * n);
     a.barrier = c.take<unsigned>((size_t)kBarrierSlots * kBarrierSlotWords);
+    a.x_alt = c.take<double>((size_t)batch * n);
     return a;
 }
 
@@ -368,6 +369,32 @@ int csg_depth_potential(const float *pts, int batch, int n, const int *faces, in
                         int max_iter, double omega, double *x, double *relres, void *ws,
                         size_t ws_bytes, void *stream)
 {
+    return csg_depth_potential_patched(pts, batch, n, faces, nf, inc_off, inc, row_off, cols,
+                                       contrib_off, contrib, nnz, colour_off, colour_verts,
+                                       ncolours, ell_cols, ell_src, ell_width, alpha, max_iter,
+                                       omega, nullptr, x, relres, ws, ws_bytes, stream);
+}
+
+int csg_depth_potential_patched(const float *pts, int batch, int n, const int *faces, int nf,
+                                const int *inc_off, const int *inc, const int *row_off,
+                                const int *cols, const int *contrib_off, const int *contrib,
+                                int nnz, const int *colour_off, const int *colour_verts,
+                                int ncolours, const int *ell_cols, const int *ell_src,
+                                int ell_width, double alpha, int max_iter, double omega,
+                                const csg_sor_patches *patches, double *x, double *relres,
+                                void *ws, size_t ws_bytes, void *stream)
+{
+    SorPatches pp{};
+    if (patches) {
+        const csg_sor_patches &q = *patches;
+        if (q.count < 1 || q.rings < 1 || q.max_local < 1 || q.max_rows < 0 || !q.vstart ||
+            !q.ncore || !q.lverts || !q.estart || !q.e_row || !q.e_local || !q.cstart ||
+            !q.active || !q.e_lcol)
+            return fail(kInvalidArgument, "invalid SOR patches");
+        pp = SorPatches{q.count,  q.rings, q.max_local, q.max_rows, q.vstart,
+                        q.ncore,  q.lverts, q.estart,   q.e_row,    q.e_local,
+                        q.cstart, q.active, q.e_lcol};
+    }
     if (n < 1 || nf < 1 || batch < 1 || max_iter < 0 || colour_off == nullptr ||
         ell_width < 1)
         return fail(kInvalidArgument, "invalid depth-potential arguments");
@@ -402,7 +429,7 @@ int csg_depth_potential(const float *pts, int batch, int n, const int *faces, in
     for (int i = 0; i <= ncolours; ++i)
         co.off[i] = colour_off[i];
     CSG_TRY(depth_setup(a, s));
-    CSG_TRY(depth_sor(a, co, max_iter, omega, s));
+    CSG_TRY(depth_sor(a, co, max_iter, omega, patches ? &pp : nullptr, s));
     if (relres)
         CSG_TRY(depth_residual(a, relres, s));
     int timeout = 0;

@@ -38,11 +38,21 @@ struct DepthArgs {
     double *ell_val, *rhs_c, *diag_c;   // [batch][ell_width * n], [batch][n] in colour order
     unsigned *barrier;  // kBarrierSlots x kBarrierSlotWords; the last slot holds the timeout flag
     double *x;          // [batch][n]
+    double *x_alt;      // [batch][n]: the patch-blocked SOR's second buffer
+};
+
+// Patches of the mesh for the LDS-blocked SOR (device arrays, as csg_sor_patches).
+struct SorPatches {
+    int count, rings, max_local, max_rows;
+    const int *vstart, *ncore, *lverts, *estart, *e_row, *e_local, *cstart, *active;
+    const unsigned short *e_lcol;
 };
 
 int depth_setup(const DepthArgs &a, hipStream_t s);
+// patches (may be null): sweep in LDS on those patches when they fit the
+// device (one workgroup per patch and mesh, all resident), else as without.
 int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double omega,
-              hipStream_t s);
+              const SorPatches *patches, hipStream_t s);
 int depth_residual(const DepthArgs &a, double *out, hipStream_t s);
 // Process-wide placement of the SOR sweeps: 2 (default) each mesh's rows on
 // one XCD, x stored through that XCD's L2; 1 the same with x stores written

@@ -524,6 +524,110 @@ k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restr
     }
 }
 
+// The SOR sweeps blocked in time on mesh patches.  A patch lists its core
+// vertices (ring 0) and `rings` rings of their graph neighbourhood, rings =
+// the colour count: sweep colour t (t = 0 .. rings-1) only needs the current
+// values of its rows' columns, so updating the rows of rings <= rings-1-t in
+// phase t leaves the core exact after a whole iteration with no data from
+// outside the patch (the same rows, columns and order as k_sor_xcd_local; the
+// halo rows repeat what their owners compute).  One workgroup per (patch,
+// mesh) on the mesh's XCD (b % 8) stages the patch's x in LDS, runs the
+// iteration's colour phases behind workgroup barriers, writes its core and
+// meets the XCD group's barrier once per iteration.  Iteration s reads
+// x_buf[s & 1] and writes the cores to x_buf[(s + 1) & 1]: a workgroup that is
+// ahead never overwrites values another one is still loading for the same
+// iteration, so the one barrier suffices.  Thread t holds row-table
+// entries t, t + T, .. (kSorPatchRows of them, ELL values in registers); the
+// table is sorted by (colour, ring) so that phase t's rows are a prefix of the
+// colour's entries.
+constexpr int kSorPatchThreads = 1024;
+constexpr int kSorPatchRows = 3;
+
+template <int WMAX>
+__global__ void __launch_bounds__(kSorPatchThreads)
+k_sor_patched(SorPatches pp, int ncolours, const double *__restrict__ ell_val,
+              const double *__restrict__ rhs_c, const double *__restrict__ diag_c, int W,
+              double omega, int n, int max_iter, SorXcdPlan plan, double *x0, double *x1,
+              unsigned *bar, int *timeout)
+{
+    extern __shared__ double g_sor_lds[];
+    int xcd, slot;
+    xcd_claim(bar, kBarrierSlotWords, xcd, slot);
+    const int nblocks = plan.blocks[xcd];
+    if (slot >= nblocks)
+        return;
+    unsigned *cnt = bar + (size_t)(kXcds + xcd) * kBarrierSlotWords;
+    const int p = slot % pp.count;
+    const int b = xcd + kXcds * (slot / pp.count);
+    const int tid = threadIdx.x;
+    const int s0 = pp.vstart[p], L = pp.vstart[p + 1] - s0, core = pp.ncore[p];
+    const int *lv = pp.lverts + s0;
+    const int e0 = pp.estart[p], E = pp.estart[p + 1] - e0;
+    const int *cst = pp.cstart + (size_t)p * (ncolours + 1);
+    const int *act = pp.active + (size_t)p * pp.rings;
+    double *xs = g_sor_lds;
+    double *xbuf[2] = {x0 + (size_t)b * n, x1 + (size_t)b * n};
+    const size_t wn = (size_t)W * n;
+    int ent[kSorPatchRows], lj[kSorPatchRows], lc[kSorPatchRows][WMAX];
+    double val[kSorPatchRows][WMAX], rh[kSorPatchRows], dg[kSorPatchRows];
+#pragma unroll
+    for (int q = 0; q < kSorPatchRows; ++q) {
+        const int e = tid + q * kSorPatchThreads;
+        ent[q] = e < E ? e : -1;
+        lj[q] = 0;
+        rh[q] = 0.0;
+        dg[q] = 1.0;
+#pragma unroll
+        for (int k = 0; k < WMAX; ++k) {
+            lc[q][k] = 0;
+            val[q][k] = 0.0;
+        }
+        if (e < E) {
+            const int r = pp.e_row[e0 + e];
+            lj[q] = pp.e_local[e0 + e];
+            rh[q] = rhs_c[(size_t)b * n + r];
+            dg[q] = diag_c[(size_t)b * n + r];
+#pragma unroll
+            for (int k = 0; k < WMAX; ++k)
+                if (k < W) {
+                    val[q][k] = ell_val[(size_t)b * wn + (size_t)k * n + r];
+                    lc[q][k] = pp.e_lcol[(size_t)(e0 + e) * W + k];
+                }
+        }
+    }
+    const long long total = (long long)max_iter * ncolours;
+    unsigned target = 0;
+    int step = 0;
+    for (long long ph0 = 0; ph0 < total; ph0 += pp.rings, ++step) {
+        const double *src = xbuf[step & 1];
+        double *dst = xbuf[(step + 1) & 1];
+        for (int j = tid; j < L; j += kSorPatchThreads)
+            xs[j] = load_l2(src + lv[j]);
+        __syncthreads();
+        const int m = total - ph0 < pp.rings ? (int)(total - ph0) : pp.rings;
+        for (int t = 0; t < m; ++t) {
+            const int lo = cst[t], hi = lo + act[t];
+#pragma unroll
+            for (int q = 0; q < kSorPatchRows; ++q) {
+                if (ent[q] >= lo && ent[q] < hi) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int k = 0; k < WMAX; ++k)
+                        if (k < W)
+                            s += val[q][k] * xs[lc[q][k]];
+                    xs[lj[q]] = xs[lj[q]] + omega * (rh[q] - s) / dg[q];
+                }
+            }
+            __syncthreads();
+        }
+        for (int j = tid; j < core; j += kSorPatchThreads)
+            dst[lv[j]] = xs[j];
+        target += (unsigned)nblocks;
+        if (!group_barrier(cnt, target, timeout))
+            return;
+    }
+}
+
 // r = rhs - A x; per-block Σ r², Σ rhs².
 __global__ void __launch_bounds__(kBlock)
 k_residual_norms(const int *__restrict__ row_off, const int *__restrict__ cols,
@@ -605,7 +709,7 @@ int set_sor_xcd_local(int mode)
 }
 
 int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double omega,
-              hipStream_t s)
+              const SorPatches *patches, hipStream_t s)
 {
     const int W = a.ell_width;
     const size_t total = (size_t)W * a.n + a.n;
@@ -668,8 +772,43 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     SorColours colours = co;
     // per row and sweep: the row's ELL values and columns (12 B per slot), its
     // x gathers (8 B per slot), rhs, diagonal and the x update (24 B)
+    const double algorithmic = (double)max_iter * batch * n * (20.0 * W + 24.0);
+    if (patches && patches->rings == co.count && co.count <= kSorRegColours &&
+        W <= kSorRegWidth && cus % kXcds == 0 && a.batch >= 1 &&
+        patches->max_rows <= kSorPatchRows * kSorPatchThreads) {
+        // every (patch, mesh) workgroup of an XCD must be resident at once
+        const size_t lds = sizeof(double) * (size_t)patches->max_local;
+        int patch_per_cu = 0;
+        CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &patch_per_cu, k_sor_patched<kSorRegWidth>, kSorPatchThreads, lds));
+        SorXcdPlan pplan{};
+        bool fits = patch_per_cu > 0;
+        int pgrid = 0;
+        for (int xcd = 0; xcd < kXcds && fits; ++xcd) {
+            const int meshes = a.batch > xcd ? (a.batch - xcd + kXcds - 1) / kXcds : 0;
+            pplan.blocks[xcd] = patches->count * meshes;
+            fits = pplan.blocks[xcd] <= (cus / kXcds) * patch_per_cu;
+            pgrid = pplan.blocks[xcd] > pgrid ? pplan.blocks[xcd] : pgrid;
+        }
+        if (fits) {
+            pgrid = (pgrid + kXcdSpare) * kXcds;
+            TimedLaunch ptimer(kTimedSorPatched, s, algorithmic);
+            hipLaunchKernelGGL(k_sor_patched<kSorRegWidth>, dim3(pgrid), dim3(kSorPatchThreads),
+                               lds, s, *patches, co.count, ell_val, rhs_c, diag_c, width, omega,
+                               n, max_iter, pplan, x, a.x_alt, bar, timeout);
+            CSG_LAUNCH_CHECK();
+            ptimer.done();
+            // after an odd number of iterations the result is in the second buffer
+            const long long steps = ((long long)max_iter * co.count + patches->rings - 1) /
+                                    patches->rings;
+            if (steps & 1)
+                CSG_HIP_CHECK(hipMemcpyAsync(x, a.x_alt, sizeof(double) * (size_t)batch * n,
+                                             hipMemcpyDeviceToDevice, s));
+            return kOk;
+        }
+    }
     TimedLaunch timer(local ? kTimedSorXcdLocal : registers ? kTimedSorRegisters : kTimedSor, s,
-                      (double)max_iter * batch * n * (20.0 * W + 24.0));
+                      algorithmic);
     // A regular launch: the grid is capped at what the device holds at once
     // (occupancy x CUs), and every barrier wait is bounded, so blocks that were
     // not resident turn into a timeout error rather than a hang.

@@ -19,6 +19,7 @@ enum TimedKernel : int {
     kTimedSor,             // all SOR sweeps of one depth-potential solve (rows from memory)
     kTimedSorRegisters,    // the same with the rows held in registers
     kTimedSorXcdLocal,     // the same with each mesh's rows on one XCD
+    kTimedSorPatched,      // the SOR iterated in LDS on mesh patches, one exchange per sweep
     kTimedCount
 };
 

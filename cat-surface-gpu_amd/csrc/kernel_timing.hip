@@ -58,7 +58,7 @@ bool capturing(hipStream_t s)
 
 const char *kNames[kTimedCount] = {"k_relax_tile",  "k_fmg_coarse",     "k_restrict_residual",
                                    "k_relax_color", "k_sor_persistent", "k_sor_registers",
-                                   "k_sor_xcd_local"};
+                                   "k_sor_xcd_local", "k_sor_patched"};
 
 }  // namespace
 

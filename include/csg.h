@@ -299,6 +299,37 @@ int csg_depth_potential(const float *pts, int batch, int n, const int *faces, in
                         int max_iter, double omega, double *x, double *relres, void *ws,
                         size_t ws_bytes, void *stream);
 
+/* A partition of the mesh for csg_depth_potential_patched (device arrays; built by
+ * surface_stencil.sor_patches).  Patch p's local vertices are lverts[vstart[p] ..
+ * vstart[p + 1]) (global ids): its ncore[p] core vertices, then the `rings` rings of
+ * their graph neighbourhood (rings = the colour count).  Its row table (entries
+ * estart[p] .. estart[p + 1]) lists the vertices of rings < `rings`, sorted by (colour,
+ * ring): e_row = the vertex's row in solve order, e_local = its local index, e_lcol
+ * [entry][ell_width] = the local index of each ELL column; cstart [p][ncolours + 1] are the
+ * colours' first entries (relative to estart[p]) and active [p][rings] the number of
+ * colour t's entries of rings <= rings - 1 - t.  max_local / max_rows: the largest
+ * patch's vertex and entry counts. */
+typedef struct csg_sor_patches {
+    int count, rings, max_local, max_rows;
+    const int *vstart, *ncore, *lverts, *estart, *e_row, *e_local, *cstart, *active;
+    const unsigned short *e_lcol;
+} csg_sor_patches;
+
+/* csg_depth_potential with the SOR sweeps blocked in time on mesh patches: one workgroup
+ * per (patch, mesh), all of a mesh's on one XCD, iterates the colours of each sweep on the
+ * patch in LDS and exchanges the cores once per sweep (instead of once per colour).
+ * Identical values; falls back to csg_depth_potential's schedule when the patches do not
+ * fit the device (rows per patch, resident workgroups per XCD).  No counterpart in the
+ * reference (a schedule of rotation_feature.py:253-317). */
+int csg_depth_potential_patched(const float *pts, int batch, int n, const int *faces, int nf,
+                                const int *inc_off, const int *inc, const int *row_off,
+                                const int *cols, const int *contrib_off, const int *contrib,
+                                int nnz, const int *colour_off, const int *colour_verts,
+                                int ncolours, const int *ell_cols, const int *ell_src,
+                                int ell_width, double alpha, int max_iter, double omega,
+                                const csg_sor_patches *patches, double *x, double *relres,
+                                void *ws, size_t ws_bytes, void *stream);
+
 /* Sphere points through a flow (out: [n][3] of dtype).  Replaces
  * surface_warp.apply_flow_to_sphere (src/cat_surface_gpu/surface_warp.py:500-584). */
 int csg_apply_flow(const float *unit, int n, const void *flow, int nx, int ny, int dtype,

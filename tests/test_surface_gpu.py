@@ -131,7 +131,8 @@ def test_sor_xcd_local_equals_grid_barrier(level, batch, mode, cuda):
     """Each mesh's sweeps on one XCD give exactly the device-wide-barrier values."""
 
     import cat_surface_gpu as csg
-    from cat_surface_gpu.rotation_feature import depth_potential, set_sor_placement
+    from cat_surface_gpu.rotation_feature import (depth_potential, set_sor_patches,
+                                                  set_sor_placement)
     from cat_surface_gpu.stencil_builder import icosphere
 
     v, f = icosphere(level)
@@ -146,15 +147,79 @@ def test_sor_xcd_local_equals_grid_barrier(level, batch, mode, cuda):
     bumps = 1.0 + 0.05 * rng.standard_normal((batch, n, 1))
     pts = torch.as_tensor(v[None] * 60.0 * bumps, dtype=torch.float32, device=cuda)
     prev = set_sor_placement(mode)
+    prev_patches = set_sor_patches(False)
     try:
         xl, _, rl = depth_potential(st, pts, max_iter=200)
         set_sor_placement(0)
         xg, _, rg = depth_potential(st, pts, max_iter=200)
     finally:
         set_sor_placement(prev)
+        set_sor_patches(prev_patches)
     assert torch.isfinite(xl).all()
     assert torch.equal(xl, xg)
     assert torch.equal(rl, rg)
+
+
+def _timed_launches(name: str) -> int:
+    """Launches of a timed kernel since the last read (reading clears them)."""
+
+    import ctypes
+
+    from cat_surface_gpu import _lib
+
+    lib = _lib.lib()
+    k = 0
+    while True:
+        label = lib.csg_kernel_timing_name(k)
+        if not label:
+            return 0
+        if label.decode() == name:
+            ms, nbytes, launches = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+            _lib.check(lib.csg_kernel_timing_read(k, ctypes.byref(ms), ctypes.byref(nbytes),
+                                                  ctypes.byref(launches)), "timing")
+            return int(launches.value)
+        k += 1
+
+
+@pytest.mark.parametrize("level,batch,iters,patched", [
+    (6, 1, 300, True), (6, 4, 300, True), (6, 8, 300, True), (4, 2, 301, True),
+    (5, 3, 1, True), (5, 9, 300, False)])
+def test_sor_patched_equals_per_colour(level, batch, iters, patched, cuda):
+    """The SOR blocked in time on mesh patches (one exchange per sweep) gives
+    exactly the values of one synchronisation per colour; ``patched`` says
+    whether the patches fit the device (else the call falls back)."""
+
+    import cat_surface_gpu as csg
+    from cat_surface_gpu import _lib
+    from cat_surface_gpu.rotation_feature import depth_potential, set_sor_patches
+    from cat_surface_gpu.stencil_builder import icosphere
+
+    v, f = icosphere(level)
+    n = v.shape[0]
+    idx = np.repeat(np.arange(n, dtype=np.int32)[:, None], 3, axis=1)
+    w = np.tile(np.array([1.0, 0.0, 0.0]), (n, 1))
+    st = csg.SurfaceStencil(
+        sphere_points=v, faces=f.astype(np.int32), surface_indices=idx, surface_weights=w,
+        sheet_indices=np.zeros((1, 1, 3), np.int32), sheet_weights=np.zeros((1, 1, 3)),
+        nx=1, ny=1, source_points=n).to(cuda)
+    rng = np.random.default_rng(level * 10 + batch)
+    bumps = 1.0 + 0.05 * rng.standard_normal((batch, n, 1))
+    pts = torch.as_tensor(v[None] * 60.0 * bumps, dtype=torch.float32, device=cuda)
+    prev = set_sor_patches(False)
+    try:
+        xg, _, rg = depth_potential(st, pts, max_iter=iters)
+        set_sor_patches(True)
+        _lib.lib().csg_kernel_timing(1)
+        _timed_launches("k_sor_patched")  # reading clears the totals
+        xp, _, rp = depth_potential(st, pts, max_iter=iters)
+        used = _timed_launches("k_sor_patched")
+        _lib.lib().csg_kernel_timing(0)
+    finally:
+        set_sor_patches(prev)
+    assert torch.isfinite(xp).all()
+    assert torch.equal(xp, xg)
+    assert torch.equal(rp, rg)
+    assert used == (1 if patched else 0)
 
 
 def test_rotation_index_costs_and_search(golden, cuda):
