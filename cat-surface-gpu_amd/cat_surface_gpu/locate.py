@@ -96,6 +96,22 @@ def build_grid_index_async(points: torch.Tensor, faces: torch.Tensor, grid_size:
     return GridIndex(int(grid_size), int(margin), cell_off, cell_faces)
 
 
+# Closest-triangle queries skip candidates by their faces' bounding spheres
+# (csg_closest_triangles_pruned); False: every candidate is evaluated
+# (csg_closest_triangles).  Identical results.
+_PRUNE = True
+
+
+def set_closest_pruning(enabled: bool) -> bool:
+    """Select bounding-sphere pruning of the closest-triangle queries; returns the
+    previous setting."""
+
+    global _PRUNE
+    previous = _PRUNE
+    _PRUNE = bool(enabled)
+    return previous
+
+
 @dataclass
 class LocateCheck:
     """What an asynchronous closest_triangles still has to confirm: status (device
@@ -146,29 +162,38 @@ def closest_triangles(queries: torch.Tensor, verts: torch.Tensor, faces: torch.T
     face = torch.empty(nq, dtype=torch.int32, device=dev)
     idx = torch.empty((nq, 3), dtype=torch.int32, device=dev)
     w = torch.empty((nq, 3), dtype=torch.float64, device=dev)
-    ws = _lib.workspace(256, dev)
-    key = (int(f.shape[0]), int(grid_size), int(margin))
+    nf = int(f.shape[0])
+    ws = _lib.workspace(lib.csg_closest_triangles_workspace_bytes(nf), dev)
+    stream = _lib.stream_handle(dev)
+
+    def locate(index, reach, far_dev, far_host):
+        if _PRUNE:
+            _lib.check(lib.csg_closest_triangles_pruned(
+                _lib.ptr(q), nq, _lib.ptr(v), _lib.ptr(f), nf, _lib.ptr(index.cell_off),
+                _lib.ptr(index.cell_faces), grid_size, reach, _lib.ptr(face), _lib.ptr(idx),
+                _lib.ptr(w), _lib.ptr(far_dev), far_host, _lib.ptr(ws), ws.numel(), stream),
+                "csg_closest_triangles_pruned")
+        else:
+            if far_host is None:
+                far_dev.zero_()
+            _lib.check(lib.csg_closest_triangles(
+                _lib.ptr(q), nq, _lib.ptr(v), _lib.ptr(f), _lib.ptr(index.cell_off),
+                _lib.ptr(index.cell_faces), grid_size, reach, _lib.ptr(face), _lib.ptr(idx),
+                _lib.ptr(w), far_host, _lib.ptr(far_dev), 4, stream), "csg_closest_triangles")
+
+    key = (nf, int(grid_size), int(margin))
     if deferred and key in _CAPACITY:
         capacity = _CAPACITY[key]
         status = torch.zeros(2, dtype=torch.int32, device=dev)
         index = build_grid_index_async(v, f, grid_size, margin, capacity, status[0:1])
-        reach = margin * 2.0 / grid_size
-        _lib.check(lib.csg_closest_triangles(_lib.ptr(q), nq, _lib.ptr(v), _lib.ptr(f),
-                                             _lib.ptr(index.cell_off), _lib.ptr(index.cell_faces),
-                                             grid_size, reach, _lib.ptr(face), _lib.ptr(idx),
-                                             _lib.ptr(w), None, _lib.ptr(status[1:]), 4,
-                                             _lib.stream_handle(dev)), "csg_closest_triangles")
+        locate(index, margin * 2.0 / grid_size, status[1:], None)
         return face, idx, w, LocateCheck(status, capacity, key)
+    far_dev = torch.zeros(1, dtype=torch.int32, device=dev)
     while True:
         index = build_grid_index(v, f, grid_size, margin)
         far = ctypes.c_int()
         reach = margin * 2.0 / grid_size
-        _lib.check(lib.csg_closest_triangles(_lib.ptr(q), nq, _lib.ptr(v), _lib.ptr(f),
-                                             _lib.ptr(index.cell_off), _lib.ptr(index.cell_faces),
-                                             grid_size, reach, _lib.ptr(face), _lib.ptr(idx),
-                                             _lib.ptr(w), ctypes.byref(far), _lib.ptr(ws),
-                                             ws.numel(), _lib.stream_handle(dev)),
-                   "csg_closest_triangles")
+        locate(index, reach, far_dev, ctypes.byref(far))
         if far.value == 0:
             return (face, idx, w, None) if deferred else (face, idx, w)
         if margin >= max_margin:

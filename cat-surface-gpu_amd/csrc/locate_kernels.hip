@@ -754,18 +754,49 @@ __device__ double closest_on_triangle(const double *p, const double *a, const do
     return e;
 }
 
+// Bounding sphere of each face: centre (vertex mean) and a radius enlarged by
+// a relative 1e-12 and an absolute 1e-14 (coordinates are within [-1, 1]), so
+// that |p - c| - r is a lower bound of p's distance to the face despite
+// rounding.
+__global__ void __launch_bounds__(kBlock)
+k_face_spheres(const double *__restrict__ verts, const int *__restrict__ faces, int nf,
+               double4 *__restrict__ sph)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf)
+        return;
+    const double *a = verts + 3 * faces[3 * f], *b = verts + 3 * faces[3 * f + 1],
+                 *c = verts + 3 * faces[3 * f + 2];
+    double m[3], r2 = 0.0;
+    for (int d = 0; d < 3; ++d)
+        m[d] = (a[d] + b[d] + c[d]) / 3.0;
+    for (const double *v : {a, b, c}) {
+        double e = 0.0;
+        for (int d = 0; d < 3; ++d)
+            e += (v[d] - m[d]) * (v[d] - m[d]);
+        r2 = e > r2 ? e : r2;
+    }
+    sph[f] = make_double4(m[0], m[1], m[2], sqrt(r2) * (1.0 + 1e-12) + 1e-14);
+}
+
 // For each query: closest triangle among the candidates of its cell (first
 // minimum); `far` counts queries whose best squared distance exceeds
 // reach2, i.e. whose true closest triangle may lie outside the searched cells.
 // kLocateLanes consecutive lanes share one query, each scanning every
 // kLocateLanes-th candidate; the lanes' bests are combined as in the
-// sequential scan (smaller distance, then earlier candidate).
+// sequential scan (smaller distance, then earlier candidate).  With face
+// spheres (sph), a lane skips a candidate whose sphere lies farther than
+// sqrt(best)·(1 + 1e-9) + r: its distance then exceeds the lane's best, which
+// is at least the final best, so it could neither win nor tie (the skip
+// changes no result).
+template <bool PRUNE>
 __global__ void __launch_bounds__(kBlock)
 k_closest_triangles(const double *__restrict__ q, int nq, const double *__restrict__ verts,
                     const int *__restrict__ faces, const int *__restrict__ cell_off,
                     const int *__restrict__ cell_faces, int g, double reach2,
-                    int *__restrict__ face_out, int *__restrict__ idx_out,
-                    double *__restrict__ w_out, int *__restrict__ far_count)
+                    const double4 *__restrict__ sph, int *__restrict__ face_out,
+                    int *__restrict__ idx_out, double *__restrict__ w_out,
+                    int *__restrict__ far_count)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = t / kLocateLanes, lane = t % kLocateLanes;
@@ -785,16 +816,25 @@ k_closest_triangles(const double *__restrict__ q, int nq, const double *__restri
         begin = cell_off[id];
         end = cell_off[id + 1];
     }
-    double best = INFINITY, bw[3] = {0, 0, 0};
+    double best = INFINITY, bw[3] = {0, 0, 0}, reach = INFINITY;
     int bf = -1, bpos = 0x7fffffff;
     for (int k = begin + lane; k < end; k += kLocateLanes) {
         int f = cell_faces[k];
+        if (PRUNE) {
+            const double4 s = sph[f];
+            const double dx = p[0] - s.x, dy = p[1] - s.y, dz = p[2] - s.z;
+            const double lim = s.w + reach;
+            if (dx * dx + dy * dy + dz * dz > lim * lim)
+                continue;
+        }
         const int *fv = faces + 3 * f;
         double w[3];
         double dist = closest_on_triangle(p, verts + 3 * fv[0], verts + 3 * fv[1],
                                           verts + 3 * fv[2], w);
         if (dist < best) {
             best = dist;
+            if (PRUNE)
+                reach = sqrt(best) * (1.0 + 1e-9);
             bf = f;
             bpos = k;
             bw[0] = w[0];
@@ -834,10 +874,27 @@ int launch_closest_triangles(const double *q, int nq, const double *verts, const
 {
     if (nq <= 0)
         return kOk;
-    hipLaunchKernelGGL(k_closest_triangles, dim3(grid_for((size_t)nq * kLocateLanes)),
-                       dim3(kBlock), 0, s, q, nq, verts,
-                       faces, cell_off, cell_faces, g, reach2, face_out, idx_out, w_out,
-                       far_count);
+    hipLaunchKernelGGL(k_closest_triangles<false>, dim3(grid_for((size_t)nq * kLocateLanes)),
+                       dim3(kBlock), 0, s, q, nq, verts, faces, cell_off, cell_faces, g, reach2,
+                       nullptr, face_out, idx_out, w_out, far_count);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+int launch_closest_triangles_pruned(const double *q, int nq, const double *verts,
+                                    const int *faces, int nf, const int *cell_off,
+                                    const int *cell_faces, int g, double reach2, double4 *sph,
+                                    int *face_out, int *idx_out, double *w_out, int *far_count,
+                                    hipStream_t s)
+{
+    if (nq <= 0)
+        return kOk;
+    hipLaunchKernelGGL(k_face_spheres, dim3(grid_for(nf)), dim3(kBlock), 0, s, verts, faces, nf,
+                       sph);
+    CSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_closest_triangles<true>, dim3(grid_for((size_t)nq * kLocateLanes)),
+                       dim3(kBlock), 0, s, q, nq, verts, faces, cell_off, cell_faces, g, reach2,
+                       sph, face_out, idx_out, w_out, far_count);
     CSG_LAUNCH_CHECK();
     return kOk;
 }

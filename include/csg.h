@@ -454,6 +454,17 @@ int csg_closest_triangles(const double *q, int nq, const double *verts, const in
                           int *face_out, int *idx_out, double *w_out, int *far_host,
                           void *ws, size_t ws_bytes, void *stream);
 
+/* csg_closest_triangles with each candidate first tested against its face's bounding
+ * sphere (computed into ws, csg_closest_triangles_workspace_bytes(nf) bytes): a candidate
+ * farther than the scanning lane's best is skipped, which changes no result.  The far
+ * count goes to far_dev (device int), and to far_host after a synchronisation if given. */
+size_t csg_closest_triangles_workspace_bytes(int nf);
+int csg_closest_triangles_pruned(const double *q, int nq, const double *verts, const int *faces,
+                                 int nf, const int *cell_off, const int *cell_faces, int grid,
+                                 double reach, int *face_out, int *idx_out, double *w_out,
+                                 int *far_dev, int *far_host, void *ws, size_t ws_bytes,
+                                 void *stream);
+
 #ifdef __cplusplus
 }
 #endif

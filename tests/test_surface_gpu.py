@@ -628,3 +628,31 @@ def test_geometry_smoothing_patched_equals_per_iteration(level, iterations, patc
         _lib.stream_handle(cuda)), "patched")
     torch.cuda.synchronize()
     assert torch.equal(out, ref)
+
+
+@pytest.mark.parametrize("level,grid", [(5, 32), (6, 32), (4, 64)])
+def test_closest_triangles_pruned_equals_full_scan(level, grid, cuda):
+    """Bounding-sphere pruning of the closest-triangle queries returns the faces,
+    vertex indices and weights of the full candidate scan, including queries off
+    the surface and exactly at vertices (ties)."""
+
+    from cat_surface_gpu.locate import closest_triangles, set_closest_pruning
+    from cat_surface_gpu.stencil_builder import icosphere, uv_points
+
+    sv, sf = icosphere(level)
+    rng = np.random.default_rng(level + grid)
+    verts = torch.as_tensor(sv * (1.0 + 0.01 * rng.standard_normal((sv.shape[0], 1))),
+                            device=cuda)
+    faces = torch.as_tensor(sf, device=cuda)
+    q = np.concatenate([uv_points(64, 32) * rng.uniform(0.97, 1.03, (2048, 1)),
+                        sv[:500] * 0.99, sv[500:600]])
+    q = torch.as_tensor(q, device=cuda)
+    prev = set_closest_pruning(False)
+    try:
+        ref = closest_triangles(q, verts, faces, grid_size=grid, margin=1)
+        set_closest_pruning(True)
+        got = closest_triangles(q, verts, faces, grid_size=grid, margin=1)
+    finally:
+        set_closest_pruning(prev)
+    for a, b in zip(got, ref):
+        assert torch.equal(a, b)
