@@ -322,8 +322,12 @@ int fmg_run(const T *hes, const T *rhs, const double *params, int cycles, int ni
     plan.A[0] = const_cast<T *>(hes);
     plan.B0[0] = plan.B[0] = const_cast<T *>(rhs);
     plan.U[0] = out;
+    const int c0 = fused_first_coarse_level<T>(plan, nit);
     // level pyramids of A and the original right-hand side.  dartel_grid.py:1247-1272
-    for (int l = 1; l < plan.L; ++l) {
+    // (the fused schedule's coarse kernel builds the levels below c0 itself from
+    // level c0, so only the fine levels and c0 are made here)
+    const int top = c0 >= 0 ? c0 : plan.L - 1;
+    for (int l = 1; l <= top; ++l) {
         size_t P = plan.P(l - 1), Pc = plan.P(l);
         CSG_TRY(launch_resize<T>(plan.A[l - 1], 3, plan.nx[l - 1], plan.ny[l - 1], plan.nx[l],
                                  plan.ny[l], batch, 3 * P, 3 * Pc, plan.A[l], 0, s));
@@ -336,7 +340,6 @@ int fmg_run(const T *hes, const T *rhs, const double *params, int cycles, int ni
     } else {
         CSG_TRY(fill_async<T>(out, (T)0, 2 * P0 * batch, s));
     }
-    const int c0 = fused_first_coarse_level<T>(plan, nit);
     if (c0 >= 0) {
         FusedFmg<T> run{plan, params, nit, batch, c0, s};
         run.setup();
