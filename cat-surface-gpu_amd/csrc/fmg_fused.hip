@@ -63,8 +63,10 @@ k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restri
     const int npx = ex * (yhi - ylo);
     T *u0 = lds, *u1 = lds + npx, *a0 = lds + 2 * npx, *a1 = lds + 3 * npx, *a2 = lds + 4 * npx,
       *b0 = lds + 5 * npx, *b1 = lds + 6 * npx;
+    const float inv_ex = 1.0f / (float)ex;
     for (int q = threadIdx.x; q < npx; q += blockDim.x) {
-        const int r = q / ex, i = q - r * ex;
+        const DivMod ri = divmod_small(q, ex, inv_ex);
+        const int r = ri.q, i = ri.r;
         const int gx = wrap_x(x0 + i, nx), gy = ylo + r, p = gy * nx + gx;
         T ux = Uin[p], uy = Uin[P + p];
         if (Uc) {
@@ -84,14 +86,22 @@ k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restri
         b1[q] = B[P + p];
     }
     __syncthreads();
-    const int hex = ex / 2;  // pixels of one colour per tile row (ex is even)
+    // Half-sweep s only updates the pixels at distance >= s + 1 from the tile's
+    // cut edges (a row range that reaches the grid's y boundary is not cut):
+    // the others cannot influence the interior any more, so the interior's
+    // values are those of updating every pixel whose neighbours are in the tile.
     for (int s = 0; s < H; ++s) {
         const int colour = (s & 1) ? 0 : 1;
-        for (int j = threadIdx.x; j < hex * (yhi - ylo); j += blockDim.x) {
-            const int r = j / hex, k = j - r * hex;
+        const int ilo = s + 1, hw = (ex - 2 * ilo) / 2;  // ex is even
+        const int rlo = ylo == 0 ? 0 : ilo;
+        const int rhi = (yhi - ylo) - (yhi == ny ? 0 : ilo);
+        const float inv_hw = 1.0f / (float)hw;
+        for (int j = threadIdx.x; j < hw * (rhi - rlo); j += blockDim.x) {
+            const DivMod rk = divmod_small(j, hw, inv_hw);
+            const int r = rlo + rk.q;
             const int gy = ylo + r;
             // the column of this row with (x0 + i + gy) & 1 == colour
-            const int i = 2 * k + ((colour - x0 - gy) & 1);
+            const int i = ilo + 2 * rk.r + ((colour - x0 - gy - ilo) & 1);
             if (i == 0 || i == ex - 1)
                 continue;
             const int ym = reflect_y(gy - 1, ny), yp = reflect_y(gy + 1, ny);
