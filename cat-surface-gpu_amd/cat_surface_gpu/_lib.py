@@ -111,15 +111,16 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_grid_index_fill": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _sz, _vp]),
     "csg_rotation_costs_workspace_bytes": (_sz, [_i, _i]),
     "csg_rotation_cost_matrices_workspace_bytes": (_sz, [_i, _i, _i]),
-    "csg_rotation_cost_matrices": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _i,
-                                        _vp, _vp, _sz, _vp, _vp]),
+    "csg_face_spheres": (_i, [_vp, _i, _vp, _vp]),
+    "csg_rotation_cost_matrices": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i,
+                                        _i, _vp, _vp, _sz, _vp, _vp]),
     "csg_rotation_map_costs_workspace_bytes": (_sz, [_i, _i, _i, _i]),
     "csg_rotation_map_costs": (_i, [_vp, _vp, _i, _i, _i, _vp, _i, _vp, _vp, _sz, _vp]),
     "csg_rotation_refine_workspace_bytes": (_sz, [_i, _i]),
-    "csg_rotation_refine": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _i, _d,
-                                 _d, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
-    "csg_rotation_costs": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp,
-                                _vp, _vp, _sz, ctypes.POINTER(_i), _vp, _vp]),
+    "csg_rotation_refine": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _i,
+                                 _d, _d, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
+    "csg_rotation_costs": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp,
+                                _vp, _vp, _vp, _sz, ctypes.POINTER(_i), _vp, _vp]),
     "csg_closest_triangles": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i, _d, _vp, _vp, _vp,
                                    ctypes.POINTER(_i), _vp, _sz, _vp]),
     "csg_closest_triangles_workspace_bytes": (_sz, [_i]),

@@ -213,6 +213,21 @@ class RotationSearchResult:
     iterations: int
 
 
+# The rotation locator skips candidates by their faces' bounding spheres
+# (identical results); False scans every candidate of a cell.
+_LOCATOR_SKIP = True
+
+
+def set_locator_skip(enabled: bool) -> bool:
+    """Select the bounding-sphere candidate skip of the rotation locator; returns
+    the previous setting."""
+
+    global _LOCATOR_SKIP
+    previous = _LOCATOR_SKIP
+    _LOCATOR_SKIP = bool(enabled)
+    return previous
+
+
 @dataclass
 class RotationGridIndexDevice:
     """Evaluate CAT rotation candidates against a device-resident index."""
@@ -240,6 +255,23 @@ class RotationGridIndexDevice:
     @property
     def device(self) -> torch.device:
         return self.target_points.device
+
+    def face_spheres(self) -> torch.Tensor | None:
+        """The target faces' bounding spheres ([F, 4] float32, built once) that let
+        the locator skip candidates (csg_face_spheres); None when skipping is off."""
+
+        if not _LOCATOR_SKIP:
+            return None
+        sph = self.__dict__.get("_spheres")
+        if sph is None:
+            tri = self.target_triangles.contiguous()
+            nf = int(tri.shape[0])
+            sph = torch.empty((nf, 4), dtype=torch.float32, device=self.device)
+            _lib.check(_lib.lib().csg_face_spheres(_lib.ptr(tri), nf, _lib.ptr(sph),
+                                                   _lib.stream_handle(self.device)),
+                       "csg_face_spheres")
+            self.__dict__["_spheres"] = sph
+        return sph
 
     def rotation_matrix(self, angles) -> torch.Tensor:
         a = torch.as_tensor(angles, dtype=torch.float64).reshape(-1)
@@ -285,8 +317,9 @@ class RotationGridIndexDevice:
         _lib.check(lib.csg_rotation_costs(
             _lib.ptr(inputs.source_points), _lib.ptr(inputs.source_values), ns,
             _lib.ptr(inputs.target_values), _lib.ptr(self.target_faces),
-            _lib.ptr(self.target_triangles), _lib.ptr(self.candidate_offsets),
-            _lib.ptr(self.candidate_faces), int(self.grid_size), _lib.ptr(ang), na,
+            _lib.ptr(self.target_triangles), _lib.ptr(self.face_spheres()),
+            _lib.ptr(self.candidate_offsets), _lib.ptr(self.candidate_faces),
+            int(self.grid_size), _lib.ptr(ang), na,
             _lib.ptr(costs), _lib.ptr(loc_f), _lib.ptr(loc_b), _lib.ptr(ws), ws.numel(),
             None if uncovered_dev is not None else ctypes.byref(uncovered),
             _lib.ptr(uncovered_dev), _lib.stream_handle(self.device)), "csg_rotation_costs")
@@ -313,8 +346,9 @@ class RotationGridIndexDevice:
         _lib.check(lib.csg_rotation_cost_matrices(
             _lib.ptr(inputs.source_points), _lib.ptr(inputs.source_values), ns,
             _lib.ptr(inputs.target_values), _lib.ptr(self.target_faces),
-            _lib.ptr(self.target_triangles), _lib.ptr(self.candidate_offsets),
-            _lib.ptr(self.candidate_faces), int(self.grid_size), _lib.ptr(mats), na,
+            _lib.ptr(self.target_triangles), _lib.ptr(self.face_spheres()),
+            _lib.ptr(self.candidate_offsets), _lib.ptr(self.candidate_faces),
+            int(self.grid_size), _lib.ptr(mats), na,
             int(gradient), _lib.ptr(out), _lib.ptr(ws), ws.numel(), _lib.ptr(uncovered_dev),
             _lib.stream_handle(self.device)), "csg_rotation_cost_matrices")
         return out
@@ -556,8 +590,9 @@ def refine_adjoint_launch(index, inputs, start: torch.Tensor, *, start_is_angles
     _lib.check(lib.csg_rotation_refine(
         _lib.ptr(inputs.source_points), _lib.ptr(inputs.source_values), ns,
         _lib.ptr(inputs.target_values), _lib.ptr(index.target_faces),
-        _lib.ptr(index.target_triangles), _lib.ptr(index.candidate_offsets),
-        _lib.ptr(index.candidate_faces), int(index.grid_size), _lib.ptr(st),
+        _lib.ptr(index.target_triangles), _lib.ptr(index.face_spheres()),
+        _lib.ptr(index.candidate_offsets), _lib.ptr(index.candidate_faces),
+        int(index.grid_size), _lib.ptr(st),
         int(start_is_angles), m, int(steps), float(max_step), float(damping), _lib.ptr(mats),
         _lib.ptr(angles), _lib.ptr(costs), _lib.ptr(ws), ws.numel(), _lib.ptr(unc),
         _lib.stream_handle(index.device)), "csg_rotation_refine")

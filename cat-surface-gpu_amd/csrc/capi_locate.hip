@@ -119,10 +119,17 @@ size_t csg_rotation_costs_workspace_bytes(int ns, int na)
     return align256(sizeof(double) * (size_t)rotation_cost_blocks(ns) * na) + 256;
 }
 
+int csg_face_spheres(const float *tri, int nf, float *sph, void *stream)
+{
+    if (nf < 1 || !tri || !sph)
+        return fail(kInvalidArgument, "invalid face sphere arguments");
+    return launch_face_spheres_f32(tri, nf, reinterpret_cast<float4 *>(sph), S(stream));
+}
+
 int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
                        const double *tgt_vals, const int *faces, const float *tri,
-                       const int *cell_off, const int *cell_faces, int grid,
-                       const double *angles, int na, double *costs, int *loc_face,
+                       const float *sph, const int *cell_off, const int *cell_faces,
+                       int grid, const double *angles, int na, double *costs, int *loc_face,
                        float *loc_bary, void *ws, size_t ws_bytes, int *uncovered_host,
                        int *uncovered_dev, void *stream)
 {
@@ -139,6 +146,7 @@ int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
     a.tgt_vals = tgt_vals;
     a.faces = faces;
     a.tri = tri;
+    a.sph = reinterpret_cast<const float4 *>(sph);
     a.cell_off = cell_off;
     a.cell_faces = cell_faces;
     a.grid = grid;
@@ -171,8 +179,8 @@ size_t csg_rotation_cost_matrices_workspace_bytes(int ns, int na, int with_gradi
 
 int csg_rotation_cost_matrices(const float *src_pts, const double *src_vals, int ns,
                                const double *tgt_vals, const int *faces, const float *tri,
-                               const int *cell_off, const int *cell_faces, int grid,
-                               const float *mats, int na, int with_gradient, double *out,
+                               const float *sph, const int *cell_off, const int *cell_faces,
+                               int grid, const float *mats, int na, int with_gradient, double *out,
                                void *ws, size_t ws_bytes, int *uncovered_dev, void *stream)
 {
     if (ns < 1 || na < 1 || grid < 1 || mats == nullptr || out == nullptr ||
@@ -190,6 +198,7 @@ int csg_rotation_cost_matrices(const float *src_pts, const double *src_vals, int
     a.tgt_vals = tgt_vals;
     a.faces = faces;
     a.tri = tri;
+    a.sph = reinterpret_cast<const float4 *>(sph);
     a.cell_off = cell_off;
     a.cell_faces = cell_faces;
     a.grid = grid;

@@ -36,6 +36,7 @@ struct RotationCostArgs {
     const double *tgt_vals;
     const int *faces;
     const float *tri;
+    const float4 *sph;  // face bounding spheres (may be null: every candidate is scanned)
     const int *cell_off, *cell_faces;
     int grid;
     const double *angles;
@@ -48,6 +49,8 @@ struct RotationCostArgs {
 };
 
 int launch_rotation_costs(const RotationCostArgs &a, hipStream_t s);
+// Bounding spheres of the faces of tri for the locator's candidate skip.
+int launch_face_spheres_f32(const float *tri, int nf, float4 *sph, hipStream_t s);
 
 // Float64 sums per candidate of the matrix-input cost with gradient: cost,
 // gradient (3) and Gauss-Newton matrix (xx, yy, zz, xy, xz, yz).

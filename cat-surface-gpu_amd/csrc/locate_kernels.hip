@@ -281,13 +281,30 @@ struct LocateBest {
     float bp[3];
 };
 
+//
+// With face spheres (sph: centre, radius enlarged by 2 % and 1e-6, see
+// k_face_spheres_f32), a candidate whose sphere lies farther than
+// sqrt(best)·(1 + 1e-4) + 1e-6 from q is skipped.  Its distance would exceed
+// the lane's best: outside the triangle the distance is to a point of the
+// triangle (>= |q - c| - r); inside, the in-plane point lies within 1.004 r
+// of c (the 1e-3 barycentric slack), so the plane distance is >= |q - c| -
+// 1.004 r.  Skipping it changes no result.
 __device__ __forceinline__ LocateBest scan_candidates(const float *q, const float *__restrict__ tri,
                                                      const int *__restrict__ cand, int ncand,
-                                                     int lane, int lanes)
+                                                     int lane, int lanes,
+                                                     const float4 *__restrict__ sph)
 {
     LocateBest r{INFINITY, 0x7fffffff, -1, {0.0f, 0.0f, 0.0f}};
+    float reach = INFINITY;
     for (int c = lane; c < ncand; c += lanes) {
         int face = cand[c];
+        if (sph) {
+            const float4 b = sph[face];
+            const float dx = q[0] - b.x, dy = q[1] - b.y, dz = q[2] - b.z;
+            const float lim = b.w + reach;
+            if (dx * dx + dy * dy + dz * dz > lim * lim)
+                continue;
+        }
         const float *t = tri + 9 * (size_t)face;
         float p0[3] = {t[0], t[1], t[2]}, p1[3] = {t[3], t[4], t[5]}, p2[3] = {t[6], t[7], t[8]};
         float ea[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
@@ -353,6 +370,7 @@ __device__ __forceinline__ LocateBest scan_candidates(const float *q, const floa
         }
         if (dist < r.dist) {
             r.dist = dist;
+            reach = sqrtf(dist) * 1.0001f + 1e-6f;
             r.pos = c;
             r.face = face;
             r.bp[0] = cpt[0];
@@ -419,8 +437,9 @@ __device__ __forceinline__ int cell_of(const float *p, int g)
 __global__ void __launch_bounds__(kBlock)
 k_rotation_cost(const float *__restrict__ src_pts, const double *__restrict__ src_vals, int ns,
                 const double *__restrict__ tgt_vals, const int *__restrict__ faces,
-                const float *__restrict__ tri, const int *__restrict__ cell_off,
-                const int *__restrict__ cell_faces, int g, const double *__restrict__ angles,
+                const float *__restrict__ tri, const float4 *__restrict__ sph,
+                const int *__restrict__ cell_off, const int *__restrict__ cell_faces, int g,
+                const double *__restrict__ angles,
                 double *__restrict__ partial, int nblocks, int *__restrict__ uncovered,
                 int *__restrict__ loc_face, float *__restrict__ loc_bary)
 {
@@ -444,7 +463,7 @@ k_rotation_cost(const float *__restrict__ src_pts, const double *__restrict__ sr
         b = cell_off[cell];
         nc = cell_off[cell + 1] - b;
     }
-    LocateBest best = scan_candidates(q, tri, cell_faces + b, nc, lane, kLocateLanes);
+    LocateBest best = scan_candidates(q, tri, cell_faces + b, nc, lane, kLocateLanes, sph);
     best = reduce_lanes(best, kLocateLanes);
     if (i < ns && lane == 0) {
         if (best.face < 0) {
@@ -482,8 +501,9 @@ template <bool GRAD>
 __global__ void __launch_bounds__(kBlock)
 k_rotation_cost_mats(const float *__restrict__ src_pts, const double *__restrict__ src_vals,
                      int ns, const double *__restrict__ tgt_vals, const int *__restrict__ faces,
-                     const float *__restrict__ tri, const int *__restrict__ cell_off,
-                     const int *__restrict__ cell_faces, int g, const float *__restrict__ mats,
+                     const float *__restrict__ tri, const float4 *__restrict__ sph,
+                     const int *__restrict__ cell_off, const int *__restrict__ cell_faces, int g,
+                     const float *__restrict__ mats,
                      double *__restrict__ partial, int nblocks, int *__restrict__ uncovered)
 {
     constexpr int kTerms = GRAD ? kRotationGradTerms : 1;
@@ -508,7 +528,7 @@ k_rotation_cost_mats(const float *__restrict__ src_pts, const double *__restrict
         b = cell_off[cell];
         nc = cell_off[cell + 1] - b;
     }
-    LocateBest best = scan_candidates(q, tri, cell_faces + b, nc, lane, kLocateLanes);
+    LocateBest best = scan_candidates(q, tri, cell_faces + b, nc, lane, kLocateLanes, sph);
     best = reduce_lanes(best, kLocateLanes);
     if (i < ns && lane == 0) {
         if (best.face < 0) {
@@ -641,12 +661,43 @@ k_sum_rows(const double *__restrict__ partial, int nblocks, int rows, double *__
         out[r] = s;
 }
 
+// Bounding sphere of each face of tri ([nf][3][3] float32): the vertex mean
+// and the largest vertex distance enlarged by 2 % and 1e-6 (covering the
+// locator's barycentric slack and float32 rounding; unit-sphere scale).
+__global__ void __launch_bounds__(kBlock)
+k_face_spheres_f32(const float *__restrict__ tri, int nf, float4 *__restrict__ sph)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf)
+        return;
+    const float *t = tri + 9 * (size_t)f;
+    float c[3], r2 = 0.0f;
+    for (int d = 0; d < 3; ++d)
+        c[d] = (t[d] + t[3 + d] + t[6 + d]) / 3.0f;
+    for (int v = 0; v < 3; ++v) {
+        const float e0 = t[3 * v] - c[0], e1 = t[3 * v + 1] - c[1], e2 = t[3 * v + 2] - c[2];
+        const float e = e0 * e0 + e1 * e1 + e2 * e2;
+        r2 = e > r2 ? e : r2;
+    }
+    sph[f] = make_float4(c[0], c[1], c[2], sqrtf(r2) * 1.02f + 1e-6f);
+}
+
+int launch_face_spheres_f32(const float *tri, int nf, float4 *sph, hipStream_t s)
+{
+    if (nf <= 0)
+        return kOk;
+    hipLaunchKernelGGL(k_face_spheres_f32, dim3(grid_for(nf)), dim3(kBlock), 0, s, tri, nf, sph);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
 int launch_rotation_costs(const RotationCostArgs &a, hipStream_t s)
 {
     int nb = rotation_cost_blocks(a.ns);
     dim3 grid(nb, a.na);
     hipLaunchKernelGGL(k_rotation_cost, grid, dim3(kBlock), 0, s, a.src_pts, a.src_vals, a.ns,
-                       a.tgt_vals, a.faces, a.tri, a.cell_off, a.cell_faces, a.grid, a.angles,
+                       a.tgt_vals, a.faces, a.tri, a.sph, a.cell_off, a.cell_faces, a.grid,
+                       a.angles,
                        a.partial, nb, a.uncovered, a.loc_face, a.loc_bary);
     CSG_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sum_rows, dim3(a.na), dim3(kBlock), 0, s, a.partial, nb, a.na, a.costs);
@@ -661,11 +712,11 @@ int launch_rotation_cost_mats(const RotationCostArgs &a, const float *mats, bool
     dim3 grid(nb, a.na);
     if (gradient)
         hipLaunchKernelGGL(k_rotation_cost_mats<true>, grid, dim3(kBlock), 0, s, a.src_pts,
-                           a.src_vals, a.ns, a.tgt_vals, a.faces, a.tri, a.cell_off,
+                           a.src_vals, a.ns, a.tgt_vals, a.faces, a.tri, a.sph, a.cell_off,
                            a.cell_faces, a.grid, mats, a.partial, nb, a.uncovered);
     else
         hipLaunchKernelGGL(k_rotation_cost_mats<false>, grid, dim3(kBlock), 0, s, a.src_pts,
-                           a.src_vals, a.ns, a.tgt_vals, a.faces, a.tri, a.cell_off,
+                           a.src_vals, a.ns, a.tgt_vals, a.faces, a.tri, a.sph, a.cell_off,
                            a.cell_faces, a.grid, mats, a.partial, nb, a.uncovered);
     CSG_LAUNCH_CHECK();
     const int rows = a.na * (gradient ? kRotationGradTerms : 1);

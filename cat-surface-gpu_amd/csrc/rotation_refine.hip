@@ -229,9 +229,9 @@ size_t csg_rotation_refine_workspace_bytes(int ns, int m)
 
 int csg_rotation_refine(const float *src_pts, const double *src_vals, int ns,
                         const double *tgt_vals, const int *faces, const float *tri,
-                        const int *cell_off, const int *cell_faces, int grid, const double *start,
-                        int start_is_angles, int m, int steps, double max_step, double damping,
-                        double *mats_out, double *angles_out, double *costs_out, void *ws,
+                        const float *sph, const int *cell_off, const int *cell_faces, int grid,
+                        const double *start, int start_is_angles, int m, int steps,
+                        double max_step, double damping, double *mats_out, double *angles_out, double *costs_out, void *ws,
                         size_t ws_bytes, int *uncovered_dev, void *stream)
 {
     if (ns < 1 || grid < 1 || m < 1 || m > kRefineMax || steps < 0 || !(max_step > 0.0) ||
@@ -262,6 +262,7 @@ int csg_rotation_refine(const float *src_pts, const double *src_vals, int ns,
     a.tgt_vals = tgt_vals;
     a.faces = faces;
     a.tri = tri;
+    a.sph = reinterpret_cast<const float4 *>(sph);
     a.cell_off = cell_off;
     a.cell_faces = cell_faces;
     a.grid = grid;

@@ -389,10 +389,16 @@ int csg_grid_index_build(const void *pts, int pts_dtype, const int *faces, int n
  * synchronisation.
  * Replaces RotationGridIndexDevice.cost_batch_prepared (surface_rotation.py:512-546). */
 size_t csg_rotation_costs_workspace_bytes(int ns, int na);
+/* Face bounding spheres for the rotation locator (tri: [nf][3][3] float32 → sph: [nf][4]
+ * float32 centre and radius, enlarged to cover the locator's tolerances).  Passed as `sph`
+ * to csg_rotation_costs / _cost_matrices / _refine, it lets the locator skip candidates
+ * that cannot be the closest; NULL scans every candidate.  The results are the same. */
+int csg_face_spheres(const float *tri, int nf, float *sph, void *stream);
+
 int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
                        const double *tgt_vals, const int *faces, const float *tri,
-                       const int *cell_off, const int *cell_faces, int grid,
-                       const double *angles, int na, double *costs, int *loc_face,
+                       const float *sph, const int *cell_off, const int *cell_faces,
+                       int grid, const double *angles, int na, double *costs, int *loc_face,
                        float *loc_bary, void *ws, size_t ws_bytes, int *uncovered_host,
                        int *uncovered_dev, void *stream);
 
@@ -408,8 +414,8 @@ int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
 size_t csg_rotation_cost_matrices_workspace_bytes(int ns, int na, int with_gradient);
 int csg_rotation_cost_matrices(const float *src_pts, const double *src_vals, int ns,
                                const double *tgt_vals, const int *faces, const float *tri,
-                               const int *cell_off, const int *cell_faces, int grid,
-                               const float *mats, int na, int with_gradient, double *out,
+                               const float *sph, const int *cell_off, const int *cell_faces,
+                               int grid, const float *mats, int na, int with_gradient, double *out,
                                void *ws, size_t ws_bytes, int *uncovered_dev, void *stream);
 
 /* Rotation SSD of na <= 65535 candidate matrices (float32 [na][9], row-major) on feature
@@ -437,9 +443,9 @@ int csg_rotation_map_costs(const double *src_map, const double *tgt_map, int nx,
 size_t csg_rotation_refine_workspace_bytes(int ns, int m);
 int csg_rotation_refine(const float *src_pts, const double *src_vals, int ns,
                         const double *tgt_vals, const int *faces, const float *tri,
-                        const int *cell_off, const int *cell_faces, int grid, const double *start,
-                        int start_is_angles, int m, int steps, double max_step, double damping,
-                        double *mats_out, double *angles_out, double *costs_out, void *ws,
+                        const float *sph, const int *cell_off, const int *cell_faces, int grid,
+                        const double *start, int start_is_angles, int m, int steps,
+                        double max_step, double damping, double *mats_out, double *angles_out, double *costs_out, void *ws,
                         size_t ws_bytes, int *uncovered_dev, void *stream);
 
 /* Exact closest triangle (first minimum among the query cell's candidates) with

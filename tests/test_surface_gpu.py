@@ -656,3 +656,47 @@ def test_closest_triangles_pruned_equals_full_scan(level, grid, cuda):
         set_closest_pruning(prev)
     for a, b in zip(got, ref):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("case", ["golden", "level6"])
+def test_rotation_locator_skip_equals_full_scan(case, golden, cuda):
+    """The locator's bounding-sphere skip leaves every location, cost, gradient
+    term and refined rotation unchanged (bit for bit)."""
+
+    import cat_surface_gpu as csg
+    from cat_surface_gpu.stencil_builder import icosphere
+    from cat_surface_gpu.surface_rotation import (axis_angle_matrix, refine_adjoint_launch,
+                                                  rotation_matrix_np, set_locator_skip)
+
+    rng = np.random.default_rng(21)
+    if case == "golden":
+        g = golden("rotation")
+        idx = csg.RotationGridIndex.from_geometry(g["verts"], g["faces"], grid_size=32,
+                                                  margin=1, device=cuda).to(cuda)
+        inputs = idx.prepare_cost_inputs(g["src_pts"], g["src_vals"], g["tgt_vals"])
+    else:
+        v, f = icosphere(6)
+        src, _ = icosphere(5)
+        idx = csg.RotationGridIndex.from_geometry(v, f, grid_size=128, margin=1,
+                                                  device=cuda).to(cuda)
+        inputs = idx.prepare_cost_inputs(src.astype(np.float32), rng.normal(size=src.shape[0]),
+                                         rng.normal(size=v.shape[0]))
+    angles = torch.as_tensor(rng.uniform(-0.7, 0.7, size=(32, 3)))
+    mats = np.stack([axis_angle_matrix(rng.normal(scale=0.3, size=3)) @
+                     rotation_matrix_np(a) for a in angles.numpy()[:8]])
+    out = {}
+    prev = set_locator_skip(False)
+    try:
+        for skip in (False, True):
+            set_locator_skip(skip)
+            faces, bary = idx.locate(inputs, angles[:6])
+            costs = idx.cost_batch_prepared(inputs, angles)
+            terms = idx.cost_matrices(inputs, mats, gradient=True)
+            refined = refine_adjoint_launch(idx, inputs,
+                                            torch.as_tensor(mats.reshape(-1, 9), device=cuda),
+                                            start_is_angles=False, steps=5)
+            out[skip] = [faces, bary, costs, *terms, *refined[:3]]
+    finally:
+        set_locator_skip(prev)
+    for a, b in zip(out[False], out[True]):
+        assert torch.equal(a, b)
