@@ -124,6 +124,8 @@ struct CoarseLevels {
 size_t fmg_lds_budget();
 template <typename T>
 size_t tile_lds_bytes(int sweeps, TileShape tile);
+// Whether the tile's pixels of one colour fit the relaxation kernel's threads.
+bool tile_fits_threads(int sweeps, TileShape tile);
 template <typename T>
 size_t coarse_lds_bytes(const int *nx, const int *ny, int c0, int L);
 

@@ -275,8 +275,8 @@ int fused_first_coarse_level(const FmgPlan<T> &p, int nit)
     for (int l = 0; l < c0; ++l) {
         if ((p.nx[l] & 1) || p.nx[l] < 2)
             return -1;
-        if (tile_lds_bytes<T>(2 * nit, relax_tile_shape(p.nx[l], p.ny[l], (int)sizeof(T))) >
-            budget)
+        const TileShape tile = relax_tile_shape(p.nx[l], p.ny[l], (int)sizeof(T));
+        if (tile_lds_bytes<T>(2 * nit, tile) > budget || !tile_fits_threads(2 * nit, tile))
             return -1;
     }
     return c0;

@@ -123,17 +123,37 @@ struct Stencil {
     T syy, sxy;    // channel 1
 };
 
+// The pixel's 2x2 block A + w00 (diagonal dx, dy, off-diagonal cr) and its
+// regularised determinant; constant over a relax_membrane call.
+template <typename T>
+struct RelaxCoef {
+    T dx, dy, cr, det;
+};
+
+template <typename T>
+__device__ __forceinline__ RelaxCoef<T> relax_coef(T a0, T a1, T a2, T w00)
+{
+    const T dx = a0 + w00, dy = a1 + w00, cr = a2;
+    return {dx, dy, cr, dx * dy * (T)1.0000000001 - cr * cr};
+}
+
+// Gauss-Seidel block update of one pixel from its block.
+template <typename T>
+__device__ __forceinline__ Pair<T> relax_solve(T bx, T by, const Stencil<T> &n,
+                                               const RelaxCoef<T> &k, T w01, T w10)
+{
+    T rx = bx - w01 * n.syx - w10 * n.sxx;
+    T ry = by - w01 * n.syy - w10 * n.sxy;
+    return {(k.dy * rx - k.cr * ry) / k.det, (-k.cr * rx + k.dx * ry) / k.det};
+}
+
 // Gauss-Seidel block update of one pixel: solve the 2x2 system
 // (A + w00) u = b - w01·Σ_y u - w10·Σ_x u.  dartel_triton.py:241-291
 template <typename T>
 __device__ __forceinline__ Pair<T> relax_pixel(T bx, T by, const Stencil<T> &n, T a0, T a1,
                                                T a2, T w00, T w01, T w10)
 {
-    T rx = bx - w01 * n.syx - w10 * n.sxx;
-    T ry = by - w01 * n.syy - w10 * n.sxy;
-    T dx = a0 + w00, dy = a1 + w00, cr = a2;
-    T det = dx * dy * (T)1.0000000001 - cr * cr;
-    return {(dy * rx - cr * ry) / det, (-cr * rx + dx * ry) / det};
+    return relax_solve<T>(bx, by, n, relax_coef<T>(a0, a1, a2, w00), w01, w10);
 }
 
 // (L + A) u at one pixel.  dartel_triton.py:189-239

@@ -33,11 +33,19 @@ namespace {
 // x0 - H .. x0 + TX + H - 1 in unwrapped coordinates (nx is even, so a
 // pixel's colour is the same unwrapped or wrapped) and the rows of
 // y0 - H .. y0 + TY + H - 1 that exist (the y boundary reflects onto rows
-// inside the grid).  Pixels whose neighbours lie outside the tile are not
-// updated; after s sweeps every pixel at distance >= s from the tile edge
-// holds the value of the global sweep, so the interior is exact after H.
+// inside the grid).  Half-sweep s updates the pixels at distance >= s + 1
+// from the tile's cut edges (a row range that reaches the grid's y boundary
+// is not cut): after s sweeps every pixel at distance >= s holds the value of
+// the global sweep, so the interior (distance >= H) is exact after H.
+//
+// Only u (two planes) is staged in LDS.  Each thread owns up to kTilePix
+// pixels of each colour for the whole call (colour c's pixels of the tile in
+// row-major order, entries tid + k·kTileThreads), with their LDS offsets,
+// right-hand side and 2x2 block (relax_coef) in registers.
 //
 // HC > 0 fixes the halo (= sweeps) at compile time; HC == 0 takes it from H.
+constexpr int kTilePix = 4;
+
 template <typename T, int HC>
 __global__ void __launch_bounds__(kTileThreads)
 k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restrict__ Uin,
@@ -60,9 +68,9 @@ k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restri
     const int ex = TX + 2 * H;
     const int x0 = txi * TX - H;
     const int ylo = max(0, tyi * TY - H), yhi = min(ny, tyi * TY + TY + H);
-    const int npx = ex * (yhi - ylo);
-    T *u0 = lds, *u1 = lds + npx, *a0 = lds + 2 * npx, *a1 = lds + 3 * npx, *a2 = lds + 4 * npx,
-      *b0 = lds + 5 * npx, *b1 = lds + 6 * npx;
+    const int rows = yhi - ylo;
+    const int npx = ex * rows;
+    T *u0 = lds, *u1 = lds + npx;
     const float inv_ex = 1.0f / (float)ex;
     for (int q = threadIdx.x; q < npx; q += blockDim.x) {
         const DivMod ri = divmod_small(q, ex, inv_ex);
@@ -79,41 +87,63 @@ k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restri
         }
         u0[q] = ux;
         u1[q] = uy;
-        a0[q] = A[p];
-        a1[q] = A[P + p];
-        a2[q] = A[2 * P + p];
-        b0[q] = B[p];
-        b1[q] = B[P + p];
     }
-    __syncthreads();
-    // Half-sweep s only updates the pixels at distance >= s + 1 from the tile's
-    // cut edges (a row range that reaches the grid's y boundary is not cut):
-    // the others cannot influence the interior any more, so the interior's
-    // values are those of updating every pixel whose neighbours are in the tile.
-    for (int s = 0; s < H; ++s) {
-        const int colour = (s & 1) ? 0 : 1;
-        const int ilo = s + 1, hw = (ex - 2 * ilo) / 2;  // ex is even
-        const int rlo = ylo == 0 ? 0 : ilo;
-        const int rhi = (yhi - ylo) - (yhi == ny ? 0 : ilo);
-        const float inv_hw = 1.0f / (float)hw;
-        for (int j = threadIdx.x; j < hw * (rhi - rlo); j += blockDim.x) {
-            const DivMod rk = divmod_small(j, hw, inv_hw);
-            const int r = rlo + rk.q;
-            const int gy = ylo + r;
-            // the column of this row with (x0 + i + gy) & 1 == colour
-            const int i = ilo + 2 * rk.r + ((colour - x0 - gy - ilo) & 1);
-            if (i == 0 || i == ex - 1)
+    // this thread's pixels
+    const int hex = ex / 2;  // pixels of one colour per tile row (ex is even)
+    const float inv_hex = 1.0f / (float)hex;
+    int pq[2][kTilePix], pym[2][kTilePix], pyp[2][kTilePix], plim[2][kTilePix];
+    RelaxCoef<T> pk[2][kTilePix];
+    T pbx[2][kTilePix], pby[2][kTilePix];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+#pragma unroll
+        for (int k = 0; k < kTilePix; ++k) {
+            const int j = threadIdx.x + k * kTileThreads;
+            plim[c][k] = 0;
+            pq[c][k] = pym[c][k] = pyp[c][k] = 0;
+            pk[c][k] = RelaxCoef<T>{(T)1, (T)1, (T)0, (T)1};
+            pbx[c][k] = pby[c][k] = (T)0;
+            if (j >= hex * rows)
+                continue;
+            const DivMod rk = divmod_small(j, hex, inv_hex);
+            const int r = rk.q, gy = ylo + r;
+            // the column of this row with (x0 + i + gy) & 1 == c
+            const int i = 2 * rk.r + ((c - x0 - gy) & 1);
+            int d = min(i, ex - 1 - i);
+            if (ylo > 0)
+                d = min(d, r);
+            if (yhi < ny)
+                d = min(d, rows - 1 - r);
+            if (d < 1)
                 continue;
             const int ym = reflect_y(gy - 1, ny), yp = reflect_y(gy + 1, ny);
-            if (ym < ylo || ym >= yhi || yp < ylo || yp >= yhi)
-                continue;
-            const int q = r * ex + i;
-            const int qym = (ym - ylo) * ex + i, qyp = (yp - ylo) * ex + i;
-            const Stencil<T> n{u0[q],           u1[q],           u0[qym] + u0[qyp],
-                               u0[q - 1] + u0[q + 1], u1[qym] + u1[qyp], u1[q - 1] + u1[q + 1]};
-            const Pair<T> v = relax_pixel<T>(b0[q], b1[q], n, a0[q], a1[q], a2[q], w00, w01, w10);
-            u0[q] = v.x;
-            u1[q] = v.y;
+            const int p = gy * nx + wrap_x(x0 + i, nx);
+            plim[c][k] = d;
+            pq[c][k] = r * ex + i;
+            pym[c][k] = (ym - ylo) * ex + i;
+            pyp[c][k] = (yp - ylo) * ex + i;
+            pk[c][k] = relax_coef<T>(A[p], A[P + p], A[2 * P + p], w00);
+            pbx[c][k] = B[p];
+            pby[c][k] = B[P + p];
+        }
+    }
+    __syncthreads();
+    for (int s = 0; s < H; ++s) {
+        const int c = (s & 1) ? 0 : 1;
+#pragma unroll
+        for (int k = 0; k < kTilePix; ++k) {
+            const int q = pq[c][k], qym = pym[c][k], qyp = pyp[c][k];
+            if (s < plim[c][k]) {
+                const Stencil<T> n{(T)0,
+                                   (T)0,
+                                   u0[qym] + u0[qyp],
+                                   u0[q - 1] + u0[q + 1],
+                                   u1[qym] + u1[qyp],
+                                   u1[q - 1] + u1[q + 1]};
+                const Pair<T> v = relax_solve<T>(pbx[c][k], pby[c][k], n, pk[c][k], w01, w10);
+                u0[q] = v.x;
+                u1[q] = v.y;
+            }
         }
         __syncthreads();
     }
@@ -498,7 +528,15 @@ size_t fmg_lds_budget()
 template <typename T>
 size_t tile_lds_bytes(int sweeps, TileShape tile)
 {
-    return (size_t)(tile.x + 2 * sweeps) * (tile.y + 2 * sweeps) * 7 * sizeof(T);
+    return (size_t)(tile.x + 2 * sweeps) * (tile.y + 2 * sweeps) * 2 * sizeof(T);
+}
+
+// Pixels of one colour in a tile with its halo: each thread owns at most
+// kTilePix of them.
+bool tile_fits_threads(int sweeps, TileShape tile)
+{
+    return (size_t)(tile.x + 2 * sweeps) / 2 * (tile.y + 2 * sweeps) <=
+           (size_t)kTilePix * kTileThreads;
 }
 
 namespace {
@@ -535,10 +573,13 @@ int set_relax_tile_shape(int nx, int ny, int elem, int tx, int ty)
     if (tx < 2 || (tx & 1) || ty < 1)
         return fail(kInvalidArgument, "tile width must be even and >= 2, height >= 1 (got %d x %d)",
                     tx, ty);
-    const size_t need = (size_t)(tx + 12) * (ty + 12) * 7 * elem;
+    const size_t need = (size_t)(tx + 12) * (ty + 12) * 2 * elem;
     if (need > fmg_lds_budget())
         return fail(kInvalidArgument, "a %d x %d tile needs %zu bytes of LDS, over the budget",
                     tx, ty, need);
+    if (!tile_fits_threads(6, TileShape{tx, ty}))
+        return fail(kInvalidArgument, "a %d x %d tile has more pixels of one colour than its "
+                                      "threads hold", tx, ty);
     std::lock_guard<std::mutex> lock(g_tile_mutex);
     g_tile_table[TileKey{nx, ny, elem}] = TileShape{tx, ty};
     return kOk;
@@ -599,8 +640,9 @@ int launch_relax_tile(const T *a, const T *b, const T *uin, T *uout, const T *uc
         return fail(kInvalidArgument, "tiled relaxation needs an even width, got %d", nx);
     const TileShape tile = relax_tile_shape(nx, ny, (int)sizeof(T));
     size_t lds = tile_lds_bytes<T>(sweeps, tile);
-    if (lds > fmg_lds_budget())
-        return fail(kInvalidArgument, "tiled relaxation needs %zu bytes of LDS", lds);
+    if (lds > fmg_lds_budget() || !tile_fits_threads(sweeps, tile))
+        return fail(kInvalidArgument, "tiled relaxation of %d x %d tiles with %d sweeps does not "
+                                      "fit (%zu bytes of LDS)", tile.x, tile.y, sweeps, lds);
     const int tiles_x = (nx + tile.x - 1) / tile.x, tiles_y = (ny + tile.y - 1) / tile.y;
     dim3 grid(tiles_x * tiles_y, batch);
     // reference bytes: 9 values per pixel per red+black iteration, plus the
