@@ -124,6 +124,9 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_closest_triangles": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i, _d, _vp, _vp, _vp,
                                    ctypes.POINTER(_i), _vp, _sz, _vp]),
     "csg_closest_triangles_workspace_bytes": (_sz, [_i]),
+    "csg_stencil_sphere": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp]),
+    "csg_resample_unit": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "csg_unit_points64": (_i, [_vp, _i, _vp, _vp]),
     "csg_closest_triangles_pruned": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _i, _d, _vp, _vp, _vp,
                                           _vp, ctypes.POINTER(_i), _vp, _sz, _vp]),
 }

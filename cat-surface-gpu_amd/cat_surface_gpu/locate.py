@@ -146,15 +146,11 @@ def closest_triangles(queries: torch.Tensor, verts: torch.Tensor, faces: torch.T
     (check_all) and otherwise recomputes with ``deferred=False``.
     """
 
-    q = queries.to(torch.float64)
-    v = verts.to(torch.float64)
-    # the grid spans [-1, 1]^3: scale both point sets into it when they reach
-    # outside (argmin and weights are invariant under a common scale); decided
-    # on the device, without a host round trip
-    extent = torch.maximum(q.abs().max(), v.abs().max())
-    scale = torch.where(extent > 1.0, extent * 1.0001, torch.ones_like(extent))
-    q = (q / scale).contiguous()
-    v = (v / scale).contiguous()
+    # the grid spans [-1, 1]^3; points outside fall into the boundary cells (the
+    # kernels clamp cell indices), which keeps every list a superset of what
+    # the reach argument needs, so no rescaling is required
+    q = queries.to(torch.float64).contiguous()
+    v = verts.to(torch.float64).contiguous()
     f = faces.to(torch.int32).contiguous()
     nq = int(q.shape[0])
     dev = q.device

@@ -29,6 +29,8 @@ import math
 import numpy as np
 import torch
 
+from . import _lib
+
 from .locate import closest_triangles
 from .surface_stencil import SurfaceStencil
 
@@ -157,34 +159,42 @@ def build_device_stencil(sphere_vertices, sphere_faces, *, surface_vertices=None
     from .surface_stencil import SurfaceStencilDevice
 
     dev = resolve_device("auto" if device is None else device)
-    sv = torch.as_tensor(sphere_vertices, device=dev).to(torch.float32)
+    sv = torch.as_tensor(sphere_vertices, device=dev).to(torch.float32).contiguous()
     sf = torch.as_tensor(sphere_faces, device=dev).to(torch.int32).contiguous()
     if sv.ndim != 2 or sv.shape[1] != 3 or sf.ndim != 2 or sf.shape[1] != 3:
         raise ValueError("sphere vertices and faces must have shape [n, 3]")
     if surface_vertices is not None and tuple(np.shape(surface_vertices)) != tuple(sv.shape):
         raise ValueError("Surface and sphere vertex counts do not match")
-    pts = sv.to(torch.float64)
-    if normalise:
-        pts = pts - pts.mean(dim=0, keepdim=True)
-        pts = pts / torch.linalg.vector_norm(pts, dim=1, keepdim=True)
-    radius = 0.975 * torch.linalg.vector_norm(pts, dim=1).mean()
-    lo = pts.min(dim=0).values
-    hi = pts.max(dim=0).values
-    centre = lo + hi  # the helper passes the bound sums as the centre
+    lib = _lib.lib()
+    stream = _lib.stream_handle(dev)
+    n = int(sv.shape[0])
     ico_v, ico_f, topo = _device_icosphere(subdivisions_for(n_triangles), dev)
-    out_sphere = centre[None, :] + radius * ico_v
+    m = int(ico_v.shape[0])
+    # float64 (normalised) sphere and the output sphere at 0.975 of its mean
+    # radius around the bound sums (the helper's choice of centre)
+    pts = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    out_sphere = torch.empty((m, 3), dtype=torch.float64, device=dev)
+    params = torch.empty(8, dtype=torch.float64, device=dev)
+    _lib.check(lib.csg_stencil_sphere(_lib.ptr(sv), n, int(bool(normalise)), _lib.ptr(ico_v), m,
+                                      _lib.ptr(pts), _lib.ptr(out_sphere), _lib.ptr(params),
+                                      stream), "csg_stencil_sphere")
     # surface stencil: closest input-sphere triangle per resampled vertex
     first = closest_triangles(out_sphere, pts, sf, grid_size=32, margin=1, deferred=deferred)
     surf_idx, surf_w = first[1], first[2]
-    resampled = (pts[surf_idx.long()] * surf_w[..., None]).sum(dim=1)
+    resampled = torch.empty((m, 3), dtype=torch.float64, device=dev)
+    unit_res = torch.empty((m, 3), dtype=torch.float64, device=dev)
+    _lib.check(lib.csg_resample_unit(_lib.ptr(pts), _lib.ptr(surf_idx), _lib.ptr(surf_w), m,
+                                     _lib.ptr(resampled), _lib.ptr(unit_res), stream),
+               "csg_resample_unit")
     # sheet stencil on the unit resampled sphere
-    unit_res = resampled / torch.linalg.vector_norm(resampled, dim=1, keepdim=True)
     second = closest_triangles(_device_uv(nx, ny, dev), unit_res, ico_f, grid_size=64,
                                margin=1, deferred=deferred)
     sheet_idx, sheet_w = second[1], second[2]
-    unit_pts = (pts / torch.linalg.vector_norm(pts, dim=1, keepdim=True)).to(torch.float32)
+    unit_pts = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    _lib.check(lib.csg_unit_points64(_lib.ptr(pts), n, _lib.ptr(unit_pts), stream),
+               "csg_unit_points64")
     stencil = SurfaceStencilDevice(
-        sphere_points=resampled.to(torch.float32).contiguous(),
+        sphere_points=resampled.to(torch.float32),
         faces=topo.faces,
         surface_indices=surf_idx.contiguous(),
         surface_weights=surf_w.contiguous(),
@@ -193,8 +203,8 @@ def build_device_stencil(sphere_vertices, sphere_faces, *, surface_vertices=None
         nx=nx,
         ny=ny,
         source_points=int(sv.shape[0]),
-        unit_sphere_points=unit_pts.contiguous(),
+        unit_sphere_points=unit_pts,
         topology=topo,
-        sphere_points64=resampled.contiguous(),
+        sphere_points64=resampled,
     )
     return (stencil, [first[3], second[3]]) if deferred else stencil

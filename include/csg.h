@@ -460,6 +460,18 @@ int csg_closest_triangles(const double *q, int nq, const double *verts, const in
                           int *face_out, int *idx_out, double *w_out, int *far_host,
                           void *ws, size_t ws_bytes, void *stream);
 
+/* Steps of the in-process stencil builder around its closest-triangle searches
+ * (native/cat_surface_stencil_builder.c).  csg_stencil_sphere: pts = float64(v), centred
+ * on the mean and scaled to unit length when normalise; out_sphere[j] = (lo + hi) + 0.975 ·
+ * mean |pts| · ico[j] with lo / hi the coordinate bounds of pts (params: 8 doubles of
+ * device scratch).  csg_resample_unit: res[j] = sum_k w[j][k] pts[idx[j][k]], unit[j] =
+ * res[j] / |res[j]|.  csg_unit_points64: out = float32(p / |p|). */
+int csg_stencil_sphere(const float *v, int n, int normalise, const double *ico, int m,
+                       double *pts, double *out_sphere, double *params, void *stream);
+int csg_resample_unit(const double *pts, const int *idx, const double *w, int m, double *res,
+                      double *unit, void *stream);
+int csg_unit_points64(const double *pts, int n, float *out, void *stream);
+
 /* csg_closest_triangles with each candidate first tested against its face's bounding
  * sphere (computed into ws, csg_closest_triangles_workspace_bytes(nf) bytes): a candidate
  * farther than the scanning lane's best is skipped, which changes no result.  The far
