@@ -113,6 +113,14 @@ def build_stencil(sphere_vertices, sphere_faces, *, surface_vertices=None, norma
     )
 
 
+# Index grids of the two searches (cells per axis over [-1, 1]^3): the input
+# sphere's faces for the resampled vertices, the resampled sphere's faces for
+# the map pixels.  Any grid whose one-cell reach covers the queries gives the
+# same stencil; finer ones scan fewer candidates but sort more pairs (measured
+# at 61,442 vertices: 32/64 0.86 ms per build, 64/128 0.93 ms).
+_GRID_SURFACE = 32
+_GRID_SHEET = 64
+
 _DEVICE_SPHERES: dict = {}
 _DEVICE_UV: dict = {}
 
@@ -179,7 +187,11 @@ def build_device_stencil(sphere_vertices, sphere_faces, *, surface_vertices=None
                                       _lib.ptr(pts), _lib.ptr(out_sphere), _lib.ptr(params),
                                       stream), "csg_stencil_sphere")
     # surface stencil: closest input-sphere triangle per resampled vertex
-    first = closest_triangles(out_sphere, pts, sf, grid_size=32, margin=1, deferred=deferred)
+    # (the result is the global closest triangle, first by face order on ties,
+    # whatever the grid: the grid only sets how many candidates a query scans;
+    # the output sphere lies 2.5 % inside the input sphere, within one cell)
+    first = closest_triangles(out_sphere, pts, sf, grid_size=_GRID_SURFACE, margin=1,
+                              deferred=deferred)
     surf_idx, surf_w = first[1], first[2]
     resampled = torch.empty((m, 3), dtype=torch.float64, device=dev)
     unit_res = torch.empty((m, 3), dtype=torch.float64, device=dev)
@@ -187,8 +199,8 @@ def build_device_stencil(sphere_vertices, sphere_faces, *, surface_vertices=None
                                      _lib.ptr(resampled), _lib.ptr(unit_res), stream),
                "csg_resample_unit")
     # sheet stencil on the unit resampled sphere
-    second = closest_triangles(_device_uv(nx, ny, dev), unit_res, ico_f, grid_size=64,
-                               margin=1, deferred=deferred)
+    second = closest_triangles(_device_uv(nx, ny, dev), unit_res, ico_f,
+                               grid_size=_GRID_SHEET, margin=1, deferred=deferred)
     sheet_idx, sheet_w = second[1], second[2]
     unit_pts = torch.empty((n, 3), dtype=torch.float32, device=dev)
     _lib.check(lib.csg_unit_points64(_lib.ptr(pts), n, _lib.ptr(unit_pts), stream),

@@ -419,8 +419,10 @@ def test_deferred_stencil_equals_synchronous(cuda):
         np.testing.assert_array_equal(_np(getattr(got, name)), _np(getattr(ref, name)), name)
     # a query set whose closest triangles lie beyond one cell fails its check
     far_q = torch.as_tensor(np.array([[0.0, 0.0, 0.0], [0.05, 0.0, 0.0]]), device=cuda)
-    face, idx, w, check = closest_triangles(far_q, sphere.double(), faces, grid_size=32,
-                                            margin=1, deferred=True)
+    from cat_surface_gpu.stencil_builder import _GRID_SURFACE
+
+    face, idx, w, check = closest_triangles(far_q, sphere.double(), faces,
+                                            grid_size=_GRID_SURFACE, margin=1, deferred=True)
     assert check is not None and check_all([check]) == [False]
 
 
