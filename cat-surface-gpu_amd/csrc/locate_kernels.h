@@ -46,6 +46,9 @@ struct RotationCostArgs {
     int *uncovered;
     int *loc_face;    // [na][ns] or null
     float *loc_bary;  // [na][ns][3] or null
+    // [na][ns] or null (matrix costs only): each point's face for each candidate
+    // from the previous launch (-1: none), read to start the scan, then updated
+    int *hint;
 };
 
 int launch_rotation_costs(const RotationCostArgs &a, hipStream_t s);

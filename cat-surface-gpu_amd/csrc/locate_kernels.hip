@@ -281,6 +281,76 @@ struct LocateBest {
     float bp[3];
 };
 
+// Squared distance from q to one face and the closest point (the reference's
+// projection test with its 1e-3 barycentric slack, else the nearest vertex's
+// two edges).
+__device__ __forceinline__ float face_distance(const float *q, const float *__restrict__ tri,
+                                               int face, float *cpt)
+{
+    const float *t = tri + 9 * (size_t)face;
+    float p0[3] = {t[0], t[1], t[2]}, p1[3] = {t[3], t[4], t[5]}, p2[3] = {t[6], t[7], t[8]};
+    float ea[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    float eb[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+    float nrm[3];
+    cross3(ea, eb, nrm);
+    float nsq = dot3(nrm, nrm);
+    float off[3] = {p0[0] - q[0], p0[1] - q[1], p0[2] - q[2]};
+    float pt = nsq != 0.0f ? dot3(off, nrm) / nsq : 0.0f;
+    float proj[3] = {q[0] + pt * nrm[0], q[1] + pt * nrm[1], q[2] + pt * nrm[2]};
+    float po[3] = {proj[0] - p0[0], proj[1] - p0[1], proj[2] - p0[2]};
+    float xx = dot3(ea, ea), xy = dot3(ea, eb), xv = dot3(ea, po), yy = dot3(eb, eb),
+          yv = dot3(eb, po);
+    float den = xx * yy - xy * xy;
+    float xp = den != 0.0f ? (xv * yy - yv * xy) / den : 0.0f;
+    float yp = den != 0.0f ? (yv * xx - xv * xy) / den : 0.0f;
+    bool inside = den != 0.0f && xp >= -1.0e-3f && xp <= 1.0f + 1.0e-3f && yp >= -1.0e-3f &&
+                  yp <= 1.0f + 1.0e-3f && xp + yp >= -1.0e-3f && xp + yp <= 1.0f + 1.0e-3f;
+    if (inside) {
+        cpt[0] = proj[0];
+        cpt[1] = proj[1];
+        cpt[2] = proj[2];
+        return pt * pt * nsq;
+    }
+    // nearest vertex (first minimum), then the two edges through it; selects
+    // instead of indexed pointer tables keep this in registers
+    float e0 = q[0] - p0[0], e1 = q[1] - p0[1], e2 = q[2] - p0[2];
+    const float dv0 = e0 * e0 + e1 * e1 + e2 * e2;
+    e0 = q[0] - p1[0], e1 = q[1] - p1[1], e2 = q[2] - p1[2];
+    const float dv1 = e0 * e0 + e1 * e1 + e2 * e2;
+    e0 = q[0] - p2[0], e1 = q[1] - p2[1], e2 = q[2] - p2[2];
+    const float dv2 = e0 * e0 + e1 * e1 + e2 * e2;
+    int cv = 0;
+    float dbest = dv0;
+    if (dv1 < dbest) {
+        cv = 1;
+        dbest = dv1;
+    }
+    if (dv2 < dbest)
+        cv = 2;
+    float prev[3], cur[3], next[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        cur[d] = cv == 0 ? p0[d] : (cv == 1 ? p1[d] : p2[d]);
+        prev[d] = cv == 0 ? p2[d] : (cv == 1 ? p0[d] : p1[d]);
+        next[d] = cv == 0 ? p1[d] : (cv == 1 ? p2[d] : p0[d]);
+    }
+    float d1, d2, q1[3], q2[3];
+    seg_dist(q, prev, cur, &d1, q1);
+    seg_dist(q, cur, next, &d2, q2);
+    if (d1 < d2) {
+        cpt[0] = q1[0];
+        cpt[1] = q1[1];
+        cpt[2] = q1[2];
+        return d1;
+    }
+    cpt[0] = q2[0];
+    cpt[1] = q2[1];
+    cpt[2] = q2[2];
+    return d2;
+}
+
+// Ties are broken by the smaller face index, which is the sequential scan's
+// first minimum (a cell lists its faces in increasing order).
 //
 // With face spheres (sph: centre, radius enlarged by 2 % and 1e-6, see
 // k_face_spheres_f32), a candidate whose sphere lies farther than
@@ -289,93 +359,61 @@ struct LocateBest {
 // triangle (>= |q - c| - r); inside, the in-plane point lies within 1.004 r
 // of c (the 1e-3 barycentric slack), so the plane distance is >= |q - c| -
 // 1.004 r.  Skipping it changes no result.
+//
+// `start` (a face known to be in this cell's list, or -1) is evaluated first:
+// it is one of the candidates, and a close start lets the sphere test skip
+// most of the others.
 __device__ __forceinline__ LocateBest scan_candidates(const float *q, const float *__restrict__ tri,
                                                      const int *__restrict__ cand, int ncand,
                                                      int lane, int lanes,
-                                                     const float4 *__restrict__ sph)
+                                                     const float4 *__restrict__ sph,
+                                                     int start = -1)
 {
     LocateBest r{INFINITY, 0x7fffffff, -1, {0.0f, 0.0f, 0.0f}};
     float reach = INFINITY;
-    for (int c = lane; c < ncand; c += lanes) {
-        int face = cand[c];
-        if (sph) {
-            const float4 b = sph[face];
-            const float dx = q[0] - b.x, dy = q[1] - b.y, dz = q[2] - b.z;
-            const float lim = b.w + reach;
-            if (dx * dx + dy * dy + dz * dz > lim * lim)
-                continue;
-        }
-        const float *t = tri + 9 * (size_t)face;
-        float p0[3] = {t[0], t[1], t[2]}, p1[3] = {t[3], t[4], t[5]}, p2[3] = {t[6], t[7], t[8]};
-        float ea[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
-        float eb[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
-        float nrm[3];
-        cross3(ea, eb, nrm);
-        float nsq = dot3(nrm, nrm);
-        float off[3] = {p0[0] - q[0], p0[1] - q[1], p0[2] - q[2]};
-        float pt = nsq != 0.0f ? dot3(off, nrm) / nsq : 0.0f;
-        float proj[3] = {q[0] + pt * nrm[0], q[1] + pt * nrm[1], q[2] + pt * nrm[2]};
-        float po[3] = {proj[0] - p0[0], proj[1] - p0[1], proj[2] - p0[2]};
-        float xx = dot3(ea, ea), xy = dot3(ea, eb), xv = dot3(ea, po), yy = dot3(eb, eb),
-              yv = dot3(eb, po);
-        float den = xx * yy - xy * xy;
-        float xp = den != 0.0f ? (xv * yy - yv * xy) / den : 0.0f;
-        float yp = den != 0.0f ? (yv * xx - xv * xy) / den : 0.0f;
-        bool inside = den != 0.0f && xp >= -1.0e-3f && xp <= 1.0f + 1.0e-3f && yp >= -1.0e-3f &&
-                      yp <= 1.0f + 1.0e-3f && xp + yp >= -1.0e-3f && xp + yp <= 1.0f + 1.0e-3f;
-        float dist, cpt[3];
-        if (inside) {
-            dist = pt * pt * nsq;
-            cpt[0] = proj[0];
-            cpt[1] = proj[1];
-            cpt[2] = proj[2];
-        } else {
-            // nearest vertex (first minimum), then the two edges through it;
-            // selects instead of indexed pointer tables keep this in registers
-            float e0 = q[0] - p0[0], e1 = q[1] - p0[1], e2 = q[2] - p0[2];
-            const float dv0 = e0 * e0 + e1 * e1 + e2 * e2;
-            e0 = q[0] - p1[0], e1 = q[1] - p1[1], e2 = q[2] - p1[2];
-            const float dv1 = e0 * e0 + e1 * e1 + e2 * e2;
-            e0 = q[0] - p2[0], e1 = q[1] - p2[1], e2 = q[2] - p2[2];
-            const float dv2 = e0 * e0 + e1 * e1 + e2 * e2;
-            int cv = 0;
-            float dbest = dv0;
-            if (dv1 < dbest) {
-                cv = 1;
-                dbest = dv1;
-            }
-            if (dv2 < dbest)
-                cv = 2;
-            float prev[3], cur[3], next[3];
+    if (start >= 0) {
+        r.dist = face_distance(q, tri, start, r.bp);
+        r.face = start;
+        reach = sqrtf(r.dist) * 1.0001f + 1e-6f;
+    }
+    // kScanBatch candidates per lane at a time: their indices, then their
+    // spheres, are fetched together (one memory round trip each instead of two
+    // per candidate); the order of evaluation does not matter (ties go to the
+    // smaller face index)
+    constexpr int kScanBatch = 4;
+    for (int c0 = lane; c0 < ncand; c0 += lanes * kScanBatch) {
+        int fc[kScanBatch];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                cur[d] = cv == 0 ? p0[d] : (cv == 1 ? p1[d] : p2[d]);
-                prev[d] = cv == 0 ? p2[d] : (cv == 1 ? p0[d] : p1[d]);
-                next[d] = cv == 0 ? p1[d] : (cv == 1 ? p2[d] : p0[d]);
-            }
-            float d1, d2, q1[3], q2[3];
-            seg_dist(q, prev, cur, &d1, q1);
-            seg_dist(q, cur, next, &d2, q2);
-            if (d1 < d2) {
-                dist = d1;
-                cpt[0] = q1[0];
-                cpt[1] = q1[1];
-                cpt[2] = q1[2];
-            } else {
-                dist = d2;
-                cpt[0] = q2[0];
-                cpt[1] = q2[1];
-                cpt[2] = q2[2];
-            }
+        for (int u = 0; u < kScanBatch; ++u) {
+            const int c = c0 + u * lanes;
+            fc[u] = c < ncand ? cand[c] : -1;
         }
-        if (dist < r.dist) {
-            r.dist = dist;
-            reach = sqrtf(dist) * 1.0001f + 1e-6f;
-            r.pos = c;
-            r.face = face;
-            r.bp[0] = cpt[0];
-            r.bp[1] = cpt[1];
-            r.bp[2] = cpt[2];
+        float4 bs[kScanBatch];
+#pragma unroll
+        for (int u = 0; u < kScanBatch; ++u)
+            bs[u] = (sph && fc[u] >= 0) ? sph[fc[u]] : make_float4(0.0f, 0.0f, 0.0f, INFINITY);
+#pragma unroll
+        for (int u = 0; u < kScanBatch; ++u) {
+            const int face = fc[u];
+            if (face < 0)
+                continue;
+            if (sph) {
+                const float dx = q[0] - bs[u].x, dy = q[1] - bs[u].y, dz = q[2] - bs[u].z;
+                const float lim = bs[u].w + reach;
+                if (dx * dx + dy * dy + dz * dz > lim * lim)
+                    continue;
+            }
+            float cpt[3];
+            const float dist = face_distance(q, tri, face, cpt);
+            if (dist < r.dist || (dist == r.dist && face < r.face)) {
+                r.dist = dist;
+                reach = sqrtf(dist) * 1.0001f + 1e-6f;
+                r.pos = c0 + u * lanes;
+                r.face = face;
+                r.bp[0] = cpt[0];
+                r.bp[1] = cpt[1];
+                r.bp[2] = cpt[2];
+            }
         }
     }
     return r;
@@ -394,7 +432,7 @@ __device__ __forceinline__ LocateBest reduce_lanes(LocateBest r, int lanes)
         o.bp[0] = __shfl_xor(r.bp[0], m);
         o.bp[1] = __shfl_xor(r.bp[1], m);
         o.bp[2] = __shfl_xor(r.bp[2], m);
-        if (o.dist < r.dist || (o.dist == r.dist && o.pos < r.pos))
+        if (o.dist < r.dist || (o.dist == r.dist && (unsigned)o.face < (unsigned)r.face))
             r = o;
     }
     return r;
@@ -504,7 +542,8 @@ k_rotation_cost_mats(const float *__restrict__ src_pts, const double *__restrict
                      const float *__restrict__ tri, const float4 *__restrict__ sph,
                      const int *__restrict__ cell_off, const int *__restrict__ cell_faces, int g,
                      const float *__restrict__ mats,
-                     double *__restrict__ partial, int nblocks, int *__restrict__ uncovered)
+                     double *__restrict__ partial, int nblocks, int *__restrict__ uncovered,
+                     int *__restrict__ hint)
 {
     constexpr int kTerms = GRAD ? kRotationGradTerms : 1;
     __shared__ float m[9];
@@ -528,8 +567,24 @@ k_rotation_cost_mats(const float *__restrict__ src_pts, const double *__restrict
         b = cell_off[cell];
         nc = cell_off[cell + 1] - b;
     }
-    LocateBest best = scan_candidates(q, tri, cell_faces + b, nc, lane, kLocateLanes, sph);
+    // the face this point found for this candidate last time starts the scan
+    // when it is in the cell's list
+    int start = -1;
+    if (hint) {
+        const int h = i < ns ? hint[(size_t)a * ns + i] : -1;
+        int found = 0;
+        if (h >= 0)
+            for (int c = lane; c < nc; c += kLocateLanes)
+                found |= cell_faces[b + c] == h;
+        for (int m = 1; m < kLocateLanes; m <<= 1)
+            found |= __shfl_xor(found, m);
+        start = found ? h : -1;
+    }
+    LocateBest best = scan_candidates(q, tri, cell_faces + b, nc, lane, kLocateLanes, sph,
+                                      start);
     best = reduce_lanes(best, kLocateLanes);
+    if (hint && i < ns && lane == 0)
+        hint[(size_t)a * ns + i] = best.face;
     if (i < ns && lane == 0) {
         if (best.face < 0) {
             atomicAdd(uncovered, 1);
@@ -713,11 +768,11 @@ int launch_rotation_cost_mats(const RotationCostArgs &a, const float *mats, bool
     if (gradient)
         hipLaunchKernelGGL(k_rotation_cost_mats<true>, grid, dim3(kBlock), 0, s, a.src_pts,
                            a.src_vals, a.ns, a.tgt_vals, a.faces, a.tri, a.sph, a.cell_off,
-                           a.cell_faces, a.grid, mats, a.partial, nb, a.uncovered);
+                           a.cell_faces, a.grid, mats, a.partial, nb, a.uncovered, a.hint);
     else
         hipLaunchKernelGGL(k_rotation_cost_mats<false>, grid, dim3(kBlock), 0, s, a.src_pts,
                            a.src_vals, a.ns, a.tgt_vals, a.faces, a.tri, a.sph, a.cell_off,
-                           a.cell_faces, a.grid, mats, a.partial, nb, a.uncovered);
+                           a.cell_faces, a.grid, mats, a.partial, nb, a.uncovered, a.hint);
     CSG_LAUNCH_CHECK();
     const int rows = a.na * (gradient ? kRotationGradTerms : 1);
     hipLaunchKernelGGL(k_sum_rows, dim3(rows), dim3(kBlock), 0, s, a.partial, nb, rows, a.costs);

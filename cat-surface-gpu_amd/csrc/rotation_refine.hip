@@ -202,8 +202,16 @@ __global__ void k_refine_final(RefineState st, int m, double *mats_out, double *
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// out[0 .. n) = v (a kernel rather than a memset node, for graph capture)
+__global__ void __launch_bounds__(kBlock) k_fill_int(int *__restrict__ out, int v, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = v;
+}
+
 struct RefineLayout {
-    size_t state, terms, partial, total;
+    size_t state, terms, partial, hint, total;
 };
 
 RefineLayout refine_layout(int ns, int m)
@@ -214,7 +222,8 @@ RefineLayout refine_layout(int ns, int m)
     l.terms = align256(sizeof(double) * (size_t)m * kRotationGradTerms);
     l.partial = align256(sizeof(double) * (size_t)rotation_cost_blocks(ns) * m *
                          kRotationGradTerms);
-    l.total = l.state + l.terms + l.partial;
+    l.hint = align256(sizeof(int) * (size_t)ns * m);
+    l.total = l.state + l.terms + l.partial + l.hint;
     return l;
 }
 
@@ -231,8 +240,9 @@ int csg_rotation_refine(const float *src_pts, const double *src_vals, int ns,
                         const double *tgt_vals, const int *faces, const float *tri,
                         const float *sph, const int *cell_off, const int *cell_faces, int grid,
                         const double *start, int start_is_angles, int m, int steps,
-                        double max_step, double damping, double *mats_out, double *angles_out, double *costs_out, void *ws,
-                        size_t ws_bytes, int *uncovered_dev, void *stream)
+                        double max_step, double damping, double *mats_out, double *angles_out,
+                        double *costs_out, void *ws, size_t ws_bytes, int *uncovered_dev,
+                        void *stream)
 {
     if (ns < 1 || grid < 1 || m < 1 || m > kRefineMax || steps < 0 || !(max_step > 0.0) ||
         !(damping > 0.0) || start == nullptr || uncovered_dev == nullptr)
@@ -270,6 +280,15 @@ int csg_rotation_refine(const float *src_pts, const double *src_vals, int ns,
     a.costs = terms;
     a.partial = (double *)(b + l.state + l.terms);
     a.uncovered = uncovered_dev;
+    // with face spheres, each step's located faces start the next step's scans
+    // (the trials move the points only slightly, so most candidates are then
+    // skipped); none for the first
+    a.hint = sph ? (int *)(b + l.state + l.terms + l.partial) : nullptr;
+    if (a.hint) {
+        hipLaunchKernelGGL(k_fill_int, dim3(grid_for((size_t)ns * m)), dim3(kBlock), 0, s,
+                           a.hint, -1, (size_t)ns * m);
+        CSG_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_refine_init, dim3(1), dim3(kRefineMax), 0, s, st, start,
                        start_is_angles, m, damping, uncovered_dev);
     CSG_LAUNCH_CHECK();
@@ -284,6 +303,7 @@ int csg_rotation_refine(const float *src_pts, const double *src_vals, int ns,
                        angles_out);
     CSG_LAUNCH_CHECK();
     // the reference's angle formulation of the final candidates
+    a.hint = nullptr;
     a.angles = angles_out;
     a.costs = costs_out;
     a.loc_face = nullptr;

@@ -445,8 +445,9 @@ int csg_rotation_refine(const float *src_pts, const double *src_vals, int ns,
                         const double *tgt_vals, const int *faces, const float *tri,
                         const float *sph, const int *cell_off, const int *cell_faces, int grid,
                         const double *start, int start_is_angles, int m, int steps,
-                        double max_step, double damping, double *mats_out, double *angles_out, double *costs_out, void *ws,
-                        size_t ws_bytes, int *uncovered_dev, void *stream);
+                        double max_step, double damping, double *mats_out, double *angles_out,
+                        double *costs_out, void *ws, size_t ws_bytes, int *uncovered_dev,
+                        void *stream);
 
 /* Exact closest triangle (first minimum among the query cell's candidates) with
  * barycentric weights.  The count of queries farther than `reach` from their best

@@ -696,7 +696,7 @@ def test_rotation_locator_skip_equals_full_scan(case, golden, cuda):
             terms = idx.cost_matrices(inputs, mats, gradient=True)
             refined = refine_adjoint_launch(idx, inputs,
                                             torch.as_tensor(mats.reshape(-1, 9), device=cuda),
-                                            start_is_angles=False, steps=5)
+                                            start_is_angles=False, steps=12)
             out[skip] = [faces, bary, costs, *terms, *refined[:3]]
     finally:
         set_locator_skip(prev)
