@@ -846,10 +846,49 @@ def type5_sheets_many(stencils, geometries, fwhms, out_dtype=torch.float64) -> l
     ems = edge_means(stencils, geoms)
     geom_iters = [heat_parameters_from_edge_mean(em, 50.0)[0] for em in ems]
     smoothed = smooth_geometry_many(stencils, geoms, geom_iters)
-    out = []
-    for st, g, em, gi, sm in zip(stencils, geoms, ems, geom_iters, smoothed):
-        params = [heat_parameters_from_edge_mean(em, f) for f in fw]
-        out.append(st._type5(g, gi, params, out_dtype, smoothed=sm.contiguous()))
+    out: list[torch.Tensor | None] = [None] * len(stencils)
+    if not fw:
+        return [torch.empty((0, st.ny, st.nx), dtype=out_dtype, device=st.device)
+                for st in stencils]
+    # sides of one topology and sheet shape share one value-smoothing loop
+    groups: dict[tuple, list[int]] = {}
+    for i, st in enumerate(stencils):
+        groups.setdefault((id(st.topology), st.nx, st.ny, str(st.device)), []).append(i)
+    per_call = max(1, 32 // len(fw))
+    for members in groups.values():
+        for start in range(0, len(members), per_call):
+            chunk = members[start:start + per_call]
+            out_chunk = _type5_batch([stencils[i] for i in chunk], [geoms[i] for i in chunk],
+                                     [smoothed[i] for i in chunk], [ems[i] for i in chunk], fw,
+                                     out_dtype)
+            for k, i in enumerate(chunk):
+                out[i] = out_chunk[k]
+    return out
+
+
+def _type5_batch(stencils, geoms, smoothed, ems, fw, out_dtype) -> torch.Tensor:
+    """csg_type5_sheets_batch over sides of one topology and sheet shape."""
+
+    st0 = stencils[0]
+    topo = st0.topology
+    sides, nfw = len(stencils), len(fw)
+    n = int(geoms[0].shape[0])
+    m = st0.nx * st0.ny
+    g = torch.stack(geoms).contiguous()
+    sm = torch.stack(smoothed).contiguous()
+    params = [heat_parameters_from_edge_mean(em, f) for em in ems for f in fw]
+    sig = _lib.doubles([p[1] for p in params])
+    its = (_lib.ctypes.c_int * len(params))(*[int(p[0]) for p in params])
+    idx = (_lib.ctypes.c_void_p * sides)(*[_lib.ptr(st.sheet_indices) for st in stencils])
+    wts = (_lib.ctypes.c_void_p * sides)(*[_lib.ptr(st.sheet_weights) for st in stencils])
+    out = torch.empty((sides, nfw, st0.ny, st0.nx), dtype=out_dtype, device=st0.device)
+    ws = _lib.workspace(_lib.lib().csg_type5_sheets_batch_workspace_bytes(
+        n, topo.nnz, sides, nfw, m), st0.device)
+    _lib.check(_lib.lib().csg_type5_sheets_batch(
+        _lib.ptr(g), _lib.ptr(sm), n, sides, _lib.ptr(topo.faces), _lib.ptr(topo.nbr_off),
+        _lib.ptr(topo.nbr), topo.nnz, _lib.ptr(topo.inc_off), _lib.ptr(topo.inc), nfw, sig, its,
+        idx, wts, m, _lib.dtype_code(out_dtype), _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+        st0._stream()), "csg_type5_sheets_batch")
     return out
 
 

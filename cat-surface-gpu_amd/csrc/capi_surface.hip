@@ -318,6 +318,83 @@ int csg_type5_sheets(const float *geom, int n, const int *faces, const int *nbr_
                                            (double *)out, (size_t)m, s);
 }
 
+size_t csg_type5_sheets_batch_workspace_bytes(int n, int nnz, int sides, int nfw, int m)
+{
+    if (n < 1 || sides < 1 || nfw < 1 || m < 1)
+        return 0;
+    const int nb = sides * nfw;
+    Carve c{nullptr, 0};
+    c.take<float>((size_t)3 * n * sides);  // normals
+    c.take<double>((size_t)n * sides);     // base values
+    carve_heat(c, n, nnz, nb);
+    c.take<double>((size_t)nb * n);  // normalised values
+    c.take<double>((size_t)nb * m);  // raw sheets
+    c.take<double>((size_t)nb * 64 * 2 + 64);
+    return c.used;
+}
+
+// csg_type5_sheets for `sides` meshes of one topology with their fwhm-50
+// smoothed copies given (geom, smoothed [sides][n][3]); every side has nfw
+// rows (sigma / iterations [sides][nfw]) and its own sheet stencil (host arrays
+// of device pointers, m points each); out [sides][nfw][m].  All rows share one
+// value-smoothing loop; each row's values equal the single-mesh call's.
+int csg_type5_sheets_batch(const float *geom, const float *smoothed, int n, int sides,
+                           const int *faces, const int *nbr_off, const int *nbr, int nnz,
+                           const int *inc_off, const int *inc, int nfw, const double *sigma,
+                           const int *iterations, const int *const *sheet_idx,
+                           const double *const *sheet_w, int m, int out_dtype, void *out,
+                           void *ws, size_t ws_bytes, void *stream)
+{
+    if (n < 1 || sides < 1 || nfw < 1 || m < 1 || !geom || !smoothed || !sigma || !iterations ||
+        !sheet_idx || !sheet_w || !out)
+        return fail(kInvalidArgument, "invalid batched type-5 arguments");
+    if (out_dtype != 0 && out_dtype != 1)
+        return fail(kUnsupported, "type-5 sheets: dtype code %d", out_dtype);
+    if ((long long)sides * nfw > kMaxHeatBatch)
+        return fail(kInvalidArgument, "batched type-5: sides x fwhms must be <= %d, got %lld",
+                    kMaxHeatBatch, (long long)sides * nfw);
+    const int nb = sides * nfw;
+    int mesh_of[kMaxHeatBatch];
+    for (int b = 0; b < nb; ++b)
+        mesh_of[b] = b / nfw;
+    HeatBatch hb;
+    CSG_TRY(make_batch(nb, sigma, iterations, hb, mesh_of, sides));
+    if (ws_bytes < csg_type5_sheets_batch_workspace_bytes(n, nnz, sides, nfw, m))
+        return fail(kInvalidArgument, "batched type-5 workspace too small");
+    for (int k = 0; k < sides; ++k)
+        if (!sheet_idx[k] || !sheet_w[k])
+            return fail(kInvalidArgument, "batched type-5: side %d has no sheet stencil", k);
+    hipStream_t s = S(stream);
+    Carve c{(char *)ws, 0};
+    float *nr = c.take<float>((size_t)3 * n * sides);
+    double *base = c.take<double>((size_t)n * sides);
+    HeatWs h = carve_heat(c, n, nnz, nb);
+    double *normed = c.take<double>((size_t)nb * n);
+    double *sheets = c.take<double>((size_t)nb * m);
+    double *mm = c.take<double>((size_t)nb * 64 * 2 + 64);
+    for (int k = 0; k < sides; ++k) {
+        const size_t g = (size_t)3 * n * k;
+        CSG_TRY(launch_vertex_normals(smoothed + g, faces, inc_off, inc, n, nr + g, s));
+        CSG_TRY(launch_type5_base(geom + g, smoothed + g, nr + g, n, base + (size_t)n * k, s));
+        for (int f = 0; f < nfw; ++f)
+            CSG_HIP_CHECK(hipMemcpyAsync(h.buf[0] + (size_t)(k * nfw + f) * n,
+                                         base + (size_t)n * k, sizeof(double) * n,
+                                         hipMemcpyDeviceToDevice, s));
+    }
+    double *res = nullptr;
+    CSG_TRY(run_heat(geom, nbr_off, nbr, n, nnz, hb, 1, h, &res, s));
+    CSG_TRY(launch_minmax_normalise<double>(res, (size_t)n, nb, (size_t)n, 0.0, mm, normed,
+                                            (size_t)n, s));
+    for (int k = 0; k < sides; ++k)
+        CSG_TRY(launch_sheet_gather(normed + (size_t)k * nfw * n, (size_t)n, sheet_idx[k],
+                                    sheet_w[k], m, nfw, sheets + (size_t)k * nfw * m, s));
+    if (out_dtype == 0)
+        return launch_minmax_normalise<float>(sheets, (size_t)m, nb, (size_t)m, 0.0, mm,
+                                              (float *)out, (size_t)m, s);
+    return launch_minmax_normalise<double>(sheets, (size_t)m, nb, (size_t)m, 0.0, mm,
+                                           (double *)out, (size_t)m, s);
+}
+
 namespace {
 
 DepthArgs carve_depth(Carve &c, int batch, int n, int nf, int nnz, int width)

@@ -271,6 +271,21 @@ int csg_type5_sheets(const float *geom, int n, const int *faces, const int *nbr_
                      const int *iterations, const int *sheet_idx, const double *sheet_w, int m,
                      int out_dtype, void *out, void *ws, size_t ws_bytes, void *stream);
 
+/* csg_type5_sheets for `sides` meshes of one topology in shared launches, their fwhm-50
+ * smoothed copies given: geom and smoothed [sides][n][3]; nfw rows per side (sigma /
+ * iterations host arrays [sides][nfw], sides * nfw <= 32); sheet_idx / sheet_w host
+ * arrays of each side's device stencil (m points); out [sides][nfw][m].  One
+ * value-smoothing loop serves every row; each row equals the single-mesh call's.  Replaces
+ * the per-side curvature_type5_to_sheet_many calls of build_maps
+ * (surface_warp.py:427-448, surface_stencil.py:521-550). */
+size_t csg_type5_sheets_batch_workspace_bytes(int n, int nnz, int sides, int nfw, int m);
+int csg_type5_sheets_batch(const float *geom, const float *smoothed, int n, int sides,
+                           const int *faces, const int *nbr_off, const int *nbr, int nnz,
+                           const int *inc_off, const int *inc, int nfw, const double *sigma,
+                           const int *iterations, const int *const *sheet_idx,
+                           const double *const *sheet_w, int m, int out_dtype, void *out,
+                           void *ws, size_t ws_bytes, void *stream);
+
 /* Depth potential (L + alpha diag(area)) x = rhs by colour-ordered SOR for `batch`
  * meshes of one topology (pts [batch][n][3], x and relres [batch]).  The system is a
  * CSR with duplicate (row, col) entries merged (contrib_off / contrib list each merged

@@ -79,6 +79,42 @@ def test_normals_and_type5_sheets(small, cuda):
     np.testing.assert_allclose(_np(f32[0]), g["type5_sheet0"], atol=3e-6)
 
 
+def test_type5_sheets_batch_equals_per_side(small, cuda):
+    """Sides of one topology with their own geometry, smoothing counts and sheet
+    stencils in one csg_type5_sheets_batch call equal per-side csg_type5_sheets."""
+
+    import dataclasses
+
+    from cat_surface_gpu.surface_stencil import (edge_means, heat_parameters_from_edge_mean,
+                                                 smooth_geometry_many, type5_sheets_many)
+
+    g, dev = small
+    surf = torch.as_tensor(g["surface"], device=cuda)
+    flipped = dataclasses.replace(
+        dev, sheet_indices=torch.flip(dev.sheet_indices, (0,)).contiguous(),
+        sheet_weights=torch.flip(dev.sheet_weights, (0,)).contiguous())
+    assert flipped.topology is dev.topology
+    gen = torch.Generator(device="cpu").manual_seed(7)
+    noise = (torch.rand(surf.shape, generator=gen) - 0.5).to(cuda) * 0.05
+    sides = [dev, flipped, dev, flipped]
+    geoms = [surf, surf * 1.25, surf + noise, surf * 0.8 - noise]
+    fw = (30.0, 12.0)  # 16 and 3 heat steps on the unscaled surface
+    ems = edge_means(sides, geoms)
+    gis = [heat_parameters_from_edge_mean(em, 50.0)[0] for em in ems]
+    assert len(set(heat_parameters_from_edge_mean(em, fw[0])[0] for em in ems)) > 1
+    smoothed = smooth_geometry_many(sides, geoms, gis)
+    for dt in (torch.float64, torch.float32):
+        batched = type5_sheets_many(sides, geoms, fw, dt)
+        for st, geo, em, gi, sm, got in zip(sides, geoms, ems, gis, smoothed, batched):
+            params = [heat_parameters_from_edge_mean(em, f) for f in fw]
+            single = st._type5(geo, gi, params, dt, smoothed=sm.contiguous())
+            assert got.dtype == dt and torch.equal(got, single)
+    # more rows than one loop holds: split into calls of at most 32 rows
+    many = type5_sheets_many([dev] * 17, [surf] * 17, fw)
+    for got in many:
+        assert torch.equal(got, many[0])
+
+
 def test_rotation_feature_matches_golden(golden, cuda):
     import cat_surface_gpu as csg
     from cat_surface_gpu.rotation_feature import compute_rotation_feature
