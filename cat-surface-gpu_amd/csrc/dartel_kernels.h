@@ -75,6 +75,12 @@ inline MembraneWeights membrane_weights(const double *p)
 template <typename T>
 int launch_resize(const T *in, int channels, int snx, int sny, int tnx, int tny, int batch,
                   size_t istride, size_t ostride, T *out, int accumulate, hipStream_t s);
+// Two resizes of one source and target grid (each with its own planes and
+// strides) in one launch; values equal two launch_resize calls.
+template <typename T>
+int launch_resize_two(const T *in0, int channels0, size_t istride0, size_t ostride0, T *out0,
+                      const T *in1, int channels1, size_t istride1, size_t ostride1, T *out1,
+                      int snx, int sny, int tnx, int tny, int batch, hipStream_t s);
 template <typename T>
 int launch_relax_sweep(const T *a, const T *b, T *u, MembraneWeights w, int nx, int ny,
                        int batch, size_t astride, size_t bstride, int color, hipStream_t s);

@@ -329,10 +329,9 @@ int fmg_run(const T *hes, const T *rhs, const double *params, int cycles, int ni
     const int top = c0 >= 0 ? c0 : plan.L - 1;
     for (int l = 1; l <= top; ++l) {
         size_t P = plan.P(l - 1), Pc = plan.P(l);
-        CSG_TRY(launch_resize<T>(plan.A[l - 1], 3, plan.nx[l - 1], plan.ny[l - 1], plan.nx[l],
-                                 plan.ny[l], batch, 3 * P, 3 * Pc, plan.A[l], 0, s));
-        CSG_TRY(launch_resize<T>(plan.B0[l - 1], 2, plan.nx[l - 1], plan.ny[l - 1], plan.nx[l],
-                                 plan.ny[l], batch, 2 * P, 2 * Pc, plan.B0[l], 0, s));
+        CSG_TRY(launch_resize_two<T>(plan.A[l - 1], 3, 3 * P, 3 * Pc, plan.A[l], plan.B0[l - 1],
+                                     2, 2 * P, 2 * Pc, plan.B0[l], plan.nx[l - 1],
+                                     plan.ny[l - 1], plan.nx[l], plan.ny[l], batch, s));
     }
     if (initial) {
         if (initial != out)

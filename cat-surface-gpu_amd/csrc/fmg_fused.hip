@@ -178,14 +178,14 @@ k_restrict_residual(const T *__restrict__ A, const T *__restrict__ B, const T *_
     Uc += job * cstride;
     const int cy = q / cnx, cx = q - cy * cnx;
     const Taps<T> tx = resize_taps<T>(cx, nx, cnx), ty = resize_taps<T>(cy, ny, cny);
-    auto res = [&](int x, int y, int ch) {
+    const Pair<T> r = resize_pair_at<T>(tx, ty, nx, ny, [&](int x, int y) {
         const int p = y * nx + x;
         const Pair<T> m = membrane_pixel<T>(stencil_at<T>(U, P, nx, ny, x, y), A[p], A[P + p],
                                             A[2 * P + p], w00, w01, w10);
-        return ch ? B[P + p] - m.y : B[p] - m.x;
-    };
-    Bc[q] = resize_at<T>(tx, ty, nx, ny, [&](int x, int y) { return res(x, y, 0); });
-    Bc[CP + q] = resize_at<T>(tx, ty, nx, ny, [&](int x, int y) { return res(x, y, 1); });
+        return Pair<T>{B[p] - m.x, B[P + p] - m.y};
+    });
+    Bc[q] = r.x;
+    Bc[CP + q] = r.y;
     Uc[q] = (T)0;
     Uc[CP + q] = (T)0;
 }

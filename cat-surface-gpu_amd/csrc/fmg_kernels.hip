@@ -36,6 +36,37 @@ k_resize(const T *__restrict__ in, int channels, int snx, int sny, int tnx, int 
     }
 }
 
+// Two k_resize problems of one grid geometry in one launch (blockIdx.z picks
+// the problem): the level pyramids of A and of the right-hand side.
+template <typename T>
+struct ResizeTwo {
+    const T *in[2];
+    T *out[2];
+    int channels[2];
+    size_t istride[2], ostride[2];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_resize_two(ResizeTwo<T> r, int snx, int sny, int tnx, int tny)
+{
+    const int TP = tnx * tny;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= TP)
+        return;
+    const int z = blockIdx.z;
+    const T *in = r.in[z] + blockIdx.y * r.istride[z];
+    T *out = r.out[z] + blockIdx.y * r.ostride[z];
+    int tx = p % tnx, ty = p / tnx;
+    const Taps<T> txp = resize_taps<T>(tx, snx, tnx), typ = resize_taps<T>(ty, sny, tny);
+    const int SP = snx * sny;
+    for (int c = 0; c < r.channels[z]; ++c) {
+        const T *f = in + (size_t)c * SP;
+        out[(size_t)c * TP + p] =
+            resize_at<T>(txp, typ, snx, sny, [&](int x, int y) { return f[y * snx + x]; });
+    }
+}
+
 // One colour of the red-black sweep, in place.  Pixels with (x + y) & 1 ==
 // color are updated from their four neighbours (opposite colour for even nx,
 // or the pixel itself through the boundaries).  dartel_triton.py:241-291
@@ -173,6 +204,19 @@ int launch_resize(const T *in, int channels, int snx, int sny, int tnx, int tny,
 }
 
 template <typename T>
+int launch_resize_two(const T *in0, int channels0, size_t istride0, size_t ostride0, T *out0,
+                      const T *in1, int channels1, size_t istride1, size_t ostride1, T *out1,
+                      int snx, int sny, int tnx, int tny, int batch, hipStream_t s)
+{
+    ResizeTwo<T> r{{in0, in1}, {out0, out1}, {channels0, channels1}, {istride0, istride1},
+                   {ostride0, ostride1}};
+    dim3 g(grid_for((size_t)tnx * tny), batch, 2);
+    hipLaunchKernelGGL(k_resize_two<T>, g, dim3(kBlock), 0, s, r, snx, sny, tnx, tny);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+template <typename T>
 int launch_relax_sweep(const T *a, const T *b, T *u, MembraneWeights w, int nx, int ny,
                        int batch, size_t astride, size_t bstride, int color, hipStream_t s)
 {
@@ -245,6 +289,9 @@ int launch_any_nonzero(const T *u, size_t n, int batch, size_t stride, int *flag
 #define CSG_INSTANTIATE_FMG(T)                                                                 \
     template int launch_resize<T>(const T *, int, int, int, int, int, int, size_t, size_t, T *, \
                                   int, hipStream_t);                                           \
+    template int launch_resize_two<T>(const T *, int, size_t, size_t, T *, const T *, int,     \
+                                      size_t, size_t, T *, int, int, int, int, int,            \
+                                      hipStream_t);                                            \
     template int launch_relax_sweep<T>(const T *, const T *, T *, MembraneWeights, int, int,   \
                                        int, size_t, size_t, int, hipStream_t);                 \
     template int launch_relax<T>(const T *, const T *, T *, MembraneWeights, int, int, int,    \

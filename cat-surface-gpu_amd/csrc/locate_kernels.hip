@@ -142,11 +142,14 @@ int grid_index_count(const P *pts, const int *faces, int nf, int g, int margin, 
 
 size_t grid_sort_scratch_bytes(int npairs)
 {
+    // size queries only (no device work); a failed query leaves the size at 0
+    // and the sort itself reports the error
     size_t need = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, need, (int *)nullptr, (int *)nullptr,
-                                       (int *)nullptr, (int *)nullptr, npairs, 0, 32);
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, need, (int *)nullptr, (int *)nullptr,
+                                             (int *)nullptr, (int *)nullptr, npairs, 0, 32);
     size_t scan = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (int *)nullptr, (int *)nullptr, 1 << 22);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (int *)nullptr, (int *)nullptr,
+                                           1 << 22);
     return (need > scan ? need : scan) + 1024;
 }
 
