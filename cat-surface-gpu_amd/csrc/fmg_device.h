@@ -114,26 +114,6 @@ struct Pair {
     T x, y;
 };
 
-// resize_at of both channels of a two-channel source, f(x, y) returning the
-// pair at one source pixel: each source pair is evaluated once, and each
-// channel is summed in resize_from's order.
-template <typename T, typename F>
-__device__ __forceinline__ Pair<T> resize_pair_at(const Taps<T> &tx, const Taps<T> &ty, int snx,
-                                                  int sny, F f)
-{
-    const int xs[3] = {wrap_x(tx.o - 1, snx), wrap_x(tx.o, snx), wrap_x(tx.o + 1, snx)};
-    const int y0 = reflect_y(ty.o - 1, sny), y1 = reflect_y(ty.o, sny),
-              y2 = reflect_y(ty.o + 1, sny);
-    Pair<T> col[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const Pair<T> a = f(xs[k], y0), b = f(xs[k], y1), c = f(xs[k], y2);
-        col[k] = {ty.wm * a.x + ty.w0 * b.x + ty.wp * c.x, ty.wm * a.y + ty.w0 * b.y + ty.wp * c.y};
-    }
-    return {tx.wm * col[0].x + tx.w0 * col[1].x + tx.wp * col[2].x,
-            tx.wm * col[0].y + tx.w0 * col[1].y + tx.wp * col[2].y};
-}
-
 // Membrane stencil values around one pixel, per channel: u at the pixel and
 // the sums of its two y and two x neighbours.
 template <typename T>

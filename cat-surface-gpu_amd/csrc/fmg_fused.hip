@@ -160,6 +160,12 @@ k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restri
 
 // Residual of the finer level restricted to the coarser one; the coarser
 // level's correction starts from zero.  dartel_grid.py:1356-1368
+//
+// Four lanes per coarse pixel: lane k < 3 evaluates the three residual pairs
+// of x tap k and their y-weighted column sum, and lane 0 combines the three
+// columns (resize_from's order per channel, so the values are the same).  Splitting the
+// 9 stencil evaluations over lanes gives each thread a third of the loads and
+// four times as many waves to hide their latency.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_restrict_residual(const T *__restrict__ A, const T *__restrict__ B, const T *__restrict__ U,
@@ -167,25 +173,36 @@ k_restrict_residual(const T *__restrict__ A, const T *__restrict__ B, const T *_
                     size_t bstride, size_t cstride, T *__restrict__ Bc, T *__restrict__ Uc)
 {
     const int CP = cnx * cny, P = nx * ny;
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= CP)
-        return;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = gid >> 2, k = gid & 3;
+    const bool live = q < CP;
     const int job = blockIdx.y;
     A += job * astride;
     B += job * bstride;
     U += job * bstride;
     Bc += job * cstride;
     Uc += job * cstride;
-    const int cy = q / cnx, cx = q - cy * cnx;
+    const int cy = live ? q / cnx : 0, cx = live ? q - cy * cnx : 0;
     const Taps<T> tx = resize_taps<T>(cx, nx, cnx), ty = resize_taps<T>(cy, ny, cny);
-    const Pair<T> r = resize_pair_at<T>(tx, ty, nx, ny, [&](int x, int y) {
-        const int p = y * nx + x;
-        const Pair<T> m = membrane_pixel<T>(stencil_at<T>(U, P, nx, ny, x, y), A[p], A[P + p],
-                                            A[2 * P + p], w00, w01, w10);
-        return Pair<T>{B[p] - m.x, B[P + p] - m.y};
-    });
-    Bc[q] = r.x;
-    Bc[CP + q] = r.y;
+    Pair<T> col{(T)0, (T)0};
+    if (live && k < 3) {
+        auto res = [&](int x, int y) {
+            const int p = y * nx + x;
+            const Pair<T> m = membrane_pixel<T>(stencil_at<T>(U, P, nx, ny, x, y), A[p],
+                                                A[P + p], A[2 * P + p], w00, w01, w10);
+            return Pair<T>{B[p] - m.x, B[P + p] - m.y};
+        };
+        const int x = wrap_x(tx.o - 1 + k, nx);
+        const Pair<T> a = res(x, reflect_y(ty.o - 1, ny)), b = res(x, reflect_y(ty.o, ny)),
+                      c = res(x, reflect_y(ty.o + 1, ny));
+        col = {ty.wm * a.x + ty.w0 * b.x + ty.wp * c.x, ty.wm * a.y + ty.w0 * b.y + ty.wp * c.y};
+    }
+    const T c1x = __shfl_down(col.x, 1, 4), c1y = __shfl_down(col.y, 1, 4);
+    const T c2x = __shfl_down(col.x, 2, 4), c2y = __shfl_down(col.y, 2, 4);
+    if (!live || k != 0)
+        return;
+    Bc[q] = tx.wm * col.x + tx.w0 * c1x + tx.wp * c2x;
+    Bc[CP + q] = tx.wm * col.y + tx.w0 * c1y + tx.wp * c2y;
     Uc[q] = (T)0;
     Uc[CP + q] = (T)0;
 }
@@ -218,16 +235,32 @@ struct LevelTable {
 
 enum CoarseOp : int { kRelax, kDown, kUpAdd, kProlong, kPyramidA, kPyramidB };
 
+constexpr int kWaveSize = 64;
+
+// End of one coarse step: a workgroup barrier, or for steps run by one wave a
+// fence that keeps the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void coarse_sync(int wave)
+{
+    if (wave) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // LDS resize of `channels` planes from resolved taps: out = resize(in) [+ out].
 // Channels are spread over threads as well as pixels.
 template <typename T>
 __device__ __forceinline__ void lds_resize(const T *in, int snx, int sny, T *out, int tnx,
                                            int tny, int channels, bool accumulate,
-                                           const TapIdx<T> *tabx, const TapIdx<T> *taby)
+                                           const TapIdx<T> *tabx, const TapIdx<T> *taby,
+                                           int stride)
 {
     const int SP = snx * sny, TP = tnx * tny;
     const float inv_tp = 1.0f / (float)TP, inv_tnx = 1.0f / (float)tnx;
-    for (int q = threadIdx.x; q < channels * TP; q += kCoarseThreads) {
+    for (int q = threadIdx.x; q < channels * TP; q += stride) {
         const DivMod cp = divmod_small(q, TP, inv_tp);
         const DivMod yx = divmod_small(cp.r, tnx, inv_tnx);
         const T *f = in + cp.q * SP;
@@ -240,12 +273,15 @@ __device__ __forceinline__ void lds_resize(const T *in, int snx, int sny, T *out
 
 // One step of the schedule on level (nx, ny) at LDS offset `off`; the other
 // level (onx, ony, ooff) is l+1 for kDown / kUpAdd / kProlong and l-1 for the
-// pyramid steps.  Ends with a workgroup barrier.
+// pyramid steps.  Ends with a workgroup barrier, or with wave = 1 (the call
+// made by the first wave alone) with a wave-level fence: LDS operations of one
+// wave complete in issue order, so the wave needs no workgroup barrier.
 template <typename T>
 __device__ __noinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w00, T w01, T w10,
                                        int onx, int ony, unsigned ooff, unsigned roff,
-                                       unsigned tx_off, unsigned ty_off, int nit)
+                                       unsigned tx_off, unsigned ty_off, int nit, int wave)
 {
+    const int stride = wave ? kWaveSize : kCoarseThreads;
     T *base = reinterpret_cast<T *>(g_lds);
     const TapIdx<T> *tabx = reinterpret_cast<const TapIdx<T> *>(g_lds) + tx_off;
     const TapIdx<T> *taby = reinterpret_cast<const TapIdx<T> *>(g_lds) + ty_off;
@@ -261,7 +297,7 @@ __device__ __noinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w
             const int colour = (it % 2 == 0) ? 1 : 0;
             for (int phase = 0; phase < (odd ? 2 : 1); ++phase) {
                 const int count = phase == 0 ? half * ny : ny;
-                for (int q = threadIdx.x; q < count; q += kCoarseThreads) {
+                for (int q = threadIdx.x; q < count; q += stride) {
                     int x, y;
                     if (phase == 0) {
                         const DivMod yk = divmod_small(q, half, inv_half);
@@ -282,14 +318,14 @@ __device__ __noinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w
                     u[p] = v.x;
                     u[P + p] = v.y;
                 }
-                __syncthreads();
+                coarse_sync(wave);
             }
         }
     } else if (op == kDown) {
         // r = b - (L + A) u; b[l+1] = restrict(r); u[l+1] = 0.  dartel_grid.py:1327-1368
         T *r = base + roff;
         const float inv_nx = 1.0f / (float)nx;
-        for (int p = threadIdx.x; p < P; p += kCoarseThreads) {
+        for (int p = threadIdx.x; p < P; p += stride) {
             const DivMod yx = divmod_small(p, nx, inv_nx);
             const int y = yx.q, x = yx.r;
             const Pair<T> m = membrane_pixel<T>(stencil_at<T>(u, P, nx, ny, x, y), a[p],
@@ -297,27 +333,28 @@ __device__ __noinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w
             r[p] = b[p] - m.x;
             r[P + p] = b[P + p] - m.y;
         }
-        __syncthreads();
-        lds_resize<T>(r, nx, ny, ob, onx, ony, 2, false, tabx, taby);
-        for (int p = threadIdx.x; p < 2 * OP; p += kCoarseThreads)
+        coarse_sync(wave);
+        lds_resize<T>(r, nx, ny, ob, onx, ony, 2, false, tabx, taby, stride);
+        for (int p = threadIdx.x; p < 2 * OP; p += stride)
             ou[p] = (T)0;
-        __syncthreads();
+        coarse_sync(wave);
     } else if (op == kUpAdd || op == kProlong) {
-        lds_resize<T>(ou, onx, ony, u, nx, ny, 2, op == kUpAdd, tabx, taby);
-        __syncthreads();
+        lds_resize<T>(ou, onx, ony, u, nx, ny, 2, op == kUpAdd, tabx, taby, stride);
+        coarse_sync(wave);
     } else {
         // pyramid: A (or the original b) of this level from the finer one
         if (op == kPyramidA)
-            lds_resize<T>(oa, onx, ony, a, nx, ny, 3, false, tabx, taby);
+            lds_resize<T>(oa, onx, ony, a, nx, ny, 3, false, tabx, taby, stride);
         else
-            lds_resize<T>(ob, onx, ony, b, nx, ny, 2, false, tabx, taby);
-        __syncthreads();
+            lds_resize<T>(ob, onx, ony, b, nx, ny, 2, false, tabx, taby, stride);
+        coarse_sync(wave);
     }
 }
 
 // coarse_op at level l of the table (the table is in LDS, the call takes scalars).
 template <typename T>
-__device__ __forceinline__ void run_op(int op, int l, const LevelTable<T> &t, int nit)
+__device__ __forceinline__ void run_op(int op, int l, const LevelTable<T> &t, int nit,
+                                       int wave = 0)
 {
     // the other level: none for a relaxation (pass l itself), l - 1 for the
     // pyramid steps, l + 1 otherwise
@@ -337,7 +374,41 @@ __device__ __forceinline__ void run_op(int op, int l, const LevelTable<T> &t, in
         ty = t.down_y[l - 1];
     }
     coarse_op<T>(op, t.nx[l], t.ny[l], t.off[l], t.w00[l], t.w01[l], t.w10[l], t.nx[o], t.ny[o],
-                 t.off[o], t.roff, tx, ty, nit);
+                 t.off[o], t.roff, tx, ty, nit, wave);
+}
+
+// One multigrid cycle from `level` (dartel_grid.py:1341-1391).  Levels from
+// wave_from on (at most kCoarseWavePixels pixels) run as one sub-cycle by the
+// first wave alone, followed by a workgroup barrier.
+template <typename T>
+__device__ __forceinline__ void coarse_cycle(int level, int last, int wave_from,
+                                             const LevelTable<T> &t, int nit)
+{
+    const int w = wave_from > level ? wave_from : level;
+    for (int l = level; l < last && l < w; ++l) {
+        run_op<T>(kRelax, l, t, nit);
+        run_op<T>(kDown, l, t, nit);
+    }
+    if (w <= last) {
+        if (threadIdx.x < kWaveSize) {
+            for (int l = w; l < last; ++l) {
+                run_op<T>(kRelax, l, t, nit, 1);
+                run_op<T>(kDown, l, t, nit, 1);
+            }
+            run_op<T>(kRelax, last, t, nit, 1);
+            for (int l = last - 1; l >= w; --l) {
+                run_op<T>(kUpAdd, l, t, nit, 1);
+                run_op<T>(kRelax, l, t, nit, 1);
+            }
+        }
+        __syncthreads();
+    } else {
+        run_op<T>(kRelax, last, t, nit);
+    }
+    for (int l = (w <= last ? w : last) - 1; l >= level; --l) {
+        run_op<T>(kUpAdd, l, t, nit);
+        run_op<T>(kRelax, l, t, nit);
+    }
 }
 
 template <typename T>
@@ -447,31 +518,12 @@ k_fmg_coarse(CoarseLevels<T> lv, const T *__restrict__ A, const T *__restrict__ 
         // been overwritten by then)
         for (int level = last - 1; level >= c0; --level) {
             run_op<T>(kProlong, level, tab, nit);
-            for (int cyc = 0; cyc < cycles; ++cyc) {
-                // cycle(level), dartel_grid.py:1341-1391
-                for (int l = level; l < last; ++l) {
-                    run_op<T>(kRelax, l, tab, nit);
-                    run_op<T>(kDown, l, tab, nit);
-                }
-                run_op<T>(kRelax, last, tab, nit);
-                for (int l = last - 1; l >= level; --l) {
-                    run_op<T>(kUpAdd, l, tab, nit);
-                    run_op<T>(kRelax, l, tab, nit);
-                }
-            }
+            for (int cyc = 0; cyc < cycles; ++cyc)
+                coarse_cycle<T>(level, last, lv.wave_from, tab, nit);
         }
     } else {
-        for (int cyc = 0; cyc < cycles; ++cyc) {
-            for (int l = c0; l < last; ++l) {
-                run_op<T>(kRelax, l, tab, nit);
-                run_op<T>(kDown, l, tab, nit);
-            }
-            run_op<T>(kRelax, last, tab, nit);
-            for (int l = last - 1; l >= c0; --l) {
-                run_op<T>(kUpAdd, l, tab, nit);
-                run_op<T>(kRelax, l, tab, nit);
-            }
-        }
+        for (int cyc = 0; cyc < cycles; ++cyc)
+            coarse_cycle<T>(c0, last, lv.wave_from, tab, nit);
     }
     for (int p = threadIdx.x; p < P0; p += kCoarseThreads) {
         U[p] = tu[p];
@@ -668,7 +720,7 @@ int launch_restrict_residual(const T *a, const T *b, const T *u, MembraneWeights
                              int ny, int cnx, int cny, int batch, size_t astride, size_t bstride,
                              size_t cstride, T *bc, T *uc, hipStream_t s)
 {
-    dim3 g(grid_for((size_t)cnx * cny), batch);
+    dim3 g(grid_for((size_t)cnx * cny * 4), batch);
     // reference bytes: apply_membrane (A, u in, R out: 7), b (2), restriction
     // (R in: 2, b_c out: 0.5) and the zeroed u_c (0.5) per fine pixel
     TimedLaunch timer(kTimedRestrict, s, 12.0 * nx * ny * sizeof(T) * batch);
