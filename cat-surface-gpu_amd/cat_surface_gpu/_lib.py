@@ -90,7 +90,6 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_type5_workspace_bytes": (_sz, [_i, _i, _i, _i]),
     "csg_type5_sheets": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _vp, _i, _dp,
                               ctypes.POINTER(_i), _vp, _vp, _i, _i, _vp, _vp, _sz, _vp]),
-    "csg_set_coarse_wave_pixels": (_i, [_i]),
     "csg_type5_sheets_batch_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
     "csg_type5_sheets_batch": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _dp,
                                     ctypes.POINTER(_i), ctypes.POINTER(_vp),

@@ -88,11 +88,6 @@ int csg_set_fmg_schedule(int fused)
     return set_fused_fmg_schedule(fused);
 }
 
-int csg_set_coarse_wave_pixels(int pixels)
-{
-    return set_coarse_wave_pixels(pixels);
-}
-
 int csg_set_relax_tile(int nx, int ny, int dtype, int tile_x, int tile_y)
 {
     if (dtype != 0 && dtype != 1)

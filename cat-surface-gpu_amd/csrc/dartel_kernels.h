@@ -121,7 +121,6 @@ constexpr int kCoarseThreads = 1024;
 template <typename T>
 struct CoarseLevels {
     int L, c0;
-    int wave_from;  // first level whose cycles run on one wave (L: none)
     int nx[kFmgMaxLevels], ny[kFmgMaxLevels];
     T w00[kFmgMaxLevels], w01[kFmgMaxLevels], w10[kFmgMaxLevels];
 };

@@ -18,10 +18,6 @@ size_t fmg_workspace_bytes(int nx, int ny, int batch);
 // single-workgroup coarse hierarchy, where it applies), 0 = one launch per
 // operator call.  Both give identical values.  Returns the previous setting.
 int set_fused_fmg_schedule(int enable);
-// Coarse levels of at most `pixels` pixels run their multigrid cycles on a
-// single wave of the coarse kernel (0: none); identical values.  Returns the
-// previous setting.
-int set_coarse_wave_pixels(int pixels);
 // Levels of an nx x ny FMG that the fused schedule relaxes on LDS tiles (the
 // levels above its single-workgroup coarse hierarchy); 0 when the fused
 // schedule does not apply.

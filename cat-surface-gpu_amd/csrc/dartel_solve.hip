@@ -183,10 +183,6 @@ struct FmgRun {
 
 // The fused schedule: fine levels on LDS tiles, levels c0.. in one
 // workgroup per job (fmg_fused.hip).  Same operations, same order.
-// coarse levels of at most this many pixels run their multigrid cycles on one
-// wave of the coarse kernel (0: every level on the whole workgroup)
-std::atomic<int> g_coarse_wave_pixels{128};
-
 template <typename T>
 struct FusedFmg {
     FmgPlan<T> &p;
@@ -217,10 +213,6 @@ struct FusedFmg {
             lv.w01[l] = (T)w.w01;
             lv.w10[l] = (T)w.w10;
         }
-        const int wave_px = g_coarse_wave_pixels.load(std::memory_order_relaxed);
-        lv.wave_from = p.L;
-        for (int l = p.L - 1; l >= c0 && p.nx[l] * p.ny[l] <= wave_px; --l)
-            lv.wave_from = l;
         lds = coarse_lds_bytes<T>(p.nx, p.ny, c0, p.L);
     }
     // relax_membrane at level l, optionally after u[l] += prolong(u[l+1])
@@ -306,11 +298,6 @@ int fused_tiled_levels(int nx, int ny, int nit, int elem)
     const int c0 = elem == 8 ? fused_first_coarse_level<double>(pd, nit)
                              : fused_first_coarse_level<float>(pf, nit);
     return c0 < 0 ? 0 : c0;
-}
-
-int set_coarse_wave_pixels(int pixels)
-{
-    return g_coarse_wave_pixels.exchange(pixels < 0 ? 0 : pixels);
 }
 
 int set_fused_fmg_schedule(int enable)

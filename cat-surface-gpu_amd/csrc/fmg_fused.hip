@@ -235,32 +235,16 @@ struct LevelTable {
 
 enum CoarseOp : int { kRelax, kDown, kUpAdd, kProlong, kPyramidA, kPyramidB };
 
-constexpr int kWaveSize = 64;
-
-// End of one coarse step: a workgroup barrier, or for steps run by one wave a
-// fence that keeps the compiler from moving LDS accesses across it.
-__device__ __forceinline__ void coarse_sync(int wave)
-{
-    if (wave) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    } else {
-        __syncthreads();
-    }
-}
-
 // LDS resize of `channels` planes from resolved taps: out = resize(in) [+ out].
 // Channels are spread over threads as well as pixels.
 template <typename T>
 __device__ __forceinline__ void lds_resize(const T *in, int snx, int sny, T *out, int tnx,
                                            int tny, int channels, bool accumulate,
-                                           const TapIdx<T> *tabx, const TapIdx<T> *taby,
-                                           int stride)
+                                           const TapIdx<T> *tabx, const TapIdx<T> *taby)
 {
     const int SP = snx * sny, TP = tnx * tny;
     const float inv_tp = 1.0f / (float)TP, inv_tnx = 1.0f / (float)tnx;
-    for (int q = threadIdx.x; q < channels * TP; q += stride) {
+    for (int q = threadIdx.x; q < channels * TP; q += kCoarseThreads) {
         const DivMod cp = divmod_small(q, TP, inv_tp);
         const DivMod yx = divmod_small(cp.r, tnx, inv_tnx);
         const T *f = in + cp.q * SP;
@@ -273,15 +257,12 @@ __device__ __forceinline__ void lds_resize(const T *in, int snx, int sny, T *out
 
 // One step of the schedule on level (nx, ny) at LDS offset `off`; the other
 // level (onx, ony, ooff) is l+1 for kDown / kUpAdd / kProlong and l-1 for the
-// pyramid steps.  Ends with a workgroup barrier, or with wave = 1 (the call
-// made by the first wave alone) with a wave-level fence: LDS operations of one
-// wave complete in issue order, so the wave needs no workgroup barrier.
+// pyramid steps.  Ends with a workgroup barrier.
 template <typename T>
 __device__ __noinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w00, T w01, T w10,
                                        int onx, int ony, unsigned ooff, unsigned roff,
-                                       unsigned tx_off, unsigned ty_off, int nit, int wave)
+                                       unsigned tx_off, unsigned ty_off, int nit)
 {
-    const int stride = wave ? kWaveSize : kCoarseThreads;
     T *base = reinterpret_cast<T *>(g_lds);
     const TapIdx<T> *tabx = reinterpret_cast<const TapIdx<T> *>(g_lds) + tx_off;
     const TapIdx<T> *taby = reinterpret_cast<const TapIdx<T> *>(g_lds) + ty_off;
@@ -297,7 +278,7 @@ __device__ __noinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w
             const int colour = (it % 2 == 0) ? 1 : 0;
             for (int phase = 0; phase < (odd ? 2 : 1); ++phase) {
                 const int count = phase == 0 ? half * ny : ny;
-                for (int q = threadIdx.x; q < count; q += stride) {
+                for (int q = threadIdx.x; q < count; q += kCoarseThreads) {
                     int x, y;
                     if (phase == 0) {
                         const DivMod yk = divmod_small(q, half, inv_half);
@@ -318,14 +299,14 @@ __device__ __noinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w
                     u[p] = v.x;
                     u[P + p] = v.y;
                 }
-                coarse_sync(wave);
+                __syncthreads();
             }
         }
     } else if (op == kDown) {
         // r = b - (L + A) u; b[l+1] = restrict(r); u[l+1] = 0.  dartel_grid.py:1327-1368
         T *r = base + roff;
         const float inv_nx = 1.0f / (float)nx;
-        for (int p = threadIdx.x; p < P; p += stride) {
+        for (int p = threadIdx.x; p < P; p += kCoarseThreads) {
             const DivMod yx = divmod_small(p, nx, inv_nx);
             const int y = yx.q, x = yx.r;
             const Pair<T> m = membrane_pixel<T>(stencil_at<T>(u, P, nx, ny, x, y), a[p],
@@ -333,28 +314,27 @@ __device__ __noinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w
             r[p] = b[p] - m.x;
             r[P + p] = b[P + p] - m.y;
         }
-        coarse_sync(wave);
-        lds_resize<T>(r, nx, ny, ob, onx, ony, 2, false, tabx, taby, stride);
-        for (int p = threadIdx.x; p < 2 * OP; p += stride)
+        __syncthreads();
+        lds_resize<T>(r, nx, ny, ob, onx, ony, 2, false, tabx, taby);
+        for (int p = threadIdx.x; p < 2 * OP; p += kCoarseThreads)
             ou[p] = (T)0;
-        coarse_sync(wave);
+        __syncthreads();
     } else if (op == kUpAdd || op == kProlong) {
-        lds_resize<T>(ou, onx, ony, u, nx, ny, 2, op == kUpAdd, tabx, taby, stride);
-        coarse_sync(wave);
+        lds_resize<T>(ou, onx, ony, u, nx, ny, 2, op == kUpAdd, tabx, taby);
+        __syncthreads();
     } else {
         // pyramid: A (or the original b) of this level from the finer one
         if (op == kPyramidA)
-            lds_resize<T>(oa, onx, ony, a, nx, ny, 3, false, tabx, taby, stride);
+            lds_resize<T>(oa, onx, ony, a, nx, ny, 3, false, tabx, taby);
         else
-            lds_resize<T>(ob, onx, ony, b, nx, ny, 2, false, tabx, taby, stride);
-        coarse_sync(wave);
+            lds_resize<T>(ob, onx, ony, b, nx, ny, 2, false, tabx, taby);
+        __syncthreads();
     }
 }
 
 // coarse_op at level l of the table (the table is in LDS, the call takes scalars).
 template <typename T>
-__device__ __forceinline__ void run_op(int op, int l, const LevelTable<T> &t, int nit,
-                                       int wave = 0)
+__device__ __forceinline__ void run_op(int op, int l, const LevelTable<T> &t, int nit)
 {
     // the other level: none for a relaxation (pass l itself), l - 1 for the
     // pyramid steps, l + 1 otherwise
@@ -374,41 +354,7 @@ __device__ __forceinline__ void run_op(int op, int l, const LevelTable<T> &t, in
         ty = t.down_y[l - 1];
     }
     coarse_op<T>(op, t.nx[l], t.ny[l], t.off[l], t.w00[l], t.w01[l], t.w10[l], t.nx[o], t.ny[o],
-                 t.off[o], t.roff, tx, ty, nit, wave);
-}
-
-// One multigrid cycle from `level` (dartel_grid.py:1341-1391).  Levels from
-// wave_from on (at most kCoarseWavePixels pixels) run as one sub-cycle by the
-// first wave alone, followed by a workgroup barrier.
-template <typename T>
-__device__ __forceinline__ void coarse_cycle(int level, int last, int wave_from,
-                                             const LevelTable<T> &t, int nit)
-{
-    const int w = wave_from > level ? wave_from : level;
-    for (int l = level; l < last && l < w; ++l) {
-        run_op<T>(kRelax, l, t, nit);
-        run_op<T>(kDown, l, t, nit);
-    }
-    if (w <= last) {
-        if (threadIdx.x < kWaveSize) {
-            for (int l = w; l < last; ++l) {
-                run_op<T>(kRelax, l, t, nit, 1);
-                run_op<T>(kDown, l, t, nit, 1);
-            }
-            run_op<T>(kRelax, last, t, nit, 1);
-            for (int l = last - 1; l >= w; --l) {
-                run_op<T>(kUpAdd, l, t, nit, 1);
-                run_op<T>(kRelax, l, t, nit, 1);
-            }
-        }
-        __syncthreads();
-    } else {
-        run_op<T>(kRelax, last, t, nit);
-    }
-    for (int l = (w <= last ? w : last) - 1; l >= level; --l) {
-        run_op<T>(kUpAdd, l, t, nit);
-        run_op<T>(kRelax, l, t, nit);
-    }
+                 t.off[o], t.roff, tx, ty, nit);
 }
 
 template <typename T>
@@ -518,12 +464,31 @@ k_fmg_coarse(CoarseLevels<T> lv, const T *__restrict__ A, const T *__restrict__ 
         // been overwritten by then)
         for (int level = last - 1; level >= c0; --level) {
             run_op<T>(kProlong, level, tab, nit);
-            for (int cyc = 0; cyc < cycles; ++cyc)
-                coarse_cycle<T>(level, last, lv.wave_from, tab, nit);
+            for (int cyc = 0; cyc < cycles; ++cyc) {
+                // cycle(level), dartel_grid.py:1341-1391
+                for (int l = level; l < last; ++l) {
+                    run_op<T>(kRelax, l, tab, nit);
+                    run_op<T>(kDown, l, tab, nit);
+                }
+                run_op<T>(kRelax, last, tab, nit);
+                for (int l = last - 1; l >= level; --l) {
+                    run_op<T>(kUpAdd, l, tab, nit);
+                    run_op<T>(kRelax, l, tab, nit);
+                }
+            }
         }
     } else {
-        for (int cyc = 0; cyc < cycles; ++cyc)
-            coarse_cycle<T>(c0, last, lv.wave_from, tab, nit);
+        for (int cyc = 0; cyc < cycles; ++cyc) {
+            for (int l = c0; l < last; ++l) {
+                run_op<T>(kRelax, l, tab, nit);
+                run_op<T>(kDown, l, tab, nit);
+            }
+            run_op<T>(kRelax, last, tab, nit);
+            for (int l = last - 1; l >= c0; --l) {
+                run_op<T>(kUpAdd, l, tab, nit);
+                run_op<T>(kRelax, l, tab, nit);
+            }
+        }
     }
     for (int p = threadIdx.x; p < P0; p += kCoarseThreads) {
         U[p] = tu[p];

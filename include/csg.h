@@ -41,11 +41,6 @@ int csg_device_count(int *count);
  * in the same order.  Process-wide; returns the previous setting.  No counterpart in
  * the reference (it has only the per-operator schedule of dartel_grid.py:1211-1417). */
 int csg_set_fmg_schedule(int fused);
-/* The fused schedule's coarse kernel runs the multigrid cycles of levels of at most
- * `pixels` pixels (default 128) on a single wave without workgroup barriers; 0 keeps
- * every level on the whole workgroup.  Identical values either way.  Process-wide;
- * returns the previous setting.  No counterpart in the reference. */
-int csg_set_coarse_wave_pixels(int pixels);
 /* Tile interior (tile_x even, tile_y) of the fused schedule's fine-level relaxation for
  * levels of nx x ny in dtype; 0 x 0 restores the default 32 x 16.  Every shape computes
  * the same values; the shape only changes the speed (set by an autotuner).  Process-wide;

@@ -69,31 +69,6 @@ def test_fused_fmg_is_bit_identical(csg, cuda, dtype, shape, mode):
     assert torch.equal(ref, fused)
 
 
-@pytest.mark.parametrize("wave_pixels", [0, 32, 1 << 20])
-@pytest.mark.parametrize("shape", [(512, 256), (130, 66), (9, 7), (6, 1)])
-@pytest.mark.parametrize("mode", ["coarse_init", "initial"])
-def test_coarse_wave_levels_do_not_change_values(csg, cuda, wave_pixels, shape, mode):
-    """Coarse levels cycled by one wave (none, the small ones, all of them) give
-    the per-operator schedule's bits."""
-
-    from cat_surface_gpu import _lib
-
-    nx, ny = shape
-    print(f"coarse wave check {shape} {wave_pixels} {mode}", flush=True)
-    rng = np.random.default_rng(nx * 31 + ny)
-    a, b, u0 = _system(nx, ny, rng, cuda, torch.float32)
-    kwargs = dict(cycles=3, nit=3, device=cuda, dtype=torch.float32)
-    if mode == "initial":
-        kwargs["initial"] = u0
-    params = (1.0, 1.0, 0.125, 0.0, 1e-3)
-    previous = _lib.lib().csg_set_coarse_wave_pixels(wave_pixels)
-    try:
-        ref, fused = _both(csg, lambda: csg.fmg2_membrane(a, b, params, **kwargs))
-    finally:
-        _lib.lib().csg_set_coarse_wave_pixels(previous)
-    assert torch.equal(ref, fused)
-
-
 @pytest.mark.parametrize("nit,cycles", [(1, 2), (2, 1), (0, 2)])
 def test_fused_fmg_other_sweep_counts(csg, cuda, nit, cycles):
     rng = np.random.default_rng(nit * 10 + cycles)
