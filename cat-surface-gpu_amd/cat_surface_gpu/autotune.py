@@ -30,7 +30,8 @@ import torch
 from . import _lib
 
 CANDIDATES: tuple[tuple[int, int], ...] = (
-    (32, 16), (64, 8), (16, 16), (32, 8), (64, 16), (16, 32), (128, 4), (32, 32), (16, 8))
+    (32, 16), (64, 8), (16, 16), (32, 8), (64, 16), (16, 32), (128, 4), (32, 32), (16, 8),
+    (64, 32), (128, 16), (128, 8))
 
 _TUNED: dict[tuple, dict[tuple[int, int], tuple[int, int]]] = {}
 _SOURCE: dict[tuple, str] = {}
