@@ -84,6 +84,9 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_smooth_values_workspace_bytes": (_sz, [_i, _i, _i]),
     "csg_smooth_values": (_i, [_vp, _i, _i, _vp, _i, ctypes.POINTER(_i), _vp, _vp, _i, _dp,
                                ctypes.POINTER(_i), _i, _vp, _vp, _sz, _vp]),
+    "csg_smooth_values_patched": (_i, [_vp, _i, _i, _vp, _i, ctypes.POINTER(_i), _vp, _vp, _i,
+                                       _dp, ctypes.POINTER(_i), _i, _i, _i, _vp, _vp, _vp, _vp,
+                                       _vp, _vp, _vp, _sz, _vp, _vp, _sz, _vp]),
     "csg_minmax_workspace_bytes": (_sz, [_i]),
     "csg_minmax_normalise": (_i, [_vp, _sz, _i, _d, _i, _vp, _vp, _sz, _vp]),
     "csg_sheet_gather": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _vp]),

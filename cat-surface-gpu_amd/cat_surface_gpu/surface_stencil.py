@@ -113,7 +113,8 @@ class Topology:
 
 @dataclass
 class SmoothingPatches:
-    """A partition of a mesh for csg_smooth_geometry_patched (device arrays)."""
+    """A partition of a mesh for csg_smooth_geometry_patched and
+    csg_smooth_values_patched (device arrays)."""
 
     count: int
     rings: int
@@ -126,6 +127,7 @@ class SmoothingPatches:
     perm: torch.Tensor  # working order: working index i holds vertex perm[i]
     lds_bytes: int
     largest: int  # local vertices of the largest patch
+    value_lds_bytes: int = 0  # LDS of csg_smooth_values_patched: 26 L + 10 E, largest patch
 
 
 def _morton_order(points: np.ndarray) -> np.ndarray:
@@ -249,6 +251,8 @@ def smoothing_patches(topology: Topology, positions, *, core: int = 256,
         raise ValueError("a smoothing patch has more than 65535 adjacency entries")
     lds = 24 * sizes + 2 * (sizes + 1) + 2 * edges
     lds_bytes = int((lds.max() + 15) // 16 * 16)
+    vlds = 26 * sizes + 10 * edges + 2
+    value_lds_bytes = int((vlds.max() + 15) // 16 * 16)
 
     def dev(a, dtype=torch.int32):
         return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=topology.nbr.device)
@@ -260,7 +264,7 @@ def smoothing_patches(topology: Topology, positions, *, core: int = 256,
         count=count, rings=int(rings), vstart=dev(vstart), ring_end=dev(ring_end.reshape(-1)),
         lverts=dev(rank[vert]), aoff=dev(aoff), astart=dev(astart), perm=dev(order),
         ladj=dev(ladj.astype(np.uint16).view(np.int16), torch.int16), lds_bytes=lds_bytes,
-        largest=int(sizes.max()))
+        largest=int(sizes.max()), value_lds_bytes=value_lds_bytes)
     topology.patch_plans[key] = plan
     return plan
 
@@ -650,6 +654,18 @@ class SurfaceStencilDevice:
         sig = _lib.doubles([p[1] for p in params])
         its = (_lib.ctypes.c_int * nb)(*[int(p[0]) for p in params])
         mesh = None if mesh_of is None else (_lib.ctypes.c_int * nb)(*[int(m) for m in mesh_of])
+        if _VALUE_PATCHES is not None and topo.nbr_off_np is not None:
+            pp = smoothing_patches(topo, self.sphere_points, core=_VALUE_PATCHES[0],
+                                   rings=_VALUE_PATCHES[1])
+            if pp.value_lds_bytes <= 160 * 1024:
+                self._call("csg_smooth_values_patched", _lib.ptr(vals), n, nb, _lib.ptr(geom),
+                           meshes, mesh, _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), topo.nnz,
+                           sig, its, int(squared_norm), pp.count, pp.rings, _lib.ptr(pp.perm),
+                           _lib.ptr(pp.vstart), _lib.ptr(pp.ring_end), _lib.ptr(pp.lverts),
+                           _lib.ptr(pp.aoff), _lib.ptr(pp.astart), _lib.ptr(pp.ladj),
+                           pp.value_lds_bytes, _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+                           self._stream())
+                return out
         self._call("csg_smooth_values", _lib.ptr(vals), n, nb, _lib.ptr(geom), meshes, mesh,
                    _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), topo.nnz, sig, its,
                    int(squared_norm), _lib.ptr(out), _lib.ptr(ws), ws.numel(), self._stream())
@@ -772,6 +788,26 @@ def edge_means(stencils, geometries) -> list[float]:
 # Patch shape (core vertices, rings = iterations per launch) of the batched
 # geometry smoothing; None: one launch per iteration.
 _GEOMETRY_PATCHES: tuple[int, int] | None = (320, 4)
+# value smoothing: (core, rings) patches iterated in LDS, or None for one launch
+# per iteration; identical values
+_VALUE_PATCHES: tuple[int, int] | None = (256, 4)
+
+
+def set_value_smoothing(patches: tuple[int, int] | None) -> tuple[int, int] | None:
+    """Select the value smoothing's schedule: (core, rings) patches iterated in
+    LDS (csg_smooth_values_patched), or None for one launch per iteration
+    (csg_smooth_values).  Both give identical values.  Returns the previous
+    setting."""
+
+    global _VALUE_PATCHES
+    previous = _VALUE_PATCHES
+    if patches is not None:
+        core, rings = (int(v) for v in patches)
+        if core < 1 or not 1 <= rings <= 64:
+            raise ValueError("patches must be (core >= 1, 1 <= rings <= 64)")
+        patches = (core, rings)
+    _VALUE_PATCHES = patches
+    return previous
 
 
 def set_geometry_smoothing(patches: tuple[int, int] | None) -> tuple[int, int] | None:

@@ -235,6 +235,49 @@ int csg_smooth_values(const double *values, int n, int nb, const float *geom, in
     return kOk;
 }
 
+int csg_smooth_values_patched(const double *values, int n, int nb, const float *geom, int meshes,
+                              const int *mesh_of, const int *nbr_off, const int *nbr, int nnz,
+                              const double *sigma, const int *iterations, int squared_norm,
+                              int npatches, int rings, const int *perm, const int *vstart,
+                              const int *ring_end, const int *lverts, const int *aoff,
+                              const int *astart, const unsigned short *ladj, size_t lds_bytes,
+                              double *out, void *ws, size_t ws_bytes, void *stream)
+{
+    HeatBatch hb;
+    CSG_TRY(make_batch(nb, sigma, iterations, hb, mesh_of, meshes < 1 ? 1 : meshes));
+    if (ws_bytes < csg_smooth_values_workspace_bytes(n, nnz, nb))
+        return fail(kInvalidArgument, "smooth_values workspace too small");
+    if (npatches < 1 || npatches > 65535 || rings < 1 || rings > 64 || !perm || !vstart ||
+        !ring_end || !lverts || !aoff || !astart || !ladj)
+        return fail(kInvalidArgument, "invalid smoothing patches");
+    if (lds_bytes < 16 || lds_bytes > 160 * 1024)
+        return fail(kInvalidArgument, "patch LDS size must be in [16 B, 160 KiB]");
+    hipStream_t s = S(stream);
+    Carve c{(char *)ws, 0};
+    HeatWs h = carve_heat(c, n, nnz, nb);
+    const SmoothPatches pp{npatches, rings, vstart, ring_end, lverts, aoff, astart, perm, ladj};
+    CSG_TRY(launch_heat_weights(geom, nbr_off, nbr, n, hb, squared_norm, (size_t)nnz, h.w,
+                                h.wsum, s));
+    int most = 0;
+    for (int i = 0; i < hb.nb; ++i)
+        most = hb.counts[i] > most ? hb.counts[i] : most;
+    // each launch advances every row by up to `rings` iterations and writes all
+    // of its vertices (the patch cores partition the mesh)
+    const double *cur = values;
+    int which = 0;
+    for (int it0 = 0; it0 < most; it0 += rings) {
+        double *nxt = (it0 + rings >= most) ? out : h.buf[which];
+        CSG_TRY(launch_smooth_values_patched(cur, nbr_off, h.w, n, (size_t)nnz, hb, pp, it0,
+                                             lds_bytes, nxt, s));
+        cur = nxt;
+        which ^= 1;
+    }
+    if (most == 0)
+        CSG_HIP_CHECK(hipMemcpyAsync(out, values, sizeof(double) * n * nb,
+                                     hipMemcpyDeviceToDevice, s));
+    return kOk;
+}
+
 size_t csg_minmax_workspace_bytes(int nb) { return (size_t)nb * 64 * 2 * 8 + 1024; }
 
 int csg_minmax_normalise(const double *x, size_t n, int nb, double floor_, int out_dtype,

@@ -36,6 +36,11 @@ struct SmoothPatches {
     const int *vstart, *ring_end, *lverts, *aoff, *astart, *perm;
     const unsigned short *ladj;
 };
+// Iterations it0 .. it0 + rings - 1 of launch_smooth_values on mesh patches
+// (lds_bytes: 26 L + 10 E of the largest patch, rounded up).
+int launch_smooth_values_patched(const double *cur, const int *goff, const double *w, int n,
+                                 size_t wstride, const HeatBatch &hb, const SmoothPatches &pp,
+                                 int it0, size_t lds_bytes, double *nxt, hipStream_t s);
 int launch_smooth_geometry_patched(const float *v, float4 *buf0, float4 *buf1, int n,
                                    const SmoothPatches &pp, const SmoothBatch &sb, int most,
                                    size_t lds_bytes, float *out, hipStream_t s);

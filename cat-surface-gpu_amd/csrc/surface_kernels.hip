@@ -249,6 +249,107 @@ k_smooth_geometry_patch(const float4 *__restrict__ in, float4 *__restrict__ out,
         dst[lv[j]] = make_float4(fx[j], fy[j], fz[j], 0.0f);
 }
 
+// Up to `rings` iterations of k_smooth_values in one launch, one workgroup per
+// (patch, batch row), on the patches of k_smooth_geometry_patch (same ring
+// schedule).  Rows stay in vertex order (perm maps a patch entry to its
+// vertex); each updated vertex's weights are staged in CSR order with their
+// sum, so every iterate equals the per-iteration kernel's.
+__global__ void __launch_bounds__(kPatchThreads)
+k_smooth_values_patch(const double *__restrict__ in, double *__restrict__ out, int n,
+                      const int *__restrict__ goff, const double *__restrict__ w,
+                      size_t wstride, SmoothPatches pp, HeatBatch hb, int it0)
+{
+    extern __shared__ double g_value_lds[];
+    const int p = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+    const int s0 = pp.vstart[p], L = pp.vstart[p + 1] - s0;
+    const int a0 = pp.astart[p], E = pp.astart[p + 1] - a0;
+    const int *lv = pp.lverts + s0;
+    const int *ring_end = pp.ring_end + p * (pp.rings + 1);
+    const double *src = in + (size_t)b * n;
+    double *dst = out + (size_t)b * n;
+    int m = hb.counts[b] - it0;
+    m = m < 0 ? 0 : (m > pp.rings ? pp.rings : m);
+    const int core = ring_end[0];
+    if (m == 0) {
+        for (int j = tid; j < core; j += kPatchThreads) {
+            const int g = pp.perm[lv[j]];
+            dst[g] = src[g];
+        }
+        return;
+    }
+    // LDS: two value buffers (L), the weights (E) and weight sums (L), then the
+    // local CSR offsets (L + 1) and neighbours (E) as 16-bit indices
+    double *buf[2] = {g_value_lds, g_value_lds + L};
+    double *wl = g_value_lds + 2 * L, *wsl = wl + E;
+    unsigned short *off = reinterpret_cast<unsigned short *>(wsl + L);
+    unsigned short *adj = off + L + 1;
+    const int *aoff = pp.aoff + s0 + p;
+    const double *wb = w + (size_t)b * wstride;
+    for (int j = tid; j < L; j += kPatchThreads)
+        buf[0][j] = src[pp.perm[lv[j]]];
+    for (int j = tid; j <= L; j += kPatchThreads)
+        off[j] = (unsigned short)aoff[j];
+    for (int k = tid; k < E; k += kPatchThreads)
+        adj[k] = pp.ladj[a0 + k];
+    const int updated = ring_end[pp.rings - 1];
+    for (int j = tid; j < updated; j += kPatchThreads) {
+        const int bo = aoff[j], e = aoff[j + 1];
+        const double *wk = wb + goff[pp.perm[lv[j]]];
+        const int d = e - bo;
+        // the first kPatchGather weights are loaded together, then summed in order
+        double x[kPatchGather];
+#pragma unroll
+        for (int k = 0; k < kPatchGather; ++k)
+            x[k] = k < d ? wk[k] : 0.0;
+        double ws = 0.0;
+#pragma unroll
+        for (int k = 0; k < kPatchGather; ++k)
+            if (k < d) {
+                wl[bo + k] = x[k];
+                ws += x[k];
+            }
+        for (int k = kPatchGather; k < d; ++k) {
+            const double y = wk[k];
+            wl[bo + k] = y;
+            ws += y;
+        }
+        wsl[j] = ws;
+    }
+    __syncthreads();
+    for (int t = 0; t < m; ++t) {
+        const double *cv = buf[t & 1];
+        double *nv = buf[(t + 1) & 1];
+        const int count = ring_end[m - 1 - t];
+        for (int j = tid; j < count; j += kPatchThreads) {
+            const int bo = off[j], e = off[j + 1];
+            // the first kPatchGather neighbours' loads are issued together; the
+            // sum still runs in neighbour order
+            int q[kPatchGather];
+#pragma unroll
+            for (int k = 0; k < kPatchGather; ++k)
+                q[k] = bo + k < e ? adj[bo + k] : j;
+            double qv[kPatchGather], qw[kPatchGather];
+#pragma unroll
+            for (int k = 0; k < kPatchGather; ++k) {
+                qv[k] = cv[q[k]];
+                qw[k] = bo + k < e ? wl[bo + k] : 0.0;
+            }
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < kPatchGather; ++k)
+                if (bo + k < e)
+                    s += qw[k] * qv[k];
+            for (int k = bo + kPatchGather; k < e; ++k)
+                s += wl[k] * cv[adj[k]];
+            nv[j] = (cv[j] + s) / (1.0 + wsl[j]);
+        }
+        __syncthreads();
+    }
+    const double *fin = buf[m & 1];
+    for (int j = tid; j < core; j += kPatchThreads)
+        dst[pp.perm[lv[j]]] = fin[j];
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_pack4(const float *__restrict__ v, int n, float4 *__restrict__ out)
 {
@@ -844,6 +945,32 @@ int launch_heat_weights(const float *geom, const int *off, const int *nbr, int n
 {
     hipLaunchKernelGGL(k_heat_weights, dim3(grid_for(n), hb.nb), dim3(kBlock), 0, s, geom, off,
                        nbr, n, hb, squared_norm, wstride, w, wsum);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+int launch_smooth_values_patched(const double *cur, const int *goff, const double *w, int n,
+                                 size_t wstride, const HeatBatch &hb, const SmoothPatches &pp,
+                                 int it0, size_t lds_bytes, double *nxt, hipStream_t s)
+{
+    static std::once_flag once;
+    static int limit = 0;
+    std::call_once(once, [] {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                hipSuccess)
+            limit = 0;
+        if (limit > 0)
+            (void)hipFuncSetAttribute((const void *)k_smooth_values_patch,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, limit);
+        (void)hipGetLastError();
+    });
+    if (lds_bytes > (size_t)limit)
+        return fail(kInvalidArgument, "value smoothing patches need %zu bytes of LDS (limit %d)",
+                    lds_bytes, limit);
+    hipLaunchKernelGGL(k_smooth_values_patch, dim3(pp.count, hb.nb), dim3(kPatchThreads),
+                       lds_bytes, s, cur, nxt, n, goff, w, wstride, pp, hb, it0);
     CSG_LAUNCH_CHECK();
     return kOk;
 }

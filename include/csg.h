@@ -250,6 +250,18 @@ int csg_smooth_values(const double *values, int n, int nb, const float *geom, in
                       const double *sigma, const int *iterations, int squared_norm, double *out,
                       void *ws, size_t ws_bytes, void *stream);
 
+/* csg_smooth_values with up to `rings` iterations per launch on mesh patches (the plan of
+ * csg_smooth_geometry_patched, built by surface_stencil.smoothing_patches; lds_bytes =
+ * the plan's 26 L + 10 E bound).  Values equal csg_smooth_values'.  values and out must
+ * not overlap.  No counterpart in the reference (it iterates surface_stencil.py:390-476). */
+int csg_smooth_values_patched(const double *values, int n, int nb, const float *geom, int meshes,
+                              const int *mesh_of, const int *nbr_off, const int *nbr, int nnz,
+                              const double *sigma, const int *iterations, int squared_norm,
+                              int npatches, int rings, const int *perm, const int *vstart,
+                              const int *ring_end, const int *lverts, const int *aoff,
+                              const int *astart, const unsigned short *ladj, size_t lds_bytes,
+                              double *out, void *ws, size_t ws_bytes, void *stream);
+
 /* (x - min) / max(max - min, floor) per row of nb rows of n doubles. */
 size_t csg_minmax_workspace_bytes(int nb);
 int csg_minmax_normalise(const double *x, size_t n, int nb, double floor_, int out_dtype,

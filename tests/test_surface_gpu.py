@@ -668,6 +668,48 @@ def test_geometry_smoothing_patched_equals_per_iteration(level, iterations, patc
     assert torch.equal(out, ref)
 
 
+@pytest.mark.parametrize("level,counts,patches", [
+    (6, (0, 1, 4, 7, 375, 166), (320, 4)),
+    (5, (3, 9, 8, 13), (64, 7)),
+    (3, (5, 2), (100000, 1)),
+])
+def test_value_smoothing_patched_equals_per_iteration(level, counts, patches, cuda):
+    """Patch-local value smoothing (csg_smooth_values_patched) gives the bits of
+    one launch per iteration (csg_smooth_values) for rows on two meshes with
+    iteration counts of 0, below, at and between multiples of the rings per
+    launch, in both distance forms."""
+
+    import cat_surface_gpu as csg
+    from cat_surface_gpu.stencil_builder import icosphere
+    from cat_surface_gpu.surface_stencil import set_value_smoothing
+
+    v, f = icosphere(level)
+    n = v.shape[0]
+    idx = np.repeat(np.arange(n, dtype=np.int32)[:, None], 3, axis=1)
+    w = np.tile(np.array([1.0, 0.0, 0.0]), (n, 1))
+    st = csg.SurfaceStencil(sphere_points=v, faces=f.astype(np.int32), surface_indices=idx,
+                            surface_weights=w, sheet_indices=np.zeros((1, 1, 3), np.int32),
+                            sheet_weights=np.zeros((1, 1, 3)), nx=1, ny=1,
+                            source_points=n).to(cuda)
+    rng = np.random.default_rng(level + 7)
+    geom = torch.as_tensor(v[None] * 50.0 * (1.0 + 0.05 * rng.standard_normal((2, n, 1))),
+                           dtype=torch.float32, device=cuda).contiguous()
+    rows = torch.as_tensor(rng.standard_normal((len(counts), n)), dtype=torch.float64,
+                           device=cuda)
+    params = [(c, 1.0 + 0.3 * k) for k, c in enumerate(counts)]
+    mesh_of = [k % 2 for k in range(len(counts))]
+    for squared in (False, True):
+        previous = set_value_smoothing(None)
+        try:
+            ref = st._smooth_values_batch(rows, geom, params, squared, mesh_of=mesh_of)
+            set_value_smoothing(patches)
+            out = st._smooth_values_batch(rows, geom, params, squared, mesh_of=mesh_of)
+        finally:
+            set_value_smoothing(previous)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref)
+
+
 @pytest.mark.parametrize("level,grid", [(5, 32), (6, 32), (4, 64)])
 def test_closest_triangles_pruned_equals_full_scan(level, grid, cuda):
     """Bounding-sphere pruning of the closest-triangle queries returns the faces,
