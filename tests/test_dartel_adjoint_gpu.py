@@ -101,4 +101,4 @@ def test_gradient_descends_the_energy(cuda):
     step = 1e-3 / float(g.abs().max())
     e1 = float(exp_warp_ssd(v - step * g, f, t))
     predicted = step * float((g * g).sum())
-    assert abs((float(e0) - e1) - predicted) <= 0.05 * predicted
+    assert abs((float(e0.detach()) - e1) - predicted) <= 0.05 * predicted
