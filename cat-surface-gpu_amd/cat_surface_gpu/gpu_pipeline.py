@@ -610,6 +610,7 @@ def run_cat_surface_gpu_pipelines(
         if avg:
             t = time.perf_counter()
             unrotate = rotation_matrix_np(-pole)
+            combined_all = []
             for st in states:
                 main_warped = apply_flow_to_stenciled_sphere(st.final_prewarp, st.last_flow,
                                                              st.cur_stencil, inverse=True,
@@ -623,6 +624,9 @@ def run_cat_surface_gpu_pipelines(
                     _lib.ptr(avg_warped), _lib.ptr(main_warped), int(main_warped.shape[0]),
                     _lib.dtype_code(solve_dtype), _lib.doubles(unrotate.reshape(-1)),
                     _lib.ptr(combined), _lib.stream_handle(dev)), "csg_average_xz")
+                combined_all.append(combined)
+            # one host copy after every job's launches (no synchronisation between jobs)
+            for st, combined in zip(states, combined_all):
                 st.current_sphere = combined.cpu().numpy()
             shared["avg_seconds"] = time.perf_counter() - t
         shared["dartel_total_seconds"] = time.perf_counter() - t_dartel
