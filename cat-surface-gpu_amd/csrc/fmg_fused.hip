@@ -274,9 +274,51 @@ __device__ __noinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w
         const int half = (nx + 1) >> 1;
         const float inv_half = 1.0f / (float)half;
         const bool odd = (nx & 1) && nx > 1;
+        // one first-phase pixel per thread and colour: its stencil offsets, 2x2
+        // block and right-hand side stay in registers for the whole call (A and
+        // b do not change during a relaxation), the same values relax_pixel
+        // reads.  The odd-width last column keeps the general loop below.
+        const bool in_regs = half * ny <= kCoarseThreads;
+        int rp[2] = {0, 0}, rxm[2] = {0, 0}, rxp[2] = {0, 0}, rym[2] = {0, 0}, ryp[2] = {0, 0};
+        bool rok[2] = {false, false};
+        RelaxCoef<T> rk[2] = {};
+        T rbx[2] = {(T)0, (T)0}, rby[2] = {(T)0, (T)0};
+        if (in_regs && (int)threadIdx.x < half * ny) {
+            const DivMod yk = divmod_small((int)threadIdx.x, half, inv_half);
+            const int y = yk.q, row = y * nx;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int x = 2 * yk.r + ((c + y) & 1);
+                rok[c] = x < nx && !(odd && x == nx - 1);
+                if (!rok[c])
+                    continue;
+                const int p = row + x;
+                rp[c] = p;
+                rxm[c] = row + (x == 0 ? nx - 1 : x - 1);
+                rxp[c] = row + (x == nx - 1 ? 0 : x + 1);
+                rym[c] = (y == 0 ? 0 : y - 1) * nx + x;
+                ryp[c] = (y == ny - 1 ? y : y + 1) * nx + x;
+                rk[c] = relax_coef<T>(a[p], a[P + p], a[2 * P + p], w00);
+                rbx[c] = b[p];
+                rby[c] = b[P + p];
+            }
+        }
         for (int it = 0; it < 2 * nit; ++it) {
             const int colour = (it % 2 == 0) ? 1 : 0;
-            for (int phase = 0; phase < (odd ? 2 : 1); ++phase) {
+            if (in_regs) {
+                if (rok[colour]) {
+                    const int p = rp[colour], xm = rxm[colour], xp = rxp[colour],
+                              ym = rym[colour], yp = ryp[colour];
+                    const Stencil<T> st{u[p],           u[P + p],           u[ym] + u[yp],
+                                        u[xm] + u[xp], u[P + ym] + u[P + yp], u[P + xm] + u[P + xp]};
+                    const Pair<T> v =
+                        relax_solve<T>(rbx[colour], rby[colour], st, rk[colour], w01, w10);
+                    u[p] = v.x;
+                    u[P + p] = v.y;
+                }
+                __syncthreads();
+            }
+            for (int phase = in_regs ? 1 : 0; phase < (odd ? 2 : 1); ++phase) {
                 const int count = phase == 0 ? half * ny : ny;
                 for (int q = threadIdx.x; q < count; q += kCoarseThreads) {
                     int x, y;
