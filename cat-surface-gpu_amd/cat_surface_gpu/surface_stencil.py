@@ -870,18 +870,32 @@ def smooth_geometry_many(stencils, geometries, iterations) -> list[torch.Tensor]
     return out
 
 
-def type5_sheets_many(stencils, geometries, fwhms, out_dtype=torch.float64) -> list[torch.Tensor]:
-    """curvature_type5_to_sheet_many for several (stencil, resampled geometry)
-    pairs: one host synchronisation for all edge means and the fwhm-50 geometry
-    smoothing of every mesh in shared launches.  Returns [len(fwhms), ny, nx]
-    sheets per pair, equal to the per-pair calls."""
+def prepare_type5(stencils, geometries) -> tuple[list, list, list]:
+    """The fwhm-independent part of curvature_type5_to_sheet_many for several
+    (stencil, resampled geometry) pairs: float32 geometries, edge means (one
+    host synchronisation) and the fwhm-50 smoothed geometries (shared launches).
+    Pairs with different fwhm lists can share it (type5_sheets_many's
+    ``prepared``)."""
 
-    fw = tuple(float(f) for f in fwhms)
     geoms = [torch.as_tensor(g, dtype=torch.float32, device=st.device).contiguous()
              for st, g in zip(stencils, geometries)]
     ems = edge_means(stencils, geoms)
     geom_iters = [heat_parameters_from_edge_mean(em, 50.0)[0] for em in ems]
     smoothed = smooth_geometry_many(stencils, geoms, geom_iters)
+    return geoms, ems, smoothed
+
+
+def type5_sheets_many(stencils, geometries, fwhms, out_dtype=torch.float64, *,
+                      prepared=None) -> list[torch.Tensor]:
+    """curvature_type5_to_sheet_many for several (stencil, resampled geometry)
+    pairs: one host synchronisation for all edge means and the fwhm-50 geometry
+    smoothing of every mesh in shared launches (or ``prepared``, prepare_type5's
+    result for these pairs).  Returns [len(fwhms), ny, nx] sheets per pair,
+    equal to the per-pair calls."""
+
+    fw = tuple(float(f) for f in fwhms)
+    geoms, ems, smoothed = (prepared if prepared is not None
+                            else prepare_type5(stencils, geometries))
     out: list[torch.Tensor | None] = [None] * len(stencils)
     if not fw:
         return [torch.empty((0, st.ny, st.nx), dtype=out_dtype, device=st.device)

@@ -294,7 +294,7 @@ def solve_dartel_from_surfaces_jobs(requests, *, n_steps: int = 2, loop: int = 6
     and all solves run as one batched solve (solve_dartel_maps_jobs).
     """
 
-    from .surface_stencil import SurfaceStencil, type5_sheets_many
+    from .surface_stencil import SurfaceStencil, prepare_type5, type5_sheets_many
 
     dev = resolve_device(device)
     stencils, geoms, fws = [], [], []
@@ -312,9 +312,14 @@ def solve_dartel_from_surfaces_jobs(requests, *, n_steps: int = 2, loop: int = 6
     by_fwhm: dict[tuple, list[int]] = {}
     for i, f in enumerate(fws):
         by_fwhm.setdefault(f, []).append(i)
+    # the fwhm-50 geometry smoothing does not depend on the fwhms: every side's in
+    # the same launches
+    g32, ems, smoothed = prepare_type5(stencils, geoms)
     for f, members in by_fwhm.items():
         res = type5_sheets_many([stencils[i] for i in members], [geoms[i] for i in members], f,
-                                dtype)
+                                dtype, prepared=([g32[i] for i in members],
+                                                 [ems[i] for i in members],
+                                                 [smoothed[i] for i in members]))
         for i, r in zip(members, res):
             sheets[i] = r
     # one batched solve per map shape; shapes run on streams of their own so
