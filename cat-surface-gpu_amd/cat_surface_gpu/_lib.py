@@ -32,6 +32,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_device_count": (_i, [ctypes.POINTER(_i)]),
     "csg_set_fmg_schedule": (_i, [_i]),
     "csg_set_sor_placement": (_i, [_i]),
+    "csg_set_sor_force_xcd": (_i, [_i]),
+    "csg_reset_sor_xcd_check": (_i, []),
     "csg_set_relax_tile": (_i, [_i, _i, _i, _i, _i]),
     "csg_fmg_tiled_levels": (_i, [_i, _i, _i, _i]),
     "csg_get_relax_tile": (_i, [_i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),

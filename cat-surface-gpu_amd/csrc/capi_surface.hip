@@ -1,9 +1,11 @@
 // C ABI of the surface side of the hot path (declared in include/csg.h).
 #include "../../include/csg.h"
 #include "depth_kernels.h"
+#include "xcd_sync.h"
 #include "surface_kernels.h"
 
 #include <algorithm>
+#include <atomic>
 
 using namespace csg;
 
@@ -440,6 +442,28 @@ int csg_type5_sheets_batch(const float *geom, const float *smoothed, int n, int 
 
 namespace {
 
+// Devices on which an XCD-local SOR left a group without workgroups.
+std::atomic<unsigned long long> g_xcd_local_unusable{0};
+
+int current_device()
+{
+    int dev = 0;
+    return hipGetDevice(&dev) == hipSuccess ? dev : 0;
+}
+
+bool xcd_local_unusable()
+{
+    const int dev = current_device();
+    return dev < 64 && ((g_xcd_local_unusable.load() >> dev) & 1ull);
+}
+
+void mark_xcd_local_unusable()
+{
+    const int dev = current_device();
+    if (dev < 64)
+        g_xcd_local_unusable.fetch_or(1ull << dev);
+}
+
 DepthArgs carve_depth(Carve &c, int batch, int n, int nf, int nnz, int width)
 {
     DepthArgs a{};
@@ -476,6 +500,20 @@ int csg_set_sor_placement(int mode)
     if (mode < 0 || mode > 2)
         return fail(kInvalidArgument, "SOR placement must be 0, 1 or 2, got %d", mode);
     return set_sor_xcd_local(mode);
+}
+
+int csg_set_sor_force_xcd(int xcd)
+{
+    if (xcd >= kXcds)
+        return fail(kInvalidArgument, "XCD must be below %d, got %d", kXcds, xcd);
+    return set_sor_force_xcd(xcd);
+}
+
+int csg_reset_sor_xcd_check(void)
+{
+    const int dev = current_device();
+    const unsigned long long bit = dev < 64 ? (1ull << dev) : 0ull;
+    return (g_xcd_local_unusable.fetch_and(~bit) & bit) ? 1 : 0;
 }
 
 // Depth potential of the rotation feature for `batch` meshes of one topology:
@@ -549,7 +587,19 @@ int csg_depth_potential_patched(const float *pts, int batch, int n, const int *f
     for (int i = 0; i <= ncolours; ++i)
         co.off[i] = colour_off[i];
     CSG_TRY(depth_setup(a, s));
-    CSG_TRY(depth_sor(a, co, max_iter, omega, patches ? &pp : nullptr, s));
+    SorRun run{};
+    const bool allow_local = !xcd_local_unusable();
+    CSG_TRY(depth_sor(a, co, max_iter, omega, patches ? &pp : nullptr, allow_local, &run, s));
+    int rc = kOk;
+    if (!sor_placement_complete(a, run, s, &rc)) {
+        if (rc != kOk)
+            return rc;
+        // some XCD group never ran (the device does not spread workgroups over
+        // eight XCDs): solve again on the device-wide barrier, and keep to it
+        mark_xcd_local_unusable();
+        CSG_TRY(depth_setup(a, s));
+        CSG_TRY(depth_sor(a, co, max_iter, omega, nullptr, false, &run, s));
+    }
     if (relres)
         CSG_TRY(depth_residual(a, relres, s));
     int timeout = 0;

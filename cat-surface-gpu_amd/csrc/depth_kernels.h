@@ -20,6 +20,7 @@ struct SorColours {
 // Workgroups of each XCD's group in the XCD-local SOR (0: the XCD holds no mesh).
 struct SorXcdPlan {
     int blocks[8];
+    int force_xcd;  // < 0: each workgroup claims on the XCD it runs on (see xcd_claim)
 };
 
 struct DepthArgs {
@@ -48,16 +49,34 @@ struct SorPatches {
     const unsigned short *e_lcol;
 };
 
+// How a depth_sor call placed its sweeps: `xcd_local` when each mesh's
+// workgroups were meant to run on XCD b % 8 (plan.blocks[x] workgroups of
+// XCD x), which holds only where the device spreads workgroups over eight
+// XCDs; sor_placement_complete checks that afterwards.
+struct SorRun {
+    bool xcd_local;
+    SorXcdPlan plan;
+};
+
 int depth_setup(const DepthArgs &a, hipStream_t s);
 // patches (may be null): sweep in LDS on those patches when they fit the
 // device (one workgroup per patch and mesh, all resident), else as without.
+// allow_xcd_local = false keeps the sweeps on the device-wide grid barrier.
 int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double omega,
-              const SorPatches *patches, hipStream_t s);
+              const SorPatches *patches, bool allow_xcd_local, SorRun *run, hipStream_t s);
+// After the stream has drained: whether every XCD group of an XCD-local run
+// received all its workgroups.  On a device that does not spread workgroups
+// over eight XCDs (a CPX/DPX partition reports one XCC_ID for all of them)
+// some groups claim nothing, their meshes are never swept, and nothing waits
+// for them: the caller must then solve again without the XCD-local placement.
+bool sor_placement_complete(const DepthArgs &a, const SorRun &run, hipStream_t s, int *rc);
 int depth_residual(const DepthArgs &a, double *out, hipStream_t s);
 // Process-wide placement of the SOR sweeps: 2 (default) each mesh's rows on
 // one XCD, x stored through that XCD's L2; 1 the same with x stores written
 // through to memory; 0 the device-wide grid barrier.  1 and 2 apply when the
 // meshes fit.  Returns the previous setting.  All compute identical values.
 int set_sor_xcd_local(int mode);
+// Test hook: every XCD-local SOR workgroup claims on XCD `xcd` (< 0: off).
+int set_sor_force_xcd(int xcd);
 
 }  // namespace csg

@@ -461,7 +461,7 @@ k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restr
                 int max_iter, SorXcdPlan plan, double *x, unsigned *bar, int *timeout)
 {
     int xcd, slot;
-    xcd_claim(bar, kBarrierSlotWords, xcd, slot);
+    xcd_claim(bar, kBarrierSlotWords, plan.force_xcd, xcd, slot);
     const int nblocks = plan.blocks[xcd];
     if (slot >= nblocks)
         return;
@@ -552,7 +552,7 @@ k_sor_patched(SorPatches pp, int ncolours, const double *__restrict__ ell_val,
 {
     extern __shared__ double g_sor_lds[];
     int xcd, slot;
-    xcd_claim(bar, kBarrierSlotWords, xcd, slot);
+    xcd_claim(bar, kBarrierSlotWords, plan.force_xcd, xcd, slot);
     const int nblocks = plan.blocks[xcd];
     if (slot >= nblocks)
         return;
@@ -701,6 +701,12 @@ int depth_setup(const DepthArgs &a, hipStream_t s)
 
 namespace {
 std::atomic<int> g_sor_xcd_local{2};
+std::atomic<int> g_sor_force_xcd{-1};
+}
+
+int set_sor_force_xcd(int xcd)
+{
+    return g_sor_force_xcd.exchange(xcd < 0 ? -1 : xcd);
 }
 
 int set_sor_xcd_local(int mode)
@@ -709,8 +715,10 @@ int set_sor_xcd_local(int mode)
 }
 
 int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double omega,
-              const SorPatches *patches, hipStream_t s)
+              const SorPatches *patches, bool allow_xcd_local, SorRun *run, hipStream_t s)
 {
+    if (run)
+        *run = SorRun{};
     const int W = a.ell_width;
     const size_t total = (size_t)W * a.n + a.n;
     hipLaunchKernelGGL(k_colour_order, dim3(grid_for(total), a.batch), dim3(kBlock), 0, s,
@@ -737,10 +745,11 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     // XCD-local placement: mesh b on XCD b % 8; every XCD's group must fit on
     // its share of the CUs at the register kernel's occupancy
     SorXcdPlan plan{};
+    plan.force_xcd = g_sor_force_xcd.load();
     int local_per_cu = 0;
     CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &local_per_cu, k_sor_xcd_local<kSorRegColours, kSorRegWidth, true>, kSorRegBlock, 0));
-    const int placement = g_sor_xcd_local.load();
+    const int placement = allow_xcd_local ? g_sor_xcd_local.load() : 0;
     bool local = placement != 0 && co.count <= kSorRegColours &&
                  W <= kSorRegWidth && cus % kXcds == 0;
     int local_grid = 0;
@@ -773,7 +782,7 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     // per row and sweep: the row's ELL values and columns (12 B per slot), its
     // x gathers (8 B per slot), rhs, diagonal and the x update (24 B)
     const double algorithmic = (double)max_iter * batch * n * (20.0 * W + 24.0);
-    if (patches && patches->rings == co.count && co.count <= kSorRegColours &&
+    if (allow_xcd_local && patches && patches->rings == co.count && co.count <= kSorRegColours &&
         W <= kSorRegWidth && cus % kXcds == 0 && a.batch >= 1 &&
         patches->max_rows <= kSorPatchRows * kSorPatchThreads) {
         // every (patch, mesh) workgroup of an XCD must be resident at once
@@ -782,6 +791,7 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
         CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &patch_per_cu, k_sor_patched<kSorRegWidth>, kSorPatchThreads, lds));
         SorXcdPlan pplan{};
+        pplan.force_xcd = plan.force_xcd;
         bool fits = patch_per_cu > 0;
         int pgrid = 0;
         for (int xcd = 0; xcd < kXcds && fits; ++xcd) {
@@ -798,6 +808,8 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
                                n, max_iter, pplan, x, a.x_alt, bar, timeout);
             CSG_LAUNCH_CHECK();
             ptimer.done();
+            if (run)
+                *run = SorRun{true, pplan};
             // after an odd number of iterations the result is in the second buffer
             const long long steps = ((long long)max_iter * co.count + patches->rings - 1) /
                                     patches->rings;
@@ -832,7 +844,30 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
                            x, bar, timeout);
     CSG_LAUNCH_CHECK();
     timer.done();
+    if (run && local)
+        *run = SorRun{true, plan};
     return kOk;
+}
+
+bool sor_placement_complete(const DepthArgs &a, const SorRun &run, hipStream_t s, int *rc)
+{
+    *rc = kOk;
+    if (!run.xcd_local)
+        return true;
+    // slot x of the barrier state holds XCD x's claim counter: the number of
+    // workgroups that ran on that XCD (spares included)
+    unsigned claims[kXcds * kBarrierSlotWords];
+    const hipError_t e1 = hipMemcpyAsync(claims, a.barrier, sizeof(claims),
+                                         hipMemcpyDeviceToHost, s);
+    const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(s) : e1;
+    if (e2 != hipSuccess) {
+        *rc = fail(kRuntimeError, "reading the SOR placement: %s", hipGetErrorString(e2));
+        return false;
+    }
+    for (int x = 0; x < kXcds; ++x)
+        if ((long long)claims[(size_t)x * kBarrierSlotWords] < (long long)run.plan.blocks[x])
+            return false;
+    return true;
 }
 
 int depth_residual(const DepthArgs &a, double *out, hipStream_t s)

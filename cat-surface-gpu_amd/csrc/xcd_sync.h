@@ -36,12 +36,15 @@ __device__ __forceinline__ int xcd_id()
 }
 
 // Thread 0 claims the next slot of its XCD from claim[xcd * stride]; every
-// thread returns (xcd, slot).
-__device__ __forceinline__ void xcd_claim(unsigned *claim, int stride, int &xcd, int &slot)
+// thread returns (xcd, slot).  force_xcd >= 0 makes every workgroup claim as
+// if it ran on that XCD (what a partition whose XCDs all report one XCC_ID
+// does; set only by the test hook csg_set_sor_force_xcd).
+__device__ __forceinline__ void xcd_claim(unsigned *claim, int stride, int force_xcd, int &xcd,
+                                          int &slot)
 {
     __shared__ int s_xcd, s_slot;
     if (threadIdx.x == 0) {
-        const int x = xcd_id();
+        const int x = force_xcd >= 0 ? (force_xcd & (kXcds - 1)) : xcd_id();
         s_xcd = x;
         s_slot = (int)__hip_atomic_fetch_add(claim + (size_t)x * stride, 1u, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);

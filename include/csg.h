@@ -317,6 +317,17 @@ int csg_type5_sheets_batch(const float *geom, const float *smoothed, int n, int 
  * barrier.  All compute the same values in the same order.  Process-wide; returns the
  * previous setting.  No counterpart in the reference. */
 int csg_set_sor_placement(int mode);
+/* The XCD-local placements assume the device spreads workgroups over eight XCDs (workgroup
+ * i on XCD i mod 8, each reporting its own XCC_ID).  After each XCD-local solve the claim
+ * counters are read back: if an XCD's group received fewer workgroups than planned (a
+ * CPX/DPX partition, where every workgroup reports one XCC_ID), the solve is repeated on
+ * the device-wide grid barrier and the device keeps to that placement for the rest of the
+ * process.  csg_reset_sor_xcd_check forgets that for the current device (returns 1 if it
+ * had been recorded).  csg_set_sor_force_xcd (test hook; < 0 turns it off) makes every
+ * XCD-local workgroup claim on XCD `xcd`, as such a partition does; returns the previous
+ * value.  No counterpart in the reference. */
+int csg_set_sor_force_xcd(int xcd);
+int csg_reset_sor_xcd_check(void);
 size_t csg_depth_workspace_bytes(int n, int nf, int nnz, int batch, int ell_width);
 int csg_depth_potential(const float *pts, int batch, int n, const int *faces, int nf,
                         const int *inc_off, const int *inc, const int *row_off, const int *cols,

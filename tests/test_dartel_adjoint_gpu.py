@@ -85,7 +85,36 @@ def test_float32_adjoint_against_float64_and_dartel_rhs(cuda, nx, ny):
     rel = relative_l2(grads[torch.float32], rhs)
     cos = cosine(grads[torch.float32], rhs)
     print(f"\n{nx}x{ny}: rel L2 adjoint vs DARTEL rhs {rel:.3e}, cosine {cos:.6f}")
-    assert cos > 0.9
+    # measured on MI355X: rel 6.2e-2 / 4.3e-2, cosine 0.998 / 0.999 (256x128 / 512x256)
+    assert rel <= 0.07
+    assert cos >= 0.998
+
+
+@pytest.mark.parametrize("shift", [0, 3])
+def test_adjoint_equals_dartel_rhs_at_integer_translations(cuda, shift):
+    """With no squarings and a velocity that is an integer translation (zero
+    included), every sample sits on a pixel and the small-deformation Jacobian
+    is the identity: the exact gradient of warp + SSD is then DARTEL's code-0
+    right-hand side (the forward-difference bilinear derivative times the
+    residual, dartel_grid.py:723-775), which the oracle computes.  The membrane
+    term vanishes on a constant field."""
+
+    from cat_surface_gpu.dartel_adjoint import exp_warp_ssd
+
+    _, f, t = _problem(64, 32, cuda)
+    v = torch.zeros((2, 32, 64), dtype=torch.float64, device=cuda)
+    v[0] = float(shift)
+    w = v.clone().requires_grad_(True)
+    exp_warp_ssd(w, f, t, squarings=0, membrane=0.01).backward()
+    fn, tn = f.cpu().numpy(), t.cpu().numpy()
+    phi = O.make_identity_map(64, 32)
+    phi[0] += float(shift)
+    ident = np.zeros((4, 32, 64))
+    ident[0] = ident[3] = 1.0
+    _, rhs, _ = O.initialise_objfun(fn, tn, phi, ident)
+    got = w.grad.cpu().numpy()
+    assert np.abs(rhs).max() > 1e-3
+    np.testing.assert_allclose(got, rhs, rtol=0, atol=1e-12 * np.abs(rhs).max())
 
 
 def test_gradient_descends_the_energy(cuda):

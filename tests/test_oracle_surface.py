@@ -94,10 +94,11 @@ def test_seed_grid_and_rotation_costs(rot):
     costs, lf, lb = O.rotation_costs(rot["src_pts"], rot["src_vals"], rot["tgt_vals"],
                                      rot["verts"], rot["faces"], rot["seeds"], 32, 1,
                                      return_location=True)
-    np.testing.assert_allclose(costs, rot["seed_costs"], rtol=1e-6)
+    # float32 location, float64 sums in another order than torch's: 5.7e-8 relative
+    np.testing.assert_allclose(costs, rot["seed_costs"], rtol=2e-7)
     assert int(np.argmin(costs)) == int(np.argmin(rot["seed_costs"]))
-    agree = (lf[:5, :200] == rot["loc_faces"]).mean()
-    assert agree > 0.995, agree
+    np.testing.assert_array_equal(lf[:5, :200], rot["loc_faces"])
+    np.testing.assert_allclose(lb[:5, :200], rot["loc_bary"], rtol=0, atol=2e-6)
 
 
 def test_identity_rotation_of_the_target_itself_costs_nothing():
@@ -118,9 +119,10 @@ def test_nelder_mead_reaches_reference_minimum(rot):
         return O.rotation_costs(rot["src_pts"], rot["src_vals"], rot["tgt_vals"], rot["verts"],
                                 rot["faces"], angles, 32, 1, index=index)
 
-    angle, c, _ = O.nelder_mead(cost, rot["seed_angle"], max_iter=60)
-    np.testing.assert_allclose(angle, rot["nm_angle"], atol=2e-3)
-    assert abs(c - float(rot["nm_cost"])) <= 5e-3 * abs(float(rot["nm_cost"])) + 1e-12
+    angle, c, iterations = O.nelder_mead(cost, rot["seed_angle"], max_iter=60)
+    np.testing.assert_allclose(angle, rot["nm_angle"], rtol=0, atol=1e-12)
+    assert iterations == int(rot["nm_iters"])
+    assert abs(c - float(rot["nm_cost"])) <= 2e-6 * abs(float(rot["nm_cost"]))
 
 
 def test_rotation_matrix_layout(golden):

@@ -258,6 +258,54 @@ def test_sor_patched_equals_per_colour(level, batch, iters, patched, cuda):
     assert used == (1 if patched else 0)
 
 
+@pytest.mark.parametrize("patched", [True, False])
+def test_sor_partition_falls_back_to_grid_barrier(patched, cuda):
+    """A device whose workgroups all report one XCC_ID (a CPX/DPX partition,
+    forced from the host) leaves the XCD groups of meshes 1..3 without
+    workgroups: the solve notices, repeats on the device-wide barrier and
+    returns exactly its values, and the device keeps to that placement."""
+
+    import cat_surface_gpu as csg
+    from cat_surface_gpu import _lib
+    from cat_surface_gpu.rotation_feature import (depth_potential, set_sor_patches,
+                                                  set_sor_placement)
+    from cat_surface_gpu.stencil_builder import icosphere
+
+    v, f = icosphere(6)
+    n = v.shape[0]
+    idx = np.repeat(np.arange(n, dtype=np.int32)[:, None], 3, axis=1)
+    w = np.tile(np.array([1.0, 0.0, 0.0]), (n, 1))
+    st = csg.SurfaceStencil(
+        sphere_points=v, faces=f.astype(np.int32), surface_indices=idx, surface_weights=w,
+        sheet_indices=np.zeros((1, 1, 3), np.int32), sheet_weights=np.zeros((1, 1, 3)),
+        nx=1, ny=1, source_points=n).to(cuda)
+    rng = np.random.default_rng(17)
+    bumps = 1.0 + 0.05 * rng.standard_normal((4, n, 1))
+    pts = torch.as_tensor(v[None] * 60.0 * bumps, dtype=torch.float32, device=cuda)
+    lib = _lib.lib()
+    lib.csg_reset_sor_xcd_check()
+    prev_place = set_sor_placement(0)
+    prev_patches = set_sor_patches(False)
+    try:
+        xg, _, rg = depth_potential(st, pts, max_iter=50)
+        set_sor_placement(2)
+        set_sor_patches(patched)
+        xl, _, _ = depth_potential(st, pts, max_iter=50)  # the real placement: no fallback
+        assert lib.csg_reset_sor_xcd_check() == 0
+        lib.csg_set_sor_force_xcd(0)
+        xf, _, rf = depth_potential(st, pts, max_iter=50)
+        assert lib.csg_set_sor_force_xcd(-1) == 0
+        assert lib.csg_reset_sor_xcd_check() == 1
+    finally:
+        lib.csg_set_sor_force_xcd(-1)
+        lib.csg_reset_sor_xcd_check()
+        set_sor_placement(prev_place)
+        set_sor_patches(prev_patches)
+    assert torch.equal(xl, xg)
+    assert torch.equal(xf, xg)
+    assert torch.equal(rf, rg)
+
+
 def test_rotation_index_costs_and_search(golden, cuda):
     import cat_surface_gpu as csg
 
@@ -269,15 +317,33 @@ def test_rotation_index_costs_and_search(golden, cuda):
     dev = idx.to(cuda)
     inputs = dev.prepare_cost_inputs(g["src_pts"], g["src_vals"], g["tgt_vals"])
     costs = _np(dev.cost_batch_prepared(inputs, torch.as_tensor(g["seeds"])))
+    # float32 location (barycentric weights in another operation order than torch's) and
+    # float64 sums in another order: measured 3.4e-7 relative on MI355X (3.8e-7 against
+    # the oracle, weights 1.1e-6 apart); the faces themselves agree exactly
+    print(f"\nseed costs: max rel vs reference {np.abs(costs / g['seed_costs'] - 1).max():.2e}")
     np.testing.assert_allclose(costs, g["seed_costs"], rtol=1e-6)
     faces, bary = dev.locate(inputs, torch.as_tensor(g["seeds"][:5]))
-    agree = (_np(faces)[:, :200] == g["loc_faces"]).mean()
-    assert agree > 0.995, agree
+    # point location is index work: the located faces are exact
+    np.testing.assert_array_equal(_np(faces)[:, :200], g["loc_faces"])
+    ocost, oface, obary = O.rotation_costs(g["src_pts"], g["src_vals"], g["tgt_vals"],
+                                           g["verts"], g["faces"], g["seeds"], 32, 1,
+                                           return_location=True)
+    np.testing.assert_array_equal(_np(faces), oface[:5])
+    print(f"located weights: max |GPU - oracle| {np.abs(_np(bary) - obary[:5]).max():.2e}; "
+          f"costs: max rel {np.abs(costs / ocost - 1).max():.2e}")
+    np.testing.assert_allclose(_np(bary), obary[:5], rtol=0, atol=2e-6)
+    np.testing.assert_allclose(costs, ocost, rtol=1e-6)
     angle, cost, seeds = dev.coarse_seed_search_prepared(inputs)
-    np.testing.assert_allclose(_np(angle), g["seed_angle"])
+    np.testing.assert_array_equal(_np(angle), g["seed_angle"])
     a, c, it = dev.refine_nelder_mead_prepared(inputs, angle, max_iter=60)
-    np.testing.assert_allclose(_np(a), g["nm_angle"], atol=2e-3)
-    np.testing.assert_allclose(float(c), float(g["nm_cost"]), rtol=5e-3)
+    print(f"Nelder-Mead: |angle - reference| {np.abs(_np(a) - g['nm_angle']).max():.2e}, "
+          f"iterations {int(it)} vs {int(g['nm_iters'])}, cost rel "
+          f"{abs(float(c) / float(g['nm_cost']) - 1):.2e}")
+    # the simplex moves are decided by cost comparisons that no reduction-order
+    # difference flips here: the trajectory, its end point and length are the reference's
+    np.testing.assert_allclose(_np(a), g["nm_angle"], rtol=0, atol=1e-12)
+    assert int(it) == int(g["nm_iters"])
+    np.testing.assert_allclose(float(c), float(g["nm_cost"]), rtol=2e-6)
 
 
 def test_rotation_costs_match_oracle_on_random_angles(golden, cuda):
