@@ -2,11 +2,54 @@
 #include "../../include/csg.h"
 #include "locate_kernels.h"
 
+#include <mutex>
+
 using namespace csg;
 
 namespace {
 
 inline hipStream_t S(void *s) { return (hipStream_t)s; }
+
+// Clears a device counter with a kernel rather than hipMemsetAsync, so that a
+// caller may capture the entry point into a graph.
+__global__ void k_clear_int(int *p) { *p = 0; }
+
+int clear_int(int *p, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(1), 0, s, p);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+// Scratch of csg_rotation_costs_noalloc: the per-block partial sums and the
+// uncovered-point counter, in a file-scope device array (one copy per GPU,
+// held until the process ends).  In host code `g_rotation_scratch` names a
+// host-side stand-in; its device address is looked up once per device with
+// hipGetSymbolAddress and only that address is ever passed to kernels.
+constexpr size_t kRotationScratchBytes = size_t(4) << 20;
+constexpr int kMaxScratchDevices = 64;
+static __device__ __align__(256) unsigned char g_rotation_scratch[kRotationScratchBytes];
+void *g_rotation_scratch_ptr[kMaxScratchDevices] = {};
+std::mutex g_rotation_scratch_mutex;
+
+int rotation_scratch(void **out)
+{
+    int dev = 0;
+    CSG_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        return fail(kRuntimeError, "device %d is beyond the scratch table", dev);
+    std::lock_guard<std::mutex> lock(g_rotation_scratch_mutex);
+    if (!g_rotation_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(g_rotation_scratch));
+        if (e != hipSuccess || ptr == nullptr)
+            return fail(kRuntimeError, "hipGetSymbolAddress(rotation scratch): %s",
+                        hipGetErrorString(e));
+        g_rotation_scratch_ptr[dev] = ptr;
+    }
+    *out = g_rotation_scratch_ptr[dev];
+    return kOk;
+}
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -159,7 +202,7 @@ int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
                       : (int *)(b + align256(sizeof(double) * (size_t)rotation_cost_blocks(ns) * na));
     a.loc_face = loc_face;
     a.loc_bary = loc_bary;
-    CSG_HIP_CHECK(hipMemsetAsync(a.uncovered, 0, sizeof(int), s));
+    CSG_TRY(clear_int(a.uncovered, s));
     int rc = launch_rotation_costs(a, s);
     if (rc)
         return rc;
@@ -169,6 +212,30 @@ int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
         CSG_HIP_CHECK(hipStreamSynchronize(s));
     }
     return kOk;
+}
+
+size_t csg_rotation_costs_noalloc_max_bytes(void) { return kRotationScratchBytes; }
+
+int csg_rotation_costs_noalloc(const float *src_pts, const double *src_vals, int ns,
+                               const double *tgt_vals, const int *faces, const float *tri,
+                               const float *sph, const int *cell_off, const int *cell_faces,
+                               int grid, const double *angles, int na, double *costs,
+                               int *loc_face, float *loc_bary, int *uncovered_dev, void *stream)
+{
+    if (ns < 1 || na < 1 || grid < 1 || uncovered_dev == nullptr)
+        return fail(kInvalidArgument, "invalid rotation cost arguments");
+    const size_t need = csg_rotation_costs_workspace_bytes(ns, na);
+    if (need > kRotationScratchBytes)
+        return fail(kInvalidArgument,
+                    "rotation costs for %d points x %d angles need %zu bytes of scratch; the "
+                    "no-allocation entry point holds %zu (use csg_rotation_costs with a "
+                    "workspace)",
+                    ns, na, need, kRotationScratchBytes);
+    void *scratch = nullptr;
+    CSG_TRY(rotation_scratch(&scratch));
+    return csg_rotation_costs(src_pts, src_vals, ns, tgt_vals, faces, tri, sph, cell_off,
+                              cell_faces, grid, angles, na, costs, loc_face, loc_bary, scratch,
+                              kRotationScratchBytes, nullptr, uncovered_dev, stream);
 }
 
 size_t csg_rotation_cost_matrices_workspace_bytes(int ns, int na, int with_gradient)
@@ -238,7 +305,7 @@ int csg_closest_triangles(const double *q, int nq, const double *verts, const in
         return fail(kInvalidArgument, "invalid closest-triangle arguments");
     hipStream_t s = S(stream);
     int *far_count = (int *)ws;
-    CSG_HIP_CHECK(hipMemsetAsync(far_count, 0, sizeof(int), s));
+    CSG_TRY(clear_int(far_count, s));
     int rc = launch_closest_triangles(q, nq, verts, faces, cell_off, cell_faces, grid,
                                       reach * reach, face_out, idx_out, w_out, far_count, s);
     if (rc)
@@ -266,7 +333,7 @@ int csg_closest_triangles_pruned(const double *q, int nq, const double *verts, c
     if (ws_bytes < csg_closest_triangles_workspace_bytes(nf))
         return fail(kInvalidArgument, "closest-triangle workspace too small");
     hipStream_t s = S(stream);
-    CSG_HIP_CHECK(hipMemsetAsync(far_dev, 0, sizeof(int), s));
+    CSG_TRY(clear_int(far_dev, s));
     double4 *sph = (double4 *)((char *)ws + 256);
     CSG_TRY(launch_closest_triangles_pruned(q, nq, verts, faces, nf, cell_off, cell_faces, grid,
                                             reach * reach, sph, face_out, idx_out, w_out,

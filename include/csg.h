@@ -440,6 +440,24 @@ int csg_rotation_costs(const float *src_pts, const double *src_vals, int ns,
                        float *loc_bary, void *ws, size_t ws_bytes, int *uncovered_host,
                        int *uncovered_dev, void *stream);
 
+/* csg_rotation_costs without a workspace argument, for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight Nelder-Mead / serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  It allocates nothing: the partial
+ * sums live in a fixed
+ * device array of csg_rotation_costs_noalloc_max_bytes() bytes (4 MiB) per GPU, held
+ * until the process ends; requests needing more (csg_rotation_costs_workspace_bytes(ns,
+ * na) above it, e.g. more than 204 angles at 163,842 points) are rejected with -1.  The
+ * count of points outside the index goes to uncovered_dev (required; cleared by a kernel,
+ * not a memset, at each call).  Every call on one GPU shares the array, which keeps the
+ * last call's partial sums, so calls that overlap on one GPU (different streams or
+ * threads) race.  Does not synchronise; the results equal csg_rotation_costs'. */
+size_t csg_rotation_costs_noalloc_max_bytes(void);
+int csg_rotation_costs_noalloc(const float *src_pts, const double *src_vals, int ns,
+                               const double *tgt_vals, const int *faces, const float *tri,
+                               const float *sph, const int *cell_off, const int *cell_faces,
+                               int grid, const double *angles, int na, double *costs,
+                               int *loc_face, float *loc_bary, int *uncovered_dev, void *stream);
+
 /* The same SSD for na candidate rotation matrices (float32 [na][9], row-major, q = M p;
  * na <= 65535) instead of angle triples: cost_batch_prepared's formulation
  * (surface_rotation.py:512-546) for rotations that come as matrices (the Hopf grid of
