@@ -108,6 +108,7 @@ static inline double wrapped(const double *b, const Corners *c, double period)
 void oracle_composition(const double *a, const double *b, int nx, int ny, double *out)
 {
     size_t P = PLANE(nx, ny);
+#pragma omp parallel for schedule(static) if (P >= 8192)
     for (size_t p = 0; p < P; ++p) {
         Corners c = corners_at(a[p] - 1.0, a[P + p] - 1.0, nx, ny);
         out[p] = wrapped(b, &c, (double)nx);
@@ -121,6 +122,7 @@ void oracle_composition_jacobian(const double *a, const double *ja, const double
                                  double *out_jac)
 {
     size_t P = PLANE(nx, ny);
+#pragma omp parallel for schedule(static) if (P >= 8192)
     for (size_t p = 0; p < P; ++p) {
         Corners c = corners_at(a[p] - 1.0, a[P + p] - 1.0, nx, ny);
         double cx = wrapped(b, &c, (double)nx);
@@ -146,6 +148,7 @@ void oracle_jacobian_of_displacement(const double *v, double scale, int nx, int 
 {
     size_t P = PLANE(nx, ny);
     double s = scale * 0.5;
+#pragma omp parallel for schedule(static) if (P >= 8192)
     for (int y = 0; y < ny; ++y) {
         for (int x = 0; x < nx; ++x) {
             int xm, ym, xp, yp, x0, y0, x1, y1;
@@ -169,6 +172,7 @@ void oracle_smalldef_jac(const double *v, double scale, int nx, int ny, double *
                          double *out_jac)
 {
     size_t P = PLANE(nx, ny);
+#pragma omp parallel for schedule(static) if (P >= 8192)
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x) {
             size_t p = (size_t)y * nx + x;
@@ -185,6 +189,7 @@ void oracle_jac_div_smalldef(const double *jac, const double *v, double scale, i
     size_t P = PLANE(nx, ny);
     double *d = (double *)malloc(sizeof(double) * 4 * P);
     oracle_jacobian_of_displacement(v, scale, nx, ny, d);
+#pragma omp parallel for schedule(static) if (P >= 8192)
     for (size_t p = 0; p < P; ++p) {
         double d00 = d[p], d10 = d[P + p], d01 = d[2 * P + p], d11 = d[3 * P + p];
         double det = d00 * d11 - d01 * d10;
@@ -208,6 +213,7 @@ void oracle_expdef(const double *v, int k, int with_jac, int nx, int ny, double 
     double *m2 = (double *)malloc(sizeof(double) * 2 * P);
     double *j = (double *)malloc(sizeof(double) * 4 * P);
     double *j2 = (double *)malloc(sizeof(double) * 4 * P);
+#pragma omp parallel for schedule(static) if (P >= 8192)
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x) {
             size_t p = (size_t)y * nx + x;
@@ -240,6 +246,8 @@ double oracle_initialise_objfun(const double *src, const double *tgt, const doub
 {
     size_t P = PLANE(nx, ny);
     double objective = 0.0;
+    double *terms = (double *)malloc(sizeof(double) * P);
+#pragma omp parallel for schedule(static) if (P >= 8192)
     for (size_t p = 0; p < P; ++p) {
         Corners c = corners_at(map[p] - 1.0, map[P + p] - 1.0, nx, ny);
         double k22 = src[c.i22], k12 = src[c.i12], k21 = src[c.i21], k11 = src[c.i11];
@@ -255,8 +263,11 @@ double oracle_initialise_objfun(const double *src, const double *tgt, const doub
         hes[2 * P + p] = gx * gy * w;
         rhs[p] = gx * diff * w;
         rhs[P + p] = gy * diff * w;
-        objective += diff * diff * w;
+        terms[p] = diff * diff * w;
     }
+    for (size_t p = 0; p < P; ++p)
+        objective += terms[p];
+    free(terms);
     return 0.5 * objective;
 }
 
@@ -270,6 +281,7 @@ static void squaring_step(double *b, double *a, const double *map, const double 
     double *a0 = (double *)malloc(sizeof(double) * 3 * P);
     memcpy(b0, b, sizeof(double) * 2 * P);
     memcpy(a0, a, sizeof(double) * 3 * P);
+#pragma omp parallel for schedule(static) if (P >= 8192)
     for (size_t p = 0; p < P; ++p) {
         Corners c = corners_at(map[p] - 1.0, map[P + p] - 1.0, nx, ny);
         double j00 = jac[p], j10 = jac[P + p], j01 = jac[2 * P + p], j11 = jac[3 * P + p];
@@ -356,6 +368,7 @@ void oracle_regularization(const double *v, const double *params, int rtype, int
     size_t P = PLANE(nx, ny);
     double sx = params[0], sy = params[1], mu = params[2], lam = params[3], id = params[4];
     double sx2 = sx * sx, sy2 = sy * sy;
+#pragma omp parallel for schedule(static) if (P >= 8192)
     for (int y = 0; y < ny; ++y) {
         for (int x = 0; x < nx; ++x) {
             size_t p = (size_t)y * nx + x;
@@ -411,6 +424,7 @@ void oracle_apply_membrane(const double *hes, const double *u, const double *par
 {
     size_t P = PLANE(nx, ny);
     oracle_regularization(u, params, 1, nx, ny, out);
+#pragma omp parallel for schedule(static) if (P >= 8192)
     for (size_t p = 0; p < P; ++p) {
         double ux = u[p], uy = u[P + p];
         out[p] = out[p] + hes[p] * ux + hes[2 * P + p] * uy;
@@ -430,7 +444,11 @@ void oracle_relax(const double *hes, const double *rhs, const double *params, in
     double w00 = mu * (2.0 * sx * sx + 2.0 * sy * sy) + params[4];
     double w01 = -mu * sy * sy;
     double w10 = -mu * sx * sx;
+    /* rows of one colour are independent when nx is even or 1 (x wraps onto the
+     * other colour); for odd nx the reference's order is sequential */
+    const int rows_parallel = (nx % 2 == 0) || nx == 1;
     for (int it = 0; it < 2 * nit; ++it) {
+#pragma omp parallel for schedule(static) if (rows_parallel && P >= 8192)
         for (int y = 0; y < ny; ++y) {
             int start = ((it % 2) == (y % 2)) ? 1 : 0;
             for (int x = start; x < nx; x += 2) {
@@ -473,6 +491,7 @@ void oracle_resize(const double *in, int channels, int snx, int sny, int tnx, in
     double *mid = (double *)malloc(sizeof(double) * (size_t)tny * snx);
     for (int c = 0; c < channels; ++c) {
         const double *f = in + c * SP;
+#pragma omp parallel for schedule(static) if (SP >= 8192)
         for (int ty = 0; ty < tny; ++ty) {
             double loc = ((double)ty + 0.5) * sny / tny - 0.5;
             long o = (long)floor(loc + 0.5);
@@ -487,6 +506,7 @@ void oracle_resize(const double *in, int channels, int snx, int sny, int tnx, in
                                             w * f[(size_t)y0 * snx + x] +
                                             wp * f[(size_t)yp * snx + x];
         }
+#pragma omp parallel for schedule(static) if (SP >= 8192)
         for (int tx = 0; tx < tnx; ++tx) {
             double loc = ((double)tx + 0.5) * snx / tnx - 0.5;
             long o = (long)floor(loc + 0.5);
@@ -584,6 +604,7 @@ void oracle_fmg2(const double *hes, const double *rhs, const double *params, int
                     RELAX(l);
                     size_t P = PLANE(lnx[l], lny[l]);
                     oracle_apply_membrane(A[l], U[l], lp[l], lnx[l], lny[l], R[l]);
+#pragma omp parallel for schedule(static) if (P >= 8192)
                     for (size_t p = 0; p < 2 * P; ++p)
                         R[l][p] = B[l][p] - R[l][p];
                     oracle_resize(R[l], 2, lnx[l], lny[l], lnx[l + 1], lny[l + 1], B[l + 1]);
@@ -593,6 +614,7 @@ void oracle_fmg2(const double *hes, const double *rhs, const double *params, int
                 for (int l = L - 2; l >= level; --l) {
                     size_t P = PLANE(lnx[l], lny[l]);
                     oracle_resize(U[l + 1], 2, lnx[l + 1], lny[l + 1], lnx[l], lny[l], R[l]);
+#pragma omp parallel for schedule(static) if (P >= 8192)
                     for (size_t p = 0; p < 2 * P; ++p)
                         U[l][p] = U[l][p] + R[l][p];
                     RELAX(l);
@@ -605,6 +627,7 @@ void oracle_fmg2(const double *hes, const double *rhs, const double *params, int
                 RELAX(l);
                 size_t P = PLANE(lnx[l], lny[l]);
                 oracle_apply_membrane(A[l], U[l], lp[l], lnx[l], lny[l], R[l]);
+#pragma omp parallel for schedule(static) if (P >= 8192)
                 for (size_t p = 0; p < 2 * P; ++p)
                     R[l][p] = B[l][p] - R[l][p];
                 oracle_resize(R[l], 2, lnx[l], lny[l], lnx[l + 1], lny[l + 1], B[l + 1]);
@@ -614,6 +637,7 @@ void oracle_fmg2(const double *hes, const double *rhs, const double *params, int
             for (int l = L - 2; l >= 0; --l) {
                 size_t P = PLANE(lnx[l], lny[l]);
                 oracle_resize(U[l + 1], 2, lnx[l + 1], lny[l + 1], lnx[l], lny[l], R[l]);
+#pragma omp parallel for schedule(static) if (P >= 8192)
                 for (size_t p = 0; p < 2 * P; ++p)
                     U[l][p] = U[l][p] + R[l][p];
                 RELAX(l);
@@ -677,8 +701,10 @@ void oracle_dartel_step(const double *src, const double *tgt, const double *v, i
     }
     oracle_regularization(v, params, 1, nx, ny, reg);
     double reg_obj = 0.0, norm = 0.0;
-    for (size_t p = 0; p < 2 * P; ++p) {
+#pragma omp parallel for schedule(static) if (P >= 8192)
+    for (size_t p = 0; p < 2 * P; ++p)
         b[p] = b[p] * scale + reg[p];
+    for (size_t p = 0; p < 2 * P; ++p) {
         reg_obj += reg[p] * v[p];
         norm += b[p] * b[p];
     }
@@ -686,6 +712,7 @@ void oracle_dartel_step(const double *src, const double *tgt, const double *v, i
         a[p] = a[p] * scale;
     double sp[5] = {params[0], params[1], params[2], params[3], params[4] + lmreg};
     oracle_fmg2(a, b, sp, cycles, nit, scratch, assume_nonzero, nx, ny, sol);
+#pragma omp parallel for schedule(static) collapse(2) if (P >= 8192)
     for (int c = 0; c < 2; ++c)
         for (int y = 0; y < ny; ++y)
             for (int x = 0; x < nx; ++x) {

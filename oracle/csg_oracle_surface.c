@@ -16,6 +16,7 @@
  * order.  src/cat_surface_gpu/surface_stencil.py:282-306 */
 void oracle_resample_f32(const float *x, const int *idx, const double *w, int n, float *out)
 {
+#pragma omp parallel for schedule(static) if (n >= 4096)
     for (int r = 0; r < n; ++r)
         for (int d = 0; d < 3; ++d) {
             float acc = 0.0f;
@@ -60,6 +61,7 @@ void oracle_smooth_geometry(const float *v, const int *off, const int *nbr, int 
     float *nxt = (float *)malloc(sizeof(float) * 3 * n);
     memcpy(cur, v, sizeof(float) * 3 * n);
     for (int it = 0; it < iters; ++it) {
+#pragma omp parallel for schedule(static) if (n >= 4096)
         for (int i = 0; i < n; ++i) {
             int deg = off[i + 1] - off[i];
             double den = 1.0 + (double)deg / 3.0;
@@ -83,6 +85,7 @@ void oracle_smooth_geometry(const float *v, const int *off, const int *nbr, int 
 void oracle_heat_weights(const double *geom, const int *off, const int *nbr, int n, double sigma,
                          int squared_norm, double *w)
 {
+#pragma omp parallel for schedule(static) if (n >= 4096)
     for (int i = 0; i < n; ++i)
         for (int k = off[i]; k < off[i + 1]; ++k) {
             int j = nbr[k];
@@ -106,6 +109,7 @@ void oracle_smooth_values(const double *vals, const int *off, const int *nbr, co
     double *nxt = (double *)malloc(sizeof(double) * n);
     memcpy(cur, vals, sizeof(double) * n);
     for (int it = 0; it < iters; ++it) {
+#pragma omp parallel for schedule(static) if (n >= 4096)
         for (int i = 0; i < n; ++i) {
             double s = 0.0, ws = 0.0;
             for (int k = off[i]; k < off[i + 1]; ++k) {
@@ -358,10 +362,13 @@ int oracle_rotation_costs(const float *src_pts, const double *src_vals, int ns,
                           const double *angles, int na, double *costs, int *loc_faces,
                           float *loc_bary)
 {
-    for (int a = 0; a < na; ++a) {
+    /* each point's squared difference in parallel, then the sum in point order */
+    double *d2 = (double *)malloc(sizeof(double) * (size_t)ns);
+    int failed = 0;
+    for (int a = 0; a < na && !failed; ++a) {
         float m[9];
         oracle_rotation_matrix_f32(angles + 3 * a, m);
-        double cost = 0.0;
+#pragma omp parallel for schedule(static) if (ns >= 4096) reduction(| : failed)
         for (int i = 0; i < ns; ++i) {
             const float *p = src_pts + 3 * i;
             float q[3];
@@ -369,26 +376,31 @@ int oracle_rotation_costs(const float *src_pts, const double *src_vals, int ns,
                 q[r] = p[0] * m[3 * r] + p[1] * m[3 * r + 1] + p[2] * m[3 * r + 2];
             int cell = cell_of(q, grid);
             int nc = cell_off[cell + 1] - cell_off[cell];
-            if (nc < 1)
-                return -1;
-            float w[3];
-            int f = oracle_locate(q, tri, cell_faces + cell_off[cell], nc, w);
-            if (f < 0)
-                return -1;
+            float w[3] = {0.0f, 0.0f, 0.0f};
+            int f = nc < 1 ? -1 : oracle_locate(q, tri, cell_faces + cell_off[cell], nc, w);
+            if (f < 0) {
+                failed |= 1;
+                d2[i] = 0.0;
+                continue;
+            }
             double s = tgt_vals[faces[3 * f]] * (double)w[0] +
                        tgt_vals[faces[3 * f + 1]] * (double)w[1] +
                        tgt_vals[faces[3 * f + 2]] * (double)w[2];
             double d = src_vals[i] - s;
-            cost += d * d;
+            d2[i] = d * d;
             if (loc_faces) {
                 loc_faces[(size_t)a * ns + i] = f;
                 for (int k = 0; k < 3; ++k)
                     loc_bary[((size_t)a * ns + i) * 3 + k] = w[k];
             }
         }
+        double cost = 0.0;
+        for (int i = 0; i < ns; ++i)
+            cost += d2[i];
         costs[a] = cost;
     }
-    return 0;
+    free(d2);
+    return failed ? -1 : 0;
 }
 
 /* ------------------------------------------------------------ closest point */
@@ -406,7 +418,7 @@ double oracle_closest_on_triangle(const double *p, const double *a, const double
     }
 #define DOT(u, v) ((u)[0] * (v)[0] + (u)[1] * (v)[1] + (u)[2] * (v)[2])
     double d1 = DOT(ab, ap), d2 = DOT(ac, ap);
-    double u, v, t;
+    double v, t;
     if (d1 <= 0.0 && d2 <= 0.0) {
         w[0] = 1; w[1] = 0; w[2] = 0;
     } else {

@@ -11,6 +11,7 @@ Build: ``make -C oracle`` (``__graft_entry__.build()`` runs it).
 from __future__ import annotations
 
 import ctypes
+import hashlib
 import math
 from pathlib import Path
 
@@ -347,17 +348,29 @@ def rotate_vertices(vertices, angles) -> np.ndarray:
 # ---------------------------------------------------------------- surface side
 
 
+_CSR_CACHE: dict = {}
+
+
 def neighbour_csr(faces, n_points: int):
     """Ascending neighbour lists per vertex (surface_stencil.py:36-64)."""
 
-    f = np.asarray(faces, dtype=np.int64)
-    directed = np.concatenate([f[:, [0, 1]], f[:, [1, 0]], f[:, [1, 2]], f[:, [2, 1]],
-                               f[:, [2, 0]], f[:, [0, 2]]])
-    directed = np.unique(directed, axis=0)  # sorted by (source, target)
-    counts = np.bincount(directed[:, 0], minlength=n_points)
+    f = np.ascontiguousarray(np.asarray(faces, dtype=np.int64))
+    key = (hashlib.sha1(f.tobytes()).hexdigest(), f.shape, int(n_points))
+    hit = _CSR_CACHE.get(key)
+    if hit is not None:
+        return hit
+    src = np.concatenate([f[:, 0], f[:, 1], f[:, 1], f[:, 2], f[:, 2], f[:, 0]])
+    dst = np.concatenate([f[:, 1], f[:, 0], f[:, 2], f[:, 1], f[:, 0], f[:, 2]])
+    # unique directed edges sorted by (source, target), as one integer key each
+    pairs = np.unique(src * np.int64(n_points) + dst)
+    counts = np.bincount(pairs // n_points, minlength=n_points)
     off = np.zeros(n_points + 1, np.int32)
     off[1:] = np.cumsum(counts)
-    return off, np.ascontiguousarray(directed[:, 1].astype(np.int32))
+    out = (off, np.ascontiguousarray((pairs % n_points).astype(np.int32)))
+    if len(_CSR_CACHE) > 16:
+        _CSR_CACHE.clear()
+    _CSR_CACHE[key] = out
+    return out
 
 
 def resample_vertices_f32(x, idx, w):
