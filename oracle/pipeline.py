@@ -259,6 +259,17 @@ class RotationResult:
     iterations: int
 
 
+def rotation_cost(target: Stencil, source_points, source_values, target_values, angles, *,
+                  grid_size: int = 128, margin: int = 1) -> np.ndarray:
+    """The reference's rotation SSD (surface_rotation.py:512-546) of angle triples."""
+
+    index = grid_index(target.sphere64, target.faces, grid_size, margin)
+    return O.rotation_costs(np.asarray(source_points, dtype=np.float32), source_values,
+                            target_values, target.sphere64, target.faces,
+                            np.asarray(angles, dtype=np.float64).reshape(-1, 3), grid_size,
+                            margin, index=index)
+
+
 def search_rotation(target: Stencil, source_points, source_values, target_values, *,
                     grid_size: int = 128, margin: int = 1, refine: bool = True,
                     max_iter: int = 500, tol: float = 1e-4,
@@ -323,6 +334,7 @@ class SurfWarpResult:
     features: tuple[np.ndarray, np.ndarray]
     flows: list = field(default_factory=list)
     stage_seconds: dict = field(default_factory=dict)
+    stencils: tuple = ()              # (source, target) stencils of the input spheres
 
 
 def surfwarp(source_surface, source_sphere, source_faces, target_surface, target_sphere,
@@ -401,7 +413,7 @@ def surfwarp(source_surface, source_sphere, source_faces, target_surface, target
         current = timed("warp", O.average_xz_surfaces, avg_warped @ unrotate.T, main_warped)
     return SurfWarpResult(vertices=np.ascontiguousarray(current, dtype=np.float32),
                           rotation=rot, features=(src_vals, tgt_vals), flows=flows,
-                          stage_seconds=stage)
+                          stage_seconds=stage, stencils=(src_st, tgt_st))
 
 
 def timed_surfwarp(*args, **kwargs):

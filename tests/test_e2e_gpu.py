@@ -84,8 +84,10 @@ def _errors(a, b):
 
 
 # FP64 output spheres are float32 arrays: a few float32 ulps (6e-8 at unit radius)
-# apart at best; the bound also covers float32 rounding of the intermediate spheres.
-TOLERANCE = {"float32": 1e-3, "float64": 1e-5}
+# apart at best, as are the intermediate spheres the stencils are built from
+# (measured on MI355X: 8.9e-8 / 2.0e-7 / 1.5e-7 for FP64; 1.7e-6 / 1.5e-4 / 9.9e-5 for
+# FP32 against the 1e-3 bar).
+TOLERANCE = {"float32": 1e-3, "float64": 1e-6}
 
 
 @pytest.mark.parametrize("dtype", ["float64", "float32"])
@@ -101,16 +103,25 @@ def test_surfwarp_matches_oracle_chain(name, dtype, cuda):
     print(f"[{name}] {dtype}: vertex error max {mx:.3e} mean {mean:.3e} p99 {p99:.3e}; "
           f"rotation |d angle| {dang:.2e} rad, cost {res.rotation_cost:.6f} vs "
           f"{ref.rotation.cost:.6f}", flush=True)
-    # the rotation is the same Nelder-Mead trajectory up to cost rounding
-    assert dang <= 1e-4
+    # the same 46-seed scan and Nelder-Mead trajectory (measured: identical angles)
+    assert dang <= 1e-12
     assert mx <= TOLERANCE[dtype]
 
 
 def test_per_step_velocities_float32_and_oracle_512x256(cuda):
     """Every Gauss-Newton step of both 512x256 solves of the 61,442-vertex pair's
-    first DARTEL run: the float32 velocity within relative L2 1e-4 of the float64
-    one, and the float64 one within 1e-9 of the oracle's, each trajectory
-    advancing from its own previous velocity."""
+    first DARTEL run (sheets from the oracle chain's stencils).
+
+    - float64 against the oracle, each trajectory from its own previous velocity:
+      equal to 1e-12 relative L2 (measured: bit-identical);
+    - float32 per step: one float32 step taken from the float64 trajectory's
+      velocity against the float64 step, relative L2 <= 1e-4 (measured <= 1.4e-5);
+    - float32 trajectory against the float64 one: <= 1e-3 (measured 3.5e-4).  The
+      step is discontinuous in v where a sample crosses a pixel boundary (the
+      bilinear derivative of dartel_grid.py:758-761 jumps there): the float64 step
+      alone maps the 1e-5-pixel difference of the two trajectories' velocities
+      after one step to a 0.04-pixel change at one pixel (1.3e-4 relative), while
+      random 1e-7 perturbations decay."""
 
     import cat_surface_gpu as csg
     from oracle import oracle as O
@@ -123,26 +134,115 @@ def test_per_step_velocities_float32_and_oracle_512x256(cuda):
     src = P.type5_maps(p.src_surface, src_st, fwhm)
     tgt = P.type5_maps(p.tgt_surface, tgt_st, fwhm)
     params = O.default_dartel_parameters(6)
-    worst32 = worst64 = 0.0
+    f32, f64 = torch.float32, torch.float64
+    local = accumulated = oracle_gap = 0.0
     for s in range(2):
         maps = {dt: (torch.as_tensor(src[s], dtype=dt, device=cuda),
-                     torch.as_tensor(tgt[s], dtype=dt, device=cuda))
-                for dt in (torch.float32, torch.float64)}
-        v = {dt: torch.zeros((2, 256, 512), dtype=dt, device=cuda)
-             for dt in (torch.float32, torch.float64)}
+                     torch.as_tensor(tgt[s], dtype=dt, device=cuda)) for dt in (f32, f64)}
+        v = {dt: torch.zeros((2, 256, 512), dtype=dt, device=cuda) for dt in (f32, f64)}
         vo = np.zeros((2, 256, 512))
         for k in range(6):
+            kw = dict(k=k, params=tuple(params[k]), device=cuda)
             for _ in range(3):
-                for dt in (torch.float32, torch.float64):
-                    v[dt], _ = csg.dartel_step(*maps[dt], v[dt], k=k, params=tuple(params[k]),
-                                               device=cuda, dtype=dt)
+                one, _ = csg.dartel_step(*maps[f32], v[f64].to(f32), dtype=f32, **kw)
+                v[f32], _ = csg.dartel_step(*maps[f32], v[f32], dtype=f32, **kw)
+                v[f64], _ = csg.dartel_step(*maps[f64], v[f64], dtype=f64, **kw)
                 vo, _ = O.dartel_step(src[s], tgt[s], vo, k, tuple(params[k]))
-                v64 = v[torch.float64].cpu().numpy()
-                v32 = v[torch.float32].double().cpu().numpy()
+                v64 = v[f64].cpu().numpy()
                 scale = np.linalg.norm(v64)
-                worst32 = max(worst32, float(np.linalg.norm(v32 - v64) / scale))
-                worst64 = max(worst64, float(np.linalg.norm(v64 - vo) / scale))
-    print(f"\nper-step velocity: float32 vs float64 worst rel L2 {worst32:.2e}; float64 vs "
-          f"oracle {worst64:.2e}", flush=True)
-    assert worst32 <= 1e-4
-    assert worst64 <= 1e-9
+                local = max(local, float(np.linalg.norm(one.double().cpu().numpy() - v64) / scale))
+                accumulated = max(accumulated, float(
+                    np.linalg.norm(v[f32].double().cpu().numpy() - v64) / scale))
+                oracle_gap = max(oracle_gap, float(np.linalg.norm(v64 - vo) / scale))
+    print(f"\nper-step velocity (rel L2, worst of 36 steps): float32 step {local:.2e}, float32 "
+          f"trajectory {accumulated:.2e}, float64 vs oracle {oracle_gap:.2e}", flush=True)
+    assert oracle_gap <= 1e-12
+    assert local <= 1e-4
+    assert accumulated <= 1e-3
+
+
+@pytest.mark.parametrize("name", ["fibonacci61442-lh", "fibonacci61442-rh"])
+def test_hopf_grid_search_matches_or_beats_the_oracle_rotation(name, cuda):
+    """8,192 Hopf-grid candidates scored in one launch, the best 8 refined by 20
+    gradient steps (the north star's batched rotation search), on the 61,442-vertex
+    pair: the chosen rotation's SSD, costed by the oracle in the reference's
+    formulation on the oracle's stencils and features, is no more than the
+    oracle's 46-seed + Nelder-Mead SSD x (1 + 1e-6)."""
+
+    from cat_surface_gpu import GiftiMesh, run_cat_surface_gpu_pipeline
+    from oracle import pipeline as P
+
+    ref = _oracle(name)
+    p = _pair(name)
+    meshes = (GiftiMesh(p.src_surface, p.src_faces), GiftiMesh(p.src_sphere, p.src_faces),
+              GiftiMesh(p.tgt_surface, p.tgt_faces), GiftiMesh(p.tgt_sphere, p.tgt_faces))
+    res = run_cat_surface_gpu_pipeline(None, None, None, None, None, meshes=meshes,
+                                       write_output=False, device=cuda, dartel_dtype="float32",
+                                       rotation_method="adjoint", rotation_candidates="hopf",
+                                       rotation_top=8, rotation_steps=20,
+                                       **WORKLOADS[name]["run"])
+    src_st, tgt_st = ref.stencils
+    cost = float(P.rotation_cost(tgt_st, src_st.sphere64, *ref.features,
+                                 [res.rotation_angle])[0])
+    print(f"\n[{name}] Hopf 8,192 + 8x20 steps: oracle-costed SSD {cost:.6f} vs oracle "
+          f"Nelder-Mead {ref.rotation.cost:.6f} (ratio {cost / ref.rotation.cost:.8f}); "
+          f"GPU-reported {res.rotation_cost:.6f}", flush=True)
+    assert cost <= ref.rotation.cost * (1.0 + 1e-6)
+    assert abs(res.rotation_cost - cost) <= 1e-6 * cost
+
+
+UNEVEN = (("icosphere", 10242, 5, (128, 64)), ("icosphere", 40962, 6, (256, 128)),
+          ("fibonacci", 61442, 6, (512, 256)), ("icosphere", 163842, 7, (512, 256)))
+
+
+@pytest.mark.parametrize("method", ["nelder-mead", "adjoint"])
+def test_uneven_batched_schedule_against_the_oracle_chain(method, cuda):
+    """Four full registrations of uneven mesh and map sizes (10,242 / 40,962 /
+    61,442 / 163,842 vertices, 128x64 / 256x128 / 512x256 / 512x256 maps; steps=2
+    runs=2 avg) in one batched schedule with tuned relaxation tiles, float32.
+    Every job's output is within 1e-3 of its own oracle chain, and its chosen
+    rotation is no worse than the oracle's (SSD within 1e-6, costed by the
+    oracle), with the reference's Nelder-Mead and with the adjoint search."""
+
+    from cat_surface_gpu import GiftiMesh, PipelineJob, run_cat_surface_gpu_pipelines
+    from cat_surface_gpu import synthetic
+    from oracle import pipeline as P
+
+    jobs, refs, pairs = [], [], []
+    for k, (kind, n, sub, size) in enumerate(UNEVEN):
+        key = ("uneven", k)
+        if key not in _CACHE:
+            p = synthetic.make_pair(kind, n, subdivisions=sub, lmax=16, noise=0.05,
+                                    max_rotation_deg=30.0, warp_deg=3.0, seed=SEED + 1000 + k)
+            ref, wall, _ = P.timed_surfwarp(p.src_surface, p.src_sphere, p.src_faces,
+                                            p.tgt_surface, p.tgt_sphere, p.tgt_faces,
+                                            steps=2, runs=2, avg=True, map_size=size)
+            print(f"\njob {k}: {n} vertices, {size[0]}x{size[1]}: oracle chain {wall:.1f} s",
+                  flush=True)
+            _CACHE[key] = (p, ref)
+        p, ref = _CACHE[key]
+        pairs.append(p)
+        refs.append(ref)
+        jobs.append(PipelineJob(meshes=(GiftiMesh(p.src_surface, p.src_faces),
+                                        GiftiMesh(p.src_sphere, p.src_faces),
+                                        GiftiMesh(p.tgt_surface, p.tgt_faces),
+                                        GiftiMesh(p.tgt_sphere, p.tgt_faces)), map_size=size))
+    out = run_cat_surface_gpu_pipelines(jobs, device=cuda, dartel_dtype="float32", steps=2,
+                                        runs=2, avg=True, write_output=False, cuda_graph=True,
+                                        autotune=True, rotation_method=method)
+    worst = []
+    for k, (p, ref, res) in enumerate(zip(pairs, refs, out)):
+        np.testing.assert_array_equal(res.faces, p.src_faces)
+        mx, mean, p99 = _errors(res.vertices, ref.vertices)
+        src_st, tgt_st = ref.stencils
+        cost = float(P.rotation_cost(tgt_st, src_st.sphere64, *ref.features,
+                                     [res.rotation_angle])[0])
+        print(f"[{method}] job {k}: vertex error max {mx:.3e} mean {mean:.3e} p99 {p99:.3e}; "
+              f"SSD {cost:.6f} vs oracle {ref.rotation.cost:.6f}", flush=True)
+        assert cost <= ref.rotation.cost * (1.0 + 1e-6)
+        # measured on MI355X: 1.6e-4 (Nelder-Mead, the oracle's own rotation) and
+        # 5.2e-4 (adjoint search: a slightly better rotation than the oracle's)
+        assert mx <= 1e-3
+        worst.append(mx)
+    print(f"[{method}] per-job max vertex error: p50 {np.percentile(worst, 50):.3e}, "
+          f"p99 {np.percentile(worst, 99):.3e}", flush=True)
