@@ -26,7 +26,8 @@ import torch
 
 from . import _lib
 from .dartel_grid import resolve_device
-from .gifti import read_gifti, write_gifti_surface, INTENT_POINTSET, INTENT_TRIANGLE
+from .gifti import (INTENT_POINTSET, INTENT_TRIANGLE, read_gifti, read_gifti_intents,
+                    read_gifti_many, write_gifti_surface)
 from .rotation_feature import compute_rotation_features
 from .rotation_pipeline import RotationPipeline, read_rotation_values
 from .locate import check_all
@@ -55,7 +56,10 @@ class CatSurfaceGpuResult:
 def read_gifti_mesh(path: str | Path) -> GiftiMesh:
     """First point-set and triangle arrays of a GIFTI surface (gpu_pipeline.py:52-71)."""
 
-    arrays = read_gifti(path)
+    return _mesh_from_arrays(read_gifti(path), path)
+
+
+def _mesh_from_arrays(arrays, path) -> GiftiMesh:
     if len(arrays) < 2:
         raise ValueError(f"GIFTI file must contain vertex and face arrays: {path}")
     vertices, faces = arrays[0].data, arrays[1].data
@@ -73,11 +77,11 @@ def write_gifti_mesh(path: str | Path, mesh: GiftiMesh, *,
 
     point_intent, triangle_intent = INTENT_POINTSET, INTENT_TRIANGLE
     if reference_path is not None:
-        arrays = read_gifti(reference_path)
-        if len(arrays) >= 1:
-            point_intent = arrays[0].intent
-        if len(arrays) >= 2:
-            triangle_intent = arrays[1].intent
+        intents = read_gifti_intents(reference_path)
+        if len(intents) >= 1:
+            point_intent = intents[0]
+        if len(intents) >= 2:
+            triangle_intent = intents[1]
     write_gifti_surface(path, mesh.vertices, mesh.faces, point_intent=point_intent,
                         triangle_intent=triangle_intent)
 
@@ -343,12 +347,17 @@ def run_cat_surface_gpu_pipelines(
     t_total = time.perf_counter()
     shared: dict[str, float] = {}
     states: list[_JobState] = []
+    # every input file of every job read together (files and arrays in parallel threads)
+    paths = [p for job in jobs if job.meshes is None
+             for p in (job.source_surface, job.source_sphere, job.target_surface,
+                       job.target_sphere)]
+    arrays = iter(read_gifti_many(paths)) if paths else iter(())
+    path_iter = iter(paths)
     for job in jobs:
         if job.meshes is not None:
             meshes = job.meshes
         else:
-            meshes = (read_gifti_mesh(job.source_surface), read_gifti_mesh(job.source_sphere),
-                      read_gifti_mesh(job.target_surface), read_gifti_mesh(job.target_sphere))
+            meshes = tuple(_mesh_from_arrays(next(arrays), next(path_iter)) for _ in range(4))
         states.append(_JobState(job, *meshes, timings={}))
     shared["input_read_seconds"] = time.perf_counter() - t_total
 

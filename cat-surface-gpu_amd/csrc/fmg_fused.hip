@@ -21,6 +21,7 @@
 
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 namespace csg {
 
@@ -211,11 +212,16 @@ k_restrict_residual(const T *__restrict__ A, const T *__restrict__ B, const T *_
 //
 // One workgroup per job holds levels c0 .. L-1 in LDS (per level: A 3 planes,
 // b 2, u 2) plus a two-plane residual buffer of level c0's size.  The schedule
-// is the reference's (dartel_grid.py:1341-1413).  Every step is one call of a
-// single non-inlined function (coarse_op) with plain scalar arguments, so the
-// kernel's code stays small whatever the number of levels and cycles; inside
-// it every access is an LDS offset and every loop strides by the fixed block
-// size, so a step costs no global memory access.
+// is the reference's (dartel_grid.py:1341-1413).  Every step is coarse_op
+// with plain scalar arguments (level sizes, LDS offsets), inlined at each of
+// its few call sites: inside it every access is an LDS offset and every loop
+// strides by the fixed block size, so a step costs no global memory access.
+// (As a called function its frame kept callee-saved registers in scratch
+// memory, and a run-time index into the relaxation's register arrays put them
+// there too: 78.5 -> 55.5 us per launch in float32, 106 -> 45 us in float64.
+// Running the levels of <= 128 pixels on one wave without workgroup barriers
+// measured slower again: the steps wait on their dependent LDS and arithmetic
+// chains, not on the barrier.)
 
 extern __shared__ __align__(16) unsigned char g_lds[];
 
@@ -259,7 +265,7 @@ __device__ __forceinline__ void lds_resize(const T *in, int snx, int sny, T *out
 // level (onx, ony, ooff) is l+1 for kDown / kUpAdd / kProlong and l-1 for the
 // pyramid steps.  Ends with a workgroup barrier.
 template <typename T>
-__device__ __noinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w00, T w01, T w10,
+__device__ __forceinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w00, T w01, T w10,
                                        int onx, int ony, unsigned ooff, unsigned roff,
                                        unsigned tx_off, unsigned ty_off, int nit)
 {
@@ -303,20 +309,35 @@ __device__ __noinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w
                 rby[c] = b[P + p];
             }
         }
+        // the half-sweeps in colour pairs (1, 0), so that every index into the
+        // register arrays is a compile-time constant (a run-time index would
+        // place them in scratch memory)
+        auto in_regs_sweep = [&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            if (rok[c]) {
+                const int p = rp[c], xm = rxm[c], xp = rxp[c], ym = rym[c], yp = ryp[c];
+                const Stencil<T> st{u[p],           u[P + p],           u[ym] + u[yp],
+                                    u[xm] + u[xp], u[P + ym] + u[P + yp], u[P + xm] + u[P + xp]};
+                const Pair<T> v = relax_solve<T>(rbx[c], rby[c], st, rk[c], w01, w10);
+                u[p] = v.x;
+                u[P + p] = v.y;
+            }
+            __syncthreads();
+        };
+        if (in_regs && !odd) {
+            for (int it = 0; it < nit; ++it) {
+                in_regs_sweep(std::integral_constant<int, 1>{});
+                in_regs_sweep(std::integral_constant<int, 0>{});
+            }
+            return;
+        }
         for (int it = 0; it < 2 * nit; ++it) {
             const int colour = (it % 2 == 0) ? 1 : 0;
             if (in_regs) {
-                if (rok[colour]) {
-                    const int p = rp[colour], xm = rxm[colour], xp = rxp[colour],
-                              ym = rym[colour], yp = ryp[colour];
-                    const Stencil<T> st{u[p],           u[P + p],           u[ym] + u[yp],
-                                        u[xm] + u[xp], u[P + ym] + u[P + yp], u[P + xm] + u[P + xp]};
-                    const Pair<T> v =
-                        relax_solve<T>(rbx[colour], rby[colour], st, rk[colour], w01, w10);
-                    u[p] = v.x;
-                    u[P + p] = v.y;
-                }
-                __syncthreads();
+                if (colour == 1)
+                    in_regs_sweep(std::integral_constant<int, 1>{});
+                else
+                    in_regs_sweep(std::integral_constant<int, 0>{});
             }
             for (int phase = in_regs ? 1 : 0; phase < (odd ? 2 : 1); ++phase) {
                 const int count = phase == 0 ? half * ny : ny;
