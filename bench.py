@@ -8,12 +8,16 @@ and smoothly warped, registered with steps=2 runs=2 avg, for both the LH and
 the x-mirrored RH copy.  One timed step = LH + RH.  Inputs are resident in
 memory (no file I/O) when the timed region starts.
 
-Prints one JSON line (rank 0).  Besides the wall-clock value it reports the
-roofline of the dominant hot-path kernel (the timed kernel with the largest
-total time): the library brackets each of its launches inside the timed
-region with HIP events on the stream it runs on and accumulates the launch's
-algorithmic bytes.  A CPU baseline from the project's C restatement of the
-DARTEL step (oracle/) is timed on a bounded sample on the host.
+Prints one JSON line (rank 0).  Besides the wall-clock value it reports:
+- the roofline of the dominant hot-path kernel (the timed kernel with the
+  largest total time) in one un-captured step of the timed mode run right
+  after the timed region: the library brackets each launch with HIP events on
+  the stream it runs on and accumulates the launch's algorithmic bytes (graph
+  replays, which the timed region uses, hide their kernels from host events);
+- the same mode with GIFTI input and output around each step;
+- the plain method on the same inputs;
+- a CPU baseline: one full hemisphere on the project's CPU restatement
+  (oracle/pipeline.py), wall and CPU time, cores stated.
 """
 
 from __future__ import annotations
@@ -32,11 +36,20 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch of kernels whose traffic counters were collected at the
-# bench's shape: two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE), each scaled
-# by what the counter reads for a 64 MiB copy.  Source: the named profile.
+# bench's shape: two rocprofv3 --pmc runs of this bench (FETCH_SIZE, WRITE_SIZE),
+# each scaled by what the counter reads for a 64 MiB copy, averaged over the run's
+# dispatches.  Source: the named profile.
+_TRAFFIC_PROFILE = "profiles/hbm_traffic_fibonacci61442_f32.md"
 MEASURED_TRAFFIC = {
-    "k_sor_patched": (35.19e6, "profiles/k_sor_patched_depth4x40962.md "
-                               "(four 40,962-vertex meshes, 1000 sweeps)"),
+    "k_relax_tile": (29.46e6, _TRAFFIC_PROFILE),
+    "k_fmg_coarse": (0.81e6, _TRAFFIC_PROFILE),
+    "k_restrict_residual": (22.34e6, _TRAFFIC_PROFILE),
+    "k_squaring": (123.67e6, _TRAFFIC_PROFILE),
+    "k_objective": (84.46e6, _TRAFFIC_PROFILE),
+    "k_compose_self": (55.87e6, _TRAFFIC_PROFILE),
+    "k_combine": (67.83e6, _TRAFFIC_PROFILE),
+    "k_smooth_geometry_patch": (27.52e6, _TRAFFIC_PROFILE),
+    "k_sor_patched": (35.57e6, _TRAFFIC_PROFILE),
 }
 
 
@@ -137,33 +150,28 @@ def kernel_rooflines() -> list[dict]:
     return sorted(out, key=lambda r: -r["total_ms"])
 
 
-def cpu_baseline() -> dict:
-    """Project C restatement (oracle) of dartel_step at 512x256 float64, one step
-    per k = 0..5, single thread; scaled to the 108 dartel steps of one hemisphere
-    (3 solves x 2 steps x 6 loops x 3 its).  DARTEL only: a lower bound of a full
-    CPU hemisphere."""
-
-    import numpy as np
+def cpu_baseline(pair) -> dict:
+    """The project's CPU restatement (oracle/pipeline.py) of one full hemisphere:
+    the same left-hemisphere inputs as the timed GPU step, all stages (stencils,
+    rotation features, 46 seeds + Nelder-Mead, three float64 DARTEL solves with
+    their curvature sheets, warps, averaging), on the host's cores (OpenMP over
+    the oracle's elementwise loops; the float64 sums stay sequential).  Wall time,
+    and user + system CPU time from getrusage."""
 
     from oracle import oracle as O
+    from oracle import pipeline as P
 
-    nx, ny = 512, 256
-    x = (np.arange(nx) + 0.5)[None, :] / nx
-    y = (np.arange(ny) + 0.5)[:, None] / ny
-    src = 0.5 + 0.3 * np.sin(2 * np.pi * 3 * x) * np.cos(2 * np.pi * y)
-    tgt = np.roll(src, 3, axis=1)
-    v = np.zeros((2, ny, nx))
-    per_k = []
-    params = O.default_dartel_parameters(6)
-    for k in range(6):
-        t = time.perf_counter()
-        v, _ = O.dartel_step(src, tgt, v, k, params[k])
-        per_k.append(time.perf_counter() - t)
-    seconds = 3 * 2 * 3 * sum(per_k)
-    return {"value": round(seconds, 3), "unit": "s/hemisphere (DARTEL stage only)", "cores": 1,
-            "kind": "port",
-            "sample": "oracle dartel_step 512x256 fp64, one step for each k=0..5 "
-                      f"({sum(per_k):.2f} s), x18 for 3 solves x 2 steps x 3 its"}
+    src_surface, src_sphere, tgt_surface, tgt_sphere = pair
+    res, wall, cpu = P.timed_surfwarp(src_surface.vertices, src_sphere.vertices,
+                                      src_sphere.faces, tgt_surface.vertices,
+                                      tgt_sphere.vertices, tgt_sphere.faces, steps=2, runs=2,
+                                      avg=True)
+    threads = int(O.lib().oracle_max_threads())
+    return {"value": round(wall, 3), "unit": "s/hemisphere", "cores": threads, "kind": "port",
+            "cpu_seconds": round(cpu, 2),
+            "stages_s": {k: round(v, 3) for k, v in res.stage_seconds.items()},
+            "sample": "one full CAT_SurfWarp hemisphere (LH of the timed pair; steps=2 runs=2 "
+                      "avg; float64) on the project CPU restatement, inputs in memory"}
 
 
 def _pair_meshes(p):
@@ -340,6 +348,68 @@ def sor_traffic_run(device, copies: int = 3) -> dict:
             "solve_s": round(solve_s, 4), "copy_bytes": nbytes, "copies": copies}
 
 
+def gifti_io_leg(pairs, args, device) -> dict:
+    """The timed mode with GIFTI input and output: the four input surfaces of each
+    hemisphere are written as GZipBase64Binary files (zlib level 6, nibabel's
+    default) before timing; each timed step reads them, registers LH and RH and
+    writes both output spheres, with the reader's caches cleared first."""
+
+    import tempfile
+
+    import torch
+
+    from cat_surface_gpu import PipelineJob, run_cat_surface_gpu_pipelines, gifti, synthetic
+    from cat_surface_gpu.synthetic import SyntheticPair
+
+    folder = Path(tempfile.mkdtemp(prefix="csg_bench_"))
+    jobs = []
+    for k, (ss, sp, ts, tp) in enumerate(pairs):
+        pair = SyntheticPair(sp.vertices, ss.vertices, sp.faces, tp.vertices, ts.vertices,
+                             tp.faces, None)
+        paths = synthetic.write_pair(pair, folder / f"h{k}", compress_level=6)
+        jobs.append(PipelineJob(paths["src_surface"], paths["src_sphere"], paths["tgt_surface"],
+                                paths["tgt_sphere"], folder / f"h{k}" / "out.sphere.reg.gii"))
+    kw = dict(device=device, dartel_dtype=args.dtype, steps=2, runs=2, avg=True,
+              write_output=True, cuda_graph=args.cuda_graph, autotune=True,
+              rotation_method="adjoint")
+    for _ in range(max(1, args.warmup)):
+        run_cat_surface_gpu_pipelines(jobs, **kw)
+    torch.cuda.synchronize()
+    sums: dict[str, float] = {}
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        gifti.clear_caches()
+        res = run_cat_surface_gpu_pipelines(jobs, **kw)
+        for key in ("input_read_seconds", "output_write_seconds"):
+            sums[key] = sums.get(key, 0.0) + res[0].timings[key]
+    torch.cuda.synchronize()
+    value = (time.perf_counter() - t) / (args.steps * len(jobs))
+    log("GIFTI leg done")
+    return {"value": round(value, 4), "unit": "s/hemisphere",
+            "definition": "read the 4 GIFTI inputs of each hemisphere, register, write the "
+                          "output sphere (GZipBase64Binary), LH+RH per step",
+            "stages_s_per_hemisphere": {k: round(v / (args.steps * len(jobs)), 4)
+                                        for k, v in sums.items()}}
+
+
+def instrumented_rooflines(pairs, args, device) -> list[dict]:
+    """Per-kernel HIP-event timing of one step of the timed mode launched without
+    graph capture, right after the timed region (a graph replay's kernels cannot
+    be bracketed by host-side events): the same kernels on the same inputs."""
+
+    import torch
+
+    from cat_surface_gpu import _lib
+
+    kernel_rooflines()  # clear
+    torch.cuda.synchronize()
+    _lib.lib().csg_kernel_timing(1)
+    run_step(pairs, args.dtype, args.mode, device, False)
+    torch.cuda.synchronize()
+    _lib.lib().csg_kernel_timing(0)
+    return kernel_rooflines()
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -350,6 +420,8 @@ def main() -> int:
                     default="batched+adjoint")
     ap.add_argument("--seed", type=int, default=20261205)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-gifti-io", action="store_true",
+                    help="skip the leg that reads and writes GIFTI files around each step")
     ap.add_argument("--compare-plain", action=argparse.BooleanOptionalAction, default=True,
                     help="after the timed mode, time the plain path on the same inputs and "
                          "report it with the largest vertex difference between the two")
@@ -384,11 +456,7 @@ def main() -> int:
         run_step(pairs, args.dtype, args.mode, device, args.cuda_graph)
         torch.cuda.synchronize()
         log(f"warmup {i} done")
-    from cat_surface_gpu import _lib
-
-    kernel_rooflines()  # clear
     torch.cuda.synchronize()
-    _lib.lib().csg_kernel_timing(1)
     t0 = time.perf_counter()
     stage_sums: dict[str, float] = {}
     for i in range(args.steps):
@@ -400,12 +468,13 @@ def main() -> int:
         log(f"step {i} done")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    _lib.lib().csg_kernel_timing(0)
     ms_per_step = elapsed / args.steps * 1e3
     hemispheres = 2
     value = elapsed / (args.steps * hemispheres)
-    kernels = kernel_rooflines()
-    roof = kernels[0] if kernels else None
+    kernels = instrumented_rooflines(pairs, args, device)
+    log("instrumented step done")
+    roof = dict(kernels[0], source="HIP events on the kernel's stream, one un-captured step "
+                                   "of the timed mode after the timed region") if kernels else None
     plain = None
     if args.compare_plain and args.mode not in ("plain", "plain-fused"):
         plain = {}
@@ -425,7 +494,12 @@ def main() -> int:
                           "max_vertex_delta": delta}
             log(f"{leg} comparison done")
         plain["definition"] = PLAIN_DEFINITION
-    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    gifti_leg = None if args.no_gifti_io else gifti_io_leg(pairs, args, device)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(pairs[0])
+        cpu["gpu_speedup"] = round(cpu["value"] / value, 1)
+        log("CPU baseline done")
     stages = {k: round(v / (args.steps * hemispheres), 4) for k, v in sorted(stage_sums.items())}
     line = {
         "metric": "s/hemisphere CAT_SurfWarp 61,442 verts (steps=2 runs=2 avg)",
@@ -444,6 +518,7 @@ def main() -> int:
                                "avg, 512x256 map", "mode": args.mode,
                    "parallelism": "single GPU", "cuda_graph": args.cuda_graph},
         "plain": plain,
+        "with_gifti_io": gifti_leg,
         "relax_tiles": _installed_tiles(),
         "roofline": roof,
         "roofline_kernels": kernels,

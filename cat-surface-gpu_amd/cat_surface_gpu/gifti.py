@@ -162,13 +162,14 @@ def read_gifti_intents(path: str | Path) -> list[int]:
     return out
 
 
-def _payload(arr: np.ndarray, code: str) -> str:
+def _payload(arr: np.ndarray, code: str, level: int | None = None) -> str:
     key = _payload_key(arr, code)
-    hit = _PAYLOADS.get(key)
-    if hit is not None:
-        return hit
+    if level is None:
+        hit = _PAYLOADS.get(key)
+        if hit is not None:
+            return hit
+        level = 0 if code.startswith("f") else 1
     data = np.ascontiguousarray(arr.astype("<" + code))
-    level = 0 if code.startswith("f") else 1
     text = base64.b64encode(zlib.compress(data.tobytes(), level)).decode("ascii")
     _remember(_PAYLOADS, key, text)
     return text
@@ -189,12 +190,18 @@ def _encode(arr: np.ndarray, intent: int, code: str, payload: str | None = None)
 
 def write_gifti_surface(path: str | Path, vertices: np.ndarray, faces: np.ndarray, *,
                         point_intent: int = INTENT_POINTSET,
-                        triangle_intent: int = INTENT_TRIANGLE) -> None:
+                        triangle_intent: int = INTENT_TRIANGLE,
+                        compress_level: int | None = None) -> None:
+    """Write a surface as GZipBase64Binary GIFTI; ``compress_level`` (0-9) forces one
+    zlib level for both arrays (default: stored floats, level-1 integers, and the
+    payload an unchanged array was read from)."""
+
     out = Path(path)
     out.parent.mkdir(parents=True, exist_ok=True)
     v = np.asarray(vertices, np.float32)
     f = np.asarray(faces, np.int32)
-    vp, fp = _pool().map(lambda a: _payload(*a), ((v, "f4"), (f, "i4")))
+    vp, fp = _pool().map(lambda a: _payload(*a),
+                         ((v, "f4", compress_level), (f, "i4", compress_level)))
     text = ('<?xml version="1.0" encoding="UTF-8"?>\n'
             '<!DOCTYPE GIFTI SYSTEM "http://www.nitrc.org/frs/download.php/115/gifti.dtd">\n'
             '<GIFTI Version="1.0" NumberOfDataArrays="2">\n  <MetaData/>\n  <LabelTable/>\n'

@@ -160,7 +160,7 @@ def make_pair(kind: str = "fibonacci", n: int = 61442, *, lmax: int = 16, noise:
                          np.ascontiguousarray(tf, np.int32), rot)
 
 
-def write_pair(pair: SyntheticPair, folder) -> dict[str, str]:
+def write_pair(pair: SyntheticPair, folder, compress_level: int | None = None) -> dict[str, str]:
     from pathlib import Path
 
     from .gifti import write_gifti_surface
@@ -173,6 +173,6 @@ def write_pair(pair: SyntheticPair, folder) -> dict[str, str]:
                        ("tgt_surface", pair.tgt_surface, pair.tgt_faces),
                        ("tgt_sphere", pair.tgt_sphere, pair.tgt_faces)):
         p = folder / f"{name}.gii"
-        write_gifti_surface(p, v, f)
+        write_gifti_surface(p, v, f, compress_level=compress_level)
         paths[name] = str(p)
     return paths

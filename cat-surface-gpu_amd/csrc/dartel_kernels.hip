@@ -10,6 +10,7 @@
 // Reference semantics: src/cat_surface_gpu/dartel_grid.py (CPU torch path)
 // and src/cat_surface_gpu/dartel_triton.py (fused squaring, lines 344-477).
 #include "dartel_kernels.h"
+#include "kernel_timing.h"
 
 namespace csg {
 
@@ -559,13 +560,17 @@ int launch_compose_self(const T *map, const T *jac, int nx, int ny, int batch, s
                         size_t jstride, T *omap, T *ojac, hipStream_t s, bool disp)
 {
     dim3 g(grid_for((size_t)nx * ny), batch);
+    // reference bytes: composition with its Jacobian, 12 values per pixel (4 without)
+    TimedLaunch timer(kTimedComposeSelf, s,
+                      (jac ? 12.0 : 4.0) * nx * ny * sizeof(T) * batch);
     if (disp)
-        hipLaunchKernelGGL((k_compose_self<T, true>), g, dim3(kBlock), 0, s, map, jac, nx, ny,
+        hipExtLaunchKernelGGL((k_compose_self<T, true>), g, dim3(kBlock), 0, s, timer.start(), timer.stop(), 0, map, jac, nx, ny,
                            mstride, jstride, omap, ojac);
     else
-        hipLaunchKernelGGL((k_compose_self<T, false>), g, dim3(kBlock), 0, s, map, jac, nx, ny,
+        hipExtLaunchKernelGGL((k_compose_self<T, false>), g, dim3(kBlock), 0, s, timer.start(), timer.stop(), 0, map, jac, nx, ny,
                            mstride, jstride, omap, ojac);
     CSG_LAUNCH_CHECK();
+    timer.done();
     return kOk;
 }
 
@@ -583,17 +588,20 @@ template <typename T>
 int launch_objective(const ObjectiveArgs<T> &a, hipStream_t s)
 {
     dim3 g(grid_for((size_t)a.nx * a.ny), a.batch);
+    // reference bytes: jac_div_smalldef 10, initialise_objfun 13, smalldef_jac 8 values
+    TimedLaunch timer(kTimedObjective, s, 31.0 * a.nx * a.ny * sizeof(T) * a.batch);
     if (a.disp)
-        hipLaunchKernelGGL((k_objective<T, true>), g, dim3(kBlock), 0, s, a.src, a.tgt, a.v,
+        hipExtLaunchKernelGGL((k_objective<T, true>), g, dim3(kBlock), 0, s, timer.start(), timer.stop(), 0, a.src, a.tgt, a.v,
                            a.map, a.jac, a.dist, (T)a.s_div, (T)a.s_small, a.nx, a.ny, a.istride,
                            a.vstride, a.mstride, a.jstride, a.dstride, a.rhs, a.hes, a.smap,
                            a.sjac, a.rstride, a.hstride, a.partial, (int)g.x);
     else
-        hipLaunchKernelGGL((k_objective<T, false>), g, dim3(kBlock), 0, s, a.src, a.tgt, a.v,
+        hipExtLaunchKernelGGL((k_objective<T, false>), g, dim3(kBlock), 0, s, timer.start(), timer.stop(), 0, a.src, a.tgt, a.v,
                            a.map, a.jac, a.dist, (T)a.s_div, (T)a.s_small, a.nx, a.ny, a.istride,
                            a.vstride, a.mstride, a.jstride, a.dstride, a.rhs, a.hes, a.smap,
                            a.sjac, a.rstride, a.hstride, a.partial, (int)g.x);
     CSG_LAUNCH_CHECK();
+    timer.done();
     return kOk;
 }
 
@@ -603,13 +611,18 @@ int launch_squaring(const T *bin, const T *ain, const T *map, const T *jac, int 
                     T *bout, T *aout, T *inc, T *omap, T *ojac, hipStream_t s, bool disp)
 {
     dim3 g(grid_for((size_t)nx * ny), batch);
+    // reference bytes: squaring step 18 values per pixel, the composition of phi
+    // and J between steps 12 more
+    TimedLaunch timer(kTimedSquaring, s,
+                      (18.0 + (omap ? 12.0 : 0.0)) * nx * ny * sizeof(T) * batch);
     if (disp)
-        hipLaunchKernelGGL((k_squaring<T, true>), g, dim3(kBlock), 0, s, bin, ain, map, jac, nx,
+        hipExtLaunchKernelGGL((k_squaring<T, true>), g, dim3(kBlock), 0, s, timer.start(), timer.stop(), 0, bin, ain, map, jac, nx,
                            ny, bstride, astride, mstride, jstride, bout, aout, inc, omap, ojac);
     else
-        hipLaunchKernelGGL((k_squaring<T, false>), g, dim3(kBlock), 0, s, bin, ain, map, jac, nx,
+        hipExtLaunchKernelGGL((k_squaring<T, false>), g, dim3(kBlock), 0, s, timer.start(), timer.stop(), 0, bin, ain, map, jac, nx,
                            ny, bstride, astride, mstride, jstride, bout, aout, inc, omap, ojac);
     CSG_LAUNCH_CHECK();
+    timer.done();
     return kOk;
 }
 
@@ -620,10 +633,13 @@ int launch_combine(const T *v, const T *bf, const T *af, const T *br, const T *a
                    double *partial_norm, int *nblocks_out, hipStream_t s)
 {
     dim3 g(grid_for((size_t)nx * ny), batch);
-    hipLaunchKernelGGL(k_combine<T>, g, dim3(kBlock), 0, s, v, bf, af, br, ar, (T)scale, (T)w00,
+    // reference bytes: the combination and the regulariser, 17 values per pixel
+    TimedLaunch timer(kTimedCombine, s, 17.0 * nx * ny * sizeof(T) * batch);
+    hipExtLaunchKernelGGL(k_combine<T>, g, dim3(kBlock), 0, s, timer.start(), timer.stop(), 0, v, bf, af, br, ar, (T)scale, (T)w00,
                        (T)w01, (T)w10, nx, ny, vstride, bstride, astride, rhs, hes, partial_reg,
                        partial_norm, (int)g.x);
     CSG_LAUNCH_CHECK();
+    timer.done();
     if (nblocks_out)
         *nblocks_out = (int)g.x;
     return kOk;

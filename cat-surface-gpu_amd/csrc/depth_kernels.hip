@@ -803,8 +803,8 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
         if (fits) {
             pgrid = (pgrid + kXcdSpare) * kXcds;
             TimedLaunch ptimer(kTimedSorPatched, s, algorithmic);
-            hipLaunchKernelGGL(k_sor_patched<kSorRegWidth>, dim3(pgrid), dim3(kSorPatchThreads),
-                               lds, s, *patches, co.count, ell_val, rhs_c, diag_c, width, omega,
+            hipExtLaunchKernelGGL(k_sor_patched<kSorRegWidth>, dim3(pgrid), dim3(kSorPatchThreads),
+                               lds, s, ptimer.start(), ptimer.stop(), 0, *patches, co.count, ell_val, rhs_c, diag_c, width, omega,
                                n, max_iter, pplan, x, a.x_alt, bar, timeout);
             CSG_LAUNCH_CHECK();
             ptimer.done();
@@ -825,21 +825,21 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     // (occupancy x CUs), and every barrier wait is bounded, so blocks that were
     // not resident turn into a timeout error rather than a hang.
     if (local && placement == 2)
-        hipLaunchKernelGGL((k_sor_xcd_local<kSorRegColours, kSorRegWidth, false>),
-                           dim3(local_grid), dim3(kSorRegBlock), 0, s, verts, colours, ell_cols,
+        hipExtLaunchKernelGGL((k_sor_xcd_local<kSorRegColours, kSorRegWidth, false>),
+                           dim3(local_grid), dim3(kSorRegBlock), 0, s, timer.start(), timer.stop(), 0, verts, colours, ell_cols,
                            ell_val, rhs_c, diag_c, width, omega, n, batch, max_iter, plan, x, bar,
                            timeout);
     else if (local)
-        hipLaunchKernelGGL((k_sor_xcd_local<kSorRegColours, kSorRegWidth, true>),
-                           dim3(local_grid), dim3(kSorRegBlock), 0, s, verts, colours, ell_cols,
+        hipExtLaunchKernelGGL((k_sor_xcd_local<kSorRegColours, kSorRegWidth, true>),
+                           dim3(local_grid), dim3(kSorRegBlock), 0, s, timer.start(), timer.stop(), 0, verts, colours, ell_cols,
                            ell_val, rhs_c, diag_c, width, omega, n, batch, max_iter, plan, x, bar,
                            timeout);
     else if (registers)
-        hipLaunchKernelGGL((k_sor_registers<kSorRegColours, kSorRegWidth>), dim3(reg_grid),
-                           dim3(kSorRegBlock), 0, s, verts, colours, ell_cols, ell_val, rhs_c,
+        hipExtLaunchKernelGGL((k_sor_registers<kSorRegColours, kSorRegWidth>), dim3(reg_grid),
+                           dim3(kSorRegBlock), 0, s, timer.start(), timer.stop(), 0, verts, colours, ell_cols, ell_val, rhs_c,
                            diag_c, width, omega, n, batch, max_iter, x, bar, timeout);
     else
-        hipLaunchKernelGGL(k_sor_persistent, dim3(grid), dim3(kSorBlock), 0, s, verts, colours,
+        hipExtLaunchKernelGGL(k_sor_persistent, dim3(grid), dim3(kSorBlock), 0, s, timer.start(), timer.stop(), 0, verts, colours,
                            ell_cols, ell_val, rhs_c, diag_c, width, omega, n, batch, max_iter,
                            x, bar, timeout);
     CSG_LAUNCH_CHECK();

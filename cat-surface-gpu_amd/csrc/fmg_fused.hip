@@ -731,11 +731,11 @@ int launch_relax_tile(const T *a, const T *b, const T *uin, T *uout, const T *uc
     TimedLaunch timer(kTimedRelaxTile, s,
                       (9.0 * (sweeps / 2.0) + (uc ? 4.5 : 0.0)) * P * sizeof(T) * batch);
     if (sweeps == 6)  // nit = 3, the reference default
-        hipLaunchKernelGGL((k_relax_tile<T, 6>), grid, dim3(kTileThreads), lds, s, a, b, uin, uout,
+        hipExtLaunchKernelGGL((k_relax_tile<T, 6>), grid, dim3(kTileThreads), lds, s, timer.start(), timer.stop(), 0, a, b, uin, uout,
                            uc, cnx, cny, nx, ny, (T)w.w00, (T)w.w01, (T)w.w10, sweeps, tile.x,
                            tile.y, tiles_x, astride, bstride, cstride);
     else
-        hipLaunchKernelGGL((k_relax_tile<T, 0>), grid, dim3(kTileThreads), lds, s, a, b, uin, uout,
+        hipExtLaunchKernelGGL((k_relax_tile<T, 0>), grid, dim3(kTileThreads), lds, s, timer.start(), timer.stop(), 0, a, b, uin, uout,
                            uc, cnx, cny, nx, ny, (T)w.w00, (T)w.w01, (T)w.w10, sweeps, tile.x,
                            tile.y, tiles_x, astride, bstride, cstride);
     CSG_LAUNCH_CHECK();
@@ -752,7 +752,7 @@ int launch_restrict_residual(const T *a, const T *b, const T *u, MembraneWeights
     // reference bytes: apply_membrane (A, u in, R out: 7), b (2), restriction
     // (R in: 2, b_c out: 0.5) and the zeroed u_c (0.5) per fine pixel
     TimedLaunch timer(kTimedRestrict, s, 12.0 * nx * ny * sizeof(T) * batch);
-    hipLaunchKernelGGL(k_restrict_residual<T>, g, dim3(kBlock), 0, s, a, b, u, nx, ny, cnx, cny,
+    hipExtLaunchKernelGGL(k_restrict_residual<T>, g, dim3(kBlock), 0, s, timer.start(), timer.stop(), 0, a, b, u, nx, ny, cnx, cny,
                        (T)w.w00, (T)w.w01, (T)w.w10, astride, bstride, cstride, bc, uc);
     CSG_LAUNCH_CHECK();
     timer.done();
@@ -768,7 +768,7 @@ int launch_fmg_coarse(const CoarseLevels<T> &lv, const T *a, const T *b, T *u, s
         return fail(kInvalidArgument, "coarse multigrid levels need %zu bytes of LDS", lds_bytes);
     TimedLaunch timer(kTimedFmgCoarse, s,
                       coarse_reference_values(lv, nit, cycles, init) * sizeof(T) * batch);
-    hipLaunchKernelGGL(k_fmg_coarse<T>, dim3(batch), dim3(kCoarseThreads), lds_bytes, s, lv, a,
+    hipExtLaunchKernelGGL(k_fmg_coarse<T>, dim3(batch), dim3(kCoarseThreads), lds_bytes, s, timer.start(), timer.stop(), 0, lv, a,
                        b, u, astride, bstride, id, nit, cycles, init);
     CSG_LAUNCH_CHECK();
     timer.done();

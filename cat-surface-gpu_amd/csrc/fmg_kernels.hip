@@ -224,12 +224,12 @@ int launch_relax_sweep(const T *a, const T *b, T *u, MembraneWeights w, int nx, 
     dim3 g(grid_for((size_t)half * ny), batch);
     // reference bytes: half of the 9 values per pixel of a red+black iteration
     TimedLaunch timer(kTimedRelaxSweep, s, 4.5 * nx * ny * sizeof(T) * batch);
-    hipLaunchKernelGGL(k_relax_color<T>, g, dim3(kBlock), 0, s, a, b, u, (T)w.w00, (T)w.w01,
+    hipExtLaunchKernelGGL(k_relax_color<T>, g, dim3(kBlock), 0, s, timer.start(), timer.stop(), 0, a, b, u, (T)w.w00, (T)w.w01,
                        (T)w.w10, nx, ny, astride, bstride, color, 0);
     CSG_LAUNCH_CHECK();
     if ((nx & 1) && nx > 1) {
         dim3 g1(grid_for((size_t)ny), batch);
-        hipLaunchKernelGGL(k_relax_color<T>, g1, dim3(kBlock), 0, s, a, b, u, (T)w.w00, (T)w.w01,
+        hipExtLaunchKernelGGL(k_relax_color<T>, g1, dim3(kBlock), 0, s, timer.start(), timer.stop(), 0, a, b, u, (T)w.w00, (T)w.w01,
                            (T)w.w10, nx, ny, astride, bstride, color, 1);
         CSG_LAUNCH_CHECK();
     }

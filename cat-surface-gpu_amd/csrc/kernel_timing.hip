@@ -56,9 +56,11 @@ bool capturing(hipStream_t s)
     return st != hipStreamCaptureStatusNone;
 }
 
-const char *kNames[kTimedCount] = {"k_relax_tile",  "k_fmg_coarse",     "k_restrict_residual",
-                                   "k_relax_color", "k_sor_persistent", "k_sor_registers",
-                                   "k_sor_xcd_local", "k_sor_patched"};
+const char *kNames[kTimedCount] = {"k_relax_tile",    "k_fmg_coarse",     "k_restrict_residual",
+                                   "k_relax_color",   "k_sor_persistent", "k_sor_registers",
+                                   "k_sor_xcd_local", "k_sor_patched",    "k_squaring",
+                                   "k_objective",     "k_compose_self",   "k_combine",
+                                   "k_smooth_geometry_patch"};
 
 }  // namespace
 
@@ -77,8 +79,12 @@ TimedLaunch::TimedLaunch(int id, hipStream_t s, double bytes) : id_(id), s_(s), 
     if (!g_on.load(std::memory_order_relaxed) || capturing(s))
         return;
     a_ = take_event();
-    if (a_ && hipEventRecord(a_, s) != hipSuccess) {
-        (void)hipGetLastError();
+    b_ = a_ ? take_event() : nullptr;
+    if (!b_) {
+        TimerState &st = state();
+        std::lock_guard<std::mutex> lock(st.m);
+        if (a_)
+            st.pool.push_back(a_);
         a_ = nullptr;
     }
 }
@@ -87,15 +93,10 @@ void TimedLaunch::done()
 {
     if (!a_)
         return;
-    hipEvent_t b = take_event();
-    if (!b || hipEventRecord(b, s_) != hipSuccess) {
-        (void)hipGetLastError();
-        return;
-    }
     TimerState &st = state();
     std::lock_guard<std::mutex> lock(st.m);
-    st.pending[id_].push_back({a_, b, bytes_});
-    a_ = nullptr;
+    st.pending[id_].push_back({a_, b_, bytes_});
+    a_ = b_ = nullptr;
 }
 
 int timing_read(int id, double *ms, double *bytes, long long *launches)

@@ -13,6 +13,9 @@
 // per-vertex list of incident (face, corner) entries sorted in the order the
 // reference adds them, so every sum is deterministic and atomic-free.
 #include "surface_kernels.h"
+#include "kernel_timing.h"
+
+#include <algorithm>
 
 #include <mutex>
 
@@ -911,9 +914,17 @@ int launch_smooth_geometry_patched(const float *v, float4 *buf0, float4 *buf1, i
     for (int it0 = 0; it0 < most; it0 += pp.rings, ++launches) {
         const float4 *in = (launches & 1) ? buf1 : buf0;
         float4 *o = (launches & 1) ? buf0 : buf1;
-        hipLaunchKernelGGL(k_smooth_geometry_patch, dim3(pp.count, sb.count),
-                           dim3(kPatchThreads), lds_bytes, s, in, o, n, pp, sb, it0);
+        // reference bytes per vertex and iteration: read and write the float32
+        // position (12 + 12, counted as 16 + 16 for the padded form) and the
+        // neighbour list (4 bytes each, mean degree 6 on a closed triangle mesh)
+        double iters = 0.0;
+        for (int m = 0; m < sb.count; ++m)
+            iters += std::max(0, std::min(pp.rings, sb.iters[m] - it0));
+        TimedLaunch timer(kTimedSmoothGeometry, s, iters * n * (32.0 + 4.0 * 6.0));
+        hipExtLaunchKernelGGL(k_smooth_geometry_patch, dim3(pp.count, sb.count),
+                           dim3(kPatchThreads), lds_bytes, s, timer.start(), timer.stop(), 0, in, o, n, pp, sb, it0);
         CSG_LAUNCH_CHECK();
+        timer.done();
     }
     // every mesh's result is in the buffer the last launch wrote
     hipLaunchKernelGGL(k_unpack4_perm, g, dim3(kBlock), 0, s, (launches & 1) ? buf1 : buf0, n,

@@ -27,6 +27,16 @@
 double oracle_closest_on_triangle(const double *p, const double *a, const double *b,
                                   const double *c, double *w);
 
+/* Threads the oracle's parallel loops use (1 without OpenMP). */
+int oracle_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 /* ------------------------------------------------------------ closest triangles */
 
 static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
