@@ -59,15 +59,23 @@ def set_sor_patches(enabled: bool) -> bool:
 
 def depth_potential(stencil: SurfaceStencilDevice, smoothed_surface, *,
                     alpha: float = 1.0 / 1000.0, max_iter: int = 1000,
-                    relaxation: float = 1.90) -> tuple[torch.Tensor, int, torch.Tensor]:
-    """Depth potential by colour-ordered SOR (rotation_feature.py:253-317).
+                    relaxation: float = 1.90, solver: str = "colored-sor",
+                    tolerance: float = 1.0e-9) -> tuple[torch.Tensor, int, torch.Tensor]:
+    """Depth potential by colour-ordered SOR (rotation_feature.py:253-317) or, with
+    ``solver="pcg"``, Jacobi-preconditioned CG to ``tolerance`` (:193-250).
 
     ``smoothed_surface`` is one [n, 3] geometry or a [batch, n, 3] stack of
     geometries on the stencil's topology; all are solved in one persistent
     launch.  Returns the values ([n] or [batch, n], float64), the iteration
-    count and the relative residuals |rhs - A x| / |rhs| (float64, on the device,
-    shape [] or [batch]).
+    count (SOR: max_iter; PCG: the largest of the batch) and the relative
+    residuals |rhs - A x| / |rhs| (float64, on the device, shape [] or [batch]).
     """
+
+    if solver not in {"colored-sor", "pcg"}:
+        raise ValueError("rotation feature solver must be 'colored-sor' or 'pcg'")
+    if solver == "pcg":
+        return _depth_potential_pcg(stencil, smoothed_surface, alpha=alpha, max_iter=max_iter,
+                                    tolerance=tolerance)
 
     topo = stencil.topology
     pts = torch.as_tensor(smoothed_surface, dtype=torch.float32,
@@ -104,6 +112,36 @@ def depth_potential(stencil: SurfaceStencilDevice, smoothed_surface, *,
     if single:
         return x[0], int(max_iter), rel[0]
     return x, int(max_iter), rel
+
+
+def _depth_potential_pcg(stencil: SurfaceStencilDevice, smoothed_surface, *, alpha: float,
+                         max_iter: int, tolerance: float):
+    topo = stencil.topology
+    pts = torch.as_tensor(smoothed_surface, dtype=torch.float32,
+                          device=stencil.device).contiguous()
+    single = pts.dim() == 2
+    if single:
+        pts = pts.unsqueeze(0)
+    if pts.dim() != 3 or pts.shape[2] != 3 or pts.shape[1] != topo.n_points:
+        raise ValueError(f"depth-potential geometry must have shape [batch, {topo.n_points}, 3]")
+    batch, n = int(pts.shape[0]), int(pts.shape[1])
+    nf = int(topo.faces.shape[0])
+    nnz = int(topo.sys_cols.numel())
+    x = torch.empty(batch, n, dtype=torch.float64, device=pts.device)
+    rel = torch.empty(batch, dtype=torch.float64, device=pts.device)
+    iters = (ctypes.c_int * batch)()
+    lib = _lib.lib()
+    ws = _lib.workspace(lib.csg_depth_pcg_workspace_bytes(n, nf, nnz, batch), pts.device)
+    _lib.check(lib.csg_depth_potential_pcg(
+        _lib.ptr(pts), batch, n, _lib.ptr(topo.faces), nf, _lib.ptr(topo.inc_off),
+        _lib.ptr(topo.inc), _lib.ptr(topo.sys_row_off), _lib.ptr(topo.sys_cols),
+        _lib.ptr(topo.sys_contrib_off), _lib.ptr(topo.sys_contrib), nnz, float(alpha),
+        int(max_iter), float(tolerance), _lib.ptr(x), _lib.ptr(rel), iters, _lib.ptr(ws),
+        ws.numel(), _lib.stream_handle(pts.device)), "csg_depth_potential_pcg")
+    its = [int(v) for v in iters]
+    if single:
+        return x[0], its[0], rel[0]
+    return x, max(its), rel
 
 
 def _feature_geometry(stencil: SurfaceStencilDevice, surface_vertices, heat_fwhm: float,
@@ -180,10 +218,6 @@ def _topology_groups(stencils: Sequence[SurfaceStencilDevice], together: bool) -
 def _check_solver(solver: str, need_solve: bool) -> None:
     if solver not in {"colored-sor", "pcg"}:
         raise ValueError("rotation feature solver must be 'colored-sor' or 'pcg'")
-    if solver == "pcg" and need_solve:
-        raise NotImplementedError(
-            "this build solves the depth potential with the reference's default "
-            "colour-ordered SOR; solver='pcg' is not built")
 
 
 def compute_rotation_feature(stencil: SurfaceStencilDevice, surface_vertices: torch.Tensor, *,
@@ -199,7 +233,8 @@ def compute_rotation_feature(stencil: SurfaceStencilDevice, surface_vertices: to
     if depth_values is not None:
         return _finish_feature(stencil, geometry, _given_depth(stencil, depth_values), 0, 0.0)
     values, iterations, residual = depth_potential(stencil, geometry, alpha=depth_alpha,
-                                                   max_iter=max_iter)
+                                                   max_iter=max_iter, solver=solver,
+                                                   tolerance=tolerance)
     return _finish_feature(stencil, geometry, values, iterations, float(residual))
 
 
@@ -246,12 +281,15 @@ def compute_rotation_features(stencils: Sequence[SurfaceStencilDevice],
                                 final_ems, groups)
     values: list[torch.Tensor | None] = [None] * len(stencils)
     residuals: list[float] = [0.0] * len(stencils)
+    iterations: list[int] = [int(max_iter)] * len(stencils)
     for g in groups:
         stack = torch.stack([geometries[i] for i in g]) if len(g) > 1 else geometries[g[0]]
-        x, _, rel = depth_potential(stencils[g[0]], stack, alpha=depth_alpha, max_iter=max_iter)
+        x, its, rel = depth_potential(stencils[g[0]], stack, alpha=depth_alpha,
+                                      max_iter=max_iter, solver=solver, tolerance=tolerance)
         rel_host = rel.reshape(-1).tolist()
         for k, i in enumerate(g):
             values[i] = x[k] if len(g) > 1 else x
             residuals[i] = float(rel_host[k])
-    return _finish_features(stencils, geometries, values, [int(max_iter)] * len(stencils),
-                            residuals, final_ems, groups)
+            iterations[i] = int(its)
+    return _finish_features(stencils, geometries, values, iterations, residuals, final_ems,
+                            groups)

@@ -611,6 +611,71 @@ int csg_depth_potential_patched(const float *pts, int batch, int n, const int *f
     return kOk;
 }
 
+size_t csg_depth_pcg_workspace_bytes(int n, int nf, int nnz, int batch)
+{
+    Carve c{nullptr, 0};
+    carve_depth(c, batch, n, nf, nnz, 1);
+    return c.used + pcg_workspace_bytes(n, batch);
+}
+
+// Depth potential by Jacobi-preconditioned CG (rotation_feature.py:193-250).
+// Synchronises once at the end to check the persistent solve and to return the
+// iteration counts.
+int csg_depth_potential_pcg(const float *pts, int batch, int n, const int *faces, int nf,
+                            const int *inc_off, const int *inc, const int *row_off,
+                            const int *cols, const int *contrib_off, const int *contrib, int nnz,
+                            double alpha, int max_iter, double tol, double *x, double *relres,
+                            int *iterations_host, void *ws, size_t ws_bytes, void *stream)
+{
+    if (n < 1 || nf < 1 || batch < 1 || max_iter < 0 || !x || !relres || !(tol >= 0.0))
+        return fail(kInvalidArgument, "invalid depth-potential arguments");
+    if (ws_bytes < csg_depth_pcg_workspace_bytes(n, nf, nnz, batch))
+        return fail(kInvalidArgument, "depth-potential workspace too small");
+    hipStream_t s = S(stream);
+    Carve c{(char *)ws, 0};
+    DepthArgs a = carve_depth(c, batch, n, nf, nnz, 1);
+    double *pcg_ws = c.take<double>(pcg_workspace_bytes(n, batch) / sizeof(double));
+    a.pts = pts;
+    a.faces = faces;
+    a.inc_off = inc_off;
+    a.inc = inc;
+    a.row_off = row_off;
+    a.cols = cols;
+    a.contrib_off = contrib_off;
+    a.contrib = contrib;
+    a.batch = batch;
+    a.n = n;
+    a.nf = nf;
+    a.nnz = nnz;
+    a.alpha = alpha;
+    a.x = x;
+    // iteration counts in the partial-sum area of the setup (not used by the solve)
+    int *iters_dev = reinterpret_cast<int *>(a.partial);
+    CSG_TRY(depth_setup(a, s));
+    SorRun run{};
+    const bool allow_local = !xcd_local_unusable();
+    CSG_TRY(depth_pcg(a, pcg_ws, max_iter, tol, allow_local, sor_force_xcd(), iters_dev, relres,
+                      &run, s));
+    int rc = kOk;
+    if (!sor_placement_complete(a, run, s, &rc)) {
+        if (rc != kOk)
+            return rc;
+        mark_xcd_local_unusable();
+        CSG_TRY(depth_setup(a, s));
+        CSG_TRY(depth_pcg(a, pcg_ws, max_iter, tol, false, -1, iters_dev, relres, &run, s));
+    }
+    int timeout = 0;
+    const unsigned *flag = a.barrier + (size_t)(kBarrierSlots - 1) * kBarrierSlotWords;
+    CSG_HIP_CHECK(hipMemcpyAsync(&timeout, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (iterations_host)
+        CSG_HIP_CHECK(hipMemcpyAsync(iterations_host, iters_dev, sizeof(int) * (size_t)batch,
+                                     hipMemcpyDeviceToHost, s));
+    CSG_HIP_CHECK(hipStreamSynchronize(s));
+    if (timeout)
+        return fail(kRuntimeError, "depth-potential PCG group barrier timed out");
+    return kOk;
+}
+
 int csg_apply_flow(const float *unit, int n, const void *flow, int nx, int ny, int dtype,
                    int inverse, void *out, void *stream)
 {

@@ -71,6 +71,16 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
 // for them: the caller must then solve again without the XCD-local placement.
 bool sor_placement_complete(const DepthArgs &a, const SorRun &run, hipStream_t s, int *rc);
 int depth_residual(const DepthArgs &a, double *out, hipStream_t s);
+// Fixed row chunks of one mesh in the PCG (also the most workgroups per mesh).
+constexpr int kPcgChunks = 32;
+// Scratch of depth_pcg: r, z, p, q of every mesh and the partial sums.
+size_t pcg_workspace_bytes(int n, int batch);
+// Jacobi-preconditioned CG on the system depth_setup assembled (x starts at 0),
+// to relative residual tol or max_iter iterations; iterations and relres are
+// device arrays [batch].  XCD-local groups when allowed and the device fits
+// them (run says so, for sor_placement_complete), else one workgroup per mesh.
+int depth_pcg(const DepthArgs &d, double *ws, int max_iter, double tol, bool allow_xcd_local,
+              int force_xcd, int *iterations, double *relres, SorRun *run, hipStream_t s);
 // Process-wide placement of the SOR sweeps: 2 (default) each mesh's rows on
 // one XCD, x stored through that XCD's L2; 1 the same with x stores written
 // through to memory; 0 the device-wide grid barrier.  1 and 2 apply when the
@@ -78,5 +88,6 @@ int depth_residual(const DepthArgs &a, double *out, hipStream_t s);
 int set_sor_xcd_local(int mode);
 // Test hook: every XCD-local SOR workgroup claims on XCD `xcd` (< 0: off).
 int set_sor_force_xcd(int xcd);
+int sor_force_xcd();
 
 }  // namespace csg

@@ -709,6 +709,11 @@ int set_sor_force_xcd(int xcd)
     return g_sor_force_xcd.exchange(xcd < 0 ? -1 : xcd);
 }
 
+int sor_force_xcd()
+{
+    return g_sor_force_xcd.load();
+}
+
 int set_sor_xcd_local(int mode)
 {
     return g_sor_xcd_local.exchange(mode < 0 ? 0 : (mode > 2 ? 2 : mode));

@@ -30,6 +30,7 @@ enum TimedKernel : int {
     kTimedComposeSelf,     // phi o phi (and J) of the exponentiation
     kTimedCombine,         // rhs = b·2^-k + L v, A·2^-k, with the step's reductions
     kTimedSmoothGeometry,  // `rings` heat-smoothing iterations of the geometry on mesh patches
+    kTimedPcg,             // the whole Jacobi-preconditioned CG of a batch of depth potentials
     kTimedCount
 };
 

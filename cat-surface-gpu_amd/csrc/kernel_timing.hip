@@ -60,7 +60,7 @@ const char *kNames[kTimedCount] = {"k_relax_tile",    "k_fmg_coarse",     "k_res
                                    "k_relax_color",   "k_sor_persistent", "k_sor_registers",
                                    "k_sor_xcd_local", "k_sor_patched",    "k_squaring",
                                    "k_objective",     "k_compose_self",   "k_combine",
-                                   "k_smooth_geometry_patch"};
+                                   "k_smooth_geometry_patch", "k_pcg"};
 
 }  // namespace
 

@@ -368,6 +368,20 @@ int csg_depth_potential_patched(const float *pts, int batch, int n, const int *f
                                 const csg_sor_patches *patches, double *x, double *relres,
                                 void *ws, size_t ws_bytes, void *stream);
 
+/* Depth potential of the rotation feature by Jacobi-preconditioned conjugate gradients from
+ * x = 0, to relative residual tol or max_iter iterations, for `batch` meshes of one topology
+ * (the same system as csg_depth_potential).  Each mesh's workgroups run on one XCD and
+ * exchange the search direction and partial sums through its L2; every reduction has a fixed
+ * order, so results repeat exactly.  Writes x [batch][n] and relres [batch]; iterations_host
+ * (optional) receives the iteration counts.  Synchronises once at the end.
+ * Replaces _solve_depth_potential_cg (rotation_feature.py:193-250). */
+size_t csg_depth_pcg_workspace_bytes(int n, int nf, int nnz, int batch);
+int csg_depth_potential_pcg(const float *pts, int batch, int n, const int *faces, int nf,
+                            const int *inc_off, const int *inc, const int *row_off,
+                            const int *cols, const int *contrib_off, const int *contrib, int nnz,
+                            double alpha, int max_iter, double tol, double *x, double *relres,
+                            int *iterations_host, void *ws, size_t ws_bytes, void *stream);
+
 /* Sphere points through a flow (out: [n][3] of dtype).  Replaces
  * surface_warp.apply_flow_to_sphere (src/cat_surface_gpu/surface_warp.py:500-584). */
 int csg_apply_flow(const float *unit, int n, const void *flow, int nx, int ny, int dtype,

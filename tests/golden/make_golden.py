@@ -463,6 +463,9 @@ def golden_rotation_feature() -> None:
     depth, iters, resid = rf._solve_depth_potential_colored_sor(
         smoothed, dev.faces, areas, normals, dev.color_groups(), alpha=1e-3, max_iter=1000)
     result = rf.compute_rotation_feature(dev, surface, heat_fwhm=15.0)
+    depth_cg, cg_iters, cg_resid = rf._solve_depth_potential_cg(
+        smoothed, dev.faces, areas, normals, alpha=1e-3, max_iter=1000, tolerance=1e-9)
+    result_cg = rf.compute_rotation_feature(dev, surface, heat_fwhm=15.0, solver="pcg")
     groups = dev.color_groups()
     colours = np.zeros(verts.shape[0], np.int32)
     for c, grp in enumerate(groups):
@@ -470,7 +473,9 @@ def golden_rotation_feature() -> None:
     _save("rotation_feature", verts=verts, faces=faces, surface=surface,
           surf_idx=stencil.surface_indices, surf_w=stencil.surface_weights,
           smoothed=smoothed, normals=normals, areas=areas, depth=depth,
-          residual=np.asarray(resid), colours=colours, feature=result.values)
+          residual=np.asarray(resid), colours=colours, feature=result.values,
+          depth_cg=depth_cg, cg_iterations=np.asarray(cg_iters),
+          cg_residual=np.asarray(cg_resid), feature_cg=result_cg.values)
 
 
 def main() -> None:

@@ -115,6 +115,18 @@ def test_type5_sheets_batch_equals_per_side(small, cuda):
         assert torch.equal(got, many[0])
 
 
+def _feature_stencil(g, cuda):
+    """The reference fixture's stencil (642-vertex sphere, resampling weights) on the GPU."""
+
+    import cat_surface_gpu as csg
+
+    n = g["verts"].shape[0]
+    return csg.SurfaceStencil(
+        sphere_points=g["verts"], faces=g["faces"], surface_indices=g["surf_idx"],
+        surface_weights=g["surf_w"], sheet_indices=np.zeros((1, 1, 3), np.int32),
+        sheet_weights=np.zeros((1, 1, 3)), nx=1, ny=1, source_points=n).to(cuda)
+
+
 def test_rotation_feature_matches_golden(golden, cuda):
     import cat_surface_gpu as csg
     from cat_surface_gpu.rotation_feature import compute_rotation_feature
@@ -159,6 +171,82 @@ def test_depth_potential_batch_matches_single(golden, cuda):
     solo = compute_rotation_feature(st, surf, heat_fwhm=10.0)
     assert torch.equal(pair[1].values, solo.values)
     np.testing.assert_allclose(_np(pair[0].values), g["feature"], atol=1e-6)
+
+
+def test_pcg_depth_potential_matches_reference_and_oracle(golden, cuda):
+    """The PCG solver against the reference's CG fixture (642 vertices) and the
+    oracle's CG; the feature built on it."""
+
+    from cat_surface_gpu.rotation_feature import compute_rotation_feature, depth_potential
+    from oracle import pipeline as P
+
+    g = golden("rotation_feature")
+    st = _feature_stencil(g, cuda)
+    x, its, rel = depth_potential(st, torch.as_tensor(g["smoothed"], device=cuda), solver="pcg",
+                                  tolerance=1e-9)
+    scale = np.abs(g["depth_cg"]).max()
+    assert its == int(g["cg_iterations"])
+    assert float(rel) <= 1e-9
+    np.testing.assert_allclose(_np(x), g["depth_cg"], rtol=0, atol=1e-9 * scale)
+    ref = P.depth_potential(g["smoothed"], g["faces"], solver="pcg", tolerance=1e-9)
+    np.testing.assert_allclose(_np(x), ref.values, rtol=0, atol=1e-9 * scale)
+    res = compute_rotation_feature(st, torch.as_tensor(g["surface"], device=cuda),
+                                   heat_fwhm=15.0, solver="pcg")
+    np.testing.assert_allclose(_np(res.values), g["feature_cg"], atol=1e-6)
+    assert res.iterations == int(g["cg_iterations"])
+
+
+@pytest.mark.parametrize("batch", [1, 4, 9])
+def test_pcg_depth_potential_at_40962_vertices(batch, cuda):
+    """PCG on icosphere-6 geometries (40,962 vertices): every mesh of a batch
+    equals the same mesh solved alone bit for bit (fixed reduction order), it
+    reaches the tolerance, and it agrees with the oracle's CG (measured: 495
+    iterations, as the oracle); when the workgroups of a batch all claim one XCD
+    (as on a partition reporting one XCC_ID) the meshes of the other XCDs get no
+    group, the solve notices and the one-workgroup-per-mesh fallback returns the
+    same values bit for bit (the reductions follow fixed row chunks)."""
+
+    import cat_surface_gpu as csg
+    from cat_surface_gpu import _lib
+    from cat_surface_gpu.rotation_feature import depth_potential
+    from cat_surface_gpu.stencil_builder import icosphere
+    from oracle import pipeline as P
+
+    v, f = icosphere(6)
+    n = v.shape[0]
+    idx = np.repeat(np.arange(n, dtype=np.int32)[:, None], 3, axis=1)
+    w = np.tile(np.array([1.0, 0.0, 0.0]), (n, 1))
+    st = csg.SurfaceStencil(
+        sphere_points=v, faces=f.astype(np.int32), surface_indices=idx, surface_weights=w,
+        sheet_indices=np.zeros((1, 1, 3), np.int32), sheet_weights=np.zeros((1, 1, 3)),
+        nx=1, ny=1, source_points=n).to(cuda)
+    rng = np.random.default_rng(40 + batch)
+    # a smooth bumpy surface at millimetre scale, as the pipeline's fwhm-15 geometry
+    dirs = rng.normal(size=(6, 3))
+    bump = 1.0 + 0.03 * np.sin(v @ dirs.T * 3.0).sum(1)
+    geoms = np.stack([v * 60.0 * (bump * (1.0 + 0.01 * k))[:, None] for k in range(batch)])
+    pts = torch.as_tensor(geoms, dtype=torch.float32, device=cuda)
+    x, its, rel = depth_potential(st, pts, solver="pcg", tolerance=1e-9)
+    x1, its1, rel1 = depth_potential(st, pts[-1], solver="pcg", tolerance=1e-9)
+    assert torch.equal(x[-1], x1) and float(rel[-1]) == float(rel1)
+    assert float(rel.max()) <= 1e-9
+    ref = P.depth_potential(geoms[-1].astype(np.float32), f, solver="pcg", tolerance=1e-9)
+    scale = np.abs(ref.values).max()
+    np.testing.assert_allclose(_np(x1), ref.values, rtol=0, atol=1e-7 * scale)
+    print(f"\nPCG 40,962 vertices, batch {batch}: {its1} iterations (oracle {ref.iterations}), "
+          f"relres {float(rel1):.2e}")
+    if batch == 1:
+        return  # one mesh: forcing every claim onto XCD 0 leaves no group short
+    lib = _lib.lib()
+    lib.csg_reset_sor_xcd_check()
+    lib.csg_set_sor_force_xcd(0)
+    try:
+        xf, _, relf = depth_potential(st, pts, solver="pcg", tolerance=1e-9)
+    finally:
+        lib.csg_set_sor_force_xcd(-1)
+        lib.csg_reset_sor_xcd_check()
+    # the chunked reductions make the fallback's iterates the grouped ones exactly
+    assert torch.equal(xf, x) and torch.equal(relf, rel)
 
 
 @pytest.mark.parametrize("mode", [1, 2])
