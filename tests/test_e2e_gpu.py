@@ -246,3 +246,41 @@ def test_uneven_batched_schedule_against_the_oracle_chain(method, cuda):
         worst.append(mx)
     print(f"[{method}] per-job max vertex error: p50 {np.percentile(worst, 50):.3e}, "
           f"p99 {np.percentile(worst, 99):.3e}", flush=True)
+
+
+def test_sor_and_pcg_depth_features_on_the_hemisphere(cuda):
+    """The rotation features of the 61,442-vertex pair from the reference's default
+    solver (colour-ordered SOR, 1000 sweeps) and from PCG (to 1e-9): both depth
+    potentials are converged on this geometry (the pipeline's fwhm-15/10 smoothed
+    surfaces), the features agree, and the pipeline picks the same rotation."""
+
+    from cat_surface_gpu import GiftiMesh, run_cat_surface_gpu_pipeline
+    from cat_surface_gpu.rotation_feature import compute_rotation_features
+    from cat_surface_gpu.stencil_builder import build_device_stencil
+
+    p = _pair("fibonacci61442-lh")
+    stencils = [build_device_stencil(p.src_sphere, p.src_faces, device=cuda),
+                build_device_stencil(p.tgt_sphere, p.tgt_faces, device=cuda)]
+    surfaces = [torch.as_tensor(p.src_surface, device=cuda),
+                torch.as_tensor(p.tgt_surface, device=cuda)]
+    sor = compute_rotation_features(stencils, surfaces, heat_fwhms=(15.0, 10.0))
+    pcg = compute_rotation_features(stencils, surfaces, heat_fwhms=(15.0, 10.0), solver="pcg")
+    for a, b in zip(sor, pcg):
+        d = float((a.values - b.values).abs().max())
+        print(f"\nSOR relres {a.relative_residual:.2e}; PCG relres {b.relative_residual:.2e} "
+              f"after {b.iterations} iterations; feature max diff {d:.2e}", flush=True)
+        assert a.relative_residual <= 1e-10
+        assert b.relative_residual <= 1e-9
+        assert d <= 1e-6
+    meshes = (GiftiMesh(p.src_surface, p.src_faces), GiftiMesh(p.src_sphere, p.src_faces),
+              GiftiMesh(p.tgt_surface, p.tgt_faces), GiftiMesh(p.tgt_sphere, p.tgt_faces))
+    runs = {s: run_cat_surface_gpu_pipeline(None, None, None, None, None, meshes=meshes,
+                                            write_output=False, device=cuda,
+                                            dartel_dtype="float32", rotation_feature_solver=s,
+                                            **WORKLOADS["fibonacci61442-lh"]["run"])
+            for s in ("colored-sor", "pcg")}
+    da = float(np.abs(runs["pcg"].rotation_angle - runs["colored-sor"].rotation_angle).max())
+    dv = _errors(runs["pcg"].vertices, runs["colored-sor"].vertices)[0]
+    print(f"rotation |d angle| SOR vs PCG features {da:.2e} rad; vertex difference {dv:.2e}",
+          flush=True)
+    assert da <= 1e-6
