@@ -130,6 +130,10 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
                                  _d, _d, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
     "csg_rotation_costs": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp,
                                 _vp, _vp, _vp, _sz, ctypes.POINTER(_i), _vp, _vp]),
+    "csg_curvature_type2": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "csg_curvature_sheet_workspace_bytes": (_sz, [_i, _i, _i]),
+    "csg_curvature_sheet": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _i, _d, _i, _vp, _vp,
+                                 _i, _i, _vp, _vp, _sz, _vp]),
     "csg_depth_pcg_workspace_bytes": (_sz, [_i, _i, _i, _i]),
     "csg_depth_potential_pcg": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _d,
                                      _i, _d, _vp, _vp, ctypes.POINTER(_i), _vp, _sz, _vp]),

@@ -317,8 +317,6 @@ def run_cat_surface_gpu_pipelines(
     for curvtype in curvtypes[:steps]:
         if int(curvtype) not in (2, 5):
             raise NotImplementedError("only upstream curvtype 2 and 5 exist")
-        if int(curvtype) != 5:
-            raise NotImplementedError("this build implements curvtype 5 (steps <= 2)")
     if rotation_max_iter < 1 or rotation_tol < 0.0 or rotation_simplex_step <= 0.0:
         raise ValueError("rotation_max_iter, rotation_tol, and rotation_simplex_step must be "
                          "positive")
@@ -551,7 +549,8 @@ def run_cat_surface_gpu_pipelines(
         shared["rotated_stencil_seconds"] = time.perf_counter() - t
 
         solve_kw = dict(n_steps=steps, loop=loop, cycles=cycles, nit=nit, its=its, code=code,
-                        device=dev, dtype=solve_dtype, cuda_graph=graph_requested)
+                        curvtypes=tuple(int(t) for t in curvtypes[:steps]), device=dev,
+                        dtype=solve_dtype, cuda_graph=graph_requested)
         if autotune:
             # relaxation tiles per map size, tuned (or read from the cache)
             # before any solve is captured; the batch is the largest solve's

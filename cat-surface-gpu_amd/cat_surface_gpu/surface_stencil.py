@@ -753,15 +753,62 @@ class SurfaceStencilDevice:
                    self._stream())
         return out
 
-    def curvature_to_sheet(self, vertices: torch.Tensor, fwhm: float, *,
-                           curvtype: int = 5) -> torch.Tensor:
-        """Supported CAT curvature mapped to a normalised sheet (:624-644)."""
+    def curvature_type2(self, vertices: torch.Tensor, fwhm: float) -> torch.Tensor:
+        """Shape-operator curvature sqrt((k1^2 + k2^2) / 2), smoothed and
+        normalised (:552-622)."""
 
-        if curvtype != 5:
-            raise NotImplementedError(
-                "this build implements curvtype 5 (used by steps <= 2); curvtype 2 is not built"
-            )
-        return self.curvature_type5_to_sheet_many(vertices, (float(fwhm),))[0]
+        geometry = torch.as_tensor(vertices, dtype=torch.float32, device=self.device).contiguous()
+        normals = self.vertex_normals(geometry)
+        topo = self.topology
+        raw = torch.empty(int(geometry.shape[0]), dtype=torch.float64, device=self.device)
+        self._call("csg_curvature_type2", _lib.ptr(geometry), _lib.ptr(normals),
+                   int(geometry.shape[0]), _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr),
+                   _lib.ptr(raw), self._stream())
+        return _minmax(self.smooth_values(raw, geometry, fwhm))
+
+    def curvature_to_sheet(self, vertices: torch.Tensor, fwhm: float, *, curvtype: int = 5,
+                           out_dtype: torch.dtype = torch.float64) -> torch.Tensor:
+        """Supported CAT curvature mapped to a normalised sheet (:624-644): one
+        fused call (geometry smoothing for type 5, curvature, value smoothing,
+        normalisation, sheet gather, normalisation)."""
+
+        if int(curvtype) not in (2, 5):
+            raise NotImplementedError("only upstream curvtype 2 and 5 exist")
+        geometry = torch.as_tensor(vertices, dtype=torch.float32, device=self.device).contiguous()
+        em = self.edge_mean_distance(geometry)
+        geom_iters = heat_parameters_from_edge_mean(em, 50.0)[0] if int(curvtype) == 5 else 0
+        iterations, sigma = heat_parameters_from_edge_mean(em, float(fwhm))
+        topo = self.topology
+        n = int(geometry.shape[0])
+        m = self.nx * self.ny
+        out = torch.empty((self.ny, self.nx), dtype=out_dtype, device=self.device)
+        ws = _lib.workspace(_lib.lib().csg_curvature_sheet_workspace_bytes(n, topo.nnz, m),
+                            self.device)
+        self._call("csg_curvature_sheet", _lib.ptr(geometry), n, _lib.ptr(topo.faces),
+                   _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), topo.nnz, _lib.ptr(topo.inc_off),
+                   _lib.ptr(topo.inc), int(curvtype), int(geom_iters), float(sigma),
+                   int(iterations), _lib.ptr(self.sheet_indices), _lib.ptr(self.sheet_weights), m,
+                   _lib.dtype_code(out_dtype), _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+                   self._stream())
+        return out
+
+    def curvature_sheets(self, vertices: torch.Tensor, fwhms: Sequence[float],
+                         curvtypes: Sequence[int],
+                         out_dtype: torch.dtype = torch.float64) -> torch.Tensor:
+        """[steps, ny, nx] maps of one surface by the reference's rule
+        (surface_warp.py:427-448): several steps that are all type 5 share one
+        fused call with sum-of-squares heat weights (curvature_type5_to_sheet_many);
+        otherwise every step is its own curvature_to_sheet."""
+
+        types = tuple(int(t) for t in curvtypes)
+        if len(types) != len(fwhms):
+            raise ValueError("one curvtype per fwhm is required")
+        if len(types) > 1 and all(t == 5 for t in types):
+            return torch.stack(self.curvature_type5_to_sheet_many(
+                vertices, tuple(float(f) for f in fwhms), out_dtype=out_dtype))
+        return torch.stack([self.curvature_to_sheet(vertices, float(f), curvtype=t,
+                                                    out_dtype=out_dtype)
+                            for f, t in zip(fwhms, types)])
 
 
 def edge_means(stencils, geometries) -> list[float]:

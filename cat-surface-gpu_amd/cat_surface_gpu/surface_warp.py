@@ -255,17 +255,27 @@ def solve_dartel_maps_jobs(source_maps, target_maps, steps_per_job: int, *, loop
     return out
 
 
-def type5_sheet_maps(vertices, stencil, fwhm: Sequence[float], dtype: torch.dtype,
-                     device: torch.device) -> torch.Tensor:
-    """[steps, ny, nx] type-5 curvature sheets of one surface (surface_warp.py:426-448)."""
+def curvature_sheet_maps(vertices, stencil, fwhm: Sequence[float], curvtypes: Sequence[int],
+                         dtype: torch.dtype, device: torch.device) -> torch.Tensor:
+    """[steps, ny, nx] curvature sheets of one surface, build_maps of
+    surface_warp.py:427-448 (resampling, then the per-step rule)."""
 
     from .surface_stencil import SurfaceStencil
 
     st = stencil.to(device) if isinstance(stencil, SurfaceStencil) else stencil
     v = torch.as_tensor(vertices, device=device).to(torch.float32).contiguous()
-    return torch.stack(st.curvature_type5_to_sheet_many(st.resample_vertices(v),
-                                                        tuple(float(f) for f in fwhm),
-                                                        out_dtype=dtype))
+    return st.curvature_sheets(st.resample_vertices(v), tuple(float(f) for f in fwhm),
+                               curvtypes, out_dtype=dtype)
+
+
+def _check_curvtypes(curvtypes: Sequence[int], n_steps: int) -> tuple[int, ...]:
+    if n_steps > len(curvtypes):
+        raise ValueError("n_steps must not exceed the length of curvtypes")
+    types = tuple(int(t) for t in curvtypes[:n_steps])
+    for t in types:
+        if t not in (2, 5):
+            raise NotImplementedError("only upstream curvtype 2 and 5 exist")
+    return types
 
 
 _SIDE_STREAMS: dict[tuple, torch.cuda.Stream] = {}
@@ -283,6 +293,7 @@ def _side_stream(device: torch.device, index: int) -> torch.cuda.Stream:
 def solve_dartel_from_surfaces_jobs(requests, *, n_steps: int = 2, loop: int = 6,
                                     lmreg: float = 1e-3, cycles: int = 3, nit: int = 3,
                                     its: int = 3, code: int = 1,
+                                    curvtypes: Sequence[int] = (5, 5, 2),
                                     device: str | torch.device = "auto",
                                     dtype: torch.dtype = torch.float64,
                                     assume_initial_nonzero: bool = False,
@@ -291,11 +302,14 @@ def solve_dartel_from_surfaces_jobs(requests, *, n_steps: int = 2, loop: int = 6
 
     ``requests`` holds (source_vertices, target_vertices, source_stencil,
     target_stencil, fwhm) tuples; their curvature sheets are computed per side
-    and all solves run as one batched solve (solve_dartel_maps_jobs).
+    (sides of several all-type-5 steps together in shared launches, any other
+    step list side by side and step by step, surface_warp.py:427-448) and all
+    solves run as one batched solve (solve_dartel_maps_jobs).
     """
 
     from .surface_stencil import SurfaceStencil, prepare_type5, type5_sheets_many
 
+    types = _check_curvtypes(curvtypes, n_steps)
     dev = resolve_device(device)
     stencils, geoms, fws = [], [], []
     for sv, tv, ss, ts, fw in requests:
@@ -307,21 +321,25 @@ def solve_dartel_from_surfaces_jobs(requests, *, n_steps: int = 2, loop: int = 6
             stencils.append(st)
             geoms.append(st.resample_vertices(v))
             fws.append(tuple(float(f) for f in fw[:n_steps]))
-    # all sides that share the fwhm list are computed together
     sheets: list[torch.Tensor | None] = [None] * len(stencils)
-    by_fwhm: dict[tuple, list[int]] = {}
-    for i, f in enumerate(fws):
-        by_fwhm.setdefault(f, []).append(i)
-    # the fwhm-50 geometry smoothing does not depend on the fwhms: every side's in
-    # the same launches
-    g32, ems, smoothed = prepare_type5(stencils, geoms)
-    for f, members in by_fwhm.items():
-        res = type5_sheets_many([stencils[i] for i in members], [geoms[i] for i in members], f,
-                                dtype, prepared=([g32[i] for i in members],
-                                                 [ems[i] for i in members],
-                                                 [smoothed[i] for i in members]))
-        for i, r in zip(members, res):
-            sheets[i] = r
+    if n_steps > 1 and all(t == 5 for t in types):
+        # all sides that share the fwhm list are computed together
+        by_fwhm: dict[tuple, list[int]] = {}
+        for i, f in enumerate(fws):
+            by_fwhm.setdefault(f, []).append(i)
+        # the fwhm-50 geometry smoothing does not depend on the fwhms: every side's
+        # in the same launches
+        g32, ems, smoothed = prepare_type5(stencils, geoms)
+        for f, members in by_fwhm.items():
+            res = type5_sheets_many([stencils[i] for i in members], [geoms[i] for i in members],
+                                    f, dtype, prepared=([g32[i] for i in members],
+                                                        [ems[i] for i in members],
+                                                        [smoothed[i] for i in members]))
+            for i, r in zip(members, res):
+                sheets[i] = r
+    else:
+        for i, (st, g, f) in enumerate(zip(stencils, geoms, fws)):
+            sheets[i] = st.curvature_sheets(g, f, types, out_dtype=dtype)
     # one batched solve per map shape; shapes run on streams of their own so
     # that small maps overlap large ones
     groups: dict[tuple, list[int]] = {}
@@ -370,14 +388,15 @@ def solve_dartel_from_surfaces(source_vertices, target_vertices, source_stencil,
                                cuda_graph: bool = False) -> DartelSolveResult:
     """Curvature sheets of both surfaces, then their DARTEL flow (surface_warp.py:370-488).
 
-    Sheets are produced directly in the solve dtype by the fused type-5 kernel
-    chain of each side.
+    Sheets are produced directly in the solve dtype by the fused curvature
+    kernel chain of each side.
     """
 
     from .surface_stencil import SurfaceStencil
 
     if n_steps < 1 or n_steps > len(fwhm) or n_steps > len(curvtypes):
         raise ValueError("n_steps must not exceed the lengths of fwhm and curvtypes")
+    types = _check_curvtypes(curvtypes, n_steps)
     if squaring_kernel not in {None, "auto", "hip", "torch", "triton"}:
         raise ValueError("squaring_kernel must be None, 'auto', or 'hip'")
     dev = resolve_device(device)
@@ -385,18 +404,9 @@ def solve_dartel_from_surfaces(source_vertices, target_vertices, source_stencil,
               else source_stencil)
     tgt_st = (target_stencil.to(dev) if isinstance(target_stencil, SurfaceStencil)
               else target_stencil)
-    for curvtype in curvtypes[:n_steps]:
-        if int(curvtype) not in (2, 5):
-            raise NotImplementedError("only upstream curvtype 2 and 5 exist")
-        if int(curvtype) != 5:
-            raise NotImplementedError("this build implements curvtype 5 (steps <= 2)")
-    source = torch.as_tensor(source_vertices, device=dev).to(torch.float32).contiguous()
-    target = torch.as_tensor(target_vertices, device=dev).to(torch.float32).contiguous()
     fw = tuple(float(f) for f in fwhm[:n_steps])
-    src_maps = torch.stack(src_st.curvature_type5_to_sheet_many(
-        src_st.resample_vertices(source), fw, out_dtype=dtype))
-    tgt_maps = torch.stack(tgt_st.curvature_type5_to_sheet_many(
-        tgt_st.resample_vertices(target), fw, out_dtype=dtype))
+    src_maps = curvature_sheet_maps(source_vertices, src_st, fw, types, dtype, dev)
+    tgt_maps = curvature_sheet_maps(target_vertices, tgt_st, fw, types, dtype, dev)
     return solve_dartel_maps(src_maps, tgt_maps, loop=loop, lmreg=lmreg, cycles=cycles, nit=nit,
                              its=its, code=code, kernel=kernel, squaring_kernel=squaring_kernel,
                              device=dev, dtype=dtype, optimized=optimized,

@@ -363,6 +363,80 @@ int csg_type5_sheets(const float *geom, int n, const int *faces, const int *nbr_
                                            (double *)out, (size_t)m, s);
 }
 
+int csg_curvature_type2(const float *geom, const float *normals, int n, const int *nbr_off,
+                        const int *nbr, double *out, void *stream)
+{
+    if (n < 1 || !geom || !normals || !nbr_off || !nbr || !out)
+        return fail(kInvalidArgument, "invalid type-2 curvature arguments");
+    return launch_curvature_type2(geom, normals, n, nbr_off, nbr, out, S(stream));
+}
+
+size_t csg_curvature_sheet_workspace_bytes(int n, int nnz, int m)
+{
+    if (n < 1 || m < 1)
+        return 0;
+    Carve c{nullptr, 0};
+    c.take<float>((size_t)3 * n);  // smoothed geometry
+    c.take<float>((size_t)3 * n);  // its ping-pong buffer
+    c.take<float>((size_t)3 * n);  // normals
+    carve_heat(c, n, nnz, 1);
+    c.take<double>((size_t)n);      // normalised values
+    c.take<double>((size_t)m);      // raw sheet
+    c.take<double>(64 * 2 + 64);
+    return c.used;
+}
+
+// curvature_to_sheet for one fwhm (surface_stencil.py:624-644): type 5 (depth along
+// the normals of the fwhm-50 smoothed geometry, :507-519) or type 2 (:552-622), the
+// values smoothed with |d|^2 heat weights (smooth_values), normalised, gathered to
+// the sheet and normalised again.
+int csg_curvature_sheet(const float *geom, int n, const int *faces, const int *nbr_off,
+                        const int *nbr, int nnz, const int *inc_off, const int *inc,
+                        int curvtype, int geom_iterations, double sigma, int iterations,
+                        const int *sheet_idx, const double *sheet_w, int m, int out_dtype,
+                        void *out, void *ws, size_t ws_bytes, void *stream)
+{
+    if (n < 1 || m < 1 || !geom || !faces || !nbr_off || !nbr || !inc_off || !inc ||
+        !sheet_idx || !sheet_w || !out)
+        return fail(kInvalidArgument, "invalid curvature sheet arguments");
+    if (curvtype != 2 && curvtype != 5)
+        return fail(kUnsupported, "only upstream curvtype 2 and 5 exist, got %d", curvtype);
+    if (out_dtype != 0 && out_dtype != 1)
+        return fail(kUnsupported, "curvature sheet: dtype code %d", out_dtype);
+    HeatBatch hb;
+    CSG_TRY(make_batch(1, &sigma, &iterations, hb));
+    if (ws_bytes < csg_curvature_sheet_workspace_bytes(n, nnz, m))
+        return fail(kInvalidArgument, "curvature sheet workspace too small");
+    hipStream_t s = S(stream);
+    Carve c{(char *)ws, 0};
+    float *sm = c.take<float>((size_t)3 * n);
+    float *pp = c.take<float>((size_t)3 * n);
+    float *nr = c.take<float>((size_t)3 * n);
+    HeatWs h = carve_heat(c, n, nnz, 1);
+    double *normed = c.take<double>((size_t)n);
+    double *sheet = c.take<double>((size_t)m);
+    double *mm = c.take<double>(64 * 2 + 64);
+    if (curvtype == 5) {
+        CSG_TRY(csg_smooth_geometry(geom, nbr_off, nbr, n, geom_iterations, sm, pp,
+                                    (size_t)3 * n * sizeof(float), stream));
+        CSG_TRY(launch_vertex_normals(sm, faces, inc_off, inc, n, nr, s));
+        CSG_TRY(launch_type5_base(geom, sm, nr, n, h.buf[0], s));
+    } else {
+        CSG_TRY(launch_vertex_normals(geom, faces, inc_off, inc, n, nr, s));
+        CSG_TRY(launch_curvature_type2(geom, nr, n, nbr_off, nbr, h.buf[0], s));
+    }
+    double *res = nullptr;
+    CSG_TRY(run_heat(geom, nbr_off, nbr, n, nnz, hb, 0, h, &res, s));
+    CSG_TRY(launch_minmax_normalise<double>(res, (size_t)n, 1, (size_t)n, 0.0, mm, normed,
+                                            (size_t)n, s));
+    CSG_TRY(launch_sheet_gather(normed, (size_t)n, sheet_idx, sheet_w, m, 1, sheet, s));
+    if (out_dtype == 0)
+        return launch_minmax_normalise<float>(sheet, (size_t)m, 1, (size_t)m, 0.0, mm,
+                                              (float *)out, (size_t)m, s);
+    return launch_minmax_normalise<double>(sheet, (size_t)m, 1, (size_t)m, 0.0, mm,
+                                           (double *)out, (size_t)m, s);
+}
+
 size_t csg_type5_sheets_batch_workspace_bytes(int n, int nnz, int sides, int nfw, int m)
 {
     if (n < 1 || sides < 1 || nfw < 1 || m < 1)

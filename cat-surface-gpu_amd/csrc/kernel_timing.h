@@ -49,7 +49,6 @@ public:
 
 private:
     int id_;
-    hipStream_t s_;
     double bytes_;
     hipEvent_t a_ = nullptr, b_ = nullptr;
 };

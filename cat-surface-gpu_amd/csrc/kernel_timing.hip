@@ -74,7 +74,7 @@ int timing_enable(int on)
     return g_on.exchange(on ? 1 : 0);
 }
 
-TimedLaunch::TimedLaunch(int id, hipStream_t s, double bytes) : id_(id), s_(s), bytes_(bytes)
+TimedLaunch::TimedLaunch(int id, hipStream_t s, double bytes) : id_(id), bytes_(bytes)
 {
     if (!g_on.load(std::memory_order_relaxed) || capturing(s))
         return;

@@ -62,6 +62,8 @@ int launch_vertex_normals(const float *v, const int *faces, const int *inc_off, 
                           int n, float *out, hipStream_t s);
 int launch_type5_base(const float *geom, const float *sm, const float *nr, int n, double *out,
                       hipStream_t s);
+int launch_curvature_type2(const float *geom, const float *nr, int n, const int *off,
+                           const int *nbr, double *out, hipStream_t s);
 int launch_heat_weights(const float *geom, const int *off, const int *nbr, int n,
                         const HeatBatch &hb, int squared_norm, size_t wstride, double *w,
                         double *wsum, hipStream_t s);

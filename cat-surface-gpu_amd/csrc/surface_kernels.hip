@@ -465,6 +465,86 @@ k_type5_base(const float *__restrict__ geom, const float *__restrict__ sm,
     out[i] = (double)s;
 }
 
+// Type-2 curvature before smoothing (surface_stencil.py:552-613): at vertex i
+// with float32 normals n, the neighbours' coordinate and normal differences
+// projected on the tangent plane, in the basis (b0, b1) built from the first
+// neighbour (ascending CSR order), fit the shape operator [[a, b], [b, c]] by
+// least squares; the value is sqrt((k1^2 + k2^2) / 2) of its eigenvalues, 0 for
+// vertices with at most two neighbours.  float64, the reference's evaluation
+// order.
+__device__ __forceinline__ double dot3d(const double *u, const double *v)
+{
+    return (u[0] * v[0] + u[1] * v[1]) + u[2] * v[2];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_curvature_type2(const float *__restrict__ geom, const float *__restrict__ nr, int n,
+                  const int *__restrict__ off, const int *__restrict__ nbr,
+                  double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const double eps = 2.220446049250313e-16;
+    const int k0 = off[i], k1e = off[i + 1], deg = k1e - k0;
+    double nn[3], pi[3];
+    for (int d = 0; d < 3; ++d) {
+        nn[d] = (double)nr[3 * i + d];
+        pi[d] = (double)geom[3 * i + d];
+    }
+    // basis from the first neighbour (the padded table's column 0)
+    double b0[3], b1[3];
+    {
+        const int j = deg > 0 ? nbr[k0] : 0;
+        double dc[3];
+        for (int d = 0; d < 3; ++d)
+            dc[d] = (double)geom[3 * j + d] - pi[d];
+        const double t = dot3d(dc, nn);
+        for (int d = 0; d < 3; ++d)
+            b0[d] = dc[d] - t * nn[d];
+        const double len = fmax(sqrt(dot3d(b0, b0)), eps);
+        for (int d = 0; d < 3; ++d)
+            b0[d] = b0[d] / len;
+        const double m0[3] = {-b0[0], -b0[1], -b0[2]};
+        b1[0] = m0[1] * nn[2] - m0[2] * nn[1];
+        b1[1] = m0[2] * nn[0] - m0[0] * nn[2];
+        b1[2] = m0[0] * nn[1] - m0[1] * nn[0];
+    }
+    double sum1 = 0.0, sum2 = 0.0, sum3 = 0.0, wx = 0.0, wy = 0.0, wxy = 0.0;
+    for (int k = k0; k < k1e; ++k) {
+        const int j = nbr[k];
+        double dc[3], dn[3], pc[3], pn[3];
+        for (int d = 0; d < 3; ++d) {
+            dc[d] = (double)geom[3 * j + d] - pi[d];
+            dn[d] = (double)nr[3 * j + d] - nn[d];
+        }
+        const double tc = dot3d(dc, nn), tn = dot3d(dn, nn);
+        for (int d = 0; d < 3; ++d) {
+            pc[d] = dc[d] - tc * nn[d];
+            pn[d] = dn[d] - tn * nn[d];
+        }
+        const double dcx = dot3d(pc, b0), dcy = dot3d(pc, b1), dnx = dot3d(pn, b0),
+                     dny = dot3d(pn, b1);
+        sum1 += dcx * dnx;
+        sum2 += dcx * dny + dcy * dnx;
+        sum3 += dcy * dny;
+        wx += dcx * dcx;
+        wy += dcy * dcy;
+        wxy += dcx * dcy;
+    }
+    const double wx2 = wx * wx, wy2 = wy * wy, wxy2 = wxy * wxy;
+    const double den = (wx + wy) * (-wxy2 + wx * wy);
+    const double sden = den != 0.0 ? den : 1.0;
+    const double a = (sum3 * wxy2 - sum2 * wxy * wy + sum1 * (-wxy2 + wx * wy + wy2)) / sden;
+    const double b = (-sum3 * wx * wxy + sum2 * wx * wy - sum1 * wxy * wy) / sden;
+    const double c = (-sum2 * wx * wxy + sum1 * wxy2 + sum3 * (wx2 - wxy2 + wx * wy)) / sden;
+    const double tr = a + c, det = a * c - b * b;
+    const double disc = tr * tr - 4.0 * det;
+    const double root = sqrt(fmax(disc, 0.0));
+    const double e1 = (tr + root) / 2.0, e2 = (tr - root) / 2.0;
+    out[i] = deg > 2 ? sqrt((e1 * e1 + e2 * e2) / 2.0) : 0.0;
+}
+
 // Gaussian weights per CSR entry for `nb` sigmas, and their row sums.
 // squared_norm: d² as a sum of squares (smooth_values_many) else |d|².
 // surface_stencil.py:390-412, :440-452
@@ -946,6 +1026,15 @@ int launch_type5_base(const float *geom, const float *sm, const float *nr, int n
                       hipStream_t s)
 {
     hipLaunchKernelGGL(k_type5_base, dim3(grid_for(n)), dim3(kBlock), 0, s, geom, sm, nr, n, out);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+int launch_curvature_type2(const float *geom, const float *nr, int n, const int *off,
+                           const int *nbr, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_curvature_type2, dim3(grid_for(n)), dim3(kBlock), 0, s, geom, nr, n, off,
+                       nbr, out);
     CSG_LAUNCH_CHECK();
     return kOk;
 }

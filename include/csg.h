@@ -283,6 +283,24 @@ int csg_type5_sheets(const float *geom, int n, const int *faces, const int *nbr_
                      const int *iterations, const int *sheet_idx, const double *sheet_w, int m,
                      int out_dtype, void *out, void *ws, size_t ws_bytes, void *stream);
 
+/* Type-2 curvature values before smoothing (out: [n] float64) of a mesh with float32
+ * vertex normals and ascending neighbour CSR.  Replaces the shape-operator part of
+ * SurfaceStencilDevice.curvature_type2 (surface_stencil.py:552-613). */
+int csg_curvature_type2(const float *geom, const float *normals, int n, const int *nbr_off,
+                        const int *nbr, double *out, void *stream);
+
+/* curvature_to_sheet for one fwhm (sigma / iterations of its heat kernel): curvtype 5 smooths
+ * the geometry geom_iterations times (the fwhm-50 smoothing) and takes the depth along the
+ * smoothed normals, curvtype 2 the type-2 curvature; the values are smoothed with |d|^2 heat
+ * weights, normalised, gathered to the sheet (m points) and normalised again.  Replaces
+ * SurfaceStencilDevice.curvature_to_sheet (surface_stencil.py:624-644). */
+size_t csg_curvature_sheet_workspace_bytes(int n, int nnz, int m);
+int csg_curvature_sheet(const float *geom, int n, const int *faces, const int *nbr_off,
+                        const int *nbr, int nnz, const int *inc_off, const int *inc,
+                        int curvtype, int geom_iterations, double sigma, int iterations,
+                        const int *sheet_idx, const double *sheet_w, int m, int out_dtype,
+                        void *out, void *ws, size_t ws_bytes, void *stream);
+
 /* csg_type5_sheets for `sides` meshes of one topology in shared launches, their fwhm-50
  * smoothed copies given: geom and smoothed [sides][n][3]; nfw rows per side (sigma /
  * iterations host arrays [sides][nfw], sides * nfw <= 32); sheet_idx / sheet_w host

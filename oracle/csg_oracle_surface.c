@@ -193,6 +193,78 @@ void oracle_type5_base(const float *geom, const float *smoothed, const float *no
     }
 }
 
+/* Type-2 curvature before smoothing.  surface_stencil.py:552-613: per vertex, with
+ * float32 normals promoted to float64, the tangent-plane basis from the first
+ * neighbour, least-squares shape operator [[a, b], [b, c]] from the projected
+ * coordinate and normal differences of all neighbours, and sqrt((k1^2 + k2^2) / 2)
+ * of its eigenvalues (0 with fewer than three neighbours). */
+static double dot3d(const double *u, const double *v)
+{
+    return u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+}
+
+static void tangent(const double *u, const double *nrm, double *out)
+{
+    double t = dot3d(u, nrm);
+    for (int d = 0; d < 3; ++d)
+        out[d] = u[d] - t * nrm[d];
+}
+
+void oracle_curvature_type2(const float *geom, const float *normals, const int *off,
+                            const int *nbr, int n, double *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double nrm[3], p[3], e0[3], e1[3], delta[3];
+        for (int d = 0; d < 3; ++d) {
+            nrm[d] = normals[3 * i + d];
+            p[d] = geom[3 * i + d];
+        }
+        int first = off[i + 1] > off[i] ? nbr[off[i]] : 0;
+        for (int d = 0; d < 3; ++d)
+            delta[d] = (double)geom[3 * first + d] - p[d];
+        tangent(delta, nrm, e0);
+        double len = sqrt(dot3d(e0, e0));
+        if (len < 2.220446049250313e-16)
+            len = 2.220446049250313e-16;
+        for (int d = 0; d < 3; ++d)
+            e0[d] /= len;
+        /* e1 = (-e0) x n */
+        e1[0] = -e0[1] * nrm[2] + e0[2] * nrm[1];
+        e1[1] = -e0[2] * nrm[0] + e0[0] * nrm[2];
+        e1[2] = -e0[0] * nrm[1] + e0[1] * nrm[0];
+        double s1 = 0, s2 = 0, s3 = 0, wx = 0, wy = 0, wxy = 0;
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            int j = nbr[k];
+            double dc[3], dn[3], tc[3], tn[3];
+            for (int d = 0; d < 3; ++d) {
+                dc[d] = (double)geom[3 * j + d] - p[d];
+                dn[d] = (double)normals[3 * j + d] - nrm[d];
+            }
+            tangent(dc, nrm, tc);
+            tangent(dn, nrm, tn);
+            double cx = dot3d(tc, e0), cy = dot3d(tc, e1);
+            double nx = dot3d(tn, e0), ny = dot3d(tn, e1);
+            s1 += cx * nx;
+            s2 += cx * ny + cy * nx;
+            s3 += cy * ny;
+            wx += cx * cx;
+            wy += cy * cy;
+            wxy += cx * cy;
+        }
+        double den = (wx + wy) * (wx * wy - wxy * wxy);
+        if (den == 0.0)
+            den = 1.0;
+        double a = (s3 * wxy * wxy - s2 * wxy * wy + s1 * (wx * wy - wxy * wxy + wy * wy)) / den;
+        double b = (s2 * wx * wy - s3 * wx * wxy - s1 * wxy * wy) / den;
+        double c = (s1 * wxy * wxy - s2 * wx * wxy + s3 * (wx * wx - wxy * wxy + wx * wy)) / den;
+        double tr = a + c, disc = tr * tr - 4.0 * (a * c - b * b);
+        double root = disc > 0.0 ? sqrt(disc) : 0.0;
+        double k1 = 0.5 * (tr + root), k2 = 0.5 * (tr - root);
+        out[i] = off[i + 1] - off[i] > 2 ? sqrt(0.5 * (k1 * k1 + k2 * k2)) : 0.0;
+    }
+}
+
 /* sheet[p] = sum_c values[idx[p,c]] * w[p,c].  surface_stencil.py:308-317 */
 void oracle_sheet_gather(const double *values, const int *idx, const double *w, int m,
                          double *out)

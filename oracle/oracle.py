@@ -451,6 +451,36 @@ def type5_base(geometry, faces):
     return out
 
 
+def curvature_type2(geometry, faces):
+    """Type-2 curvature before smoothing (surface_stencil.py:552-613)."""
+
+    g = _c32(geometry)
+    nr = vertex_normals(g, faces)
+    off, nbr = neighbour_csr(faces, g.shape[0])
+    out = np.empty(g.shape[0], np.float64)
+    lib().oracle_curvature_type2(_f(g), _f(nr), _i(off), _i(nbr), ctypes.c_int(g.shape[0]),
+                                 _d(out))
+    return out
+
+
+def curvature_to_sheet(geometry, faces, sheet_idx, sheet_w, fwhm: float, curvtype: int = 5):
+    """curvature_type5 (surface_stencil.py:507-519) or curvature_type2 (:552-622)
+    for one fwhm, smoothed with |d|^2 weights, normalised, gathered to the sheet
+    and normalised again (:624-644)."""
+
+    g = _c32(geometry)
+    if curvtype == 5:
+        raw = type5_base(g, faces)
+    elif curvtype == 2:
+        raw = curvature_type2(g, faces)
+    else:
+        raise NotImplementedError("only upstream curvtype 2 and 5 exist")
+    vals = minmax_normalise(smooth_values(raw, g, faces, fwhm, squared_norm=False))
+    ny, nx = sheet_idx.shape[:2]
+    sheet = sheet_gather(vals, sheet_idx.reshape(-1, 3), sheet_w.reshape(-1, 3))
+    return minmax_normalise(sheet).reshape(ny, nx)
+
+
 def curvature_type5_to_sheet_many(geometry, faces, sheet_idx, sheet_w, fwhms):
     """surface_stencil.py:521-550"""
 

@@ -297,22 +297,29 @@ def search_rotation(target: Stencil, source_points, source_values, target_values
 # ---------------------------------------------------------------- DARTEL side
 
 
-def type5_maps(surface_vertices, stencil: Stencil, fwhms) -> np.ndarray:
-    """[steps, ny, nx] type-5 curvature sheets of one surface (surface_warp.py:426-448,
-    surface_stencil.py:521-550)."""
+def curvature_maps(surface_vertices, stencil: Stencil, fwhms, curvtypes=None) -> np.ndarray:
+    """[steps, ny, nx] curvature sheets of one surface: build_maps of
+    surface_warp.py:427-448 (several all-type-5 steps together,
+    surface_stencil.py:521-550; else one curvature_to_sheet per step, :624-644)."""
 
+    fwhms = tuple(fwhms)
+    types = tuple(int(t) for t in (curvtypes if curvtypes is not None else (5,) * len(fwhms)))
     geometry = O.resample_vertices_f32(surface_vertices, stencil.surf_idx, stencil.surf_w)
-    return np.stack(O.curvature_type5_to_sheet_many(geometry, stencil.faces, stencil.sheet_idx,
-                                                    stencil.sheet_w, tuple(fwhms)))
+    if len(types) > 1 and all(t == 5 for t in types):
+        return np.stack(O.curvature_type5_to_sheet_many(geometry, stencil.faces,
+                                                        stencil.sheet_idx, stencil.sheet_w,
+                                                        fwhms))
+    return np.stack([O.curvature_to_sheet(geometry, stencil.faces, stencil.sheet_idx,
+                                          stencil.sheet_w, f, t) for f, t in zip(fwhms, types)])
 
 
 def solve_from_surfaces(source_vertices, target_vertices, source: Stencil, target: Stencil,
-                        fwhms, *, loop: int = 6, cycles: int = 3, nit: int = 3, its: int = 3,
-                        code: int = 1):
+                        fwhms, *, curvtypes=None, loop: int = 6, cycles: int = 3, nit: int = 3,
+                        its: int = 3, code: int = 1):
     """Sheets of both sides and their DARTEL flow (surface_warp.py:370-488)."""
 
-    src = type5_maps(source_vertices, source, fwhms)
-    tgt = type5_maps(target_vertices, target, fwhms)
+    src = curvature_maps(source_vertices, source, fwhms, curvtypes)
+    tgt = curvature_maps(target_vertices, target, fwhms, curvtypes)
     flow, metrics = O.solve_dartel_maps(src, tgt, loop=loop, lmreg=1e-3, cycles=cycles,
                                         nit=nit, its=its, code=code)
     return flow, metrics, src, tgt
@@ -339,7 +346,7 @@ class SurfWarpResult:
 
 def surfwarp(source_surface, source_sphere, source_faces, target_surface, target_sphere,
              target_faces, *, steps: int = 2, runs: int = 2, avg: bool = True,
-             fwhm=(5.0, 5.0 / 3.0, 5.0 / 9.0), loop: int = 6, cycles: int = 3, nit: int = 3,
+             fwhm=(5.0, 5.0 / 3.0, 5.0 / 9.0), curvtypes=(5, 5, 2), loop: int = 6, cycles: int = 3, nit: int = 3,
              its: int = 3, code: int = 1, map_size=(512, 256), features=None,
              rotation_grid: int = 128, rotation_margin: int = 1, max_iter: int = 500,
              tol: float = 1e-4, simplex_step: float = 0.1,
@@ -350,6 +357,7 @@ def surfwarp(source_surface, source_sphere, source_faces, target_surface, target
     rotation-feature stage (the reference's rotation_values_path)."""
 
     nx, ny = (int(s) for s in map_size)
+    types = tuple(int(t) for t in curvtypes[:steps])
     stage: dict[str, float] = {}
 
     def timed(name, fn, *a, **k):
@@ -393,7 +401,8 @@ def surfwarp(source_surface, source_sphere, source_faces, target_surface, target
             avg_src_st = timed("stencils", build_stencil, avg_src_sphere, src_faces,
                                normalise=False, nx=nx, ny=ny)
         flow, _, _, _ = timed("dartel", solve_from_surfaces, src_surf, tgt_surf, cur_st, tgt_st,
-                              run_fwhm, loop=loop, cycles=cycles, nit=nit, its=its, code=code)
+                              run_fwhm, curvtypes=types, loop=loop, cycles=cycles, nit=nit,
+                              its=its, code=code)
         flows.append(flow)
         if not final:
             current = timed("warp", warp, cur_st, flow).astype(np.float32)
@@ -404,8 +413,9 @@ def surfwarp(source_surface, source_sphere, source_faces, target_surface, target
     if avg:
         avg_fwhm = tuple(float(fwhm[0]) / (3.0 ** (steps * (runs + 1) + i)) for i in range(steps))
         avg_flow, _, _, _ = timed("dartel", solve_from_surfaces, avg_src_surface,
-                                  avg_tgt_surface, avg_src_st, avg_tgt_st, avg_fwhm, loop=loop,
-                                  cycles=cycles, nit=nit, its=its, code=code)
+                                  avg_tgt_surface, avg_src_st, avg_tgt_st, avg_fwhm,
+                                  curvtypes=types, loop=loop, cycles=cycles, nit=nit, its=its,
+                                  code=code)
         flows.append(avg_flow)
         main_warped = timed("warp", warp, cur_st, last_flow)
         avg_warped = timed("warp", warp, avg_src_st, avg_flow)

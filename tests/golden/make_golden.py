@@ -404,6 +404,8 @@ def golden_surface() -> None:
     out["type5_sheet1"] = maps[1]
     out["type5_values"] = dev.curvature_type5(surface, 5.0)
     out["type2_values"] = dev.curvature_type2(surface, 5.0)
+    out["type2_sheet"] = dev.curvature_to_sheet(surface, 5.0 / 81.0, curvtype=2)
+    out["type5_sheet_single"] = dev.curvature_to_sheet(surface, 5.0 / 9.0, curvtype=5)
     _save("surface", **out)
 
 

@@ -10,7 +10,11 @@ exactly equal, and the max / mean / p99 vertex error.
 
 Workloads (synthetic, seeded):
 - icosphere-6 pair (40,962 vertices), SH degree <= 8, target rotated <= 15
-  degrees and warped <= 3 degrees; steps=1 runs=1, no avg, 512x256 map;
+  degrees and warped <= 3 degrees; steps=1 runs=1, no avg, 512x256 map (one
+  curvature_to_sheet of type 5 per side);
+- icosphere-5 pair (10,242 vertices) of the same kind with steps=3 runs=1, no
+  avg: curvtypes 5, 5, 2, so each side's maps are per-step sheets, the last of
+  type-2 curvature;
 - 61,442-vertex jittered-Fibonacci-hull hemisphere pair, SH degree <= 16 +
   N(0, 0.05) noise, target rotated <= 30 degrees and warped; steps=2 runs=2
   avg, left and x-mirrored right copies.
@@ -30,6 +34,9 @@ WORKLOADS = {
     "icosphere6": dict(pair=dict(kind="icosphere", subdivisions=6, lmax=8, noise=0.0,
                                  max_rotation_deg=15.0, warp_deg=3.0, seed=SEED + 1),
                        run=dict(steps=1, runs=1, avg=False)),
+    "icosphere5-steps3": dict(pair=dict(kind="icosphere", subdivisions=5, lmax=8, noise=0.0,
+                                        max_rotation_deg=15.0, warp_deg=3.0, seed=SEED + 3),
+                              run=dict(steps=3, runs=1, avg=False)),
     "fibonacci61442-lh": dict(pair=dict(kind="fibonacci", n=61442, lmax=16, noise=0.05,
                                         max_rotation_deg=30.0, warp_deg=3.0, seed=SEED + 2),
                               run=dict(steps=2, runs=2, avg=True)),
@@ -131,8 +138,8 @@ def test_per_step_velocities_float32_and_oracle_512x256(cuda):
     src_st = P.build_stencil(p.src_sphere, p.src_faces, normalise=True)
     tgt_st = P.build_stencil(p.tgt_sphere, p.tgt_faces, normalise=True)
     fwhm = (5.0 / 9.0, 5.0 / 27.0)
-    src = P.type5_maps(p.src_surface, src_st, fwhm)
-    tgt = P.type5_maps(p.tgt_surface, tgt_st, fwhm)
+    src = P.curvature_maps(p.src_surface, src_st, fwhm)
+    tgt = P.curvature_maps(p.tgt_surface, tgt_st, fwhm)
     params = O.default_dartel_parameters(6)
     f32, f64 = torch.float32, torch.float64
     local = accumulated = oracle_gap = 0.0

@@ -79,6 +79,53 @@ def test_normals_and_type5_sheets(small, cuda):
     np.testing.assert_allclose(_np(f32[0]), g["type5_sheet0"], atol=3e-6)
 
 
+def test_type2_curvature_and_single_fwhm_sheets(small, cuda):
+    """Type-2 curvature (surface_stencil.py:552-622) and curvature_to_sheet for
+    both curvtypes (:624-644).  Given the reference's float32 normals the HIP
+    shape-operator fit equals the oracle's to float64 rounding (the kernel
+    contracts products into FMAs: 1e-14 relative); with its own normals (2e-6 from
+    torch's) the normalised values and sheets stay within 5e-6 of the reference."""
+
+    from oracle import oracle as O
+
+    from cat_surface_gpu import _lib
+
+    g, dev = small
+    surf = torch.as_tensor(g["surface"], device=cuda)
+    n = int(surf.shape[0])
+    topo = dev.topology
+    normals = torch.as_tensor(g["normals"], device=cuda).contiguous()
+    raw = torch.empty(n, dtype=torch.float64, device=cuda)
+    _lib.check(_lib.lib().csg_curvature_type2(_lib.ptr(surf), _lib.ptr(normals), n,
+                                              _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr),
+                                              _lib.ptr(raw), _lib.stream_handle(cuda)),
+               "csg_curvature_type2")
+    import ctypes
+
+    off, nbr = O.neighbour_csr(g["faces"], n)
+    want = np.empty(n)
+    O.lib().oracle_curvature_type2(O._f(np.ascontiguousarray(g["surface"])),
+                                   O._f(np.ascontiguousarray(g["normals"])), O._i(off),
+                                   O._i(nbr), ctypes.c_int(n), O._d(want))
+    np.testing.assert_allclose(_np(raw), want, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(_np(dev.curvature_type2(surf, 5.0)), g["type2_values"], rtol=0,
+                               atol=5e-6)
+    for dt, tol in ((torch.float64, 5e-6), (torch.float32, 6e-6)):
+        t2 = dev.curvature_to_sheet(surf, 5.0 / 81.0, curvtype=2, out_dtype=dt)
+        assert t2.dtype == dt and tuple(t2.shape) == (16, 32)
+        np.testing.assert_allclose(_np(t2), g["type2_sheet"], rtol=0, atol=tol)
+        t5 = dev.curvature_to_sheet(surf, 5.0 / 9.0, curvtype=5, out_dtype=dt)
+        np.testing.assert_allclose(_np(t5), g["type5_sheet_single"], rtol=0, atol=tol)
+    # the per-step rule of surface_warp.py:427-448
+    both5 = dev.curvature_sheets(surf, (5.0 / 9.0, 5.0 / 27.0), (5, 5))
+    np.testing.assert_allclose(_np(both5[0]), g["type5_sheet0"], atol=2e-6)
+    mixed = dev.curvature_sheets(surf, (5.0 / 9.0, 5.0 / 81.0), (5, 2))
+    np.testing.assert_allclose(_np(mixed[0]), g["type5_sheet_single"], atol=5e-6)
+    np.testing.assert_allclose(_np(mixed[1]), g["type2_sheet"], atol=5e-6)
+    with pytest.raises(NotImplementedError):
+        dev.curvature_to_sheet(surf, 1.0, curvtype=3)
+
+
 def test_type5_sheets_batch_equals_per_side(small, cuda):
     """Sides of one topology with their own geometry, smoothing counts and sheet
     stencils in one csg_type5_sheets_batch call equal per-side csg_type5_sheets."""
