@@ -13,14 +13,20 @@
 //   λ(p)         += Σ_c λ'_c(p) ∂I_c/∂x   (the sample position p + d(p))
 //   λ_c(corner k) += w_k(p) λ'_c(p)        (the four bilinear weights)
 // with ∂I_c/∂x from the unwrapped corner differences (the unwrap shifts are
-// piecewise constant).  The value term scatters, so λ accumulates with
-// hardware float atomics (summation order, hence the last bits, vary from
-// run to run).  The SSD adjoint is λ_K = (f(φ) - t) ∇f(φ) with the bilinear
-// derivative of initialise_objfun; ∂E/∂v = 2^-K λ_0 + L v.
+// piecewise constant).  The value term scatters; it is accumulated without
+// atomics and in a fixed order: every pixel writes its position term to its
+// own λ entry and its four corner terms, tagged with their destination, to a
+// list in source order (entry 4p + k); a stable radix sort by destination
+// groups them, and each destination adds its group in source order.  The
+// gradient is therefore the same bits on every run and in both schedules.
+// The SSD adjoint is λ_K = (f(φ) - t) ∇f(φ) with the bilinear derivative of
+// initialise_objfun; ∂E/∂v = 2^-K λ_0 + L v.
 //
 // Two schedules: "store" keeps d_0 .. d_K from the forward pass; "recompute"
 // keeps none and rebuilds d_i from v by i squarings for each backward step
 // (K(K+1)/2 extra squarings, two map buffers instead of K + 1).
+#include <hipcub/hipcub.hpp>
+
 #include "../../include/csg.h"
 #include "dartel_kernels.h"
 
@@ -99,14 +105,14 @@ __device__ __forceinline__ void corner_differences(const T *__restrict__ dc, con
     e11 = (T)(X11 - n11 * period - X22) + (d11 - d22);
 }
 
-__device__ __forceinline__ void add_atomic(float *p, float v) { unsafeAtomicAdd(p, v); }
-__device__ __forceinline__ void add_atomic(double *p, double v) { unsafeAtomicAdd(p, v); }
-
-// λ_in += adjoint of d' = d(id + d) applied to λ_out (λ_in cleared by the caller).
+// Adjoint of d' = d(id + d) for the upstream λ_out: the position term goes to
+// lam_in[p]; corner k of pixel p (22, 12, 21, 11) becomes entry e = 4p + k of
+// (dest, entry, cx, cy) = (corner pixel, e, w_k λx, w_k λy).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_square_adjoint(const T *__restrict__ d, const T *__restrict__ lam_out, int nx, int ny,
-                 T *__restrict__ lam_in)
+k_square_adjoint_emit(const T *__restrict__ d, const T *__restrict__ lam_out, int nx, int ny,
+                      T *__restrict__ lam_in, int *__restrict__ dest, int *__restrict__ entry,
+                      T *__restrict__ cx, T *__restrict__ cy)
 {
     const int P = nx * ny;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -123,17 +129,65 @@ k_square_adjoint(const T *__restrict__ d, const T *__restrict__ lam_out, int nx,
     const T gxy = ex21 * c.dx2 + (ex11 - ex12) * c.dx1;
     const T gyx = ey12 * c.dy2 + (ey11 - ey21) * c.dy1;
     const T gyy = ey21 * c.dx2 + (ey11 - ey12) * c.dx1;
-    add_atomic(lam_in + p, lx * gxx + ly * gyx);
-    add_atomic(lam_in + P + p, lx * gxy + ly * gyy);
-    const T w22 = c.dx2 * c.dy2, w12 = c.dx1 * c.dy2, w21 = c.dx2 * c.dy1, w11 = c.dx1 * c.dy1;
-    add_atomic(lam_in + c.i22, w22 * lx);
-    add_atomic(lam_in + c.i12, w12 * lx);
-    add_atomic(lam_in + c.i21, w21 * lx);
-    add_atomic(lam_in + c.i11, w11 * lx);
-    add_atomic(lam_in + P + c.i22, w22 * ly);
-    add_atomic(lam_in + P + c.i12, w12 * ly);
-    add_atomic(lam_in + P + c.i21, w21 * ly);
-    add_atomic(lam_in + P + c.i11, w11 * ly);
+    lam_in[p] = lx * gxx + ly * gyx;
+    lam_in[P + p] = lx * gxy + ly * gyy;
+    const int to[4] = {c.i22, c.i12, c.i21, c.i11};
+    const T w[4] = {c.dx2 * c.dy2, c.dx1 * c.dy2, c.dx2 * c.dy1, c.dx1 * c.dy1};
+    const int e0 = 4 * p;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        dest[e0 + k] = to[k];
+        entry[e0 + k] = e0 + k;
+        cx[e0 + k] = w[k] * lx;
+        cy[e0 + k] = w[k] * ly;
+    }
+}
+
+// lam_in[q] += the corner terms whose destination is q, in source order (the
+// run of q in the stably sorted destinations, found by binary search).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_square_adjoint_gather(const int *__restrict__ dest_sorted, const int *__restrict__ entry_sorted,
+                        const T *__restrict__ cx, const T *__restrict__ cy, int P,
+                        T *__restrict__ lam_in)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P)
+        return;
+    const int n = 4 * P;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (dest_sorted[mid] < q)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    T ax = lam_in[q], ay = lam_in[P + q];
+    for (int i = lo; i < n && dest_sorted[i] == q; ++i) {
+        const int e = entry_sorted[i];
+        ax = ax + cx[e];
+        ay = ay + cy[e];
+    }
+    lam_in[q] = ax;
+    lam_in[P + q] = ay;
+}
+
+int sort_bits(int P)
+{
+    int bits = 1;
+    while ((1LL << bits) < (long long)P)
+        ++bits;
+    return bits;
+}
+
+size_t sort_temp_bytes(int P)
+{
+    size_t need = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, need, (const int *)nullptr, (int *)nullptr,
+                                             (const int *)nullptr, (int *)nullptr, 4 * P, 0,
+                                             sort_bits(P), (hipStream_t)0);
+    return need;
 }
 
 // Per-block float64 partial of Σ v·(L v).
@@ -167,20 +221,26 @@ k_adjoint_gradient(const T *__restrict__ lam0, const T *__restrict__ lv, T scale
 }
 
 struct AdjLayout {
-    size_t maps, lam, lv, partial, total;
+    size_t maps, lam, lv, partial, lists, sort_tmp, sort_tmp_bytes, total;
     int nmaps;
 };
 
+// lists: dest, entry, dest sorted, entry sorted (4P int each), cx, cy (4P T each)
 AdjLayout adj_layout(int nx, int ny, int K, int dtype, int store)
 {
     AdjLayout l{};
-    const size_t plane = (size_t)2 * nx * ny * dtype_size(dtype);
+    const int P = nx * ny;
+    const size_t plane = (size_t)2 * P * dtype_size(dtype);
     l.nmaps = store ? K + 1 : 2;
     l.maps = 0;
     l.lam = l.maps + align256(plane) * l.nmaps;
     l.lv = l.lam + 2 * align256(plane);
     l.partial = l.lv + align256(plane);
-    l.total = l.partial + align256(sizeof(double) * (grid_for((size_t)2 * nx * ny) + 8));
+    l.lists = l.partial + align256(sizeof(double) * (grid_for((size_t)2 * P) + 8));
+    l.sort_tmp = l.lists + 4 * align256((size_t)4 * P * sizeof(int)) +
+                 2 * align256((size_t)4 * P * dtype_size(dtype));
+    l.sort_tmp_bytes = K > 0 ? sort_temp_bytes(P) : 0;
+    l.total = l.sort_tmp + align256(l.sort_tmp_bytes);
     return l;
 }
 
@@ -193,6 +253,10 @@ struct Adj {
     T *lam[2];
     T *lv;
     double *partial;
+    int *dest, *entry, *dest_sorted, *entry_sorted;
+    T *cx, *cy;
+    void *sort_tmp;
+    size_t sort_tmp_bytes;
     hipStream_t s;
 
     int P() const { return nx * ny; }
@@ -289,11 +353,16 @@ int backward(Adj<T> &a, const double *grad_scale, T *grad)
             d = a.maps[i];
         else
             CSG_TRY(rebuild<T>(a, i, &d));
-        hipLaunchKernelGGL(k_zero<T>, dim3(grid_for(n2)), dim3(kBlock), 0, a.s, a.lam[cur ^ 1],
-                           n2);
+        hipLaunchKernelGGL(k_square_adjoint_emit<T>, dim3(nb), dim3(kBlock), 0, a.s, d,
+                           a.lam[cur], a.nx, a.ny, a.lam[cur ^ 1], a.dest, a.entry, a.cx, a.cy);
         CSG_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_square_adjoint<T>, dim3(nb), dim3(kBlock), 0, a.s, d, a.lam[cur],
-                           a.nx, a.ny, a.lam[cur ^ 1]);
+        size_t tmp_bytes = a.sort_tmp_bytes;
+        CSG_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(a.sort_tmp, tmp_bytes,
+                                                         (const int *)a.dest, a.dest_sorted,
+                                                         (const int *)a.entry, a.entry_sorted,
+                                                         4 * P, 0, sort_bits(P), a.s));
+        hipLaunchKernelGGL(k_square_adjoint_gather<T>, dim3(nb), dim3(kBlock), 0, a.s,
+                           a.dest_sorted, a.entry_sorted, a.cx, a.cy, P, a.lam[cur ^ 1]);
         CSG_LAUNCH_CHECK();
         cur ^= 1;
     }
@@ -318,6 +387,8 @@ int make_adj(Adj<T> &a, const void *v, const void *f, const void *t, int nx, int
         return fail(kInvalidArgument,
                     "exp-warp-SSD needs nx >= 2, ny >= 1, 0 <= K <= %d and all inputs",
                     kMaxSquarings);
+    if ((long long)nx * ny > (1LL << 28))
+        return fail(kInvalidArgument, "exp-warp-SSD supports at most 2^28 pixels");
     const AdjLayout l = adj_layout(nx, ny, K, dtype, store);
     if (ws_bytes < l.total)
         return fail(kInvalidArgument, "exp-warp-SSD workspace too small");
@@ -337,6 +408,16 @@ int make_adj(Adj<T> &a, const void *v, const void *f, const void *t, int nx, int
     a.lam[1] = (T *)(b + l.lam + plane);
     a.lv = (T *)(b + l.lv);
     a.partial = (double *)(b + l.partial);
+    const size_t ilist = align256((size_t)4 * nx * ny * sizeof(int));
+    const size_t tlist = align256((size_t)4 * nx * ny * sizeof(T));
+    a.dest = (int *)(b + l.lists);
+    a.entry = (int *)(b + l.lists + ilist);
+    a.dest_sorted = (int *)(b + l.lists + 2 * ilist);
+    a.entry_sorted = (int *)(b + l.lists + 3 * ilist);
+    a.cx = (T *)(b + l.lists + 4 * ilist);
+    a.cy = (T *)(b + l.lists + 4 * ilist + tlist);
+    a.sort_tmp = b + l.sort_tmp;
+    a.sort_tmp_bytes = l.sort_tmp_bytes;
     a.s = S(stream);
     return kOk;
 }
@@ -347,7 +428,7 @@ extern "C" {
 
 size_t csg_exp_warp_ssd_workspace_bytes(int nx, int ny, int K, int dtype, int store)
 {
-    if (nx < 1 || ny < 1 || K < 0 || K > kMaxSquarings)
+    if (nx < 1 || ny < 1 || K < 0 || K > kMaxSquarings || (long long)nx * ny > (1LL << 28))
         return 0;
     return adj_layout(nx, ny, K, dtype, store ? 1 : 0).total + 256;
 }

@@ -174,8 +174,9 @@ int csg_solve_dartel_maps(const void *src, const void *tgt, int nx, int ny, int 
  * if parts != NULL, (SSD, membrane) to parts[0..1].  backward writes dE/dv (times
  * *grad_scale when given) to grad_v.  store = 1 keeps the K + 1 maps of the forward pass
  * in the workspace for backward (pass the same workspace, forward first); store = 0
- * rebuilds them by recomputation (two maps of workspace).  The adjoint accumulates with
- * float atomics: gradients vary in their last bits from run to run. */
+ * rebuilds them by recomputation (two maps of workspace).  The adjoint accumulates without
+ * atomics in a fixed order (stable sort of the scattered terms by destination): the same
+ * bits on every run and in both schedules.  At most 2^28 pixels. */
 size_t csg_exp_warp_ssd_workspace_bytes(int nx, int ny, int K, int dtype, int store);
 int csg_exp_warp_ssd_forward(const void *v, const void *feature, const void *target, int nx,
                              int ny, int K, double membrane, int dtype, int store,
