@@ -628,6 +628,25 @@ k_sor_patched(SorPatches pp, int ncolours, const double *__restrict__ ell_val,
     }
 }
 
+// Between two launches that share the barrier state: clear the eight claim and
+// eight arrival counters (slots 0 .. 15); the timeout flag and the placement
+// verdict stay.
+__global__ void k_clear_group_counters(unsigned *__restrict__ bar)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2 * kXcds * kBarrierSlotWords)
+        bar[i] = 0u;
+}
+
+// After an XCD-local launch: if some XCD claimed fewer workgroups than its
+// group needs, that group never completed; record it in the verdict word.
+__global__ void k_check_claims(unsigned *__restrict__ bar, SorXcdPlan plan)
+{
+    const int x = threadIdx.x;
+    if (x < kXcds && bar[(size_t)x * kBarrierSlotWords] < (unsigned)plan.blocks[x])
+        bar[(size_t)(kBarrierSlots - 2) * kBarrierSlotWords] = 1u;
+}
+
 // r = rhs - A x; per-block Σ r², Σ rhs².
 __global__ void __launch_bounds__(kBlock)
 k_residual_norms(const int *__restrict__ row_off, const int *__restrict__ cols,
@@ -790,37 +809,57 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
     if (allow_xcd_local && patches && patches->rings == co.count && co.count <= kSorRegColours &&
         W <= kSorRegWidth && cus % kXcds == 0 && a.batch >= 1 &&
         patches->max_rows <= kSorPatchRows * kSorPatchThreads) {
-        // every (patch, mesh) workgroup of an XCD must be resident at once
+        // every (patch, mesh) workgroup of an XCD must be resident at once: an
+        // XCD holds per_xcd meshes, a launch 8 per_xcd; larger batches run as
+        // consecutive launches of that many meshes (each mesh's sweeps depend
+        // only on its own rows, so the values do not depend on the split)
         const size_t lds = sizeof(double) * (size_t)patches->max_local;
         int patch_per_cu = 0;
         CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &patch_per_cu, k_sor_patched<kSorRegWidth>, kSorPatchThreads, lds));
-        SorXcdPlan pplan{};
-        pplan.force_xcd = plan.force_xcd;
-        bool fits = patch_per_cu > 0;
-        int pgrid = 0;
-        for (int xcd = 0; xcd < kXcds && fits; ++xcd) {
-            const int meshes = a.batch > xcd ? (a.batch - xcd + kXcds - 1) / kXcds : 0;
-            pplan.blocks[xcd] = patches->count * meshes;
-            fits = pplan.blocks[xcd] <= (cus / kXcds) * patch_per_cu;
-            pgrid = pplan.blocks[xcd] > pgrid ? pplan.blocks[xcd] : pgrid;
-        }
-        if (fits) {
-            pgrid = (pgrid + kXcdSpare) * kXcds;
-            TimedLaunch ptimer(kTimedSorPatched, s, algorithmic);
-            hipExtLaunchKernelGGL(k_sor_patched<kSorRegWidth>, dim3(pgrid), dim3(kSorPatchThreads),
-                               lds, s, ptimer.start(), ptimer.stop(), 0, *patches, co.count, ell_val, rhs_c, diag_c, width, omega,
-                               n, max_iter, pplan, x, a.x_alt, bar, timeout);
-            CSG_LAUNCH_CHECK();
-            ptimer.done();
-            if (run)
-                *run = SorRun{true, pplan};
+        const int per_xcd = patch_per_cu > 0 ? (cus / kXcds) * patch_per_cu / patches->count : 0;
+        if (per_xcd >= 1) {
+            const int chunk = per_xcd * kXcds;
             // after an odd number of iterations the result is in the second buffer
             const long long steps = ((long long)max_iter * co.count + patches->rings - 1) /
                                     patches->rings;
-            if (steps & 1)
-                CSG_HIP_CHECK(hipMemcpyAsync(x, a.x_alt, sizeof(double) * (size_t)batch * n,
-                                             hipMemcpyDeviceToDevice, s));
+            SorXcdPlan pplan{};
+            for (int b0 = 0; b0 < batch; b0 += chunk) {
+                const int nb = batch - b0 < chunk ? batch - b0 : chunk;
+                pplan = SorXcdPlan{};
+                pplan.force_xcd = plan.force_xcd;
+                int pgrid = 0;
+                for (int xcd = 0; xcd < kXcds; ++xcd) {
+                    const int meshes = nb > xcd ? (nb - xcd + kXcds - 1) / kXcds : 0;
+                    pplan.blocks[xcd] = patches->count * meshes;
+                    pgrid = pplan.blocks[xcd] > pgrid ? pplan.blocks[xcd] : pgrid;
+                }
+                pgrid = (pgrid + kXcdSpare) * kXcds;
+                if (b0 > 0) {
+                    hipLaunchKernelGGL(k_clear_group_counters,
+                                       dim3(grid_for((size_t)2 * kXcds * kBarrierSlotWords)),
+                                       dim3(kBlock), 0, s, bar);
+                    CSG_LAUNCH_CHECK();
+                }
+                const size_t off = (size_t)b0 * n;
+                TimedLaunch ptimer(kTimedSorPatched, s, algorithmic * nb / batch);
+                hipExtLaunchKernelGGL(k_sor_patched<kSorRegWidth>, dim3(pgrid),
+                                      dim3(kSorPatchThreads), lds, s, ptimer.start(),
+                                      ptimer.stop(), 0, *patches, co.count,
+                                      ell_val + off * width, rhs_c + off, diag_c + off, width,
+                                      omega, n, max_iter, pplan, x + off, a.x_alt + off, bar,
+                                      timeout);
+                CSG_LAUNCH_CHECK();
+                ptimer.done();
+                hipLaunchKernelGGL(k_check_claims, dim3(1), dim3(64), 0, s, bar, pplan);
+                CSG_LAUNCH_CHECK();
+                if (steps & 1)
+                    CSG_HIP_CHECK(hipMemcpyAsync(x + off, a.x_alt + off,
+                                                 sizeof(double) * (size_t)nb * n,
+                                                 hipMemcpyDeviceToDevice, s));
+            }
+            if (run)
+                *run = SorRun{true, pplan};
             return kOk;
         }
     }
@@ -872,7 +911,17 @@ bool sor_placement_complete(const DepthArgs &a, const SorRun &run, hipStream_t s
     for (int x = 0; x < kXcds; ++x)
         if ((long long)claims[(size_t)x * kBarrierSlotWords] < (long long)run.plan.blocks[x])
             return false;
-    return true;
+    // earlier launches of a split batch left their verdict in slot kBarrierSlots - 2
+    unsigned verdict = 0;
+    const hipError_t e3 = hipMemcpyAsync(&verdict, a.barrier + (size_t)(kBarrierSlots - 2) *
+                                                                  kBarrierSlotWords,
+                                         sizeof(verdict), hipMemcpyDeviceToHost, s);
+    const hipError_t e4 = e3 == hipSuccess ? hipStreamSynchronize(s) : e3;
+    if (e4 != hipSuccess) {
+        *rc = fail(kRuntimeError, "reading the SOR placement: %s", hipGetErrorString(e4));
+        return false;
+    }
+    return verdict == 0;
 }
 
 int depth_residual(const DepthArgs &a, double *out, hipStream_t s)

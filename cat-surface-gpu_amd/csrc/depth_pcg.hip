@@ -281,9 +281,10 @@ int depth_pcg(const DepthArgs &d, double *ws, int max_iter, double tol, bool all
     CSG_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     CSG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pcg<true>, kPcgThreads,
                                                                0));
-    // one row chunk per CU of the XCD, shared by the XCD's meshes
+    // one row chunk per CU of the XCD, shared by the XCD's meshes; the counters
+    // stay below the placement verdict (slot kBarrierSlots - 2) and the timeout
     bool local = allow_xcd_local && cus % kXcds == 0 && per_cu > 0 &&
-                 B <= kBarrierSlots - 1 - kPcgCounterBase;
+                 B <= kBarrierSlots - 2 - kPcgCounterBase;
     PcgPlan plan{};
     plan.force_xcd = force_xcd;
     int most = 0;

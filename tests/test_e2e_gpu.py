@@ -291,3 +291,51 @@ def test_sor_and_pcg_depth_features_on_the_hemisphere(cuda):
     print(f"rotation |d angle| SOR vs PCG features {da:.2e} rad; vertex difference {dv:.2e}",
           flush=True)
     assert da <= 1e-6
+
+
+def _surface_stencil(st, n_source):
+    import cat_surface_gpu as csg
+
+    ny, nx = st.sheet_idx.shape[:2]
+    return csg.SurfaceStencil(
+        sphere_points=st.sphere64, faces=st.faces, surface_indices=st.surf_idx,
+        surface_weights=st.surf_w, sheet_indices=st.sheet_idx, sheet_weights=st.sheet_w,
+        nx=nx, ny=ny, source_points=n_source, unit_sphere_points=st.unit_points)
+
+
+@pytest.mark.parametrize("curvtypes,fwhm", [((5, 5, 2), (5.0 / 27, 5.0 / 81, 5.0 / 243)),
+                                            ((5, 5), (5.0 / 9, 5.0 / 27)),
+                                            ((5,), (5.0 / 3,))])
+def test_solve_dartel_from_surfaces_matches_oracle(curvtypes, fwhm, cuda):
+    """solve_dartel_from_surfaces (surface_warp.py:370-488) on the oracle's stencils
+    of the 10,242-vertex pair, FP64: its curvature sheets against the oracle's
+    build_maps (within 5e-6: the vertex normals differ from the oracle's by float32
+    rounding, 2e-6), and its flow against the oracle's DARTEL solve of the same
+    sheets (within 1e-9 of the flow's scale)."""
+
+    from cat_surface_gpu import solve_dartel_from_surfaces
+    from oracle import oracle as O
+    from oracle import pipeline as P
+
+    p = _pair("icosphere5-steps3")
+    src_st = P.build_stencil(p.src_sphere, p.src_faces, normalise=True)
+    tgt_st = P.build_stencil(p.tgt_sphere, p.tgt_faces, normalise=True)
+    steps = len(curvtypes)
+    res = solve_dartel_from_surfaces(
+        p.src_surface, p.tgt_surface, _surface_stencil(src_st, p.src_surface.shape[0]),
+        _surface_stencil(tgt_st, p.tgt_surface.shape[0]), fwhm=fwhm, curvtypes=curvtypes,
+        n_steps=steps, device=cuda, dtype=torch.float64)
+    src = P.curvature_maps(p.src_surface, src_st, fwhm, curvtypes)
+    tgt = P.curvature_maps(p.tgt_surface, tgt_st, fwhm, curvtypes)
+    got_src = res.source_maps.cpu().numpy()
+    got_tgt = res.target_maps.cpu().numpy()
+    gap = max(float(np.abs(got_src - src).max()), float(np.abs(got_tgt - tgt).max()))
+    flow, _ = O.solve_dartel_maps(got_src, got_tgt, loop=6, lmreg=1e-3, cycles=3, nit=3, its=3,
+                                  code=1)
+    got = res.flow.cpu().numpy()
+    err = float(np.abs(got - flow).max()) / max(float(np.abs(flow).max()), 1e-300)
+    print(f"\n[curvtypes {curvtypes}] sheets vs oracle {gap:.2e}; flow vs oracle solve of "
+          f"the same sheets {err:.2e} (max |flow| {np.abs(flow).max():.3f})", flush=True)
+    assert got_src.shape == (steps, 256, 512) and res.flow.shape == (2, 256, 512)
+    assert gap <= 5e-6
+    assert err <= 1e-9

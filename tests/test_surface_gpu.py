@@ -352,13 +352,14 @@ def _timed_launches(name: str) -> int:
         k += 1
 
 
-@pytest.mark.parametrize("level,batch,iters,patched", [
-    (6, 1, 300, True), (6, 4, 300, True), (6, 8, 300, True), (4, 2, 301, True),
-    (5, 3, 1, True), (5, 9, 300, False)])
-def test_sor_patched_equals_per_colour(level, batch, iters, patched, cuda):
+@pytest.mark.parametrize("level,batch,iters,launches", [
+    (6, 1, 300, 1), (6, 4, 300, 1), (6, 8, 300, 1), (4, 2, 301, 1),
+    (5, 3, 1, 1), (5, 9, 300, 2), (6, 12, 301, 2), (5, 17, 300, 3)])
+def test_sor_patched_equals_per_colour(level, batch, iters, launches, cuda):
     """The SOR blocked in time on mesh patches (one exchange per sweep) gives
-    exactly the values of one synchronisation per colour; ``patched`` says
-    whether the patches fit the device (else the call falls back)."""
+    exactly the values of one synchronisation per colour.  One XCD holds one
+    mesh's 32 patches at once, so batches beyond 8 meshes run as consecutive
+    launches of 8 (``launches``), with the same values."""
 
     import cat_surface_gpu as csg
     from cat_surface_gpu import _lib
@@ -390,7 +391,7 @@ def test_sor_patched_equals_per_colour(level, batch, iters, patched, cuda):
     assert torch.isfinite(xp).all()
     assert torch.equal(xp, xg)
     assert torch.equal(rp, rg)
-    assert used == (1 if patched else 0)
+    assert used == launches
 
 
 @pytest.mark.parametrize("patched", [True, False])
