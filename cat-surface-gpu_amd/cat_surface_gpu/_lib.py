@@ -130,6 +130,21 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
                                  _d, _d, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
     "csg_rotation_costs": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp,
                                 _vp, _vp, _vp, _sz, ctypes.POINTER(_i), _vp, _vp]),
+    "csg_flag_above": (_i, [_vp, _i, _i, _vp, _vp]),
+    "csg_heat_parameters": (_i, [_vp, _vp, _i, _dp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _sz,
+                                 _vp]),
+    "csg_heat_parameters_from_edge_mean": (_i, [_vp, _dp, _i, _vp, _vp, _i, _vp, _vp, _vp]),
+    "csg_smooth_geometry_batch_dev": (_i, [_vp, _i, _vp, _vp, _i, _vp, _i, _vp, _vp, _sz, _vp]),
+    "csg_smooth_geometry_patched_dev": (_i, [_vp, _i, _i, _vp, _i, _i, _i, _vp, _vp, _vp, _vp,
+                                             _vp, _vp, _vp, _sz, _vp, _vp, _sz, _vp]),
+    "csg_smooth_values_dev": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _i,
+                                   _vp, _vp, _sz, _vp]),
+    "csg_smooth_values_patched_dev": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _vp, _vp, _i,
+                                           _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz,
+                                           _vp, _vp, _sz, _vp]),
+    "csg_type5_sheets_batch_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _vp,
+                                        _vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i,
+                                        _i, _vp, _vp, _sz, _vp]),
     "csg_curvature_type2": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "csg_curvature_sheet_workspace_bytes": (_sz, [_i, _i, _i]),
     "csg_curvature_sheet": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _i, _d, _i, _vp, _vp,

@@ -16,7 +16,7 @@ from typing import Sequence
 import torch
 
 from . import _lib
-from .surface_stencil import (SurfaceStencilDevice, _minmax, edge_means,
+from .surface_stencil import (SurfaceStencilDevice, _minmax, edge_heat,
                               heat_parameters_from_edge_mean, smooth_geometry_many, sor_patches)
 
 
@@ -180,20 +180,23 @@ def _finish_feature(stencil: SurfaceStencilDevice, geometry: torch.Tensor, value
 
 
 def _finish_features(stencils: Sequence[SurfaceStencilDevice], geometries, values, iterations,
-                     residuals, edge_means_, groups) -> tuple[RotationFeatureResult, ...]:
-    """_finish_feature for many sides: one smoothing and one normalisation per topology group."""
+                     residuals, heats, groups) -> tuple[RotationFeatureResult, ...]:
+    """_finish_feature for many sides: one smoothing and one normalisation per topology
+    group, the fwhm-50 parameters (heats: EdgeHeat per side) read on the device."""
 
     out: list[RotationFeatureResult | None] = [None] * len(stencils)
     for g in groups:
-        params = [heat_parameters_from_edge_mean(edge_means_[i], 50.0) for i in g]
+        its = torch.cat([heats[i].iters for i in g])
+        sig = torch.cat([heats[i].sigma for i in g])
         if len(g) > 1:
             geom = torch.stack([geometries[i] for i in g])
             vals = torch.stack([values[i].reshape(-1) for i in g])
-            smoothed = stencils[g[0]]._smooth_values_batch(vals, geom, params, False,
-                                                           mesh_of=range(len(g)))
+            smoothed = stencils[g[0]]._smooth_values_batch_dev(vals, geom, its, sig, False,
+                                                               mesh_of=range(len(g)))
         else:
-            smoothed = stencils[g[0]]._smooth_values_batch(values[g[0]].reshape(1, -1),
-                                                           geometries[g[0]], params, False)
+            smoothed = stencils[g[0]]._smooth_values_batch_dev(values[g[0]].reshape(1, -1),
+                                                               geometries[g[0]], its, sig,
+                                                               False)
         normed = _minmax(smoothed, 2.220446049250313e-16)
         for k, i in enumerate(g):
             out[i] = RotationFeatureResult(values=normed[k], iterations=iterations[i],
@@ -265,20 +268,18 @@ def compute_rotation_features(stencils: Sequence[SurfaceStencilDevice],
         # and shared smoothing launches (the same values as per-side calls)
         mapped = [st.resample_vertices(torch.as_tensor(v, device=st.device))
                   for st, v in zip(stencils, surfaces)]
-        ems = edge_means(stencils, mapped)
-        its = [heat_parameters_from_edge_mean(em, float(f))[0]
-               for em, f in zip(ems, heat_fwhms)]
-        geometries = smooth_geometry_many(stencils, mapped, its)
+        heats = edge_heat(stencils, mapped, [(float(f),) for f in heat_fwhms])
+        geometries = smooth_geometry_many(stencils, mapped, [h.iters[0:1] for h in heats])
     else:
         geometries = [_feature_geometry(stencils[i], surfaces[i], float(heat_fwhms[i]),
                                         smoothed_surfaces[i]) for i in range(len(stencils))]
-    final_ems = edge_means(stencils, geometries)
+    final_heats = edge_heat(stencils, geometries, [(50.0,)] * len(stencils))
     groups = _topology_groups(stencils, parallel_sides)
     if depth_values is not None:
         given = [_given_depth(stencils[i], depth_values[i]) for i in range(len(stencils))]
         zeros = [0] * len(stencils)
         return _finish_features(stencils, geometries, given, zeros, [0.0] * len(stencils),
-                                final_ems, groups)
+                                final_heats, groups)
     values: list[torch.Tensor | None] = [None] * len(stencils)
     residuals: list[float] = [0.0] * len(stencils)
     iterations: list[int] = [int(max_iter)] * len(stencils)
@@ -291,5 +292,5 @@ def compute_rotation_features(stencils: Sequence[SurfaceStencilDevice],
             values[i] = x[k] if len(g) > 1 else x
             residuals[i] = float(rel_host[k])
             iterations[i] = int(its)
-    return _finish_features(stencils, geometries, values, iterations, residuals, final_ems,
+    return _finish_features(stencils, geometries, values, iterations, residuals, final_heats,
                             groups)

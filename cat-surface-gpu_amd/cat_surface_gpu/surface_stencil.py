@@ -12,6 +12,7 @@ and cached on the device, as the reference caches its neighbour tables
 from __future__ import annotations
 
 from dataclasses import dataclass, field
+import math
 from pathlib import Path
 import struct
 from typing import Sequence
@@ -19,7 +20,7 @@ from typing import Sequence
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, schedule
 
 _HEADER = struct.Struct("<8i")
 _MAGIC = 0x46534354
@@ -604,15 +605,7 @@ class SurfaceStencilDevice:
     def edge_mean_distance(self, vertices: torch.Tensor) -> float:
         """Mean face-edge length (surface_stencil.py:349-357); synchronises."""
 
-        v = torch.as_tensor(vertices, dtype=torch.float32, device=self.device).contiguous()
-        nf = int(self.faces.shape[0])
-        partial = torch.empty(_lib.lib().csg_edge_mean_workspace_bytes(nf) // 8,
-                              dtype=torch.float64, device=self.device)
-        nb = _lib.ctypes.c_int()
-        _lib.check(_lib.lib().csg_edge_sum(_lib.ptr(v), _lib.ptr(self.faces), nf,
-                                           _lib.ptr(partial), _lib.ctypes.byref(nb),
-                                           self._stream()), "csg_edge_sum")
-        return float(partial[: nb.value].sum().item()) / (3.0 * nf)
+        return edge_means([self], [vertices])[0]
 
     def _heat_parameters(self, vertices: torch.Tensor, fwhm: float) -> tuple[int, float]:
         return heat_parameters_from_edge_mean(self.edge_mean_distance(vertices), fwhm)
@@ -669,6 +662,45 @@ class SurfaceStencilDevice:
         self._call("csg_smooth_values", _lib.ptr(vals), n, nb, _lib.ptr(geom), meshes, mesh,
                    _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), topo.nnz, sig, its,
                    int(squared_norm), _lib.ptr(out), _lib.ptr(ws), ws.numel(), self._stream())
+        return out
+
+    def _smooth_values_batch_dev(self, values: torch.Tensor, geometry: torch.Tensor,
+                                 iters: torch.Tensor, sigma: torch.Tensor, squared_norm: bool,
+                                 mesh_of: Sequence[int] | None = None) -> torch.Tensor:
+        """_smooth_values_batch with the rows' iterations and widths in device
+        arrays (iters int32 [nb], sigma float64 [nb]); launches to
+        schedule.launch_bound."""
+
+        topo = self.topology
+        geom = geometry.to(torch.float32).contiguous()
+        meshes = 1 if geom.ndim == 2 else int(geom.shape[0])
+        n = int(geom.shape[-2])
+        nb = int(iters.numel())
+        its = iters.to(torch.int32).contiguous()
+        sig = sigma.to(torch.float64).contiguous()
+        bound = schedule.launch_bound([its], "value smoothing iterations")
+        vals = values.to(torch.float64).contiguous()
+        out = torch.empty((nb, n), dtype=torch.float64, device=self.device)
+        ws = _lib.workspace(_lib.lib().csg_smooth_values_workspace_bytes(n, topo.nnz, nb),
+                            self.device)
+        mesh = None if mesh_of is None else (_lib.ctypes.c_int * nb)(*[int(m) for m in mesh_of])
+        if _VALUE_PATCHES is not None and topo.nbr_off_np is not None:
+            pp = smoothing_patches(topo, self.sphere_points, core=_VALUE_PATCHES[0],
+                                   rings=_VALUE_PATCHES[1])
+            if pp.value_lds_bytes <= 160 * 1024:
+                self._call("csg_smooth_values_patched_dev", _lib.ptr(vals), n, nb,
+                           _lib.ptr(geom), meshes, mesh, _lib.ptr(topo.nbr_off),
+                           _lib.ptr(topo.nbr), topo.nnz, _lib.ptr(sig), _lib.ptr(its),
+                           int(bound), int(squared_norm), pp.count, pp.rings, _lib.ptr(pp.perm),
+                           _lib.ptr(pp.vstart), _lib.ptr(pp.ring_end), _lib.ptr(pp.lverts),
+                           _lib.ptr(pp.aoff), _lib.ptr(pp.astart), _lib.ptr(pp.ladj),
+                           pp.value_lds_bytes, _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+                           self._stream())
+                return out
+        self._call("csg_smooth_values_dev", _lib.ptr(vals), n, nb, _lib.ptr(geom), meshes, mesh,
+                   _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), topo.nnz, _lib.ptr(sig),
+                   _lib.ptr(its), int(bound), int(squared_norm), _lib.ptr(out), _lib.ptr(ws),
+                   ws.numel(), self._stream())
         return out
 
     def smooth_values(self, values: torch.Tensor, vertices: torch.Tensor,
@@ -811,25 +843,97 @@ class SurfaceStencilDevice:
                             for f, t in zip(fwhms, types)])
 
 
-def edge_means(stencils, geometries) -> list[float]:
-    """edge_mean_distance of several meshes with a single host synchronisation
-    (surface_stencil.py:349-357): the same per-mesh partial sums and totals."""
+# ---- heat-kernel parameters on the device
 
-    totals, nfs = [], []
-    for st, g in zip(stencils, geometries):
+_POW_ROOTS: dict[str, tuple] = {}
+_POW_ROOT_LIMIT = 1 << 20
+
+
+def _pow_root_table(device: torch.device):
+    """Counts k <= 2^20 whose k ** 0.5 (the host's libm pow, as
+    heat_parameters_from_edge_mean evaluates it) differs from the correctly
+    rounded square root, with those pow values, on the device (built once)."""
+
+    key = str(device)
+    hit = _POW_ROOTS.get(key)
+    if hit is None:
+        ks, vs = [], []
+        for k in range(1, _POW_ROOT_LIMIT + 1):
+            r = k ** 0.5
+            if r != math.sqrt(k):
+                ks.append(k)
+                vs.append(r)
+        hit = (torch.tensor(ks, dtype=torch.int32, device=device),
+               torch.tensor(vs, dtype=torch.float64, device=device), len(ks))
+        _POW_ROOTS[key] = hit
+    return hit
+
+
+@dataclass
+class EdgeHeat:
+    """One mesh's heat-kernel parameters for a list of fwhms, on the device
+    (csg_heat_parameters): iterations [k] int32, sigma [k] float64 and the mean
+    face-edge length [1] float64 (surface_stencil.py:349-370)."""
+
+    fwhms: tuple
+    iters: torch.Tensor
+    sigma: torch.Tensor
+    edge_mean: torch.Tensor
+
+    def host(self) -> list[tuple[int, float]]:
+        """(iterations, sigma) per fwhm (synchronises)."""
+
+        its = self.iters.cpu().tolist()
+        sig = self.sigma.cpu().tolist()
+        return list(zip(its, sig))
+
+
+def edge_heat(stencils, geometries, fwhm_lists) -> list[EdgeHeat]:
+    """Heat parameters of several meshes for their fwhm lists, computed and kept
+    on the device (no synchronisation)."""
+
+    out = []
+    for st, g, fws in zip(stencils, geometries, fwhm_lists):
         v = torch.as_tensor(g, dtype=torch.float32, device=st.device).contiguous()
         nf = int(st.faces.shape[0])
-        partial = torch.empty(_lib.lib().csg_edge_mean_workspace_bytes(nf) // 8,
-                              dtype=torch.float64, device=st.device)
-        nb = _lib.ctypes.c_int()
-        _lib.check(_lib.lib().csg_edge_sum(_lib.ptr(v), _lib.ptr(st.faces), nf, _lib.ptr(partial),
-                                           _lib.ctypes.byref(nb), st._stream()), "csg_edge_sum")
-        totals.append(partial[: nb.value].sum())
-        nfs.append(nf)
-    if not totals:
+        fws = tuple(float(f) for f in fws)
+        ws = _lib.workspace(_lib.lib().csg_edge_mean_workspace_bytes(nf), st.device)
+        iters = torch.empty(len(fws), dtype=torch.int32, device=st.device)
+        sigma = torch.empty(len(fws), dtype=torch.float64, device=st.device)
+        em = torch.empty(1, dtype=torch.float64, device=st.device)
+        rn, rv, nr = _pow_root_table(st.device)
+        _lib.check(_lib.lib().csg_heat_parameters(
+            _lib.ptr(v), _lib.ptr(st.faces), nf, _lib.doubles(fws), len(fws), _lib.ptr(rn),
+            _lib.ptr(rv), nr, _lib.ptr(iters), _lib.ptr(sigma), _lib.ptr(em), _lib.ptr(ws),
+            ws.numel(), st._stream()), "csg_heat_parameters")
+        out.append(EdgeHeat(fws, iters, sigma, em))
+    return out
+
+
+def heat_from_edge_mean(heat: EdgeHeat, fwhms) -> EdgeHeat:
+    """Parameters for other fwhms from an EdgeHeat's edge mean (on the device)."""
+
+    dev = heat.edge_mean.device
+    fws = tuple(float(f) for f in fwhms)
+    iters = torch.empty(len(fws), dtype=torch.int32, device=dev)
+    sigma = torch.empty(len(fws), dtype=torch.float64, device=dev)
+    rn, rv, nr = _pow_root_table(dev)
+    _lib.check(_lib.lib().csg_heat_parameters_from_edge_mean(
+        _lib.ptr(heat.edge_mean), _lib.doubles(fws), len(fws), _lib.ptr(rn), _lib.ptr(rv), nr,
+        _lib.ptr(iters), _lib.ptr(sigma), _lib.stream_handle(dev)),
+        "csg_heat_parameters_from_edge_mean")
+    return EdgeHeat(fws, iters, sigma, heat.edge_mean)
+
+
+def edge_means(stencils, geometries) -> list[float]:
+    """edge_mean_distance of several meshes with a single host synchronisation
+    (surface_stencil.py:349-357): each mesh's edge-sum partials added in order on
+    the device (the edge mean csg_heat_parameters uses)."""
+
+    heats = edge_heat(stencils, geometries, [()] * len(stencils))
+    if not heats:
         return []
-    host = torch.stack(totals).cpu().tolist()
-    return [float(t) / (3.0 * nf) for t, nf in zip(host, nfs)]
+    return torch.cat([h.edge_mean for h in heats]).cpu().tolist()
 
 
 # Patch shape (core vertices, rings = iterations per launch) of the batched
@@ -878,8 +982,14 @@ def smooth_geometry_many(stencils, geometries, iterations) -> list[torch.Tensor]
     """SurfaceStencilDevice.smooth_geometry of several meshes with given
     iteration counts; meshes that share a topology are smoothed in the same
     launches (csg_smooth_geometry_patched, or csg_smooth_geometry_batch when
-    patches are off).  Results equal one call per mesh."""
+    patches are off).  Results equal one call per mesh.  ``iterations`` holds
+    host ints, or one-element device int32 tensors (EdgeHeat.iters slices):
+    then the _dev entries read the counts on the device and the launches run to
+    schedule.launch_bound."""
 
+    device_counts = len(iterations) > 0 and isinstance(iterations[0], torch.Tensor)
+    bound = (schedule.launch_bound(list(iterations), "geometry smoothing iterations")
+             if device_counts else 0)
     out: list[torch.Tensor | None] = [None] * len(stencils)
     groups: dict[int, list[int]] = {}
     for i, st in enumerate(stencils):
@@ -895,6 +1005,13 @@ def smooth_geometry_many(stencils, geometries, iterations) -> list[torch.Tensor]
             res = torch.empty_like(v)
             ws = _lib.workspace(_lib.lib().csg_smooth_geometry_batch_workspace_bytes(
                 n, len(chunk)), dev)
+            if device_counts:
+                its_dev = torch.cat([iterations[i].reshape(1) for i in chunk]).to(
+                    torch.int32).contiguous()
+                _smooth_geometry_dev(stencils[chunk[0]], v, its_dev, bound, res, ws)
+                for k, i in enumerate(chunk):
+                    out[i] = res[k]
+                continue
             its = (_lib.ctypes.c_int * len(chunk))(*[int(iterations[i]) for i in chunk])
             if _GEOMETRY_PATCHES is not None and topo.nbr_off_np is not None:
                 # the partition follows the stencil's sphere (compact patches)
@@ -917,19 +1034,39 @@ def smooth_geometry_many(stencils, geometries, iterations) -> list[torch.Tensor]
     return out
 
 
+def _smooth_geometry_dev(st, v: torch.Tensor, its_dev: torch.Tensor, bound: int,
+                         res: torch.Tensor, ws: torch.Tensor) -> None:
+    topo = st.topology
+    batch, n = int(v.shape[0]), int(v.shape[1])
+    if _GEOMETRY_PATCHES is not None and topo.nbr_off_np is not None:
+        pp = smoothing_patches(topo, st.sphere_points, core=_GEOMETRY_PATCHES[0],
+                               rings=_GEOMETRY_PATCHES[1])
+        _lib.check(_lib.lib().csg_smooth_geometry_patched_dev(
+            _lib.ptr(v), batch, n, _lib.ptr(its_dev), int(bound), pp.count, pp.rings,
+            _lib.ptr(pp.perm), _lib.ptr(pp.vstart), _lib.ptr(pp.ring_end), _lib.ptr(pp.lverts),
+            _lib.ptr(pp.aoff), _lib.ptr(pp.astart), _lib.ptr(pp.ladj), pp.lds_bytes,
+            _lib.ptr(res), _lib.ptr(ws), ws.numel(), st._stream()),
+            "csg_smooth_geometry_patched_dev")
+        return
+    _lib.check(_lib.lib().csg_smooth_geometry_batch_dev(
+        _lib.ptr(v), batch, _lib.ptr(topo.nbr_off), _lib.ptr(topo.nbr), n, _lib.ptr(its_dev),
+        int(bound), _lib.ptr(res), _lib.ptr(ws), ws.numel(), st._stream()),
+        "csg_smooth_geometry_batch_dev")
+
+
 def prepare_type5(stencils, geometries) -> tuple[list, list, list]:
     """The fwhm-independent part of curvature_type5_to_sheet_many for several
-    (stencil, resampled geometry) pairs: float32 geometries, edge means (one
-    host synchronisation) and the fwhm-50 smoothed geometries (shared launches).
+    (stencil, resampled geometry) pairs: float32 geometries, their device heat
+    parameters (EdgeHeat, fwhm 50) and the fwhm-50 smoothed geometries (shared
+    launches).
     Pairs with different fwhm lists can share it (type5_sheets_many's
     ``prepared``)."""
 
     geoms = [torch.as_tensor(g, dtype=torch.float32, device=st.device).contiguous()
              for st, g in zip(stencils, geometries)]
-    ems = edge_means(stencils, geoms)
-    geom_iters = [heat_parameters_from_edge_mean(em, 50.0)[0] for em in ems]
-    smoothed = smooth_geometry_many(stencils, geoms, geom_iters)
-    return geoms, ems, smoothed
+    heats = edge_heat(stencils, geoms, [(50.0,)] * len(geoms))
+    smoothed = smooth_geometry_many(stencils, geoms, [h.iters[0:1] for h in heats])
+    return geoms, heats, smoothed
 
 
 def type5_sheets_many(stencils, geometries, fwhms, out_dtype=torch.float64, *,
@@ -941,8 +1078,8 @@ def type5_sheets_many(stencils, geometries, fwhms, out_dtype=torch.float64, *,
     equal to the per-pair calls."""
 
     fw = tuple(float(f) for f in fwhms)
-    geoms, ems, smoothed = (prepared if prepared is not None
-                            else prepare_type5(stencils, geometries))
+    geoms, heats, smoothed = (prepared if prepared is not None
+                              else prepare_type5(stencils, geometries))
     out: list[torch.Tensor | None] = [None] * len(stencils)
     if not fw:
         return [torch.empty((0, st.ny, st.nx), dtype=out_dtype, device=st.device)
@@ -956,15 +1093,16 @@ def type5_sheets_many(stencils, geometries, fwhms, out_dtype=torch.float64, *,
         for start in range(0, len(members), per_call):
             chunk = members[start:start + per_call]
             out_chunk = _type5_batch([stencils[i] for i in chunk], [geoms[i] for i in chunk],
-                                     [smoothed[i] for i in chunk], [ems[i] for i in chunk], fw,
+                                     [smoothed[i] for i in chunk], [heats[i] for i in chunk], fw,
                                      out_dtype)
             for k, i in enumerate(chunk):
                 out[i] = out_chunk[k]
     return out
 
 
-def _type5_batch(stencils, geoms, smoothed, ems, fw, out_dtype) -> torch.Tensor:
-    """csg_type5_sheets_batch over sides of one topology and sheet shape."""
+def _type5_batch(stencils, geoms, smoothed, heats, fw, out_dtype) -> torch.Tensor:
+    """csg_type5_sheets_batch_dev over sides of one topology and sheet shape, the
+    rows' heat parameters computed on the device from each side's edge mean."""
 
     st0 = stencils[0]
     topo = st0.topology
@@ -973,19 +1111,20 @@ def _type5_batch(stencils, geoms, smoothed, ems, fw, out_dtype) -> torch.Tensor:
     m = st0.nx * st0.ny
     g = torch.stack(geoms).contiguous()
     sm = torch.stack(smoothed).contiguous()
-    params = [heat_parameters_from_edge_mean(em, f) for em in ems for f in fw]
-    sig = _lib.doubles([p[1] for p in params])
-    its = (_lib.ctypes.c_int * len(params))(*[int(p[0]) for p in params])
+    rows = [heat_from_edge_mean(h, fw) for h in heats]
+    its = torch.cat([r.iters for r in rows]).contiguous()
+    sig = torch.cat([r.sigma for r in rows]).contiguous()
+    bound = schedule.launch_bound([its], "sheet smoothing iterations")
     idx = (_lib.ctypes.c_void_p * sides)(*[_lib.ptr(st.sheet_indices) for st in stencils])
     wts = (_lib.ctypes.c_void_p * sides)(*[_lib.ptr(st.sheet_weights) for st in stencils])
     out = torch.empty((sides, nfw, st0.ny, st0.nx), dtype=out_dtype, device=st0.device)
     ws = _lib.workspace(_lib.lib().csg_type5_sheets_batch_workspace_bytes(
         n, topo.nnz, sides, nfw, m), st0.device)
-    _lib.check(_lib.lib().csg_type5_sheets_batch(
+    _lib.check(_lib.lib().csg_type5_sheets_batch_dev(
         _lib.ptr(g), _lib.ptr(sm), n, sides, _lib.ptr(topo.faces), _lib.ptr(topo.nbr_off),
-        _lib.ptr(topo.nbr), topo.nnz, _lib.ptr(topo.inc_off), _lib.ptr(topo.inc), nfw, sig, its,
-        idx, wts, m, _lib.dtype_code(out_dtype), _lib.ptr(out), _lib.ptr(ws), ws.numel(),
-        st0._stream()), "csg_type5_sheets_batch")
+        _lib.ptr(topo.nbr), topo.nnz, _lib.ptr(topo.inc_off), _lib.ptr(topo.inc), nfw,
+        _lib.ptr(sig), _lib.ptr(its), int(bound), idx, wts, m, _lib.dtype_code(out_dtype),
+        _lib.ptr(out), _lib.ptr(ws), ws.numel(), st0._stream()), "csg_type5_sheets_batch_dev")
     return out
 
 

@@ -329,11 +329,11 @@ def solve_dartel_from_surfaces_jobs(requests, *, n_steps: int = 2, loop: int = 6
             by_fwhm.setdefault(f, []).append(i)
         # the fwhm-50 geometry smoothing does not depend on the fwhms: every side's
         # in the same launches
-        g32, ems, smoothed = prepare_type5(stencils, geoms)
+        g32, heats, smoothed = prepare_type5(stencils, geoms)
         for f, members in by_fwhm.items():
             res = type5_sheets_many([stencils[i] for i in members], [geoms[i] for i in members],
                                     f, dtype, prepared=([g32[i] for i in members],
-                                                        [ems[i] for i in members],
+                                                        [heats[i] for i in members],
                                                         [smoothed[i] for i in members]))
             for i, r in zip(members, res):
                 sheets[i] = r

@@ -61,6 +61,32 @@ int make_batch(int nb, const double *sigma, const int *iterations, HeatBatch &hb
     return kOk;
 }
 
+// The same batch with the counts and widths in device memory (iters_dev,
+// sigma_dev: nb entries each); host loops run to `bound` and rows past their
+// count are copied through.
+int make_batch_dev(int nb, const double *sigma_dev, const int *iters_dev, int bound,
+                   HeatBatch &hb, const int *mesh = nullptr, int meshes = 1)
+{
+    if (nb < 1 || nb > kMaxHeatBatch)
+        return fail(kInvalidArgument, "value smoothing batch must be in [1, %d], got %d",
+                    kMaxHeatBatch, nb);
+    if (!sigma_dev || !iters_dev || bound < 0)
+        return fail(kInvalidArgument, "device heat parameters and a non-negative bound are "
+                                      "required");
+    hb.nb = nb;
+    for (int i = 0; i < nb; ++i) {
+        hb.counts[i] = bound;
+        hb.sigma[i] = 0.0;
+        hb.mesh[i] = mesh ? mesh[i] : 0;
+        if (hb.mesh[i] < 0 || hb.mesh[i] >= meshes)
+            return fail(kInvalidArgument, "value smoothing mesh index %d out of range",
+                        hb.mesh[i]);
+    }
+    hb.dcounts = iters_dev;
+    hb.dsigma = sigma_dev;
+    return kOk;
+}
+
 // values (already in h.buf[0], nb rows of n) smoothed in place; returns the buffer
 // holding the result.
 int run_heat(const float *geom, const int *off, const int *nbr, int n, int nnz,
@@ -78,6 +104,99 @@ int run_heat(const float *geom, const int *off, const int *nbr, int n, int nnz,
     }
     *result = h.buf[cur];
     return kOk;
+}
+
+// flag[0] = 1 when some value exceeds bound (threads that find one store 1).
+__global__ void k_flag_above(const int *__restrict__ values, int n, int bound,
+                             int *__restrict__ flag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && values[i] > bound)
+        flag[0] = 1;
+}
+
+// csg_smooth_values_patched after its batch is set up (host or device counts).
+int smooth_values_patched_run(const double *values, int n, int nb, const float *geom,
+                              const int *nbr_off, const int *nbr, int nnz, const HeatBatch &hb,
+                              int squared_norm, int npatches, int rings, const int *perm,
+                              const int *vstart, const int *ring_end, const int *lverts,
+                              const int *aoff, const int *astart, const unsigned short *ladj,
+                              size_t lds_bytes, double *out, void *ws, size_t ws_bytes,
+                              hipStream_t s)
+{
+    if (ws_bytes < csg_smooth_values_workspace_bytes(n, nnz, nb))
+        return fail(kInvalidArgument, "smooth_values workspace too small");
+    if (npatches < 1 || npatches > 65535 || rings < 1 || rings > 64 || !perm || !vstart ||
+        !ring_end || !lverts || !aoff || !astart || !ladj)
+        return fail(kInvalidArgument, "invalid smoothing patches");
+    if (lds_bytes < 16 || lds_bytes > 160 * 1024)
+        return fail(kInvalidArgument, "patch LDS size must be in [16 B, 160 KiB]");
+    Carve c{(char *)ws, 0};
+    HeatWs h = carve_heat(c, n, nnz, nb);
+    const SmoothPatches pp{npatches, rings, vstart, ring_end, lverts, aoff, astart, perm, ladj};
+    CSG_TRY(launch_heat_weights(geom, nbr_off, nbr, n, hb, squared_norm, (size_t)nnz, h.w,
+                                h.wsum, s));
+    int most = 0;
+    for (int i = 0; i < hb.nb; ++i)
+        most = hb.counts[i] > most ? hb.counts[i] : most;
+    // each launch advances every row by up to `rings` iterations and writes all
+    // of its vertices (the patch cores partition the mesh)
+    const double *cur = values;
+    int which = 0;
+    for (int it0 = 0; it0 < most; it0 += rings) {
+        double *nxt = (it0 + rings >= most) ? out : h.buf[which];
+        CSG_TRY(launch_smooth_values_patched(cur, nbr_off, h.w, n, (size_t)nnz, hb, pp, it0,
+                                             lds_bytes, nxt, s));
+        cur = nxt;
+        which ^= 1;
+    }
+    if (most == 0)
+        CSG_HIP_CHECK(hipMemcpyAsync(out, values, sizeof(double) * n * nb,
+                                     hipMemcpyDeviceToDevice, s));
+    return kOk;
+}
+
+// csg_type5_sheets_batch after its batch is set up (host or device counts).
+int type5_batch_run(const float *geom, const float *smoothed, int n, int sides, const int *faces,
+                    const int *nbr_off, const int *nbr, int nnz, const int *inc_off,
+                    const int *inc, int nfw, const HeatBatch &hb, const int *const *sheet_idx,
+                    const double *const *sheet_w, int m, int out_dtype, void *out, void *ws,
+                    size_t ws_bytes, hipStream_t s)
+{
+    const int nb = hb.nb;
+    if (ws_bytes < csg_type5_sheets_batch_workspace_bytes(n, nnz, sides, nfw, m))
+        return fail(kInvalidArgument, "batched type-5 workspace too small");
+    for (int k = 0; k < sides; ++k)
+        if (!sheet_idx[k] || !sheet_w[k])
+            return fail(kInvalidArgument, "batched type-5: side %d has no sheet stencil", k);
+    Carve c{(char *)ws, 0};
+    float *nr = c.take<float>((size_t)3 * n * sides);
+    double *base = c.take<double>((size_t)n * sides);
+    HeatWs h = carve_heat(c, n, nnz, nb);
+    double *normed = c.take<double>((size_t)nb * n);
+    double *sheets = c.take<double>((size_t)nb * m);
+    double *mm = c.take<double>((size_t)nb * 64 * 2 + 64);
+    for (int k = 0; k < sides; ++k) {
+        const size_t g = (size_t)3 * n * k;
+        CSG_TRY(launch_vertex_normals(smoothed + g, faces, inc_off, inc, n, nr + g, s));
+        CSG_TRY(launch_type5_base(geom + g, smoothed + g, nr + g, n, base + (size_t)n * k, s));
+        for (int f = 0; f < nfw; ++f)
+            CSG_HIP_CHECK(hipMemcpyAsync(h.buf[0] + (size_t)(k * nfw + f) * n,
+                                         base + (size_t)n * k, sizeof(double) * n,
+                                         hipMemcpyDeviceToDevice, s));
+    }
+    double *res = nullptr;
+    CSG_TRY(run_heat(geom, nbr_off, nbr, n, nnz, hb, 1, h, &res, s));
+    CSG_TRY(launch_minmax_normalise<double>(res, (size_t)n, nb, (size_t)n, 0.0, mm, normed,
+                                            (size_t)n, s));
+    for (int k = 0; k < sides; ++k)
+        CSG_TRY(launch_sheet_gather(normed + (size_t)k * nfw * n, (size_t)n, sheet_idx[k],
+                                    sheet_w[k], m, nfw, sheets + (size_t)k * nfw * m, s));
+    if (out_dtype == 0)
+        return launch_minmax_normalise<float>(sheets, (size_t)m, nb, (size_t)m, 0.0, mm,
+                                              (float *)out, (size_t)m, s);
+    return launch_minmax_normalise<double>(sheets, (size_t)m, nb, (size_t)m, 0.0, mm,
+                                           (double *)out, (size_t)m, s);
 }
 
 }  // namespace
@@ -247,37 +366,9 @@ int csg_smooth_values_patched(const double *values, int n, int nb, const float *
 {
     HeatBatch hb;
     CSG_TRY(make_batch(nb, sigma, iterations, hb, mesh_of, meshes < 1 ? 1 : meshes));
-    if (ws_bytes < csg_smooth_values_workspace_bytes(n, nnz, nb))
-        return fail(kInvalidArgument, "smooth_values workspace too small");
-    if (npatches < 1 || npatches > 65535 || rings < 1 || rings > 64 || !perm || !vstart ||
-        !ring_end || !lverts || !aoff || !astart || !ladj)
-        return fail(kInvalidArgument, "invalid smoothing patches");
-    if (lds_bytes < 16 || lds_bytes > 160 * 1024)
-        return fail(kInvalidArgument, "patch LDS size must be in [16 B, 160 KiB]");
-    hipStream_t s = S(stream);
-    Carve c{(char *)ws, 0};
-    HeatWs h = carve_heat(c, n, nnz, nb);
-    const SmoothPatches pp{npatches, rings, vstart, ring_end, lverts, aoff, astart, perm, ladj};
-    CSG_TRY(launch_heat_weights(geom, nbr_off, nbr, n, hb, squared_norm, (size_t)nnz, h.w,
-                                h.wsum, s));
-    int most = 0;
-    for (int i = 0; i < hb.nb; ++i)
-        most = hb.counts[i] > most ? hb.counts[i] : most;
-    // each launch advances every row by up to `rings` iterations and writes all
-    // of its vertices (the patch cores partition the mesh)
-    const double *cur = values;
-    int which = 0;
-    for (int it0 = 0; it0 < most; it0 += rings) {
-        double *nxt = (it0 + rings >= most) ? out : h.buf[which];
-        CSG_TRY(launch_smooth_values_patched(cur, nbr_off, h.w, n, (size_t)nnz, hb, pp, it0,
-                                             lds_bytes, nxt, s));
-        cur = nxt;
-        which ^= 1;
-    }
-    if (most == 0)
-        CSG_HIP_CHECK(hipMemcpyAsync(out, values, sizeof(double) * n * nb,
-                                     hipMemcpyDeviceToDevice, s));
-    return kOk;
+    return smooth_values_patched_run(values, n, nb, geom, nbr_off, nbr, nnz, hb, squared_norm,
+                                     npatches, rings, perm, vstart, ring_end, lverts, aoff,
+                                     astart, ladj, lds_bytes, out, ws, ws_bytes, S(stream));
 }
 
 size_t csg_minmax_workspace_bytes(int nb) { return (size_t)nb * 64 * 2 * 8 + 1024; }
@@ -437,6 +528,143 @@ int csg_curvature_sheet(const float *geom, int n, const int *faces, const int *n
                                            (double *)out, (size_t)m, s);
 }
 
+// ---- the same operators with device-resident heat parameters.  The counts
+// (and widths) come from csg_heat_parameters on the device; the host launches
+// up to `bound` iterations and rows or meshes past their count are copied
+// through, so the values equal the host-count entries' whenever every count
+// is at most `bound` (the caller checks that on the device).
+
+int csg_flag_above(const int *values, int n, int bound, int *flag, void *stream)
+{
+    if (n < 0 || (n > 0 && (!values || !flag)))
+        return fail(kInvalidArgument, "invalid flag_above arguments");
+    if (n == 0)
+        return kOk;
+    hipLaunchKernelGGL(k_flag_above, dim3(grid_for(n)), dim3(kBlock), 0, S(stream), values, n,
+                       bound, flag);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+int csg_heat_parameters(const float *v, const int *faces, int nf, const double *fwhm, int nfw,
+                        const int *root_n, const double *root_v, int nroots, int *iters_out,
+                        double *sigma_out, double *em_out, void *ws, size_t ws_bytes,
+                        void *stream)
+{
+    if (nf < 1 || nfw < 0 || nfw > kMaxHeatFwhm || (nfw > 0 && (!fwhm || !iters_out ||
+                                                                 !sigma_out)))
+        return fail(kInvalidArgument, "heat parameters: nf >= 1 and at most %d fwhms",
+                    kMaxHeatFwhm);
+    if (nroots < 0 || (nroots > 0 && (!root_n || !root_v)))
+        return fail(kInvalidArgument, "heat parameters: invalid root table");
+    if (ws_bytes < csg_edge_mean_workspace_bytes(nf))
+        return fail(kInvalidArgument, "heat parameters workspace too small");
+    hipStream_t s = S(stream);
+    int nb = 0;
+    CSG_TRY(launch_edge_sum(v, faces, nf, (double *)ws, &nb, s));
+    return launch_heat_parameters((const double *)ws, nb, nf, fwhm, nfw, root_n, root_v, nroots,
+                                  iters_out, sigma_out, em_out, s);
+}
+
+int csg_heat_parameters_from_edge_mean(const double *em, const double *fwhm, int nfw,
+                                       const int *root_n, const double *root_v, int nroots,
+                                       int *iters_out, double *sigma_out, void *stream)
+{
+    if (!em || nfw < 1 || nfw > kMaxHeatFwhm || !fwhm || !iters_out || !sigma_out)
+        return fail(kInvalidArgument, "heat parameters: an edge mean and 1..%d fwhms",
+                    kMaxHeatFwhm);
+    if (nroots < 0 || (nroots > 0 && (!root_n || !root_v)))
+        return fail(kInvalidArgument, "heat parameters: invalid root table");
+    return launch_heat_parameters(em, -1, 1, fwhm, nfw, root_n, root_v, nroots, iters_out,
+                                  sigma_out, nullptr, S(stream));
+}
+
+int csg_smooth_geometry_batch_dev(const float *v, int batch, const int *nbr_off, const int *nbr,
+                                  int n, const int *iters_dev, int bound, float *out, void *ws,
+                                  size_t ws_bytes, void *stream)
+{
+    if (n < 1 || batch < 1 || batch > kMaxSmoothBatch || !iters_dev || bound < 0)
+        return fail(kInvalidArgument, "invalid batched smoothing arguments (batch <= %d)",
+                    kMaxSmoothBatch);
+    if (ws_bytes < csg_smooth_geometry_batch_workspace_bytes(n, batch))
+        return fail(kInvalidArgument, "smooth_geometry_batch workspace too small");
+    SmoothBatch sb{};
+    sb.count = batch;
+    for (int b = 0; b < batch; ++b)
+        sb.iters[b] = bound;
+    sb.diters = iters_dev;
+    float4 *b0 = (float4 *)ws;
+    return launch_smooth_geometry4(v, b0, b0 + (size_t)n * batch, nbr_off, nbr, n, sb, bound, out,
+                                   S(stream));
+}
+
+int csg_smooth_geometry_patched_dev(const float *v, int batch, int n, const int *iters_dev,
+                                    int bound, int npatches, int rings, const int *perm,
+                                    const int *vstart, const int *ring_end, const int *lverts,
+                                    const int *aoff, const int *astart,
+                                    const unsigned short *ladj, size_t lds_bytes, float *out,
+                                    void *ws, size_t ws_bytes, void *stream)
+{
+    if (n < 1 || batch < 1 || batch > kMaxSmoothBatch || !iters_dev || bound < 0)
+        return fail(kInvalidArgument, "invalid patched smoothing arguments (batch <= %d)",
+                    kMaxSmoothBatch);
+    if (npatches < 1 || npatches > 65535 || rings < 1 || rings > 64 || !perm || !vstart ||
+        !ring_end || !lverts || !aoff || !astart || !ladj)
+        return fail(kInvalidArgument, "invalid smoothing patches");
+    if (lds_bytes < sizeof(float4) || lds_bytes > 160 * 1024)
+        return fail(kInvalidArgument, "patch LDS size must be in (0, 160 KiB]");
+    if (ws_bytes < csg_smooth_geometry_batch_workspace_bytes(n, batch))
+        return fail(kInvalidArgument, "patched smoothing workspace too small");
+    SmoothBatch sb{};
+    sb.count = batch;
+    for (int b = 0; b < batch; ++b)
+        sb.iters[b] = bound;
+    sb.diters = iters_dev;
+    SmoothPatches pp{npatches, rings, vstart, ring_end, lverts, aoff, astart, perm, ladj};
+    float4 *b0 = (float4 *)ws;
+    return launch_smooth_geometry_patched(v, b0, b0 + (size_t)n * batch, n, pp, sb, bound,
+                                          lds_bytes, out, S(stream));
+}
+
+int csg_smooth_values_dev(const double *values, int n, int nb, const float *geom, int meshes,
+                          const int *mesh_of, const int *nbr_off, const int *nbr, int nnz,
+                          const double *sigma_dev, const int *iters_dev, int bound,
+                          int squared_norm, double *out, void *ws, size_t ws_bytes, void *stream)
+{
+    HeatBatch hb;
+    CSG_TRY(make_batch_dev(nb, sigma_dev, iters_dev, bound, hb, mesh_of,
+                           meshes < 1 ? 1 : meshes));
+    if (ws_bytes < csg_smooth_values_workspace_bytes(n, nnz, nb))
+        return fail(kInvalidArgument, "smooth_values workspace too small");
+    hipStream_t s = S(stream);
+    Carve c{(char *)ws, 0};
+    HeatWs h = carve_heat(c, n, nnz, nb);
+    CSG_HIP_CHECK(hipMemcpyAsync(h.buf[0], values, sizeof(double) * n * nb,
+                                 hipMemcpyDeviceToDevice, s));
+    double *res = nullptr;
+    CSG_TRY(run_heat(geom, nbr_off, nbr, n, nnz, hb, squared_norm, h, &res, s));
+    CSG_HIP_CHECK(hipMemcpyAsync(out, res, sizeof(double) * n * nb, hipMemcpyDeviceToDevice, s));
+    return kOk;
+}
+
+int csg_smooth_values_patched_dev(const double *values, int n, int nb, const float *geom,
+                                  int meshes, const int *mesh_of, const int *nbr_off,
+                                  const int *nbr, int nnz, const double *sigma_dev,
+                                  const int *iters_dev, int bound, int squared_norm, int npatches,
+                                  int rings, const int *perm, const int *vstart,
+                                  const int *ring_end, const int *lverts, const int *aoff,
+                                  const int *astart, const unsigned short *ladj,
+                                  size_t lds_bytes, double *out, void *ws, size_t ws_bytes,
+                                  void *stream)
+{
+    HeatBatch hb;
+    CSG_TRY(make_batch_dev(nb, sigma_dev, iters_dev, bound, hb, mesh_of,
+                           meshes < 1 ? 1 : meshes));
+    return smooth_values_patched_run(values, n, nb, geom, nbr_off, nbr, nnz, hb, squared_norm,
+                                     npatches, rings, perm, vstart, ring_end, lverts, aoff,
+                                     astart, ladj, lds_bytes, out, ws, ws_bytes, S(stream));
+}
+
 size_t csg_type5_sheets_batch_workspace_bytes(int n, int nnz, int sides, int nfw, int m)
 {
     if (n < 1 || sides < 1 || nfw < 1 || m < 1)
@@ -478,40 +706,33 @@ int csg_type5_sheets_batch(const float *geom, const float *smoothed, int n, int 
         mesh_of[b] = b / nfw;
     HeatBatch hb;
     CSG_TRY(make_batch(nb, sigma, iterations, hb, mesh_of, sides));
-    if (ws_bytes < csg_type5_sheets_batch_workspace_bytes(n, nnz, sides, nfw, m))
-        return fail(kInvalidArgument, "batched type-5 workspace too small");
-    for (int k = 0; k < sides; ++k)
-        if (!sheet_idx[k] || !sheet_w[k])
-            return fail(kInvalidArgument, "batched type-5: side %d has no sheet stencil", k);
-    hipStream_t s = S(stream);
-    Carve c{(char *)ws, 0};
-    float *nr = c.take<float>((size_t)3 * n * sides);
-    double *base = c.take<double>((size_t)n * sides);
-    HeatWs h = carve_heat(c, n, nnz, nb);
-    double *normed = c.take<double>((size_t)nb * n);
-    double *sheets = c.take<double>((size_t)nb * m);
-    double *mm = c.take<double>((size_t)nb * 64 * 2 + 64);
-    for (int k = 0; k < sides; ++k) {
-        const size_t g = (size_t)3 * n * k;
-        CSG_TRY(launch_vertex_normals(smoothed + g, faces, inc_off, inc, n, nr + g, s));
-        CSG_TRY(launch_type5_base(geom + g, smoothed + g, nr + g, n, base + (size_t)n * k, s));
-        for (int f = 0; f < nfw; ++f)
-            CSG_HIP_CHECK(hipMemcpyAsync(h.buf[0] + (size_t)(k * nfw + f) * n,
-                                         base + (size_t)n * k, sizeof(double) * n,
-                                         hipMemcpyDeviceToDevice, s));
-    }
-    double *res = nullptr;
-    CSG_TRY(run_heat(geom, nbr_off, nbr, n, nnz, hb, 1, h, &res, s));
-    CSG_TRY(launch_minmax_normalise<double>(res, (size_t)n, nb, (size_t)n, 0.0, mm, normed,
-                                            (size_t)n, s));
-    for (int k = 0; k < sides; ++k)
-        CSG_TRY(launch_sheet_gather(normed + (size_t)k * nfw * n, (size_t)n, sheet_idx[k],
-                                    sheet_w[k], m, nfw, sheets + (size_t)k * nfw * m, s));
-    if (out_dtype == 0)
-        return launch_minmax_normalise<float>(sheets, (size_t)m, nb, (size_t)m, 0.0, mm,
-                                              (float *)out, (size_t)m, s);
-    return launch_minmax_normalise<double>(sheets, (size_t)m, nb, (size_t)m, 0.0, mm,
-                                           (double *)out, (size_t)m, s);
+    return type5_batch_run(geom, smoothed, n, sides, faces, nbr_off, nbr, nnz, inc_off, inc, nfw,
+                           hb, sheet_idx, sheet_w, m, out_dtype, out, ws, ws_bytes, S(stream));
+}
+
+int csg_type5_sheets_batch_dev(const float *geom, const float *smoothed, int n, int sides,
+                               const int *faces, const int *nbr_off, const int *nbr, int nnz,
+                               const int *inc_off, const int *inc, int nfw,
+                               const double *sigma_dev, const int *iters_dev, int bound,
+                               const int *const *sheet_idx, const double *const *sheet_w, int m,
+                               int out_dtype, void *out, void *ws, size_t ws_bytes, void *stream)
+{
+    if (n < 1 || sides < 1 || nfw < 1 || m < 1 || !geom || !smoothed || !sigma_dev ||
+        !iters_dev || !sheet_idx || !sheet_w || !out)
+        return fail(kInvalidArgument, "invalid batched type-5 arguments");
+    if (out_dtype != 0 && out_dtype != 1)
+        return fail(kUnsupported, "type-5 sheets: dtype code %d", out_dtype);
+    if ((long long)sides * nfw > kMaxHeatBatch)
+        return fail(kInvalidArgument, "batched type-5: sides x fwhms must be <= %d, got %lld",
+                    kMaxHeatBatch, (long long)sides * nfw);
+    const int nb = sides * nfw;
+    int mesh_of[kMaxHeatBatch];
+    for (int b = 0; b < nb; ++b)
+        mesh_of[b] = b / nfw;
+    HeatBatch hb;
+    CSG_TRY(make_batch_dev(nb, sigma_dev, iters_dev, bound, hb, mesh_of, sides));
+    return type5_batch_run(geom, smoothed, n, sides, faces, nbr_off, nbr, nnz, inc_off, inc, nfw,
+                           hb, sheet_idx, sheet_w, m, out_dtype, out, ws, ws_bytes, S(stream));
 }
 
 namespace {

@@ -8,20 +8,53 @@ namespace csg {
 // Up to kMaxHeatBatch value-smoothing problems on meshes of one topology,
 // each with its own iteration count, Gaussian width and geometry.
 constexpr int kMaxHeatBatch = 32;
+// With dcounts / dsigma set, the counts and widths are read from device
+// memory instead (computed there by csg_heat_parameters); the host then
+// launches up to a bound and rows past their count are copied through.
 struct HeatBatch {
     int nb;
     int counts[kMaxHeatBatch];
     double sigma[kMaxHeatBatch];
     int mesh[kMaxHeatBatch];  // geometry of each problem: mesh[b] * 3 n floats in
+    const int *dcounts = nullptr;
+    const double *dsigma = nullptr;
 };
 
 // Up to kMaxSmoothBatch meshes of one topology smoothed together, each with
-// its own number of steps.
+// its own number of steps (from device memory when diters is set).
 constexpr int kMaxSmoothBatch = 32;
 struct SmoothBatch {
     int count;
     int iters[kMaxSmoothBatch];
+    const int *diters = nullptr;
 };
+
+__device__ __forceinline__ int heat_count(const HeatBatch &hb, int b)
+{
+    return hb.dcounts ? hb.dcounts[b] : hb.counts[b];
+}
+
+__device__ __forceinline__ double heat_sigma(const HeatBatch &hb, int b)
+{
+    return hb.dsigma ? hb.dsigma[b] : hb.sigma[b];
+}
+
+__device__ __forceinline__ int smooth_iters(const SmoothBatch &sb, int b)
+{
+    return sb.diters ? sb.diters[b] : sb.iters[b];
+}
+
+// Edge mean and heat-kernel parameters of one mesh on the device: em = (the
+// nblocks edge-sum partials added in order) / (3 nf); for each fwhm f,
+// iterations = max(1, ceil(f^2 0.541011 / em^2)), sigma = f 0.7355345 /
+// iterations^0.5 (surface_stencil.py:349-370).  The host's iterations^0.5 is
+// its libm pow, which differs from the correctly rounded square root by one
+// ulp for a few integers: those (root_n ascending, root_v their pow values)
+// are passed in, and every other count uses sqrt.
+constexpr int kMaxHeatFwhm = 8;
+int launch_heat_parameters(const double *partial, int nblocks, int nf, const double *fwhm,
+                           int nfw, const int *root_n, const double *root_v, int nroots,
+                           int *iters, double *sigma, double *em, hipStream_t s);
 
 // Patches for k_smooth_geometry_patch (device arrays).  The working buffers
 // hold the vertices in the order perm (working index i = vertex perm[i]), the

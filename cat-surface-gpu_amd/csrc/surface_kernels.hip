@@ -109,9 +109,9 @@ k_smooth_geometry_batch(float *__restrict__ buf0, float *__restrict__ buf1,
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
-    if (i >= n || it >= sb.iters[b])
+    if (i >= n || it >= smooth_iters(sb, b))
         return;
-    const bool from1 = ((sb.iters[b] + it) & 1) != 0;
+    const bool from1 = ((smooth_iters(sb, b) + it) & 1) != 0;
     const float *cur = (from1 ? buf1 : buf0) + (size_t)b * 3 * n;
     float *nxt = (from1 ? buf0 : buf1) + (size_t)b * 3 * n;
     int bo = off[i], e = off[i + 1];
@@ -139,7 +139,7 @@ k_smooth_geometry4(float4 *__restrict__ buf0, float4 *__restrict__ buf1,
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
-    if (i >= n || it >= sb.iters[b])
+    if (i >= n || it >= smooth_iters(sb, b))
         return;
     const float4 *cur = ((it & 1) ? buf1 : buf0) + (size_t)b * n;
     float4 *nxt = ((it & 1) ? buf0 : buf1) + (size_t)b * n;
@@ -182,7 +182,7 @@ k_smooth_geometry_patch(const float4 *__restrict__ in, float4 *__restrict__ out,
     const int *ring_end = pp.ring_end + p * (pp.rings + 1);
     const float4 *src = in + (size_t)b * n;
     float4 *dst = out + (size_t)b * n;
-    int m = sb.iters[b] - it0;
+    int m = smooth_iters(sb, b) - it0;
     m = m < 0 ? 0 : (m > pp.rings ? pp.rings : m);
     const int core = ring_end[0];
     if (m == 0) {
@@ -270,7 +270,7 @@ k_smooth_values_patch(const double *__restrict__ in, double *__restrict__ out, i
     const int *ring_end = pp.ring_end + p * (pp.rings + 1);
     const double *src = in + (size_t)b * n;
     double *dst = out + (size_t)b * n;
-    int m = hb.counts[b] - it0;
+    int m = heat_count(hb, b) - it0;
     m = m < 0 ? 0 : (m > pp.rings ? pp.rings : m);
     const int core = ring_end[0];
     if (m == 0) {
@@ -373,7 +373,7 @@ k_unpack4(const float4 *__restrict__ buf0, const float4 *__restrict__ buf1, int 
     const size_t b = blockIdx.y;
     if (i >= n)
         return;
-    const float4 q = ((sb.iters[b] & 1) ? buf1 : buf0)[b * n + i];
+    const float4 q = ((smooth_iters(sb, (int)b) & 1) ? buf1 : buf0)[b * n + i];
     float *o = out + (b * n + i) * 3;
     o[0] = q.x;
     o[1] = q.y;
@@ -557,7 +557,7 @@ k_heat_weights(const float *__restrict__ geom, const int *__restrict__ off,
     if (i >= n)
         return;
     int b = blockIdx.y;
-    double sg = hb.sigma[b];
+    double sg = heat_sigma(hb, b);
     geom += (size_t)hb.mesh[b] * 3 * n;
     double gx = geom[3 * i], gy = geom[3 * i + 1], gz = geom[3 * i + 2];
     double ws = 0.0;
@@ -590,7 +590,7 @@ k_smooth_values(const double *__restrict__ cur, const int *__restrict__ off,
     int b = blockIdx.y;
     const double *cv = cur + (size_t)b * n;
     double *nv = nxt + (size_t)b * n;
-    if (iteration >= hb.counts[b]) {
+    if (iteration >= heat_count(hb, b)) {
         nv[i] = cv[i];
         return;
     }
@@ -999,7 +999,7 @@ int launch_smooth_geometry_patched(const float *v, float4 *buf0, float4 *buf1, i
         // neighbour list (4 bytes each, mean degree 6 on a closed triangle mesh)
         double iters = 0.0;
         for (int m = 0; m < sb.count; ++m)
-            iters += std::max(0, std::min(pp.rings, sb.iters[m] - it0));
+            iters += std::max(0, std::min(pp.rings, (sb.diters ? most : sb.iters[m]) - it0));
         TimedLaunch timer(kTimedSmoothGeometry, s, iters * n * (32.0 + 4.0 * 6.0));
         hipExtLaunchKernelGGL(k_smooth_geometry_patch, dim3(pp.count, sb.count),
                            dim3(kPatchThreads), lds_bytes, s, timer.start(), timer.stop(), 0, in, o, n, pp, sb, it0);
@@ -1035,6 +1035,67 @@ int launch_curvature_type2(const float *geom, const float *nr, int n, const int 
 {
     hipLaunchKernelGGL(k_curvature_type2, dim3(grid_for(n)), dim3(kBlock), 0, s, geom, nr, n, off,
                        nbr, out);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+struct FwhmList {
+    double f[kMaxHeatFwhm];
+};
+
+// One thread: the products and quotients are rounded one by one (no
+// contraction), as the host's double arithmetic rounds them.
+__global__ void k_heat_parameters(const double *__restrict__ partial, int nblocks, int nf,
+                                  FwhmList fw, int nfw, const int *__restrict__ root_n,
+                                  const double *__restrict__ root_v, int nroots,
+                                  int *__restrict__ iters, double *__restrict__ sigma,
+                                  double *__restrict__ em_out)
+{
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    double em;
+    if (nblocks < 0) {  // partial holds the edge mean itself
+        em = partial[0];
+    } else {
+        double total = 0.0;
+        for (int k = 0; k < nblocks; ++k)
+            total += partial[k];
+        em = total / (3.0 * (double)nf);
+    }
+    if (em_out)
+        em_out[0] = em;
+    for (int j = 0; j < nfw; ++j) {
+        const double f = fw.f[j];
+        const double raw = f * f * 0.541011 / (em * em);
+        const double c = ceil(raw);
+        int it = c >= 2147483647.0 ? 2147483647 : (int)c;
+        it = it < 1 ? 1 : it;
+        iters[j] = it;
+        double root = sqrt((double)it);
+        int lo = 0, hi = nroots;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (root_n[mid] < it)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        if (lo < nroots && root_n[lo] == it)
+            root = root_v[lo];
+        sigma[j] = f * 0.7355345 / root;
+    }
+}
+
+int launch_heat_parameters(const double *partial, int nblocks, int nf, const double *fwhm,
+                           int nfw, const int *root_n, const double *root_v, int nroots,
+                           int *iters, double *sigma, double *em, hipStream_t s)
+{
+    FwhmList fw{};
+    for (int j = 0; j < nfw; ++j)
+        fw.f[j] = fwhm[j];
+    hipLaunchKernelGGL(k_heat_parameters, dim3(1), dim3(64), 0, s, partial, nblocks, nf, fw, nfw,
+                       root_n, root_v, nroots, iters, sigma, em);
     CSG_LAUNCH_CHECK();
     return kOk;
 }

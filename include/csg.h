@@ -317,6 +317,61 @@ int csg_type5_sheets_batch(const float *geom, const float *smoothed, int n, int 
                            const double *const *sheet_w, int m, int out_dtype, void *out,
                            void *ws, size_t ws_bytes, void *stream);
 
+/* ---- Device-resident heat parameters.  csg_heat_parameters computes a mesh's mean face-edge
+ * length (the csg_edge_sum partials added in order, over 3 nf) and, for each of nfw <= 8
+ * fwhms, iterations = max(1, ceil(fwhm^2 0.541011 / em^2)) and sigma = fwhm 0.7355345 /
+ * iterations^0.5 into device arrays (em_out optional), without a host round trip.  The
+ * host's iterations^0.5 is its libm pow: the integers where that differs from the correctly
+ * rounded square root are passed as root_n (ascending) with their pow values root_v (the
+ * Python wrapper lists them once).  ws: csg_edge_mean_workspace_bytes(nf).  Replaces
+ * _edge_mean_distance + _heat_parameters (surface_stencil.py:349-370) where a schedule
+ * must not wait for the host.
+ *
+ * The _dev smoothing entries equal csg_smooth_geometry_batch / _patched,
+ * csg_smooth_values / _patched and csg_type5_sheets_batch with the per-row iteration counts
+ * (and widths) read from device arrays: the host launches up to `bound` iterations and rows
+ * past their count are copied through, so the results equal the host-count entries' as long
+ * as no count exceeds `bound` (callers verify that on the device). */
+/* flag[0] = 1 when some of the n device ints exceeds bound (left as is otherwise): the device
+ * check that a bounded loop covered every count.  No counterpart in the reference. */
+int csg_flag_above(const int *values, int n, int bound, int *flag, void *stream);
+int csg_heat_parameters(const float *v, const int *faces, int nf, const double *fwhm, int nfw,
+                        const int *root_n, const double *root_v, int nroots, int *iters_out,
+                        double *sigma_out, double *em_out, void *ws, size_t ws_bytes,
+                        void *stream);
+/* The same parameters from an edge mean already on the device (em: one double). */
+int csg_heat_parameters_from_edge_mean(const double *em, const double *fwhm, int nfw,
+                                       const int *root_n, const double *root_v, int nroots,
+                                       int *iters_out, double *sigma_out, void *stream);
+int csg_smooth_geometry_batch_dev(const float *v, int batch, const int *nbr_off, const int *nbr,
+                                  int n, const int *iters_dev, int bound, float *out, void *ws,
+                                  size_t ws_bytes, void *stream);
+int csg_smooth_geometry_patched_dev(const float *v, int batch, int n, const int *iters_dev,
+                                    int bound, int npatches, int rings, const int *perm,
+                                    const int *vstart, const int *ring_end, const int *lverts,
+                                    const int *aoff, const int *astart,
+                                    const unsigned short *ladj, size_t lds_bytes, float *out,
+                                    void *ws, size_t ws_bytes, void *stream);
+int csg_smooth_values_dev(const double *values, int n, int nb, const float *geom, int meshes,
+                          const int *mesh_of, const int *nbr_off, const int *nbr, int nnz,
+                          const double *sigma_dev, const int *iters_dev, int bound,
+                          int squared_norm, double *out, void *ws, size_t ws_bytes, void *stream);
+int csg_smooth_values_patched_dev(const double *values, int n, int nb, const float *geom,
+                                  int meshes, const int *mesh_of, const int *nbr_off,
+                                  const int *nbr, int nnz, const double *sigma_dev,
+                                  const int *iters_dev, int bound, int squared_norm, int npatches,
+                                  int rings, const int *perm, const int *vstart,
+                                  const int *ring_end, const int *lverts, const int *aoff,
+                                  const int *astart, const unsigned short *ladj,
+                                  size_t lds_bytes, double *out, void *ws, size_t ws_bytes,
+                                  void *stream);
+int csg_type5_sheets_batch_dev(const float *geom, const float *smoothed, int n, int sides,
+                               const int *faces, const int *nbr_off, const int *nbr, int nnz,
+                               const int *inc_off, const int *inc, int nfw,
+                               const double *sigma_dev, const int *iters_dev, int bound,
+                               const int *const *sheet_idx, const double *const *sheet_w, int m,
+                               int out_dtype, void *out, void *ws, size_t ws_bytes, void *stream);
+
 /* Depth potential (L + alpha diag(area)) x = rhs by colour-ordered SOR for `batch`
  * meshes of one topology (pts [batch][n][3], x and relres [batch]).  The system is a
  * CSR with duplicate (row, col) entries merged (contrib_off / contrib list each merged

@@ -126,6 +126,38 @@ def test_type2_curvature_and_single_fwhm_sheets(small, cuda):
         dev.curvature_to_sheet(surf, 1.0, curvtype=3)
 
 
+def test_device_heat_parameters_equal_the_host_formula(small, cuda):
+    """csg_heat_parameters (edge mean, iterations and sigma on the device) equals
+    heat_parameters_from_edge_mean on the host bit for bit, the iteration counts
+    whose host square root (libm pow) is not the correctly rounded one included."""
+
+    import math
+
+    from cat_surface_gpu.surface_stencil import (EdgeHeat, edge_heat, edge_means,
+                                                 heat_from_edge_mean,
+                                                 heat_parameters_from_edge_mean)
+
+    g, dev = small
+    surf = torch.as_tensor(g["surface"], device=cuda)
+    fws = (50.0, 15.0, 5.0 / 9.0)
+    h = edge_heat([dev], [surf], [fws])[0]
+    em = edge_means([dev], [surf])[0]
+    assert abs(em - float(g["edge_mean"])) <= 1e-12 * em
+    assert float(h.edge_mean.item()) == em
+    assert h.host() == [heat_parameters_from_edge_mean(em, f) for f in fws]
+    # counts where k ** 0.5 != sqrt(k) (2921, 3541, 5579) and ordinary ones
+    counts = [1, 2, 44, 501, 2921, 3541, 5579, 7827, 2 ** 20 - 1]
+    assert any(k ** 0.5 != math.sqrt(k) for k in counts)
+    e = 1.7
+    fw = [math.sqrt((k - 0.5) * e * e / 0.541011) for k in counts]
+    base = EdgeHeat((), h.iters[:0], h.sigma[:0],
+                    torch.tensor([e], dtype=torch.float64, device=cuda))
+    for chunk in (fw[:5], fw[5:]):
+        got = heat_from_edge_mean(base, chunk).host()
+        want = [heat_parameters_from_edge_mean(e, f) for f in chunk]
+        assert got == want
+
+
 def test_type5_sheets_batch_equals_per_side(small, cuda):
     """Sides of one topology with their own geometry, smoothing counts and sheet
     stencils in one csg_type5_sheets_batch call equal per-side csg_type5_sheets."""
