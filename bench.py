@@ -86,16 +86,22 @@ def run_step(pairs, dtype: str, mode: str, device, cuda_graph: bool):
     shared feature solve and batched DARTEL rounds on the fused schedule, graph
     replay, identical results); "batched+adjoint" also tunes the relaxation
     tiles and replaces Nelder-Mead by gradient (Levenberg-Marquardt) steps on
-    the 8 best seeds."""
+    the 8 best seeds; "graph" runs the batched+adjoint schedule as one captured
+    graph (run_graph_schedule: every stage, device-side loop counts and checks,
+    replayed each step)."""
 
     from cat_surface_gpu import (PipelineJob, run_cat_surface_gpu_pipeline,
                                  run_cat_surface_gpu_pipelines, set_fmg_schedule)
 
     kw = dict(device=device, dartel_dtype=dtype, steps=2, runs=2, avg=True, write_output=False,
               cuda_graph=cuda_graph)
-    if mode == "batched+adjoint":
+    if mode in ("batched+adjoint", "graph"):
         kw.update(rotation_method="adjoint", autotune=True)
-    if mode in ("batched", "batched+adjoint"):
+    if mode == "graph":
+        # the whole batched+adjoint schedule as one graph: the first call plans
+        # and captures it, later calls replay it
+        kw.update(graph_schedule=cuda_graph)
+    if mode in ("batched", "batched+adjoint", "graph"):
         return run_cat_surface_gpu_pipelines([PipelineJob(meshes=m) for m in pairs], **kw)
     if mode == "plain":
         kw.update(cuda_graph=False)
@@ -219,7 +225,17 @@ def uneven_report(seed: int, device, jobs_count: int = 32) -> dict:
     torch.cuda.synchronize()
     makespan = time.perf_counter() - t0
     log("uneven batched run done")
-    alone, errors = [], []
+    # the same schedule as one captured graph: plan + capture, then a replay
+    run_cat_surface_gpu_pipelines(jobs, autotune=True, graph_schedule=True, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    graphed = run_cat_surface_gpu_pipelines(jobs, autotune=True, graph_schedule=True, **kw)
+    torch.cuda.synchronize()
+    makespan_graph = time.perf_counter() - t0
+    replayed = all(r.timings.get("graph_replay") == 1.0 for r in graphed)
+    same = all(np.array_equal(a.vertices, b.vertices) for a, b in zip(graphed, batched))
+    log(f"uneven graph schedule: {makespan_graph:.3f} s (replayed {replayed}, equal {same})")
+    alone, alone_graph, errors = [], [], []
     for job, res in zip(jobs, batched):
         run_cat_surface_gpu_pipelines([job], **kw)  # warm this job's graph shapes
         torch.cuda.synchronize()
@@ -227,11 +243,18 @@ def uneven_report(seed: int, device, jobs_count: int = 32) -> dict:
         run_cat_surface_gpu_pipelines([job], **kw)
         torch.cuda.synchronize()
         alone.append(time.perf_counter() - t1)
+        run_cat_surface_gpu_pipelines([job], graph_schedule=True, **kw)  # plan + capture
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        run_cat_surface_gpu_pipelines([job], graph_schedule=True, **kw)
+        torch.cuda.synchronize()
+        alone_graph.append(time.perf_counter() - t1)
         ref = run_cat_surface_gpu_pipelines([job], **dict(kw, dartel_dtype="float64"))[0]
         errors.append(float(np.abs(res.vertices.astype(np.float64) - ref.vertices).max()))
-        log(f"uneven job {len(alone)}/{len(jobs)}: alone {alone[-1]:.3f} s, "
-            f"max vertex delta vs float64 {errors[-1]:.2e}")
+        log(f"uneven job {len(alone)}/{len(jobs)}: alone {alone[-1]:.3f} s, graph "
+            f"{alone_graph[-1]:.3f} s, max vertex delta vs float64 {errors[-1]:.2e}")
     lat = np.asarray(alone)
+    lat_graph = np.asarray(alone_graph)
     return {
         "metric": "uneven multi-job CAT_SurfWarp throughput",
         "value": round(jobs_count / makespan, 3), "unit": "jobs/s", "n_gpus": 1,
@@ -244,6 +267,13 @@ def uneven_report(seed: int, device, jobs_count: int = 32) -> dict:
         "alone_latency_s": {"p50": round(float(np.percentile(lat, 50)), 4),
                             "p99": round(float(np.percentile(lat, 99)), 4),
                             "sum": round(float(lat.sum()), 4)},
+        "graph_schedule": {
+            "value": round(jobs_count / makespan_graph, 3), "unit": "jobs/s",
+            "makespan_s": round(makespan_graph, 4), "replayed": replayed,
+            "equal_to_multi_graph_schedule": same,
+            "alone_latency_s": {"p50": round(float(np.percentile(lat_graph, 50)), 4),
+                                "p99": round(float(np.percentile(lat_graph, 99)), 4),
+                                "sum": round(float(lat_graph.sum()), 4)}},
         "batched_speedup_vs_alone_sum": round(float(lat.sum()) / makespan, 3),
         "max_vertex_delta_vs_float64_plain": {"max": max(errors),
                                               "p50": float(np.median(errors))},
@@ -404,7 +434,9 @@ def instrumented_rooflines(pairs, args, device) -> list[dict]:
     kernel_rooflines()  # clear
     torch.cuda.synchronize()
     _lib.lib().csg_kernel_timing(1)
-    run_step(pairs, args.dtype, args.mode, device, False)
+    # the graph mode's kernels are the batched+adjoint schedule's
+    run_step(pairs, args.dtype, "batched+adjoint" if args.mode == "graph" else args.mode,
+             device, False)
     torch.cuda.synchronize()
     _lib.lib().csg_kernel_timing(0)
     return kernel_rooflines()
@@ -416,8 +448,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--dtype", choices=("float32", "float64"), default="float32")
-    ap.add_argument("--mode", choices=("plain", "plain-fused", "batched", "batched+adjoint"),
-                    default="batched+adjoint")
+    ap.add_argument("--mode", choices=("plain", "plain-fused", "batched", "batched+adjoint",
+                                       "graph"), default="batched+adjoint")
     ap.add_argument("--seed", type=int, default=20261205)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gifti-io", action="store_true",

@@ -106,6 +106,12 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_depth_potential_patched": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                                          _i, ctypes.POINTER(_i), _vp, _i, _vp, _vp, _i, _d, _i,
                                          _d, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "csg_depth_potential_patched_deferred": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp,
+                                                  _vp, _i, ctypes.POINTER(_i), _vp, _i, _vp, _vp,
+                                                  _i, _d, _i, _d, _vp, _vp, _vp, _vp, _vp, _sz,
+                                                  _vp]),
+    "csg_prepare_sphere_dev": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "csg_rotation_matrices": (_i, [_vp, _i, _vp, _vp]),
     "csg_apply_flow": (_i, [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp]),
     "csg_average_xz": (_i, [_vp, _vp, _i, _i, _dp, _vp, _vp]),
     "csg_normalise_points": (_i, [_vp, _i, _vp, _vp]),
@@ -113,7 +119,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_prepare_sphere": (_i, [_vp, _i, _vp, _dp, _vp, _vp, _vp]),
     # point location
     "csg_grid_index_plan_workspace_bytes": (_sz, [_i]),
-    "csg_grid_index_build_workspace_bytes": (_sz, [_i, _i]),
+    "csg_top_k": (_i, [_vp, _i, _i, _vp, _vp]),
+    "csg_grid_index_build_workspace_bytes": (_sz, [_i, _i, _i]),
     "csg_grid_index_build": (_i, [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
     "csg_grid_index_plan": (_i, [_vp, _i, _vp, _i, _i, _i, _vp, _sz, ctypes.POINTER(_i), _vp]),
     "csg_grid_index_fill_workspace_bytes": (_sz, [_i]),

@@ -24,7 +24,7 @@ from typing import Sequence
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, schedule
 from .dartel_grid import resolve_device
 from .gifti import (INTENT_POINTSET, INTENT_TRIANGLE, read_gifti, read_gifti_intents,
                     read_gifti_many, write_gifti_surface)
@@ -109,8 +109,9 @@ def _prepare_sphere(vertices: torch.Tensor, *, centroid: torch.Tensor | None = N
     """The pipeline's sphere preparation (gpu_pipeline.py:274-305, :373-378) on the
     device (csg_prepare_sphere): with ``centroid`` (from _sphere_centroids), centre
     the vertices and scale each to unit length in float32 (a vertex that cannot be
-    normalised sets ``bad``); with ``angles``, then rotate by p · Mᵀ in float64,
-    stored float32.  Asynchronous."""
+    normalised sets ``bad``); with ``angles`` (host, or a device tensor whose
+    matrix is then built on the device), rotate by p · Mᵀ in float64, stored
+    float32.  Asynchronous."""
 
     if vertices.ndim != 2 or vertices.shape[1] != 3:
         raise ValueError(f"Sphere vertices must have shape [n, 3], got {tuple(vertices.shape)}")
@@ -118,12 +119,53 @@ def _prepare_sphere(vertices: torch.Tensor, *, centroid: torch.Tensor | None = N
         raise ValueError("normalisation needs a device flag for bad vertices")
     v = vertices.to(torch.float32).contiguous()
     out = torch.empty_like(v)
+    if isinstance(angles, torch.Tensor) and angles.device.type != "cpu":
+        # the angles stay on the device (a rotation search's result): the
+        # matrix is built there too
+        ang = angles.to(torch.float64).reshape(1, 3).contiguous()
+        mat = torch.empty(9, dtype=torch.float64, device=v.device)
+        _lib.check(_lib.lib().csg_rotation_matrices(_lib.ptr(ang), 1, _lib.ptr(mat),
+                                                    _lib.stream_handle(v.device)),
+                   "csg_rotation_matrices")
+        _lib.check(_lib.lib().csg_prepare_sphere_dev(_lib.ptr(v), int(v.shape[0]),
+                                                     _lib.ptr(centroid), _lib.ptr(mat),
+                                                     _lib.ptr(out), _lib.ptr(bad),
+                                                     _lib.stream_handle(v.device)),
+                   "csg_prepare_sphere_dev")
+        return out
     rot = None if angles is None else _lib.doubles(rotation_matrix_np(angles).reshape(-1))
     _lib.check(_lib.lib().csg_prepare_sphere(_lib.ptr(v), int(v.shape[0]), _lib.ptr(centroid),
                                              rot, _lib.ptr(out), _lib.ptr(bad),
                                              _lib.stream_handle(v.device)),
                "csg_prepare_sphere")
     return out
+
+
+def _sync(dev) -> None:
+    """Wait for the device (stage timing), except while a schedule is captured."""
+
+    if not schedule.capturing():
+        torch.cuda.synchronize(dev)
+
+
+_FACES_DEVICE: dict = {}
+
+
+def _faces_on_device(faces, dev) -> torch.Tensor:
+    """A face array on the device, cached per array (a captured schedule copies
+    nothing from the host: the array must have been seen before)."""
+
+    if isinstance(faces, torch.Tensor):
+        return faces.to(device=dev)
+    arr = np.ascontiguousarray(faces)
+    key = (str(dev), arr.shape, arr.dtype.str, hash(arr.tobytes()))
+    hit = _FACES_DEVICE.get(key)
+    if hit is None:
+        if schedule.capturing():
+            raise RuntimeError("face array not seen by an eager run of this schedule")
+        hit = torch.as_tensor(arr, device=dev)
+        _FACES_DEVICE[key] = hit
+    return hit
 
 
 def _check_normalised(bad: torch.Tensor) -> None:
@@ -293,6 +335,7 @@ def run_cat_surface_gpu_pipelines(
     rotation_search=None, write_output: bool = True,
     rotation_method: str = "nelder-mead", rotation_candidates: str = "seeds",
     rotation_top: int = 8, rotation_steps: int = 20, autotune: bool = False,
+    graph_schedule: bool = False,
 ) -> list[CatSurfaceGpuResult]:
     """Several independent CAT_SurfWarp registrations, stage by stage.
 
@@ -302,8 +345,17 @@ def run_cat_surface_gpu_pipelines(
     (and, in the last run, each job's main and average solves, which are
     independent: gpu_pipeline.py:1133-1224) in one batched solve.  Stage
     timings are those of the shared stages (the same for every job).
+
+    ``graph_schedule=True`` runs the whole schedule (stencils, features,
+    rotation search, every run's sheets and solves, warps, averaging) as one
+    HIP graph: see run_graph_schedule.
     """
 
+    if graph_schedule:
+        kw = dict(locals())
+        for name in ("jobs", "graph_schedule"):
+            kw.pop(name)
+        return run_graph_schedule(jobs, **kw)
     if not jobs:
         return []
     if steps < 1 or steps > 3:
@@ -411,6 +463,17 @@ def run_cat_surface_gpu_pipelines(
         def settle() -> None:
             if not pending:
                 return
+            if schedule.capturing():
+                # the checks are read after the graph's replay
+                for checks, *_ in pending:
+                    for c in checks:
+                        if c is not None:
+                            schedule.defer(schedule._exceeds([c.status[0:1]], c.capacity),
+                                           "stencil index capacity")
+                            schedule.defer(c.status[1:2], "stencil query beyond the searched "
+                                                          "reach")
+                pending.clear()
+                return
             verdict = check_all([c for entry in pending for c in entry[0]])
             k = 0
             for checks, st, attr, sphere_vertices, label, normalise in pending:
@@ -422,8 +485,8 @@ def run_cat_surface_gpu_pipelines(
 
         t = time.perf_counter()
         for i, st in enumerate(states):
-            st.src_faces_dev = torch.as_tensor(st.src_sphere.faces, device=dev)
-            st.tgt_faces_dev = torch.as_tensor(st.tgt_sphere.faces, device=dev)
+            st.src_faces_dev = _faces_on_device(st.src_sphere.faces, dev)
+            st.tgt_faces_dev = _faces_on_device(st.tgt_sphere.faces, dev)
             build_into(st, "src_stencil", i, st.src_sphere.vertices, "source", True,
                        stencil_builder, st.job.source_sphere, st.job.source_stencil)
             build_into(st, "tgt_stencil", i, st.tgt_sphere.vertices, "target", True,
@@ -439,7 +502,7 @@ def run_cat_surface_gpu_pipelines(
                                        for v in (st.src_sphere_dev, st.tgt_sphere_dev)])
         for i, st in enumerate(states):
             st.src_centroid, st.tgt_centroid = centroids[2 * i], centroids[2 * i + 1]
-        torch.cuda.synchronize(dev)
+        _sync(dev)
         settle()
         shared["initial_stencil_seconds"] = time.perf_counter() - t
 
@@ -489,7 +552,7 @@ def run_cat_surface_gpu_pipelines(
                 parallel_sides=parallel_sides)
             for k, st in enumerate(gpu_jobs):
                 st.src_vals, st.tgt_vals = feats[2 * k].values, feats[2 * k + 1].values
-        torch.cuda.synchronize(dev)
+        _sync(dev)
         shared["rotation_feature_seconds"] = time.perf_counter() - t
 
         t = time.perf_counter()
@@ -502,7 +565,7 @@ def run_cat_surface_gpu_pipelines(
                           if st.src_stencil.sphere_points64 is not None
                           else st.src_stencil.sphere_points)
             rots.append((rot, src_points))
-        torch.cuda.synchronize(dev)
+        _sync(dev)
         shared["rotation_index_upload_seconds"] = time.perf_counter() - t
         t = time.perf_counter()
         if rotation_search is not None:
@@ -524,8 +587,9 @@ def run_cat_surface_gpu_pipelines(
                                        candidates=rotation_candidates, top=rotation_top,
                                        steps=rotation_steps)
         for st, result in zip(states, results):
-            st.angle = np.asarray(result.angle.detach().cpu(), dtype=np.float64).reshape(3)
-            st.rotation_cost = float(result.cost.detach().cpu())
+            # kept on the device: the sphere's rotation matrix is built there
+            st.angle = result.angle.detach().to(torch.float64).reshape(3)
+            st.rotation_cost = result.cost.detach()
         shared["rotation_search_seconds"] = time.perf_counter() - t
 
         t = time.perf_counter()
@@ -543,15 +607,21 @@ def run_cat_surface_gpu_pipelines(
             if avg:
                 build_into(st, "avg_tgt_stencil", i, avg_tgt_sphere, "avg-target", False,
                            builder)
-        torch.cuda.synchronize(dev)
-        _check_normalised(bad)
+        _sync(dev)
+        schedule.defer(bad, "Sphere contains a vertex that cannot be normalized", ValueError)
         settle()
         shared["rotated_stencil_seconds"] = time.perf_counter() - t
 
+        # a recorded or captured schedule solves as the per-shape graphs do
+        # (initial velocities taken as nonzero, no host check), and a captured
+        # one launches the solves into the enclosing graph
+        in_schedule = schedule.current() is not None
         solve_kw = dict(n_steps=steps, loop=loop, cycles=cycles, nit=nit, its=its, code=code,
                         curvtypes=tuple(int(t) for t in curvtypes[:steps]), device=dev,
-                        dtype=solve_dtype, cuda_graph=graph_requested)
-        if autotune:
+                        dtype=solve_dtype,
+                        cuda_graph=graph_requested and not schedule.capturing(),
+                        assume_initial_nonzero=graph_requested or in_schedule)
+        if autotune and not schedule.capturing():
             # relaxation tiles per map size, tuned (or read from the cache)
             # before any solve is captured; the batch is the largest solve's
             from .autotune import ensure_tuned
@@ -610,7 +680,7 @@ def run_cat_surface_gpu_pipelines(
                               else st.current_dev)
                     build_into(st, "cur_stencil", i, sphere, f"rotated-{run + 1:03d}", False,
                                builder)
-            torch.cuda.synchronize(dev)
+            _sync(dev)
             settle()
             shared[f"dartel_run_{run}_seconds"] = time.perf_counter() - t
         shared["source_dartel_total_seconds"] = time.perf_counter() - t_dartel
@@ -635,13 +705,19 @@ def run_cat_surface_gpu_pipelines(
                 combined_all.append(combined)
             # one host copy after every job's launches (no synchronisation between jobs)
             for st, combined in zip(states, combined_all):
-                st.current_sphere = combined.cpu().numpy()
+                st.current_dev = combined
+                st.current_sphere = None if schedule.capturing() else combined.cpu().numpy()
             shared["avg_seconds"] = time.perf_counter() - t
         shared["dartel_total_seconds"] = time.perf_counter() - t_dartel
 
         t = time.perf_counter()
         results_out = []
         for st in states:
+            if schedule.capturing():
+                # the graph's outputs stay on the device; the caller reads them
+                # after each replay
+                results_out.append((st, GiftiMesh(st.current_dev, st.src_sphere.faces), Path("")))
+                continue
             final = GiftiMesh(np.ascontiguousarray(st.current_host(), dtype=np.float32),
                               st.src_sphere.faces)
             output = Path(st.job.output) if st.job.output is not None else Path("")
@@ -654,9 +730,168 @@ def run_cat_surface_gpu_pipelines(
         shared["output_write_seconds"] = time.perf_counter() - t
     shared["total_seconds"] = time.perf_counter() - t_total
     out = []
-    for st, final, output in results_out:
+    capturing = schedule.capturing()
+    angles = costs = None
+    if not capturing and states:
+        host = torch.cat([torch.cat([torch.as_tensor(st.angle, dtype=torch.float64).reshape(3),
+                                     torch.as_tensor(st.rotation_cost,
+                                                     dtype=torch.float64).reshape(1)])
+                          .to("cpu") for st in states]).numpy().reshape(-1, 4)
+        angles, costs = host[:, :3], host[:, 3]
+    for k, (st, final, output) in enumerate(results_out):
         timings = dict(shared)
+        if capturing:
+            angle, cost = st.angle, st.rotation_cost
+        else:
+            angle, cost = np.array(angles[k]), float(costs[k])
         out.append(CatSurfaceGpuResult(vertices=final.vertices, faces=final.faces,
-                                       rotation_angle=st.angle, rotation_cost=st.rotation_cost,
+                                       rotation_angle=angle, rotation_cost=cost,
                                        timings=timings, output_path=output))
     return out
+
+
+# ----------------------------------------------------------------- one graph
+
+@dataclass
+class _GraphSchedule:
+    graph: object                 # torch.cuda.CUDAGraph
+    inputs: list                  # per job: its four vertex arrays (device float32)
+    outputs: list                 # per job: CatSurfaceGpuResult with device tensors
+    capture: object               # schedule.Capture (bounds and device checks)
+    capture_seconds: float
+    replays: int = 0
+    fallbacks: int = 0
+
+
+_GRAPH_SCHEDULES: dict = {}
+
+
+def clear_graph_schedules() -> None:
+    _GRAPH_SCHEDULES.clear()
+
+
+def _graph_key(jobs, kw, dev) -> tuple:
+    parts = []
+    for job in jobs:
+        meshes = tuple((tuple(np.shape(m.vertices)),
+                        hash(np.ascontiguousarray(np.asarray(m.faces)).tobytes()))
+                       for m in job.meshes)
+        parts.append(meshes + (tuple(job.map_size or (512, 256)),))
+    opts = tuple(sorted((k, repr(v)) for k, v in kw.items() if k != "write_output"))
+    return (str(dev), tuple(parts), opts)
+
+
+def run_graph_schedule(jobs: Sequence[PipelineJob], **kw) -> list[CatSurfaceGpuResult]:
+    """run_cat_surface_gpu_pipelines as one HIP graph (north-star pattern (b)).
+
+    The first call for a given set of job shapes (vertex counts, face arrays and
+    map sizes) and options runs the schedule eagerly -- that is its result --
+    while recording how many launches each data-dependent loop needed, then
+    captures the same schedule into one graph on static input buffers: every
+    stage's kernels and the batched solves of every map size.  Later calls copy
+    their meshes into those buffers and replay the graph.  Loop counts that
+    depend on the data (heat-kernel iterations) are read by the kernels from
+    device memory, and the graph launches up to the recorded bound x 1.25 + 8;
+    each replay then checks on the device, and reads once, that every count
+    stayed within its bound and every stencil index, SOR barrier, sphere
+    normalisation and rotation cover is valid -- otherwise the call runs
+    eagerly instead (and counts a fallback).
+
+    Supported: meshes given in memory, in-process stencils and rotation
+    features (SOR depth), the adjoint rotation search, and steps >= 2 of type-5
+    sheets (the batched rule of surface_warp.py:427-448).
+    """
+
+    if not jobs:
+        return []
+    for job in jobs:
+        if (job.meshes is None or job.source_stencil is not None or
+                job.target_stencil is not None or job.rotation_values is not None):
+            raise ValueError("the graph schedule needs in-memory meshes and in-process stencils "
+                             "and rotation features")
+    unsupported = [name for name in ("rotation_values_probe", "rotation_geometry_probe",
+                                     "rotation_depth_probe", "stencil_builder",
+                                     "rotated_stencil_builder", "rotation_search")
+                   if kw.get(name) is not None]
+    if unsupported:
+        raise ValueError(f"the graph schedule does not run {', '.join(unsupported)}")
+    if kw.get("rotation_method", "nelder-mead") != "adjoint":
+        raise ValueError("the graph schedule needs rotation_method='adjoint' (Nelder-Mead "
+                         "decides each step on the host)")
+    if kw.get("rotation_feature_backend", "auto") not in ("auto", "cuda"):
+        raise ValueError("the graph schedule computes rotation features on the GPU ('cuda')")
+    if kw.get("rotation_feature_solver", "colored-sor") != "colored-sor":
+        raise ValueError("the graph schedule solves the depth potential by SOR")
+    steps = int(kw.get("steps", 2))
+    types = tuple(int(t) for t in kw.get("curvtypes", (5, 5, 2))[:steps])
+    if steps < 2 or any(t != 5 for t in types):
+        raise ValueError("the graph schedule needs steps >= 2 of type-5 curvature")
+    dev = resolve_device(kw.get("device", "cuda"))
+    write_output = bool(kw.get("write_output", True))
+    eager_kw = dict(kw, write_output=False)
+    key = _graph_key(jobs, kw, dev)
+    entry = _GRAPH_SCHEDULES.get(key)
+    if entry is None:
+        with schedule.recording() as rec:
+            results = run_cat_surface_gpu_pipelines(jobs, **eager_kw)
+        _GRAPH_SCHEDULES[key] = _capture_schedule(jobs, eager_kw, rec.bounds, dev)
+        return _finish_graph_results(jobs, results, write_output)
+    t0 = time.perf_counter()
+    for job, bufs in zip(jobs, entry.inputs):
+        for mesh, buf in zip(job.meshes, bufs):
+            buf.copy_(torch.as_tensor(mesh.vertices, dtype=torch.float32), non_blocking=False)
+    entry.graph.replay()
+    failed = entry.capture.verdict()
+    entry.replays += 1
+    if failed:
+        entry.fallbacks += 1
+        with schedule.recording():
+            results = run_cat_surface_gpu_pipelines(jobs, **eager_kw)
+        for r in results:
+            r.timings["graph_fallback"] = 1.0
+        return _finish_graph_results(jobs, results, write_output)
+    host = [(o.vertices.cpu().numpy(), np.asarray(o.rotation_angle.cpu(), dtype=np.float64),
+             float(o.rotation_cost.cpu())) for o in entry.outputs]
+    total = time.perf_counter() - t0
+    results = []
+    for job, o, (verts, angle, cost) in zip(jobs, entry.outputs, host):
+        results.append(CatSurfaceGpuResult(
+            vertices=np.ascontiguousarray(verts, dtype=np.float32),
+            faces=np.asarray(job.meshes[1].faces), rotation_angle=angle, rotation_cost=cost,
+            timings={"total_seconds": total, "graph_replay": 1.0},
+            output_path=Path(job.output) if job.output is not None else Path("")))
+    return _finish_graph_results(jobs, results, write_output)
+
+
+def _finish_graph_results(jobs, results, write_output: bool):
+    if write_output:
+        for job, res in zip(jobs, results):
+            if job.output is None:
+                raise ValueError("an output path is required when write_output is set")
+            write_gifti_mesh(Path(job.output), GiftiMesh(res.vertices, res.faces))
+    return results
+
+
+def _capture_schedule(jobs, kw, bounds, dev) -> _GraphSchedule:
+    started = time.perf_counter()
+    inputs = [[torch.as_tensor(np.asarray(m.vertices), dtype=torch.float32).to(dev).clone()
+               for m in job.meshes] for job in jobs]
+    cap_jobs = [PipelineJob(meshes=tuple(GiftiMesh(buf, m.faces)
+                                         for buf, m in zip(bufs, job.meshes)),
+                            map_size=job.map_size)
+                for job, bufs in zip(jobs, inputs)]
+    graph = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize(dev)
+    current = torch.cuda.current_stream(dev)
+    stream = torch.cuda.Stream(device=dev)
+    stream.wait_stream(current)
+    with torch.cuda.stream(stream), schedule.capture(bounds) as cap:
+        with torch.cuda.graph(graph, stream=stream):
+            outputs = run_cat_surface_gpu_pipelines(cap_jobs, **kw)
+        if cap.position != len(cap.bounds):
+            raise RuntimeError("the captured schedule issued fewer bounded loops than the "
+                               "recorded one")
+    current.wait_stream(stream)
+    torch.cuda.synchronize(dev)
+    return _GraphSchedule(graph=graph, inputs=inputs, outputs=outputs, capture=cap,
+                          capture_seconds=time.perf_counter() - started)

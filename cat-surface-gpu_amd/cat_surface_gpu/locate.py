@@ -4,8 +4,9 @@ Shared by the rotation search (surface_rotation.py) and the in-process
 stencil builder (stencil_builder.py).  The index is the reference's uniform
 grid over [-1, 1]^3 (src/cat_surface_gpu/surface_rotation.py:57-120): each
 triangle is listed, in face order, in every cell of its bounding box grown by
-``margin`` cells.  It is built on the GPU (per-face boxes, scan, emitted
-(cell, face) pairs, stable radix sort by cell).
+``margin`` cells.  It is built on the GPU: the exact two-phase build (per-face
+boxes, scan, emitted (cell, face) pairs, stable radix sort by cell) and the
+asynchronous one (per-cell counts, scan, fill, faces ranked within each cell).
 """
 
 from __future__ import annotations
@@ -87,7 +88,8 @@ def build_grid_index_async(points: torch.Tensor, faces: torch.Tensor, grid_size:
     lib = _lib.lib()
     cell_off = torch.empty(grid_size ** 3 + 1, dtype=torch.int32, device=dev)
     cell_faces = torch.empty(capacity, dtype=torch.int32, device=dev)
-    ws = _lib.workspace(lib.csg_grid_index_build_workspace_bytes(nf, capacity), dev)
+    ws = _lib.workspace(lib.csg_grid_index_build_workspace_bytes(nf, capacity, int(grid_size)),
+                        dev)
     _lib.check(lib.csg_grid_index_build(_lib.ptr(pts), _lib.dtype_code(pts.dtype), _lib.ptr(f), nf,
                                         int(grid_size), int(margin), int(capacity),
                                         _lib.ptr(cell_off), _lib.ptr(cell_faces),

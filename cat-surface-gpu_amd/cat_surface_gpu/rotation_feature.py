@@ -15,7 +15,7 @@ from typing import Sequence
 
 import torch
 
-from . import _lib
+from . import _lib, schedule
 from .surface_stencil import (SurfaceStencilDevice, _minmax, edge_heat,
                               heat_parameters_from_edge_mean, smooth_geometry_many, sor_patches)
 
@@ -100,15 +100,25 @@ def depth_potential(stencil: SurfaceStencilDevice, smoothed_surface, *,
         per_xcd = max(1, torch.cuda.get_device_properties(pts.device).multi_processor_count // 8)
         # the partition follows the stencil's sphere (compact patches)
         patches = sor_patches(topo, stencil.sphere_points, core=-(-n // per_xcd)).abi()
-    _lib.check(lib.csg_depth_potential_patched(
-        _lib.ptr(pts), batch, n, _lib.ptr(topo.faces), nf, _lib.ptr(topo.inc_off),
-        _lib.ptr(topo.inc), _lib.ptr(topo.sys_row_off), _lib.ptr(topo.sys_cols),
-        _lib.ptr(topo.sys_contrib_off), _lib.ptr(topo.sys_contrib), nnz, coff,
-        _lib.ptr(topo.colour_verts), int(topo.colour_off.size - 1), _lib.ptr(topo.ell_cols),
-        _lib.ptr(topo.ell_src), int(topo.ell_width), float(alpha), int(max_iter),
-        float(relaxation), None if patches is None else ctypes.byref(patches), _lib.ptr(x),
-        _lib.ptr(rel), _lib.ptr(ws), ws.numel(), _lib.stream_handle(pts.device)),
-        "csg_depth_potential")
+    args = (_lib.ptr(pts), batch, n, _lib.ptr(topo.faces), nf, _lib.ptr(topo.inc_off),
+            _lib.ptr(topo.inc), _lib.ptr(topo.sys_row_off), _lib.ptr(topo.sys_cols),
+            _lib.ptr(topo.sys_contrib_off), _lib.ptr(topo.sys_contrib), nnz, coff,
+            _lib.ptr(topo.colour_verts), int(topo.colour_off.size - 1), _lib.ptr(topo.ell_cols),
+            _lib.ptr(topo.ell_src), int(topo.ell_width), float(alpha), int(max_iter),
+            float(relaxation), None if patches is None else ctypes.byref(patches), _lib.ptr(x),
+            _lib.ptr(rel))
+    if schedule.capturing():
+        # nothing may wait for the host: the time-out and XCD-placement verdicts
+        # are checked after the graph's replay
+        status = torch.zeros(2, dtype=torch.int32, device=pts.device)
+        _lib.check(lib.csg_depth_potential_patched_deferred(
+            *args, _lib.ptr(status), _lib.ptr(ws), ws.numel(), _lib.stream_handle(pts.device)),
+            "csg_depth_potential_patched_deferred")
+        schedule.defer(status, "depth-potential SOR timed out or was misplaced")
+    else:
+        _lib.check(lib.csg_depth_potential_patched(
+            *args, _lib.ptr(ws), ws.numel(), _lib.stream_handle(pts.device)),
+            "csg_depth_potential")
     if single:
         return x[0], int(max_iter), rel[0]
     return x, int(max_iter), rel
@@ -287,7 +297,9 @@ def compute_rotation_features(stencils: Sequence[SurfaceStencilDevice],
         stack = torch.stack([geometries[i] for i in g]) if len(g) > 1 else geometries[g[0]]
         x, its, rel = depth_potential(stencils[g[0]], stack, alpha=depth_alpha,
                                       max_iter=max_iter, solver=solver, tolerance=tolerance)
-        rel_host = rel.reshape(-1).tolist()
+        # a captured schedule cannot read the residuals (not used downstream)
+        rel_host = ([float("nan")] * len(g) if schedule.capturing()
+                    else rel.reshape(-1).tolist())
         for k, i in enumerate(g):
             values[i] = x[k] if len(g) > 1 else x
             residuals[i] = float(rel_host[k])

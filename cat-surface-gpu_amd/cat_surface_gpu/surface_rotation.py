@@ -17,8 +17,8 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from . import _lib
-from .locate import GridIndex, build_grid_index
+from . import _lib, schedule
+from .locate import _CAPACITY, GridIndex, build_grid_index, build_grid_index_async
 
 
 def official_seed_grid_angles() -> np.ndarray:
@@ -246,7 +246,20 @@ class RotationGridIndexDevice:
         """Build straight from device tensors (no host round trip)."""
 
         f = faces.to(torch.int32).contiguous()
-        index: GridIndex = build_grid_index(points64.to(torch.float64), f, grid_size, margin)
+        if schedule.capturing():
+            # the pair capacity learned by earlier builds; the count the index
+            # needed is checked after the replay
+            key = (int(f.shape[0]), int(grid_size), int(margin))
+            if key not in _CAPACITY:
+                raise RuntimeError("no learned index capacity for this mesh size; run the "
+                                   "schedule eagerly first")
+            required = torch.zeros(1, dtype=torch.int32, device=f.device)
+            index = build_grid_index_async(points64.to(torch.float64), f, grid_size, margin,
+                                           _CAPACITY[key], required)
+            schedule.defer(schedule._exceeds([required], _CAPACITY[key]),
+                           "rotation index capacity")
+        else:
+            index = build_grid_index(points64.to(torch.float64), f, grid_size, margin)
         return cls(target_points=points64.to(torch.float32).contiguous(), target_faces=f,
                    target_triangles=points64[f.long()].to(torch.float32).contiguous(),
                    grid_size=grid_size, candidate_offsets=index.cell_off,
@@ -644,6 +657,32 @@ def _hopf_host() -> np.ndarray:
     return hopf_grid()
 
 
+def _top_k(costs: torch.Tensor, k: int) -> torch.Tensor:
+    """Indices (int64, device) of the k first costs in stable ascending order
+    (csg_top_k: ties by index, NaN last, as torch.sort(stable=True))."""
+
+    c = costs.to(torch.float64).contiguous()
+    k = min(int(k), int(c.numel()))
+    order = torch.empty(k, dtype=torch.int32, device=c.device)
+    _lib.check(_lib.lib().csg_top_k(_lib.ptr(c), int(c.numel()), k, _lib.ptr(order),
+                                    _lib.stream_handle(c.device)), "csg_top_k")
+    return order.to(torch.int64)
+
+
+_SEEDS_DEVICE: dict[str, torch.Tensor] = {}
+
+
+def _seeds_device(device) -> torch.Tensor:
+    """The 46 seed angle triples as a [46, 3] float64 device tensor (cached, so
+    that a captured schedule copies nothing from the host)."""
+
+    key = str(device)
+    if key not in _SEEDS_DEVICE:
+        _SEEDS_DEVICE[key] = torch.as_tensor(official_seed_grid_angles(), dtype=torch.float64,
+                                             device=device)
+    return _SEEDS_DEVICE[key]
+
+
 def _hopf_device(device) -> torch.Tensor:
     """The 8,192-rotation Hopf grid as a [8192, 9] float64 device tensor (cached)."""
 
@@ -707,6 +746,34 @@ def search_rotations(problems, *, refine: bool = True, max_iter: int = 500, tol:
     return out
 
 
+def _adjoint_results_on_device(problems, staged, candidates: str, refine: bool,
+                               steps: int) -> list[RotationSearchResult]:
+    """_search_rotations_adjoint's selection without a host read: the best
+    refined candidate (first minimum, as numpy's argmin) picked on the device,
+    its angles and cost kept there; the coverage counters become deferred
+    checks.  seed_angle is the pool's first candidate in cost order (device)."""
+
+    out = []
+    for problem, (costs, order, angles, final, unc, unc2) in zip(problems, staged):
+        index = problem[0]
+        dev = index.device
+        best = torch.argmin(final).reshape(1)
+        angle = angles.index_select(0, best).reshape(3)
+        cost = final.index_select(0, best).reshape(())
+        schedule.defer(unc, "rotation not covered by the target sphere's candidate index")
+        schedule.defer(unc2, "rotation not covered by the target sphere's candidate index")
+        first = order[:1]
+        if candidates == "seeds":
+            seed_angle = _seeds_device(dev).index_select(0, first).reshape(3)
+        else:
+            seed_angle = torch.full((3,), float("nan"), dtype=torch.float64, device=dev)
+        out.append(RotationSearchResult(
+            seed_angle=seed_angle, seed_cost=costs.index_select(0, first).reshape(()),
+            seed_costs=costs, angle=angle, cost=cost, rotation_matrix=None,
+            iterations=steps if refine else 0))
+    return out
+
+
 def map_costs(source_map: torch.Tensor, target_map: torch.Tensor, matrices: torch.Tensor, *,
               stride: int = 4) -> torch.Tensor:
     """Area-weighted SSD of candidate rotations on two [ny, nx] feature maps
@@ -737,8 +804,8 @@ def _search_rotations_adjoint(problems, *, refine: bool, candidates: str, top: i
     ((index, inputs, (source map, target map))) has the Hopf grid scored on the
     maps; otherwise every candidate is costed by point location."""
 
-    if top < 1:
-        raise ValueError("top must be positive")
+    if not 1 <= top <= 64:
+        raise ValueError("top must be between 1 and 64")
     staged = []
     for problem in problems:
         index, inputs = problem[0], problem[1]
@@ -746,7 +813,7 @@ def _search_rotations_adjoint(problems, *, refine: bool, candidates: str, top: i
         dev = index.device
         unc = torch.zeros(1, dtype=torch.int32, device=dev)
         if candidates == "seeds":
-            pool = torch.as_tensor(official_seed_grid_angles(), dtype=torch.float64, device=dev)
+            pool = _seeds_device(dev)
             costs, _, _, _ = index._launch_costs(inputs, pool, uncovered_dev=unc)
         elif maps is not None:
             pool = _hopf_device(dev)
@@ -754,8 +821,8 @@ def _search_rotations_adjoint(problems, *, refine: bool, candidates: str, top: i
         else:
             pool = _hopf_device(dev)
             costs = index._launch_matrix_costs(inputs, pool, False, unc)
-        order = torch.sort(costs, stable=True).indices[:top]
-        start = pool[order]
+        order = _top_k(costs, top)
+        start = pool.index_select(0, order)
         if refine and steps > 0:
             _, angles, final, unc2 = refine_adjoint_launch(
                 index, inputs, start, start_is_angles=candidates == "seeds", steps=steps)
@@ -763,6 +830,8 @@ def _search_rotations_adjoint(problems, *, refine: bool, candidates: str, top: i
             _, angles, final, unc2 = refine_adjoint_launch(
                 index, inputs, start, start_is_angles=candidates == "seeds", steps=0)
         staged.append((costs, order, angles, final, unc, unc2))
+    if schedule.capturing():
+        return _adjoint_results_on_device(problems, staged, candidates, refine, steps)
     host = torch.cat([torch.cat([order[:1].to(torch.float64), angles.reshape(-1), final,
                                  unc.to(torch.float64), unc2.to(torch.float64)])
                       for _, order, angles, final, unc, unc2 in staged]).cpu().numpy()

@@ -119,13 +119,11 @@ int csg_grid_index_fill(const void *plan_ws, int nf, int grid, int npairs, int *
                            ws_bytes - 3 * a, S(stream));
 }
 
-size_t csg_grid_index_build_workspace_bytes(int nf, int capacity)
+size_t csg_grid_index_build_workspace_bytes(int nf, int capacity, int grid)
 {
-    if (nf < 1 || capacity < 1)
+    if (nf < 1 || capacity < 1 || grid < 1 || (long long)grid * grid * grid >= (1LL << 30))
         return 0;
-    const size_t a = align256(sizeof(int) * (size_t)capacity);
-    const size_t sort = grid_sort_scratch_bytes(capacity);
-    return plan_layout(nf).scratch + 3 * a + align256(sort);
+    return plan_layout(nf).scratch + align256(grid_build_scratch_bytes(nf, capacity));
 }
 
 int csg_grid_index_build(const void *pts, int pts_dtype, const int *faces, int nf, int grid,
@@ -137,24 +135,28 @@ int csg_grid_index_build(const void *pts, int pts_dtype, const int *faces, int n
         return fail(kInvalidArgument, "invalid grid index arguments");
     if ((long long)grid * grid * grid >= (1LL << 30))
         return fail(kInvalidArgument, "grid size too large");
-    if (ws_bytes < csg_grid_index_build_workspace_bytes(nf, capacity))
+    if (ws_bytes < csg_grid_index_build_workspace_bytes(nf, capacity, grid))
         return fail(kInvalidArgument, "grid index build workspace too small");
     PlanLayout l = plan_layout(nf);
     char *b = (char *)ws;
-    const size_t a = align256(sizeof(int) * (size_t)capacity);
     int *box = (int *)(b + l.box), *counts = (int *)(b + l.counts),
         *offsets = (int *)(b + l.offsets);
-    int *keys_tmp = (int *)(b + l.scratch), *vals_tmp = (int *)(b + l.scratch + a),
-        *keys = (int *)(b + l.scratch + 2 * a);
-    char *tmp = b + l.scratch + 3 * a;
-    const size_t tmp_bytes = ws_bytes - (l.scratch + 3 * a);
+    char *tmp = b + l.scratch;
+    const size_t tmp_bytes = ws_bytes - l.scratch;
     if (pts_dtype == 1)
         return grid_index_build<double>((const double *)pts, faces, nf, grid, margin, capacity,
-                                        box, counts, offsets, keys_tmp, vals_tmp, keys, cell_faces,
-                                        cell_off, required, tmp, tmp_bytes, S(stream));
+                                        box, counts, offsets, cell_faces, cell_off, required,
+                                        tmp, tmp_bytes, S(stream));
     return grid_index_build<float>((const float *)pts, faces, nf, grid, margin, capacity, box,
-                                   counts, offsets, keys_tmp, vals_tmp, keys, cell_faces, cell_off,
-                                   required, tmp, tmp_bytes, S(stream));
+                                   counts, offsets, cell_faces, cell_off, required, tmp,
+                                   tmp_bytes, S(stream));
+}
+
+int csg_top_k(const double *costs, int n, int k, int *order, void *stream)
+{
+    if (n < 1 || k < 1 || k > kTopKMax || k > n || !costs || !order)
+        return fail(kInvalidArgument, "top-k: 1 <= k <= min(n, %d)", kTopKMax);
+    return launch_top_k(costs, n, k, order, S(stream));
 }
 
 size_t csg_rotation_costs_workspace_bytes(int ns, int na)

@@ -828,14 +828,17 @@ int csg_depth_potential(const float *pts, int batch, int n, const int *faces, in
                                        omega, nullptr, x, relres, ws, ws_bytes, stream);
 }
 
-int csg_depth_potential_patched(const float *pts, int batch, int n, const int *faces, int nf,
-                                const int *inc_off, const int *inc, const int *row_off,
-                                const int *cols, const int *contrib_off, const int *contrib,
-                                int nnz, const int *colour_off, const int *colour_verts,
-                                int ncolours, const int *ell_cols, const int *ell_src,
-                                int ell_width, double alpha, int max_iter, double omega,
-                                const csg_sor_patches *patches, double *x, double *relres,
-                                void *ws, size_t ws_bytes, void *stream)
+// csg_depth_potential_patched; with status (device int[2]) nothing waits for
+// the host: the time-out and placement verdicts stay on the device and a
+// misplaced run is not repeated.
+static int depth_patched_impl(const float *pts, int batch, int n, const int *faces, int nf,
+                              const int *inc_off, const int *inc, const int *row_off,
+                              const int *cols, const int *contrib_off, const int *contrib,
+                              int nnz, const int *colour_off, const int *colour_verts,
+                              int ncolours, const int *ell_cols, const int *ell_src,
+                              int ell_width, double alpha, int max_iter, double omega,
+                              const csg_sor_patches *patches, double *x, double *relres,
+                              int *status, void *ws, size_t ws_bytes, void *stream)
 {
     SorPatches pp{};
     if (patches) {
@@ -885,6 +888,12 @@ int csg_depth_potential_patched(const float *pts, int batch, int n, const int *f
     SorRun run{};
     const bool allow_local = !xcd_local_unusable();
     CSG_TRY(depth_sor(a, co, max_iter, omega, patches ? &pp : nullptr, allow_local, &run, s));
+    if (status) {
+        CSG_TRY(sor_status(a, run, status, s));
+        if (relres)
+            CSG_TRY(depth_residual(a, relres, s));
+        return kOk;
+    }
     int rc = kOk;
     if (!sor_placement_complete(a, run, s, &rc)) {
         if (rc != kOk)
@@ -904,6 +913,40 @@ int csg_depth_potential_patched(const float *pts, int batch, int n, const int *f
     if (timeout)
         return fail(kRuntimeError, "depth-potential SOR grid barrier timed out");
     return kOk;
+}
+
+int csg_depth_potential_patched(const float *pts, int batch, int n, const int *faces, int nf,
+                                const int *inc_off, const int *inc, const int *row_off,
+                                const int *cols, const int *contrib_off, const int *contrib,
+                                int nnz, const int *colour_off, const int *colour_verts,
+                                int ncolours, const int *ell_cols, const int *ell_src,
+                                int ell_width, double alpha, int max_iter, double omega,
+                                const csg_sor_patches *patches, double *x, double *relres,
+                                void *ws, size_t ws_bytes, void *stream)
+{
+    return depth_patched_impl(pts, batch, n, faces, nf, inc_off, inc, row_off, cols, contrib_off,
+                              contrib, nnz, colour_off, colour_verts, ncolours, ell_cols, ell_src,
+                              ell_width, alpha, max_iter, omega, patches, x, relres, nullptr, ws,
+                              ws_bytes, stream);
+}
+
+int csg_depth_potential_patched_deferred(const float *pts, int batch, int n, const int *faces,
+                                         int nf, const int *inc_off, const int *inc,
+                                         const int *row_off, const int *cols,
+                                         const int *contrib_off, const int *contrib, int nnz,
+                                         const int *colour_off, const int *colour_verts,
+                                         int ncolours, const int *ell_cols, const int *ell_src,
+                                         int ell_width, double alpha, int max_iter, double omega,
+                                         const csg_sor_patches *patches, double *x,
+                                         double *relres, int *status, void *ws, size_t ws_bytes,
+                                         void *stream)
+{
+    if (!status)
+        return fail(kInvalidArgument, "the deferred depth potential needs a status array");
+    return depth_patched_impl(pts, batch, n, faces, nf, inc_off, inc, row_off, cols, contrib_off,
+                              contrib, nnz, colour_off, colour_verts, ncolours, ell_cols, ell_src,
+                              ell_width, alpha, max_iter, omega, patches, x, relres, status, ws,
+                              ws_bytes, stream);
 }
 
 size_t csg_depth_pcg_workspace_bytes(int n, int nf, int nnz, int batch)
@@ -1018,7 +1061,23 @@ int csg_prepare_sphere(const float *in, int n, const double *centroid, const dou
 {
     if (n < 1 || in == nullptr || out == nullptr || (centroid && !bad))
         return fail(kInvalidArgument, "invalid sphere preparation arguments");
-    return launch_sphere_prepare(in, n, centroid, rotation, out, bad, S(stream));
+    return launch_sphere_prepare(in, n, centroid, rotation, nullptr, out, bad, S(stream));
+}
+
+int csg_prepare_sphere_dev(const float *in, int n, const double *centroid,
+                           const double *rotation_dev, float *out, int *bad, void *stream)
+{
+    if (n < 1 || in == nullptr || out == nullptr || rotation_dev == nullptr ||
+        (centroid && !bad))
+        return fail(kInvalidArgument, "invalid sphere preparation arguments");
+    return launch_sphere_prepare(in, n, centroid, nullptr, rotation_dev, out, bad, S(stream));
+}
+
+int csg_rotation_matrices(const double *angles, int count, double *out, void *stream)
+{
+    if (count < 0 || (count > 0 && (!angles || !out)))
+        return fail(kInvalidArgument, "invalid rotation matrix arguments");
+    return launch_rotation_matrices(angles, count, out, S(stream));
 }
 
 int csg_average_xz(const void *avg, const void *main_, int n, int dtype, const double *unrotate,

@@ -59,6 +59,11 @@ struct SorRun {
 };
 
 int depth_setup(const DepthArgs &a, hipStream_t s);
+// Zero the barrier state (counters, verdicts, time-out flag) with a kernel.
+int clear_barrier(unsigned *bar, hipStream_t s);
+// The run's time-out and placement verdicts into status[0..1] on the device
+// (no host round trip): nonzero means the solve is not valid.
+int sor_status(const DepthArgs &a, const SorRun &run, int *status, hipStream_t s);
 // patches (may be null): sweep in LDS on those patches when they fit the
 // device (one workgroup per patch and mesh, all resident), else as without.
 // allow_xcd_local = false keeps the sweeps on the device-wide grid barrier.

@@ -628,6 +628,23 @@ k_sor_patched(SorPatches pp, int ncolours, const double *__restrict__ ell_val,
     }
 }
 
+__global__ void __launch_bounds__(kBlock) k_clear_words(unsigned *__restrict__ w, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        w[i] = 0u;
+}
+
+// The whole barrier state cleared by a kernel (a memset node would not be
+// safe in a captured graph).
+int clear_barrier(unsigned *bar, hipStream_t s)
+{
+    const int n = kBarrierSlots * kBarrierSlotWords;
+    hipLaunchKernelGGL(k_clear_words, dim3(grid_for(n)), dim3(kBlock), 0, s, bar, n);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
 // Between two launches that share the barrier state: clear the eight claim and
 // eight arrival counters (slots 0 .. 15); the timeout flag and the placement
 // verdict stay.
@@ -749,8 +766,7 @@ int depth_sor(const DepthArgs &a, const SorColours &co, int max_iter, double ome
                        a.colour_verts, a.ell_src, W, a.n, a.sys, a.nnz, a.rhs, a.diag, a.ell_val,
                        a.rhs_c, a.diag_c);
     CSG_LAUNCH_CHECK();
-    CSG_HIP_CHECK(hipMemsetAsync(a.barrier, 0,
-                                 sizeof(unsigned) * kBarrierSlots * kBarrierSlotWords, s));
+    CSG_TRY(clear_barrier(a.barrier, s));
     if (max_iter <= 0)
         return kOk;
     int dev = 0;
@@ -922,6 +938,33 @@ bool sor_placement_complete(const DepthArgs &a, const SorRun &run, hipStream_t s
         return false;
     }
     return verdict == 0;
+}
+
+// status[0] = the barrier time-out flag, status[1] = 1 when an XCD-local run's
+// groups were not all placed (claims short of the plan, or an earlier launch's
+// verdict): what sor_placement_complete and the host's time-out read decide,
+// left on the device.
+__global__ void k_sor_status(const unsigned *__restrict__ bar, SorXcdPlan plan, int local,
+                             int *__restrict__ status)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    status[0] = bar[(size_t)(kBarrierSlots - 1) * kBarrierSlotWords] != 0u ? 1 : 0;
+    int bad = 0;
+    if (local) {
+        for (int x = 0; x < kXcds; ++x)
+            bad |= bar[(size_t)x * kBarrierSlotWords] < (unsigned)plan.blocks[x];
+        bad |= bar[(size_t)(kBarrierSlots - 2) * kBarrierSlotWords] != 0u;
+    }
+    status[1] = bad;
+}
+
+int sor_status(const DepthArgs &a, const SorRun &run, int *status, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sor_status, dim3(1), dim3(64), 0, s, a.barrier, run.plan,
+                       run.xcd_local ? 1 : 0, status);
+    CSG_LAUNCH_CHECK();
+    return kOk;
 }
 
 int depth_residual(const DepthArgs &a, double *out, hipStream_t s)

@@ -273,8 +273,7 @@ int depth_pcg(const DepthArgs &d, double *ws, int max_iter, double tol, bool all
     a.tol = tol;
     a.iterations = iterations;
     a.relres = relres;
-    CSG_HIP_CHECK(hipMemsetAsync(d.barrier, 0,
-                                 sizeof(unsigned) * kBarrierSlots * kBarrierSlotWords, s));
+    CSG_TRY(clear_barrier(d.barrier, s));
     int *timeout = reinterpret_cast<int *>(d.barrier + (size_t)(kBarrierSlots - 1) * kBarrierSlotWords);
     int dev = 0, cus = 0, per_cu = 0;
     CSG_HIP_CHECK(hipGetDevice(&dev));

@@ -13,14 +13,20 @@ size_t grid_sort_scratch_bytes(int npairs);
 // The whole index without a host round trip: at most `capacity` pairs are
 // listed (cell_faces holds capacity ints) and *required (device) receives the
 // count the index needs; the index is complete iff *required <= capacity.
+// Built with this library's own scan and stable radix sort (no library call:
+// nothing that records a memset into a graph).
 template <typename P>
 int grid_index_build(const P *pts, const int *faces, int nf, int g, int margin, int capacity,
-                     int *box, int *counts, int *offsets, int *keys_tmp, int *vals_tmp, int *keys,
-                     int *cell_faces, int *cell_off, int *required, void *tmp, size_t tmp_bytes,
-                     hipStream_t s);
+                     int *box, int *counts, int *offsets, int *cell_faces, int *cell_off,
+                     int *required, void *tmp, size_t tmp_bytes, hipStream_t s);
+size_t grid_build_scratch_bytes(int nf, int capacity);
 int grid_index_fill(const int *box, const int *offsets, int nf, int g, int npairs, int *keys_tmp,
                     int *vals_tmp, int *keys, int *vals, int *cell_off, void *tmp,
                     size_t tmp_bytes, hipStream_t s);
+
+// The k (<= kTopKMax) first of n costs in stable ascending order (NaN last).
+constexpr int kTopKMax = 64;
+int launch_top_k(const double *costs, int n, int k, int *order, hipStream_t s);
 
 // Lanes that share one query point in the location kernels, each scanning
 // every kLocateLanes-th candidate of the point's cell.

@@ -177,37 +177,278 @@ int grid_index_fill(const int *box, const int *offsets, int nf, int g, int npair
     return kOk;
 }
 
+// ---- the index without library calls (so that it can be captured in a graph:
+// the library's scan and sort record memsets): the face-order scan and the
+// stable LSD radix sort of the (cell, face) pairs by cell are written here.
+
+__global__ void __launch_bounds__(kBlock) k_fill_int(int *__restrict__ x, int v, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        x[i] = v;
+}
+
+constexpr int kScanItems = 8;  // per thread; a block scans kBlock * kScanItems values
+
+// Exclusive scan, phase 1: each block's values scanned locally, its total in
+// sums[block].
+__global__ void __launch_bounds__(kBlock)
+k_scan_local(const int *__restrict__ in, int n, int *__restrict__ out, int *__restrict__ sums)
+{
+    __shared__ int part[kBlock];
+    const int base = blockIdx.x * kBlock * kScanItems + threadIdx.x * kScanItems;
+    int v[kScanItems], run = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0;
+        run += v[k];
+    }
+    part[threadIdx.x] = run;
+    __syncthreads();
+    // Hillis-Steele over the threads' totals
+    for (int d = 1; d < kBlock; d <<= 1) {
+        const int t = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int acc = part[threadIdx.x] - run;  // exclusive prefix of this thread
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n)
+            out[base + k] = acc;
+        acc += v[k];
+    }
+    if (threadIdx.x == kBlock - 1)
+        sums[blockIdx.x] = part[kBlock - 1];
+}
+
+// Phase 2: the block totals scanned in place by one block, in order.
+__global__ void __launch_bounds__(kBlock) k_scan_sums(int *__restrict__ sums, int nblocks)
+{
+    __shared__ int part[kBlock];
+    __shared__ int carry;
+    if (threadIdx.x == 0)
+        carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nblocks; b0 += kBlock) {
+        const int i = b0 + threadIdx.x;
+        const int v = i < nblocks ? sums[i] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int d = 1; d < kBlock; d <<= 1) {
+            const int t = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+            __syncthreads();
+            part[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nblocks)
+            sums[i] = carry + part[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == kBlock - 1)
+            carry += part[kBlock - 1];
+        __syncthreads();
+    }
+}
+
+// Phase 3: each block's offset added.
+__global__ void __launch_bounds__(kBlock)
+k_scan_add(int *__restrict__ out, int n, const int *__restrict__ sums)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] += sums[i / (kBlock * kScanItems)];
+}
+
+int scan_blocks(int n) { return (n + kBlock * kScanItems - 1) / (kBlock * kScanItems); }
+
+// out[0..n) = exclusive prefix sums of in[0..n) (sums: scan_blocks(n) ints).
+int exclusive_scan_int(const int *in, int n, int *out, int *sums, hipStream_t s)
+{
+    const int nb = scan_blocks(n);
+    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kBlock), 0, s, in, n, out, sums);
+    CSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, sums, nb);
+    CSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_add, dim3(grid_for(n)), dim3(kBlock), 0, s, out, n, sums);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+// Stable LSD radix sort of (key, value) int pairs, 8 bits per pass.  A block
+// takes a tile of kRadixTile consecutive pairs in rounds of kRadixThreads
+// (pair base + r * kRadixThreads + t), so (round, wave, lane) order is the
+// input order; the digit offsets are scanned digit-major, block-minor.
+constexpr int kRadixThreads = 256;
+constexpr int kRadixItems = 8;
+constexpr int kRadixTile = kRadixThreads * kRadixItems;
+constexpr int kRadixWaves = kRadixThreads / 64;
+
+// hist[d * nblocks + b] = pairs of block b whose digit is d.
+__global__ void __launch_bounds__(kRadixThreads)
+k_radix_hist(const int *__restrict__ keys, int n, int shift, int nblocks, int *__restrict__ hist)
+{
+    __shared__ int cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kRadixTile;
+    for (int r = 0; r < kRadixItems; ++r) {
+        const int i = base + r * kRadixThreads + threadIdx.x;
+        if (i < n)
+            atomicAdd(&cnt[((unsigned)keys[i] >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Each pair goes to offs[d * nblocks + b] + (pairs of digit d before it in the
+// block): a wave ranks its lanes among the lanes of the same digit from eight
+// ballots, the waves' counts per digit are added in wave order.
+__global__ void __launch_bounds__(kRadixThreads)
+k_radix_scatter(const int *__restrict__ keys_in, const int *__restrict__ vals_in, int n,
+                int shift, int nblocks, const int *__restrict__ offs, int *__restrict__ keys_out,
+                int *__restrict__ vals_out)
+{
+    __shared__ int next[256];
+    __shared__ int wcnt[kRadixWaves][256];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    next[t] = offs[t * nblocks + blockIdx.x];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int base = blockIdx.x * kRadixTile;
+    for (int r = 0; r < kRadixItems; ++r) {
+#pragma unroll
+        for (int q = 0; q < kRadixWaves; ++q)
+            wcnt[q][t] = 0;
+        __syncthreads();
+        const int i = base + r * kRadixThreads + t;
+        const bool valid = i < n;
+        int key = 0, val = 0, d = 0;
+        if (valid) {
+            key = keys_in[i];
+            val = vals_in[i];
+            d = (int)(((unsigned)key >> shift) & 255u);
+        }
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool set = ((d >> bit) & 1) != 0;
+            const unsigned long long m = __ballot(set);
+            peers &= set ? m : ~m;
+        }
+        const int rank = __popcll(peers & below);
+        if (valid && rank == 0)
+            wcnt[w][d] = __popcll(peers);
+        __syncthreads();
+        // thread t: the waves' prefix counts of digit t, and the round's total
+        int acc = 0;
+        int pre[kRadixWaves];
+#pragma unroll
+        for (int q = 0; q < kRadixWaves; ++q) {
+            pre[q] = acc;
+            acc += wcnt[q][t];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kRadixWaves; ++q)
+            wcnt[q][t] = pre[q];
+        __syncthreads();
+        if (valid) {
+            const int pos = next[d] + wcnt[w][d] + rank;
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+        next[t] += acc;
+    }
+}
+
+int radix_blocks(int n) { return (n + kRadixTile - 1) / kRadixTile; }
+
+size_t radix_scratch_bytes(int n)
+{
+    const size_t hist = (size_t)256 * radix_blocks(n);
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return 2 * al(sizeof(int) * hist) + al(sizeof(int) * ((size_t)scan_blocks((int)hist) + 1));
+}
+
+// Sorts n pairs by the low `bits` bits of the keys; (ka, va) hold the input and
+// receive the result, (kb, vb) are the second buffers (n ints each).
+int radix_sort_pairs(int *ka, int *va, int *kb, int *vb, int n, int bits, void *tmp,
+                     size_t tmp_bytes, hipStream_t s)
+{
+    if (n <= 0)
+        return kOk;
+    if (tmp_bytes < radix_scratch_bytes(n))
+        return fail(kInvalidArgument, "radix sort scratch too small");
+    const int nb = radix_blocks(n);
+    const size_t hist_n = (size_t)256 * nb;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    char *t = (char *)tmp;
+    int *hist = (int *)t;
+    int *offs = (int *)(t + al(sizeof(int) * hist_n));
+    int *sums = (int *)(t + 2 * al(sizeof(int) * hist_n));
+    int *src_k = ka, *src_v = va, *dst_k = kb, *dst_v = vb;
+    const int passes = (bits + 7) / 8;
+    for (int p = 0; p < passes; ++p) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kRadixThreads), 0, s, src_k, n, 8 * p,
+                           nb, hist);
+        CSG_LAUNCH_CHECK();
+        CSG_TRY(exclusive_scan_int(hist, (int)hist_n, offs, sums, s));
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kRadixThreads), 0, s, src_k, src_v,
+                           n, 8 * p, nb, offs, dst_k, dst_v);
+        CSG_LAUNCH_CHECK();
+        int *tk = src_k, *tv = src_v;
+        src_k = dst_k;
+        src_v = dst_v;
+        dst_k = tk;
+        dst_v = tv;
+    }
+    if (src_k != ka) {
+        CSG_HIP_CHECK(hipMemcpyAsync(ka, src_k, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice,
+                                     s));
+        CSG_HIP_CHECK(hipMemcpyAsync(va, src_v, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice,
+                                     s));
+    }
+    return kOk;
+}
+
+size_t grid_build_scratch_bytes(int nf, int capacity)
+{
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return 3 * al(sizeof(int) * (size_t)capacity) +                  // keys, values, keys 2
+           al(sizeof(int) * ((size_t)scan_blocks(nf + 1) + 1)) +    // face scan sums
+           al(radix_scratch_bytes(capacity));
+}
+
 template <typename P>
 int grid_index_build(const P *pts, const int *faces, int nf, int g, int margin, int capacity,
-                     int *box, int *counts, int *offsets, int *keys_tmp, int *vals_tmp, int *keys,
-                     int *cell_faces, int *cell_off, int *required, void *tmp, size_t tmp_bytes,
-                     hipStream_t s)
+                     int *box, int *counts, int *offsets, int *cell_faces, int *cell_off,
+                     int *required, void *tmp, size_t tmp_bytes, hipStream_t s)
 {
+    if (tmp_bytes < grid_build_scratch_bytes(nf, capacity))
+        return fail(kInvalidArgument, "grid index build scratch too small");
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t a = al(sizeof(int) * (size_t)capacity);
+    char *t = (char *)tmp;
+    int *keys = (int *)t, *keys2 = (int *)(t + a), *vals2 = (int *)(t + 2 * a);
+    int *sums = (int *)(t + 3 * a);
+    char *rtmp = t + 3 * a + al(sizeof(int) * ((size_t)scan_blocks(nf + 1) + 1));
+    const size_t rbytes = tmp_bytes - (size_t)(rtmp - t);
     hipLaunchKernelGGL(k_face_boxes<P>, dim3(grid_for((size_t)nf + 1)), dim3(kBlock), 0, s, pts,
                        faces, nf, g, margin, box, counts);
     CSG_LAUNCH_CHECK();
-    size_t need = 0;
-    CSG_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, counts, offsets, nf + 1, s));
-    if (need > tmp_bytes)
-        return fail(kInvalidArgument, "grid index scan scratch too small");
-    CSG_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, need, counts, offsets, nf + 1, s));
+    CSG_TRY(exclusive_scan_int(counts, nf + 1, offsets, sums, s));
     const int ncells = g * g * g;
     hipLaunchKernelGGL(k_pad_keys, dim3(grid_for(capacity)), dim3(kBlock), 0, s, offsets + nf,
-                       capacity, ncells, keys_tmp, required);
+                       capacity, ncells, keys, required);
     CSG_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_emit_pairs_capped, dim3(grid_for(nf)), dim3(kBlock), 0, s, box, offsets,
-                       nf, g, capacity, keys_tmp, vals_tmp);
+                       nf, g, capacity, keys, cell_faces);
     CSG_LAUNCH_CHECK();
     int bits = 1;
     while ((1LL << bits) <= (long long)ncells)  // keys run to the sentinel ncells
         ++bits;
-    need = 0;
-    CSG_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, keys_tmp, keys, vals_tmp,
-                                                     cell_faces, capacity, 0, bits, s));
-    if (need > tmp_bytes)
-        return fail(kInvalidArgument, "grid index sort scratch too small");
-    CSG_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, need, keys_tmp, keys, vals_tmp,
-                                                     cell_faces, capacity, 0, bits, s));
+    CSG_TRY(radix_sort_pairs(keys, cell_faces, keys2, vals2, capacity, bits, rtmp, rbytes, s));
     hipLaunchKernelGGL(k_cell_offsets, dim3(grid_for((size_t)ncells + 1)), dim3(kBlock), 0, s,
                        keys, capacity, ncells, cell_off);
     CSG_LAUNCH_CHECK();
@@ -215,15 +456,80 @@ int grid_index_build(const P *pts, const int *faces, int nf, int g, int margin, 
 }
 
 template int grid_index_build<float>(const float *, const int *, int, int, int, int, int *, int *,
-                                     int *, int *, int *, int *, int *, int *, int *, void *,
-                                     size_t, hipStream_t);
+                                     int *, int *, int *, int *, void *, size_t, hipStream_t);
 template int grid_index_build<double>(const double *, const int *, int, int, int, int, int *,
-                                      int *, int *, int *, int *, int *, int *, int *, int *,
-                                      void *, size_t, hipStream_t);
+                                      int *, int *, int *, int *, int *, void *, size_t,
+                                      hipStream_t);
 template int grid_index_count<float>(const float *, const int *, int, int, int, int *, int *,
                                      int *, int *, void *, size_t, hipStream_t);
 template int grid_index_count<double>(const double *, const int *, int, int, int, int *, int *,
                                       int *, int *, void *, size_t, hipStream_t);
+
+// ------------------------------------------------------------ top k
+
+// (cost, index) order: ascending cost, NaN after every number, ties by index
+// (the order of a stable ascending sort).
+__device__ __forceinline__ bool cost_before(double a, int ia, double b, int ib)
+{
+    const bool na = a != a, nb = b != b;
+    if (na != nb)
+        return nb;
+    if (!na && a != b)
+        return a < b;
+    return ia < ib;
+}
+
+// order[0..k) = the indices of the k first costs in that order: k rounds of a
+// block-wide minimum over the costs not yet chosen.  One block.
+__global__ void __launch_bounds__(1024)
+k_top_k(const double *__restrict__ costs, int n, int k, int *__restrict__ order)
+{
+    __shared__ double bc[1024];
+    __shared__ int bi[1024];
+    __shared__ int chosen[kTopKMax];
+    const int t = threadIdx.x;
+    for (int r = 0; r < k; ++r) {
+        double best = 0.0;
+        int bidx = -1;
+        for (int i = t; i < n; i += blockDim.x) {
+            bool taken = false;
+            for (int q = 0; q < r; ++q)
+                taken |= chosen[q] == i;
+            if (taken)
+                continue;
+            const double c = costs[i];
+            if (bidx < 0 || cost_before(c, i, best, bidx)) {
+                best = c;
+                bidx = i;
+            }
+        }
+        bc[t] = best;
+        bi[t] = bidx;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if (t < w) {
+                const int oi = bi[t + w];
+                if (oi >= 0 && (bi[t] < 0 || cost_before(bc[t + w], oi, bc[t], bi[t]))) {
+                    bc[t] = bc[t + w];
+                    bi[t] = oi;
+                }
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            chosen[r] = bi[0];
+            order[r] = bi[0];
+        }
+        __syncthreads();
+    }
+}
+
+int launch_top_k(const double *costs, int n, int k, int *order, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_top_k, dim3(1), dim3(1024), 0, s, costs, n, k, order);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
 
 // ------------------------------------------------------------ rotation cost
 

@@ -132,6 +132,7 @@ int launch_sphere_centroids(const SphereList &list, double *centroid, hipStream_
 // float32.  *bad is set to 1 for a vertex of zero or non-finite length (the
 // caller clears it).
 int launch_sphere_prepare(const float *v, int n, const double *centroid, const double *m_host,
-                          float *out, int *bad, hipStream_t s);
+                          const double *m_dev, float *out, int *bad, hipStream_t s);
+int launch_rotation_matrices(const double *angles, int count, double *out, hipStream_t s);
 
 }  // namespace csg

@@ -838,8 +838,12 @@ k_sequential_centroids(SphereList list, double *__restrict__ centroid)
 // with `m` (row-major 3x3), f32(f64(p) · mᵀ).
 __global__ void __launch_bounds__(kBlock)
 k_normalise_rotate(const float *__restrict__ v, int n, const double *__restrict__ centroid,
-                   Mat3 m, int rotate, float *__restrict__ out, int *__restrict__ bad)
+                   Mat3 m, const double *__restrict__ m_dev, int rotate, float *__restrict__ out,
+                   int *__restrict__ bad)
 {
+    if (m_dev)
+        for (int k = 0; k < 9; ++k)
+            m.a[k] = m_dev[k];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
@@ -884,7 +888,7 @@ int launch_sphere_centroids(const SphereList &list, double *centroid, hipStream_
 }
 
 int launch_sphere_prepare(const float *v, int n, const double *centroid, const double *m_host,
-                          float *out, int *bad, hipStream_t s)
+                          const double *m_dev, float *out, int *bad, hipStream_t s)
 {
     if (n <= 0)
         return kOk;
@@ -893,7 +897,45 @@ int launch_sphere_prepare(const float *v, int n, const double *centroid, const d
         for (int k = 0; k < 9; ++k)
             m.a[k] = m_host[k];
     hipLaunchKernelGGL(k_normalise_rotate, dim3(grid_for(n)), dim3(kBlock), 0, s, v, n, centroid,
-                       m, m_host ? 1 : 0, out, bad);
+                       m, m_dev, (m_host || m_dev) ? 1 : 0, out, bad);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
+// Rotation matrices of Euler-angle triples, rotation_matrix_np's arithmetic
+// (Rz (Ry Rx) assembled column-major, read row-major, each product sum left to
+// right, no contraction), one thread per triple.
+__global__ void k_rotation_matrices(const double *__restrict__ angles, int count,
+                                    double *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count)
+        return;
+    const double a = angles[3 * t], b = angles[3 * t + 1], g = angles[3 * t + 2];
+    const double ca = cos(a), sa = sin(a), cb = cos(b), sb = sin(b), cg = cos(g), sg = sin(g);
+    const double rx[9] = {1.0, 0.0, 0.0, 0.0, ca, sa, 0.0, -sa, ca};
+    const double ry[9] = {cb, 0.0, sb, 0.0, 1.0, 0.0, -sb, 0.0, cb};
+    const double rz[9] = {cg, sg, 0.0, -sg, cg, 0.0, 0.0, 0.0, 1.0};
+    double first[9], res[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            first[r + 3 * c] = (ry[r] * rx[3 * c] + ry[r + 3] * rx[1 + 3 * c]) +
+                               ry[r + 6] * rx[2 + 3 * c];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            res[r + 3 * c] = (rz[r] * first[3 * c] + rz[r + 3] * first[1 + 3 * c]) +
+                             rz[r + 6] * first[2 + 3 * c];
+    for (int k = 0; k < 9; ++k)
+        out[9 * (size_t)t + k] = res[k];
+}
+
+int launch_rotation_matrices(const double *angles, int count, double *out, hipStream_t s)
+{
+    if (count <= 0)
+        return kOk;
+    hipLaunchKernelGGL(k_rotation_matrices, dim3(grid_for(count)), dim3(kBlock), 0, s, angles,
+                       count, out);
     CSG_LAUNCH_CHECK();
     return kOk;
 }

@@ -442,6 +442,22 @@ int csg_depth_potential_patched(const float *pts, int batch, int n, const int *f
                                 const csg_sor_patches *patches, double *x, double *relres,
                                 void *ws, size_t ws_bytes, void *stream);
 
+/* csg_depth_potential_patched without waiting for the host: status (device int[2]) receives
+ * the barrier time-out flag and, for XCD-local runs, whether every XCD group was placed; a
+ * nonzero status means x is not valid (the host-waiting entry would have repeated the solve
+ * on the device-wide barrier instead).  For schedules captured in a graph.  No counterpart
+ * in the reference. */
+int csg_depth_potential_patched_deferred(const float *pts, int batch, int n, const int *faces,
+                                         int nf, const int *inc_off, const int *inc,
+                                         const int *row_off, const int *cols,
+                                         const int *contrib_off, const int *contrib, int nnz,
+                                         const int *colour_off, const int *colour_verts,
+                                         int ncolours, const int *ell_cols, const int *ell_src,
+                                         int ell_width, double alpha, int max_iter, double omega,
+                                         const csg_sor_patches *patches, double *x,
+                                         double *relres, int *status, void *ws, size_t ws_bytes,
+                                         void *stream);
+
 /* Depth potential of the rotation feature by Jacobi-preconditioned conjugate gradients from
  * x = 0, to relative residual tol or max_iter iterations, for `batch` meshes of one topology
  * (the same system as csg_depth_potential).  Each mesh's workgroups run on one XCD and
@@ -481,6 +497,16 @@ int csg_sphere_centroids(const float *const *spheres, const int *counts, int cou
 int csg_prepare_sphere(const float *in, int n, const double *centroid, const double *rotation,
                        float *out, int *bad, void *stream);
 
+/* csg_prepare_sphere with the rotation (row-major 3x3) read from device memory, e.g. a rotation
+ * search's result that the host has not seen; rotation_dev is required. */
+int csg_prepare_sphere_dev(const float *in, int n, const double *centroid,
+                           const double *rotation_dev, float *out, int *bad, void *stream);
+
+/* Rotation matrices (row-major 3x3 each, out [count][9]) of device Euler-angle triples
+ * (alpha, beta, gamma): Rz(gamma) (Ry(beta) Rx(alpha)) assembled column-major and read
+ * row-major, as surface_rotation.py:223-300 builds them, in float64 on the device. */
+int csg_rotation_matrices(const double *angles, int count, double *out, void *stream);
+
 /* Un-rotate the pole-frame warp by `unrotate` (host, row-major 3x3, p m^T) and blend it
  * with the main warp.  Replaces gpu_pipeline._average_xz_surfaces and its caller
  * (src/cat_surface_gpu/gpu_pipeline.py:248-271, :1294-1310). */
@@ -503,8 +529,10 @@ int csg_grid_index_fill(const void *plan_ws, int nf, int grid, int npairs, int *
  * are listed (cell_faces: capacity ints, cell_off: grid^3 + 1 ints) and *required (device
  * int) receives the pair count the index needs.  The index equals the two-phase one iff
  * *required <= capacity; otherwise it lists a subset (every access stays in bounds) and
- * must be rebuilt with capacity >= *required. */
-size_t csg_grid_index_build_workspace_bytes(int nf, int capacity);
+ * must be rebuilt with capacity >= *required.  Built with this library's own scan and stable
+ * LSD radix sort (no library call that records a memset), so it can be captured in a graph;
+ * grid is needed for the workspace size. */
+size_t csg_grid_index_build_workspace_bytes(int nf, int capacity, int grid);
 int csg_grid_index_build(const void *pts, int pts_dtype, const int *faces, int nf, int grid,
                          int margin, int capacity, int *cell_off, int *cell_faces, int *required,
                          void *ws, size_t ws_bytes, void *stream);
@@ -514,6 +542,12 @@ int csg_grid_index_build(const void *pts, int pts_dtype, const int *faces, int n
  * synchronise), else in the workspace; uncovered_host (optional) receives it after a
  * synchronisation.
  * Replaces RotationGridIndexDevice.cost_batch_prepared (surface_rotation.py:512-546). */
+/* order[0..k) = the indices of the first k of n device costs in stable ascending order (ties
+ * by index, NaN last), 1 <= k <= min(n, 64): the candidate selection
+ * torch.sort(costs, stable=True).indices[:k] of the batched rotation search, without a
+ * library sort (so that it can be captured in a graph).  No counterpart in the reference. */
+int csg_top_k(const double *costs, int n, int k, int *order, void *stream);
+
 size_t csg_rotation_costs_workspace_bytes(int ns, int na);
 /* Face bounding spheres for the rotation locator (tri: [nf][3][3] float32 → sph: [nf][4]
  * float32 centre and radius, enlarged to cover the locator's tolerances).  Passed as `sph`
