@@ -348,17 +348,15 @@ def solve_dartel_from_surfaces_jobs(requests, *, n_steps: int = 2, loop: int = 6
     results: list[DartelSolveResult | None] = [None] * len(requests)
     kw = dict(loop=loop, lmreg=lmreg, cycles=cycles, nit=nit, its=its, code=code, device=dev,
               dtype=dtype, assume_initial_nonzero=assume_initial_nonzero, cuda_graph=cuda_graph)
-    if len(groups) == 1 or schedule.capturing():
-        # one shape, or a schedule being captured into one graph (one stream)
-        results = [None] * len(requests)
-        for members in groups.values():
-            src = torch.cat([sheets[2 * k] for k in members])
-            tgt = torch.cat([sheets[2 * k + 1] for k in members])
-            for k, r in zip(members, solve_dartel_maps_jobs(src, tgt, n_steps, **kw)):
-                results[k] = r
-        return results
+    if len(groups) == 1:
+        members = next(iter(groups.values()))
+        src = torch.cat([sheets[2 * k] for k in members])
+        tgt = torch.cat([sheets[2 * k + 1] for k in members])
+        return solve_dartel_maps_jobs(src, tgt, n_steps, **kw)
     current = torch.cuda.current_stream(dev)
     streams = []
+    # (while a schedule is captured, the side streams' work joins the graph as
+    # parallel branches: they wait on the capturing stream and it waits on them)
     for members in groups.values():
         src = torch.cat([sheets[2 * k] for k in members])
         tgt = torch.cat([sheets[2 * k + 1] for k in members])

@@ -70,6 +70,23 @@ def test_graph_schedule_replays_equal_the_eager_schedule(cuda):
     assert entry.replays == 4 and entry.fallbacks == 0
 
 
+def test_graph_schedule_with_uneven_map_sizes(cuda):
+    """Jobs of two mesh sizes and two map sizes: the solves of each map size run as
+    parallel branches of the one graph; replays equal the eager schedule."""
+
+    from cat_surface_gpu import gpu_pipeline as gp
+
+    gp.clear_graph_schedules()
+    jobs = _jobs((51,), subdivisions=4, map_size=(128, 64)) + \
+        _jobs((52,), subdivisions=5, map_size=(256, 128))
+    eager = gp.run_cat_surface_gpu_pipelines(jobs, device=cuda, **KW)
+    gp.run_cat_surface_gpu_pipelines(jobs, device=cuda, graph_schedule=True, **KW)
+    for _ in range(2):
+        replayed = gp.run_cat_surface_gpu_pipelines(jobs, device=cuda, graph_schedule=True, **KW)
+        assert all(r.timings.get("graph_replay") == 1.0 for r in replayed)
+        _same(replayed, eager)
+
+
 def test_graph_schedule_falls_back_when_counts_exceed_the_bounds(cuda):
     """Surfaces at half the size have a quarter of the edge length squared, so
     their heat-kernel iteration counts are four times the recorded ones: the
