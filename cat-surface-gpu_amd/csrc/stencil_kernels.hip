@@ -103,6 +103,134 @@ k_stencil_sphere(const float *__restrict__ v, int n, int normalise, double *__re
     }
 }
 
+// The same over many workgroups: each block reduces a fixed chunk of
+// kSphereChunk vertices (so the partials do not depend on the launch) and one
+// block adds the chunks' partials in order.
+constexpr int kSphereChunk = 1024;
+constexpr int kSphereBlock = 256;
+
+__device__ __forceinline__ void block_reduce3(double &s, double &lo, double &hi, double *ws,
+                                              double *wl, double *wh)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_down(s, off, 64);
+        lo = fmin(lo, __shfl_down(lo, off, 64));
+        hi = fmax(hi, __shfl_down(hi, off, 64));
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        ws[wave] = s;
+        wl[wave] = lo;
+        wh[wave] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = INFINITY, c = -INFINITY;
+        for (int w = 0; w < kSphereBlock / 64; ++w) {
+            a += ws[w];
+            b = fmin(b, wl[w]);
+            c = fmax(c, wh[w]);
+        }
+        s = a;
+        lo = b;
+        hi = c;
+    }
+}
+
+// part[b][0..2] = the coordinate sums of chunk b.
+__global__ void __launch_bounds__(kSphereBlock)
+k_sphere_sums(const float *__restrict__ v, int n, double *__restrict__ part)
+{
+    __shared__ double ws[kSphereBlock / 64], wl[kSphereBlock / 64], wh[kSphereBlock / 64];
+    const int i0 = blockIdx.x * kSphereChunk, i1 = min(n, i0 + kSphereChunk);
+    for (int d = 0; d < 3; ++d) {
+        double s = 0.0, lo = 0.0, hi = 0.0;
+        for (int i = i0 + threadIdx.x; i < i1; i += kSphereBlock)
+            s += (double)v[3 * i + d];
+        block_reduce3(s, lo, hi, ws, wl, wh);
+        if (threadIdx.x == 0)
+            part[3 * blockIdx.x + d] = s;
+        __syncthreads();
+    }
+}
+
+// centre[d] = (sum of the chunks' sums in order) / n.
+__global__ void k_sphere_centre(const double *__restrict__ part, int nb, int n,
+                                double *__restrict__ centre)
+{
+    if (threadIdx.x >= 3 || blockIdx.x != 0)
+        return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b)
+        s += part[3 * b + threadIdx.x];
+    centre[threadIdx.x] = s / (double)n;
+}
+
+// pts of chunk b (centred and normalised when centre is given); part2[b] =
+// (sum of |p|, lo x, y, z, hi x, y, z).
+__global__ void __launch_bounds__(kSphereBlock)
+k_sphere_points(const float *__restrict__ v, int n, const double *__restrict__ centre,
+                double *__restrict__ pts, double *__restrict__ part2)
+{
+    __shared__ double ws[kSphereBlock / 64], wl[kSphereBlock / 64], wh[kSphereBlock / 64];
+    const int i0 = blockIdx.x * kSphereChunk, i1 = min(n, i0 + kSphereChunk);
+    double c[3] = {0.0, 0.0, 0.0};
+    if (centre)
+        for (int d = 0; d < 3; ++d)
+            c[d] = centre[d];
+    double len = 0.0, lo[3] = {INFINITY, INFINITY, INFINITY},
+           hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = i0 + threadIdx.x; i < i1; i += kSphereBlock) {
+        double p[3];
+        for (int d = 0; d < 3; ++d)
+            p[d] = (double)v[3 * i + d] - c[d];
+        if (centre) {
+            const double r = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+            for (int d = 0; d < 3; ++d)
+                p[d] = p[d] / r;
+        }
+        for (int d = 0; d < 3; ++d) {
+            pts[3 * i + d] = p[d];
+            lo[d] = fmin(lo[d], p[d]);
+            hi[d] = fmax(hi[d], p[d]);
+        }
+        len += sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    }
+    for (int d = 0; d < 3; ++d) {
+        double s = d == 0 ? len : 0.0;
+        block_reduce3(s, lo[d], hi[d], ws, wl, wh);
+        if (threadIdx.x == 0) {
+            if (d == 0)
+                part2[7 * blockIdx.x] = s;
+            part2[7 * blockIdx.x + 1 + d] = lo[d];
+            part2[7 * blockIdx.x + 4 + d] = hi[d];
+        }
+        __syncthreads();
+    }
+}
+
+// params = (lo x, y, z, hi x, y, z, 0.975 · mean |p|) from the chunks in order.
+__global__ void k_sphere_params(const double *__restrict__ part2, int nb, int n,
+                                double *__restrict__ params)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    double len = 0.0, lo[3] = {INFINITY, INFINITY, INFINITY},
+           hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = 0; b < nb; ++b) {
+        len += part2[7 * b];
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], part2[7 * b + 1 + d]);
+            hi[d] = fmax(hi[d], part2[7 * b + 4 + d]);
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        params[d] = lo[d];
+        params[3 + d] = hi[d];
+    }
+    params[6] = 0.975 * (len / (double)n);
+}
+
 // out[j] = (lo + hi) + radius · ico[j] (the helper passes the bound sums as the centre).
 __global__ void __launch_bounds__(kBlock)
 k_out_sphere(const double *__restrict__ ico, int m, const double *__restrict__ params,
@@ -166,9 +294,30 @@ int csg_stencil_sphere(const float *v, int n, int normalise, const double *ico, 
 {
     if (n < 1 || m < 1 || !v || !ico || !pts || !out_sphere || !params)
         return fail(kInvalidArgument, "invalid stencil sphere arguments");
-    hipLaunchKernelGGL(k_stencil_sphere, dim3(1), dim3(kSphereThreads), 0, S(stream), v, n,
-                       normalise, pts, params);
-    CSG_LAUNCH_CHECK();
+    const int nb = (n + kSphereChunk - 1) / kSphereChunk;
+    if ((size_t)10 * nb + 3 <= (size_t)3 * m) {
+        // the chunk partials live in out_sphere until k_out_sphere overwrites it
+        double *part = out_sphere, *centre = out_sphere + 3 * (size_t)nb,
+               *part2 = centre + 3;
+        if (normalise) {
+            hipLaunchKernelGGL(k_sphere_sums, dim3(nb), dim3(kSphereBlock), 0, S(stream), v, n,
+                               part);
+            CSG_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_sphere_centre, dim3(1), dim3(64), 0, S(stream), part, nb, n,
+                               centre);
+            CSG_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(k_sphere_points, dim3(nb), dim3(kSphereBlock), 0, S(stream), v, n,
+                           normalise ? centre : nullptr, pts, part2);
+        CSG_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_sphere_params, dim3(1), dim3(64), 0, S(stream), part2, nb, n,
+                           params);
+        CSG_LAUNCH_CHECK();
+    } else {
+        hipLaunchKernelGGL(k_stencil_sphere, dim3(1), dim3(kSphereThreads), 0, S(stream), v, n,
+                           normalise, pts, params);
+        CSG_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_out_sphere, dim3(grid_for(m)), dim3(kBlock), 0, S(stream), ico, m,
                        params, out_sphere);
     CSG_LAUNCH_CHECK();
