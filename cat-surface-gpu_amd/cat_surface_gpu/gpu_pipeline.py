@@ -39,8 +39,31 @@ from .surface_warp import apply_flow_to_stenciled_sphere, solve_dartel_from_surf
 
 @dataclass(frozen=True)
 class GiftiMesh:
+    """Vertices [n, 3] and faces [m, 3]: host arrays, or tensors already on the
+    device (then nothing is copied from the host, and a result's faces are the
+    source sphere's tensor as given)."""
+
     vertices: np.ndarray
     faces: np.ndarray
+
+
+def _host_array(a) -> np.ndarray:
+    """A host copy of a device tensor; host arrays unchanged."""
+
+    if isinstance(a, torch.Tensor):
+        return a.detach().cpu().numpy()
+    return np.asarray(a)
+
+
+def _faces_key(faces) -> tuple:
+    """Identity of a face array's content: a hash of a host array's bytes; for a
+    tensor, its storage, shape and version counter (in-place writes bump it)."""
+
+    if isinstance(faces, torch.Tensor):
+        return ("tensor", str(faces.device), faces.data_ptr(), tuple(faces.shape),
+                faces._version)
+    arr = np.ascontiguousarray(faces)
+    return (arr.shape, arr.dtype.str, hash(arr.tobytes()))
 
 
 @dataclass(frozen=True)
@@ -82,8 +105,8 @@ def write_gifti_mesh(path: str | Path, mesh: GiftiMesh, *,
             point_intent = intents[0]
         if len(intents) >= 2:
             triangle_intent = intents[1]
-    write_gifti_surface(path, mesh.vertices, mesh.faces, point_intent=point_intent,
-                        triangle_intent=triangle_intent)
+    write_gifti_surface(path, _host_array(mesh.vertices), _host_array(mesh.faces),
+                        point_intent=point_intent, triangle_intent=triangle_intent)
 
 
 def _sphere_centroids(spheres: Sequence[torch.Tensor]) -> torch.Tensor:
@@ -157,13 +180,12 @@ def _faces_on_device(faces, dev) -> torch.Tensor:
 
     if isinstance(faces, torch.Tensor):
         return faces.to(device=dev)
-    arr = np.ascontiguousarray(faces)
-    key = (str(dev), arr.shape, arr.dtype.str, hash(arr.tobytes()))
+    key = (str(dev),) + _faces_key(faces)
     hit = _FACES_DEVICE.get(key)
     if hit is None:
         if schedule.capturing():
             raise RuntimeError("face array not seen by an eager run of this schedule")
-        hit = torch.as_tensor(arr, device=dev)
+        hit = torch.as_tensor(np.ascontiguousarray(faces), device=dev)
         _FACES_DEVICE[key] = hit
     return hit
 
@@ -427,7 +449,8 @@ def run_cat_surface_gpu_pipelines(
                     sphere_vertices = sphere_vertices.cpu().numpy()
                 if sphere_path is None:
                     sphere_path = tmp / f"job{index}-{label}.gii"
-                    write_gifti_surface(sphere_path, sphere_vertices, faces)
+                    write_gifti_surface(sphere_path, _host_array(sphere_vertices),
+                                        _host_array(faces))
                 opts = ([] if normalise else ["--no-normalize"]) + [
                     "--threads", str(stencil_threads)]
                 return _external_stencil(builder, sphere_path,
@@ -773,8 +796,7 @@ def clear_graph_schedules() -> None:
 def _graph_key(jobs, kw, dev) -> tuple:
     parts = []
     for job in jobs:
-        meshes = tuple((tuple(np.shape(m.vertices)),
-                        hash(np.ascontiguousarray(np.asarray(m.faces)).tobytes()))
+        meshes = tuple((tuple(np.shape(m.vertices)), _faces_key(m.faces))
                        for m in job.meshes)
         parts.append(meshes + (tuple(job.map_size or (512, 256)),))
     opts = tuple(sorted((k, repr(v)) for k, v in kw.items() if k != "write_output"))
@@ -839,7 +861,7 @@ def run_graph_schedule(jobs: Sequence[PipelineJob], **kw) -> list[CatSurfaceGpuR
     t0 = time.perf_counter()
     for job, bufs in zip(jobs, entry.inputs):
         for mesh, buf in zip(job.meshes, bufs):
-            buf.copy_(torch.as_tensor(mesh.vertices, dtype=torch.float32), non_blocking=False)
+            buf.copy_(torch.as_tensor(mesh.vertices).to(torch.float32), non_blocking=False)
     entry.graph.replay()
     failed = entry.capture.verdict()
     entry.replays += 1
@@ -857,7 +879,9 @@ def run_graph_schedule(jobs: Sequence[PipelineJob], **kw) -> list[CatSurfaceGpuR
     for job, o, (verts, angle, cost) in zip(jobs, entry.outputs, host):
         results.append(CatSurfaceGpuResult(
             vertices=np.ascontiguousarray(verts, dtype=np.float32),
-            faces=np.asarray(job.meshes[1].faces), rotation_angle=angle, rotation_cost=cost,
+            faces=(job.meshes[1].faces if isinstance(job.meshes[1].faces, torch.Tensor)
+                   else np.asarray(job.meshes[1].faces)),
+            rotation_angle=angle, rotation_cost=cost,
             timings={"total_seconds": total, "graph_replay": 1.0},
             output_path=Path(job.output) if job.output is not None else Path("")))
     return _finish_graph_results(jobs, results, write_output)
@@ -874,7 +898,7 @@ def _finish_graph_results(jobs, results, write_output: bool):
 
 def _capture_schedule(jobs, kw, bounds, dev) -> _GraphSchedule:
     started = time.perf_counter()
-    inputs = [[torch.as_tensor(np.asarray(m.vertices), dtype=torch.float32).to(dev).clone()
+    inputs = [[torch.as_tensor(m.vertices).to(device=dev, dtype=torch.float32).clone()
                for m in job.meshes] for job in jobs]
     cap_jobs = [PipelineJob(meshes=tuple(GiftiMesh(buf, m.faces)
                                          for buf, m in zip(bufs, job.meshes)),

@@ -75,6 +75,33 @@ def test_batched_jobs_equal_single_runs(pair_paths, cuda, tmp_path):
         np.testing.assert_array_equal(res.faces, alone.faces)
 
 
+@pytest.mark.parametrize("graph_schedule", [False, True])
+def test_device_resident_meshes_equal_host_arrays(pair_paths, cuda, graph_schedule):
+    """Meshes handed over as device tensors (nothing copied from the host) give
+    the host arrays' results bit for bit; the result's faces are the tensor given."""
+
+    from cat_surface_gpu import GiftiMesh, PipelineJob, run_cat_surface_gpu_pipelines
+    from cat_surface_gpu import gpu_pipeline as gp
+
+    pair, paths, folder = pair_paths
+    host = (GiftiMesh(pair.src_surface, pair.src_faces), GiftiMesh(pair.src_sphere, pair.src_faces),
+            GiftiMesh(pair.tgt_surface, pair.tgt_faces), GiftiMesh(pair.tgt_sphere, pair.tgt_faces))
+    resident = tuple(GiftiMesh(torch.as_tensor(m.vertices, device=cuda),
+                               torch.as_tensor(m.faces, device=cuda)) for m in host)
+    kw = dict(device=cuda, dartel_dtype="float32", steps=2, runs=2, avg=True,
+              write_output=False, rotation_method="adjoint", graph_schedule=graph_schedule)
+    gp.clear_graph_schedules()
+    ref = run_cat_surface_gpu_pipelines([PipelineJob(meshes=host)], **kw)[0]
+    for _ in range(2):  # with graph_schedule: capture, then replay
+        got = run_cat_surface_gpu_pipelines([PipelineJob(meshes=resident)], **kw)[0]
+        np.testing.assert_array_equal(got.vertices, ref.vertices)
+        np.testing.assert_array_equal(np.asarray(got.rotation_angle),
+                                      np.asarray(ref.rotation_angle))
+        assert got.faces is resident[1].faces
+    if graph_schedule:
+        assert got.timings.get("graph_replay") == 1.0
+
+
 def test_adjoint_rotation_pipeline_matches_nelder_mead(pair_paths, cuda):
     """Gradient-refined rotation: no higher SSD than Nelder-Mead's choice (1e-6
     relative) and a registration within the 1e-3 vertex tolerance of it."""
