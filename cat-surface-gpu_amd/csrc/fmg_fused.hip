@@ -39,24 +39,74 @@ namespace {
 // is not cut): after s sweeps every pixel at distance >= s holds the value of
 // the global sweep, so the interior (distance >= H) is exact after H.
 //
-// Only u (two planes) is staged in LDS.  Each thread owns up to kTilePix
+// u is staged in LDS with the two channels of a pixel side by side (one
+// 8- or 16-byte LDS access per neighbour).  Each thread owns up to kTilePix
 // pixels of each colour for the whole call (colour c's pixels of the tile in
 // row-major order, entries tid + k·kTileThreads), with their LDS offsets,
-// right-hand side and 2x2 block (relax_coef) in registers.
+// right-hand side and 2x2 block (relax_coef) in registers.  The kernel is
+// bound by VALU issue, so the index arithmetic around the sweeps is kept
+// short: tile columns are wrapped by one compare when the tiles stay within
+// one period of the grid, and the store loop divides through a reciprocal.
+//
+// With a coarse correction Uc the tile loads u + prolong(Uc).  `colsum` (set
+// by the host for exact 2:1 levels, nx = 2·cnx and ny = 2·cny) evaluates the
+// prolongation's y pass once per (tile row, coarse column) into LDS and the x
+// pass per pixel from those sums, with the taps of every tile row and column
+// resolved once into LDS tables: the same operations in the same order as
+// resize_from, with 3 coarse reads per column sum instead of 18 per pixel.  At
+// 2:1 resize_taps' origin for fine column g is exactly g >> 1, so unwrapped
+// tile column x0 + i reads the column sums of unwrapped coarse columns
+// ((x0 + i) >> 1) - 1 .. + 1, which are wrapped when the coarse values are
+// fetched.
 //
 // HC > 0 fixes the halo (= sweeps) at compile time; HC == 0 takes it from H.
 constexpr int kTilePix = 4;
+// tile pixels one thread loads (tile_fits_threads: at most 2·kTilePix each)
+constexpr int kTileLoad = 2 * kTilePix;
+
+template <typename T>
+struct alignas(2 * sizeof(T)) U2 {
+    T x, y;
+};
+
+// Resolved y taps of one tile row (coarse row offsets premultiplied by cnx) and
+// x taps of one tile column (first column-sum slot m) of the 2:1 prolongation.
+template <typename T>
+struct RowTaps {
+    int ym, y0, yp;
+    T wm, w0, wp;
+};
+
+template <typename T>
+struct ColTaps {
+    int m;
+    T wm, w0, wp;
+};
+
+// LDS bytes of the column-sum prolongation for a tile of ex x rows pixels.
+template <typename T>
+constexpr size_t colsum_lds_bytes(size_t ex, size_t rows)
+{
+    return rows * (ex / 2 + 3) * sizeof(U2<T>) + rows * sizeof(RowTaps<T>) +
+           ex * sizeof(ColTaps<T>);
+}
+
+// ix wrapped into [0, n) for -n <= ix < 2n.
+__device__ __forceinline__ int wrap_once(int ix, int n)
+{
+    return ix < 0 ? ix + n : (ix >= n ? ix - n : ix);
+}
 
 template <typename T, int HC>
 __global__ void __launch_bounds__(kTileThreads)
 k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restrict__ Uin,
              T *__restrict__ Uout, const T *__restrict__ Uc, int cnx, int cny, int nx, int ny,
-             T w00, T w01, T w10, int H_arg, int TX, int TY, int tiles_x, size_t astride,
-             size_t bstride, size_t cstride)
+             T w00, T w01, T w10, int H_arg, int TX, int TY, int tiles_x, int colsum,
+             size_t astride, size_t bstride, size_t cstride)
 {
     const int H = HC > 0 ? HC : H_arg;
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    T *lds = reinterpret_cast<T *>(lds_raw);
+    U2<T> *uu = reinterpret_cast<U2<T> *>(lds_raw);
     const int P = nx * ny, CP = cnx * cny;
     const int job = blockIdx.y;
     A += job * astride;
@@ -71,23 +121,89 @@ k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restri
     const int ylo = max(0, tyi * TY - H), yhi = min(ny, tyi * TY + TY + H);
     const int rows = yhi - ylo;
     const int npx = ex * rows;
-    T *u0 = lds, *u1 = lds + npx;
+    // every unwrapped tile column lies in [-nx + 2, 2nx - 4), so one compare
+    // wraps it, and the coarse columns of its prolongation taps (2:1) lie in
+    // [-cnx, 2cnx)
+    const bool once = H + 2 <= nx && tiles_x * TX + H + 4 <= 2 * nx;
+    auto wrapc = [&](int ix) { return once ? wrap_once(ix, nx) : wrap_x(ix, nx); };
     const float inv_ex = 1.0f / (float)ex;
-    for (int q = threadIdx.x; q < npx; q += blockDim.x) {
-        const DivMod ri = divmod_small(q, ex, inv_ex);
-        const int r = ri.q, i = ri.r;
-        const int gx = wrap_x(x0 + i, nx), gy = ylo + r, p = gy * nx + gx;
-        T ux = Uin[p], uy = Uin[P + p];
-        if (Uc) {
-            const Taps<T> tx = resize_taps<T>(gx, cnx, nx), ty = resize_taps<T>(gy, cny, ny);
-            T rx = resize_at<T>(tx, ty, cnx, cny, [&](int x, int y) { return Uc[y * cnx + x]; });
-            T ry = resize_at<T>(tx, ty, cnx, cny,
-                                [&](int x, int y) { return Uc[CP + y * cnx + x]; });
-            ux = ux + rx;
-            uy = uy + ry;
+    if (!Uc || colsum) {
+        // u of the tile (every load issued before the first use)
+        U2<T> v[kTileLoad];
+#pragma unroll
+        for (int j = 0; j < kTileLoad; ++j) {
+            const int q = threadIdx.x + j * kTileThreads;
+            v[j] = U2<T>{(T)0, (T)0};
+            if (q < npx) {
+                const DivMod ri = divmod_small(q, ex, inv_ex);
+                const int p = (ylo + ri.q) * nx + wrapc(x0 + ri.r);
+                v[j] = U2<T>{Uin[p], Uin[P + p]};
+            }
         }
-        u0[q] = ux;
-        u1[q] = uy;
+        if (Uc) {
+            // y pass of the prolongation: column sums of the unwrapped coarse
+            // columns cbase .. cbase + W - 1 for every tile row, both channels
+            const int cbase = (x0 >> 1) - 1;
+            const int W = ((x0 + ex - 1) >> 1) + 2 - cbase;
+            U2<T> *cs = uu + npx;
+            RowTaps<T> *rt = reinterpret_cast<RowTaps<T> *>(cs + rows * W);
+            ColTaps<T> *ct = reinterpret_cast<ColTaps<T> *>(rt + rows);
+            for (int t = threadIdx.x; t < rows + ex; t += kTileThreads) {
+                if (t < rows) {
+                    const Taps<T> ty = resize_taps<T>(ylo + t, cny, ny);
+                    rt[t] = RowTaps<T>{reflect_y(ty.o - 1, cny) * cnx, reflect_y(ty.o, cny) * cnx,
+                                       reflect_y(ty.o + 1, cny) * cnx, ty.wm, ty.w0, ty.wp};
+                } else {
+                    const int xu = x0 + (t - rows), gx = wrapc(xu);
+                    const Taps<T> tx = resize_taps<T>(gx, cnx, nx);
+                    // unwrapped origin: tx.o plus the whole coarse periods of xu - gx
+                    ct[t - rows] = ColTaps<T>{tx.o + ((xu - gx) >> 1) - 1 - cbase, tx.wm, tx.w0,
+                                              tx.wp};
+                }
+            }
+            __syncthreads();
+            const float inv_w = 1.0f / (float)W;
+            for (int q = threadIdx.x; q < rows * W; q += kTileThreads) {
+                const DivMod rk = divmod_small(q, W, inv_w);
+                const RowTaps<T> r = rt[rk.q];
+                const int cx = once ? wrap_once(cbase + rk.r, cnx) : wrap_x(cbase + rk.r, cnx);
+                cs[q] = U2<T>{r.wm * Uc[r.ym + cx] + r.w0 * Uc[r.y0 + cx] + r.wp * Uc[r.yp + cx],
+                              r.wm * Uc[CP + r.ym + cx] + r.w0 * Uc[CP + r.y0 + cx] +
+                                  r.wp * Uc[CP + r.yp + cx]};
+            }
+            __syncthreads();
+            // x pass per tile pixel, added to u
+#pragma unroll
+            for (int j = 0; j < kTileLoad; ++j) {
+                const int q = threadIdx.x + j * kTileThreads;
+                if (q < npx) {
+                    const DivMod ri = divmod_small(q, ex, inv_ex);
+                    const ColTaps<T> c = ct[ri.r];
+                    const U2<T> *sm = cs + ri.q * W + c.m;
+                    const U2<T> a = sm[0], b = sm[1], d = sm[2];
+                    v[j].x = v[j].x + (c.wm * a.x + c.w0 * b.x + c.wp * d.x);
+                    v[j].y = v[j].y + (c.wm * a.y + c.w0 * b.y + c.wp * d.y);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kTileLoad; ++j) {
+            const int q = threadIdx.x + j * kTileThreads;
+            if (q < npx)
+                uu[q] = v[j];
+        }
+    } else {
+        // u + prolong(Uc) per pixel from the coarse level (any size ratio)
+        for (int q = threadIdx.x; q < npx; q += kTileThreads) {
+            const DivMod ri = divmod_small(q, ex, inv_ex);
+            const int gx = wrapc(x0 + ri.r), gy = ylo + ri.q, p = gy * nx + gx;
+            const Taps<T> tx = resize_taps<T>(gx, cnx, nx), ty = resize_taps<T>(gy, cny, ny);
+            const T rx = resize_at<T>(tx, ty, cnx, cny,
+                                      [&](int x, int y) { return Uc[y * cnx + x]; });
+            const T ry = resize_at<T>(tx, ty, cnx, cny,
+                                      [&](int x, int y) { return Uc[CP + y * cnx + x]; });
+            uu[q] = U2<T>{Uin[p] + rx, Uin[P + p] + ry};
+        }
     }
     // this thread's pixels
     const int hex = ex / 2;  // pixels of one colour per tile row (ex is even)
@@ -117,45 +233,51 @@ k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restri
                 d = min(d, rows - 1 - r);
             if (d < 1)
                 continue;
-            const int ym = reflect_y(gy - 1, ny), yp = reflect_y(gy + 1, ny);
-            const int p = gy * nx + wrap_x(x0 + i, nx);
+            const int q = r * ex + i;
+            const int p = gy * nx + wrapc(x0 + i);
             plim[c][k] = d;
-            pq[c][k] = r * ex + i;
-            pym[c][k] = (ym - ylo) * ex + i;
-            pyp[c][k] = (yp - ylo) * ex + i;
+            pq[c][k] = q;
+            // the y neighbours; the grid's first and last rows reflect onto themselves
+            pym[c][k] = gy == 0 ? q : q - ex;
+            pyp[c][k] = gy == ny - 1 ? q : q + ex;
             pk[c][k] = relax_coef<T>(A[p], A[P + p], A[2 * P + p], w00);
             pbx[c][k] = B[p];
             pby[c][k] = B[P + p];
         }
     }
     __syncthreads();
-    for (int s = 0; s < H; ++s) {
-        const int c = (s & 1) ? 0 : 1;
+    // half-sweep s relaxes colour 1 - (s & 1); the colour is a compile-time
+    // index into the register arrays (a run-time one would put them in
+    // scratch memory), so the sweeps run in (1, 0) pairs
+    auto sweep = [&](auto cc, int s) {
+        constexpr int c = decltype(cc)::value;
 #pragma unroll
         for (int k = 0; k < kTilePix; ++k) {
-            const int q = pq[c][k], qym = pym[c][k], qyp = pyp[c][k];
             if (s < plim[c][k]) {
-                const Stencil<T> n{(T)0,
-                                   (T)0,
-                                   u0[qym] + u0[qyp],
-                                   u0[q - 1] + u0[q + 1],
-                                   u1[qym] + u1[qyp],
-                                   u1[q - 1] + u1[q + 1]};
+                const int q = pq[c][k];
+                const U2<T> a = uu[pym[c][k]], b = uu[pyp[c][k]], l = uu[q - 1], r = uu[q + 1];
+                const Stencil<T> n{(T)0, (T)0, a.x + b.x, l.x + r.x, a.y + b.y, l.y + r.y};
                 const Pair<T> v = relax_solve<T>(pbx[c][k], pby[c][k], n, pk[c][k], w01, w10);
-                u0[q] = v.x;
-                u1[q] = v.y;
+                uu[q] = U2<T>{v.x, v.y};
             }
         }
         __syncthreads();
+    };
+    for (int s = 0; s + 1 < H; s += 2) {
+        sweep(std::integral_constant<int, 1>{}, s);
+        sweep(std::integral_constant<int, 0>{}, s + 1);
     }
-    for (int q = threadIdx.x; q < TX * TY; q += blockDim.x) {
-        const int ly = q / TX, lx = q - ly * TX;
-        const int gx = txi * TX + lx, gy = tyi * TY + ly;
+    if (H & 1)
+        sweep(std::integral_constant<int, 1>{}, H - 1);
+    const float inv_tx = 1.0f / (float)TX;
+    for (int q = threadIdx.x; q < TX * TY; q += kTileThreads) {
+        const DivMod yx = divmod_small(q, TX, inv_tx);
+        const int gx = txi * TX + yx.r, gy = tyi * TY + yx.q;
         if (gx >= nx || gy >= ny)
             continue;
-        const int t = (gy - ylo) * ex + H + lx;
-        Uout[gy * nx + gx] = u0[t];
-        Uout[P + gy * nx + gx] = u1[t];
+        const U2<T> w = uu[(gy - ylo) * ex + H + yx.r];
+        Uout[gy * nx + gx] = w.x;
+        Uout[P + gy * nx + gx] = w.y;
     }
 }
 
@@ -723,6 +845,17 @@ int launch_relax_tile(const T *a, const T *b, const T *uin, T *uout, const T *uc
     if (lds > fmg_lds_budget() || !tile_fits_threads(sweeps, tile))
         return fail(kInvalidArgument, "tiled relaxation of %d x %d tiles with %d sweeps does not "
                                       "fit (%zu bytes of LDS)", tile.x, tile.y, sweeps, lds);
+    // the prolongation's column sums and tap tables in LDS (exact 2:1 levels,
+    // when they fit) after the tile
+    int colsum = 0;
+    if (uc && nx == 2 * cnx && ny == 2 * cny) {
+        const size_t ex = (size_t)tile.x + 2 * sweeps, rows = (size_t)tile.y + 2 * sweeps;
+        const size_t extra = colsum_lds_bytes<T>(ex, rows);
+        if (lds + extra <= fmg_lds_budget()) {
+            lds += extra;
+            colsum = 1;
+        }
+    }
     const int tiles_x = (nx + tile.x - 1) / tile.x, tiles_y = (ny + tile.y - 1) / tile.y;
     dim3 grid(tiles_x * tiles_y, batch);
     // reference bytes: 9 values per pixel per red+black iteration, plus the
@@ -733,11 +866,11 @@ int launch_relax_tile(const T *a, const T *b, const T *uin, T *uout, const T *uc
     if (sweeps == 6)  // nit = 3, the reference default
         hipExtLaunchKernelGGL((k_relax_tile<T, 6>), grid, dim3(kTileThreads), lds, s, timer.start(), timer.stop(), 0, a, b, uin, uout,
                            uc, cnx, cny, nx, ny, (T)w.w00, (T)w.w01, (T)w.w10, sweeps, tile.x,
-                           tile.y, tiles_x, astride, bstride, cstride);
+                           tile.y, tiles_x, colsum, astride, bstride, cstride);
     else
         hipExtLaunchKernelGGL((k_relax_tile<T, 0>), grid, dim3(kTileThreads), lds, s, timer.start(), timer.stop(), 0, a, b, uin, uout,
                            uc, cnx, cny, nx, ny, (T)w.w00, (T)w.w01, (T)w.w10, sweeps, tile.x,
-                           tile.y, tiles_x, astride, bstride, cstride);
+                           tile.y, tiles_x, colsum, astride, bstride, cstride);
     CSG_LAUNCH_CHECK();
     timer.done();
     return kOk;
