@@ -539,7 +539,9 @@ k_sor_xcd_local(const int *__restrict__ verts, SorColours co, const int *__restr
 // iteration, so the one barrier suffices.  Thread t holds row-table
 // entries t, t + T, .. (kSorPatchRows of them, ELL values in registers); the
 // table is sorted by (colour, ring) so that phase t's rows are a prefix of the
-// colour's entries.
+// colour's entries.  A workgroup's own core is exact in its LDS after each
+// iteration (it is what the workgroup stores), so later iterations load only
+// the halo: 7.6 -> 6.6 ms for four 40,962-vertex meshes.
 constexpr int kSorPatchThreads = 1024;
 constexpr int kSorPatchRows = 3;
 
@@ -601,7 +603,7 @@ k_sor_patched(SorPatches pp, int ncolours, const double *__restrict__ ell_val,
     for (long long ph0 = 0; ph0 < total; ph0 += pp.rings, ++step) {
         const double *src = xbuf[step & 1];
         double *dst = xbuf[(step + 1) & 1];
-        for (int j = tid; j < L; j += kSorPatchThreads)
+        for (int j = step == 0 ? tid : core + tid; j < L; j += kSorPatchThreads)
             xs[j] = load_l2(src + lv[j]);
         __syncthreads();
         const int m = total - ph0 < pp.rings ? (int)(total - ph0) : pp.rings;
