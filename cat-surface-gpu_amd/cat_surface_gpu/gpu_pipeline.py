@@ -658,6 +658,9 @@ def run_cat_surface_gpu_pipelines(
                 ensure_tuned([shape], solve_dtype, batch, device=dev)
             shared["autotune_seconds"] = time.perf_counter() - t
         t_dartel = time.perf_counter()
+        # each job's target side is the same surface and stencil in every run:
+        # its fwhm-independent sheet preparation is made once
+        prepared_sides: dict = {}
         for run in range(runs):
             t = time.perf_counter()
             run_fwhm = tuple(float(fwhm[0]) / (3.0 ** (steps * (run + 1) + i))
@@ -685,7 +688,8 @@ def run_cat_surface_gpu_pipelines(
                                      st.avg_src_stencil, st.avg_tgt_stencil, avg_fwhm))
                     owners.append((st, "avg"))
             settle()  # the average solves read the avg-source stencils
-            results = solve_dartel_from_surfaces_jobs(requests, **solve_kw)
+            results = solve_dartel_from_surfaces_jobs(requests, prepared_sides=prepared_sides,
+                                                      **solve_kw)
             for (st, role), res in zip(owners, results):
                 if role == "avg":
                     st.avg_flow = res.flow

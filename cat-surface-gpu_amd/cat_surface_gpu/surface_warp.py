@@ -316,7 +316,9 @@ def solve_dartel_from_surfaces_jobs(requests, *, n_steps: int = 2, loop: int = 6
                                     device: str | torch.device = "auto",
                                     dtype: torch.dtype = torch.float64,
                                     assume_initial_nonzero: bool = False,
-                                    cuda_graph: bool = False) -> list[DartelSolveResult]:
+                                    cuda_graph: bool = False,
+                                    prepared_sides: dict | None = None
+                                    ) -> list[DartelSolveResult]:
     """solve_dartel_from_surfaces for several independent registrations at once.
 
     ``requests`` holds (source_vertices, target_vertices, source_stencil,
@@ -324,33 +326,62 @@ def solve_dartel_from_surfaces_jobs(requests, *, n_steps: int = 2, loop: int = 6
     (sides of several all-type-5 steps together in shared launches, any other
     step list side by side and step by step, surface_warp.py:427-448) and all
     solves run as one batched solve (solve_dartel_maps_jobs).
+
+    ``prepared_sides`` (a dict the caller keeps across calls) holds the
+    fwhm-independent type-5 preparation (resampled geometry, its fwhm-50 heat
+    parameters and smoothed geometry) of each side, keyed by the identity of
+    its vertex tensor and stencil: a side passed again with the same two
+    objects (a registration's target in every DARTEL run) reuses it.  The
+    preparation is a pure function of those two inputs, so the sheets are the
+    same bits.
     """
 
     from .surface_stencil import SurfaceStencil, prepare_type5, type5_sheets_many
 
     types = _check_curvtypes(curvtypes, n_steps)
     dev = resolve_device(device)
-    stencils, geoms, fws = [], [], []
+    stencils, geoms, fws, origins = [], [], [], []
     for sv, tv, ss, ts, fw in requests:
         if len(fw) < n_steps:
             raise ValueError("each request needs at least n_steps fwhm values")
         for st, vert in ((ss, sv), (ts, tv)):
+            origins.append((st, vert))
             st = st.to(dev) if isinstance(st, SurfaceStencil) else st
-            v = torch.as_tensor(vert, device=dev).to(torch.float32).contiguous()
             stencils.append(st)
-            geoms.append(st.resample_vertices(v))
+            geoms.append(None)
             fws.append(tuple(float(f) for f in fw[:n_steps]))
+    all_type5 = n_steps > 1 and all(t == 5 for t in types)
+    kept: list[tuple | None] = [None] * len(stencils)
+    if all_type5 and prepared_sides is not None:
+        for i, (st, vert) in enumerate(origins):
+            entry = prepared_sides.get((id(st), id(vert)))
+            if entry is not None and entry[0] is st and entry[1] is vert:
+                kept[i] = entry[2:]
+    for i, (st, vert) in enumerate(origins):
+        if kept[i] is None:
+            v = torch.as_tensor(vert, device=dev).to(torch.float32).contiguous()
+            geoms[i] = stencils[i].resample_vertices(v)
     sheets: list[torch.Tensor | None] = [None] * len(stencils)
-    if n_steps > 1 and all(t == 5 for t in types):
+    if all_type5:
         # all sides that share the fwhm list are computed together
         by_fwhm: dict[tuple, list[int]] = {}
         for i, f in enumerate(fws):
             by_fwhm.setdefault(f, []).append(i)
         # the fwhm-50 geometry smoothing does not depend on the fwhms: every side's
-        # in the same launches
-        g32, heats, smoothed = prepare_type5(stencils, geoms)
+        # in the same launches (sides prepared by an earlier call are reused)
+        g32: list = [k[0] if k is not None else None for k in kept]
+        heats: list = [k[1] if k is not None else None for k in kept]
+        smoothed: list = [k[2] if k is not None else None for k in kept]
+        todo = [i for i in range(len(stencils)) if kept[i] is None]
+        if todo:
+            pg, ph, ps = prepare_type5([stencils[i] for i in todo], [geoms[i] for i in todo])
+            for k, i in enumerate(todo):
+                g32[i], heats[i], smoothed[i] = pg[k], ph[k], ps[k]
+                if prepared_sides is not None:
+                    st, vert = origins[i]
+                    prepared_sides[(id(st), id(vert))] = (st, vert, pg[k], ph[k], ps[k])
         for f, members in by_fwhm.items():
-            res = type5_sheets_many([stencils[i] for i in members], [geoms[i] for i in members],
+            res = type5_sheets_many([stencils[i] for i in members], [g32[i] for i in members],
                                     f, dtype, prepared=([g32[i] for i in members],
                                                         [heats[i] for i in members],
                                                         [smoothed[i] for i in members]))

@@ -118,12 +118,22 @@ int set_relax_tile_shape(int nx, int ny, int elem, int tx, int ty);
 constexpr int kCoarseThreads = 1024;
 
 // Levels c0 .. L-1 of the pyramid, relaxed by one workgroup per job in LDS.
+// What does not change during one multigrid solve is kept in global memory
+// between its coarse launches: `meta` (coarse_meta_bytes) holds the level
+// table and the resolved resize taps, apyr[l] (l > c0) level l's A, 3·nx·ny
+// values per job.  The first launch of a solve builds and stores them, the
+// later ones load them.
 template <typename T>
 struct CoarseLevels {
     int L, c0;
     int nx[kFmgMaxLevels], ny[kFmgMaxLevels];
     T w00[kFmgMaxLevels], w01[kFmgMaxLevels], w10[kFmgMaxLevels];
+    unsigned char *meta;
+    T *apyr[kFmgMaxLevels];
 };
+// Bytes of CoarseLevels::meta for a pyramid of L levels (any first coarse level).
+template <typename T>
+size_t coarse_meta_bytes(const int *nx, const int *ny, int L);
 
 // Largest dynamic LDS allocation a block may use on the current device (the
 // kernels' limits are raised to it once per process).
@@ -149,10 +159,12 @@ int launch_restrict_residual(const T *a, const T *b, const T *u, MembraneWeights
 // Levels c0.. of the FMG in one workgroup per job: init != 0 runs the coarse
 // initialisation (coarse solve, then prolong + `cycles` cycles per level down
 // to c0) from b = B0[c0]; otherwise `cycles` cycles from level c0 with b = B[c0]
-// and u as the initial guess.  u (level c0) is overwritten.
+// and u as the initial guess.  u (level c0) is overwritten.  build != 0 for
+// the first launch of a solve (required with init), which stores what the
+// solve's later launches load (CoarseLevels::meta, apyr).
 template <typename T>
 int launch_fmg_coarse(const CoarseLevels<T> &lv, const T *a, const T *b, T *u, size_t astride,
-                      size_t bstride, double id, int nit, int cycles, int init, int batch,
-                      size_t lds_bytes, hipStream_t s);
+                      size_t bstride, double id, int nit, int cycles, int init, int build,
+                      int batch, size_t lds_bytes, hipStream_t s);
 
 }  // namespace csg

@@ -95,6 +95,7 @@ struct FmgPlan {
     int nx[kMaxLevels], ny[kMaxLevels];
     T *A[kMaxLevels], *B0[kMaxLevels], *B[kMaxLevels], *U[kMaxLevels], *R[kMaxLevels];
     T *U2[kMaxLevels];  // second correction buffer of the fused schedule (aliases R)
+    unsigned char *meta;  // the fused schedule's coarse state (coarse_meta_bytes)
     size_t P(int l) const { return (size_t)nx[l] * ny[l]; }
 };
 static_assert(kMaxLevels == kFmgMaxLevels, "one level limit");
@@ -116,6 +117,7 @@ static void carve_fmg(Carver &c, int nx, int ny, int batch, FmgPlan<T> &plan)
         plan.B[l] = c.template take<T>(2 * P * batch);
         plan.U[l] = c.template take<T>(2 * P * batch);
     }
+    plan.meta = c.template take<unsigned char>(coarse_meta_bytes<T>(plan.nx, plan.ny, plan.L));
 }
 
 template <typename T>
@@ -192,6 +194,7 @@ struct FusedFmg {
     int cur[kMaxLevels] = {};
     CoarseLevels<T> lv;
     size_t lds = 0;
+    bool built = false;  // the coarse state of this solve is stored
 
     T *ubuf(int l, int which) { return which ? p.U2[l] : p.U[l]; }
     T *u(int l) { return ubuf(l, cur[l]); }
@@ -212,7 +215,10 @@ struct FusedFmg {
             lv.w00[l] = (T)w.w00;
             lv.w01[l] = (T)w.w01;
             lv.w10[l] = (T)w.w10;
+            // levels below c0 are made by the coarse kernel, which keeps A there
+            lv.apyr[l] = l > c0 ? p.A[l] : nullptr;
         }
+        lv.meta = p.meta;
         lds = coarse_lds_bytes<T>(p.nx, p.ny, c0, p.L);
     }
     // relax_membrane at level l, optionally after u[l] += prolong(u[l+1])
@@ -238,8 +244,10 @@ struct FusedFmg {
     int coarse(int init, int ncycles)
     {
         size_t P = p.P(c0);
+        const int build = built ? 0 : 1;
+        built = true;
         return launch_fmg_coarse<T>(lv, p.A[c0], init ? p.B0[c0] : p.B[c0], u(c0), 3 * P, 2 * P,
-                                    params[4], nit, ncycles, init, batch, lds, s);
+                                    params[4], nit, ncycles, init, build, batch, lds, s);
     }
     int cycle(int start)
     {

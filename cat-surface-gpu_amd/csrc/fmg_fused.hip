@@ -359,7 +359,15 @@ struct LevelTable {
     unsigned down_x[kFmgMaxLevels], down_y[kFmgMaxLevels], up_x[kFmgMaxLevels],
         up_y[kFmgMaxLevels];
     T w00[kFmgMaxLevels], w01[kFmgMaxLevels], w10[kFmgMaxLevels];
+    unsigned tap0, tap1;  // the tap entries' range
 };
+
+// CoarseLevels::meta: the table, then the tap entries from a 256-byte boundary.
+template <typename T>
+constexpr size_t meta_taps_offset()
+{
+    return (sizeof(LevelTable<T>) + 255) / 256 * 256;
+}
 
 enum CoarseOp : int { kRelax, kDown, kUpAdd, kProlong, kPyramidA, kPyramidB };
 
@@ -542,11 +550,15 @@ __device__ __forceinline__ void run_op(int op, int l, const LevelTable<T> &t, in
                  t.off[o], t.roff, tx, ty, nit);
 }
 
+// `build`: the first launch of a multigrid solve, which makes the level table,
+// the resolved taps and the A pyramid and stores them in lv.meta / lv.apyr;
+// the solve's later launches load them (the same values: A does not change
+// during a solve).
 template <typename T>
 __global__ void __launch_bounds__(kCoarseThreads)
 k_fmg_coarse(CoarseLevels<T> lv, const T *__restrict__ A, const T *__restrict__ B,
              T *__restrict__ U, size_t astride, size_t bstride, double id, int nit, int cycles,
-             int init)
+             int init, int build)
 {
     __shared__ LevelTable<T> tab;
     T *base = reinterpret_cast<T *>(g_lds);
@@ -555,7 +567,24 @@ k_fmg_coarse(CoarseLevels<T> lv, const T *__restrict__ A, const T *__restrict__ 
     B += job * bstride;
     U += job * bstride;
     const int c0 = lv.c0, L = lv.L, last = L - 1;
-    if (threadIdx.x == 0) {
+    TapIdx<T> *taps = reinterpret_cast<TapIdx<T> *>(g_lds);
+    const TapIdx<T> *gtaps = reinterpret_cast<const TapIdx<T> *>(lv.meta + meta_taps_offset<T>());
+    if (!build) {
+        const unsigned *src = reinterpret_cast<const unsigned *>(lv.meta);
+        unsigned *dst = reinterpret_cast<unsigned *>(&tab);
+        for (int i = threadIdx.x; i < (int)(sizeof(LevelTable<T>) / 4); i += kCoarseThreads)
+            dst[i] = src[i];
+        __syncthreads();
+        for (int t = threadIdx.x; t < (int)(tab.tap1 - tab.tap0); t += kCoarseThreads)
+            taps[tab.tap0 + t] = gtaps[t];
+        for (int l = c0 + 1; l <= last; ++l) {
+            const int P = tab.nx[l] * tab.ny[l];
+            const T *ga = lv.apyr[l] + (size_t)job * 3 * P;
+            T *la = base + tab.off[l];
+            for (int p = threadIdx.x; p < 3 * P; p += kCoarseThreads)
+                la[p] = ga[p];
+        }
+    } else if (threadIdx.x == 0) {
         // constant indices only: a run-time index into the argument would
         // copy it to scratch memory
         unsigned off = 0;
@@ -575,6 +604,7 @@ k_fmg_coarse(CoarseLevels<T> lv, const T *__restrict__ A, const T *__restrict__ 
         off += 2u * (unsigned)(lv.nx[c0] * lv.ny[c0]);
         // tap entries start at the first TapIdx-aligned position after the planes
         unsigned e = (unsigned)((off * sizeof(T) + sizeof(TapIdx<T>) - 1) / sizeof(TapIdx<T>));
+        tab.tap0 = e;
 #pragma unroll
         for (int l = 0; l < kFmgMaxLevels - 1; ++l) {
             if (l >= c0 && l < L - 1) {
@@ -588,11 +618,11 @@ k_fmg_coarse(CoarseLevels<T> lv, const T *__restrict__ A, const T *__restrict__ 
                 e += (unsigned)lv.ny[l];
             }
         }
+        tab.tap1 = e;
     }
     __syncthreads();
     // resolve every resize tap of the sub-pyramid once
-    TapIdx<T> *taps = reinterpret_cast<TapIdx<T> *>(g_lds);
-    for (int l = c0; l < last; ++l) {
+    for (int l = c0; l < last && build; ++l) {
         const int fx = tab.nx[l], fy = tab.ny[l], cx = tab.nx[l + 1], cy = tab.ny[l + 1];
         for (int t = threadIdx.x; t < cx + cy + fx + fy; t += kCoarseThreads) {
             if (t < cx)
@@ -622,9 +652,29 @@ k_fmg_coarse(CoarseLevels<T> lv, const T *__restrict__ A, const T *__restrict__ 
     __syncthreads();
     // pyramids of A (and, for the coarse initialisation, of the original b)
     for (int l = c0 + 1; l <= last; ++l) {
-        run_op<T>(kPyramidA, l, tab, nit);
+        if (build)
+            run_op<T>(kPyramidA, l, tab, nit);
         if (init)
             run_op<T>(kPyramidB, l, tab, nit);
+    }
+    if (build) {
+        // kept for the solve's later launches
+        for (int l = c0 + 1; l <= last; ++l) {
+            const int P = tab.nx[l] * tab.ny[l];
+            T *ga = lv.apyr[l] + (size_t)job * 3 * P;
+            const T *la = base + tab.off[l];
+            for (int p = threadIdx.x; p < 3 * P; p += kCoarseThreads)
+                ga[p] = la[p];
+        }
+        if (job == 0) {
+            const unsigned *src = reinterpret_cast<const unsigned *>(&tab);
+            unsigned *dst = reinterpret_cast<unsigned *>(lv.meta);
+            for (int i = threadIdx.x; i < (int)(sizeof(LevelTable<T>) / 4); i += kCoarseThreads)
+                dst[i] = src[i];
+            TapIdx<T> *out = reinterpret_cast<TapIdx<T> *>(lv.meta + meta_taps_offset<T>());
+            for (int t = threadIdx.x; t < (int)(tab.tap1 - tab.tap0); t += kCoarseThreads)
+                out[t] = taps[tab.tap0 + t];
+        }
     }
     if (init) {
         // dartel_grid.py:1301-1326: direct 2x2 solve against the original b of
@@ -893,16 +943,27 @@ int launch_restrict_residual(const T *a, const T *b, const T *u, MembraneWeights
 }
 
 template <typename T>
+size_t coarse_meta_bytes(const int *nx, const int *ny, int L)
+{
+    size_t entries = 0;
+    for (int l = 0; l + 1 < L; ++l)
+        entries += (size_t)nx[l + 1] + ny[l + 1] + nx[l] + ny[l];
+    return meta_taps_offset<T>() + entries * sizeof(TapIdx<T>);
+}
+
+template <typename T>
 int launch_fmg_coarse(const CoarseLevels<T> &lv, const T *a, const T *b, T *u, size_t astride,
-                      size_t bstride, double id, int nit, int cycles, int init, int batch,
-                      size_t lds_bytes, hipStream_t s)
+                      size_t bstride, double id, int nit, int cycles, int init, int build,
+                      int batch, size_t lds_bytes, hipStream_t s)
 {
     if (lds_bytes > fmg_lds_budget())
         return fail(kInvalidArgument, "coarse multigrid levels need %zu bytes of LDS", lds_bytes);
+    if (!lv.meta || (init && !build))
+        return fail(kInvalidArgument, "coarse multigrid launch without its kept state");
     TimedLaunch timer(kTimedFmgCoarse, s,
                       coarse_reference_values(lv, nit, cycles, init) * sizeof(T) * batch);
     hipExtLaunchKernelGGL(k_fmg_coarse<T>, dim3(batch), dim3(kCoarseThreads), lds_bytes, s, timer.start(), timer.stop(), 0, lv, a,
-                       b, u, astride, bstride, id, nit, cycles, init);
+                       b, u, astride, bstride, id, nit, cycles, init, build);
     CSG_LAUNCH_CHECK();
     timer.done();
     return kOk;
@@ -917,8 +978,9 @@ int launch_fmg_coarse(const CoarseLevels<T> &lv, const T *a, const T *b, T *u, s
     template int launch_restrict_residual<T>(const T *, const T *, const T *, MembraneWeights,   \
                                              int, int, int, int, int, size_t, size_t, size_t,    \
                                              T *, T *, hipStream_t);                             \
+    template size_t coarse_meta_bytes<T>(const int *, const int *, int);                        \
     template int launch_fmg_coarse<T>(const CoarseLevels<T> &, const T *, const T *, T *,       \
-                                      size_t, size_t, double, int, int, int, int, size_t,        \
+                                      size_t, size_t, double, int, int, int, int, int, size_t,   \
                                       hipStream_t);
 
 CSG_INSTANTIATE_FUSED(float)
