@@ -359,6 +359,9 @@ struct LevelTable {
     unsigned down_x[kFmgMaxLevels], down_y[kFmgMaxLevels], up_x[kFmgMaxLevels],
         up_y[kFmgMaxLevels];
     T w00[kFmgMaxLevels], w01[kFmgMaxLevels], w10[kFmgMaxLevels];
+    // reciprocals for the float divmods of each level: 1 / nx, 1 / (nx·ny) and
+    // 1 / ceil(nx / 2) (the same expressions the steps evaluated themselves)
+    float inv_nx[kFmgMaxLevels], inv_p[kFmgMaxLevels], inv_half[kFmgMaxLevels];
     unsigned tap0, tap1;  // the tap entries' range
 };
 
@@ -376,10 +379,10 @@ enum CoarseOp : int { kRelax, kDown, kUpAdd, kProlong, kPyramidA, kPyramidB };
 template <typename T>
 __device__ __forceinline__ void lds_resize(const T *in, int snx, int sny, T *out, int tnx,
                                            int tny, int channels, bool accumulate,
-                                           const TapIdx<T> *tabx, const TapIdx<T> *taby)
+                                           const TapIdx<T> *tabx, const TapIdx<T> *taby,
+                                           float inv_tp, float inv_tnx)
 {
     const int SP = snx * sny, TP = tnx * tny;
-    const float inv_tp = 1.0f / (float)TP, inv_tnx = 1.0f / (float)tnx;
     for (int q = threadIdx.x; q < channels * TP; q += kCoarseThreads) {
         const DivMod cp = divmod_small(q, TP, inv_tp);
         const DivMod yx = divmod_small(cp.r, tnx, inv_tnx);
@@ -393,11 +396,13 @@ __device__ __forceinline__ void lds_resize(const T *in, int snx, int sny, T *out
 
 // One step of the schedule on level (nx, ny) at LDS offset `off`; the other
 // level (onx, ony, ooff) is l+1 for kDown / kUpAdd / kProlong and l-1 for the
-// pyramid steps.  Ends with a workgroup barrier.
+// pyramid steps.  inv_* are the two levels' reciprocals from the table.  Ends
+// with a workgroup barrier.
 template <typename T>
 __device__ __forceinline__ void coarse_op(int op, int nx, int ny, unsigned off, T w00, T w01, T w10,
                                        int onx, int ony, unsigned ooff, unsigned roff,
-                                       unsigned tx_off, unsigned ty_off, int nit)
+                                       unsigned tx_off, unsigned ty_off, int nit, float inv_nx,
+                                       float inv_p, float inv_half, float oinv_nx, float oinv_p)
 {
     T *base = reinterpret_cast<T *>(g_lds);
     const TapIdx<T> *tabx = reinterpret_cast<const TapIdx<T> *>(g_lds) + tx_off;
@@ -408,7 +413,6 @@ __device__ __forceinline__ void coarse_op(int op, int nx, int ny, unsigned off, 
     if (op == kRelax) {
         // relax_membrane: 2·nit half-sweeps, two-phase order for odd widths
         const int half = (nx + 1) >> 1;
-        const float inv_half = 1.0f / (float)half;
         const bool odd = (nx & 1) && nx > 1;
         // one first-phase pixel per thread and colour: its stencil offsets, 2x2
         // block and right-hand side stay in registers for the whole call (A and
@@ -422,22 +426,34 @@ __device__ __forceinline__ void coarse_op(int op, int nx, int ny, unsigned off, 
         if (in_regs && (int)threadIdx.x < half * ny) {
             const DivMod yk = divmod_small((int)threadIdx.x, half, inv_half);
             const int y = yk.q, row = y * nx;
+            // both colours without branches, so that their loads and divisions
+            // interleave: a pixel that does not exist (rok false) is replaced by
+            // column 2·r of the row, which does, and is never updated
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-                const int x = 2 * yk.r + ((c + y) & 1);
-                rok[c] = x < nx && !(odd && x == nx - 1);
-                if (!rok[c])
-                    continue;
+                const int xc = 2 * yk.r + ((c + y) & 1);
+                rok[c] = xc < nx && !(odd && xc == nx - 1);
+                const int x = xc < nx ? xc : 2 * yk.r;
                 const int p = row + x;
                 rp[c] = p;
                 rxm[c] = row + (x == 0 ? nx - 1 : x - 1);
                 rxp[c] = row + (x == nx - 1 ? 0 : x + 1);
                 rym[c] = (y == 0 ? 0 : y - 1) * nx + x;
                 ryp[c] = (y == ny - 1 ? y : y + 1) * nx + x;
-                rk[c] = relax_coef<T>(a[p], a[P + p], a[2 * P + p], w00);
+            }
+            T a0[2], a1[2], a2[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int p = rp[c];
+                a0[c] = a[p];
+                a1[c] = a[P + p];
+                a2[c] = a[2 * P + p];
                 rbx[c] = b[p];
                 rby[c] = b[P + p];
             }
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+                rk[c] = relax_coef<T>(a0[c], a1[c], a2[c], w00);
         }
         // the half-sweeps in colour pairs (1, 0), so that every index into the
         // register arrays is a compile-time constant (a run-time index would
@@ -498,7 +514,6 @@ __device__ __forceinline__ void coarse_op(int op, int nx, int ny, unsigned off, 
     } else if (op == kDown) {
         // r = b - (L + A) u; b[l+1] = restrict(r); u[l+1] = 0.  dartel_grid.py:1327-1368
         T *r = base + roff;
-        const float inv_nx = 1.0f / (float)nx;
         for (int p = threadIdx.x; p < P; p += kCoarseThreads) {
             const DivMod yx = divmod_small(p, nx, inv_nx);
             const int y = yx.q, x = yx.r;
@@ -508,19 +523,19 @@ __device__ __forceinline__ void coarse_op(int op, int nx, int ny, unsigned off, 
             r[P + p] = b[P + p] - m.y;
         }
         __syncthreads();
-        lds_resize<T>(r, nx, ny, ob, onx, ony, 2, false, tabx, taby);
+        lds_resize<T>(r, nx, ny, ob, onx, ony, 2, false, tabx, taby, oinv_p, oinv_nx);
         for (int p = threadIdx.x; p < 2 * OP; p += kCoarseThreads)
             ou[p] = (T)0;
         __syncthreads();
     } else if (op == kUpAdd || op == kProlong) {
-        lds_resize<T>(ou, onx, ony, u, nx, ny, 2, op == kUpAdd, tabx, taby);
+        lds_resize<T>(ou, onx, ony, u, nx, ny, 2, op == kUpAdd, tabx, taby, inv_p, inv_nx);
         __syncthreads();
     } else {
         // pyramid: A (or the original b) of this level from the finer one
         if (op == kPyramidA)
-            lds_resize<T>(oa, onx, ony, a, nx, ny, 3, false, tabx, taby);
+            lds_resize<T>(oa, onx, ony, a, nx, ny, 3, false, tabx, taby, inv_p, inv_nx);
         else
-            lds_resize<T>(ob, onx, ony, b, nx, ny, 2, false, tabx, taby);
+            lds_resize<T>(ob, onx, ony, b, nx, ny, 2, false, tabx, taby, inv_p, inv_nx);
         __syncthreads();
     }
 }
@@ -547,7 +562,8 @@ __device__ __forceinline__ void run_op(int op, int l, const LevelTable<T> &t, in
         ty = t.down_y[l - 1];
     }
     coarse_op<T>(op, t.nx[l], t.ny[l], t.off[l], t.w00[l], t.w01[l], t.w10[l], t.nx[o], t.ny[o],
-                 t.off[o], t.roff, tx, ty, nit);
+                 t.off[o], t.roff, tx, ty, nit, t.inv_nx[l], t.inv_p[l], t.inv_half[l],
+                 t.inv_nx[o], t.inv_p[o]);
 }
 
 // `build`: the first launch of a multigrid solve, which makes the level table,
@@ -596,6 +612,9 @@ k_fmg_coarse(CoarseLevels<T> lv, const T *__restrict__ A, const T *__restrict__ 
                 tab.w00[l] = lv.w00[l];
                 tab.w01[l] = lv.w01[l];
                 tab.w10[l] = lv.w10[l];
+                tab.inv_nx[l] = 1.0f / (float)lv.nx[l];
+                tab.inv_p[l] = 1.0f / (float)(lv.nx[l] * lv.ny[l]);
+                tab.inv_half[l] = 1.0f / (float)((lv.nx[l] + 1) >> 1);
                 tab.off[l] = off;
                 off += 7u * (unsigned)(lv.nx[l] * lv.ny[l]);
             }
