@@ -127,6 +127,52 @@ k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restri
     const bool once = H + 2 <= nx && tiles_x * TX + H + 4 <= 2 * nx;
     auto wrapc = [&](int ix) { return once ? wrap_once(ix, nx) : wrap_x(ix, nx); };
     const float inv_ex = 1.0f / (float)ex;
+    // this thread's pixels, and their A and b loads issued before the tile's
+    // (both sets of global loads in flight at once; the relaxation
+    // coefficients are formed after the tile is in LDS)
+    const int hex = ex / 2;  // pixels of one colour per tile row (ex is even)
+    const float inv_hex = 1.0f / (float)hex;
+    int pq[2][kTilePix], pym[2][kTilePix], pyp[2][kTilePix], plim[2][kTilePix];
+    RelaxCoef<T> pk[2][kTilePix];
+    T pa0[2][kTilePix], pa1[2][kTilePix], pa2[2][kTilePix];
+    T pbx[2][kTilePix], pby[2][kTilePix];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+#pragma unroll
+        for (int k = 0; k < kTilePix; ++k) {
+            const int j = threadIdx.x + k * kTileThreads;
+            plim[c][k] = 0;
+            pq[c][k] = pym[c][k] = pyp[c][k] = 0;
+            pa0[c][k] = pa1[c][k] = (T)1;
+            pa2[c][k] = (T)0;
+            pbx[c][k] = pby[c][k] = (T)0;
+            if (j >= hex * rows)
+                continue;
+            const DivMod rk = divmod_small(j, hex, inv_hex);
+            const int r = rk.q, gy = ylo + r;
+            // the column of this row with (x0 + i + gy) & 1 == c
+            const int i = 2 * rk.r + ((c - x0 - gy) & 1);
+            int d = min(i, ex - 1 - i);
+            if (ylo > 0)
+                d = min(d, r);
+            if (yhi < ny)
+                d = min(d, rows - 1 - r);
+            if (d < 1)
+                continue;
+            const int q = r * ex + i;
+            const int p = gy * nx + wrapc(x0 + i);
+            plim[c][k] = d;
+            pq[c][k] = q;
+            // the y neighbours; the grid's first and last rows reflect onto themselves
+            pym[c][k] = gy == 0 ? q : q - ex;
+            pyp[c][k] = gy == ny - 1 ? q : q + ex;
+            pa0[c][k] = A[p];
+            pa1[c][k] = A[P + p];
+            pa2[c][k] = A[2 * P + p];
+            pbx[c][k] = B[p];
+            pby[c][k] = B[P + p];
+        }
+    }
     if (!Uc || colsum) {
         // u of the tile (every load issued before the first use)
         U2<T> v[kTileLoad];
@@ -205,46 +251,12 @@ k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restri
             uu[q] = U2<T>{Uin[p] + rx, Uin[P + p] + ry};
         }
     }
-    // this thread's pixels
-    const int hex = ex / 2;  // pixels of one colour per tile row (ex is even)
-    const float inv_hex = 1.0f / (float)hex;
-    int pq[2][kTilePix], pym[2][kTilePix], pyp[2][kTilePix], plim[2][kTilePix];
-    RelaxCoef<T> pk[2][kTilePix];
-    T pbx[2][kTilePix], pby[2][kTilePix];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < 2; ++c)
 #pragma unroll
-        for (int k = 0; k < kTilePix; ++k) {
-            const int j = threadIdx.x + k * kTileThreads;
-            plim[c][k] = 0;
-            pq[c][k] = pym[c][k] = pyp[c][k] = 0;
-            pk[c][k] = RelaxCoef<T>{(T)1, (T)1, (T)0, (T)1};
-            pbx[c][k] = pby[c][k] = (T)0;
-            if (j >= hex * rows)
-                continue;
-            const DivMod rk = divmod_small(j, hex, inv_hex);
-            const int r = rk.q, gy = ylo + r;
-            // the column of this row with (x0 + i + gy) & 1 == c
-            const int i = 2 * rk.r + ((c - x0 - gy) & 1);
-            int d = min(i, ex - 1 - i);
-            if (ylo > 0)
-                d = min(d, r);
-            if (yhi < ny)
-                d = min(d, rows - 1 - r);
-            if (d < 1)
-                continue;
-            const int q = r * ex + i;
-            const int p = gy * nx + wrapc(x0 + i);
-            plim[c][k] = d;
-            pq[c][k] = q;
-            // the y neighbours; the grid's first and last rows reflect onto themselves
-            pym[c][k] = gy == 0 ? q : q - ex;
-            pyp[c][k] = gy == ny - 1 ? q : q + ex;
-            pk[c][k] = relax_coef<T>(A[p], A[P + p], A[2 * P + p], w00);
-            pbx[c][k] = B[p];
-            pby[c][k] = B[P + p];
-        }
-    }
+        for (int k = 0; k < kTilePix; ++k)
+            pk[c][k] = plim[c][k] > 0 ? relax_coef<T>(pa0[c][k], pa1[c][k], pa2[c][k], w00)
+                                      : RelaxCoef<T>{(T)1, (T)1, (T)0, (T)1};
     __syncthreads();
     // half-sweep s relaxes colour 1 - (s & 1); the colour is a compile-time
     // index into the register arrays (a run-time one would put them in
