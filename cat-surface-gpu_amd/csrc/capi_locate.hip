@@ -321,7 +321,9 @@ int csg_closest_triangles(const double *q, int nq, const double *verts, const in
 
 size_t csg_closest_triangles_workspace_bytes(int nf)
 {
-    return nf < 1 ? 0 : 256 + align256(sizeof(double4) * (size_t)nf);
+    // far count, face spheres, face-major corner copies
+    return nf < 1 ? 0 : 256 + align256(sizeof(double4) * (size_t)nf) +
+                            align256(9 * sizeof(double) * (size_t)nf);
 }
 
 int csg_closest_triangles_pruned(const double *q, int nq, const double *verts, const int *faces,
@@ -337,8 +339,9 @@ int csg_closest_triangles_pruned(const double *q, int nq, const double *verts, c
     hipStream_t s = S(stream);
     CSG_TRY(clear_int(far_dev, s));
     double4 *sph = (double4 *)((char *)ws + 256);
+    double *tri = (double *)((char *)sph + align256(sizeof(double4) * (size_t)nf));
     CSG_TRY(launch_closest_triangles_pruned(q, nq, verts, faces, nf, cell_off, cell_faces, grid,
-                                            reach * reach, sph, face_out, idx_out, w_out,
+                                            reach * reach, sph, tri, face_out, idx_out, w_out,
                                             far_dev, s));
     if (far_host) {
         CSG_HIP_CHECK(hipMemcpyAsync(far_host, far_dev, sizeof(int), hipMemcpyDeviceToHost, s));

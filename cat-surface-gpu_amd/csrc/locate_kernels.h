@@ -78,12 +78,12 @@ int launch_closest_triangles(const double *q, int nq, const double *verts, const
                              const int *cell_off, const int *cell_faces, int g, double reach2,
                              int *face_out, int *idx_out, double *w_out, int *far_count,
                              hipStream_t s);
-// The same, skipping candidates by their faces' bounding spheres (sph: nf
-// double4 of scratch); identical results.
+// The same, skipping candidates by their faces' bounding spheres (scratch:
+// sph, nf double4; tri, 9·nf doubles for the faces' corners); identical results.
 int launch_closest_triangles_pruned(const double *q, int nq, const double *verts,
                                     const int *faces, int nf, const int *cell_off,
                                     const int *cell_faces, int g, double reach2, double4 *sph,
-                                    int *face_out, int *idx_out, double *w_out, int *far_count,
-                                    hipStream_t s);
+                                    double *tri, int *face_out, int *idx_out, double *w_out,
+                                    int *far_count, hipStream_t s);
 
 }  // namespace csg

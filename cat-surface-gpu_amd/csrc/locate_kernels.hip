@@ -1175,13 +1175,20 @@ __device__ double closest_on_triangle(const double *p, const double *a, const do
 // rounding.
 __global__ void __launch_bounds__(kBlock)
 k_face_spheres(const double *__restrict__ verts, const int *__restrict__ faces, int nf,
-               double4 *__restrict__ sph)
+               double4 *__restrict__ sph, double *__restrict__ tri)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nf)
         return;
     const double *a = verts + 3 * faces[3 * f], *b = verts + 3 * faces[3 * f + 1],
                  *c = verts + 3 * faces[3 * f + 2];
+    // the face's corners side by side (one gather per candidate in the scan)
+    double *t = tri + 9 * (size_t)f;
+    for (int d = 0; d < 3; ++d) {
+        t[d] = a[d];
+        t[3 + d] = b[d];
+        t[6 + d] = c[d];
+    }
     double m[3], r2 = 0.0;
     for (int d = 0; d < 3; ++d)
         m[d] = (a[d] + b[d] + c[d]) / 3.0;
@@ -1199,19 +1206,93 @@ k_face_spheres(const double *__restrict__ verts, const int *__restrict__ faces, 
 // reach2, i.e. whose true closest triangle may lie outside the searched cells.
 // kLocateLanes consecutive lanes share one query, each scanning every
 // kLocateLanes-th candidate; the lanes' bests are combined as in the
-// sequential scan (smaller distance, then earlier candidate).  With face
-// spheres (sph), a lane skips a candidate whose sphere lies farther than
-// sqrt(best)·(1 + 1e-9) + r: its distance then exceeds the lane's best, which
-// is at least the final best, so it could neither win nor tie (the skip
-// changes no result).
-template <bool PRUNE>
+// sequential scan (smaller distance, then earlier candidate).
 __global__ void __launch_bounds__(kBlock)
 k_closest_triangles(const double *__restrict__ q, int nq, const double *__restrict__ verts,
                     const int *__restrict__ faces, const int *__restrict__ cell_off,
                     const int *__restrict__ cell_faces, int g, double reach2,
-                    const double4 *__restrict__ sph, int *__restrict__ face_out,
-                    int *__restrict__ idx_out, double *__restrict__ w_out,
-                    int *__restrict__ far_count)
+                    int *__restrict__ face_out, int *__restrict__ idx_out,
+                    double *__restrict__ w_out, int *__restrict__ far_count)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / kLocateLanes, lane = t % kLocateLanes;
+    double p[3] = {0.0, 0.0, 0.0};
+    int begin = 0, end = 0;
+    if (i < nq) {
+        p[0] = q[3 * i];
+        p[1] = q[3 * i + 1];
+        p[2] = q[3 * i + 2];
+        double cell = 2.0 / (double)g;
+        int c[3];
+        for (int d = 0; d < 3; ++d) {
+            long k = (long)floor((p[d] + 1.0) / cell);
+            c[d] = (int)(k < 0 ? 0 : (k > g - 1 ? g - 1 : k));
+        }
+        int id = c[0] + g * (c[1] + g * c[2]);
+        begin = cell_off[id];
+        end = cell_off[id + 1];
+    }
+    double best = INFINITY, bw[3] = {0, 0, 0};
+    int bf = -1, bpos = 0x7fffffff;
+    for (int k = begin + lane; k < end; k += kLocateLanes) {
+        int f = cell_faces[k];
+        const int *fv = faces + 3 * f;
+        double w[3];
+        double dist = closest_on_triangle(p, verts + 3 * fv[0], verts + 3 * fv[1],
+                                          verts + 3 * fv[2], w);
+        if (dist < best) {
+            best = dist;
+            bf = f;
+            bpos = k;
+            bw[0] = w[0];
+            bw[1] = w[1];
+            bw[2] = w[2];
+        }
+    }
+    for (int m = 1; m < kLocateLanes; m <<= 1) {
+        double ob = __shfl_xor(best, m);
+        int of = __shfl_xor(bf, m), op = __shfl_xor(bpos, m);
+        double o0 = __shfl_xor(bw[0], m), o1 = __shfl_xor(bw[1], m), o2 = __shfl_xor(bw[2], m);
+        if (ob < best || (ob == best && op < bpos)) {
+            best = ob;
+            bf = of;
+            bpos = op;
+            bw[0] = o0;
+            bw[1] = o1;
+            bw[2] = o2;
+        }
+    }
+    if (i >= nq || lane != 0)
+        return;
+    if (bf < 0 || best > reach2)
+        atomicAdd(far_count, 1);
+    face_out[i] = bf;
+    for (int d = 0; d < 3; ++d) {
+        w_out[3 * i + d] = bw[d];
+        if (idx_out)
+            idx_out[3 * i + d] = bf >= 0 ? faces[3 * bf + d] : 0;
+    }
+}
+
+// k_closest_triangles with each candidate first tested against its face's
+// bounding sphere (sph): a lane skips a candidate whose sphere lies farther
+// than sqrt(best)·(1 + 1e-9) + r: its distance then exceeds the lane's best,
+// which is at least the final best, so it could neither win nor tie (the skip
+// changes no result).  The distances are taken on face-major corner copies
+// (tri, 9 values per face, written by k_face_spheres): a face's corners are
+// one gather instead of an index and three vertex gathers.  Each lane fetches
+// kClosestBatch candidate indices, then their spheres, at once, and evaluates
+// them in the order of the sequential scan (increasing position), so the
+// result is the same.
+constexpr int kClosestBatch = 4;
+
+__global__ void __launch_bounds__(kBlock)
+k_closest_triangles_tri(const double *__restrict__ q, int nq, const double *__restrict__ tri,
+                        const int *__restrict__ faces, const int *__restrict__ cell_off,
+                        const int *__restrict__ cell_faces, int g, double reach2,
+                        const double4 *__restrict__ sph, int *__restrict__ face_out,
+                        int *__restrict__ idx_out, double *__restrict__ w_out,
+                        int *__restrict__ far_count)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = t / kLocateLanes, lane = t % kLocateLanes;
@@ -1233,28 +1314,37 @@ k_closest_triangles(const double *__restrict__ q, int nq, const double *__restri
     }
     double best = INFINITY, bw[3] = {0, 0, 0}, reach = INFINITY;
     int bf = -1, bpos = 0x7fffffff;
-    for (int k = begin + lane; k < end; k += kLocateLanes) {
-        int f = cell_faces[k];
-        if (PRUNE) {
-            const double4 s = sph[f];
-            const double dx = p[0] - s.x, dy = p[1] - s.y, dz = p[2] - s.z;
-            const double lim = s.w + reach;
+    for (int k0 = begin + lane; k0 < end; k0 += kLocateLanes * kClosestBatch) {
+        int fb[kClosestBatch];
+        double4 sb[kClosestBatch];
+#pragma unroll
+        for (int b = 0; b < kClosestBatch; ++b) {
+            const int k = k0 + b * kLocateLanes;
+            fb[b] = k < end ? cell_faces[k] : -1;
+        }
+#pragma unroll
+        for (int b = 0; b < kClosestBatch; ++b)
+            sb[b] = fb[b] >= 0 ? sph[fb[b]] : make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+        for (int b = 0; b < kClosestBatch; ++b) {
+            if (fb[b] < 0)
+                continue;
+            const double dx = p[0] - sb[b].x, dy = p[1] - sb[b].y, dz = p[2] - sb[b].z;
+            const double lim = sb[b].w + reach;
             if (dx * dx + dy * dy + dz * dz > lim * lim)
                 continue;
-        }
-        const int *fv = faces + 3 * f;
-        double w[3];
-        double dist = closest_on_triangle(p, verts + 3 * fv[0], verts + 3 * fv[1],
-                                          verts + 3 * fv[2], w);
-        if (dist < best) {
-            best = dist;
-            if (PRUNE)
+            const double *tf = tri + 9 * (size_t)fb[b];
+            double w[3];
+            const double dist = closest_on_triangle(p, tf, tf + 3, tf + 6, w);
+            if (dist < best) {
+                best = dist;
                 reach = sqrt(best) * (1.0 + 1e-9);
-            bf = f;
-            bpos = k;
-            bw[0] = w[0];
-            bw[1] = w[1];
-            bw[2] = w[2];
+                bf = fb[b];
+                bpos = k0 + b * kLocateLanes;
+                bw[0] = w[0];
+                bw[1] = w[1];
+                bw[2] = w[2];
+            }
         }
     }
     for (int m = 1; m < kLocateLanes; m <<= 1) {
@@ -1289,9 +1379,9 @@ int launch_closest_triangles(const double *q, int nq, const double *verts, const
 {
     if (nq <= 0)
         return kOk;
-    hipLaunchKernelGGL(k_closest_triangles<false>, dim3(grid_for((size_t)nq * kLocateLanes)),
+    hipLaunchKernelGGL(k_closest_triangles, dim3(grid_for((size_t)nq * kLocateLanes)),
                        dim3(kBlock), 0, s, q, nq, verts, faces, cell_off, cell_faces, g, reach2,
-                       nullptr, face_out, idx_out, w_out, far_count);
+                       face_out, idx_out, w_out, far_count);
     CSG_LAUNCH_CHECK();
     return kOk;
 }
@@ -1299,16 +1389,16 @@ int launch_closest_triangles(const double *q, int nq, const double *verts, const
 int launch_closest_triangles_pruned(const double *q, int nq, const double *verts,
                                     const int *faces, int nf, const int *cell_off,
                                     const int *cell_faces, int g, double reach2, double4 *sph,
-                                    int *face_out, int *idx_out, double *w_out, int *far_count,
-                                    hipStream_t s)
+                                    double *tri, int *face_out, int *idx_out, double *w_out,
+                                    int *far_count, hipStream_t s)
 {
     if (nq <= 0)
         return kOk;
     hipLaunchKernelGGL(k_face_spheres, dim3(grid_for(nf)), dim3(kBlock), 0, s, verts, faces, nf,
-                       sph);
+                       sph, tri);
     CSG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_closest_triangles<true>, dim3(grid_for((size_t)nq * kLocateLanes)),
-                       dim3(kBlock), 0, s, q, nq, verts, faces, cell_off, cell_faces, g, reach2,
+    hipLaunchKernelGGL(k_closest_triangles_tri, dim3(grid_for((size_t)nq * kLocateLanes)),
+                       dim3(kBlock), 0, s, q, nq, tri, faces, cell_off, cell_faces, g, reach2,
                        sph, face_out, idx_out, w_out, far_count);
     CSG_LAUNCH_CHECK();
     return kOk;

@@ -652,9 +652,10 @@ int csg_resample_unit(const double *pts, const int *idx, const double *w, int m,
 int csg_unit_points64(const double *pts, int n, float *out, void *stream);
 
 /* csg_closest_triangles with each candidate first tested against its face's bounding
- * sphere (computed into ws, csg_closest_triangles_workspace_bytes(nf) bytes): a candidate
- * farther than the scanning lane's best is skipped, which changes no result.  The far
- * count goes to far_dev (device int), and to far_host after a synchronisation if given. */
+ * sphere (computed into ws, csg_closest_triangles_workspace_bytes(nf) bytes, with a
+ * face-major copy of the faces' corners): a candidate farther than the scanning lane's
+ * best is skipped, which changes no result.  The far count goes to far_dev (device int),
+ * and to far_host after a synchronisation if given. */
 size_t csg_closest_triangles_workspace_bytes(int nf);
 int csg_closest_triangles_pruned(const double *q, int nq, const double *verts, const int *faces,
                                  int nf, const int *cell_off, const int *cell_faces, int grid,
