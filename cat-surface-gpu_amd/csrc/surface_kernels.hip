@@ -77,6 +77,37 @@ k_edge_sum(const float *__restrict__ v, const int *__restrict__ faces, int nf,
 
 // One heat iteration on geometry: v <- f32((v + Σ_nbr v / 3) / (1 + deg / 3)).
 // surface_stencil.py:372-388
+//
+// The divisors are 3 and den = 1 + deg / 3 (a handful of values), so each
+// division x / d is taken from y = 1 / d, correctly rounded and computed once:
+// q = RN(x·y) is within one ulp of x / d, r = x - q·d is exact (FMA), and
+// RN(q + r·y) is the correctly rounded quotient (Markstein's theorem, IEEE
+// binary64, round to nearest, no underflow or overflow at these magnitudes):
+// the same bits as x / d for three FP64 operations instead of the ~15 of a
+// division.
+__device__ __forceinline__ double div_known(double x, double d, double y)
+{
+    const double q = x * y;
+    return fma(fma(-q, d, x), y, q);
+}
+
+struct SmoothDen {
+    double den, inv;
+};
+
+__device__ __forceinline__ SmoothDen smooth_den(int deg)
+{
+    const double den = 1.0 + (double)deg / 3.0;
+    return {den, 1.0 / den};
+}
+
+// f32((c + s / 3) / den) with both divisions through div_known.
+__device__ __forceinline__ float smooth_coord(float c, double s, SmoothDen d)
+{
+    constexpr double kThird = 1.0 / 3.0;
+    return (float)div_known((double)c + div_known(s, 3.0, kThird), d.den, d.inv);
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_smooth_geometry(const float *__restrict__ cur, const int *__restrict__ off,
                   const int *__restrict__ nbr, int n, float *__restrict__ nxt)
@@ -92,10 +123,10 @@ k_smooth_geometry(const float *__restrict__ cur, const int *__restrict__ off,
         sy += (double)p[1];
         sz += (double)p[2];
     }
-    double den = 1.0 + (double)(e - b) / 3.0;
-    nxt[3 * i] = (float)(((double)cur[3 * i] + sx / 3.0) / den);
-    nxt[3 * i + 1] = (float)(((double)cur[3 * i + 1] + sy / 3.0) / den);
-    nxt[3 * i + 2] = (float)(((double)cur[3 * i + 2] + sz / 3.0) / den);
+    const SmoothDen den = smooth_den(e - b);
+    nxt[3 * i] = smooth_coord(cur[3 * i], sx, den);
+    nxt[3 * i + 1] = smooth_coord(cur[3 * i + 1], sy, den);
+    nxt[3 * i + 2] = smooth_coord(cur[3 * i + 2], sz, den);
 }
 
 // One smoothing step for a batch of meshes of one topology, each with its own
@@ -122,10 +153,10 @@ k_smooth_geometry_batch(float *__restrict__ buf0, float *__restrict__ buf1,
         sy += (double)p[1];
         sz += (double)p[2];
     }
-    double den = 1.0 + (double)(e - bo) / 3.0;
-    nxt[3 * i] = (float)(((double)cur[3 * i] + sx / 3.0) / den);
-    nxt[3 * i + 1] = (float)(((double)cur[3 * i + 1] + sy / 3.0) / den);
-    nxt[3 * i + 2] = (float)(((double)cur[3 * i + 2] + sz / 3.0) / den);
+    const SmoothDen den = smooth_den(e - bo);
+    nxt[3 * i] = smooth_coord(cur[3 * i], sx, den);
+    nxt[3 * i + 1] = smooth_coord(cur[3 * i + 1], sy, den);
+    nxt[3 * i + 2] = smooth_coord(cur[3 * i + 2], sz, den);
 }
 
 // The batched step on padded float4 vertices (x, y, z, 0): one 16-byte load
@@ -152,10 +183,9 @@ k_smooth_geometry4(float4 *__restrict__ buf0, float4 *__restrict__ buf1,
         sz += (double)p.z;
     }
     const float4 c = cur[i];
-    double den = 1.0 + (double)(e - bo) / 3.0;
-    nxt[i] = make_float4((float)(((double)c.x + sx / 3.0) / den),
-                         (float)(((double)c.y + sy / 3.0) / den),
-                         (float)(((double)c.z + sz / 3.0) / den), 0.0f);
+    const SmoothDen den = smooth_den(e - bo);
+    nxt[i] = make_float4(smooth_coord(c.x, sx, den), smooth_coord(c.y, sy, den),
+                         smooth_coord(c.z, sz, den), 0.0f);
 }
 
 // Up to `rings` iterations of k_smooth_geometry4 in one launch, one
@@ -168,6 +198,7 @@ k_smooth_geometry4(float4 *__restrict__ buf0, float4 *__restrict__ buf1,
 // arithmetic, neighbours in the same order).  Meshes with fewer than m
 // iterations left (m = 0: a copy) are advanced by what they have left.
 constexpr int kPatchThreads = 1024;
+constexpr int kDenTable = 16;
 constexpr int kPatchGather = 6;  // neighbours gathered together (icosphere degrees 5, 6)
 
 __global__ void __launch_bounds__(kPatchThreads)
@@ -206,6 +237,13 @@ k_smooth_geometry_patch(const float4 *__restrict__ in, float4 *__restrict__ out,
         off[j] = (unsigned short)aoff[j];
     for (int k = tid; k < E; k += kPatchThreads)
         adj[k] = pp.ladj[a0 + k];
+    // den and 1 / den of the small degrees, once per workgroup
+    __shared__ double den_t[kDenTable], inv_t[kDenTable];
+    if (tid < kDenTable) {
+        const SmoothDen d = smooth_den(tid);
+        den_t[tid] = d.den;
+        inv_t[tid] = d.inv;
+    }
     __syncthreads();
     for (int t = 0; t < m; ++t) {
         const float *cx = buf[t & 1], *cy = cx + L, *cz = cy + L;
@@ -240,10 +278,12 @@ k_smooth_geometry_patch(const float4 *__restrict__ in, float4 *__restrict__ out,
                 sy += (double)cy[r];
                 sz += (double)cz[r];
             }
-            const double den = 1.0 + (double)(e - bo) / 3.0;
-            nx[j] = (float)(((double)cx[j] + sx / 3.0) / den);
-            ny[j] = (float)(((double)cy[j] + sy / 3.0) / den);
-            nz[j] = (float)(((double)cz[j] + sz / 3.0) / den);
+            const int deg = e - bo;
+            const SmoothDen den = deg < kDenTable ? SmoothDen{den_t[deg], inv_t[deg]}
+                                                  : smooth_den(deg);
+            nx[j] = smooth_coord(cx[j], sx, den);
+            ny[j] = smooth_coord(cy[j], sy, den);
+            nz[j] = smooth_coord(cz[j], sz, den);
         }
         __syncthreads();
     }
@@ -1015,15 +1055,20 @@ int launch_smooth_geometry_patched(const float *v, float4 *buf0, float4 *buf1, i
     static std::once_flag once;
     static size_t limit = 65536;
     std::call_once(once, [] {
+        // the dynamic allocation may use what the kernel's static tables leave
         int dev = 0, per_block = 0;
+        hipFuncAttributes attr{};
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
                 hipSuccess &&
-            per_block > 0 &&
-            hipFuncSetAttribute((const void *)k_smooth_geometry_patch,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                per_block) == hipSuccess)
-            limit = (size_t)per_block;
+            hipFuncGetAttributes(&attr, (const void *)k_smooth_geometry_patch) == hipSuccess &&
+            per_block > (int)attr.sharedSizeBytes) {
+            const int dynamic = per_block - (int)attr.sharedSizeBytes;
+            if (hipFuncSetAttribute((const void *)k_smooth_geometry_patch,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    dynamic) == hipSuccess)
+                limit = (size_t)dynamic;
+        }
         (void)hipGetLastError();
     });
     if (lds_bytes > limit)

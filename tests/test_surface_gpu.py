@@ -870,7 +870,8 @@ def test_geometry_smoothing_patched_equals_per_iteration(level, iterations, patc
     """Patch-local iterations in LDS (csg_smooth_geometry_patched) give the bits of
     one launch per iteration (csg_smooth_geometry_batch), for iteration counts
     below, at and between multiples of the rings per launch, and for one patch
-    covering the whole mesh."""
+    covering the whole mesh; both equal the oracle's float64 arithmetic
+    (correctly rounded divisions, the same neighbour order) bit for bit."""
 
     from cat_surface_gpu import _lib
     from cat_surface_gpu.stencil_builder import icosphere
@@ -900,6 +901,14 @@ def test_geometry_smoothing_patched_equals_per_iteration(level, iterations, patc
         _lib.stream_handle(cuda)), "patched")
     torch.cuda.synchronize()
     assert torch.equal(out, ref)
+    off = np.ascontiguousarray(topo.nbr_off.cpu().numpy(), dtype=np.int32)
+    nbr = np.ascontiguousarray(topo.nbr.cpu().numpy(), dtype=np.int32)
+    host = geom.cpu().numpy()
+    for b, k in enumerate(iterations):
+        want = np.empty_like(host[b])
+        O.lib().oracle_smooth_geometry(O._f(np.ascontiguousarray(host[b])), O._i(off),
+                                       O._i(nbr), n, int(k), O._f(want))
+        np.testing.assert_array_equal(ref[b].cpu().numpy(), want)
 
 
 @pytest.mark.parametrize("level,counts,patches", [
