@@ -57,6 +57,24 @@ inline unsigned grid_for(size_t n, int block = kBlock)
 }
 
 // x circular.
+// The calling workgroup's (x, y) block with the blocks relabelled XCD by XCD.
+// The hardware places workgroup i (flattened x first, then y) on XCD i mod 8;
+// here XCD k takes the k-th contiguous share of the flattened blocks, so that
+// neighbouring blocks, whose bilinear gathers and stencil reads overlap, share
+// that XCD's L2.  A bijection: every block's work and results are unchanged.
+struct Block2 {
+    int x, y;
+};
+
+__device__ __forceinline__ Block2 xcd_block()
+{
+    const unsigned gx = gridDim.x, total = gx * gridDim.y;
+    const unsigned l = blockIdx.x + gx * blockIdx.y;
+    const unsigned k = l & 7u, j = l >> 3, per = total >> 3, extra = total & 7u;
+    const unsigned m = k * per + (k < extra ? k : extra) + j;
+    return {(int)(m % gx), (int)(m / gx)};
+}
+
 __device__ __forceinline__ int wrap_x(int ix, int nx)
 {
     if ((unsigned)ix < (unsigned)nx)

@@ -92,14 +92,15 @@ k_compose_self(const T *__restrict__ map, const T *__restrict__ jac, int nx, int
                size_t mstride, size_t jstride, T *__restrict__ omap, T *__restrict__ ojac)
 {
     const int P = nx * ny;
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const Block2 blk = xcd_block();
+    int p = blk.x * blockDim.x + threadIdx.x;
     if (p >= P)
         return;
-    map += blockIdx.y * mstride;
-    omap += blockIdx.y * mstride;
+    map += blk.y * mstride;
+    omap += blk.y * mstride;
     if (jac) {
-        jac += blockIdx.y * jstride;
-        ojac += blockIdx.y * jstride;
+        jac += blk.y * jstride;
+        ojac += blk.y * jstride;
     }
     MapCorners<T, D> m = map_corners<T, D>(map, p, P, nx, ny);
     const Corners<T> &c = m.c;
@@ -159,8 +160,9 @@ k_objective(const T *__restrict__ src, const T *__restrict__ tgt, const T *__res
 {
     __shared__ double red[kBlock / 64];
     const int P = nx * ny;
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
+    const Block2 blk = xcd_block();
+    int p = blk.x * blockDim.x + threadIdx.x;
+    const int b = blk.y;
     src += b * istride;
     tgt += b * istride;
     map += b * mstride;
@@ -233,7 +235,7 @@ k_objective(const T *__restrict__ src, const T *__restrict__ tgt, const T *__res
     }
     double s = block_sum(e, red);
     if (threadIdx.x == 0 && partial)
-        partial[(size_t)b * nblocks + blockIdx.x] = s;
+        partial[(size_t)b * nblocks + blk.x] = s;
 }
 
 // One squaring step: b += det·Jᵀ(b∘φ), A += det·Jᵀ(A∘φ)J with the increment
@@ -247,10 +249,11 @@ k_squaring(const T *__restrict__ bin, const T *__restrict__ ain, const T *__rest
            T *__restrict__ inc, T *__restrict__ omap, T *__restrict__ ojac)
 {
     const int P = nx * ny;
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const Block2 blk = xcd_block();
+    int p = blk.x * blockDim.x + threadIdx.x;
     if (p >= P)
         return;
-    const int bi = blockIdx.y;
+    const int bi = blk.y;
     bin += bi * bstride;
     ain += bi * astride;
     bout += bi * bstride;
@@ -308,8 +311,9 @@ k_combine(const T *__restrict__ v, const T *__restrict__ bf, const T *__restrict
 {
     __shared__ double red[kBlock / 64];
     const int P = nx * ny;
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const int bi = blockIdx.y;
+    const Block2 blk = xcd_block();
+    int p = blk.x * blockDim.x + threadIdx.x;
+    const int bi = blk.y;
     v += bi * vstride;
     bf += bi * bstride;
     af += bi * astride;
@@ -349,8 +353,8 @@ k_combine(const T *__restrict__ v, const T *__restrict__ bf, const T *__restrict
     double r1 = block_sum(sreg, red);
     double r2 = block_sum(snorm, red);
     if (threadIdx.x == 0) {
-        partial_reg[(size_t)bi * nblocks + blockIdx.x] = r1;
-        partial_norm[(size_t)bi * nblocks + blockIdx.x] = r2;
+        partial_reg[(size_t)bi * nblocks + blk.x] = r1;
+        partial_norm[(size_t)bi * nblocks + blk.x] = r2;
     }
 }
 

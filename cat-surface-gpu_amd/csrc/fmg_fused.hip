@@ -108,14 +108,16 @@ k_relax_tile(const T *__restrict__ A, const T *__restrict__ B, const T *__restri
     extern __shared__ __align__(16) unsigned char lds_raw[];
     U2<T> *uu = reinterpret_cast<U2<T> *>(lds_raw);
     const int P = nx * ny, CP = cnx * cny;
-    const int job = blockIdx.y;
+    // tiles relabelled XCD by XCD: a tile's halo lies in neighbouring tiles
+    const Block2 blk = xcd_block();
+    const int job = blk.y;
     A += job * astride;
     B += job * bstride;
     Uin += job * bstride;
     Uout += job * bstride;
     if (Uc)
         Uc += job * cstride;
-    const int tyi = blockIdx.x / tiles_x, txi = blockIdx.x - tyi * tiles_x;
+    const int tyi = blk.x / tiles_x, txi = blk.x - tyi * tiles_x;
     const int ex = TX + 2 * H;
     const int x0 = txi * TX - H;
     const int ylo = max(0, tyi * TY - H), yhi = min(ny, tyi * TY + TY + H);
@@ -308,10 +310,11 @@ k_restrict_residual(const T *__restrict__ A, const T *__restrict__ B, const T *_
                     size_t bstride, size_t cstride, T *__restrict__ Bc, T *__restrict__ Uc)
 {
     const int CP = cnx * cny, P = nx * ny;
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const Block2 blk = xcd_block();
+    const int gid = blk.x * blockDim.x + threadIdx.x;
     const int q = gid >> 2, k = gid & 3;
     const bool live = q < CP;
-    const int job = blockIdx.y;
+    const int job = blk.y;
     A += job * astride;
     B += job * bstride;
     U += job * bstride;
