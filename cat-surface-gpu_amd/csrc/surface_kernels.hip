@@ -206,7 +206,10 @@ k_smooth_geometry_patch(const float4 *__restrict__ in, float4 *__restrict__ out,
                         SmoothPatches pp, SmoothBatch sb, int it0)
 {
     extern __shared__ float g_patch_lds[];
-    const int p = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+    // patches (consecutive in Morton order) relabelled XCD by XCD: a patch's
+    // halo lies in its neighbours' cores
+    const Block2 blk = xcd_block();
+    const int p = blk.x, b = blk.y, tid = threadIdx.x;
     const int s0 = pp.vstart[p], L = pp.vstart[p + 1] - s0;
     const int a0 = pp.astart[p], E = pp.astart[p + 1] - a0;
     const int *lv = pp.lverts + s0;
@@ -303,7 +306,8 @@ k_smooth_values_patch(const double *__restrict__ in, double *__restrict__ out, i
                       size_t wstride, SmoothPatches pp, HeatBatch hb, int it0)
 {
     extern __shared__ double g_value_lds[];
-    const int p = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+    const Block2 blk = xcd_block();  // as k_smooth_geometry_patch
+    const int p = blk.x, b = blk.y, tid = threadIdx.x;
     const int s0 = pp.vstart[p], L = pp.vstart[p + 1] - s0;
     const int a0 = pp.astart[p], E = pp.astart[p + 1] - a0;
     const int *lv = pp.lverts + s0;
