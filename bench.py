@@ -46,15 +46,15 @@ HBM_PEAK_GBS = 8000.0
 # dispatches.  Source: the named profile.
 _TRAFFIC_PROFILE = "profiles/hbm_traffic_fibonacci61442_f32.md"
 MEASURED_TRAFFIC = {
-    "k_relax_tile": (27.09e6, _TRAFFIC_PROFILE),
-    "k_fmg_coarse": (0.84e6, _TRAFFIC_PROFILE),
-    "k_restrict_residual": (22.33e6, _TRAFFIC_PROFILE),
-    "k_squaring": (123.65e6, _TRAFFIC_PROFILE),
-    "k_objective": (84.36e6, _TRAFFIC_PROFILE),
-    "k_compose_self": (55.86e6, _TRAFFIC_PROFILE),
-    "k_combine": (67.82e6, _TRAFFIC_PROFILE),
-    "k_smooth_geometry_patch": (23.09e6, _TRAFFIC_PROFILE),
-    "k_sor_patched": (35.96e6, _TRAFFIC_PROFILE),
+    "k_relax_tile": (13.21e6, _TRAFFIC_PROFILE),
+    "k_fmg_coarse": (0.83e6, _TRAFFIC_PROFILE),
+    "k_restrict_residual": (11.11e6, _TRAFFIC_PROFILE),
+    "k_squaring": (69.26e6, _TRAFFIC_PROFILE),
+    "k_objective": (66.15e6, _TRAFFIC_PROFILE),
+    "k_compose_self": (32.11e6, _TRAFFIC_PROFILE),
+    "k_combine": (53.59e6, _TRAFFIC_PROFILE),
+    "k_smooth_geometry_patch": (15.45e6, _TRAFFIC_PROFILE),
+    "k_sor_patched": (35.25e6, _TRAFFIC_PROFILE),
 }
 
 
