@@ -52,10 +52,16 @@ struct RotationCostArgs {
     int *uncovered;
     int *loc_face;    // [na][ns] or null
     float *loc_bary;  // [na][ns][3] or null
-    // [na][ns] or null (matrix costs only): each point's face for each candidate
-    // from the previous launch (-1: none), read to start the scan, then updated
+    // [na][ns] or null (matrix costs only): each point's cell and the position of
+    // its face in that cell's list for each candidate from the previous launch
+    // (cell << kHintPosBits | position; -1: none), read to start the scan, then
+    // updated
     int *hint;
+    // [nf][3] or null (matrix costs only): tgt_vals at each face's corners, read
+    // instead of faces then tgt_vals
+    const double *face_vals;
 };
+constexpr int kHintPosBits = 10;
 
 int launch_rotation_costs(const RotationCostArgs &a, hipStream_t s);
 // Bounding spheres of the faces of tri for the locator's candidate skip.

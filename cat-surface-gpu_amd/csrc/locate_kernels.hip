@@ -669,20 +669,21 @@ __device__ __forceinline__ float face_distance(const float *q, const float *__re
 // of c (the 1e-3 barycentric slack), so the plane distance is >= |q - c| -
 // 1.004 r.  Skipping it changes no result.
 //
-// `start` (a face known to be in this cell's list, or -1) is evaluated first:
-// it is one of the candidates, and a close start lets the sphere test skip
-// most of the others.
+// `start` (a face known to be in this cell's list, at position start_pos, or
+// -1) is evaluated first: it is one of the candidates, and a close start lets
+// the sphere test skip most of the others.
 __device__ __forceinline__ LocateBest scan_candidates(const float *q, const float *__restrict__ tri,
                                                      const int *__restrict__ cand, int ncand,
                                                      int lane, int lanes,
                                                      const float4 *__restrict__ sph,
-                                                     int start = -1)
+                                                     int start = -1, int start_pos = 0x7fffffff)
 {
     LocateBest r{INFINITY, 0x7fffffff, -1, {0.0f, 0.0f, 0.0f}};
     float reach = INFINITY;
     if (start >= 0) {
         r.dist = face_distance(q, tri, start, r.bp);
         r.face = start;
+        r.pos = start_pos;
         reach = sqrtf(r.dist) * 1.0001f + 1e-6f;
     }
     // kScanBatch candidates per lane at a time: their indices, then their
@@ -852,7 +853,7 @@ k_rotation_cost_mats(const float *__restrict__ src_pts, const double *__restrict
                      const int *__restrict__ cell_off, const int *__restrict__ cell_faces, int g,
                      const float *__restrict__ mats,
                      double *__restrict__ partial, int nblocks, int *__restrict__ uncovered,
-                     int *__restrict__ hint)
+                     int *__restrict__ hint, const double *__restrict__ face_vals)
 {
     constexpr int kTerms = GRAD ? kRotationGradTerms : 1;
     __shared__ float m[9];
@@ -867,33 +868,34 @@ k_rotation_cost_mats(const float *__restrict__ src_pts, const double *__restrict
     for (int k = 0; k < kTerms; ++k)
         term[k] = 0.0;
     float q[3] = {0.0f, 0.0f, 0.0f};
-    int b = 0, nc = 0;
+    int b = 0, nc = 0, cell = -1;
     if (i < ns) {
         const float *p = src_pts + 3 * i;
         for (int r = 0; r < 3; ++r)
             q[r] = p[0] * m[3 * r] + p[1] * m[3 * r + 1] + p[2] * m[3 * r + 2];
-        int cell = cell_of(q, g);
+        cell = cell_of(q, g);
         b = cell_off[cell];
         nc = cell_off[cell + 1] - b;
     }
     // the face this point found for this candidate last time starts the scan
-    // when it is in the cell's list
-    int start = -1;
+    // when the point is still in the same cell (the face is then entry `pos` of
+    // the cell's list: one load, no search of the list)
+    int start = -1, start_pos = 0x7fffffff;
     if (hint) {
         const int h = i < ns ? hint[(size_t)a * ns + i] : -1;
-        int found = 0;
-        if (h >= 0)
-            for (int c = lane; c < nc; c += kLocateLanes)
-                found |= cell_faces[b + c] == h;
-        for (int m = 1; m < kLocateLanes; m <<= 1)
-            found |= __shfl_xor(found, m);
-        start = found ? h : -1;
+        if (h >= 0 && (h >> kHintPosBits) == cell && (h & ((1 << kHintPosBits) - 1)) < nc) {
+            start_pos = h & ((1 << kHintPosBits) - 1);
+            start = cell_faces[b + start_pos];
+        }
     }
     LocateBest best = scan_candidates(q, tri, cell_faces + b, nc, lane, kLocateLanes, sph,
-                                      start);
+                                      start, start_pos);
     best = reduce_lanes(best, kLocateLanes);
     if (hint && i < ns && lane == 0)
-        hint[(size_t)a * ns + i] = best.face;
+        hint[(size_t)a * ns + i] =
+            best.face >= 0 && best.pos < (1 << kHintPosBits) && cell < (1 << (31 - kHintPosBits))
+                ? (cell << kHintPosBits) | best.pos
+                : -1;
     if (i < ns && lane == 0) {
         if (best.face < 0) {
             atomicAdd(uncovered, 1);
@@ -901,8 +903,16 @@ k_rotation_cost_mats(const float *__restrict__ src_pts, const double *__restrict
             const int f = best.face;
             float w[3];
             locate_weights(tri, f, best.bp, w);
-            const double t0 = tgt_vals[faces[3 * f]], t1 = tgt_vals[faces[3 * f + 1]],
-                         t2 = tgt_vals[faces[3 * f + 2]];
+            double t0, t1, t2;
+            if (face_vals) {
+                t0 = face_vals[3 * (size_t)f];
+                t1 = face_vals[3 * (size_t)f + 1];
+                t2 = face_vals[3 * (size_t)f + 2];
+            } else {
+                t0 = tgt_vals[faces[3 * f]];
+                t1 = tgt_vals[faces[3 * f + 1]];
+                t2 = tgt_vals[faces[3 * f + 2]];
+            }
             const double d = src_vals[i] - (t0 * (double)w[0] + t1 * (double)w[1] +
                                             t2 * (double)w[2]);
             term[0] = d * d;
@@ -1077,11 +1087,13 @@ int launch_rotation_cost_mats(const RotationCostArgs &a, const float *mats, bool
     if (gradient)
         hipLaunchKernelGGL(k_rotation_cost_mats<true>, grid, dim3(kBlock), 0, s, a.src_pts,
                            a.src_vals, a.ns, a.tgt_vals, a.faces, a.tri, a.sph, a.cell_off,
-                           a.cell_faces, a.grid, mats, a.partial, nb, a.uncovered, a.hint);
+                           a.cell_faces, a.grid, mats, a.partial, nb, a.uncovered, a.hint,
+                           a.face_vals);
     else
         hipLaunchKernelGGL(k_rotation_cost_mats<false>, grid, dim3(kBlock), 0, s, a.src_pts,
                            a.src_vals, a.ns, a.tgt_vals, a.faces, a.tri, a.sph, a.cell_off,
-                           a.cell_faces, a.grid, mats, a.partial, nb, a.uncovered, a.hint);
+                           a.cell_faces, a.grid, mats, a.partial, nb, a.uncovered, a.hint,
+                           a.face_vals);
     CSG_LAUNCH_CHECK();
     const int rows = a.na * (gradient ? kRotationGradTerms : 1);
     hipLaunchKernelGGL(k_sum_rows, dim3(rows), dim3(kBlock), 0, s, a.partial, nb, rows, a.costs);
