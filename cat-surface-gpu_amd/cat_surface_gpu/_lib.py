@@ -170,6 +170,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "csg_unit_points64": (_i, [_vp, _i, _vp, _vp]),
     "csg_closest_triangles_pruned": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _i, _d, _vp, _vp, _vp,
                                           _vp, ctypes.POINTER(_i), _vp, _sz, _vp]),
+    "csg_closest_triangles_block": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _i, _i, _vp, _vp, _vp,
+                                         _vp, ctypes.POINTER(_i), _vp, _sz, _vp]),
 }
 
 

@@ -132,12 +132,17 @@ class LocateCheck:
 
 def closest_triangles(queries: torch.Tensor, verts: torch.Tensor, faces: torch.Tensor, *,
                       grid_size: int, margin: int = 1, max_margin: int = 8,
-                      deferred: bool = False):
+                      deferred: bool = False, block: bool = False):
     """Exact closest triangle and barycentric weights for each query (float64).
 
     The search covers the query's cell; a query farther than ``margin`` cells
     from its best triangle could have its true closest triangle elsewhere, so
     the index is rebuilt with a larger margin until no query is that far.
+    With ``block=True`` (for queries on or near the mesh) the index lists each
+    face only in the cells its bounding box overlaps and a query searches the
+    block of cells within ``margin`` of its own only when its own cell cannot
+    decide (csg_closest_triangles_block): far fewer index pairs, the same
+    result; ``margin`` then grows the block instead of the index.
     Returns (face [n] int32, vertex indices [n, 3] int32, weights [n, 3] f64).
 
     ``deferred=True`` returns a fourth item: None when the result is final, else
@@ -165,7 +170,13 @@ def closest_triangles(queries: torch.Tensor, verts: torch.Tensor, faces: torch.T
     stream = _lib.stream_handle(dev)
 
     def locate(index, reach, far_dev, far_host):
-        if _PRUNE:
+        if block:
+            _lib.check(lib.csg_closest_triangles_block(
+                _lib.ptr(q), nq, _lib.ptr(v), _lib.ptr(f), nf, _lib.ptr(index.cell_off),
+                _lib.ptr(index.cell_faces), grid_size, int(round(reach * grid_size / 2.0)),
+                _lib.ptr(face), _lib.ptr(idx), _lib.ptr(w), _lib.ptr(far_dev), far_host,
+                _lib.ptr(ws), ws.numel(), stream), "csg_closest_triangles_block")
+        elif _PRUNE:
             _lib.check(lib.csg_closest_triangles_pruned(
                 _lib.ptr(q), nq, _lib.ptr(v), _lib.ptr(f), nf, _lib.ptr(index.cell_off),
                 _lib.ptr(index.cell_faces), grid_size, reach, _lib.ptr(face), _lib.ptr(idx),
@@ -179,16 +190,18 @@ def closest_triangles(queries: torch.Tensor, verts: torch.Tensor, faces: torch.T
                 _lib.ptr(index.cell_faces), grid_size, reach, _lib.ptr(face), _lib.ptr(idx),
                 _lib.ptr(w), far_host, _lib.ptr(far_dev), 4, stream), "csg_closest_triangles")
 
-    key = (nf, int(grid_size), int(margin))
+    index_margin = (lambda m: 0) if block else (lambda m: m)
+    key = (nf, int(grid_size), index_margin(int(margin)))
     if deferred and key in _CAPACITY:
         capacity = _CAPACITY[key]
         status = torch.zeros(2, dtype=torch.int32, device=dev)
-        index = build_grid_index_async(v, f, grid_size, margin, capacity, status[0:1])
+        index = build_grid_index_async(v, f, grid_size, index_margin(margin), capacity,
+                                       status[0:1])
         locate(index, margin * 2.0 / grid_size, status[1:], None)
         return face, idx, w, LocateCheck(status, capacity, key)
     far_dev = torch.zeros(1, dtype=torch.int32, device=dev)
     while True:
-        index = build_grid_index(v, f, grid_size, margin)
+        index = build_grid_index(v, f, grid_size, index_margin(margin))
         far = ctypes.c_int()
         reach = margin * 2.0 / grid_size
         locate(index, reach, far_dev, ctypes.byref(far))

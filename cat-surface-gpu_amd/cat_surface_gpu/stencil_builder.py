@@ -198,9 +198,11 @@ def build_device_stencil(sphere_vertices, sphere_faces, *, surface_vertices=None
     _lib.check(lib.csg_resample_unit(_lib.ptr(pts), _lib.ptr(surf_idx), _lib.ptr(surf_w), m,
                                      _lib.ptr(resampled), _lib.ptr(unit_res), stream),
                "csg_resample_unit")
-    # sheet stencil on the unit resampled sphere
+    # sheet stencil on the unit resampled sphere: the pixel directions lie on
+    # that mesh (within its chords' sagitta), so the block search on an index
+    # without margin decides almost every query in its own cell
     second = closest_triangles(_device_uv(nx, ny, dev), unit_res, ico_f,
-                               grid_size=_GRID_SHEET, margin=1, deferred=deferred)
+                               grid_size=_GRID_SHEET, margin=1, deferred=deferred, block=True)
     sheet_idx, sheet_w = second[1], second[2]
     unit_pts = torch.empty((n, 3), dtype=torch.float32, device=dev)
     _lib.check(lib.csg_unit_points64(_lib.ptr(pts), n, _lib.ptr(unit_pts), stream),

@@ -350,4 +350,28 @@ int csg_closest_triangles_pruned(const double *q, int nq, const double *verts, c
     return kOk;
 }
 
+int csg_closest_triangles_block(const double *q, int nq, const double *verts, const int *faces,
+                                int nf, const int *cell_off, const int *cell_faces, int grid,
+                                int span, int *face_out, int *idx_out, double *w_out,
+                                int *far_dev, int *far_host, void *ws, size_t ws_bytes,
+                                void *stream)
+{
+    if (nq < 0 || nf < 1 || grid < 1 || span < 0 || !far_dev)
+        return fail(kInvalidArgument, "invalid closest-triangle arguments");
+    if (ws_bytes < csg_closest_triangles_workspace_bytes(nf))
+        return fail(kInvalidArgument, "closest-triangle workspace too small");
+    hipStream_t s = S(stream);
+    CSG_TRY(clear_int(far_dev, s));
+    double4 *sph = (double4 *)((char *)ws + 256);
+    double *tri = (double *)((char *)sph + align256(sizeof(double4) * (size_t)nf));
+    CSG_TRY(launch_closest_triangles_block(q, nq, verts, faces, nf, cell_off, cell_faces, grid,
+                                           span, sph, tri, face_out, idx_out, w_out, far_dev,
+                                           s));
+    if (far_host) {
+        CSG_HIP_CHECK(hipMemcpyAsync(far_host, far_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+        CSG_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return kOk;
+}
+
 }  // extern "C"

@@ -91,5 +91,13 @@ int launch_closest_triangles_pruned(const double *q, int nq, const double *verts
                                     const int *cell_faces, int g, double reach2, double4 *sph,
                                     double *tri, int *face_out, int *idx_out, double *w_out,
                                     int *far_count, hipStream_t s);
+// The block search on an index built without margin (k_closest_triangles_block):
+// the query's cell, then the (2·span + 1)^3 block around it where needed; the
+// same scratch as launch_closest_triangles_pruned.
+int launch_closest_triangles_block(const double *q, int nq, const double *verts,
+                                   const int *faces, int nf, const int *cell_off,
+                                   const int *cell_faces, int g, int span, double4 *sph,
+                                   double *tri, int *face_out, int *idx_out, double *w_out,
+                                   int *far_count, hipStream_t s);
 
 }  // namespace csg

@@ -1384,6 +1384,173 @@ k_closest_triangles_tri(const double *__restrict__ q, int nq, const double *__re
     }
 }
 
+// Exact closest triangle on an index built without margin (each face listed
+// in the cells its bounding box overlaps), for queries near the mesh.  A query
+// scans its own cell's list; if its best distance is below the distance from
+// the query to that cell's walls (less 1e-9 of a cell), every face with a
+// closer point would have that point inside the cell and so be in the list,
+// and the result is exact.  Otherwise it also scans the other cells of the
+// (2·span + 1)^3 block around its cell, exact when the best distance is below
+// the distance to the block's walls; beyond that it counts as far.  A wall on
+// the grid's boundary is at infinity (points outside the grid fall into, and
+// clipped boxes are listed in, the boundary cells).  Candidates are tested
+// against their face spheres as in k_closest_triangles_tri; ties go to the
+// smaller face index, which is the first minimum in face order (a face may be
+// met in several lists).
+__device__ __forceinline__ void scan_face_list(const double *p, const double *__restrict__ tri,
+                                               const double4 *__restrict__ sph,
+                                               const int *__restrict__ list, int n, int lane,
+                                               double &best, int &bf, double *bw,
+                                               double &reach)
+{
+    for (int k0 = lane; k0 < n; k0 += kLocateLanes * kClosestBatch) {
+        int fb[kClosestBatch];
+        double4 sb[kClosestBatch];
+#pragma unroll
+        for (int b = 0; b < kClosestBatch; ++b) {
+            const int k = k0 + b * kLocateLanes;
+            fb[b] = k < n ? list[k] : -1;
+        }
+#pragma unroll
+        for (int b = 0; b < kClosestBatch; ++b)
+            sb[b] = fb[b] >= 0 ? sph[fb[b]] : make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+        for (int b = 0; b < kClosestBatch; ++b) {
+            if (fb[b] < 0)
+                continue;
+            const double dx = p[0] - sb[b].x, dy = p[1] - sb[b].y, dz = p[2] - sb[b].z;
+            const double lim = sb[b].w + reach;
+            if (dx * dx + dy * dy + dz * dz > lim * lim)
+                continue;
+            const double *tf = tri + 9 * (size_t)fb[b];
+            double w[3];
+            const double dist = closest_on_triangle(p, tf, tf + 3, tf + 6, w);
+            if (dist < best || (dist == best && fb[b] < bf)) {
+                best = dist;
+                reach = sqrt(best) * (1.0 + 1e-9);
+                bf = fb[b];
+                bw[0] = w[0];
+                bw[1] = w[1];
+                bw[2] = w[2];
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void reduce_best(double &best, int &bf, double *bw)
+{
+    for (int m = 1; m < kLocateLanes; m <<= 1) {
+        const double ob = __shfl_xor(best, m);
+        const int of = __shfl_xor(bf, m);
+        const double o0 = __shfl_xor(bw[0], m), o1 = __shfl_xor(bw[1], m),
+                     o2 = __shfl_xor(bw[2], m);
+        if (ob < best || (ob == best && (unsigned)of < (unsigned)bf)) {
+            best = ob;
+            bf = of;
+            bw[0] = o0;
+            bw[1] = o1;
+            bw[2] = o2;
+        }
+    }
+}
+
+// Distance from p to the walls of the cell block [lo, hi] (cell indices per
+// axis) of a g^3 grid over [-1, 1]^3 of cell size `cell`; walls on the grid's
+// boundary do not count.
+__device__ __forceinline__ double block_reach(const double *p, const int *lo, const int *hi,
+                                              int g, double cell)
+{
+    double r = INFINITY;
+    for (int d = 0; d < 3; ++d) {
+        if (lo[d] > 0)
+            r = fmin(r, p[d] - ((double)lo[d] * cell - 1.0));
+        if (hi[d] < g - 1)
+            r = fmin(r, ((double)(hi[d] + 1) * cell - 1.0) - p[d]);
+    }
+    return r;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_closest_triangles_block(const double *__restrict__ q, int nq, const double *__restrict__ tri,
+                          const int *__restrict__ faces, const int *__restrict__ cell_off,
+                          const int *__restrict__ cell_faces, int g, int span,
+                          const double4 *__restrict__ sph, int *__restrict__ face_out,
+                          int *__restrict__ idx_out, double *__restrict__ w_out,
+                          int *__restrict__ far_count)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / kLocateLanes, lane = t % kLocateLanes;
+    const double cell = 2.0 / (double)g, slack = 1e-9 * cell;
+    double p[3] = {0.0, 0.0, 0.0};
+    int c[3] = {0, 0, 0}, begin = 0, end = 0;
+    if (i < nq) {
+        p[0] = q[3 * i];
+        p[1] = q[3 * i + 1];
+        p[2] = q[3 * i + 2];
+        for (int d = 0; d < 3; ++d) {
+            long k = (long)floor((p[d] + 1.0) / cell);
+            c[d] = (int)(k < 0 ? 0 : (k > g - 1 ? g - 1 : k));
+        }
+        const int id = c[0] + g * (c[1] + g * c[2]);
+        begin = cell_off[id];
+        end = cell_off[id + 1];
+    }
+    double best = INFINITY, bw[3] = {0, 0, 0}, reach = INFINITY;
+    int bf = -1;
+    scan_face_list(p, tri, sph, cell_faces + begin, end - begin, lane, best, bf, bw, reach);
+    reduce_best(best, bf, bw);
+    const double own = block_reach(p, c, c, g, cell) - slack;
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = c[d] - span < 0 ? 0 : c[d] - span;
+        hi[d] = c[d] + span > g - 1 ? g - 1 : c[d] + span;
+    }
+    // the same decision on every lane of the query (best is reduced)
+    if (i < nq && span > 0 && !(bf >= 0 && sqrt(best) <= own)) {
+        reach = bf >= 0 ? sqrt(best) * (1.0 + 1e-9) : INFINITY;
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int x = lo[0]; x <= hi[0]; ++x) {
+                    if (x == c[0] && y == c[1] && z == c[2])
+                        continue;
+                    const int id = x + g * (y + g * z);
+                    const int b0 = cell_off[id], b1 = cell_off[id + 1];
+                    scan_face_list(p, tri, sph, cell_faces + b0, b1 - b0, lane, best, bf, bw,
+                                   reach);
+                }
+        reduce_best(best, bf, bw);
+    }
+    if (i >= nq || lane != 0)
+        return;
+    const double limit = (span > 0 ? block_reach(p, lo, hi, g, cell) : own + slack) - slack;
+    if (bf < 0 || !(sqrt(best) <= limit))
+        atomicAdd(far_count, 1);
+    face_out[i] = bf;
+    for (int d = 0; d < 3; ++d) {
+        w_out[3 * i + d] = bw[d];
+        if (idx_out)
+            idx_out[3 * i + d] = bf >= 0 ? faces[3 * bf + d] : 0;
+    }
+}
+
+int launch_closest_triangles_block(const double *q, int nq, const double *verts,
+                                   const int *faces, int nf, const int *cell_off,
+                                   const int *cell_faces, int g, int span, double4 *sph,
+                                   double *tri, int *face_out, int *idx_out, double *w_out,
+                                   int *far_count, hipStream_t s)
+{
+    if (nq <= 0)
+        return kOk;
+    hipLaunchKernelGGL(k_face_spheres, dim3(grid_for(nf)), dim3(kBlock), 0, s, verts, faces, nf,
+                       sph, tri);
+    CSG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_closest_triangles_block, dim3(grid_for((size_t)nq * kLocateLanes)),
+                       dim3(kBlock), 0, s, q, nq, tri, faces, cell_off, cell_faces, g, span, sph,
+                       face_out, idx_out, w_out, far_count);
+    CSG_LAUNCH_CHECK();
+    return kOk;
+}
+
 int launch_closest_triangles(const double *q, int nq, const double *verts, const int *faces,
                              const int *cell_off, const int *cell_faces, int g, double reach2,
                              int *face_out, int *idx_out, double *w_out, int *far_count,

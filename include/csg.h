@@ -657,6 +657,20 @@ int csg_unit_points64(const double *pts, int n, float *out, void *stream);
  * best is skipped, which changes no result.  The far count goes to far_dev (device int),
  * and to far_host after a synchronisation if given. */
 size_t csg_closest_triangles_workspace_bytes(int nf);
+/* Exact closest triangles for queries near the mesh on an index built with margin 0 (each
+ * face in the cells its bounding box overlaps; csg_grid_index_build with margin 0): a query
+ * scans its own cell, and the (2·span + 1)^3 block of cells around it only when its best
+ * distance exceeds the distance to its cell's walls; a query whose best distance exceeds the
+ * distance to the block's walls counts as far (far_dev, far_host as in
+ * csg_closest_triangles_pruned, same workspace).  Ties go to the smaller face index.  The
+ * results equal csg_closest_triangles_pruned's on an index with margin `span` whenever
+ * neither counts a query as far.  No counterpart in the reference (a search strategy for the
+ * stencil builder's query, native/cat_surface_stencil_builder.c:116-131). */
+int csg_closest_triangles_block(const double *q, int nq, const double *verts, const int *faces,
+                                int nf, const int *cell_off, const int *cell_faces, int grid,
+                                int span, int *face_out, int *idx_out, double *w_out,
+                                int *far_dev, int *far_host, void *ws, size_t ws_bytes,
+                                void *stream);
 int csg_closest_triangles_pruned(const double *q, int nq, const double *verts, const int *faces,
                                  int nf, const int *cell_off, const int *cell_faces, int grid,
                                  double reach, int *face_out, int *idx_out, double *w_out,

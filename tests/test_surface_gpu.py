@@ -981,6 +981,34 @@ def test_closest_triangles_pruned_equals_full_scan(level, grid, cuda):
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("level,grid,spread", [
+    (6, 64, 0.0), (6, 32, 0.0), (5, 64, 0.03), (6, 128, 0.0), (4, 16, 0.03), (6, 64, 0.025)])
+def test_closest_triangles_block_equals_margin_search(level, grid, spread, cuda):
+    """The block search on an index without margin (own cell, then the 3x3x3
+    block where the own cell cannot decide) returns the faces, vertex indices
+    and weights of the margin search, for map pixel directions on a perturbed
+    unit mesh, points 2.5 % inside it (the surface stencil's queries, which
+    mostly need the block), points at vertices (ties) and scattered points."""
+
+    from cat_surface_gpu.locate import closest_triangles
+    from cat_surface_gpu.stencil_builder import icosphere, uv_points
+
+    sv, sf = icosphere(level)
+    rng = np.random.default_rng(level * 7 + grid)
+    pert = sv * (1.0 + 0.004 * rng.standard_normal((sv.shape[0], 1)))
+    pert /= np.linalg.norm(pert, axis=1, keepdims=True)
+    verts = torch.as_tensor(pert, device=cuda)
+    faces = torch.as_tensor(sf, device=cuda)
+    uv = uv_points(128, 64)
+    scale = (1.0 if spread == 0.0 else rng.uniform(1.0 - spread, 1.0 + spread, (uv.shape[0], 1)))
+    q = np.concatenate([uv * scale, sv[:400] * 0.975, pert[400:500]])
+    q = torch.as_tensor(q, device=cuda)
+    ref = closest_triangles(q, verts, faces, grid_size=grid, margin=1)
+    got = closest_triangles(q, verts, faces, grid_size=grid, margin=1, block=True)
+    for a, b in zip(got, ref):
+        assert torch.equal(a, b)
+
+
 @pytest.mark.parametrize("case", ["golden", "level6"])
 def test_rotation_locator_skip_equals_full_scan(case, golden, cuda):
     """The locator's bounding-sphere skip leaves every location, cost, gradient
