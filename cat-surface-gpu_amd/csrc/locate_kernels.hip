@@ -1297,7 +1297,9 @@ k_closest_triangles(const double *__restrict__ q, int nq, const double *__restri
 // them in the order of the sequential scan (increasing position), so the
 // result is the same.
 constexpr int kClosestBatch = 4;
+constexpr int kClosestLanes = 16;
 
+template <int LANES>
 __global__ void __launch_bounds__(kBlock)
 k_closest_triangles_tri(const double *__restrict__ q, int nq, const double *__restrict__ tri,
                         const int *__restrict__ faces, const int *__restrict__ cell_off,
@@ -1307,7 +1309,7 @@ k_closest_triangles_tri(const double *__restrict__ q, int nq, const double *__re
                         int *__restrict__ far_count)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = t / kLocateLanes, lane = t % kLocateLanes;
+    const int i = t / LANES, lane = t % LANES;
     double p[3] = {0.0, 0.0, 0.0};
     int begin = 0, end = 0;
     if (i < nq) {
@@ -1326,12 +1328,12 @@ k_closest_triangles_tri(const double *__restrict__ q, int nq, const double *__re
     }
     double best = INFINITY, bw[3] = {0, 0, 0}, reach = INFINITY;
     int bf = -1, bpos = 0x7fffffff;
-    for (int k0 = begin + lane; k0 < end; k0 += kLocateLanes * kClosestBatch) {
+    for (int k0 = begin + lane; k0 < end; k0 += LANES * kClosestBatch) {
         int fb[kClosestBatch];
         double4 sb[kClosestBatch];
 #pragma unroll
         for (int b = 0; b < kClosestBatch; ++b) {
-            const int k = k0 + b * kLocateLanes;
+            const int k = k0 + b * LANES;
             fb[b] = k < end ? cell_faces[k] : -1;
         }
 #pragma unroll
@@ -1352,14 +1354,14 @@ k_closest_triangles_tri(const double *__restrict__ q, int nq, const double *__re
                 best = dist;
                 reach = sqrt(best) * (1.0 + 1e-9);
                 bf = fb[b];
-                bpos = k0 + b * kLocateLanes;
+                bpos = k0 + b * LANES;
                 bw[0] = w[0];
                 bw[1] = w[1];
                 bw[2] = w[2];
             }
         }
     }
-    for (int m = 1; m < kLocateLanes; m <<= 1) {
+    for (int m = 1; m < LANES; m <<= 1) {
         double ob = __shfl_xor(best, m);
         int of = __shfl_xor(bf, m), op = __shfl_xor(bpos, m);
         double o0 = __shfl_xor(bw[0], m), o1 = __shfl_xor(bw[1], m), o2 = __shfl_xor(bw[2], m);
@@ -1576,9 +1578,13 @@ int launch_closest_triangles_pruned(const double *q, int nq, const double *verts
     hipLaunchKernelGGL(k_face_spheres, dim3(grid_for(nf)), dim3(kBlock), 0, s, verts, faces, nf,
                        sph, tri);
     CSG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_closest_triangles_tri, dim3(grid_for((size_t)nq * kLocateLanes)),
-                       dim3(kBlock), 0, s, q, nq, tri, faces, cell_off, cell_faces, g, reach2,
-                       sph, face_out, idx_out, w_out, far_count);
+    // sixteen lanes per query: a query of the surface stencil has ~80 candidates
+    // and the launch only ~41k queries, so four lanes left the device at ~2.5
+    // waves per SIMD with each lane scanning its share in sequence
+    hipLaunchKernelGGL(k_closest_triangles_tri<kClosestLanes>,
+                       dim3(grid_for((size_t)nq * kClosestLanes)), dim3(kBlock), 0, s, q, nq,
+                       tri, faces, cell_off, cell_faces, g, reach2, sph, face_out, idx_out,
+                       w_out, far_count);
     CSG_LAUNCH_CHECK();
     return kOk;
 }
