@@ -57,6 +57,17 @@ inline unsigned grid_for(size_t n, int block = kBlock)
 }
 
 // x circular.
+// x / d for a divisor whose reciprocal y = 1 / d (correctly rounded) was taken
+// in advance: q = RN(x·y) is within one ulp of x / d, r = x - q·d is exact
+// (FMA), and RN(q + r·y) is the correctly rounded quotient (Markstein's
+// theorem; IEEE binary64, round to nearest, no underflow or overflow): the
+// bits of x / d for three FP64 operations instead of the ~15 of a division.
+__device__ __forceinline__ double div_known(double x, double d, double y)
+{
+    const double q = x * y;
+    return fma(fma(-q, d, x), y, q);
+}
+
 // The calling workgroup's (x, y) block with the blocks relabelled XCD by XCD.
 // The hardware places workgroup i (flattened x first, then y) on XCD i mod 8;
 // here XCD k takes the k-th contiguous share of the flattened blocks, so that

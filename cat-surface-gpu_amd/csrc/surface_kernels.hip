@@ -79,17 +79,7 @@ k_edge_sum(const float *__restrict__ v, const int *__restrict__ faces, int nf,
 // surface_stencil.py:372-388
 //
 // The divisors are 3 and den = 1 + deg / 3 (a handful of values), so each
-// division x / d is taken from y = 1 / d, correctly rounded and computed once:
-// q = RN(x·y) is within one ulp of x / d, r = x - q·d is exact (FMA), and
-// RN(q + r·y) is the correctly rounded quotient (Markstein's theorem, IEEE
-// binary64, round to nearest, no underflow or overflow at these magnitudes):
-// the same bits as x / d for three FP64 operations instead of the ~15 of a
-// division.
-__device__ __forceinline__ double div_known(double x, double d, double y)
-{
-    const double q = x * y;
-    return fma(fma(-q, d, x), y, q);
-}
+// division x / d is taken from y = 1 / d, computed once (div_known).
 
 struct SmoothDen {
     double den, inv;
