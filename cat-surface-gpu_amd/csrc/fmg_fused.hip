@@ -589,7 +589,7 @@ template <typename T>
 __global__ void __launch_bounds__(kCoarseThreads)
 k_fmg_coarse(CoarseLevels<T> lv, const T *__restrict__ A, const T *__restrict__ B,
              T *__restrict__ U, size_t astride, size_t bstride, double id, int nit, int cycles,
-             int init, int build)
+             int init, int build, int P0)
 {
     __shared__ LevelTable<T> tab;
     T *base = reinterpret_cast<T *>(g_lds);
@@ -600,6 +600,20 @@ k_fmg_coarse(CoarseLevels<T> lv, const T *__restrict__ A, const T *__restrict__ 
     const int c0 = lv.c0, L = lv.L, last = L - 1;
     TapIdx<T> *taps = reinterpret_cast<TapIdx<T> *>(g_lds);
     const TapIdx<T> *gtaps = reinterpret_cast<const TapIdx<T> *>(lv.meta + meta_taps_offset<T>());
+    // level c0 (at LDS offset 0 in every layout) does not depend on the table:
+    // its loads go out with the table's
+    T *ta = base, *tb = ta + 3 * P0, *tu = ta + 5 * P0;
+    for (int p = threadIdx.x; p < P0; p += kCoarseThreads) {
+        ta[p] = A[p];
+        ta[P0 + p] = A[P0 + p];
+        ta[2 * P0 + p] = A[2 * P0 + p];
+        tb[p] = B[p];
+        tb[P0 + p] = B[P0 + p];
+        if (!init) {
+            tu[p] = U[p];
+            tu[P0 + p] = U[P0 + p];
+        }
+    }
     if (!build) {
         const unsigned *src = reinterpret_cast<const unsigned *>(lv.meta);
         unsigned *dst = reinterpret_cast<unsigned *>(&tab);
@@ -667,20 +681,6 @@ k_fmg_coarse(CoarseLevels<T> lv, const T *__restrict__ A, const T *__restrict__ 
                 taps[tab.up_x[l] + t - cx - cy] = taps_x<T>(t - cx - cy, cx, fx);
             else
                 taps[tab.up_y[l] + t - cx - cy - fx] = taps_y<T>(t - cx - cy - fx, cy, fy);
-        }
-    }
-    __syncthreads();
-    const int P0 = tab.nx[c0] * tab.ny[c0];
-    T *ta = base + tab.off[c0], *tb = ta + 3 * P0, *tu = ta + 5 * P0;
-    for (int p = threadIdx.x; p < P0; p += kCoarseThreads) {
-        ta[p] = A[p];
-        ta[P0 + p] = A[P0 + p];
-        ta[2 * P0 + p] = A[2 * P0 + p];
-        tb[p] = B[p];
-        tb[P0 + p] = B[P0 + p];
-        if (!init) {
-            tu[p] = U[p];
-            tu[P0 + p] = U[P0 + p];
         }
     }
     __syncthreads();
@@ -997,7 +997,8 @@ int launch_fmg_coarse(const CoarseLevels<T> &lv, const T *a, const T *b, T *u, s
     TimedLaunch timer(kTimedFmgCoarse, s,
                       coarse_reference_values(lv, nit, cycles, init) * sizeof(T) * batch);
     hipExtLaunchKernelGGL(k_fmg_coarse<T>, dim3(batch), dim3(kCoarseThreads), lds_bytes, s, timer.start(), timer.stop(), 0, lv, a,
-                       b, u, astride, bstride, id, nit, cycles, init, build);
+                       b, u, astride, bstride, id, nit, cycles, init, build,
+                       lv.nx[lv.c0] * lv.ny[lv.c0]);
     CSG_LAUNCH_CHECK();
     timer.done();
     return kOk;
