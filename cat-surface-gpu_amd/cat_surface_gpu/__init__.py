@@ -68,6 +68,7 @@ from .gpu_pipeline import (
     read_gifti_mesh,
     run_cat_surface_gpu_pipeline,
     run_cat_surface_gpu_pipelines,
+    run_cat_surface_gpu_stream,
     write_gifti_mesh,
 )
 from .dartel_adjoint import ExpWarpSSD, dartel_rhs, exp_warp_ssd

@@ -8,12 +8,20 @@ either endianness and either index order.
 
 Speed (the registration itself takes ~30 ms per hemisphere on MI355X, so file
 I/O matters): the payloads of a file's arrays are decoded in parallel threads
-(zlib and base64 release the GIL), identical payloads are decoded once, an
+(zlib and base64 release the GIL), identical payloads of one read (a surface
+and its sphere share the triangles) are decoded once, an
 array written again unchanged (the output sphere's triangles are the input
 sphere's) reuses the payload it was read from, float payloads are stored
 uncompressed inside the zlib stream (vertex coordinates compress by ~10% at
 level 1, at ~25 MB/s) and integer payloads at level 1.  Every file written is
-ordinary GZipBase64Binary GIFTI.
+ordinary GZipBase64Binary GIFTI.  A file of the plain shape GIFTI writers emit
+is not run through an XML parser (expat holds the interpreter lock, ~7 ms per
+61,442-vertex surface, as long as the inflation of its payloads): its
+DataArray start tags and Data texts are located by pattern matching on the
+bytes, which gives what the parser gives for such a file; anything else
+(comments, entity or character references, a DOCTYPE internal subset, CDATA
+holding markup, a Data element with children, or a file not closed by
+</GIFTI>) goes through xml.etree.
 """
 
 from __future__ import annotations
@@ -23,6 +31,7 @@ from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 import hashlib
 from pathlib import Path
+import re
 import threading
 import xml.etree.ElementTree as ET
 import zlib
@@ -47,20 +56,18 @@ class GiftiArray:
 
 
 # (dtype code, shape, sha1 of the little-endian row-major bytes) -> GZipBase64Binary
-# payload text, and sha1 of a payload text -> decoded flat array; both bounded
+# payload text, bounded
 _PAYLOADS: dict[tuple, str] = {}
-_DECODED: dict[str, np.ndarray] = {}
 _CACHE_LOCK = threading.Lock()
 _CACHE_ENTRIES = 64
 _POOL: ThreadPoolExecutor | None = None
 
 
 def clear_caches() -> None:
-    """Forget every remembered payload and decoded array."""
+    """Forget every remembered payload."""
 
     with _CACHE_LOCK:
         _PAYLOADS.clear()
-        _DECODED.clear()
 
 
 def _pool() -> ThreadPoolExecutor:
@@ -82,8 +89,22 @@ def _payload_key(data: np.ndarray, code: str) -> tuple:
     return (code, tuple(data.shape), hashlib.sha1(memoryview(raw).cast("B")).hexdigest())
 
 
-def _decode(elem: ET.Element) -> GiftiArray:
-    attrs = elem.attrib
+@dataclass
+class _Item:
+    """One DataArray's attributes, resolved, and its Data text."""
+
+    intent: int
+    code: str
+    dims: list
+    encoding: str
+    endian: str
+    order: str
+    text: str | bytes
+    key: tuple | None  # identifies the decoded payload (binary encodings)
+
+
+def _prepare(item: tuple[dict, str | bytes]) -> _Item:
+    attrs, text = item
     intent_raw = attrs.get("Intent", "NIFTI_INTENT_NONE")
     intent = _INTENT_NAMES.get(intent_raw, None)
     if intent is None:
@@ -94,55 +115,174 @@ def _decode(elem: ET.Element) -> GiftiArray:
     dims = [int(attrs[f"Dim{i}"]) for i in range(int(attrs.get("Dimensionality", "1")))]
     encoding = attrs.get("Encoding", "ASCII")
     endian = "<" if attrs.get("Endian", "LittleEndian") == "LittleEndian" else ">"
-    text = (elem.findtext("Data") or "").strip()
-    count = int(np.prod(dims)) if dims else 0
-    if encoding == "ASCII":
-        flat = np.asarray(text.split(), dtype=np.dtype(code)) if text else np.zeros(0, code)
-    elif encoding in ("Base64Binary", "GZipBase64Binary"):
-        text_key = hashlib.sha1(text.encode("ascii", "ignore")).hexdigest() + encoding + code
-        flat = _DECODED.get(text_key)
-        if flat is None:
-            raw = base64.b64decode(text)
-            if encoding == "GZipBase64Binary":
-                raw = zlib.decompress(raw)
-            flat = np.frombuffer(raw, dtype=np.dtype(endian + code), count=count)
-            _remember(_DECODED, text_key, flat)
-    else:
+    order = "F" if attrs.get("ArrayIndexingOrder") == "ColumnMajorOrder" else "C"
+    text = text.strip()
+    key = None
+    if encoding in ("Base64Binary", "GZipBase64Binary"):
+        btext = text if isinstance(text, bytes) else text.encode("ascii", "ignore")
+        key = (encoding, endian + code, hashlib.sha1(btext).hexdigest())
+    elif encoding != "ASCII":
         raise ValueError(f"unsupported GIFTI Encoding {encoding}")
+    return _Item(intent, code, dims, encoding, endian, order, text, key)
+
+
+def _payload_values(it: _Item) -> np.ndarray:
+    """The flat values of a DataArray's Data text."""
+
+    if it.encoding == "ASCII":
+        text = it.text.decode("ascii") if isinstance(it.text, bytes) else it.text
+        return np.asarray(text.split(), dtype=np.dtype(it.code)) if text else np.zeros(0, it.code)
+    raw = base64.b64decode(it.text)
+    if it.encoding == "GZipBase64Binary":
+        raw = zlib.decompress(raw)
+    return np.frombuffer(raw, dtype=np.dtype(it.endian + it.code))
+
+
+def _finish(it: _Item, flat: np.ndarray) -> GiftiArray:
+    count = int(np.prod(it.dims)) if it.dims else 0
     if flat.size != count:
         raise ValueError("GIFTI data array length does not match its dimensions")
-    order = "F" if attrs.get("ArrayIndexingOrder") == "ColumnMajorOrder" else "C"
-    data = np.ascontiguousarray(
-        np.asarray(flat.reshape(dims, order=order)).astype(np.dtype(code).newbyteorder("=")))
-    if encoding == "GZipBase64Binary" and endian == "<" and order == "C":
+    data = np.ascontiguousarray(np.asarray(flat.reshape(it.dims, order=it.order)).astype(
+        np.dtype(it.code).newbyteorder("=")))
+    if it.encoding == "GZipBase64Binary" and it.endian == "<" and it.order == "C":
         # an unchanged array written back reuses this payload
-        _remember(_PAYLOADS, _payload_key(data, code), text)
-    return GiftiArray(data, intent)
+        _remember(_PAYLOADS, _payload_key(data, it.code),
+                  it.text.decode("ascii") if isinstance(it.text, bytes) else it.text)
+    return GiftiArray(data, it.intent)
 
 
-def _elements(path) -> list[ET.Element]:
-    root = ET.parse(str(path)).getroot()
+def _decode(item: tuple[dict, str | bytes]) -> GiftiArray:
+    """One DataArray from its attributes and the text of its Data element."""
+
+    it = _prepare(item)
+    return _finish(it, _payload_values(it))
+
+
+# The plain file shape: an optional XML declaration and DOCTYPE without an
+# internal subset, the GIFTI root, DataArray elements whose attribute values
+# hold no references and whose Data element holds only character data.
+_ROOT = re.compile(rb"(?:\xef\xbb\xbf)?\s*(?:<\?xml[^>]*\?>)?\s*(?:<!DOCTYPE[^>\[]*>)?\s*<GIFTI[\s>/]")
+_ATTRIBUTE = re.compile(rb"""([A-Za-z_:][-\w:.]*)\s*=\s*(?:"([^"<]*)"|'([^'<]*)')""")
+_CDATA = re.compile(rb"<!\[CDATA\[(.*?)\]\]>", re.S)
+
+
+def _plain_markup(part: bytes) -> bool:
+    """Whether markup outside the payloads has no comment, entity declaration or
+    CDATA section holding markup."""
+
+    if b"<!--" in part or b"<!ENTITY" in part:
+        return False
+    if b"<![CDATA[" in part:
+        for m in _CDATA.finditer(part):
+            if b"<" in m.group(1) or b">" in m.group(1):
+                return False
+    return True
+
+
+def _scan(raw: bytes) -> list[tuple[dict, bytes]] | None:
+    """(attributes, Data text) of each DataArray of a plain-shaped file, or None.
+
+    A payload is passed over by two byte searches (its first "<", which must
+    open "</Data>", and "&"); the markup between payloads is checked in full."""
+
+    if not _ROOT.match(raw) or not raw[-64:].rstrip().endswith(b"</GIFTI>"):
+        return None
+    out, pos, gap = [], 0, 0
+    while True:
+        i = raw.find(b"<DataArray", pos)
+        if i < 0:
+            return out if _plain_markup(raw[gap:]) else None
+        j = raw.find(b">", i)
+        if j < 0 or raw[i + 10:i + 11] not in (b" ", b"\t", b"\r", b"\n", b">"):
+            return None
+        head = raw[i + 10:j]
+        attrs = {}
+        for a in _ATTRIBUTE.finditer(head):
+            value = a.group(2) if a.group(2) is not None else a.group(3)
+            if b"&" in value or b"\t" in value or b"\n" in value or b"\r" in value:
+                return None
+            attrs[a.group(1).decode("ascii")] = value.decode("utf-8")
+        if _ATTRIBUTE.sub(b"", head).strip():
+            return None  # self-closing, or something other than attributes
+        # the Data child (not DataSpace and the like) before the DataArray's end
+        text, q, after = b"", j, j
+        while True:
+            d = raw.find(b"<Data", q)
+            close = raw.find(b"</DataArray", q, d if d >= 0 else len(raw))
+            if d < 0 or close >= 0:
+                break
+            c = raw[d + 5:d + 6]
+            if c not in (b" ", b"\t", b"\r", b"\n", b">", b"/"):
+                q = d + 5
+                continue
+            e = raw.find(b">", d)
+            if e < 0:
+                return None
+            tag = raw[d + 5:e].strip()
+            if tag == b"/":
+                after = e
+                break
+            if tag:
+                return None
+            end = raw.find(b"<", e + 1)
+            if (end < 0 or raw[end:end + 6] != b"</Data"
+                    or raw[end + 6:end + 7] not in (b" ", b"\t", b"\r", b"\n", b">")
+                    or raw.find(b"&", e + 1, end) >= 0):
+                return None
+            if not _plain_markup(raw[gap:e + 1]):
+                return None
+            text, gap, after = raw[e + 1:end], end, end
+            break
+        k = raw.find(b"</DataArray", after)
+        if k < 0:
+            return None
+        out.append((attrs, text))
+        pos = k + 11
+
+
+def _elements(path) -> list[tuple[dict, str | bytes]]:
+    """(attributes, Data text) of each DataArray element of a GIFTI file."""
+
+    raw = Path(path).read_bytes()
+    items = _scan(raw)
+    return items if items is not None else _elements_xml(raw, path)
+
+
+def _elements_xml(raw: bytes, path) -> list[tuple[dict, str]]:
+    """_elements through xml.etree (any well-formed file)."""
+
+    root = ET.fromstring(raw)
     if root.tag != "GIFTI":
         raise ValueError(f"not a GIFTI file: {path}")
-    return root.findall("DataArray")
+    return [(e.attrib, e.findtext("Data") or "") for e in root.findall("DataArray")]
 
 
 def read_gifti(path: str | Path) -> list[GiftiArray]:
     """The data arrays of a GIFTI file, decoded in parallel threads."""
 
-    elems = _elements(path)
-    if len(elems) < 2:
-        return [_decode(e) for e in elems]
-    return list(_pool().map(_decode, elems))
+    return read_gifti_many([path])[0]
 
 
 def read_gifti_many(paths) -> list[list[GiftiArray]]:
-    """read_gifti of several files: every array of every file in parallel threads."""
+    """read_gifti of several files: every array of every file in parallel threads,
+    each distinct payload decoded once."""
 
     paths = [str(p) for p in paths]
-    parsed = list(_pool().map(_elements, paths))
-    flat = [e for elems in parsed for e in elems]
-    decoded = list(_pool().map(_decode, flat))
+    pool = _pool()
+    parsed = list(pool.map(_elements, paths))
+    items = list(pool.map(_prepare, [e for elems in parsed for e in elems]))
+    first: dict[tuple, int] = {}
+    unique = []
+    for k, it in enumerate(items):
+        if it.key is None or it.key not in first:
+            if it.key is not None:
+                first[it.key] = len(unique)
+            unique.append(k)
+    values = list(pool.map(lambda k: _payload_values(items[k]), unique))
+    slot = {k: n for n, k in enumerate(unique)}
+    decoded = list(pool.map(
+        lambda k: _finish(items[k], values[slot[k] if k in slot else first[items[k].key]]),
+        range(len(items))))
     out, k = [], 0
     for elems in parsed:
         out.append(decoded[k:k + len(elems)])

@@ -12,6 +12,8 @@ runs it.
 
 from __future__ import annotations
 
+import dataclasses
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 import ctypes
 import math
@@ -889,6 +891,61 @@ def run_graph_schedule(jobs: Sequence[PipelineJob], **kw) -> list[CatSurfaceGpuR
             timings={"total_seconds": total, "graph_replay": 1.0},
             output_path=Path(job.output) if job.output is not None else Path("")))
     return _finish_graph_results(jobs, results, write_output)
+
+
+def _read_job_meshes(batch: Sequence[PipelineJob]) -> list[PipelineJob]:
+    """The batch with every job's four GIFTI inputs read into memory (all files
+    of the batch in parallel threads); jobs given in memory pass unchanged."""
+
+    paths = [p for j in batch if j.meshes is None
+             for p in (j.source_surface, j.source_sphere, j.target_surface, j.target_sphere)]
+    arrays = iter(read_gifti_many(paths)) if paths else iter(())
+    path_iter = iter(paths)
+    out = []
+    for j in batch:
+        if j.meshes is not None:
+            out.append(j)
+            continue
+        meshes = tuple(_mesh_from_arrays(next(arrays), next(path_iter)) for _ in range(4))
+        out.append(dataclasses.replace(j, meshes=meshes, source_surface=None,
+                                       source_sphere=None, target_surface=None,
+                                       target_sphere=None))
+    return out
+
+
+def run_cat_surface_gpu_stream(batches, **kw) -> list[list[CatSurfaceGpuResult]]:
+    """Batches of registrations (GIFTI jobs, e.g. LH + RH of one subject per
+    batch) through the captured schedule (run_graph_schedule, same options),
+    one after another with the file work off the GPU's path: batch k + 1's
+    inputs are read in a background thread while batch k registers, and batch
+    k's output spheres are written while batch k + 1 registers.  Each batch's
+    results equal run_cat_surface_gpu_pipelines(batch, graph_schedule=True);
+    every output is written when the call returns."""
+
+    write_output = bool(kw.pop("write_output", True))
+    kw.pop("graph_schedule", None)
+    batches = [list(b) for b in batches]
+    if not batches:
+        return []
+    out: list[list[CatSurfaceGpuResult]] = []
+    io = ThreadPoolExecutor(max_workers=2, thread_name_prefix="csg-stream")
+    try:
+        pending = []
+        upcoming = io.submit(_read_job_meshes, batches[0])
+        for k in range(len(batches)):
+            jobs = upcoming.result()
+            if k + 1 < len(batches):
+                upcoming = io.submit(_read_job_meshes, batches[k + 1])
+            results = run_cat_surface_gpu_pipelines(jobs, graph_schedule=True,
+                                                    write_output=False, **kw)
+            if write_output:
+                pending.append(io.submit(_finish_graph_results, jobs, results, True))
+            out.append(results)
+        for f in pending:
+            f.result()
+    finally:
+        io.shutdown(wait=True)
+    return out
 
 
 def _finish_graph_results(jobs, results, write_output: bool):

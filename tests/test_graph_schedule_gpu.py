@@ -117,3 +117,36 @@ def test_graph_schedule_rejects_host_decided_stages(cuda):
     with pytest.raises(ValueError, match="type-5"):
         gp.run_cat_surface_gpu_pipelines(jobs, device=cuda, graph_schedule=True,
                                          **dict(KW, steps=3))
+
+
+def test_stream_of_gifti_batches_equals_the_graph_schedule(cuda, tmp_path):
+    """run_cat_surface_gpu_stream (batch k + 1 read and batch k written while the
+    GPU registers) gives each batch the results of run_graph_schedule on the
+    same meshes, and writes them."""
+
+    from cat_surface_gpu import PipelineJob, gifti, gpu_pipeline as gp, run_cat_surface_gpu_stream
+
+    gp.clear_graph_schedules()
+    batches = []
+    for b, seeds in enumerate(((11, 12), (21, 22), (31, 32))):
+        batch = []
+        for seed, job in zip(seeds, _jobs(seeds)):
+            paths = [tmp_path / f"b{b}_{seed}_{n}.gii"
+                     for n in ("src_surface", "src_sphere", "tgt_surface", "tgt_sphere")]
+            for path, mesh in zip(paths, job.meshes):
+                gifti.write_gifti_surface(path, mesh.vertices, mesh.faces, compress_level=6)
+            batch.append(PipelineJob(*paths, output=tmp_path / f"b{b}_{len(batch)}.sphere.reg.gii",
+                                     map_size=job.map_size))
+        batches.append(batch)
+    streamed = run_cat_surface_gpu_stream(batches, device=cuda, **dict(KW, write_output=True))
+    assert len(streamed) == len(batches)
+    for batch, results in zip(batches, streamed):
+        direct = gp.run_cat_surface_gpu_pipelines(gp._read_job_meshes(batch), device=cuda,
+                                                  graph_schedule=True, **KW)
+        _same(results, direct)
+        for job, r in zip(batch, results):
+            written = gifti.read_gifti(job.output)
+            np.testing.assert_array_equal(written[0].data, r.vertices)
+            np.testing.assert_array_equal(written[1].data, r.faces)
+    entry = next(iter(gp._GRAPH_SCHEDULES.values()))
+    assert entry.fallbacks == 0

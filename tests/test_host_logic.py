@@ -9,6 +9,7 @@ is uploaded to the device is checked here against a brute-force construction.
 
 from __future__ import annotations
 
+import re
 import numpy as np
 import pytest
 import torch
@@ -177,6 +178,97 @@ def test_gifti_round_trip(tmp_path):
     (tmp_path / "bad.gii").write_text("<NOTGIFTI/>")
     with pytest.raises(ValueError):
         gifti.read_gifti(tmp_path / "bad.gii")
+
+
+_GIFTI_ARRAY = (' <DataArray Intent="{intent}" DataType="{dtype}" ArrayIndexingOrder="{order}"'
+                ' Dimensionality="2" Dim0="{d0}" Dim1="{d1}" Encoding="{enc}" Endian="{endian}"'
+                ' ExternalFileName="" ExternalFileOffset="">{meta}\n  <Data>{data}</Data>\n'
+                ' </DataArray>\n')
+
+
+def _gifti_text(arrays, head="", root_extra="", tail=""):
+    import base64
+    import zlib
+    parts = []
+    for arr, intent, enc, endian, order, meta in arrays:
+        dtype = {"f": "NIFTI_TYPE_FLOAT32", "i": "NIFTI_TYPE_INT32"}[arr.dtype.kind]
+        raw = np.asarray(arr, arr.dtype.newbyteorder("<" if endian == "LittleEndian" else ">"))
+        raw = raw.tobytes(order="F" if order == "ColumnMajorOrder" else "C")
+        if enc == "ASCII":
+            flat = arr.flatten(order="F" if order == "ColumnMajorOrder" else "C")
+            data = " ".join(repr(x) for x in flat.tolist())
+        elif enc == "Base64Binary":
+            data = base64.b64encode(raw).decode()
+        else:
+            data = base64.b64encode(zlib.compress(raw, 6)).decode()
+        parts.append(_GIFTI_ARRAY.format(intent=intent, dtype=dtype, order=order, d0=arr.shape[0],
+                                         d1=arr.shape[1], enc=enc, endian=endian, meta=meta,
+                                         data=data))
+    return ('<?xml version="1.0" encoding="UTF-8"?>\n' + head +
+            f'<GIFTI Version="1.0" NumberOfDataArrays="{len(arrays)}"{root_extra}>\n'
+            ' <MetaData>\n  <MD>\n   <Name><![CDATA[AnatomicalStructurePrimary]]></Name>\n'
+            '   <Value><![CDATA[CortexLeft]]></Value>\n  </MD>\n </MetaData>\n <LabelTable/>\n'
+            + "".join(parts) + tail + "</GIFTI>\n")
+
+
+def test_gifti_byte_scan_equals_the_xml_parser(tmp_path):
+    """Files of the plain shape are read by byte search instead of an XML parser:
+    every attribute and Data text equals xml.etree's, for each encoding, byte
+    order, index order and metadata form (CDATA names, a coordinate transform
+    with DataSpace children, a self-closing Data); files outside that shape
+    (comments, references, CDATA or markup in a payload) go through the parser
+    and read the same arrays."""
+
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal((37, 3)).astype(np.float32)
+    f = rng.integers(0, 37, (23, 3)).astype(np.int32)
+    xform = ('\n  <CoordinateSystemTransformMatrix>\n   <DataSpace><![CDATA[NIFTI_XFORM_UNKNOWN]]>'
+             '</DataSpace>\n   <TransformedSpace><![CDATA[NIFTI_XFORM_UNKNOWN]]></TransformedSpace>'
+             '\n   <MatrixData>1 0 0 0 0 1 0 0 0 0 1 0 0 0 0 1</MatrixData>\n'
+             '  </CoordinateSystemTransformMatrix>')
+    plain = []
+    for enc in ("GZipBase64Binary", "Base64Binary", "ASCII"):
+        for endian in ("LittleEndian", "BigEndian"):
+            for order in ("RowMajorOrder", "ColumnMajorOrder"):
+                plain.append(_gifti_text(
+                    [(v, "NIFTI_INTENT_POINTSET", enc, endian, order, xform),
+                     (f, "NIFTI_INTENT_TRIANGLE", enc, endian, order, "\n  <MetaData/>")],
+                    head='<!DOCTYPE GIFTI SYSTEM "http://www.nitrc.org/frs/download.php/115/gifti.dtd">\n'))
+    one = [(v, "NIFTI_INTENT_POINTSET", "GZipBase64Binary", "LittleEndian", "RowMajorOrder", "")]
+    empty = _gifti_text([(np.zeros((0, 3), np.float32), "NIFTI_INTENT_POINTSET", "ASCII",
+                          "LittleEndian", "RowMajorOrder", "")])
+    plain.append(re.sub(r"<Data>\s*</Data>", "<Data/>", empty))
+    parser_only = [
+        _gifti_text(one, tail=" <!-- a <DataArray> in a comment -->\n"),
+        _gifti_text(one).replace('Endian="LittleEndian"', 'Endian="Little&#69;ndian"'.replace(
+            "&#69;", "E") + ' Note="a &amp; b"', 1),
+        _gifti_text(one).replace("<Data>", "<Data><![CDATA[", 1).replace("</Data>", "]]></Data>", 1),
+        _gifti_text(one, head='<!DOCTYPE GIFTI [ <!ENTITY e "LittleEndian"> ]>\n').replace(
+            'Endian="LittleEndian"', 'Endian="&e;"', 1),
+        _gifti_text([(v, "NIFTI_INTENT_POINTSET", "ASCII", "LittleEndian", "RowMajorOrder", "")]
+                    ).replace("<Data>", "<Data>&#32;", 1),
+    ]
+    for k, text in enumerate(plain + parser_only):
+        path = tmp_path / f"g{k}.gii"
+        path.write_text(text)
+        raw = path.read_bytes()
+        scanned = gifti._scan(raw)
+        parsed = gifti._elements_xml(raw, path)
+        if k < len(plain):
+            assert scanned is not None, k
+            assert [a for a, _ in scanned] == [a for a, _ in parsed]
+            assert [t.strip().decode() for _, t in scanned] == [t.strip() for _, t in parsed]
+        else:
+            assert scanned is None, k
+        gifti.clear_caches()
+        arrays = gifti.read_gifti(path)
+        gifti.clear_caches()
+        expected = [gifti._decode(item) for item in parsed]
+        assert [a.intent for a in arrays] == [a.intent for a in expected]
+        for a, b in zip(arrays, expected):
+            np.testing.assert_array_equal(a.data, b.data)
+        if k < len(plain) - 1 or k == len(plain) + 1:
+            np.testing.assert_array_equal(arrays[0].data, v)
 
 
 def test_rotation_value_and_point_files(tmp_path):
