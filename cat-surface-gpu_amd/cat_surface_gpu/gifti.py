@@ -30,6 +30,7 @@ import base64
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 import hashlib
+import os
 from pathlib import Path
 import re
 import threading
@@ -73,7 +74,10 @@ def clear_caches() -> None:
 def _pool() -> ThreadPoolExecutor:
     global _POOL
     if _POOL is None:
-        _POOL = ThreadPoolExecutor(max_workers=8, thread_name_prefix="gifti")
+        # one worker per available core, at most 16 (a file's arrays and a
+        # batch's files are the parallel work)
+        workers = max(2, min(16, len(os.sched_getaffinity(0))))
+        _POOL = ThreadPoolExecutor(max_workers=workers, thread_name_prefix="gifti")
     return _POOL
 
 
@@ -126,13 +130,42 @@ def _prepare(item: tuple[dict, str | bytes]) -> _Item:
     return _Item(intent, code, dims, encoding, endian, order, text, key)
 
 
+# base64 alphabet -> 6-bit value; 255 elsewhere
+_B64_VALUES = np.full(256, 255, np.uint8)
+_B64_VALUES[np.frombuffer(b"ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/",
+                          np.uint8)] = np.arange(64, dtype=np.uint8)
+
+
+def _b64decode(text: str | bytes) -> bytes:
+    """base64.b64decode for a payload of alphabet characters and end padding, in
+    numpy operations, which release the interpreter lock (binascii holds it: the
+    decodes of the arrays read in parallel threads ran one at a time); any other
+    text goes to base64.b64decode."""
+
+    b = text if isinstance(text, bytes) else text.encode("ascii", "ignore")
+    n = len(b)
+    if n < 4096 or n % 4:
+        return base64.b64decode(b)
+    # every group but the last (which may hold padding)
+    v = _B64_VALUES[np.frombuffer(b, np.uint8, count=n - 4)]
+    if int(v.max()) == 255:
+        return base64.b64decode(b)
+    q = v.reshape(-1, 4).astype(np.uint32)
+    w = (q[:, 0] << 18) | (q[:, 1] << 12) | (q[:, 2] << 6) | q[:, 3]
+    out = np.empty((w.size, 3), np.uint8)
+    out[:, 0] = w >> 16
+    out[:, 1] = (w >> 8) & 255
+    out[:, 2] = w & 255
+    return out.tobytes() + base64.b64decode(b[n - 4:])
+
+
 def _payload_values(it: _Item) -> np.ndarray:
     """The flat values of a DataArray's Data text."""
 
     if it.encoding == "ASCII":
         text = it.text.decode("ascii") if isinstance(it.text, bytes) else it.text
         return np.asarray(text.split(), dtype=np.dtype(it.code)) if text else np.zeros(0, it.code)
-    raw = base64.b64decode(it.text)
+    raw = _b64decode(it.text)
     if it.encoding == "GZipBase64Binary":
         raw = zlib.decompress(raw)
     return np.frombuffer(raw, dtype=np.dtype(it.endian + it.code))

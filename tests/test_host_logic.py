@@ -211,6 +211,23 @@ def _gifti_text(arrays, head="", root_extra="", tail=""):
             + "".join(parts) + tail + "</GIFTI>\n")
 
 
+def test_gifti_base64_decoder_equals_binascii():
+    """The numpy base64 decoder (which releases the interpreter lock) gives the
+    bytes of base64.b64decode for every length and padding, and hands text with
+    anything but alphabet characters and end padding to base64.b64decode."""
+
+    import base64
+
+    rng = np.random.default_rng(9)
+    for n in list(range(0, 24)) + [3069, 3070, 3071, 3072, 3073, 3074, 12288, 100001]:
+        data = rng.integers(0, 256, n, dtype=np.uint8).tobytes()
+        text = base64.b64encode(data)
+        assert gifti._b64decode(text) == data
+        assert gifti._b64decode(text.decode("ascii")) == data
+        for odd in (text[:len(text) // 2] + b"\n" + text[len(text) // 2:], text + b"\n"):
+            assert gifti._b64decode(odd) == base64.b64decode(odd)
+
+
 def test_gifti_byte_scan_equals_the_xml_parser(tmp_path):
     """Files of the plain shape are read by byte search instead of an XML parser:
     every attribute and Data text equals xml.etree's, for each encoding, byte
