@@ -159,6 +159,28 @@ def _b64decode(text: str | bytes) -> bytes:
     return out.tobytes() + base64.b64decode(b[n - 4:])
 
 
+_B64_ALPHABET = np.frombuffer(
+    b"ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/", np.uint8)
+
+
+def _b64encode(data: bytes) -> str:
+    """base64.b64encode(data).decode(), in numpy operations for the whole 3-byte
+    groups (the interpreter lock is released, so the outputs of several jobs
+    encode at once) and base64 for the padded tail."""
+
+    m = len(data) // 3
+    if m < 1024:
+        return base64.b64encode(data).decode("ascii")
+    a = np.frombuffer(data, np.uint8, count=3 * m).reshape(-1, 3).astype(np.uint32)
+    w = (a[:, 0] << 16) | (a[:, 1] << 8) | a[:, 2]
+    out = np.empty((m, 4), np.uint8)
+    out[:, 0] = _B64_ALPHABET[w >> 18]
+    out[:, 1] = _B64_ALPHABET[(w >> 12) & 63]
+    out[:, 2] = _B64_ALPHABET[(w >> 6) & 63]
+    out[:, 3] = _B64_ALPHABET[w & 63]
+    return (out.tobytes() + base64.b64encode(data[3 * m:])).decode("ascii")
+
+
 def _payload_values(it: _Item) -> np.ndarray:
     """The flat values of a DataArray's Data text."""
 
@@ -343,7 +365,7 @@ def _payload(arr: np.ndarray, code: str, level: int | None = None) -> str:
             return hit
         level = 0 if code.startswith("f") else 1
     data = np.ascontiguousarray(arr.astype("<" + code))
-    text = base64.b64encode(zlib.compress(data.tobytes(), level)).decode("ascii")
+    text = _b64encode(zlib.compress(data.tobytes(), level))
     _remember(_PAYLOADS, key, text)
     return text
 

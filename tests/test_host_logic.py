@@ -212,9 +212,10 @@ def _gifti_text(arrays, head="", root_extra="", tail=""):
 
 
 def test_gifti_base64_decoder_equals_binascii():
-    """The numpy base64 decoder (which releases the interpreter lock) gives the
-    bytes of base64.b64decode for every length and padding, and hands text with
-    anything but alphabet characters and end padding to base64.b64decode."""
+    """The numpy base64 decoder and encoder (which release the interpreter lock)
+    give the bytes of base64.b64decode / b64encode for every length and padding;
+    the decoder hands text with anything but alphabet characters and end
+    padding to base64.b64decode."""
 
     import base64
 
@@ -226,6 +227,7 @@ def test_gifti_base64_decoder_equals_binascii():
         assert gifti._b64decode(text.decode("ascii")) == data
         for odd in (text[:len(text) // 2] + b"\n" + text[len(text) // 2:], text + b"\n"):
             assert gifti._b64decode(odd) == base64.b64decode(odd)
+        assert gifti._b64encode(data) == text.decode("ascii")
 
 
 def test_gifti_byte_scan_equals_the_xml_parser(tmp_path):
