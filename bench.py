@@ -47,14 +47,14 @@ HBM_PEAK_GBS = 8000.0
 _TRAFFIC_PROFILE = "profiles/hbm_traffic_fibonacci61442_f32.md"
 MEASURED_TRAFFIC = {
     "k_relax_tile": (13.21e6, _TRAFFIC_PROFILE),
-    "k_fmg_coarse": (0.83e6, _TRAFFIC_PROFILE),
+    "k_fmg_coarse": (0.84e6, _TRAFFIC_PROFILE),
     "k_restrict_residual": (11.11e6, _TRAFFIC_PROFILE),
     "k_squaring": (69.26e6, _TRAFFIC_PROFILE),
     "k_objective": (66.15e6, _TRAFFIC_PROFILE),
     "k_compose_self": (32.11e6, _TRAFFIC_PROFILE),
     "k_combine": (53.59e6, _TRAFFIC_PROFILE),
     "k_smooth_geometry_patch": (15.45e6, _TRAFFIC_PROFILE),
-    "k_sor_patched": (35.25e6, _TRAFFIC_PROFILE),
+    "k_sor_patched": (35.04e6, _TRAFFIC_PROFILE),
 }
 
 
