@@ -53,8 +53,8 @@ MEASURED_TRAFFIC = {
     "k_objective": (66.15e6, _TRAFFIC_PROFILE),
     "k_compose_self": (32.11e6, _TRAFFIC_PROFILE),
     "k_combine": (53.59e6, _TRAFFIC_PROFILE),
-    "k_smooth_geometry_patch": (15.45e6, _TRAFFIC_PROFILE),
-    "k_sor_patched": (35.04e6, _TRAFFIC_PROFILE),
+    "k_smooth_geometry_patch": (14.39e6, _TRAFFIC_PROFILE),
+    "k_sor_patched": (35.84e6, _TRAFFIC_PROFILE),
 }
 
 
